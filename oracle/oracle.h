@@ -1,0 +1,57 @@
+/*
+ * oracle.h -- CPU restatement of the reference's CPU (ago_haf_cpu*) algorithms for the
+ * OpenVX vision-node hot path.
+ *
+ * THIS IS TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker.  The product path (libmvx*.so)
+ * never links or calls it.
+ *
+ * Memory model shared by every function (it is the reference's, ago_util.cpp:187-200,1107):
+ *   - an image plane is `h` rows of `stride` bytes; pixel (x, y) lives at byte y*stride + x*bpp;
+ *   - neighbour reads are LINEAR: the left neighbour of (0, y) is the byte before the row, i.e.
+ *     the last byte of row y-1's stride, and the right neighbour of (w-1, y) is the first byte
+ *     of row y+1 when stride == w*bpp (ago_haf_cpu_filter.cpp:1706-1707);
+ *   - reads before byte 0 or at/after byte h*stride see the zero guard bytes of the reference's
+ *     calloc'd allocation (AGO_MEMORY_ALLOC_EXTRA_PADDING, ago_internal.h:57);
+ *   - 3x3 filters write rows 1..h-2 only; the caller passes zero-filled outputs so that rows 0
+ *     and h-1 read back as the reference's untouched calloc'd rows.
+ *
+ * Parity status of each function is recorded beside its definition ("pinned by ...").
+ */
+#ifndef MVX_ORACLE_H
+#define MVX_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- filters (ago_haf_cpu_filter.cpp) ---- */
+void oracle_gaussian3x3(int w, int h, uint8_t *dst, int dst_stride, const uint8_t *src, int src_stride);
+void oracle_box3x3(int w, int h, uint8_t *dst, int dst_stride, const uint8_t *src, int src_stride);
+void oracle_median3x3(int w, int h, uint8_t *dst, int dst_stride, const uint8_t *src, int src_stride);
+/* gx or gy may be NULL; strides in bytes */
+void oracle_sobel3x3(int w, int h, int16_t *gx, int gx_stride, int16_t *gy, int gy_stride,
+                     const uint8_t *src, int src_stride);
+
+/* ---- arithmetic (ago_haf_cpu_arithmetic.cpp) ---- */
+void oracle_magnitude(int w, int h, int16_t *mag, int mag_stride, const int16_t *gx, int gx_stride,
+                      const int16_t *gy, int gy_stride);
+void oracle_phase(int w, int h, uint8_t *phase, int phase_stride, const int16_t *gx, int gx_stride,
+                  const int16_t *gy, int gy_stride);
+/* scalar forms, exported for exhaustive tests */
+int16_t oracle_magnitude_px(int gx, int gy);
+uint8_t oracle_phase_px(int gx, int gy);
+
+/* ---- nodes and graphs on reference-layout planes (graphs.c); packed in/out ---- */
+void oracle_filter3x3(int which, int w, int h, const uint8_t *src, uint8_t *dst);
+void oracle_sobel_node(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy);
+/* Gaussian3x3 (virtual) -> Sobel3x3 -> Magnitude + Phase */
+void oracle_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16_t *mag,
+                   uint8_t *phase);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

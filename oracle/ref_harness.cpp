@@ -1,0 +1,150 @@
+// Test-infrastructure harness around the reference's own CPU engine (oracle/_ref/libopenvx_ref.so,
+// compiled from the reference sources by oracle/Makefile).  Each entry point builds a small OpenVX
+// graph with the reference's public API, runs it on the reference CPU path (AGO + ago_haf_cpu*),
+// and copies every output plane out to a tightly packed host buffer (stride = width * bytes/pixel).
+//
+// Output images are created fresh by each call, so pixels a reference kernel never writes
+// (rows 0 and h-1 of a 3x3 filter, for example) come back as the zeros of the reference's
+// calloc'd buffers (ago_util.cpp:187-200).
+//
+// Nothing in the product links this file; tests and the CPU-baseline leg of bench.py load it
+// through ctypes when it has been built.
+#include <VX/vx.h>
+#include <VX/vxu.h>
+#include <vx_ext_amd.h>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <cmath>
+#include <vector>
+#include <chrono>
+
+namespace {
+
+struct Ctx {
+    vx_context ctx;
+    vx_graph graph;
+    Ctx() { ctx = vxCreateContext(); graph = vxCreateGraph(ctx); }
+    ~Ctx() { vxReleaseGraph(&graph); vxReleaseContext(&ctx); }
+};
+
+int bpp_of(vx_df_image f) {
+    switch (f) {
+    case VX_DF_IMAGE_U8: return 1;
+    case VX_DF_IMAGE_S16: case VX_DF_IMAGE_U16: return 2;
+    case VX_DF_IMAGE_RGB: return 3;
+    case VX_DF_IMAGE_RGBX: case VX_DF_IMAGE_U32: case VX_DF_IMAGE_S32: return 4;
+    default: return 1;
+    }
+}
+
+vx_status put(vx_image img, int w, int h, const void *src, int bpp) {
+    vx_rectangle_t r = {0, 0, (vx_uint32)w, (vx_uint32)h};
+    vx_imagepatch_addressing_t a = {};
+    a.dim_x = w; a.dim_y = h; a.stride_x = bpp; a.stride_y = w * bpp;
+    a.scale_x = VX_SCALE_UNITY; a.scale_y = VX_SCALE_UNITY; a.step_x = 1; a.step_y = 1;
+    return vxCopyImagePatch(img, &r, 0, &a, (void *)src, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST);
+}
+
+vx_status get(vx_image img, int w, int h, void *dst, int bpp, int plane = 0) {
+    vx_rectangle_t r = {0, 0, (vx_uint32)w, (vx_uint32)h};
+    vx_imagepatch_addressing_t a = {};
+    a.dim_x = w; a.dim_y = h; a.stride_x = bpp; a.stride_y = w * bpp;
+    a.scale_x = VX_SCALE_UNITY; a.scale_y = VX_SCALE_UNITY; a.step_x = 1; a.step_y = 1;
+    return vxCopyImagePatch(img, &r, plane, &a, dst, VX_READ_ONLY, VX_MEMORY_TYPE_HOST);
+}
+
+vx_image make_input(Ctx &c, int w, int h, vx_df_image f, const void *src) {
+    vx_image img = vxCreateImage(c.ctx, w, h, f);
+    if (put(img, w, h, src, bpp_of(f)) != VX_SUCCESS) return nullptr;
+    return img;
+}
+
+#define CHECK(x) do { vx_status _s = (x); if (_s != VX_SUCCESS) { fprintf(stderr, "ref_harness: %s -> %d (line %d)\n", #x, _s, __LINE__); return _s; } } while (0)
+
+int run(Ctx &c) {
+    CHECK(vxVerifyGraph(c.graph));
+    CHECK(vxProcessGraph(c.graph));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// U8 -> U8 3x3 filters (vx_nodes.cpp vxGaussian3x3Node/vxBox3x3Node/vxMedian3x3Node).
+// which: 0 gaussian, 1 box, 2 median.
+int ref_filter3x3(int which, int w, int h, const uint8_t *src, uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    vx_node n = which == 0 ? vxGaussian3x3Node(c.graph, in, out)
+              : which == 1 ? vxBox3x3Node(c.graph, in, out)
+                           : vxMedian3x3Node(c.graph, in, out);
+    if (!n) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(out, w, h, dst, 1));
+    vxReleaseNode(&n); vxReleaseImage(&in); vxReleaseImage(&out);
+    return 0;
+}
+
+// Sobel3x3: U8 -> S16 gx, gy (either may be null to leave that output unconnected).
+int ref_sobel3x3(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_image ox = gx ? vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16) : nullptr;
+    vx_image oy = gy ? vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16) : nullptr;
+    vx_node n = vxSobel3x3Node(c.graph, in, ox, oy);
+    if (!n) return -1;
+    int s = run(c); if (s) return s;
+    if (gx) CHECK(get(ox, w, h, gx, 2));
+    if (gy) CHECK(get(oy, w, h, gy, 2));
+    return 0;
+}
+
+// Magnitude (S16, S16 -> S16) and Phase (S16, S16 -> U8); mag or phase may be null.
+int ref_magnitude_phase(int w, int h, const int16_t *gx, const int16_t *gy, int16_t *mag, uint8_t *phase) {
+    Ctx c;
+    vx_image ix = make_input(c, w, h, VX_DF_IMAGE_S16, gx);
+    vx_image iy = make_input(c, w, h, VX_DF_IMAGE_S16, gy);
+    vx_image om = mag ? vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16) : nullptr;
+    vx_image op = phase ? vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8) : nullptr;
+    if (mag && !vxMagnitudeNode(c.graph, ix, iy, om)) return -1;
+    if (phase && !vxPhaseNode(c.graph, ix, iy, op)) return -1;
+    int s = run(c); if (s) return s;
+    if (mag) CHECK(get(om, w, h, mag, 2));
+    if (phase) CHECK(get(op, w, h, phase, 1));
+    return 0;
+}
+
+// The plain CPU-runnable graph: Gaussian3x3 (virtual) -> Sobel3x3 (gx, gy) -> Magnitude + Phase,
+// border UNDEFINED.  If time_iters > 0 the graph is processed that many more times and the
+// mean seconds per vxProcessGraph is stored in *sec_per_frame.
+int ref_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16_t *mag, uint8_t *phase,
+               int time_iters, double *sec_per_frame) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_image g = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_U8);
+    vx_image ox = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16);
+    vx_image oy = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16);
+    vx_image om = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16);
+    vx_image op = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    if (!vxGaussian3x3Node(c.graph, in, g)) return -1;
+    if (!vxSobel3x3Node(c.graph, g, ox, oy)) return -1;
+    if (!vxMagnitudeNode(c.graph, ox, oy, om)) return -1;
+    if (!vxPhaseNode(c.graph, ox, oy, op)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(ox, w, h, gx, 2));
+    CHECK(get(oy, w, h, gy, 2));
+    CHECK(get(om, w, h, mag, 2));
+    CHECK(get(op, w, h, phase, 1));
+    if (time_iters > 0 && sec_per_frame) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < time_iters; i++) CHECK(vxProcessGraph(c.graph));
+        auto t1 = std::chrono::steady_clock::now();
+        *sec_per_frame = std::chrono::duration<double>(t1 - t0).count() / time_iters;
+    }
+    return 0;
+}
+
+}  // extern "C"
