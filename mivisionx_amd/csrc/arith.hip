@@ -1,0 +1,165 @@
+// arith.hip -- Magnitude (S16,S16 -> S16) and Phase (S16,S16 -> U8) for gfx950.
+//
+// One thread owns 8 pixels of a row: one 128-bit load of gx and of gy, one 128-bit (Magnitude) or
+// 64-bit (Phase) store.  Both are elementwise and HBM-bound (6 B/px and 5 B/px).
+// Built with -ffp-contract=off and correctly rounded fp32 divide/sqrt so that float sequences
+// follow the reference CPU code operation for operation.
+#include "mvx_common.hpp"
+
+namespace mvx {
+
+constexpr int AR_BLOCK = 256;
+
+// Magnitude SIMD-body rule of ago_haf_cpu_arithmetic.cpp:5031-5078: cvtpd_epi32(sqrt_pd(gx^2+gy^2))
+// (round half to even), packs_epi32 saturation.  Evaluated exactly in integers: with
+// k = floor(sqrt(s)), round(sqrt(s)) is k+1 iff s - k^2 > k (sqrt(s) is never a half-integer for
+// integer s, and the gap to k+1/2 is far above double rounding error, so this equals the double
+// path bit for bit).
+MVX_DEV int32_t mag_body(int32_t gx, int32_t gy)
+{
+    uint32_t s = (uint32_t)(gx * gx) + (uint32_t)(gy * gy);   // <= 2^31
+    uint32_t k = (uint32_t)sqrtf((float)s);
+    while ((uint64_t)k * k > s) k--;
+    while ((uint64_t)(k + 1) * (k + 1) <= s) k++;
+    uint32_t r = (s - k * k > k) ? k + 1 : k;
+    return (int32_t)min(r, 32767u);
+}
+
+// Magnitude postfix rule (columns w & ~7 .. w-1), ago_haf_cpu_arithmetic.cpp:5087-5091:
+// (vx_int16)round(sqrtf((float)(gx*gx) + (float)(gy*gy))), float adds and sqrt correctly rounded.
+MVX_DEV int32_t mag_tail(int32_t gx, int32_t gy)
+{
+    float t = (float)(gx * gx) + (float)(gy * gy);
+    t = __fsqrt_rn(t);
+    return (int32_t)(int16_t)(int32_t)roundf(t);
+}
+
+__global__ void __launch_bounds__(AR_BLOCK)
+k_magnitude(uint32_t w, uint32_t h, int16_t *__restrict__ dst, uint32_t dstStride, const int16_t *__restrict__ gx,
+            uint32_t gxStride, const int16_t *__restrict__ gy, uint32_t gyStride)
+{
+    const uint32_t chunks = (w + 7) / 8;
+    const uint32_t tid = blockIdx.x * AR_BLOCK + threadIdx.x;
+    const uint32_t y = blockIdx.y;
+    if (tid >= chunks || y >= h) return;
+    const uint32_t x0 = tid * 8;
+    const uint32_t body = w & ~7u;
+    const int16_t *px = (const int16_t *)((const uint8_t *)gx + (long)y * gxStride) + x0;
+    const int16_t *py = (const int16_t *)((const uint8_t *)gy + (long)y * gyStride) + x0;
+    int16_t *pd = (int16_t *)((uint8_t *)dst + (long)y * dstStride) + x0;
+    // the chunk is inside the 16-byte aligned row (stride >= ALIGN16(2w)), padding lanes read 0
+    uint4 vx = *reinterpret_cast<const uint4 *>(px);
+    uint4 vy = *reinterpret_cast<const uint4 *>(py);
+    const uint32_t ax[4] = {vx.x, vx.y, vx.z, vx.w}, ay[4] = {vy.x, vy.y, vy.z, vy.w};
+    int32_t o[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        int32_t a = (int16_t)(ax[i >> 1] >> (16 * (i & 1)));
+        int32_t b = (int16_t)(ay[i >> 1] >> (16 * (i & 1)));
+        o[i] = (x0 + i < body) ? mag_body(a, b) : mag_tail(a, b);
+    }
+    if (x0 + 8 <= w) {
+        uint4 v;
+        v.x = ((uint32_t)o[0] & 0xffff) | ((uint32_t)o[1] << 16);
+        v.y = ((uint32_t)o[2] & 0xffff) | ((uint32_t)o[3] << 16);
+        v.z = ((uint32_t)o[4] & 0xffff) | ((uint32_t)o[5] << 16);
+        v.w = ((uint32_t)o[6] & 0xffff) | ((uint32_t)o[7] << 16);
+        *reinterpret_cast<uint4 *>(pd) = v;
+    } else {
+        for (uint32_t i = 0; x0 + i < w; i++) pd[i] = (int16_t)o[i];
+    }
+}
+
+// atan2 polynomial of ago_haf_cpu.h:32-36: each constant is a product of two float literals,
+// rounded to float, and the polynomial runs in float.
+constexpr float ATAN_P1 = 0.9997878412794807f * 57.29577951308232f;
+constexpr float ATAN_P3 = -0.3258083974640975f * 57.29577951308232f;
+constexpr float ATAN_P5 = 0.1555786518463281f * 57.29577951308232f;
+constexpr float ATAN_P7 = -0.04432655554792128f * 57.29577951308232f;
+
+// HafCpu_FastAtan2_deg (ago_haf_cpu_arithmetic.cpp:9564-9589) and the Phase quantisation (:9623-9626)
+MVX_DEV uint32_t phase_px(int32_t gx, int32_t gy)
+{
+    const uint32_t ax = (uint16_t)(gx < 0 ? -gx : gx), ay = (uint16_t)(gy < 0 ? -gy : gy);
+    const float eps = (float)2.2204460492503131e-16;   // (float)DBL_EPSILON
+    float a, c, c2;
+    if (ax >= ay) {
+        c = __fdiv_rn((float)ay, (float)ax + eps);
+        c2 = c * c;
+        a = (((ATAN_P7 * c2 + ATAN_P5) * c2 + ATAN_P3) * c2 + ATAN_P1) * c;
+    } else {
+        c = __fdiv_rn((float)ax, (float)ay + eps);
+        c2 = c * c;
+        a = 90.f - (((ATAN_P7 * c2 + ATAN_P5) * c2 + ATAN_P3) * c2 + ATAN_P1) * c;
+    }
+    if (gx < 0) a = 180.f - a;
+    if (gy < 0) a = 360.f - a;
+    const float scale = 128.f / 180.f;
+    return (uint32_t)((double)(a * scale) + 0.5) & 0xffu;
+}
+
+__global__ void __launch_bounds__(AR_BLOCK)
+k_phase(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStride, const int16_t *__restrict__ gx,
+        uint32_t gxStride, const int16_t *__restrict__ gy, uint32_t gyStride)
+{
+    const uint32_t chunks = (w + 7) / 8;
+    const uint32_t tid = blockIdx.x * AR_BLOCK + threadIdx.x;
+    const uint32_t y = blockIdx.y;
+    if (tid >= chunks || y >= h) return;
+    const uint32_t x0 = tid * 8;
+    const int16_t *px = (const int16_t *)((const uint8_t *)gx + (long)y * gxStride) + x0;
+    const int16_t *py = (const int16_t *)((const uint8_t *)gy + (long)y * gyStride) + x0;
+    uint8_t *pd = dst + (long)y * dstStride + x0;
+    uint4 vx = *reinterpret_cast<const uint4 *>(px);
+    uint4 vy = *reinterpret_cast<const uint4 *>(py);
+    const uint32_t ax[4] = {vx.x, vx.y, vx.z, vx.w}, ay[4] = {vy.x, vy.y, vy.z, vy.w};
+    uint32_t o[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        int32_t a = (int16_t)(ax[i >> 1] >> (16 * (i & 1)));
+        int32_t b = (int16_t)(ay[i >> 1] >> (16 * (i & 1)));
+        o[i] = phase_px(a, b);
+    }
+    if (x0 + 8 <= w) {
+        uint2 v;
+        v.x = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+        v.y = o[4] | (o[5] << 8) | (o[6] << 16) | (o[7] << 24);
+        *reinterpret_cast<uint2 *>(pd) = v;
+    } else {
+        for (uint32_t i = 0; x0 + i < w; i++) pd[i] = (uint8_t)o[i];
+    }
+}
+
+inline bool plane_ok_ar(const void *p, uint32_t w, uint32_t stride, uint32_t bpp)
+{
+    return p && aligned16(p) && (stride % 16u) == 0 && stride >= ((w * bpp + 15u) & ~15u);
+}
+
+}  // namespace mvx
+
+using namespace mvx;
+
+extern "C" int mvx_Magnitude_S16_S16S16(mvx_stream_t stream, uint32_t w, uint32_t h, int16_t *dst, uint32_t dstStride,
+                                        const int16_t *gx, uint32_t gxStride, const int16_t *gy, uint32_t gyStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok_ar(dst, w, dstStride, 2) || !plane_ok_ar(gx, w, gxStride, 2) || !plane_ok_ar(gy, w, gyStride, 2))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    dim3 grid(((w + 7) / 8 + AR_BLOCK - 1) / AR_BLOCK, h);
+    hipLaunchKernelGGL(k_magnitude, grid, dim3(AR_BLOCK), 0, stream_of(stream), w, h, dst, dstStride, gx, gxStride, gy, gyStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_Phase_U8_S16S16(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *dst, uint32_t dstStride,
+                                   const int16_t *gx, uint32_t gxStride, const int16_t *gy, uint32_t gyStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok_ar(dst, w, dstStride, 1) || !plane_ok_ar(gx, w, gxStride, 2) || !plane_ok_ar(gy, w, gyStride, 2))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    if ((dstStride & 7u) != 0) return MVX_ERROR_INVALID_PARAMETERS;
+    dim3 grid(((w + 7) / 8 + AR_BLOCK - 1) / AR_BLOCK, h);
+    hipLaunchKernelGGL(k_phase, grid, dim3(AR_BLOCK), 0, stream_of(stream), w, h, dst, dstStride, gx, gxStride, gy, gyStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}

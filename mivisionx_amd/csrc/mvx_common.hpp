@@ -1,0 +1,65 @@
+// mvx_common.hpp -- shared device helpers and launch plumbing for the gfx950 vision kernels.
+//
+// Image planes follow the reference's memory model (ago_util.cpp:187-200,1107):
+//   * row stride is ALIGN16(width * bytes_per_pixel) and the padding bytes of every row are zero;
+//   * neighbour reads are linear: byte index y*stride + x + dx, so the left neighbour of column 0
+//     is the last byte of the previous row's stride, and indices outside [0, h*stride) read 0
+//     (the reference's zero guard bytes).
+// Kernels never write padding bytes, so a plane allocated zeroed keeps zero padding forever.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/mvx_kernels.h"
+
+#define MVX_DEV __device__ __forceinline__
+
+#define MVX_LAUNCH_CHECK()                                                     \
+    do {                                                                       \
+        hipError_t _e = hipGetLastError();                                     \
+        if (_e != hipSuccess) return MVX_ERROR_LAUNCH;                         \
+    } while (0)
+
+namespace mvx {
+
+MVX_DEV uint32_t ld_u8(const uint8_t *p, long n, long i) { return (i < 0 || i >= n) ? 0u : (uint32_t)p[i]; }
+
+// 16 bytes at a 16-byte aligned address inside the allocation
+MVX_DEV uint4 ld16(const uint8_t *p) { return *reinterpret_cast<const uint4 *>(p); }
+
+MVX_DEV uint32_t byte_of(const uint4 &v, int i)
+{
+    uint32_t w = (i < 4) ? v.x : (i < 8) ? v.y : (i < 12) ? v.z : v.w;
+    return (w >> (8 * (i & 3))) & 0xffu;
+}
+
+MVX_DEV void unpack16(const uint4 &v, uint32_t (&o)[16])
+{
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 16; i++) o[i] = (w[i >> 2] >> (8 * (i & 3))) & 0xffu;
+}
+
+MVX_DEV uint4 pack16(const uint32_t (&o)[16])
+{
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        w[k] = (o[4 * k] & 0xff) | ((o[4 * k + 1] & 0xff) << 8) | ((o[4 * k + 2] & 0xff) << 16) | ((o[4 * k + 3] & 0xff) << 24);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Store `n` (1..16) leading bytes of a 16-byte chunk; full chunks use one 128-bit store.
+MVX_DEV void st16_partial(uint8_t *p, const uint32_t (&o)[16], int n)
+{
+    if (n >= 16) {
+        *reinterpret_cast<uint4 *>(p) = pack16(o);
+    } else {
+        for (int i = 0; i < n; i++) p[i] = (uint8_t)o[i];
+    }
+}
+
+inline hipStream_t stream_of(mvx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace mvx

@@ -1,0 +1,109 @@
+"""Seeded synthetic inputs for the vision-graph workloads (the reference ships no image data).
+
+Random numbers come from a counter-based generator: value i of stream (seed, stream) is the upper
+32 bits of splitmix64's finaliser applied to i plus a per-stream key, so any slice can be produced
+independently and the sequence never depends on a library's RNG implementation.  Tests commit
+SHA-256 hashes of generated inputs, which pins the generator.
+
+Workloads
+* ``uniform_u8(w, h, seed)``: i.i.d. uniform U8 (the 1920x1080 Gaussian->Sobel->Magnitude/Phase graph).
+* ``scene_rgb(w, h, seed)``: per channel, 128 + the sum of 8 random 2-D sinusoids (amplitude
+  U(10,40), period log-uniform in [16,512] px, orientation U(0,pi), phase U(0,2pi)), then 40
+  axis-aligned rectangles of constant random intensity (corner uniform over the image, size
+  U(16,512), painted in order), then N(0, 8^2) noise by Box-Muller, rounded half-to-even and
+  clipped to [0,255] (the 3840x2160 vision graph input).
+* ``scene_u8(w, h, seed)``: the same process for one channel.
+"""
+import numpy as np
+
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(x):
+    x = x ^ (x >> np.uint64(30))
+    x = x * _M1
+    x = x ^ (x >> np.uint64(27))
+    x = x * _M2
+    return x ^ (x >> np.uint64(31))
+
+
+def u32(seed, stream, n, offset=0):
+    """n uint32 values of stream (seed, stream), starting at counter `offset`."""
+    with np.errstate(over="ignore"):
+        key = _mix64(np.array([seed], dtype=np.uint64) * _GOLD + np.uint64(stream + 1))[0]
+        idx = np.arange(offset, offset + n, dtype=np.uint64)
+        return (_mix64(idx * _GOLD + key) >> np.uint64(32)).astype(np.uint32)
+
+
+def uniform01(seed, stream, n, offset=0):
+    """n doubles in [0, 1) with 32-bit resolution."""
+    return u32(seed, stream, n, offset).astype(np.float64) * (1.0 / 4294967296.0)
+
+
+def uniform_u8(w, h, seed=0):
+    return (u32(seed, 1, w * h) >> np.uint32(24)).astype(np.uint8).reshape(h, w)
+
+
+def _scene_channel(w, h, seed, ch):
+    p = uniform01(seed, 100 + ch, 8 * 4)
+    xs = np.arange(w, dtype=np.float64)
+    ys = np.arange(h, dtype=np.float64)
+    img = np.full((h, w), 128.0)
+    for k in range(8):
+        amp = 10.0 + 30.0 * p[4 * k]
+        period = 16.0 * np.exp(np.log(512.0 / 16.0) * p[4 * k + 1])
+        theta = np.pi * p[4 * k + 2]
+        phase = 2.0 * np.pi * p[4 * k + 3]
+        a = 2.0 * np.pi * np.cos(theta) / period
+        b = 2.0 * np.pi * np.sin(theta) / period
+        # sin(a x + b y + phase) = sin(a x) cos(b y + phase) + cos(a x) sin(b y + phase)
+        sx, cx = np.sin(a * xs), np.cos(a * xs)
+        sy, cy = np.sin(b * ys + phase), np.cos(b * ys + phase)
+        img += amp * (np.outer(cy, sx) + np.outer(sy, cx))
+    return img
+
+
+def _rects(w, h, seed, nch):
+    r = uniform01(seed, 200, 40 * (4 + nch))
+    out = []
+    for k in range(40):
+        q = r[k * (4 + nch):(k + 1) * (4 + nch)]
+        x0 = int(q[0] * w)
+        y0 = int(q[1] * h)
+        rw = 16 + int(q[2] * (512 - 16 + 1))
+        rh = 16 + int(q[3] * (512 - 16 + 1))
+        vals = [int(v * 256) for v in q[4:]]
+        out.append((x0, y0, rw, rh, vals))
+    return out
+
+
+def _noise(w, h, seed, ch, sigma=8.0):
+    n = w * h
+    u1 = uniform01(seed, 300 + ch, n)
+    u2 = uniform01(seed, 400 + ch, n)
+    u1 = 1.0 - u1   # (0, 1]
+    return (sigma * np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(h, w)
+
+
+def _scene(w, h, seed, nch):
+    rects = _rects(w, h, seed, nch)
+    planes = []
+    for ch in range(nch):
+        img = _scene_channel(w, h, seed, ch)
+        for (x0, y0, rw, rh, vals) in rects:
+            img[y0:y0 + rh, x0:x0 + rw] = vals[ch]
+        img += _noise(w, h, seed, ch)
+        planes.append(np.clip(np.rint(img), 0, 255).astype(np.uint8))
+    return planes
+
+
+def scene_rgb(w, h, seed=0):
+    """(h, w, 3) uint8 RGB scene."""
+    return np.ascontiguousarray(np.stack(_scene(w, h, seed, 3), axis=-1))
+
+
+def scene_u8(w, h, seed=0):
+    """(h, w) uint8 single-channel scene."""
+    return _scene(w, h, seed, 1)[0]

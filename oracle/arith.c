@@ -47,12 +47,13 @@ void oracle_magnitude(int w, int h, int16_t *mag, int mag_stride, const int16_t 
     }
 }
 
-/* atan2 polynomial constants: ago_haf_cpu.h:32-36.  Each is a float literal multiplied by a
- * double literal, so it is a double, and the polynomial below is evaluated in double. */
-#define P1 ((double)0.9997878412794807f * 57.29577951308232)
-#define P3 ((double)-0.3258083974640975f * 57.29577951308232)
-#define P5 ((double)0.1555786518463281f * 57.29577951308232)
-#define P7 ((double)-0.04432655554792128f * 57.29577951308232)
+/* atan2 polynomial constants: ago_haf_cpu.h:32-36.  Both factors are float literals
+ * (0.9997878412794807f * 57.29577951308232f), so each constant is a rounded float product and the
+ * polynomial below is evaluated in float. */
+#define P1 (0.9997878412794807f * 57.29577951308232f)
+#define P3 (-0.3258083974640975f * 57.29577951308232f)
+#define P5 (0.1555786518463281f * 57.29577951308232f)
+#define P7 (-0.04432655554792128f * 57.29577951308232f)
 
 /* HafCpu_FastAtan2_deg: ago_haf_cpu_arithmetic.cpp:9564-9589. */
 static float fast_atan2_deg(int gx, int gy)
@@ -62,11 +63,11 @@ static float fast_atan2_deg(int gx, int gy)
     if (ax >= ay) {
         c = (float)ay / ((float)ax + (float)DBL_EPSILON);
         c2 = c * c;
-        a = (float)((((P7 * c2 + P5) * c2 + P3) * c2 + P1) * c);
+        a = (((P7 * c2 + P5) * c2 + P3) * c2 + P1) * c;
     } else {
         c = (float)ax / ((float)ay + (float)DBL_EPSILON);
         c2 = c * c;
-        a = (float)(90.f - (((P7 * c2 + P5) * c2 + P3) * c2 + P1) * c);
+        a = 90.f - (((P7 * c2 + P5) * c2 + P3) * c2 + P1) * c;
     }
     if (gx < 0) a = 180.f - a;
     if (gy < 0) a = 360.f - a;
