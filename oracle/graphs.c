@@ -78,3 +78,100 @@ void oracle_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, i
     plane_get(&op, w, h, 1, phase);
     free(in.p); free(g.p); free(ox.p); free(oy.p); free(om.p); free(op.p);
 }
+
+/* ColorConvert RGB -> IYUV (virtual) -> ChannelExtract Y: the reference keeps only
+ * COLOR_CONVERT_Y_RGB (the U/V kernels have no reader and are removed, the channel copy of a
+ * virtual plane is removed). */
+void oracle_color_y_node(int w, int h, const uint8_t *rgb, uint8_t *y)
+{
+    plane_t in = plane_new(w, h, 3), out = plane_new(w, h, 1);
+    plane_put(&in, w, h, 3, rgb);
+    oracle_color_y_rgb(w, h, out.p, out.stride, in.p, in.stride);
+    plane_get(&out, w, h, 1, y);
+    free(in.p); free(out.p);
+}
+
+void oracle_scale_node(int sw, int sh, const uint8_t *src, int dw, int dh, uint8_t *dst)
+{
+    plane_t in = plane_new(sw, sh, 1), out = plane_new(dw, dh, 1);
+    plane_put(&in, sw, sh, 1, src);
+    oracle_scale_bilinear(dw, dh, out.p, out.stride, sw, sh, in.p, in.stride);
+    plane_get(&out, dw, dh, 1, dst);
+    free(in.p); free(out.p);
+}
+
+/* Level sizes of a HALF pyramid: ceil(0.5 * previous) (ago_util.cpp:1312-1334). */
+void oracle_pyramid_dims(int w, int h, int levels, int *ws, int *hs)
+{
+    for (int l = 0; l < levels; l++) {
+        ws[l] = w; hs[l] = h;
+        w = (w + 1) / 2; h = (h + 1) / 2;
+    }
+}
+
+/* GaussianPyramid HALF: level 0 = channel copy, level l = ScaleGaussianHalf_5x5(level l-1)
+ * (ago_drama_divide.cpp:900-922).  `out` receives the packed levels back to back. */
+int oracle_pyramid_node(int w, int h, const uint8_t *src, int levels, uint8_t *out)
+{
+    int ws[32], hs[32];
+    oracle_pyramid_dims(w, h, levels, ws, hs);
+    plane_t prev = plane_new(w, h, 1);
+    plane_put(&prev, w, h, 1, src);
+    int rc = 0;
+    size_t off = 0;
+    plane_get(&prev, w, h, 1, out);
+    off += (size_t)w * h;
+    for (int l = 1; l < levels; l++) {
+        plane_t cur = plane_new(ws[l], hs[l], 1);
+        if (oracle_scale_gaussian_half_5x5(ws[l], hs[l], cur.p, cur.stride, ws[l - 1], hs[l - 1], prev.p, prev.stride))
+            rc = -1;
+        plane_get(&cur, ws[l], hs[l], 1, out + off);
+        off += (size_t)ws[l] * hs[l];
+        free(prev.p);
+        prev = cur;
+    }
+    free(prev.p);
+    return rc;
+}
+
+/* CannyEdgeDetector (gradient 3, L1): CANNY_SOBEL_U16_U8_3x3_L1NORM -> CANNY_SUPP_THRESHOLD_U8XY_U16_3x3
+ * -> CANNY_EDGE_TRACE_U8_U8XY (ago_drama_divide.cpp:1618-1701, USE_AGO_CANNY_SOBEL_SUPP_THRESHOLD 0). */
+void oracle_canny_node(int w, int h, const uint8_t *src, int lower, int upper, uint8_t *dst)
+{
+    plane_t in = plane_new(w, h, 1), mag = plane_new(w, h, 2), out = plane_new(w, h, 1);
+    plane_put(&in, w, h, 1, src);
+    oracle_canny_sobel_3x3_l1(w, h, (uint16_t *)mag.p, mag.stride, in.p, in.stride);
+    oracle_canny_supp_threshold(w, h, out.p, out.stride, (const uint16_t *)mag.p, mag.stride, lower, upper);
+    oracle_canny_trace(w, h, out.p, out.stride);
+    plane_get(&out, w, h, 1, dst);
+    free(in.p); free(mag.p); free(out.p);
+}
+
+/* Canny's U16 gradient plane alone (for kernel-level tests) */
+void oracle_canny_sobel_node(int w, int h, const uint8_t *src, uint16_t *dst)
+{
+    plane_t in = plane_new(w, h, 1), mag = plane_new(w, h, 2);
+    plane_put(&in, w, h, 1, src);
+    oracle_canny_sobel_3x3_l1(w, h, (uint16_t *)mag.p, mag.stride, in.p, in.stride);
+    plane_get(&mag, w, h, 2, dst);
+    free(in.p); free(mag.p);
+}
+
+/* HarrisCorners (gradient 3, block 3): HARRIS_SOBEL_HG3 -> HARRIS_SCORE_HVC -> NON_MAX_SUPP_XY_ANY_3x3
+ * (XYS capacity w*h) -> HARRIS_MERGE_SORT_AND_PICK (ago_drama_divide.cpp:1703-1787).  Returns the
+ * uncapped corner count; `hvc` (w*h floats, may be NULL) receives the score plane. */
+int oracle_harris_node(int w, int h, const uint8_t *src, float k, float thr, float min_distance,
+                       oracle_keypoint_t *out, int capacity, float *hvc)
+{
+    plane_t in = plane_new(w, h, 1), hg3 = plane_new(w, h, 12), vc = plane_new(w, h, 4);
+    plane_put(&in, w, h, 1, src);
+    oracle_harris_sobel_hg3(w, h, (float *)hg3.p, hg3.stride, in.p, in.stride);
+    oracle_harris_score(w, h, (float *)vc.p, vc.stride, (const float *)hg3.p, hg3.stride, k, thr, oracle_harris_norm(3));
+    if (hvc) plane_get(&vc, w, h, 4, hvc);
+    int cap = w * h;
+    oracle_xys_t *list = (oracle_xys_t *)malloc(sizeof(oracle_xys_t) * (size_t)(cap > 0 ? cap : 1));
+    int n = oracle_nonmax_xys(w, h, (const float *)vc.p, vc.stride, list, cap);
+    int count = oracle_harris_pick(list, n, min_distance, w, h, out, capacity);
+    free(list); free(in.p); free(hg3.p); free(vc.p);
+    return count;
+}

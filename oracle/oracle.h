@@ -44,12 +44,43 @@ void oracle_phase(int w, int h, uint8_t *phase, int phase_stride, const int16_t 
 int16_t oracle_magnitude_px(int gx, int gy);
 uint8_t oracle_phase_px(int gx, int gy);
 
+/* ---- colour, scaling, pyramid (geometric.c) ---- */
+void oracle_color_y_rgb(int w, int h, uint8_t *dst, int dst_stride, const uint8_t *rgb, int rgb_stride);
+void oracle_scale_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                           int src_stride);
+int oracle_scale_gaussian_half_5x5(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh,
+                                   const uint8_t *src, int src_stride);
+
+/* ---- Canny and Harris (vision.c) ---- */
+typedef struct { int16_t x, y; float s; } oracle_xys_t;                 /* ago_keypoint_xys_t */
+typedef struct { int32_t x, y; float strength, scale, orientation; int32_t tracking_status; float error; }
+    oracle_keypoint_t;                                                   /* vx_keypoint_t */
+void oracle_canny_sobel_3x3_l1(int w, int h, uint16_t *dst, int dst_stride, const uint8_t *src, int src_stride);
+int oracle_canny_supp_threshold(int w, int h, uint8_t *dst, int dst_stride, const uint16_t *mag, int mag_stride,
+                                int lower, int upper);
+void oracle_canny_trace(int w, int h, uint8_t *img, int stride);
+void oracle_harris_sobel_hg3(int w, int h, float *dst, int dst_stride, const uint8_t *src, int src_stride);
+void oracle_harris_score(int w, int h, float *dst, int dst_stride, const float *hg3, int hg3_stride,
+                         float k, float thr, float norm);
+float oracle_harris_norm(int gradient_size);
+int oracle_nonmax_xys(int w, int h, const float *img, int stride, oracle_xys_t *list, int capacity);
+int oracle_harris_pick(oracle_xys_t *list, int n, float min_distance, int width, int height,
+                       oracle_keypoint_t *out, int capacity);
+
 /* ---- nodes and graphs on reference-layout planes (graphs.c); packed in/out ---- */
 void oracle_filter3x3(int which, int w, int h, const uint8_t *src, uint8_t *dst);
 void oracle_sobel_node(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy);
 /* Gaussian3x3 (virtual) -> Sobel3x3 -> Magnitude + Phase */
 void oracle_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16_t *mag,
                    uint8_t *phase);
+void oracle_color_y_node(int w, int h, const uint8_t *rgb, uint8_t *y);
+void oracle_scale_node(int sw, int sh, const uint8_t *src, int dw, int dh, uint8_t *dst);
+void oracle_pyramid_dims(int w, int h, int levels, int *ws, int *hs);
+int oracle_pyramid_node(int w, int h, const uint8_t *src, int levels, uint8_t *out);
+void oracle_canny_node(int w, int h, const uint8_t *src, int lower, int upper, uint8_t *dst);
+void oracle_canny_sobel_node(int w, int h, const uint8_t *src, uint16_t *dst);
+int oracle_harris_node(int w, int h, const uint8_t *src, float k, float thr, float min_distance,
+                       oracle_keypoint_t *out, int capacity, float *hvc);
 
 #ifdef __cplusplus
 }

@@ -4,7 +4,8 @@
 * ``Reference``: the reference's own CPU engine driven through oracle/_ref/libref_harness.so,
   present only where oracle/_ref was built from the reference sources.
 
-Every function takes and returns packed numpy arrays (row stride = width * bytes per pixel).
+Every function takes and returns packed numpy arrays (row stride = width * bytes per pixel); the
+oracle and the reference harness lay planes out with the reference's stride internally.
 """
 import ctypes
 import os
@@ -16,56 +17,157 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libref_harness.so")
 
+# vx_keypoint_t
+KEYPOINT = np.dtype([("x", np.int32), ("y", np.int32), ("strength", np.float32), ("scale", np.float32),
+                     ("orientation", np.float32), ("tracking_status", np.int32), ("error", np.float32)])
+
 
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a):
+    return np.ascontiguousarray(a)
 
 
 def build_oracle():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def pyramid_dims(w, h, levels):
+    """HALF pyramid level sizes: ceil(0.5 * previous) (ago_util.cpp:1312-1334)"""
+    dims = []
+    for _ in range(levels):
+        dims.append((w, h))
+        w, h = (w + 1) // 2, (h + 1) // 2
+    return dims
+
+
+def split_levels(flat, dims):
+    out, off = [], 0
+    for (w, h) in dims:
+        out.append(flat[off:off + w * h].reshape(h, w))
+        off += w * h
+    return out
+
+
 class Oracle:
     def __init__(self):
         if not os.path.exists(ORACLE_SO):
             build_oracle()
-        self.lib = ctypes.CDLL(ORACLE_SO)
-        self.lib.oracle_magnitude_px.restype = ctypes.c_int16
-        self.lib.oracle_phase_px.restype = ctypes.c_uint8
+        L = self.lib = ctypes.CDLL(ORACLE_SO)
+        L.oracle_magnitude_px.restype = ctypes.c_int16
+        L.oracle_phase_px.restype = ctypes.c_uint8
+        L.oracle_harris_norm.restype = ctypes.c_float
 
+    # ---- 3x3 filters, Sobel, Magnitude, Phase ----
     def filter3x3(self, which, src):
         h, w = src.shape
         dst = np.zeros((h, w), np.uint8)
-        self.lib.oracle_filter3x3(which, w, h, _p(np.ascontiguousarray(src)), _p(dst))
+        self.lib.oracle_filter3x3(which, w, h, _p(_c(src)), _p(dst))
         return dst
 
     def sobel(self, src):
         h, w = src.shape
         gx = np.zeros((h, w), np.int16)
         gy = np.zeros((h, w), np.int16)
-        self.lib.oracle_sobel_node(w, h, _p(np.ascontiguousarray(src)), _p(gx), _p(gy))
+        self.lib.oracle_sobel_node(w, h, _p(_c(src)), _p(gx), _p(gy))
         return gx, gy
 
     def magnitude(self, gx, gy):
         h, w = gx.shape
         out = np.zeros((h, w), np.int16)
-        self.lib.oracle_magnitude(w, h, _p(out), 2 * w, _p(np.ascontiguousarray(gx)), 2 * w,
-                                  _p(np.ascontiguousarray(gy)), 2 * w)
+        self.lib.oracle_magnitude(w, h, _p(out), 2 * w, _p(_c(gx)), 2 * w, _p(_c(gy)), 2 * w)
         return out
 
     def phase(self, gx, gy):
         h, w = gx.shape
         out = np.zeros((h, w), np.uint8)
-        self.lib.oracle_phase(w, h, _p(out), w, _p(np.ascontiguousarray(gx)), 2 * w,
-                              _p(np.ascontiguousarray(gy)), 2 * w)
+        self.lib.oracle_phase(w, h, _p(out), w, _p(_c(gx)), 2 * w, _p(_c(gy)), 2 * w)
         return out
 
     def chain1(self, src):
         h, w = src.shape
         gx = np.zeros((h, w), np.int16); gy = np.zeros((h, w), np.int16)
         mag = np.zeros((h, w), np.int16); ph = np.zeros((h, w), np.uint8)
-        self.lib.oracle_chain1(w, h, _p(np.ascontiguousarray(src)), _p(gx), _p(gy), _p(mag), _p(ph))
+        self.lib.oracle_chain1(w, h, _p(_c(src)), _p(gx), _p(gy), _p(mag), _p(ph))
         return gx, gy, mag, ph
+
+    # ---- colour, scale, pyramid ----
+    def color_y(self, rgb):
+        h, w = rgb.shape[:2]
+        y = np.zeros((h, w), np.uint8)
+        self.lib.oracle_color_y_node(w, h, _p(_c(rgb)), _p(y))
+        return y
+
+    def scale(self, src, dw, dh):
+        sh, sw = src.shape
+        d = np.zeros((dh, dw), np.uint8)
+        self.lib.oracle_scale_node(sw, sh, _p(_c(src)), dw, dh, _p(d))
+        return d
+
+    def pyramid(self, src, levels):
+        h, w = src.shape
+        dims = pyramid_dims(w, h, levels)
+        flat = np.zeros(sum(a * b for a, b in dims), np.uint8)
+        if self.lib.oracle_pyramid_node(w, h, _p(_c(src)), levels, _p(flat)) != 0:
+            raise ValueError("pyramid level too small (ScaleGaussianHalf needs src >= 5, dst >= 3)")
+        return split_levels(flat, dims)
+
+    def gaussian_half(self, src, dw, dh):
+        """one ScaleGaussianHalf_5x5 on reference-layout planes"""
+        sh, sw = src.shape
+        ss, ds = (sw + 15) & ~15, (dw + 15) & ~15
+        s = np.zeros((sh, ss), np.uint8); s[:, :sw] = src
+        d = np.zeros((dh, ds), np.uint8)
+        if self.lib.oracle_scale_gaussian_half_5x5(dw, dh, _p(d), ds, sw, sh, _p(s), ss) != 0:
+            raise ValueError("ScaleGaussianHalf needs src >= 5 and dst >= 3")
+        return np.ascontiguousarray(d[:, :dw])
+
+    # ---- Canny ----
+    def canny(self, src, lower, upper):
+        h, w = src.shape
+        d = np.zeros((h, w), np.uint8)
+        self.lib.oracle_canny_node(w, h, _p(_c(src)), lower, upper, _p(d))
+        return d
+
+    def canny_sobel(self, src):
+        h, w = src.shape
+        d = np.zeros((h, w), np.uint16)
+        self.lib.oracle_canny_sobel_node(w, h, _p(_c(src)), _p(d))
+        return d
+
+    def canny_supp_threshold(self, mag, lower, upper):
+        h, w = mag.shape
+        d = np.zeros((h, w), np.uint8)
+        n = self.lib.oracle_canny_supp_threshold(w, h, _p(d), w, _p(_c(mag)), 2 * w, lower, upper)
+        return d, n
+
+    def canny_trace(self, edges):
+        h, w = edges.shape
+        img = np.ascontiguousarray(edges).copy()
+        self.lib.oracle_canny_trace(w, h, _p(img), w)
+        return img
+
+    # ---- Harris ----
+    def harris(self, src, k, thr, dist, capacity, with_score=False):
+        h, w = src.shape
+        out = np.zeros(max(capacity, 1), KEYPOINT)
+        hvc = np.zeros((h, w), np.float32)
+        n = self.lib.oracle_harris_node(w, h, _p(_c(src)), ctypes.c_float(k), ctypes.c_float(thr),
+                                        ctypes.c_float(dist), _p(out), capacity, _p(hvc))
+        res = (out[:min(n, capacity)], n)
+        return res + (hvc,) if with_score else res
+
+    def harris_norm(self, gradient_size=3):
+        return self.lib.oracle_harris_norm(gradient_size)
+
+    # ---- the 4K vision graph ----
+    def graph2(self, rgb, dw, dh, levels, lower, upper, k, thr, dist, capacity):
+        y = self.color_y(rgb)
+        kp, count = self.harris(y, k, thr, dist, capacity)
+        return dict(scaled=self.scale(y, dw, dh), pyramid=self.pyramid(y, levels), edges=self.canny(y, lower, upper),
+                    corners=kp, count=count)
 
 
 class Reference:
@@ -83,22 +185,21 @@ class Reference:
     def filter3x3(self, which, src):
         h, w = src.shape
         dst = np.zeros((h, w), np.uint8)
-        self._ok(self.lib.ref_filter3x3(which, w, h, _p(np.ascontiguousarray(src)), _p(dst)), "filter3x3")
+        self._ok(self.lib.ref_filter3x3(which, w, h, _p(_c(src)), _p(dst)), "filter3x3")
         return dst
 
     def sobel(self, src):
         h, w = src.shape
         gx = np.zeros((h, w), np.int16)
         gy = np.zeros((h, w), np.int16)
-        self._ok(self.lib.ref_sobel3x3(w, h, _p(np.ascontiguousarray(src)), _p(gx), _p(gy)), "sobel")
+        self._ok(self.lib.ref_sobel3x3(w, h, _p(_c(src)), _p(gx), _p(gy)), "sobel")
         return gx, gy
 
     def magnitude_phase(self, gx, gy):
         h, w = gx.shape
         mag = np.zeros((h, w), np.int16)
         ph = np.zeros((h, w), np.uint8)
-        self._ok(self.lib.ref_magnitude_phase(w, h, _p(np.ascontiguousarray(gx)), _p(np.ascontiguousarray(gy)),
-                                              _p(mag), _p(ph)), "magnitude_phase")
+        self._ok(self.lib.ref_magnitude_phase(w, h, _p(_c(gx)), _p(_c(gy)), _p(mag), _p(ph)), "magnitude_phase")
         return mag, ph
 
     def chain1(self, src, time_iters=0):
@@ -106,8 +207,60 @@ class Reference:
         gx = np.zeros((h, w), np.int16); gy = np.zeros((h, w), np.int16)
         mag = np.zeros((h, w), np.int16); ph = np.zeros((h, w), np.uint8)
         sec = ctypes.c_double(0.0)
-        self._ok(self.lib.ref_chain1(w, h, _p(np.ascontiguousarray(src)), _p(gx), _p(gy), _p(mag), _p(ph),
+        self._ok(self.lib.ref_chain1(w, h, _p(_c(src)), _p(gx), _p(gy), _p(mag), _p(ph),
                                      time_iters, ctypes.byref(sec)), "chain1")
         if time_iters:
             return (gx, gy, mag, ph), sec.value
         return gx, gy, mag, ph
+
+    def color_y(self, rgb):
+        h, w = rgb.shape[:2]
+        y = np.zeros((h, w), np.uint8)
+        self._ok(self.lib.ref_color_y(w, h, _p(_c(rgb)), _p(y)), "color_y")
+        return y
+
+    def scale(self, src, dw, dh):
+        sh, sw = src.shape
+        d = np.zeros((dh, dw), np.uint8)
+        self._ok(self.lib.ref_scale(sw, sh, _p(_c(src)), dw, dh, _p(d)), "scale")
+        return d
+
+    def pyramid(self, src, levels):
+        h, w = src.shape
+        dims = pyramid_dims(w, h, levels)
+        flat = np.zeros(sum(a * b for a, b in dims), np.uint8)
+        self._ok(self.lib.ref_pyramid(w, h, _p(_c(src)), levels, _p(flat)), "pyramid")
+        return split_levels(flat, dims)
+
+    def canny(self, src, lower, upper):
+        h, w = src.shape
+        d = np.zeros((h, w), np.uint8)
+        self._ok(self.lib.ref_canny(w, h, _p(_c(src)), lower, upper, _p(d)), "canny")
+        return d
+
+    def harris(self, src, k, thr, dist, capacity):
+        h, w = src.shape
+        out = np.zeros(max(capacity, 1), KEYPOINT)
+        n = ctypes.c_int(0)
+        cnt = ctypes.c_long(0)
+        self._ok(self.lib.ref_harris(w, h, _p(_c(src)), ctypes.c_float(k), ctypes.c_float(thr), ctypes.c_float(dist),
+                                     capacity, _p(out), ctypes.byref(n), ctypes.byref(cnt)), "harris")
+        return out[:n.value], cnt.value
+
+    def graph2(self, rgb, dw, dh, levels, lower, upper, k, thr, dist, capacity, time_iters=0):
+        h, w = rgb.shape[:2]
+        dims = pyramid_dims(w, h, levels)
+        scaled = np.zeros((dh, dw), np.uint8)
+        flat = np.zeros(sum(a * b for a, b in dims), np.uint8)
+        edges = np.zeros((h, w), np.uint8)
+        kp = np.zeros(max(capacity, 1), KEYPOINT)
+        n = ctypes.c_int(0)
+        cnt = ctypes.c_long(0)
+        sec = ctypes.c_double(0)
+        self._ok(self.lib.ref_graph2(w, h, _p(_c(rgb)), dw, dh, _p(scaled), levels, _p(flat), lower, upper, _p(edges),
+                                     ctypes.c_float(k), ctypes.c_float(thr), ctypes.c_float(dist), capacity, _p(kp),
+                                     ctypes.byref(n), ctypes.byref(cnt), time_iters, ctypes.byref(sec)), "graph2")
+        out = dict(scaled=scaled, pyramid=split_levels(flat, dims), edges=edges, corners=kp[:n.value], count=cnt.value)
+        if time_iters:
+            out["sec_per_frame"] = sec.value
+        return out

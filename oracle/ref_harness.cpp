@@ -147,4 +147,141 @@ int ref_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16
     return 0;
 }
 
+// ColorConvert RGB -> IYUV (virtual) -> ChannelExtract Y.
+int ref_color_y(int w, int h, const uint8_t *rgb, uint8_t *y) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_RGB, rgb);
+    vx_image iyuv = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_IYUV);
+    vx_image oy = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    if (!vxColorConvertNode(c.graph, in, iyuv)) return -1;
+    if (!vxChannelExtractNode(c.graph, iyuv, VX_CHANNEL_Y, oy)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(oy, w, h, y, 1));
+    return 0;
+}
+
+// ScaleImage bilinear (border UNDEFINED).
+int ref_scale(int sw, int sh, const uint8_t *src, int dw, int dh, uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    if (!vxScaleImageNode(c.graph, in, out, VX_INTERPOLATION_BILINEAR)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(out, dw, dh, dst, 1));
+    return 0;
+}
+
+static int get_pyramid(vx_pyramid pyr, int levels, uint8_t *out) {
+    size_t off = 0;
+    for (int l = 0; l < levels; l++) {
+        vx_image img = vxGetPyramidLevel(pyr, l);
+        vx_uint32 lw = 0, lh = 0;
+        vxQueryImage(img, VX_IMAGE_WIDTH, &lw, sizeof(lw));
+        vxQueryImage(img, VX_IMAGE_HEIGHT, &lh, sizeof(lh));
+        CHECK(get(img, lw, lh, out + off, 1));
+        off += (size_t)lw * lh;
+        vxReleaseImage(&img);
+    }
+    return 0;
+}
+
+// GaussianPyramid HALF; `out` receives the levels packed back to back.
+int ref_pyramid(int w, int h, const uint8_t *src, int levels, uint8_t *out) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_pyramid pyr = vxCreatePyramid(c.ctx, levels, VX_SCALE_PYRAMID_HALF, w, h, VX_DF_IMAGE_U8);
+    if (!vxGaussianPyramidNode(c.graph, in, pyr)) return -1;
+    int s = run(c); if (s) return s;
+    return get_pyramid(pyr, levels, out);
+}
+
+static vx_threshold make_range(Ctx &c, int lower, int upper) {
+    vx_threshold thr = vxCreateThresholdForImage(c.ctx, VX_THRESHOLD_TYPE_RANGE, VX_DF_IMAGE_U8, VX_DF_IMAGE_U8);
+    vx_pixel_value_t lo, hi;
+    memset(&lo, 0, sizeof(lo)); memset(&hi, 0, sizeof(hi));
+    lo.U8 = (vx_uint8)lower; hi.U8 = (vx_uint8)upper;
+    vxCopyThresholdRange(thr, &lo, &hi, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST);
+    return thr;
+}
+
+// CannyEdgeDetector, gradient size 3, L1 norm.
+int ref_canny(int w, int h, const uint8_t *src, int lower, int upper, uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    vx_threshold thr = make_range(c, lower, upper);
+    if (!vxCannyEdgeDetectorNode(c.graph, in, thr, 3, VX_NORM_L1, out)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(out, w, h, dst, 1));
+    return 0;
+}
+
+static vx_node harris_node(Ctx &c, vx_image in, float k, float thr, float dist, vx_array corners, vx_scalar num) {
+    vx_scalar sthr = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &thr);
+    vx_scalar sdist = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &dist);
+    vx_scalar sk = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &k);
+    return vxHarrisCornersNode(c.graph, in, sthr, sdist, sk, 3, 3, corners, num);
+}
+
+static int get_corners(vx_array corners, vx_scalar num, vx_keypoint_t *out, int *numitems, long *count) {
+    vx_size n = 0, total = 0;
+    vxQueryArray(corners, VX_ARRAY_NUMITEMS, &n, sizeof(n));
+    if (n) CHECK(vxCopyArrayRange(corners, 0, n, sizeof(vx_keypoint_t), out, VX_READ_ONLY, VX_MEMORY_TYPE_HOST));
+    CHECK(vxCopyScalar(num, &total, VX_READ_ONLY, VX_MEMORY_TYPE_HOST));
+    *numitems = (int)n;
+    *count = (long)total;
+    return 0;
+}
+
+// HarrisCorners, gradient 3, block 3.  *numitems = items in the array (<= capacity), *count = the
+// num_corners scalar (uncapped).
+int ref_harris(int w, int h, const uint8_t *src, float k, float thr, float dist, int capacity,
+               vx_keypoint_t *out, int *numitems, long *count) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_array corners = vxCreateArray(c.ctx, VX_TYPE_KEYPOINT, capacity);
+    vx_size zero = 0;
+    vx_scalar num = vxCreateScalar(c.ctx, VX_TYPE_SIZE, &zero);
+    if (!harris_node(c, in, k, thr, dist, corners, num)) return -1;
+    int s = run(c); if (s) return s;
+    return get_corners(corners, num, out, numitems, count);
+}
+
+// The 4K vision graph: RGB -> IYUV (virtual) -> Y (virtual); Y -> ScaleImage (dw x dh, bilinear);
+// Y -> GaussianPyramid (levels, HALF); Y -> Canny (3, L1, lower/upper); Y -> Harris (k, thr, dist, 3, 3).
+// If time_iters > 0 the graph is processed that many more times; *sec_per_frame gets the mean.
+int ref_graph2(int w, int h, const uint8_t *rgb, int dw, int dh, uint8_t *scaled, int levels, uint8_t *pyr_out,
+               int lower, int upper, uint8_t *edges, float k, float thr, float dist, int capacity,
+               vx_keypoint_t *kp, int *numitems, long *count, int time_iters, double *sec_per_frame) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_RGB, rgb);
+    vx_image iyuv = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_IYUV);
+    vx_image y = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_U8);
+    vx_image sc = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    vx_pyramid pyr = vxCreatePyramid(c.ctx, levels, VX_SCALE_PYRAMID_HALF, w, h, VX_DF_IMAGE_U8);
+    vx_image ed = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    vx_threshold th = make_range(c, lower, upper);
+    vx_array corners = vxCreateArray(c.ctx, VX_TYPE_KEYPOINT, capacity);
+    vx_size zero = 0;
+    vx_scalar num = vxCreateScalar(c.ctx, VX_TYPE_SIZE, &zero);
+    if (!vxColorConvertNode(c.graph, in, iyuv)) return -1;
+    if (!vxChannelExtractNode(c.graph, iyuv, VX_CHANNEL_Y, y)) return -1;
+    if (!vxScaleImageNode(c.graph, y, sc, VX_INTERPOLATION_BILINEAR)) return -1;
+    if (!vxGaussianPyramidNode(c.graph, y, pyr)) return -1;
+    if (!vxCannyEdgeDetectorNode(c.graph, y, th, 3, VX_NORM_L1, ed)) return -1;
+    if (!harris_node(c, y, k, thr, dist, corners, num)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(sc, dw, dh, scaled, 1));
+    if (get_pyramid(pyr, levels, pyr_out)) return -1;
+    CHECK(get(ed, w, h, edges, 1));
+    if (get_corners(corners, num, kp, numitems, count)) return -1;
+    if (time_iters > 0 && sec_per_frame) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < time_iters; i++) CHECK(vxProcessGraph(c.graph));
+        auto t1 = std::chrono::steady_clock::now();
+        *sec_per_frame = std::chrono::duration<double>(t1 - t0).count() / time_iters;
+    }
+    return 0;
+}
+
 }  // extern "C"
