@@ -268,8 +268,12 @@ int oracle_harris_pick(oracle_xys_t *list, int n, float min_distance, int width,
     if (min_distance > 2.0f) {
         int cell = (int)floor(min_distance / 1.41421356237309504880);
         int gw = (width + cell - 1) / cell, gh = (height + cell - 1) / cell;
-        int16_t *grid = (int16_t *)malloc(sizeof(int16_t) * 2 * (size_t)gw * gh);
-        for (long i = 0; i < 2L * gw * gh; i++) grid[i] = -1;
+        int16_t *grid = (int16_t *)calloc(2 * (size_t)gw * gh, sizeof(int16_t));
+        /* HafCpu_MemSet_U32 (ago_haf_cpu_ch_extract_combine.cpp:648-661) stores whole 16-byte vectors,
+         * so only the first (cells & ~3) cells become (-1,-1); the last cells % 4 keep the zeroed
+         * (0,0) of the node's calloc'd scratch and count as occupied. */
+        long cells = (long)gw * gh;
+        for (long i = 0; i < 2 * (cells & ~3L); i++) grid[i] = -1;
         int md2 = (int)ceilf(min_distance * min_distance);
         for (int i = 0; i < n; i++) {
             int x = list[i].x, y = list[i].y;
