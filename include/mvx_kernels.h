@@ -23,6 +23,7 @@
  */
 #ifndef MVX_KERNELS_H
 #define MVX_KERNELS_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -77,6 +78,91 @@ int mvx_Magnitude_S16_S16S16(mvx_stream_t stream, uint32_t dstWidth, uint32_t ds
 int mvx_Phase_U8_S16S16(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
                         uint8_t *pDst, uint32_t dstStride, const int16_t *pGx, uint32_t gxStride,
                         const int16_t *pGy, uint32_t gyStride);
+
+/* ------------------------------------------------------------------ colour / copy / scale -- */
+/* ColorConvert RGB -> luma.  The reference's graph keeps only this kernel for RGB -> IYUV ->
+ * ChannelExtract(Y) (ago_drama_divide.cpp:244-256 and the unused-output removal); it has no HIP
+ * launcher there (CPU HafCpu_ColorConvert_Y_RGB, ago_haf_cpu_color_convert.cpp:2913-2990). */
+int mvx_ColorConvert_Y_RGB(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                           uint8_t *pDstY, uint32_t dstStride, const uint8_t *pSrcRGB, uint32_t srcStride);
+
+/* replaces HipExec_ChannelCopy_U8_U8 (hip_host_decls.h:382); HafCpu_ChannelCopy_U8_U8
+ * (ago_haf_cpu_ch_extract_combine.cpp:290); level 0 of a Gaussian pyramid */
+int mvx_ChannelCopy_U8_U8(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                          uint8_t *pDst, uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+
+/* replaces HipExec_ScaleImage_U8_U8_Bilinear (hip_host_decls.h:728); HafCpu_ScaleImage_U8_U8_Bilinear
+ * (ago_haf_cpu_geometric.cpp:1977-2105) with the scale matrix of ago_kernel_api.cpp:19897-19916.
+ * dstStride must be a multiple of 16 (8-byte stores). */
+int mvx_ScaleImage_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                  uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                  const uint8_t *pSrc, uint32_t srcStride);
+
+/* replaces HipExec_ScaleGaussianHalf_U8_U8_5x5 (hip_host_decls.h:681); HafCpu_ScaleGaussianHalf_U8_U8_5x5
+ * (ago_haf_cpu_pyramid.cpp:94-305) as called by ago_kernel_api.cpp:15420-15436 (rows 1..h-2, parity
+ * of the source size picks the sampled row/column).  Returns -15 (VX_ERROR_INVALID_DIMENSION) when the
+ * source is below 5 or the destination below 3 in either dimension, as the reference wrapper does. */
+int mvx_ScaleGaussianHalf_U8_U8_5x5(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                    uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                    const uint8_t *pSrc, uint32_t srcStride);
+
+/* ------------------------------------------------------------------ Canny (3x3, L1) -------- */
+/* replaces HipExec_CannySobel_U16_U8_3x3_L1NORM (hip_host_decls.h:828); HafCpu_CannySobel_U16_U8_3x3_L1NORM
+ * (ago_haf_cpu_canny.cpp:44-136), including its column-0 values and, below 16 pixels of width, its
+ * write of the U16 slot before each row. */
+int mvx_CannySobel_U16_U8_3x3_L1NORM(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                     uint16_t *pDst, uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+
+/* replaces HipExec_CannySuppThreshold_U8XY_U16_3x3 (hip_host_decls.h:852); HafCpu_CannySuppThreshold_U8XY_U16_3x3
+ * (ago_haf_cpu_canny.cpp:549-596).  Writes 0/127/255 over [1,w-2]x[1,h-2].  xyStack is the
+ * ago-canny-stack buffer (capacity >= w*h items of 4 bytes, ago_drama_divide.cpp:1633); it is
+ * accepted for interface parity and used as workspace by mvx_CannyEdgeTrace_U8_U8XY. */
+int mvx_CannySuppThreshold_U8XY_U16_3x3(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                        uint8_t *pDst, uint32_t dstStride, const uint16_t *pMag, uint32_t magStride,
+                                        void *xyStack, uint32_t capacityOfXY, uint32_t hystLower, uint32_t hystUpper);
+
+/* Hysteresis of HafCpu_CannyEdgeTrace_U8_U8XY (ago_haf_cpu_canny.cpp:598-644; CPU-only in the
+ * reference, ago_kernel_list.cpp:486): 127 pixels 8-connected through 127s to a 255 become 255, all
+ * other 127s become 0.  In place on pImg.  xyStack: >= w*h * 4 bytes of device workspace. */
+int mvx_CannyEdgeTrace_U8_U8XY(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pImg,
+                               uint32_t stride, void *xyStack, uint32_t capacityOfXY);
+
+/* ------------------------------------------------------------------ Harris (3x3, 3x3) ------ */
+/* replaces HipExec_HarrisSobel_HG3_U8_3x3 (hip_host_decls.h:868); HafCpu_HarrisSobel_HG3_U8_3x3
+ * (ago_haf_cpu_harris.cpp:85-169).  HG3 is an F32x3 plane (stride >= ALIGN16(12w)); rows are laid
+ * out exactly as the reference writes them (row 1 at byte `stride`, then stride/12-pixel steps). */
+int mvx_HarrisSobel_HG3_U8_3x3(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                               float *pDstGxy, uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+
+/* replaces HipExec_HarrisScore_HVC_HG3_3x3 (hip_host_decls.h:880); HafCpu_HarrisScore_HVC_HG3_3x3
+ * (ago_haf_cpu_harris.cpp:596-659); normFactor as computed by ago_kernel_api.cpp:16941-16942. */
+int mvx_HarrisScore_HVC_HG3_3x3(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                float *pDstVc, uint32_t dstStride, const float *pSrcGxy, uint32_t srcStride,
+                                float sensitivity, float strengthThreshold, float normFactor);
+
+/* replaces HipExec_NonMaxSupp_XY_ANY_3x3 (hip_host_decls.h:898); HafCpu_NonMaxSupp_XY_ANY_3x3
+ * (ago_haf_cpu_harris.cpp:882-921).  Items are {int16 x, int16 y, float s} (ago_keypoint_xys_t) in
+ * raster order, at most `capacity`; *pDstCount (device) receives the item count.  The workspace
+ * (device, mvx_NonMaxSupp_XY_ANY_3x3_workspace_size bytes) holds per-block counts and offsets. */
+size_t mvx_NonMaxSupp_XY_ANY_3x3_workspace_size(uint32_t srcWidth, uint32_t srcHeight);
+int mvx_NonMaxSupp_XY_ANY_3x3(mvx_stream_t stream, uint32_t capacity, void *pDstList, uint32_t *pDstCount,
+                              uint32_t srcWidth, uint32_t srcHeight, const float *pSrc, uint32_t srcStride,
+                              void *workspace, size_t workspaceSize);
+
+/* Sort and pick of HafCpu_HarrisMergeSortAndPick_XY_XYS (ago_haf_cpu_harris.cpp:923-1005; CPU-only in
+ * the reference, ago_kernel_list.cpp:483), grid of ago_kernel_api.cpp:20808-20822.  Reads
+ * *pSrcCount (device) items of pSrcList (at most maxList); writes vx_keypoint_t items (28 bytes) to
+ * pDstCorners in descending key order up to `capacity`, *pDstNumItems = min(count, capacity) and
+ * *pDstCount = count (vx_size, uncapped).  width/height are the image size (below 32768).
+ * The workspace (mvx_HarrisMergeSortAndPick_XY_XYS_workspace_size(maxList) bytes) must be prepared
+ * once by mvx_HarrisMergeSortAndPick_workspace_init; every call leaves it ready for the next. */
+size_t mvx_HarrisMergeSortAndPick_XY_XYS_workspace_size(uint32_t maxList);
+int mvx_HarrisMergeSortAndPick_workspace_init(mvx_stream_t stream, uint32_t maxList, void *workspace,
+                                              size_t workspaceSize);
+int mvx_HarrisMergeSortAndPick_XY_XYS(mvx_stream_t stream, uint32_t capacity, void *pDstCorners,
+                                      uint32_t *pDstNumItems, uint64_t *pDstCount, const void *pSrcList,
+                                      const uint32_t *pSrcCount, uint32_t maxList, float minDistance,
+                                      uint32_t width, uint32_t height, void *workspace, size_t workspaceSize);
 
 #ifdef __cplusplus
 }

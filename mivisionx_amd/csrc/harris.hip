@@ -1,0 +1,463 @@
+// harris.hip -- HarrisCorners (gradient 3, block 3) for gfx950: HG3 Sobel products -> score ->
+// 3x3 non-max suppression with raster-ordered compaction -> sort-and-pick with a minimum distance.
+//
+// The reference sorts and picks on the CPU (HafCpu_HarrisMergeSortAndPick_XY_XYS has no HIP
+// version).  Its greedy pick visits the candidates in descending int64-key order and keeps one when
+// no kept point lies closer than min_distance, i.e. it computes the lexicographically-first maximal
+// independent set of the "too close" graph.  Here one workgroup sorts the keys (bitonic, in LDS when
+// the list fits) and decides the set in parallel rounds: a candidate is kept once every closer,
+// higher-priority candidate is rejected, rejected once one of them is kept.  That reproduces the
+// sequential result exactly, without copying the list back to the host.
+#include "mvx_common.hpp"
+
+namespace mvx {
+
+inline bool plane_ok_h(const void *p, uint32_t w, uint32_t stride, uint32_t bpp)
+{
+    return p && aligned16(p) && (stride % 16u) == 0 && stride >= ((w * bpp + 15u) & ~15u);
+}
+
+// ---------------------------------------------------------------- HG3 plane offsets -------------
+// The reference's two Harris kernels step HG3 rows by P = stride / 12 pixels, but the Sobel writer
+// starts row 1 at byte `stride` (ago_haf_cpu_harris.cpp:96, :166) while the score reader puts row y at
+// y*P pixels (:610-612).  Offsets in floats from the plane base:
+MVX_DEV long hg3_write_off(int y, int x, uint32_t stride) { const long P = stride / 12; return (long)(stride / 4) + (long)(y - 1) * 3 * P + 3L * x; }
+MVX_DEV long hg3_read_off(int y, int x, uint32_t stride) { const long P = stride / 12; return (long)y * 3 * P + 3L * x; }
+
+// HarrisSobel_HG3_U8_3x3 (ago_haf_cpu_harris.cpp:85-169): rows 1..h-2, all columns, linear reads.
+__global__ void __launch_bounds__(256)
+k_harris_sobel(uint32_t w, uint32_t h, float *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
+               uint32_t srcStride)
+{
+    const int x = (int)(blockIdx.x * 256 + threadIdx.x);
+    const int y = 1 + (int)blockIdx.y;
+    if (x >= (int)w || y >= (int)h - 1) return;
+    const long n = (long)h * srcStride;
+    const long bm = (long)(y - 1) * srcStride + x, b0 = (long)y * srcStride + x, bp = (long)(y + 1) * srcStride + x;
+    const int16_t gx = (int16_t)(((int)ld_u8(src, n, bp + 1) - (int)ld_u8(src, n, bp - 1)) +
+                                 ((int)ld_u8(src, n, bm + 1) - (int)ld_u8(src, n, bm - 1)) +
+                                 (((int)ld_u8(src, n, b0 + 1) - (int)ld_u8(src, n, b0 - 1)) << 1));
+    const int16_t gy = (int16_t)(((int)ld_u8(src, n, bp - 1) + ((int)ld_u8(src, n, bp) << 1) + (int)ld_u8(src, n, bp + 1)) -
+                                 ((int)ld_u8(src, n, bm - 1) + ((int)ld_u8(src, n, bm) << 1) + (int)ld_u8(src, n, bm + 1)));
+    float *d = dst + hg3_write_off(y, x, dstStride);
+    d[0] = (float)gx * (float)gx;
+    d[1] = (float)gx * (float)gy;
+    d[2] = (float)gy * (float)gy;
+}
+
+// HarrisScore_HVC_HG3_3x3 (ago_haf_cpu_harris.cpp:596-659): window sums j-major from 0, then
+// Mc = (det - (k*tr)*tr) / norm, Vc = Mc > T ? Mc : 0; rows 0, h-1 and columns 0, w-1 are 0.
+__global__ void __launch_bounds__(256)
+k_harris_score(uint32_t w, uint32_t h, float *__restrict__ dst, uint32_t dstStride, const float *__restrict__ hg3,
+               uint32_t hg3Stride, float k, float thr, float norm)
+{
+    const int x = (int)(blockIdx.x * 256 + threadIdx.x);
+    const int y = (int)blockIdx.y;
+    if (x >= (int)w || y >= (int)h) return;
+    float *d = (float *)((uint8_t *)dst + (long)y * dstStride);
+    if (y == 0 || y == (int)h - 1 || x == 0 || x == (int)w - 1) { d[x] = 0.0f; return; }
+    float gx2 = 0.0f, gy2 = 0.0f, gxy2 = 0.0f;
+#pragma unroll
+    for (int j = -1; j <= 1; j++) {
+#pragma unroll
+        for (int i = -1; i <= 1; i++) {
+            const float *s = hg3 + hg3_read_off(y + j, x + i, hg3Stride);
+            gx2 += s[0];
+            gxy2 += s[1];
+            gy2 += s[2];
+        }
+    }
+    const float tr = gx2 + gy2;
+    const float det = (gx2 * gy2) - (gxy2 * gxy2);
+    float mc = det - (k * tr * tr);
+    mc = __fdiv_rn(mc, norm);
+    d[x] = (mc > thr) ? mc : 0.0f;
+}
+
+// ---------------------------------------------------------------- non-max suppression -------------
+// NonMaxSupp_XY_ANY_3x3 (ago_haf_cpu_harris.cpp:882-921): keep (x, y) in [1,w-2] x [1,h-2] when
+// >= the three upper and the left neighbour and > the right and the three lower ones.  Output order
+// is raster order, truncated at capacity: blocks own 256-pixel row segments in raster order, count,
+// get an exclusive scan of the counts, then write at their offset (ballot prefix inside the block).
+struct XYS { int16_t x, y; float s; };
+
+MVX_DEV bool nms_keep(const float *img, uint32_t stride, int x, int y)
+{
+    const float *p9 = (const float *)((const uint8_t *)img + (long)(y - 1) * stride);
+    const float *p0 = (const float *)((const uint8_t *)img + (long)y * stride);
+    const float *p1 = (const float *)((const uint8_t *)img + (long)(y + 1) * stride);
+    const float c = p0[x];
+    return c >= p9[x - 1] && c >= p9[x] && c >= p9[x + 1] && c >= p0[x - 1] && c > p0[x + 1] && c > p1[x - 1] &&
+           c > p1[x] && c > p1[x + 1];
+}
+
+constexpr int NMS_BLOCK = 256;
+
+__global__ void __launch_bounds__(NMS_BLOCK)
+k_nms_count(uint32_t w, uint32_t h, const float *__restrict__ img, uint32_t stride, uint32_t segs,
+            uint32_t *__restrict__ blockCount)
+{
+    const uint32_t b = blockIdx.x;
+    const int y = 1 + (int)(b / segs);
+    const int x = 1 + (int)((b % segs) * NMS_BLOCK + threadIdx.x);
+    const bool keep = (x < (int)w - 1) && nms_keep(img, stride, x, y);
+    const int n = __syncthreads_count(keep);
+    if (threadIdx.x == 0) blockCount[b] = (uint32_t)n;
+}
+
+// exclusive scan of the per-block counts (one workgroup); total stored at offsets[nb]
+__global__ void __launch_bounds__(1024)
+k_scan_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint32_t nb)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t b0 = t * per, b1 = min(b0 + per, nb);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; i++) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;   // exclusive prefix of this thread's range
+    for (uint32_t i = b0; i < b1; i++) { out[i] = run; run += in[i]; }
+    if (t == 1023) out[nb] = part[1023];
+}
+
+__global__ void __launch_bounds__(NMS_BLOCK)
+k_nms_write(uint32_t w, uint32_t h, const float *__restrict__ img, uint32_t stride, uint32_t segs,
+            const uint32_t *__restrict__ offsets, uint32_t nb, XYS *__restrict__ list, uint32_t capacity,
+            uint32_t *__restrict__ count)
+{
+    __shared__ uint32_t waveTotal[NMS_BLOCK / 64];
+    const uint32_t b = blockIdx.x;
+    const int y = 1 + (int)(b / segs);
+    const int x = 1 + (int)((b % segs) * NMS_BLOCK + threadIdx.x);
+    const bool keep = (x < (int)w - 1) && nms_keep(img, stride, x, y);
+    const uint64_t ballot = __ballot(keep);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t below = (uint32_t)__popcll(ballot & ((1ull << lane) - 1ull));
+    if (lane == 0) waveTotal[wave] = (uint32_t)__popcll(ballot);
+    __syncthreads();
+    uint32_t pos = offsets[b] + below;
+    for (uint32_t i = 0; i < wave; i++) pos += waveTotal[i];
+    if (keep && pos < capacity) {
+        XYS it;
+        it.x = (int16_t)x;
+        it.y = (int16_t)y;
+        it.s = ((const float *)((const uint8_t *)img + (long)y * stride))[x];
+        list[pos] = it;
+    }
+    if (b == 0 && threadIdx.x == 0) *count = min(offsets[nb], capacity);
+}
+
+// ---------------------------------------------------------------- sort and pick --------------------
+// HarrisMergeSortAndPick_XY_XYS (ago_haf_cpu_harris.cpp:923-1005), grid set-up ago_kernel_api.cpp:20808-20822.
+struct Keypoint { int32_t x, y; float strength, scale, orientation; int32_t tracking_status; float error; };
+
+constexpr int PICK_THREADS = 1024;
+constexpr uint32_t PICK_LDS_KEYS = 8192;          // keys sorted in LDS up to this many candidates
+constexpr uint8_t ST_UNDECIDED = 0, ST_IN = 1, ST_OUT = 2;
+constexpr uint32_t HASH_EMPTY = 0xffffffffu;
+
+MVX_DEV int64_t xys_key(const XYS &it)
+{
+    // the reference sorts the 8-byte item as an int64: float bits in the high word, then y, then x
+    return (int64_t)(((uint64_t)__float_as_uint(it.s) << 32) | ((uint64_t)(uint16_t)it.y << 16) | (uint64_t)(uint16_t)it.x);
+}
+
+// bitonic sort, descending, of n (power of two) keys held in `k` (LDS or global), whole workgroup
+template <class P>
+MVX_DEV void bitonic_desc(P k, uint32_t n)
+{
+    for (uint32_t size = 2; size <= n; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = threadIdx.x; t < n / 2; t += PICK_THREADS) {
+                const uint32_t i = 2 * t - (t & (stride - 1));
+                const uint32_t j = i + stride;
+                const bool desc = ((i & size) == 0);
+                const int64_t a = k[i], b = k[j];
+                if (desc ? (a < b) : (a > b)) { k[i] = b; k[j] = a; }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+MVX_DEV uint32_t hash_u32(uint32_t v) { v ^= v >> 16; v *= 0x7feb352du; v ^= v >> 15; v *= 0x846ca68bu; v ^= v >> 16; return v; }
+
+struct PickWork {
+    int64_t *keys;        // pow2 >= maxList
+    uint32_t *hkey;       // hash slots: pixel index, HASH_EMPTY when free
+    uint32_t *hval;       // hash slots: rank
+    uint32_t *slotOf;     // rank -> slot (to clear the table afterwards)
+    uint8_t *state;       // rank -> ST_*
+    uint32_t hmask;       // table size - 1
+};
+
+MVX_DEV uint32_t hash_find(const PickWork &W, uint32_t key)
+{
+    uint32_t s = hash_u32(key) & W.hmask;
+    while (true) {
+        const uint32_t k = W.hkey[s];
+        if (k == key) return W.hval[s];
+        if (k == HASH_EMPTY) return HASH_EMPTY;
+        s = (s + 1) & W.hmask;
+    }
+}
+
+__global__ void __launch_bounds__(PICK_THREADS)
+k_harris_pick(const XYS *__restrict__ list, const uint32_t *__restrict__ listCount, uint32_t maxList, float minDistance,
+              uint32_t width, uint32_t height, Keypoint *__restrict__ out, uint32_t capacity,
+              uint32_t *__restrict__ outNumItems, uint64_t *__restrict__ outCount, PickWork W)
+{
+    __shared__ int64_t lkeys[PICK_LDS_KEYS];
+    __shared__ uint32_t sflag, ssum[PICK_THREADS];
+    const uint32_t n = min(*listCount, maxList);
+    uint32_t np = 1;
+    while (np < n) np <<= 1;
+    const bool inLds = np <= PICK_LDS_KEYS;
+    int64_t *keys = inLds ? lkeys : W.keys;
+    for (uint32_t i = threadIdx.x; i < np; i += PICK_THREADS) keys[i] = (i < n) ? xys_key(list[i]) : INT64_MIN;
+    __syncthreads();
+    if (np > 1) bitonic_desc(keys, np);
+
+    const bool useGrid = minDistance > 2.0f;
+    if (useGrid) {
+        const uint32_t cell = (uint32_t)floor((double)minDistance / 1.41421356237309504880);
+        const int gw = (int)((width + cell - 1) / cell), gh = (int)((height + cell - 1) / cell);
+        const long cells = (long)gw * gh;
+        const long firstPhantom = cells & ~3L;   // HafCpu_MemSet_U32 leaves the last cells % 4 at (0,0)
+        const int md2 = (int)ceilf(minDistance * minDistance);
+        // insert every candidate: pixel index -> rank
+        for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) {
+            const int64_t key = keys[r];
+            const uint32_t px = (uint32_t)(key & 0xffff), py = (uint32_t)((key >> 16) & 0xffff);
+            const uint32_t hk = py * width + px;
+            uint32_t s = hash_u32(hk) & W.hmask;
+            while (atomicCAS(&W.hkey[s], HASH_EMPTY, hk) != HASH_EMPTY) s = (s + 1) & W.hmask;
+            W.hval[s] = r;
+            W.slotOf[r] = s;
+            // phantom points: a candidate in a phantom cell, or within reach of (0,0) through a
+            // phantom cell in its +-2 cell window, is rejected before anything else
+            const int cx = (int)(px / cell), cy = (int)(py / cell);
+            const long own = (long)cy * gw + cx;
+            uint8_t st = ST_UNDECIDED;
+            if (own >= firstPhantom) st = ST_OUT;
+            else if ((int)(px * px + py * py) < md2) {
+                for (long c = firstPhantom; c < cells; c++) {
+                    const int tx = (int)(c % gw), ty = (int)(c / gw);
+                    if (abs(tx - cx) <= 2 && abs(ty - cy) <= 2) st = ST_OUT;
+                }
+            }
+            W.state[r] = st;
+        }
+        __syncthreads();
+        // parallel greedy rounds
+        const int rad = (int)ceilf(minDistance);
+        while (true) {
+            if (threadIdx.x == 0) sflag = 0;
+            __syncthreads();
+            for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) {
+                if (W.state[r] != ST_UNDECIDED) continue;
+                const int64_t key = keys[r];
+                const int px = (int)(key & 0xffff), py = (int)((key >> 16) & 0xffff);
+                bool pending = false, out = false;
+                for (int dy = -rad; dy <= rad && !out; dy++) {
+                    const int yy = py + dy;
+                    if (yy < 0 || yy >= (int)height) continue;
+                    for (int dx = -rad; dx <= rad; dx++) {
+                        const int xx = px + dx;
+                        if ((dx | dy) == 0 || xx < 0 || xx >= (int)width || dx * dx + dy * dy >= md2) continue;
+                        const uint32_t j = hash_find(W, (uint32_t)yy * width + (uint32_t)xx);
+                        if (j == HASH_EMPTY || j >= r) continue;   // not a candidate, or lower priority
+                        const uint8_t sj = W.state[j];
+                        if (sj == ST_IN) { out = true; break; }
+                        if (sj == ST_UNDECIDED) pending = true;
+                    }
+                }
+                if (out) { W.state[r] = ST_OUT; sflag = 1; }
+                else if (!pending) { W.state[r] = ST_IN; sflag = 1; }
+                else sflag = 1;   // still undecided: another round is needed
+            }
+            __syncthreads();
+            const uint32_t again = sflag;
+            __syncthreads();
+            if (!again) break;
+        }
+    } else {
+        for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) W.state[r] = ST_IN;
+        __syncthreads();
+    }
+
+    // output the kept points in rank order: block-wide exclusive scan over chunks of the rank range
+    const uint32_t per = (n + PICK_THREADS - 1) / PICK_THREADS;
+    const uint32_t r0 = threadIdx.x * per, r1 = min(r0 + per, n);
+    uint32_t mine = 0;
+    for (uint32_t r = r0; r < r1; r++) mine += (W.state[r] == ST_IN);
+    ssum[threadIdx.x] = mine;
+    __syncthreads();
+    for (uint32_t off = 1; off < PICK_THREADS; off <<= 1) {
+        const uint32_t v = (threadIdx.x >= off) ? ssum[threadIdx.x - off] : 0u;
+        __syncthreads();
+        ssum[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t pos = ssum[threadIdx.x] - mine;
+    const uint32_t total = ssum[PICK_THREADS - 1];
+    for (uint32_t r = r0; r < r1; r++) {
+        if (W.state[r] != ST_IN) continue;
+        if (pos < capacity) {
+            const int64_t key = keys[r];
+            Keypoint kp;
+            kp.x = (int32_t)(key & 0xffff);
+            kp.y = (int32_t)((key >> 16) & 0xffff);
+            kp.strength = __uint_as_float((uint32_t)((uint64_t)key >> 32));
+            kp.scale = 0.0f;
+            kp.orientation = 0.0f;
+            kp.tracking_status = 1;
+            kp.error = 0.0f;
+            out[pos] = kp;
+        }
+        pos++;
+    }
+    // leave the hash table empty for the next call
+    if (useGrid)
+        for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) W.hkey[W.slotOf[r]] = HASH_EMPTY;
+    if (threadIdx.x == 0) {
+        *outNumItems = min(total, capacity);
+        *outCount = total;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct PickLayout {
+    uint32_t np, hsize;
+    size_t keys, hkey, hval, slot, state, total;
+};
+
+inline PickLayout pick_layout(uint32_t maxList)
+{
+    PickLayout L;
+    L.np = 1;
+    while (L.np < maxList) L.np <<= 1;
+    L.hsize = 1024;
+    while (L.hsize < 2u * maxList) L.hsize <<= 1;
+    L.keys = 0;
+    L.hkey = align256(L.keys + (size_t)L.np * 8);
+    L.hval = align256(L.hkey + (size_t)L.hsize * 4);
+    L.slot = align256(L.hval + (size_t)L.hsize * 4);
+    L.state = align256(L.slot + (size_t)maxList * 4);
+    L.total = align256(L.state + (size_t)maxList);
+    return L;
+}
+
+}  // namespace mvx
+
+using namespace mvx;
+
+extern "C" int mvx_HarrisSobel_HG3_U8_3x3(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride,
+                                          const uint8_t *src, uint32_t srcStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok_h(dst, w, dstStride, 12) || !plane_ok_h(src, w, srcStride, 1)) return MVX_ERROR_INVALID_PARAMETERS;
+    if (h < 3) return MVX_SUCCESS;
+    dim3 grid((w + 255) / 256, h - 2);
+    hipLaunchKernelGGL(k_harris_sobel, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_HarrisScore_HVC_HG3_3x3(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride,
+                                           const float *hg3, uint32_t hg3Stride, float sensitivity,
+                                           float strengthThreshold, float normFactor)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok_h(dst, w, dstStride, 4) || !plane_ok_h(hg3, w, hg3Stride, 12)) return MVX_ERROR_INVALID_PARAMETERS;
+    dim3 grid((w + 255) / 256, h);
+    hipLaunchKernelGGL(k_harris_score, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, hg3, hg3Stride,
+                       sensitivity, strengthThreshold, normFactor);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" size_t mvx_NonMaxSupp_XY_ANY_3x3_workspace_size(uint32_t w, uint32_t h)
+{
+    if (w < 3 || h < 3) return 256;
+    const size_t segs = (w - 2 + NMS_BLOCK - 1) / NMS_BLOCK;
+    const size_t nb = segs * (h - 2);
+    return align256(nb * 4) + align256((nb + 1) * 4);
+}
+
+extern "C" int mvx_NonMaxSupp_XY_ANY_3x3(mvx_stream_t stream, uint32_t capacity, void *dstList, uint32_t *dstCount,
+                                         uint32_t w, uint32_t h, const float *src, uint32_t srcStride, void *workspace,
+                                         size_t workspaceSize)
+{
+    if (!dstList || !dstCount || !workspace || !plane_ok_h(src, w, srcStride, 4)) return MVX_ERROR_INVALID_PARAMETERS;
+    if (workspaceSize < mvx_NonMaxSupp_XY_ANY_3x3_workspace_size(w, h)) return MVX_ERROR_INVALID_PARAMETERS;
+    if (w < 3 || h < 3) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream_of(stream), dstCount, 0u, (size_t)1);
+        MVX_LAUNCH_CHECK();
+        return MVX_SUCCESS;
+    }
+    const uint32_t segs = (w - 2 + NMS_BLOCK - 1) / NMS_BLOCK;
+    const uint32_t nb = segs * (h - 2);
+    uint32_t *counts = (uint32_t *)workspace;
+    uint32_t *offsets = (uint32_t *)((uint8_t *)workspace + align256((size_t)nb * 4));
+    hipStream_t s = stream_of(stream);
+    hipLaunchKernelGGL(k_nms_count, dim3(nb), dim3(NMS_BLOCK), 0, s, w, h, src, srcStride, segs, counts);
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, s, counts, offsets, nb);
+    hipLaunchKernelGGL(k_nms_write, dim3(nb), dim3(NMS_BLOCK), 0, s, w, h, src, srcStride, segs, offsets, nb,
+                       (XYS *)dstList, capacity, dstCount);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" size_t mvx_HarrisMergeSortAndPick_XY_XYS_workspace_size(uint32_t maxList)
+{
+    return pick_layout(maxList ? maxList : 1).total;
+}
+
+extern "C" int mvx_HarrisMergeSortAndPick_workspace_init(mvx_stream_t stream, uint32_t maxList, void *workspace,
+                                                         size_t workspaceSize)
+{
+    const PickLayout L = pick_layout(maxList ? maxList : 1);
+    if (!workspace || workspaceSize < L.total) return MVX_ERROR_INVALID_PARAMETERS;
+    hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, stream_of(stream), (uint32_t *)((uint8_t *)workspace + L.hkey),
+                       HASH_EMPTY, (size_t)L.hsize);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_HarrisMergeSortAndPick_XY_XYS(mvx_stream_t stream, uint32_t capacity, void *dstCorners,
+                                                 uint32_t *dstNumItems, uint64_t *dstCount, const void *srcList,
+                                                 const uint32_t *srcCount, uint32_t maxList, float minDistance,
+                                                 uint32_t width, uint32_t height, void *workspace, size_t workspaceSize)
+{
+    if (!dstCorners || !dstNumItems || !dstCount || !srcList || !srcCount || !workspace) return MVX_ERROR_INVALID_PARAMETERS;
+    if (minDistance <= 0.0f || width == 0 || height == 0 || width > 32767 || height > 32767) return MVX_ERROR_INVALID_PARAMETERS;
+    const PickLayout L = pick_layout(maxList ? maxList : 1);
+    if (workspaceSize < L.total) return MVX_ERROR_INVALID_PARAMETERS;
+    PickWork W;
+    uint8_t *base = (uint8_t *)workspace;
+    W.keys = (int64_t *)(base + L.keys);
+    W.hkey = (uint32_t *)(base + L.hkey);
+    W.hval = (uint32_t *)(base + L.hval);
+    W.slotOf = (uint32_t *)(base + L.slot);
+    W.state = base + L.state;
+    W.hmask = L.hsize - 1;
+    hipLaunchKernelGGL(k_harris_pick, dim3(1), dim3(PICK_THREADS), 0, stream_of(stream), (const XYS *)srcList, srcCount,
+                       maxList, minDistance, width, height, (Keypoint *)dstCorners, capacity, dstNumItems, dstCount, W);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}

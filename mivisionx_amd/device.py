@@ -12,9 +12,9 @@ import torch
 
 from . import kernels
 
-BPP = {"U8": 1, "S16": 2, "U16": 2, "U32": 4, "S32": 4, "F32": 4, "RGB": 3, "RGBX": 4}
+BPP = {"U8": 1, "S16": 2, "U16": 2, "U32": 4, "S32": 4, "F32": 4, "RGB": 3, "RGBX": 4, "F32x3": 12}
 NP_DTYPE = {"U8": np.uint8, "S16": np.int16, "U16": np.uint16, "U32": np.uint32, "S32": np.int32,
-            "F32": np.float32, "RGB": np.uint8, "RGBX": np.uint8}
+            "F32": np.float32, "RGB": np.uint8, "RGBX": np.uint8, "F32x3": np.float32}
 
 
 def align16(x):
@@ -45,6 +45,8 @@ class Plane:
         out = np.ascontiguousarray(raw).view(NP_DTYPE[self.fmt])
         if self.fmt in ("RGB", "RGBX"):
             return out.reshape(self.height, self.width, self.bpp)
+        if self.fmt == "F32x3":
+            return out.reshape(self.height, self.width, 3)
         return out.reshape(self.height, self.width)
 
     @classmethod

@@ -1,0 +1,130 @@
+"""Kernel sequences of the vision graphs, launched through the C ABI on one stream.
+
+These mirror what the reference's graph optimizer schedules for the same OpenVX graphs
+(divide: ago_drama_divide.cpp; merges: ago_drama_remove.cpp) and are what the parity tests and the
+benchmark drive.  Buffers are device planes in the reference layout; workspaces are device tensors.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import kernels
+from .device import Plane, call
+
+KEYPOINT = np.dtype([("x", np.int32), ("y", np.int32), ("strength", np.float32), ("scale", np.float32),
+                     ("orientation", np.float32), ("tracking_status", np.int32), ("error", np.float32)])
+
+
+def pyramid_dims(w, h, levels):
+    dims = []
+    for _ in range(levels):
+        dims.append((w, h))
+        w, h = (w + 1) // 2, (h + 1) // 2
+    return dims
+
+
+def _dev_bytes(n):
+    return torch.zeros(max(int(n), 1), dtype=torch.uint8, device="cuda")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Canny:
+    """CannyEdgeDetector (3x3, L1): CannySobel -> CannySuppThreshold -> CannyEdgeTrace."""
+
+    def __init__(self, w, h):
+        self.w, self.h = w, h
+        self.mag = Plane(w, h, "U16")
+        self.stack = _dev_bytes(4 * w * h)
+
+    def run(self, src, dst, lower, upper):
+        w, h = self.w, self.h
+        call("CannySobel_U16_U8_3x3_L1NORM", w, h, self.mag, self.mag.stride, src, src.stride)
+        call("CannySuppThreshold_U8XY_U16_3x3", w, h, dst, dst.stride, self.mag, self.mag.stride,
+             _ptr(self.stack), w * h, lower, upper)
+        call("CannyEdgeTrace_U8_U8XY", w, h, dst, dst.stride, _ptr(self.stack), w * h)
+
+
+class Harris:
+    """HarrisCorners (gradient 3, block 3): HG3 -> score -> NMS -> sort and pick."""
+
+    def __init__(self, w, h, capacity):
+        K = kernels()
+        K.mvx_NonMaxSupp_XY_ANY_3x3_workspace_size.restype = ctypes.c_size_t
+        K.mvx_HarrisMergeSortAndPick_XY_XYS_workspace_size.restype = ctypes.c_size_t
+        self.w, self.h, self.capacity = w, h, capacity
+        self.hg3 = Plane(w, h, "F32x3")
+        self.hvc = Plane(w, h, "F32")
+        # NMS maxima are never 8-adjacent: at most one per 2x2 block
+        self.max_list = max(1, ((w - 1) // 2) * ((h - 1) // 2))
+        self.xys = _dev_bytes(8 * self.max_list)
+        self.xys_count = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.nms_ws_size = K.mvx_NonMaxSupp_XY_ANY_3x3_workspace_size(w, h)
+        self.nms_ws = _dev_bytes(self.nms_ws_size)
+        self.pick_ws_size = K.mvx_HarrisMergeSortAndPick_XY_XYS_workspace_size(self.max_list)
+        self.pick_ws = _dev_bytes(self.pick_ws_size)
+        call("HarrisMergeSortAndPick_workspace_init", self.max_list, _ptr(self.pick_ws), ctypes.c_size_t(self.pick_ws_size))
+        self.corners = _dev_bytes(28 * max(capacity, 1))
+        self.num_items = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.count = torch.zeros(1, dtype=torch.int64, device="cuda")
+
+    def run(self, src, k, thr, dist, norm):
+        w, h = self.w, self.h
+        call("HarrisSobel_HG3_U8_3x3", w, h, self.hg3, self.hg3.stride, src, src.stride)
+        call("HarrisScore_HVC_HG3_3x3", w, h, self.hvc, self.hvc.stride, self.hg3, self.hg3.stride,
+             float(k), float(thr), float(norm))
+        call("NonMaxSupp_XY_ANY_3x3", self.max_list, _ptr(self.xys), _ptr(self.xys_count), w, h, self.hvc,
+             self.hvc.stride, _ptr(self.nms_ws), ctypes.c_size_t(self.nms_ws_size))
+        call("HarrisMergeSortAndPick_XY_XYS", self.capacity, _ptr(self.corners), _ptr(self.num_items),
+             _ptr(self.count), _ptr(self.xys), _ptr(self.xys_count), self.max_list, float(dist), w, h,
+             _ptr(self.pick_ws), ctypes.c_size_t(self.pick_ws_size))
+
+    def result(self):
+        n = int(self.num_items.item())
+        raw = self.corners[:28 * n].cpu().numpy()
+        return raw.view(KEYPOINT).copy(), int(self.count.item())
+
+
+def harris_norm(gradient_size=3):
+    """(255 * 2^(g-1) * 3)^4 in float, left to right (ago_kernel_api.cpp:16941-16942)"""
+    f = np.float32(255.0) * np.float32(1 << (gradient_size - 1)) * np.float32(3)
+    return float(np.float32(np.float32(np.float32(f * f) * f) * f))
+
+
+class VisionGraph4K:
+    """RGB -> IYUV (virtual) -> Y (virtual); Y -> ScaleImage bilinear; Y -> GaussianPyramid HALF;
+    Y -> Canny (3, L1); Y -> Harris (3, 3).  The reference keeps ColorConvert_Y_RGB only, a channel
+    copy for pyramid level 0, ScaleGaussianHalf_5x5 per further level, and the Canny / Harris kernel
+    chains (ago_drama_divide.cpp:244-256, 563-616, 900-922, 1618-1701, 1703-1787)."""
+
+    def __init__(self, w, h, dw, dh, levels=5, lower=80, upper=100, k=0.04, thr=0.0005, dist=5.0, capacity=65536):
+        self.w, self.h, self.dw, self.dh, self.levels = w, h, dw, dh, levels
+        self.lower, self.upper, self.k, self.thr, self.dist = lower, upper, k, thr, dist
+        self.rgb = Plane(w, h, "RGB")
+        self.y = Plane(w, h, "U8")
+        self.scaled = Plane(dw, dh, "U8")
+        self.pyr = [Plane(a, b, "U8") for (a, b) in pyramid_dims(w, h, levels)]
+        self.edges = Plane(w, h, "U8")
+        self.canny = Canny(w, h)
+        self.harris = Harris(w, h, capacity)
+        self.norm = harris_norm(3)
+
+    def run(self, rgb_plane=None):
+        rgb = rgb_plane if rgb_plane is not None else self.rgb
+        w, h = self.w, self.h
+        call("ColorConvert_Y_RGB", w, h, self.y, self.y.stride, rgb, rgb.stride)
+        call("ScaleImage_U8_U8_Bilinear", self.dw, self.dh, self.scaled, self.scaled.stride, w, h, self.y, self.y.stride)
+        call("ChannelCopy_U8_U8", w, h, self.pyr[0], self.pyr[0].stride, self.y, self.y.stride)
+        for l in range(1, self.levels):
+            a, b = self.pyr[l], self.pyr[l - 1]
+            call("ScaleGaussianHalf_U8_U8_5x5", a.width, a.height, a, a.stride, b.width, b.height, b, b.stride)
+        self.canny.run(self.y, self.edges, self.lower, self.upper)
+        self.harris.run(self.y, self.k, self.thr, self.dist, self.norm)
+
+    def results(self):
+        kp, count = self.harris.result()
+        return dict(scaled=self.scaled.download(), pyramid=[p.download() for p in self.pyr],
+                    edges=self.edges.download(), corners=kp, count=count)

@@ -1,0 +1,107 @@
+"""GPU parity of the 4K vision-graph nodes (ColorConvert luma, ChannelCopy, ScaleImage bilinear,
+ScaleGaussianHalf/GaussianPyramid, Canny, Harris) and of the whole graph at 3840x2160: HIP kernels
+through the C ABI versus the oracle, bit-exact, plus the golden hashes of the reference engine."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mivisionx_amd import synth
+from mivisionx_amd.device import Plane, call
+from mivisionx_amd.pipelines import Canny, Harris, VisionGraph4K, harris_norm
+from util import assert_same, sha
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SIZES = [(64, 48), (100, 38), (34, 20), (16, 10), (8, 8), (6, 6), (200, 150), (1920, 1080), (333, 77)]
+
+
+@pytest.fixture(scope="module")
+def g2():
+    with open(os.path.join(ROOT, "tests", "golden", "vision_golden_graph2.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("w,h", SIZES)
+def test_color_copy_scale(gpu, orc, w, h):
+    rgb = synth.scene_rgb(w, h, 3)
+    pr = Plane.from_array(rgb, "RGB")
+    y = Plane(w, h)
+    call("ColorConvert_Y_RGB", w, h, y, y.stride, pr, pr.stride)
+    want_y = orc.color_y(rgb)
+    assert_same(y.download(), want_y, f"color {w}x{h}")
+    c = Plane(w, h)
+    call("ChannelCopy_U8_U8", w, h, c, c.stride, y, y.stride)
+    assert_same(c.download(), want_y, "copy")
+    for (dw, dh) in [((w + 1) // 2, (h + 1) // 2), (2 * w, 2 * h), (max(1, 3 * w // 4), max(1, 2 * h // 3)), (w + 5, h)]:
+        d = Plane(dw, dh)
+        call("ScaleImage_U8_U8_Bilinear", dw, dh, d, d.stride, w, h, y, y.stride)
+        assert_same(d.download(), orc.scale(want_y, dw, dh), f"scale {w}x{h}->{dw}x{dh}")
+
+
+@pytest.mark.parametrize("w,h", [(64, 48), (100, 38), (34, 21), (33, 20), (12, 9), (200, 150), (1920, 1080), (3840, 2160)])
+def test_gaussian_half(gpu, orc, w, h):
+    src = synth.uniform_u8(w, h, 6) if w * h < 4_000_000 else synth.uniform_u8(w, h, 6)
+    s = Plane.from_array(src)
+    dw, dh = (w + 1) // 2, (h + 1) // 2
+    d = Plane(dw, dh)
+    call("ScaleGaussianHalf_U8_U8_5x5", dw, dh, d, d.stride, w, h, s, s.stride)
+    assert_same(d.download(), orc.gaussian_half(src, dw, dh), f"half {w}x{h}")
+
+
+@pytest.mark.parametrize("w,h", SIZES + [(15, 9), (31, 17), (17, 5)])
+@pytest.mark.parametrize("thr", [(80, 100), (10, 30)])
+def test_canny(gpu, orc, w, h, thr):
+    for tag, src in (("scene", synth.scene_u8(w, h, 4)), ("noise", synth.uniform_u8(w, h, 5))):
+        s = Plane.from_array(src)
+        cz = Canny(w, h)
+        # the U16 gradient plane alone, including the column-0 and narrow-width behaviours
+        call("CannySobel_U16_U8_3x3_L1NORM", w, h, cz.mag, cz.mag.stride, s, s.stride)
+        assert_same(cz.mag.download(), orc.canny_sobel(src), f"canny sobel {tag} {w}x{h}")
+        out = Plane(w, h)
+        cz.run(s, out, *thr)
+        assert_same(out.download(), orc.canny(src, *thr), f"canny {tag} {thr} {w}x{h}")
+
+
+@pytest.mark.parametrize("w,h", SIZES + [(48, 20), (50, 20), (18, 20)])
+def test_harris(gpu, orc, w, h):
+    src = synth.scene_u8(w, h, 7)
+    noise = synth.uniform_u8(w, h, 8)
+    for tag, img in (("scene", src), ("noise", noise)):
+        s = Plane.from_array(img)
+        for (k, thr, dist, cap) in [(0.04, 0.0005, 5.0, 5000), (0.04, 1e-6, 5.0, 50), (0.06, 1e-5, 2.0, 300),
+                                    (0.04, 1e-7, 10.0, 100000), (0.04, 1e-7, 3.0, 100000)]:
+            hz = Harris(w, h, cap)
+            hz.run(s, k, thr, dist, harris_norm(3))
+            got, n = hz.result()
+            want, wn = orc.harris(img, k, thr, dist, cap)
+            assert n == wn, f"harris count {tag} {w}x{h} {k},{thr},{dist}: {n} != {wn}"
+            assert_same(got, want, f"harris {tag} {w}x{h} {k},{thr},{dist}")
+
+
+def test_harris_score_plane(gpu, orc):
+    w, h = 130, 70
+    img = synth.uniform_u8(w, h, 9)
+    s = Plane.from_array(img)
+    hz = Harris(w, h, 10)
+    hz.run(s, 0.04, 1e-7, 5.0, harris_norm(3))
+    _, _, hvc = orc.harris(img, 0.04, 1e-7, 5.0, 10, with_score=True)
+    assert_same(hz.hvc.download(), hvc, "hvc")
+
+
+def test_graph2_4k_bit_exact(gpu, orc, g2):
+    g = g2["graph2"]
+    rgb = synth.scene_rgb(g["w"], g["h"], g["seed"])
+    assert sha(rgb) == g["rgb_sha256"]
+    G = VisionGraph4K(g["w"], g["h"], g["dw"], g["dh"], g["levels"], g["lower"], g["upper"], g["k"], g["thr"],
+                      g["dist"], g["capacity"])
+    G.rgb.upload(rgb)
+    for _ in range(2):   # a second run must give the same answer (workspaces are reused)
+        G.run()
+        r = G.results()
+        assert sha(r["scaled"]) == g["scaled"], "scaled"
+        assert [sha(l) for l in r["pyramid"]] == g["pyramid"], "pyramid"
+        assert sha(r["edges"]) == g["edges"], "edges"
+        assert r["count"] == g["count"] and sha(r["corners"]) == g["corners"], "corners"
