@@ -330,7 +330,9 @@ k_harris_pick(const XYS *__restrict__ list, const uint32_t *__restrict__ listCou
         for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) W.hkey[W.slotOf[r]] = HASH_EMPTY;
     if (threadIdx.x == 0) {
         *outNumItems = min(total, capacity);
-        *outCount = total;
+        // the grid path counts every kept point; the copy path (min_distance <= 2) reports the
+        // copied number (ago_haf_cpu_harris.cpp:971-975 vs :988-989)
+        *outCount = useGrid ? total : min(total, capacity);
     }
 }
 
