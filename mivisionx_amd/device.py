@@ -63,9 +63,47 @@ def stream_handle():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+class KernelTimer:
+    """Brackets every ``call`` with HIP events on the launch stream while active (``with timer:``);
+    ``summary()`` gives per-entry-point launch counts and mean milliseconds."""
+
+    active = None
+
+    def __init__(self, only=None):
+        self.only = only
+        self.events = []
+
+    def __enter__(self):
+        KernelTimer.active = self
+        return self
+
+    def __exit__(self, *exc):
+        KernelTimer.active = None
+
+    def summary(self):
+        torch.cuda.synchronize()
+        out = {}
+        for name, a, b in self.events:
+            n, t = out.get(name, (0, 0.0))
+            out[name] = (n + 1, t + a.elapsed_time(b))
+        return {k: {"launches": n, "mean_ms": t / n, "total_ms": t} for k, (n, t) in out.items()}
+
+
 def call(name, *args):
     """Call ``mvx_<name>(stream, *args)`` on the current torch stream; raise on failure."""
     fn = getattr(kernels(), "mvx_" + name)
+    timer = KernelTimer.active
+    if timer is not None and (timer.only is None or name in timer.only):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        rc = _invoke(fn, name, args)
+        b.record()
+        timer.events.append((name, a, b))
+        return rc
+    return _invoke(fn, name, args)
+
+
+def _invoke(fn, name, args):
     conv = []
     for a in args:
         if isinstance(a, Plane):
