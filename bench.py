@@ -5,16 +5,21 @@ Workload (one "step" = one pass of the graph over a batch of B resident 4K RGB f
   Y -> ScaleImage 1920x1080 bilinear; Y -> GaussianPyramid 5 levels HALF;
   Y -> CannyEdgeDetector (3, L1, 80/100); Y -> HarrisCorners (k 0.04, T 0.0005, d 5, 3, 3).
 Inputs are synthetic scenes (sum of sinusoids + rectangles + noise, mivisionx_amd.synth), already in
-HBM when the timed region starts.  value = whole-job graph throughput in Mpix/s of 4K input.
+HBM when the timed region starts.  The graph runs through mvx_VisionGraph_run_stage (fused tile pass,
+hysteresis union, hysteresis resolve, pyramid levels 2-4, Harris pick; all frames per launch).
+value = whole-job graph throughput in Mpix/s of 4K input.
 
 Also reported:
-  roofline      -- the dominant entry point's algorithmic bytes per launch / its mean launch time,
-                   timed with HIP events on its launch stream inside the timed region;
-  graph_hbm     -- the whole graph's algorithmic bytes per frame / frame time;
-  cpu_baseline  -- the reference's own CPU engine (oracle/_ref, one thread) on a bounded sample of
-                   the same workload, or the oracle's C port when oracle/_ref is absent.
+  roofline       -- the dominant stage's algorithmic bytes per launch / its mean launch time, timed with
+                    HIP events on its launch stream inside the timed region;
+  graph_hbm      -- the whole graph's algorithmic bytes per frame / frame time;
+  per_image      -- the same frames through one launch per node and frame (the reference's execution
+                    model), and the batched/per-image ratio;
+  cpu_baseline   -- the reference's own CPU engine (oracle/_ref, one thread) on a bounded sample of the
+                    same workload, or the oracle's C port when oracle/_ref is absent.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -27,6 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0
 W, H, DW, DH, LEVELS = 3840, 2160, 1920, 1080, 5
 PARAMS = dict(lower=80, upper=100, k=0.04, thr=0.0005, dist=5.0, capacity=65536)
+STAGES = ["fused_tile_pass", "hysteresis_border_union", "hysteresis_resolve", "pyramid_levels_2_4", "harris_pick"]
 
 
 def pyramid_bytes(w, h, levels):
@@ -42,23 +48,17 @@ def graph_algorithmic_bytes():
     return 3 * W * H + DW * DH + pyramid_bytes(W, H, LEVELS) + W * H
 
 
-def entry_bytes(name, launches):
-    """algorithmic bytes of all launches of one entry point in one frame of the workload"""
+def stage_algorithmic_bytes(stage):
+    """graph input/output bytes one frame of each stage reads or writes"""
     px = W * H
-    table = {
-        "ColorConvert_Y_RGB": 4 * px,
-        "ScaleImage_U8_U8_Bilinear": px + DW * DH,
-        "ChannelCopy_U8_U8": 2 * px,
-        "ScaleGaussianHalf_U8_U8_5x5": pyramid_bytes(W, H, LEVELS - 1) + pyramid_bytes(W, H, LEVELS) - px,
-        "CannySobel_U16_U8_3x3_L1NORM": 3 * px,
-        "CannySuppThreshold_U8XY_U16_3x3": 3 * px,
-        "CannyEdgeTrace_U8_U8XY": 2 * px,
-        "HarrisSobel_HG3_U8_3x3": 13 * px,
-        "HarrisScore_HVC_HG3_3x3": 16 * px,
-        "NonMaxSupp_XY_ANY_3x3": 4 * px,
-        "HarrisMergeSortAndPick_XY_XYS": 0,
-    }
-    return table.get(name, 0)
+    l1 = ((W + 1) // 2) * ((H + 1) // 2)
+    return {
+        0: 3 * px + px + DW * DH + l1 + px,      # RGB in; level 0, scaled, level 1, edges out
+        1: 0,                                     # tile-border tables only
+        2: 0,                                     # re-reads/writes edge pixels already counted
+        3: pyramid_bytes(W, H, LEVELS) - px,      # level 1 in; levels 2.. out
+        4: 0,
+    }[stage]
 
 
 def cpu_baseline(rgb, budget_s=10.0):
@@ -89,90 +89,119 @@ def cpu_baseline(rgb, budget_s=10.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--frames", type=int, default=8, help="4K frames per step (resident, distinct buffers)")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=16, help="4K frames per step (resident, distinct buffers)")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-image", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the vision graph runs on a single MI355X here (asked for --gpus {args.gpus}); "
-              "see DESIGN.md for how the path shards across GPUs.", file=sys.stderr)
+              "DESIGN.md describes how the path would shard across GPUs.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
-    from mivisionx_amd import synth
-    from mivisionx_amd.device import KernelTimer
-    from mivisionx_amd.pipelines import VisionGraph4K
+    from mivisionx_amd import kernels, synth
+    from mivisionx_amd.pipelines import FusedVisionGraph, VisionGraph4K
 
     torch.cuda.set_device(0)
-    base = [synth.scene_rgb(W, H, s) for s in (0, 1)]
-    graphs = []
-    for f in range(args.frames):
-        g = VisionGraph4K(W, H, DW, DH, LEVELS, **PARAMS)
-        t = torch.from_numpy(base[f % 2]).cuda()
+    K = kernels()
+    base = [torch.from_numpy(synth.scene_rgb(W, H, s)) for s in (0, 1)]
+    G = FusedVisionGraph(W, H, DW, DH, LEVELS, frames=args.frames, **PARAMS)
+    for f, fr in enumerate(G.frames):
+        t = base[f % 2].cuda()
         if f >= 2:
             t = torch.roll(t, shifts=(37 * f, 53 * f), dims=(0, 1))
-        g.rgb.t[:, : 3 * W].copy_(t.reshape(H, 3 * W))
-        graphs.append(g)
+        fr.rgb.t[:, : 3 * W].copy_(t.reshape(H, 3 * W))
     torch.cuda.synchronize()
 
-    # warm-up, with every entry point timed once per frame to find the dominant one
-    with KernelTimer() as kt:
-        for _ in range(max(1, args.warmup)):
-            for g in graphs:
-                g.run()
-    per_entry = kt.summary()
-    frames_timed = max(1, args.warmup) * len(graphs)
-    per_frame = {k: v["total_ms"] / frames_timed for k, v in per_entry.items()}
-    dominant = max(per_frame, key=per_frame.get)
+    def stream():
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
-    # timed region: K steps, the dominant entry point bracketed by events on its launch stream
+    def run_stage(st):
+        rc = K.mvx_VisionGraph_run_stage(stream(), ctypes.byref(G.params), len(G.frames), ctypes.c_void_p(G.ws.data_ptr()),
+                                         ctypes.c_size_t(G.ws_size), st)
+        if rc:
+            raise RuntimeError(f"stage {st} returned {rc}")
+
+    # warm-up with every stage bracketed by events: find the dominant stage
+    ev = {st: [] for st in range(len(STAGES))}
+    for _ in range(max(1, args.warmup)):
+        for st in range(len(STAGES)):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(); run_stage(st); b.record()
+            ev[st].append((a, b))
     torch.cuda.synchronize()
-    with KernelTimer(only={dominant}) as kd:
-        t0 = time.perf_counter()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(args.steps):
-            for g in graphs:
-                g.run()
-        e1.record()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+    stage_ms = {st: sum(a.elapsed_time(b) for a, b in v) / len(v) for st, v in ev.items()}
+    dominant = max(stage_ms, key=stage_ms.get)
+
+    # timed region: K steps of all frames; the dominant stage bracketed by events on its stream
+    dom_ev = []
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        for st in range(len(STAGES)):
+            if st == dominant:
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(); run_stage(st); b.record()
+                dom_ev.append((a, b))
+            else:
+                run_stage(st)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     gpu_ms = e0.elapsed_time(e1)
-    dom = kd.summary()[dominant]
-    ms_per_step = gpu_ms / args.steps
-    frames = args.steps * len(graphs)
+    dom_ms = sum(a.elapsed_time(b) for a, b in dom_ev) / len(dom_ev)
+    frames = args.steps * len(G.frames)
     value = frames * W * H / (gpu_ms / 1e3) / 1e6
-
-    dom_bytes = entry_bytes(dominant, dom["launches"]) * frames / dom["launches"]
-    achieved = dom_bytes / (dom["mean_ms"] / 1e3) / 1e9
+    dom_bytes = stage_algorithmic_bytes(dominant) * len(G.frames)
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     graph_bytes = graph_algorithmic_bytes()
     graph_gbs = graph_bytes * frames / (gpu_ms / 1e3) / 1e9
-
-    # correctness guard: the last frame's corner count must match the first run's (no skipped work)
-    r = graphs[0].results()
+    corners0 = G.frames[0].results()["count"]
 
     out = {
         "metric": "graph Mpix/s at 4K U8 (achieved HBM GB/s %peak); batched vs per-image",
         "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": round(gpu_ms / args.steps, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8", "data": "synthetic",
         "config": {"workload": "3840x2160 RGB vision graph: ColorConvert->Y; ScaleImage 1920x1080 bilinear; "
                                "GaussianPyramid 5 HALF; Canny 3 L1 80/100; Harris 0.04/0.0005/5/3/3",
-                   "frames_per_step": len(graphs), "width": W, "height": H, "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                   "frames_per_step": len(G.frames), "width": W, "height": H, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "kernel": STAGES[dominant], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "algorithmic_bytes_per_launch": int(entry_bytes(dominant, 1)), "mean_launch_ms": round(dom["mean_ms"], 5),
+                     "algorithmic_bytes_per_launch": int(dom_bytes), "mean_launch_ms": round(dom_ms, 5),
                      "traffic": None},
         "graph_hbm": {"algorithmic_bytes_per_frame": graph_bytes, "achieved": round(graph_gbs, 1), "unit": "GB/s",
                       "frac": round(graph_gbs / HBM_PEAK_GBS, 4)},
-        "per_entry_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items(), key=lambda kv: -kv[1])},
-        "corners_frame0": r["count"], "wall_s": round(wall, 3),
+        "stage_ms_per_step": {STAGES[k]: round(v, 4) for k, v in stage_ms.items()},
+        "corners_frame0": corners0, "wall_s": round(wall, 3),
     }
+
+    if not args.no_per_image:
+        # the reference's execution model on the same frames: one launch per node, frame after frame
+        pg = VisionGraph4K(W, H, DW, DH, LEVELS, **PARAMS)
+        steps_pi = max(2, args.steps // 4)
+        for fr in G.frames[:2]:
+            pg.run(fr.rgb)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps_pi):
+            for fr in G.frames:
+                pg.run(fr.rgb)
+        b.record()
+        torch.cuda.synchronize()
+        pi_ms = a.elapsed_time(b)
+        pi_value = steps_pi * len(G.frames) * W * H / (pi_ms / 1e3) / 1e6
+        out["per_image"] = {"value": round(pi_value, 2), "unit": "Mpix/s",
+                            "what": "same frames, one launch per node and frame (no fusion, no batching)",
+                            "batched_over_per_image": round(value / pi_value, 2)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(base[0], args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(base[0].numpy(), args.cpu_budget)
     print(json.dumps(out))
 
 

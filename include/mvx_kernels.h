@@ -164,6 +164,48 @@ int mvx_HarrisMergeSortAndPick_XY_XYS(mvx_stream_t stream, uint32_t capacity, vo
                                       const uint32_t *pSrcCount, uint32_t maxList, float minDistance,
                                       uint32_t width, uint32_t height, void *workspace, size_t workspaceSize);
 
+/* ------------------------------------------------------------------ fused vision graph ----- */
+/* The RGB vision graph (vx nodes: ColorConvert RGB->IYUV, ChannelExtract Y, ScaleImage half bilinear,
+ * GaussianPyramid HALF, CannyEdgeDetector 3/L1, HarrisCorners 3/3) over a batch of equally sized
+ * frames in a handful of launches.  It has no counterpart among the reference's HIP launchers: the
+ * reference runs one kernel per node and the Canny trace and Harris pick on the CPU
+ * (ago_util_hip.cpp:740-744 leaves supernode launches unimplemented).  Outputs are bit-exact with the
+ * per-node entry points above.  Applies when mvx_VisionGraph_supported() returns 1 (width >= 16 and
+ * a multiple of 4, even height, scaled image exactly half size, valid pyramid steps). */
+typedef struct mvx_vision_params {
+    uint32_t width, height;              /* RGB input size */
+    uint32_t scaledWidth, scaledHeight;  /* ScaleImage output (width/2, height/2) */
+    uint32_t levels;                     /* pyramid levels (1..8), level 0 = luma */
+    uint32_t cannyLower, cannyUpper;     /* hysteresis thresholds */
+    float harrisK, harrisThreshold, harrisMinDistance;
+    uint32_t cornerCapacity;             /* vx_keypoint_t capacity of each frame's corner array */
+} mvx_vision_params_t;
+
+typedef struct mvx_vision_frame {
+    const uint8_t *rgb; uint32_t rgbStride;
+    uint8_t *pyramid[8]; uint32_t pyramidStride[8];
+    uint8_t *scaled; uint32_t scaledStride;
+    uint8_t *edges; uint32_t edgesStride;    /* pixels outside [1,w-2]x[1,h-2] are not written */
+    void *corners;                           /* vx_keypoint_t[cornerCapacity] */
+    uint32_t *numItems;                      /* device: min(count, capacity) */
+    uint64_t *numCorners;                    /* device: vx_size corner count (uncapped) */
+} mvx_vision_frame_t;
+
+int mvx_VisionGraph_supported(const mvx_vision_params_t *params);
+size_t mvx_VisionGraph_workspace_size(const mvx_vision_params_t *params, uint32_t maxFrames);
+/* once per set of frames: validates the planes and writes the frame table into the workspace (device
+ * memory, 16-byte aligned); synchronous with respect to the host. */
+int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_params_t *params, const mvx_vision_frame_t *frames,
+                            uint32_t nframes, void *workspace, size_t workspaceSize);
+/* every batch: enqueues the graph for the prepared frames; asynchronous, capturable in a hipGraph. */
+int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes, void *workspace,
+                        size_t workspaceSize);
+/* the same, one stage at a time (0 fused tile pass, 1 hysteresis border union, 2 hysteresis resolve,
+ * 3 pyramid levels 2.., 4 Harris sort and pick); stages must run in order. */
+#define MVX_VISION_STAGES 5
+int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes,
+                              void *workspace, size_t workspaceSize, int stage);
+
 #ifdef __cplusplus
 }
 #endif

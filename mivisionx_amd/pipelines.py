@@ -128,3 +128,77 @@ class VisionGraph4K:
         kp, count = self.harris.result()
         return dict(scaled=self.scaled.download(), pyramid=[p.download() for p in self.pyr],
                     edges=self.edges.download(), corners=kp, count=count)
+
+
+class VisionParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("scaledWidth", ctypes.c_uint32),
+                ("scaledHeight", ctypes.c_uint32), ("levels", ctypes.c_uint32), ("cannyLower", ctypes.c_uint32),
+                ("cannyUpper", ctypes.c_uint32), ("harrisK", ctypes.c_float), ("harrisThreshold", ctypes.c_float),
+                ("harrisMinDistance", ctypes.c_float), ("cornerCapacity", ctypes.c_uint32)]
+
+
+class VisionFrame(ctypes.Structure):
+    _fields_ = [("rgb", ctypes.c_void_p), ("rgbStride", ctypes.c_uint32),
+                ("pyramid", ctypes.c_void_p * 8), ("pyramidStride", ctypes.c_uint32 * 8),
+                ("scaled", ctypes.c_void_p), ("scaledStride", ctypes.c_uint32),
+                ("edges", ctypes.c_void_p), ("edgesStride", ctypes.c_uint32),
+                ("corners", ctypes.c_void_p), ("numItems", ctypes.c_void_p), ("numCorners", ctypes.c_void_p)]
+
+
+class FrameOutputs:
+    """Input and output planes of one frame of the vision graph."""
+
+    def __init__(self, w, h, dw, dh, levels, capacity):
+        self.rgb = Plane(w, h, "RGB")
+        self.scaled = Plane(dw, dh, "U8")
+        self.pyr = [Plane(a, b, "U8") for (a, b) in pyramid_dims(w, h, levels)]
+        self.edges = Plane(w, h, "U8")
+        self.capacity = capacity
+        self.corners = _dev_bytes(28 * max(capacity, 1))
+        self.num_items = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.count = torch.zeros(1, dtype=torch.int64, device="cuda")
+
+    def results(self):
+        n = int(self.num_items.item())
+        kp = self.corners[:28 * n].cpu().numpy().view(KEYPOINT).copy()
+        return dict(scaled=self.scaled.download(), pyramid=[p.download() for p in self.pyr],
+                    edges=self.edges.download(), corners=kp, count=int(self.count.item()))
+
+
+class FusedVisionGraph:
+    """The vision graph over a batch of frames through mvx_VisionGraph_* (fused, frame-batched)."""
+
+    def __init__(self, w, h, dw, dh, levels=5, lower=80, upper=100, k=0.04, thr=0.0005, dist=5.0, capacity=65536,
+                 frames=1):
+        K = kernels()
+        K.mvx_VisionGraph_workspace_size.restype = ctypes.c_size_t
+        self.params = VisionParams(w, h, dw, dh, levels, lower, upper, k, thr, dist, capacity)
+        if not K.mvx_VisionGraph_supported(ctypes.byref(self.params)):
+            raise ValueError("fused vision graph does not apply to these parameters")
+        self.frames = [FrameOutputs(w, h, dw, dh, levels, capacity) for _ in range(frames)]
+        arr = (VisionFrame * frames)()
+        for i, f in enumerate(self.frames):
+            d = arr[i]
+            d.rgb, d.rgbStride = f.rgb.t.data_ptr(), f.rgb.stride
+            for l, p in enumerate(f.pyr):
+                d.pyramid[l], d.pyramidStride[l] = p.t.data_ptr(), p.stride
+            d.scaled, d.scaledStride = f.scaled.t.data_ptr(), f.scaled.stride
+            d.edges, d.edgesStride = f.edges.t.data_ptr(), f.edges.stride
+            d.corners, d.numItems, d.numCorners = f.corners.data_ptr(), f.num_items.data_ptr(), f.count.data_ptr()
+        self.ws_size = K.mvx_VisionGraph_workspace_size(ctypes.byref(self.params), frames)
+        self.ws = _dev_bytes(self.ws_size)
+        self._arr = arr
+        rc = K.mvx_VisionGraph_prepare(_stream(), ctypes.byref(self.params), arr, frames, _ptr(self.ws),
+                                       ctypes.c_size_t(self.ws_size))
+        if rc != 0:
+            raise RuntimeError(f"mvx_VisionGraph_prepare returned {rc}")
+
+    def run(self):
+        rc = kernels().mvx_VisionGraph_run(_stream(), ctypes.byref(self.params), len(self.frames), _ptr(self.ws),
+                                           ctypes.c_size_t(self.ws_size))
+        if rc != 0:
+            raise RuntimeError(f"mvx_VisionGraph_run returned {rc}")
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

@@ -105,3 +105,31 @@ def test_graph2_4k_bit_exact(gpu, orc, g2):
         assert [sha(l) for l in r["pyramid"]] == g["pyramid"], "pyramid"
         assert sha(r["edges"]) == g["edges"], "edges"
         assert r["count"] == g["count"] and sha(r["corners"]) == g["corners"], "corners"
+
+
+@pytest.mark.parametrize("w,h,frames", [(3840, 2160, 2), (1920, 1080, 3), (100, 64, 2), (36, 20, 3), (644, 482, 2),
+                                        (16, 16, 1)])
+def test_fused_graph_matches_oracle(gpu, orc, w, h, frames):
+    from mivisionx_amd.pipelines import FusedVisionGraph
+    levels = 5 if min(w, h) >= 80 else 2
+    params = dict(lower=80, upper=100, k=0.04, thr=0.0005, dist=5.0, capacity=65536)
+    if w * h < 100000:
+        params.update(lower=20, upper=60, thr=1e-6)
+    G = FusedVisionGraph(w, h, w // 2, h // 2, levels, frames=frames, **params)
+    inputs = [synth.scene_rgb(w, h, 40 + f) if w * h < 3_000_000 or f == 0 else
+              np.roll(synth.scene_rgb(w, h, 40), 997 * f, axis=1) for f in range(frames)]
+    for f, rgb in zip(G.frames, inputs):
+        f.rgb.upload(rgb)
+    for rep in range(2):
+        G.run()
+        for i, (f, rgb) in enumerate(zip(G.frames, inputs)):
+            got = f.results()
+            want = orc.graph2(rgb, w // 2, h // 2, levels, params["lower"], params["upper"], params["k"], params["thr"],
+                              params["dist"], params["capacity"])
+            tag = f"{w}x{h} frame {i} rep {rep}"
+            assert_same(got["scaled"], want["scaled"], "scaled " + tag)
+            for l, (a, b) in enumerate(zip(got["pyramid"], want["pyramid"])):
+                assert_same(a, b, f"pyramid {l} " + tag)
+            assert_same(got["edges"], want["edges"], "edges " + tag)
+            assert got["count"] == want["count"], "corner count " + tag
+            assert_same(got["corners"], want["corners"], "corners " + tag)
