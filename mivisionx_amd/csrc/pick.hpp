@@ -109,11 +109,19 @@ inline PickWork pick_work(void *workspace, uint32_t maxList)
 // *listCount is cleared at the end (the list producer appends with atomics).
 MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetCount, uint32_t maxList, float minDistance,
                                uint32_t width, uint32_t height, Keypoint *out, uint32_t capacity,
-                               uint32_t *outNumItems, uint64_t *outCount, const PickWork &W)
+                               uint32_t *outNumItems, uint64_t *outCount, const PickWork &Wfull)
 {
     __shared__ int64_t lkeys[PICK_LDS_KEYS];
     __shared__ uint32_t sflag, ssum[PICK_THREADS];
     const uint32_t n = min(*listCount, maxList);
+    // use only a table of about 4n slots (>= 1024) so that the probes stay cache resident; the slots
+    // beyond it stay empty, and the ones used here are emptied again at the end
+    PickWork W = Wfull;
+    {
+        uint32_t hs = 1024;
+        while (hs < 4u * n && hs <= Wfull.hmask) hs <<= 1;
+        W.hmask = min(hs, Wfull.hmask + 1) - 1;
+    }
     uint32_t np = 1;
     while (np < n) np <<= 1;
     const bool inLds = np <= PICK_LDS_KEYS;

@@ -9,21 +9,24 @@
 //                                         Harris 3x3/3x3 (HG3 -> score -> NMS -> sort and pick) }
 //
 // Launches per batch of frames (blockIdx.z = frame):
-//   k_front     one 64x32 luma tile (+3 halo) per workgroup: RGB -> Y in LDS, then level 0, the 2:1
-//               scaled image, level 1, Canny gradients/NMS/threshold with tile-local hysteresis
-//               (LDS union-find), Harris products/exact integer window sums/score/NMS with atomic
-//               candidate append.  Nothing but graph outputs and small per-tile tables reaches HBM.
+//   k_front     one 58x32 luma tile (+3 halo) per workgroup of four waves.  Every window row of the
+//               tile (64 luma, 62 gradient, 60 score columns) fits one 64-lane wave, so lane = column
+//               and a wave walks rows: RGB -> Y in LDS, then level 0, the 2:1 scaled image, level 1,
+//               Canny gradients/NMS/threshold with tile-local hysteresis (LDS union-find), Harris
+//               products / exact integer window sums / score / NMS with candidate append.  Only graph
+//               outputs and small per-tile tables reach HBM.  Tiles away from the image border take a
+//               branch-free instantiation.
 //   k_border    unions the hysteresis components that touch tile borders, across tiles.  Strong
 //               components get ids below every weak one, so after atomicMin linking a set's root is
 //               strong iff the set holds a strong component -- no marking pass.
 //   k_resolve   only tiles left with undecided weak pixels re-read their edge tile and settle them.
 //   k_half      pyramid levels 2.. (one launch per level, all frames).
-//   k_pick      Harris sort and pick, one workgroup per frame (harris.hip's algorithm).
+//   k_pick      Harris sort and pick, one workgroup per frame (pick.hpp).
 //
 // Every output is bit-exact with the reference CPU path; the equivalences used are stated where they
-// are used.  Limits of the fused path (the caller falls back to the per-node kernels otherwise):
-// width >= 16 and a multiple of 4 (the HG3 pitch quirk is then the identity), scaled image exactly
-// half size of an even-sized input.
+// are used.  Limits of the fused path (callers use the per-node kernels otherwise): width >= 16 and a
+// multiple of 4 (the HG3 pitch quirk is then the identity), even height, scaled image exactly half
+// size, every pyramid step one the reference accepts.
 #include "mvx_common.hpp"
 #include "pick.hpp"
 #include "pyramid.hpp"
@@ -33,13 +36,15 @@
 
 namespace mvx {
 
-constexpr int TW = 64, TH = 32;                 // luma tile
+constexpr int TW = 58, TH = 32;                  // luma tile
 constexpr int HY = 3;                            // luma halo: Sobel 1 + Harris window 1 + NMS 1
-constexpr int YW = TW + 2 * HY, YH = TH + 2 * HY; // 70 x 38
-constexpr int GW = TW + 4, GH = TH + 4;          // gradients, halo 2
-constexpr int MW = TW + 2, MH = TH + 2;          // Canny magnitude / Harris score, halo 1
-constexpr int TS = TW * TH;                      // pixels per tile
-constexpr int FT = 256;                          // threads of the front kernel
+constexpr int TP = 64;                           // LDS row pitch (one wave row)
+constexpr int YH = TH + 2 * HY;                  // luma window rows (columns: TW + 6 = 64)
+constexpr int GH = TH + 4, GW = TW + 4;          // gradients, halo 2
+constexpr int MH = TH + 2, MW = TW + 2;          // Canny magnitude / Harris score, halo 1
+constexpr int TS = TH * TP;                      // tile-local pixel index space, p = ly * TP + lx
+constexpr int FT = 256;                          // threads of the tile kernels (4 waves)
+constexpr int NW = FT / 64;
 
 struct FrameDesc {
     const uint8_t *rgb; uint32_t rgbStride;
@@ -66,7 +71,8 @@ struct GraphParams {
     uint32_t capacity, candCap;
 };
 
-// luma of RGB pixel (r, c), HafCpu_ColorConvert_Y_RGB (ago_haf_cpu_color_convert.cpp:2913-2990)
+// luma of one RGB pixel, HafCpu_ColorConvert_Y_RGB (ago_haf_cpu_color_convert.cpp:2913-2990):
+// ((R * 0.2126f) + G * 0.7152f) + B * 0.0722f in float, truncated, saturated
 MVX_DEV uint32_t luma(const uint8_t *p)
 {
     float v = (float)p[0] * 0.2126f;
@@ -79,6 +85,7 @@ MVX_DEV uint32_t luma(const uint8_t *p)
 // the virtual luma plane at linear position (r, c): stride yStride, padding and guard bytes zero
 MVX_DEV uint32_t yflat(const FrameDesc &F, const GraphParams &P, int r, int c)
 {
+    if (c >= 0 && c < P.w && r >= 0 && r < P.h) return luma(F.rgb + (long)r * F.rgbStride + 3L * c);
     const long i = (long)r * P.yStride + c;
     if (i < 0 || i >= (long)P.h * P.yStride) return 0;
     r = (int)(i / P.yStride);
@@ -87,6 +94,7 @@ MVX_DEV uint32_t yflat(const FrameDesc &F, const GraphParams &P, int r, int c)
     return luma(F.rgb + (long)r * F.rgbStride + 3L * c);
 }
 
+// HafCpu_FastAtan2_Canny (ago_haf_cpu_arithmetic.cpp:9638-9656), statement order kept
 MVX_DEV uint32_t canny_sector_f(int gx, int gy)
 {
     const uint32_t ax = (uint32_t)abs(gx) & 0xffffu, ay = (uint32_t)abs(gy) & 0xffffu;
@@ -106,7 +114,8 @@ MVX_DEV uint16_t canny_pack_f(int gx, int gyn)
     return (uint16_t)t;
 }
 
-// LDS union-find over tile pixel indices
+// union-find with atomicMin linking (the larger root hangs under the smaller): the final root of
+// a component is its smallest index, whatever the order of the unions
 MVX_DEV int lfind(const int *L, int p)
 {
     int l = L[p];
@@ -147,258 +156,303 @@ MVX_DEV void gunion(uint32_t *par, uint32_t a, uint32_t b)
     }
 }
 
+struct FrontShared {
+    uint8_t Y[YH][TP];
+    int16_t Gx[GH][TP], Gy[GH][TP];
+    uint16_t M[MH][TP];
+    float V[MH][TP];
+    uint8_t S[TS];            // Canny states 0 / 127 / 255 of the tile's pixels
+    int L[TS];                // union-find labels of the weak pixels no strong pixel reaches in the tile
+    uint8_t FB[TS];           // root flag: the component touches the tile border
+    uint64_t Em[TH], Sm[TH], Rm[TH];   // per row: edge, strong, reached-from-strong pixels (bit = column)
+    uint32_t cnt, base, unres;
+};
 
-
-__global__ void __launch_bounds__(FT)
-k_front(const FrameDesc *__restrict__ frames, GraphParams P)
+// INNER: the whole luma window lies inside the image and every Canny / Harris position of the tile
+// window is an interior pixel, so no image-border rule applies.
+template <bool INNER>
+MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P, int tx0, int ty0, int tile)
 {
-    __shared__ uint8_t sY[YH][YW];
-    __shared__ int16_t sGx[GH][GW], sGy[GH][GW];
-    __shared__ uint16_t sM[MH][MW];
-    __shared__ float sV[MH][MW];
-    __shared__ uint8_t sS[TS];
-    __shared__ int sL[TS];
-    __shared__ uint8_t sFS[TS], sFB[TS];
-    __shared__ uint32_t sCnt, sBase, sUnres;
-
-    const FrameDesc &F = frames[blockIdx.z];
-    const int tx0 = (int)blockIdx.x * TW, ty0 = (int)blockIdx.y * TH;
-    const int tile = (int)blockIdx.y * P.tilesX + (int)blockIdx.x;
-    const int t = (int)threadIdx.x;
+    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
     const int w = P.w, h = P.h;
-    if (t == 0) { sCnt = 0; sUnres = 0; }
+    const int tw = INNER ? TW : min(TW, w - tx0), th = INNER ? TH : min(TH, h - ty0);
 
-    // 1. luma window (linear-address semantics of the virtual Y plane)
-    for (int i = t; i < YW * YH; i += FT) {
-        const int wy = i / YW, wx = i % YW;
-        sY[wy][wx] = (uint8_t)yflat(F, P, ty0 - HY + wy, tx0 - HY + wx);
+    // 1. luma window: lane = column (tx0-3+lane), linear-address semantics of the virtual Y plane.
+    //    Inner tiles issue all of a lane's RGB loads before converting, so their latencies overlap.
+    if (INNER) {
+        constexpr int RPW = (YH + NW - 1) / NW;   // rows per wave
+        uint8_t px[RPW][3];
+        const uint8_t *col = F.rgb + (long)(ty0 - HY + wave) * F.rgbStride + 3L * (tx0 - HY + lane);
+#pragma unroll
+        for (int k = 0; k < RPW; k++) {
+            if (wave + k * NW < YH) {
+                const uint8_t *p = col + (long)(k * NW) * F.rgbStride;
+                px[k][0] = p[0]; px[k][1] = p[1]; px[k][2] = p[2];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RPW; k++)
+            if (wave + k * NW < YH) s.Y[wave + k * NW][lane] = (uint8_t)luma(px[k]);
+    } else {
+        for (int r = wave; r < YH; r += NW) {
+            const int y = ty0 - HY + r, x = tx0 - HY + lane;
+            s.Y[r][lane] = (uint8_t)yflat(F, P, y, x);
+        }
     }
     __syncthreads();
 
-    // 2. Sobel gradients (linear-address reads) at (ty0-2+j, tx0-2+i)
-    for (int n = t; n < GW * GH; n += FT) {
-        const int j = n / GW, i = n % GW;
-        const int a0 = sY[j][i], a1 = sY[j][i + 1], a2 = sY[j][i + 2];
-        const int b0 = sY[j + 1][i], b2 = sY[j + 1][i + 2];
-        const int c0 = sY[j + 2][i], c1 = sY[j + 2][i + 1], c2 = sY[j + 2][i + 2];
-        sGx[j][i] = (int16_t)((c2 - c0) + (a2 - a0) + ((b2 - b0) << 1));
-        sGy[j][i] = (int16_t)((c0 + (c1 << 1) + c2) - (a0 + (a1 << 1) + a2));
+    // 2. Sobel gradients (linear-address reads) at (ty0-2+r, tx0-2+lane)
+    if (lane < GW) {
+        for (int r = wave; r < GH; r += NW) {
+            const int a0 = s.Y[r][lane], a1 = s.Y[r][lane + 1], a2 = s.Y[r][lane + 2];
+            const int b0 = s.Y[r + 1][lane], b2 = s.Y[r + 1][lane + 2];
+            const int c0 = s.Y[r + 2][lane], c1 = s.Y[r + 2][lane + 1], c2 = s.Y[r + 2][lane + 2];
+            s.Gx[r][lane] = (int16_t)((c2 - c0) + (a2 - a0) + ((b2 - b0) << 1));
+            s.Gy[r][lane] = (int16_t)((c0 + (c1 << 1) + c2) - (a0 + (a1 << 1) + a2));
+        }
     }
     __syncthreads();
 
-    // 3. Canny U16 magnitude|sector and Harris score at (ty0-1+j, tx0-1+i)
-    for (int n = t; n < MW * MH; n += FT) {
-        const int j = n / MW, i = n % MW;
-        const int y = ty0 - 1 + j, x = tx0 - 1 + i;
-        uint16_t m = 0;
-        float v = 0.0f;
-        if (y >= 1 && y <= h - 2 && x >= 0 && x < w) {
-            if (x == 0) {
+    // 3. at (ty0-1+r, tx0-1+lane): Canny U16 magnitude|sector; Harris score from a per-column
+    //    sliding window of 3-column sums of the HG3 products {gx^2, gx gy, gy^2}.  HG3 rows 0 and h-1
+    //    are never written (zero).  The products are integers whose 9-sums stay below 2^24, so int32
+    //    sums equal the reference's float sums exactly.
+    if (lane < MW) {
+        const int x = tx0 - 1 + lane;
+        const int r0 = wave * ((MH + NW - 1) / NW), r1 = min(r0 + (MH + NW - 1) / NW, MH);
+        int hxx[3], hxy[3], hyy[3];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int gr = r0 + q;
+            hxx[q] = hxy[q] = hyy[q] = 0;
+            if (INNER || (ty0 - 2 + gr >= 1 && ty0 - 2 + gr <= h - 2)) {
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) {
+                    const int gx = s.Gx[gr][lane + dx], gy = s.Gy[gr][lane + dx];
+                    hxx[q] += gx * gx; hxy[q] += gx * gy; hyy[q] += gy * gy;
+                }
+            }
+        }
+        for (int r = r0; r < r1; r++) {
+            const int y = ty0 - 1 + r;
+            // Canny
+            uint16_t m = 0;
+            if (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w)) {
+                m = canny_pack_f(s.Gx[r + 1][lane + 1], -(int)s.Gy[r + 1][lane + 1]);
+            } else if (y >= 1 && y <= h - 2 && x == 0) {
                 // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
-                // r1[-1] = r0[ALIGN16(w)-1]); window column 3 is x = 0
-                const int y0w = j + 2 - 1, y1w = j + 2, y2w = j + 2 + 1;   // rows y-1, y, y+1 in sY
-                const int t0_1 = (int)sY[y0w][4] + (int)sY[y2w][4] + 2 * (int)sY[y1w][4];
-                const int t1_0 = (int)sY[y2w][3] - (int)sY[y0w][3];
-                const int t1_1 = (int)sY[y2w][4] - (int)sY[y0w][4];
+                // r1[-1] = r0[ALIGN16(w)-1]); luma window column 3 is x = 0, rows r+1..r+3 are y-1..y+1
+                const int t0_1 = (int)s.Y[r + 1][4] + (int)s.Y[r + 3][4] + 2 * (int)s.Y[r + 2][4];
+                const int t1_0 = (int)s.Y[r + 3][3] - (int)s.Y[r + 1][3];
+                const int t1_1 = (int)s.Y[r + 3][4] - (int)s.Y[r + 1][4];
                 const int t1_m1 = (w % 16 == 0) ? ((int)yflat(F, P, y - 1, w - 1) + (int)yflat(F, P, y + 1, w - 1) +
                                                    2 * (int)yflat(F, P, y, w - 1)) : 0;
                 m = canny_pack_f(t0_1, -(t1_m1 + 2 * t1_0 + t1_1));
-            } else {
-                m = canny_pack_f(sGx[j + 1][i + 1], -(int)sGy[j + 1][i + 1]);
             }
-        }
-        if (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2) {
-            // window sums of HG3 = {gx^2, gx gy, gy^2}; HG3 rows 0 and h-1 are never written (zero).
-            // The terms are integers below 2^24 in sum, so int32 sums equal the reference's float sums.
-            int sxx = 0, sxy = 0, syy = 0;
+            s.M[r][lane] = m;
+            // Harris
+            const int gr = r + 2;
+            hxx[2] = hxy[2] = hyy[2] = 0;
+            if (INNER || (ty0 - 2 + gr >= 1 && ty0 - 2 + gr <= h - 2)) {
 #pragma unroll
-            for (int dy = -1; dy <= 1; dy++) {
-                const int yy = y + dy;
-                if (yy < 1 || yy > h - 2) continue;
-#pragma unroll
-                for (int dx = -1; dx <= 1; dx++) {
-                    const int gx = sGx[j + 1 + dy][i + 1 + dx], gy = sGy[j + 1 + dy][i + 1 + dx];
-                    sxx += gx * gx;
-                    sxy += gx * gy;
-                    syy += gy * gy;
+                for (int dx = 0; dx < 3; dx++) {
+                    const int gx = s.Gx[gr][lane + dx], gy = s.Gy[gr][lane + dx];
+                    hxx[2] += gx * gx; hxy[2] += gx * gy; hyy[2] += gy * gy;
                 }
             }
-            const float gx2 = (float)sxx, gxy2 = (float)sxy, gy2 = (float)syy;
-            const float tr = gx2 + gy2;
-            const float det = (gx2 * gy2) - (gxy2 * gxy2);
-            float mc = det - (P.k * tr * tr);
-            mc = __fdiv_rn(mc, P.norm);
-            v = (mc > P.thr) ? mc : 0.0f;
+            float v = 0.0f;
+            if (INNER || (x >= 1 && x <= w - 2 && y >= 1 && y <= h - 2)) {
+                const float gx2 = (float)(hxx[0] + hxx[1] + hxx[2]);
+                const float gxy2 = (float)(hxy[0] + hxy[1] + hxy[2]);
+                const float gy2 = (float)(hyy[0] + hyy[1] + hyy[2]);
+                const float tr = gx2 + gy2;
+                const float det = (gx2 * gy2) - (gxy2 * gxy2);
+                const float mc = det - (P.k * tr * tr);
+                // Vc = mc / norm > T ? mc / norm : 0; with norm > 0 and T >= 0 a non-positive mc can
+                // never pass, so the IEEE divide runs only for positive mc (exactly the same result)
+                if (mc > 0.0f || P.thr < 0.0f) {
+                    const float q = __fdiv_rn(mc, P.norm);
+                    v = (q > P.thr) ? q : 0.0f;
+                }
+            }
+            s.V[r][lane] = v;
+            hxx[0] = hxx[1]; hxy[0] = hxy[1]; hyy[0] = hyy[1];
+            hxx[1] = hxx[2]; hxy[1] = hxy[2]; hyy[1] = hyy[2];
         }
-        sM[j][i] = m;
-        sV[j][i] = v;
     }
     __syncthreads();
 
-    // 4. per interior pixel: Canny NMS + threshold, Harris NMS (candidates), labels
-    bool isCand[TS / FT];
-    int nCand = 0;
-#pragma unroll
-    for (int q = 0; q < TS / FT; q++) {
-        const int p = t + q * FT;
-        const int ly = p / TW, lx = p % TW;
-        const int y = ty0 + ly, x = tx0 + lx;
-        uint8_t s = 0;
+    // 4. tile pixels (ty0+r, tx0+lane): Canny NMS + double threshold, Harris NMS -> candidates
+    uint32_t myCand = 0;                 // bit q: row wave + q*NW is a candidate
+    for (int q = 0; q < TH / NW; q++) {
+        const int r = wave + q * NW;
+        const int p = r * TP + lane;
+        uint8_t st = 0;
         bool cand = false;
-        if (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2) {
-            const int j = ly + 1, i = lx + 1;
-            const uint32_t mv = sM[j][i];
+        const int y = ty0 + r, x = tx0 + lane;
+        if (lane < TW && (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2))) {
+            const int j = r + 1, i = lane + 1;
+            const uint32_t mv = s.M[j][i];
             const int m = (int)(mv >> 2), a = (int)(mv & 3u);
             const int dx0 = (a == 0) ? -1 : (a == 1) ? 1 : (a == 2) ? 0 : -1;
             const int dy0 = (a == 0) ? 0 : -1;
-            const int n0 = (int)(sM[j + dy0][i + dx0] >> 2), n1 = (int)(sM[j - dy0][i - dx0] >> 2);
+            const int n0 = (int)(s.M[j + dy0][i + dx0] >> 2), n1 = (int)(s.M[j - dy0][i - dx0] >> 2);
             const int edge = (m > n0 && m > n1) ? m : 0;
-            s = edge > (int)P.upper ? 255 : (edge <= (int)P.lower ? 0 : 127);
-            const float c = sV[j][i];
-            cand = c >= sV[j - 1][i - 1] && c >= sV[j - 1][i] && c >= sV[j - 1][i + 1] && c >= sV[j][i - 1] &&
-                   c > sV[j][i + 1] && c > sV[j + 1][i - 1] && c > sV[j + 1][i] && c > sV[j + 1][i + 1];
+            st = edge > (int)P.upper ? 255 : (edge <= (int)P.lower ? 0 : 127);
+            const float c = s.V[j][i];
+            cand = c >= s.V[j - 1][i - 1] && c >= s.V[j - 1][i] && c >= s.V[j - 1][i + 1] && c >= s.V[j][i - 1] &&
+                   c > s.V[j][i + 1] && c > s.V[j + 1][i - 1] && c > s.V[j + 1][i] && c > s.V[j + 1][i + 1];
         }
-        sS[p] = s;
-        sL[p] = s ? p : -1;
-        sFS[p] = 0;
-        sFB[p] = 0;
-        isCand[q] = cand;
-        nCand += cand ? 1 : 0;
+        s.S[p] = st;
+        s.FB[p] = 0;
+        if (cand) myCand |= 1u << q;
+        // row masks of edge (127/255) and strong (255) pixels: bit = column
+        const uint64_t em = __ballot(st != 0), sm = __ballot(st == 255);
+        if (lane == 0) { s.Em[r] = em; s.Sm[r] = sm; }
     }
-    // candidate append: one atomic per workgroup
+    // candidate append: one LDS atomic per thread with candidates, one global atomic per workgroup
+    const uint32_t nCand = (uint32_t)__popc(myCand);
     uint32_t myOff = 0;
-    if (nCand) myOff = atomicAdd(&sCnt, (uint32_t)nCand);
+    if (nCand) myOff = atomicAdd(&s.cnt, nCand);
     __syncthreads();
-    if (t == 0 && sCnt) sBase = atomicAdd(F.candCount, sCnt);
+    if (t == 0 && s.cnt) s.base = atomicAdd(F.candCount, s.cnt);
     __syncthreads();
     if (nCand) {
         XYS *list = (XYS *)F.cand;
-        uint32_t pos = sBase + myOff;
-#pragma unroll
-        for (int q = 0; q < TS / FT; q++) {
-            if (!isCand[q]) continue;
-            const int p = t + q * FT;
-            const int ly = p / TW, lx = p % TW;
+        uint32_t pos = s.base + myOff;
+        for (int q = 0; q < TH / NW; q++) {
+            if (!(myCand & (1u << q))) continue;
+            const int r = wave + q * NW;
             if (pos < P.candCap) {
                 XYS it;
-                it.x = (int16_t)(tx0 + lx);
-                it.y = (int16_t)(ty0 + ly);
-                it.s = sV[ly + 1][lx + 1];
+                it.x = (int16_t)(tx0 + lane);
+                it.y = (int16_t)(ty0 + r);
+                it.s = s.V[r + 1][lane + 1];
                 list[pos] = it;
             }
             pos++;
         }
     }
 
-    // 5. graph outputs that only need luma: level 0, 2:1 scaled, level 1
-    {
-        const int tw = min(TW, w - tx0), th = min(TH, h - ty0);
-        uint8_t *l0 = F.lvl[0];
-        for (int n = t; n < (TW / 4) * TH; n += FT) {
-            const int ly = n / (TW / 4), lx = (n % (TW / 4)) * 4;
-            if (ly >= th || lx >= tw) continue;
-            uint8_t *d = l0 + (long)(ty0 + ly) * F.lvlStride[0] + tx0 + lx;
-            if (lx + 4 <= tw) {
-                const uint32_t v = (uint32_t)sY[ly + HY][lx + HY] | ((uint32_t)sY[ly + HY][lx + HY + 1] << 8) |
-                                   ((uint32_t)sY[ly + HY][lx + HY + 2] << 16) | ((uint32_t)sY[ly + HY][lx + HY + 3] << 24);
-                *reinterpret_cast<uint32_t *>(d) = v;
-            } else {
-                for (int u = 0; u < tw - lx; u++) d[u] = sY[ly + HY][lx + HY + u];
-            }
-        }
+    // 5. outputs that only need luma: level 0, the 2:1 scaled image, level 1
+    if (lane < tw) {
+        uint8_t *l0 = F.lvl[0] + (long)ty0 * F.lvlStride[0] + tx0 + lane;
+        for (int r = wave; r < th; r += NW) l0[(long)r * F.lvlStride[0]] = s.Y[r + HY][lane + HY];
+    }
+    if (lane < TW / 2) {
         // ScaleImage 2:1: xm = 2X, ym = 2Y, xf = yf = 128 (ago_haf_cpu_geometric.cpp:2000-2045 with
         // xscale = yscale = 2, offsets 0.5): two-stage rounded averages; tail columns single rounding
+        const int X = tx0 / 2 + lane;
         const int body = P.dw & ~7;
-        for (int n = t; n < (TW / 2) * (TH / 2); n += FT) {
-            const int oy = n / (TW / 2), ox = n % (TW / 2);
-            const int X = tx0 / 2 + ox, Yo = ty0 / 2 + oy;
-            if (X >= P.dw || Yo >= P.dh) continue;
-            const int sy = 2 * oy + HY, sx = 2 * ox + HY;
-            const int p00 = sY[sy][sx], p01 = sY[sy][sx + 1], p10 = sY[sy + 1][sx], p11 = sY[sy + 1][sx + 1];
-            int v;
-            if (X < body) {
-                const int t0 = (128 * p00 + 128 * p01 + 0x80) >> 8, t1 = (128 * p10 + 128 * p11 + 0x80) >> 8;
-                v = (128 * t0 + 128 * t1 + 0x80) >> 8;
-            } else {
-                v = (128 * 128 * (p00 + p01 + p10 + p11) + 0x8000) >> 16;
+        if (X < P.dw) {
+            for (int oy = wave; oy < TH / 2; oy += NW) {
+                const int Yo = ty0 / 2 + oy;
+                if (Yo >= P.dh) break;
+                const int sy = 2 * oy + HY, sx = 2 * lane + HY;
+                const int p00 = s.Y[sy][sx], p01 = s.Y[sy][sx + 1], p10 = s.Y[sy + 1][sx], p11 = s.Y[sy + 1][sx + 1];
+                int v;
+                if (X < body) {
+                    const int t0 = (128 * p00 + 128 * p01 + 0x80) >> 8, t1 = (128 * p10 + 128 * p11 + 0x80) >> 8;
+                    v = (128 * t0 + 128 * t1 + 0x80) >> 8;
+                } else {
+                    v = (128 * 128 * (p00 + p01 + p10 + p11) + 0x8000) >> 16;
+                }
+                F.scaled[(long)Yo * F.scaledStride + X] = (uint8_t)v;
             }
-            F.scaled[(long)Yo * F.scaledStride + X] = (uint8_t)v;
         }
         // ScaleGaussianHalf_5x5 level 1 (ago_haf_cpu_pyramid.cpp:94-305): centre (2X+ox, 2Y+oy)
-        if (P.levels > 1) {
+        if (P.levels > 1 && X < P.lvlW[1]) {
             const int oxp = (w & 1) ? 0 : 1, oyp = (h & 1) ? 0 : 1;
-            const int l1w = P.lvlW[1], l1h = P.lvlH[1];
-            for (int n = t; n < (TW / 2) * (TH / 2); n += FT) {
-                const int oy = n / (TW / 2), ox = n % (TW / 2);
-                const int X = tx0 / 2 + ox, Yo = ty0 / 2 + oy;
-                if (X >= l1w || Yo < 1 || Yo > l1h - 2) continue;
-                const int cy = 2 * oy + oyp + HY, cx = 2 * ox + oxp + HY;   // window coordinates
+            const int l1h = P.lvlH[1];
+            for (int oy = wave; oy < TH / 2; oy += NW) {
+                const int Yo = ty0 / 2 + oy;
+                if (Yo > l1h - 2) break;
+                if (Yo < 1) continue;
+                const int cy = 2 * oy + oyp + HY, cx = 2 * lane + oxp + HY;
                 uint32_t acc = 0;
-                const uint32_t kk[5] = {1, 4, 6, 4, 1};
 #pragma unroll
                 for (int jj = 0; jj < 5; jj++) {
-                    const uint8_t *r = &sY[cy - 2 + jj][cx - 2];
-                    const uint32_t hr = r[0] + 4u * r[1] + 6u * r[2] + 4u * r[3] + r[4];
-                    acc += kk[jj] * hr;
+                    const uint8_t *rr = &s.Y[cy - 2 + jj][cx - 2];
+                    const uint32_t hr = rr[0] + 4u * rr[1] + 6u * rr[2] + 4u * rr[3] + rr[4];
+                    acc += (jj == 0 || jj == 4 ? 1u : jj == 2 ? 6u : 4u) * hr;
                 }
                 F.lvl[1][(long)Yo * F.lvlStride[1] + X] = (uint8_t)((acc & 0xffffu) >> 8);
             }
         }
     }
 
-    // 6. tile-local hysteresis: union edge pixels (8-connectivity inside the tile)
+    // 6. tile-local hysteresis.  (a) The edge pixels that a strong pixel reaches inside the tile:
+    //    one wave, lane = row, 64-bit row masks, dilate-and-mask until nothing changes.
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < TS / FT; q++) {
-        const int p = t + q * FT;
-        if (!sS[p]) continue;
-        const int ly = p / TW, lx = p % TW;
-        if (lx + 1 < TW && sS[p + 1]) lunion(sL, p, p + 1);
-        if (ly + 1 < TH) {
-            if (lx > 0 && sS[p + TW - 1]) lunion(sL, p, p + TW - 1);
-            if (sS[p + TW]) lunion(sL, p, p + TW);
-            if (lx + 1 < TW && sS[p + TW + 1]) lunion(sL, p, p + TW + 1);
+    if (wave == 0) {
+        const uint64_t e = lane < TH ? s.Em[lane] : 0ull;
+        uint64_t m = lane < TH ? s.Sm[lane] : 0ull;
+        while (true) {
+            uint64_t up = __shfl(m, lane > 0 ? lane - 1 : 0), dn = __shfl(m, lane < 63 ? lane + 1 : 63);
+            if (lane == 0) up = 0ull;
+            if (lane >= TH - 1) dn = 0ull;
+            uint64_t v = m | up | dn;
+            v |= (v << 1) | (v >> 1);
+            const uint64_t n = (v & e) | m;
+            const bool changed = n != m;
+            m = n;
+            if (!__any(changed)) break;
+        }
+        if (lane < TH) s.Rm[lane] = m;
+    }
+    __syncthreads();
+    //    (b) the weak pixels left over: union-find among themselves (8-connectivity in the tile)
+    for (int q = 0; q < TH / NW; q++) {
+        const int r = wave + q * NW;
+        const int p = r * TP + lane;
+        const bool wk = s.S[p] && !((s.Rm[r] >> lane) & 1ull);
+        s.L[p] = wk ? p : -1;
+    }
+    __syncthreads();
+    for (int q = 0; q < TH / NW; q++) {
+        const int r = wave + q * NW;
+        const int p = r * TP + lane;
+        if (s.L[p] < 0) continue;
+        if (lane + 1 < TW && s.L[p + 1] >= 0) lunion(s.L, p, p + 1);
+        if (r + 1 < TH) {
+            if (lane > 0 && s.L[p + TP - 1] >= 0) lunion(s.L, p, p + TP - 1);
+            if (s.L[p + TP] >= 0) lunion(s.L, p, p + TP);
+            if (lane + 1 < TW && s.L[p + TP + 1] >= 0) lunion(s.L, p, p + TP + 1);
         }
     }
     __syncthreads();
-    // flags on roots: holds a strong pixel / touches the tile border
-    const int tw = min(TW, w - tx0), th = min(TH, h - ty0);
-#pragma unroll
-    for (int q = 0; q < TS / FT; q++) {
-        const int p = t + q * FT;
-        if (!sS[p]) continue;
-        const int ly = p / TW, lx = p % TW;
-        const int r = lfind(sL, p);
-        sL[p] = r;   // compress (roots never change after the union pass)
-        if (sS[p] == 255) sFS[r] = 1;
-        if (lx == 0 || ly == 0 || lx == tw - 1 || ly == th - 1) sFB[r] = 1;
+    for (int q = 0; q < TH / NW; q++) {
+        const int r = wave + q * NW;
+        const int p = r * TP + lane;
+        if (s.L[p] < 0) continue;
+        const int root = lfind(s.L, p);
+        if (lane == 0 || r == 0 || lane == tw - 1 || r == th - 1) s.FB[root] = 1;
     }
     __syncthreads();
-    // settle pixels; border-touching components get global ids: strong ones below every weak one
-    const uint32_t weakBase = (uint32_t)(P.tilesX * P.tilesY) * TS;
-    uint32_t unres = 0;
-#pragma unroll
-    for (int q = 0; q < TS / FT; q++) {
-        const int p = t + q * FT;
-        const int ly = p / TW, lx = p % TW;
-        const int y = ty0 + ly, x = tx0 + lx;
-        const uint8_t s = sS[p];
-        if (s) {
-            const int r = sL[p];
-            uint8_t o;
-            if (sFS[r]) o = 255;
-            else if (sFB[r]) { o = 127; unres = 1; }
-            else o = 0;
-            if (r == p && sFB[r]) {
-                const uint32_t gid = (sFS[r] ? 0u : weakBase) + (uint32_t)tile * TS + (uint32_t)p;
-                F.parent[gid] = gid;
+    //    (c) settle: reached -> 255; leftover weak components inside the tile -> 0; leftover ones
+    //    touching the tile border stay 127 for k_border / k_resolve under global id 1 + tile*TS + root.
+    //    Every reached border pixel maps to the single strong node 0 (the smallest id, so after
+    //    atomicMin linking a set is strong iff its root is 0).
+    bool unres = false;
+    for (int q = 0; q < TH / NW; q++) {
+        const int r = wave + q * NW;
+        const int p = r * TP + lane;
+        const int y = ty0 + r, x = tx0 + lane;
+        if (lane >= TW || !(INNER || (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2))) continue;
+        uint8_t o = 0;
+        if ((s.Rm[r] >> lane) & 1ull) {
+            o = 255;
+        } else if (s.L[p] >= 0) {
+            const int root = lfind(s.L, p);
+            if (s.FB[root]) {
+                o = 127;
+                unres = true;
+                if (root == p) F.parent[1u + (uint32_t)tile * TS + (uint32_t)p] = 1u + (uint32_t)tile * TS + (uint32_t)p;
             }
-            F.edges[(long)y * F.edgesStride + x] = o;
-        } else if (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2) {
-            F.edges[(long)y * F.edgesStride + x] = 0;
         }
+        F.edges[(long)y * F.edgesStride + x] = o;
     }
-    if (unres) sUnres = 1;
+    if (unres) s.unres = 1;
     // border tables: top row, bottom row, left column, right column of the tile (actual extent)
     int32_t *bt = F.border + (long)tile * 2 * (TW + TH);
     for (int n = t; n < 2 * (TW + TH); n += FT) {
@@ -409,16 +463,27 @@ k_front(const FrameDesc *__restrict__ frames, GraphParams P)
         else { lx = tw - 1; ly = n - 2 * TW - TH; }
         int32_t g = -1;
         if (lx < tw && ly < th) {
-            const int p = ly * TW + lx;
-            if (sS[p]) {
-                const int r = sL[p];
-                g = (int32_t)((sFS[r] ? 0u : weakBase) + (uint32_t)tile * TS + (uint32_t)r);
-            }
+            const int p = ly * TP + lx;
+            if ((s.Rm[ly] >> lx) & 1ull) g = 0;
+            else if (s.L[p] >= 0) g = (int32_t)(1u + (uint32_t)tile * TS + (uint32_t)lfind(s.L, p));
         }
         bt[n] = g;
     }
     __syncthreads();
-    if (t == 0) F.tileFlag[tile] = sUnres ? 1 : 0;
+    if (t == 0) F.tileFlag[tile] = s.unres ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(FT)
+k_front(const FrameDesc *__restrict__ frames, GraphParams P)
+{
+    __shared__ FrontShared s;
+    const FrameDesc F = frames[blockIdx.z];
+    const int tx0 = (int)blockIdx.x * TW, ty0 = (int)blockIdx.y * TH;
+    const int tile = (int)blockIdx.y * P.tilesX + (int)blockIdx.x;
+    if (threadIdx.x == 0) { s.cnt = 0; s.unres = 0; }
+    const bool inner = tx0 >= 4 && tx0 + TW + 4 <= P.w && ty0 >= 4 && ty0 + TH + 4 <= P.h;
+    if (inner) front_tile<true>(s, F, P, tx0, ty0, tile);
+    else front_tile<false>(s, F, P, tx0, ty0, tile);
 }
 
 // union the border components of each tile with those of its right, lower and diagonal neighbours
@@ -448,20 +513,19 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
     // lower neighbour: my bottom row against its top row, |dc| <= 1
     if (tYi + 1 < P.tilesY) {
         const int32_t *nb = F.border + (long)(tile + P.tilesX) * 2 * (TW + TH);
-        const int ntw = tw;   // same column of tiles
         for (int c = t; c < tw; c += 256) {
             const int32_t a = bt[TW + c];
             if (a < 0) continue;
             for (int d = -1; d <= 1; d++) {
                 const int cc = c + d;
-                if (cc < 0 || cc >= ntw) continue;
+                if (cc < 0 || cc >= tw) continue;   // the tile below has the same width
                 const int32_t b = nb[cc];
                 if (b >= 0) gunion(F.parent, (uint32_t)a, (uint32_t)b);
             }
         }
         if (t == 0) {
-            // diagonal neighbours: bottom-right corner <-> lower-right tile's top-left,
-            // bottom-left corner <-> lower-left tile's top-right
+            // diagonals: bottom-right corner <-> lower-right tile's top-left,
+            // bottom-left corner <-> lower-left tile's top-right (a full-width tile)
             if (tX + 1 < P.tilesX) {
                 const int32_t a = bt[TW + tw - 1];
                 const int32_t *nd = F.border + (long)(tile + P.tilesX + 1) * 2 * (TW + TH);
@@ -469,16 +533,15 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
             }
             if (tX > 0) {
                 const int32_t a = bt[TW + 0];
-                const int ltw = TW;   // the tile to the left is always full width
                 const int32_t *nd = F.border + (long)(tile + P.tilesX - 1) * 2 * (TW + TH);
-                if (a >= 0 && nd[ltw - 1] >= 0) gunion(F.parent, (uint32_t)a, (uint32_t)nd[ltw - 1]);
+                if (a >= 0 && nd[TW - 1] >= 0) gunion(F.parent, (uint32_t)a, (uint32_t)nd[TW - 1]);
             }
         }
     }
 }
 
 // tiles holding undecided weak pixels: rebuild their components (the 127 pixels alone form exactly
-// the undecided components, and LDS union-find roots are component minima, hence the same ids) and
+// the undecided components, and union-find roots are component minima, hence the same ids) and
 // settle each against its global root
 __global__ void __launch_bounds__(FT)
 k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
@@ -490,33 +553,32 @@ k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
     if (!F.tileFlag[tile]) return;
     const int tx0 = (int)blockIdx.x * TW, ty0 = (int)blockIdx.y * TH;
     const int tw = min(TW, P.w - tx0), th = min(TH, P.h - ty0);
-    const int t = (int)threadIdx.x;
-    for (int p = t; p < TS; p += FT) {
-        const int ly = p / TW, lx = p % TW;
-        uint8_t s = 0;
-        if (lx < tw && ly < th) s = F.edges[(long)(ty0 + ly) * F.edgesStride + tx0 + lx] == 127 ? 1 : 0;
-        sS[p] = s;
-        sL[p] = s ? p : -1;
+    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int r = wave; r < TH; r += NW) {
+        const int p = r * TP + lane;
+        uint8_t st = 0;
+        if (lane < tw && r < th) st = F.edges[(long)(ty0 + r) * F.edgesStride + tx0 + lane] == 127 ? 1 : 0;
+        sS[p] = st;
+        sL[p] = st ? p : -1;
     }
     __syncthreads();
-    for (int p = t; p < TS; p += FT) {
+    for (int r = wave; r < TH; r += NW) {
+        const int p = r * TP + lane;
         if (!sS[p]) continue;
-        const int ly = p / TW, lx = p % TW;
-        if (lx + 1 < TW && sS[p + 1]) lunion(sL, p, p + 1);
-        if (ly + 1 < TH) {
-            if (lx > 0 && sS[p + TW - 1]) lunion(sL, p, p + TW - 1);
-            if (sS[p + TW]) lunion(sL, p, p + TW);
-            if (lx + 1 < TW && sS[p + TW + 1]) lunion(sL, p, p + TW + 1);
+        if (lane + 1 < TW && sS[p + 1]) lunion(sL, p, p + 1);
+        if (r + 1 < TH) {
+            if (lane > 0 && sS[p + TP - 1]) lunion(sL, p, p + TP - 1);
+            if (sS[p + TP]) lunion(sL, p, p + TP);
+            if (lane + 1 < TW && sS[p + TP + 1]) lunion(sL, p, p + TP + 1);
         }
     }
     __syncthreads();
-    const uint32_t weakBase = (uint32_t)(P.tilesX * P.tilesY) * TS;
-    for (int p = t; p < TS; p += FT) {
+    for (int r = wave; r < TH; r += NW) {
+        const int p = r * TP + lane;
         if (!sS[p]) continue;
-        const int ly = p / TW, lx = p % TW;
-        const int r = lfind(sL, p);
-        const uint32_t root = gfind(F.parent, weakBase + (uint32_t)tile * TS + (uint32_t)r);
-        F.edges[(long)(ty0 + ly) * F.edgesStride + tx0 + lx] = (root < weakBase) ? 255 : 0;
+        const int root = lfind(sL, p);
+        const uint32_t g = gfind(F.parent, 1u + (uint32_t)tile * TS + (uint32_t)root);
+        F.edges[(long)(ty0 + r) * F.edgesStride + tx0 + lane] = (g == 0) ? 255 : 0;
     }
 }
 
@@ -666,6 +728,8 @@ extern "C" int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_par
         hipLaunchKernelGGL(k_fill_bytes, dim3(256), dim3(256), 0, stream_of(stream), (uint32_t *)(fb + L.pickWs + PL.hkey),
                            HASH_EMPTY, (size_t)PL.hsize);
         hipLaunchKernelGGL(k_fill_bytes, dim3(1), dim3(64), 0, stream_of(stream), d.candCount, 0u, (size_t)1);
+        // union-find slot 0 is the strong node every strong tile-border pixel maps to: its own root
+        hipLaunchKernelGGL(k_fill_bytes, dim3(1), dim3(64), 0, stream_of(stream), d.parent, 0u, (size_t)1);
     }
     if (hipMemcpy(workspace, D.data(), sizeof(FrameDesc) * nframes, hipMemcpyHostToDevice) != hipSuccess) return MVX_FAILURE;
     MVX_LAUNCH_CHECK();
