@@ -1,0 +1,619 @@
+/*
+ * VX/vx.h -- the OpenVX 1.3 C API subset that the MI355X vision-node engine implements.
+ *
+ * Written for this engine from the Khronos OpenVX 1.3 specification.  Every enumeration value,
+ * structure layout and function signature below is the standard one (the reference ships the
+ * Khronos headers as amd_openvx/openvx/include/VX/vx_types.h, vx_api.h, vx_nodes.h), so that an
+ * application compiled against the reference's headers links against libmvx_openvx.so unchanged
+ * for the node set this engine provides:
+ *   Gaussian/Box/Median 3x3, Sobel 3x3, Magnitude, Phase, ColorConvert (RGB -> IYUV/NV12 luma
+ *   path), ChannelExtract, ScaleImage, GaussianPyramid, CannyEdgeDetector, HarrisCorners,
+ *   WarpAffine, WarpPerspective, Remap, OpticalFlowPyrLK.
+ */
+#ifndef _OPENVX_H_
+#define _OPENVX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VX_API_ENTRY
+#define VX_API_CALL
+#define VX_CALLBACK
+
+#define VX_VERSION_MAJOR(x) (((x) & 0xFF) << 8)
+#define VX_VERSION_MINOR(x) (((x) & 0xFF) << 0)
+#define VX_VERSION_1_3 (VX_VERSION_MAJOR(1) | VX_VERSION_MINOR(3))
+#define VX_VERSION VX_VERSION_1_3
+
+#define VX_MAX_KERNEL_NAME 256
+#define VX_MAX_REFERENCE_NAME 64
+#define VX_MAX_IMPLEMENTATION_NAME 64
+
+/* ------------------------------------------------------------------ scalar types ---------- */
+typedef char vx_char;
+typedef uint8_t vx_uint8;
+typedef uint16_t vx_uint16;
+typedef uint32_t vx_uint32;
+typedef uint64_t vx_uint64;
+typedef int8_t vx_int8;
+typedef int16_t vx_int16;
+typedef int32_t vx_int32;
+typedef int64_t vx_int64;
+typedef float vx_float32;
+typedef double vx_float64;
+typedef int32_t vx_enum;
+typedef size_t vx_size;
+typedef uint32_t vx_df_image;
+typedef uintptr_t vx_map_id;
+typedef vx_enum vx_status;
+typedef enum _vx_bool_e { vx_false_e = 0, vx_true_e } vx_bool;
+
+/* ------------------------------------------------------------------ opaque handles -------- */
+typedef struct _vx_reference *vx_reference;
+typedef struct _vx_context *vx_context;
+typedef struct _vx_graph *vx_graph;
+typedef struct _vx_node *vx_node;
+typedef struct _vx_kernel *vx_kernel;
+typedef struct _vx_parameter *vx_parameter;
+typedef struct _vx_image *vx_image;
+typedef struct _vx_scalar *vx_scalar;
+typedef struct _vx_threshold *vx_threshold;
+typedef struct _vx_array *vx_array;
+typedef struct _vx_pyramid *vx_pyramid;
+typedef struct _vx_matrix *vx_matrix;
+typedef struct _vx_remap *vx_remap;
+
+/* ------------------------------------------------------------------ enumeration bases ----- */
+#define VX_ID_KHRONOS 0x000
+#define VX_ID_AMD 0x00D
+#define VX_LIBRARY_KHR_BASE 0x0
+#define VX_ATTRIBUTE_BASE(vendor, object) (((vendor) << 20) | ((object) << 8))
+#define VX_KERNEL_BASE(vendor, lib) (((vendor) << 20) | ((lib) << 12))
+#define VX_ENUM_BASE(vendor, id) (((vendor) << 20) | ((id) << 12))
+#define VX_DF_IMAGE(a, b, c, d) ((a) | ((b) << 8) | ((c) << 16) | ((d) << 24))
+#define VX_FMT_REF "%p"
+#define VX_FMT_SIZE "%zu"
+#define VX_SCALE_UNITY (1024u)
+#define VX_SCALE_PYRAMID_HALF (0.5f)
+#define VX_SCALE_PYRAMID_ORB ((vx_float32)0.8408964f)
+
+enum vx_status_e {
+    VX_STATUS_MIN = -25,
+    VX_ERROR_REFERENCE_NONZERO = -24,
+    VX_ERROR_MULTIPLE_WRITERS = -23,
+    VX_ERROR_GRAPH_ABANDONED = -22,
+    VX_ERROR_GRAPH_SCHEDULED = -21,
+    VX_ERROR_INVALID_SCOPE = -20,
+    VX_ERROR_INVALID_NODE = -19,
+    VX_ERROR_INVALID_GRAPH = -18,
+    VX_ERROR_INVALID_TYPE = -17,
+    VX_ERROR_INVALID_VALUE = -16,
+    VX_ERROR_INVALID_DIMENSION = -15,
+    VX_ERROR_INVALID_FORMAT = -14,
+    VX_ERROR_INVALID_LINK = -13,
+    VX_ERROR_INVALID_REFERENCE = -12,
+    VX_ERROR_INVALID_MODULE = -11,
+    VX_ERROR_INVALID_PARAMETERS = -10,
+    VX_ERROR_OPTIMIZED_AWAY = -9,
+    VX_ERROR_NO_MEMORY = -8,
+    VX_ERROR_NO_RESOURCES = -7,
+    VX_ERROR_NOT_COMPATIBLE = -6,
+    VX_ERROR_NOT_ALLOCATED = -5,
+    VX_ERROR_NOT_SUFFICIENT = -4,
+    VX_ERROR_NOT_SUPPORTED = -3,
+    VX_ERROR_NOT_IMPLEMENTED = -2,
+    VX_FAILURE = -1,
+    VX_SUCCESS = 0,
+};
+
+enum vx_type_e {
+    VX_TYPE_INVALID = 0x000,
+    VX_TYPE_CHAR = 0x001,
+    VX_TYPE_INT8 = 0x002,
+    VX_TYPE_UINT8 = 0x003,
+    VX_TYPE_INT16 = 0x004,
+    VX_TYPE_UINT16 = 0x005,
+    VX_TYPE_INT32 = 0x006,
+    VX_TYPE_UINT32 = 0x007,
+    VX_TYPE_INT64 = 0x008,
+    VX_TYPE_UINT64 = 0x009,
+    VX_TYPE_FLOAT32 = 0x00A,
+    VX_TYPE_FLOAT64 = 0x00B,
+    VX_TYPE_ENUM = 0x00C,
+    VX_TYPE_SIZE = 0x00D,
+    VX_TYPE_DF_IMAGE = 0x00E,
+    VX_TYPE_BOOL = 0x010,
+    VX_TYPE_RECTANGLE = 0x020,
+    VX_TYPE_KEYPOINT = 0x021,
+    VX_TYPE_COORDINATES2D = 0x022,
+    VX_TYPE_COORDINATES3D = 0x023,
+    VX_TYPE_COORDINATES2DF = 0x024,
+    VX_TYPE_REFERENCE = 0x800,
+    VX_TYPE_CONTEXT = 0x801,
+    VX_TYPE_GRAPH = 0x802,
+    VX_TYPE_NODE = 0x803,
+    VX_TYPE_KERNEL = 0x804,
+    VX_TYPE_PARAMETER = 0x805,
+    VX_TYPE_DELAY = 0x806,
+    VX_TYPE_LUT = 0x807,
+    VX_TYPE_DISTRIBUTION = 0x808,
+    VX_TYPE_PYRAMID = 0x809,
+    VX_TYPE_THRESHOLD = 0x80A,
+    VX_TYPE_MATRIX = 0x80B,
+    VX_TYPE_CONVOLUTION = 0x80C,
+    VX_TYPE_SCALAR = 0x80D,
+    VX_TYPE_ARRAY = 0x80E,
+    VX_TYPE_IMAGE = 0x80F,
+    VX_TYPE_REMAP = 0x810,
+    VX_TYPE_ERROR = 0x811,
+    VX_TYPE_META_FORMAT = 0x812,
+    VX_TYPE_OBJECT_ARRAY = 0x813,
+    VX_TYPE_TENSOR = 0x815,
+};
+
+enum vx_enum_e {
+    VX_ENUM_DIRECTION = 0x00,
+    VX_ENUM_INTERPOLATION = 0x04,
+    VX_ENUM_COLOR_SPACE = 0x06,
+    VX_ENUM_COLOR_RANGE = 0x07,
+    VX_ENUM_PARAMETER_STATE = 0x08,
+    VX_ENUM_CHANNEL = 0x09,
+    VX_ENUM_THRESHOLD_TYPE = 0x0B,
+    VX_ENUM_BORDER = 0x0C,
+    VX_ENUM_MEMORY_TYPE = 0x0E,
+    VX_ENUM_TERM_CRITERIA = 0x0F,
+    VX_ENUM_NORM_TYPE = 0x10,
+    VX_ENUM_ACCESSOR = 0x11,
+    VX_ENUM_BORDER_POLICY = 0x14,
+    VX_ENUM_GRAPH_STATE = 0x15,
+};
+
+enum vx_df_image_e {
+    VX_DF_IMAGE_VIRT = VX_DF_IMAGE('V', 'I', 'R', 'T'),
+    VX_DF_IMAGE_RGB = VX_DF_IMAGE('R', 'G', 'B', '2'),
+    VX_DF_IMAGE_RGBX = VX_DF_IMAGE('R', 'G', 'B', 'A'),
+    VX_DF_IMAGE_NV12 = VX_DF_IMAGE('N', 'V', '1', '2'),
+    VX_DF_IMAGE_NV21 = VX_DF_IMAGE('N', 'V', '2', '1'),
+    VX_DF_IMAGE_UYVY = VX_DF_IMAGE('U', 'Y', 'V', 'Y'),
+    VX_DF_IMAGE_YUYV = VX_DF_IMAGE('Y', 'U', 'Y', 'V'),
+    VX_DF_IMAGE_IYUV = VX_DF_IMAGE('I', 'Y', 'U', 'V'),
+    VX_DF_IMAGE_YUV4 = VX_DF_IMAGE('Y', 'U', 'V', '4'),
+    VX_DF_IMAGE_U8 = VX_DF_IMAGE('U', '0', '0', '8'),
+    VX_DF_IMAGE_U16 = VX_DF_IMAGE('U', '0', '1', '6'),
+    VX_DF_IMAGE_S16 = VX_DF_IMAGE('S', '0', '1', '6'),
+    VX_DF_IMAGE_U32 = VX_DF_IMAGE('U', '0', '3', '2'),
+    VX_DF_IMAGE_S32 = VX_DF_IMAGE('S', '0', '3', '2'),
+};
+
+enum vx_direction_e {
+    VX_INPUT = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_DIRECTION) + 0x0,
+    VX_OUTPUT = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_DIRECTION) + 0x1,
+    VX_BIDIRECTIONAL = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_DIRECTION) + 0x2,
+};
+
+enum vx_parameter_state_e {
+    VX_PARAMETER_STATE_REQUIRED = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_PARAMETER_STATE) + 0x0,
+    VX_PARAMETER_STATE_OPTIONAL = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_PARAMETER_STATE) + 0x1,
+};
+
+enum vx_interpolation_type_e {
+    VX_INTERPOLATION_NEAREST_NEIGHBOR = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_INTERPOLATION) + 0x0,
+    VX_INTERPOLATION_BILINEAR = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_INTERPOLATION) + 0x1,
+    VX_INTERPOLATION_AREA = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_INTERPOLATION) + 0x2,
+};
+
+enum vx_color_space_e {
+    VX_COLOR_SPACE_NONE = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_COLOR_SPACE) + 0x0,
+    VX_COLOR_SPACE_BT601_525 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_COLOR_SPACE) + 0x1,
+    VX_COLOR_SPACE_BT601_625 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_COLOR_SPACE) + 0x2,
+    VX_COLOR_SPACE_BT709 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_COLOR_SPACE) + 0x3,
+    VX_COLOR_SPACE_DEFAULT = VX_COLOR_SPACE_BT709,
+};
+
+enum vx_channel_range_e {
+    VX_CHANNEL_RANGE_FULL = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_COLOR_RANGE) + 0x0,
+    VX_CHANNEL_RANGE_RESTRICTED = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_COLOR_RANGE) + 0x1,
+};
+
+enum vx_channel_e {
+    VX_CHANNEL_0 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x0,
+    VX_CHANNEL_1 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x1,
+    VX_CHANNEL_2 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x2,
+    VX_CHANNEL_3 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x3,
+    VX_CHANNEL_R = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x10,
+    VX_CHANNEL_G = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x11,
+    VX_CHANNEL_B = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x12,
+    VX_CHANNEL_A = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x13,
+    VX_CHANNEL_Y = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x14,
+    VX_CHANNEL_U = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x15,
+    VX_CHANNEL_V = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_CHANNEL) + 0x16,
+};
+
+enum vx_threshold_type_e {
+    VX_THRESHOLD_TYPE_BINARY = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_THRESHOLD_TYPE) + 0x0,
+    VX_THRESHOLD_TYPE_RANGE = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_THRESHOLD_TYPE) + 0x1,
+};
+
+enum vx_border_e {
+    VX_BORDER_UNDEFINED = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_BORDER) + 0x0,
+    VX_BORDER_CONSTANT = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_BORDER) + 0x1,
+    VX_BORDER_REPLICATE = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_BORDER) + 0x2,
+};
+#define VX_BORDER_MODE_UNDEFINED VX_BORDER_UNDEFINED
+#define VX_BORDER_MODE_CONSTANT VX_BORDER_CONSTANT
+#define VX_BORDER_MODE_REPLICATE VX_BORDER_REPLICATE
+
+enum vx_memory_type_e {
+    VX_MEMORY_TYPE_NONE = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_MEMORY_TYPE) + 0x0,
+    VX_MEMORY_TYPE_HOST = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_MEMORY_TYPE) + 0x1,
+};
+
+enum vx_termination_criteria_e {
+    VX_TERM_CRITERIA_ITERATIONS = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_TERM_CRITERIA) + 0x0,
+    VX_TERM_CRITERIA_EPSILON = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_TERM_CRITERIA) + 0x1,
+    VX_TERM_CRITERIA_BOTH = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_TERM_CRITERIA) + 0x2,
+};
+
+enum vx_norm_type_e {
+    VX_NORM_L1 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_NORM_TYPE) + 0x0,
+    VX_NORM_L2 = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_NORM_TYPE) + 0x1,
+};
+
+enum vx_accessor_e {
+    VX_READ_ONLY = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_ACCESSOR) + 0x1,
+    VX_WRITE_ONLY = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_ACCESSOR) + 0x2,
+    VX_READ_AND_WRITE = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_ACCESSOR) + 0x3,
+};
+
+enum vx_map_flag_e { VX_NOGAP_X = 1 };
+
+enum vx_graph_state_e {
+    VX_GRAPH_STATE_UNVERIFIED = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_GRAPH_STATE) + 0x0,
+    VX_GRAPH_STATE_VERIFIED = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_GRAPH_STATE) + 0x1,
+    VX_GRAPH_STATE_RUNNING = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_GRAPH_STATE) + 0x2,
+    VX_GRAPH_STATE_ABANDONED = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_GRAPH_STATE) + 0x3,
+    VX_GRAPH_STATE_COMPLETED = VX_ENUM_BASE(VX_ID_KHRONOS, VX_ENUM_GRAPH_STATE) + 0x4,
+};
+
+/* ------------------------------------------------------------------ Khronos kernel ids ---- */
+enum vx_kernel_e {
+    VX_KERNEL_COLOR_CONVERT = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x1,
+    VX_KERNEL_CHANNEL_EXTRACT = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x2,
+    VX_KERNEL_SOBEL_3x3 = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x4,
+    VX_KERNEL_MAGNITUDE = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x5,
+    VX_KERNEL_PHASE = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x6,
+    VX_KERNEL_SCALE_IMAGE = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x7,
+    VX_KERNEL_MEDIAN_3x3 = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x11,
+    VX_KERNEL_BOX_3x3 = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x12,
+    VX_KERNEL_GAUSSIAN_3x3 = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x13,
+    VX_KERNEL_GAUSSIAN_PYRAMID = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x15,
+    VX_KERNEL_CANNY_EDGE_DETECTOR = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x1B,
+    VX_KERNEL_WARP_AFFINE = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x23,
+    VX_KERNEL_WARP_PERSPECTIVE = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x24,
+    VX_KERNEL_HARRIS_CORNERS = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x25,
+    VX_KERNEL_OPTICAL_FLOW_PYR_LK = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x27,
+    VX_KERNEL_REMAP = VX_KERNEL_BASE(VX_ID_KHRONOS, VX_LIBRARY_KHR_BASE) + 0x28,
+};
+
+/* ------------------------------------------------------------------ attributes ------------ */
+enum vx_reference_attribute_e {
+    VX_REFERENCE_COUNT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_REFERENCE) + 0x0,
+    VX_REFERENCE_TYPE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_REFERENCE) + 0x1,
+    VX_REFERENCE_NAME = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_REFERENCE) + 0x2,
+};
+
+enum vx_context_attribute_e {
+    VX_CONTEXT_VENDOR_ID = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0x0,
+    VX_CONTEXT_VERSION = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0x1,
+    VX_CONTEXT_UNIQUE_KERNELS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0x2,
+    VX_CONTEXT_MODULES = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0x3,
+    VX_CONTEXT_REFERENCES = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0x4,
+    VX_CONTEXT_IMPLEMENTATION = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0x5,
+    VX_CONTEXT_OPTICAL_FLOW_MAX_WINDOW_DIMENSION = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0x9,
+    VX_CONTEXT_IMMEDIATE_BORDER = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_CONTEXT) + 0xA,
+};
+
+enum vx_kernel_attribute_e {
+    VX_KERNEL_PARAMETERS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_KERNEL) + 0x0,
+    VX_KERNEL_NAME = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_KERNEL) + 0x1,
+    VX_KERNEL_ENUM = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_KERNEL) + 0x2,
+    VX_KERNEL_LOCAL_DATA_SIZE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_KERNEL) + 0x3,
+};
+
+enum vx_graph_attribute_e {
+    VX_GRAPH_NUMNODES = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_GRAPH) + 0x0,
+    VX_GRAPH_PERFORMANCE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_GRAPH) + 0x2,
+    VX_GRAPH_NUMPARAMETERS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_GRAPH) + 0x3,
+    VX_GRAPH_STATE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_GRAPH) + 0x4,
+};
+
+enum vx_node_attribute_e {
+    VX_NODE_STATUS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x0,
+    VX_NODE_PERFORMANCE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x1,
+    VX_NODE_BORDER = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x2,
+    VX_NODE_LOCAL_DATA_SIZE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x3,
+    VX_NODE_PARAMETERS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x5,
+};
+#define VX_NODE_ATTRIBUTE_BORDER_MODE VX_NODE_BORDER
+#define VX_NODE_ATTRIBUTE_STATUS VX_NODE_STATUS
+
+enum vx_parameter_attribute_e {
+    VX_PARAMETER_INDEX = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PARAMETER) + 0x0,
+    VX_PARAMETER_DIRECTION = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PARAMETER) + 0x1,
+    VX_PARAMETER_TYPE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PARAMETER) + 0x2,
+    VX_PARAMETER_STATE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PARAMETER) + 0x3,
+    VX_PARAMETER_REF = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PARAMETER) + 0x4,
+};
+
+enum vx_image_attribute_e {
+    VX_IMAGE_WIDTH = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x0,
+    VX_IMAGE_HEIGHT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x1,
+    VX_IMAGE_FORMAT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x2,
+    VX_IMAGE_PLANES = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x3,
+    VX_IMAGE_SPACE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x4,
+    VX_IMAGE_RANGE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x5,
+    VX_IMAGE_MEMORY_TYPE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x7,
+    VX_IMAGE_IS_UNIFORM = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x8,
+    VX_IMAGE_UNIFORM_VALUE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_IMAGE) + 0x9,
+};
+
+enum vx_scalar_attribute_e {
+    VX_SCALAR_TYPE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_SCALAR) + 0x0,
+};
+
+enum vx_threshold_attribute_e {
+    VX_THRESHOLD_TYPE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x0,
+    VX_THRESHOLD_INPUT_FORMAT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x7,
+    VX_THRESHOLD_OUTPUT_FORMAT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x8,
+};
+/* OpenVX 1.0/1.1 threshold attributes, still accepted by vxSet/QueryThresholdAttribute */
+#define VX_THRESHOLD_THRESHOLD_VALUE (VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x1)
+#define VX_THRESHOLD_THRESHOLD_LOWER (VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x2)
+#define VX_THRESHOLD_THRESHOLD_UPPER (VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x3)
+#define VX_THRESHOLD_TRUE_VALUE (VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x4)
+#define VX_THRESHOLD_FALSE_VALUE (VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x5)
+#define VX_THRESHOLD_DATA_TYPE (VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_THRESHOLD) + 0x6)
+
+enum vx_matrix_attribute_e {
+    VX_MATRIX_TYPE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_MATRIX) + 0x0,
+    VX_MATRIX_ROWS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_MATRIX) + 0x1,
+    VX_MATRIX_COLUMNS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_MATRIX) + 0x2,
+    VX_MATRIX_SIZE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_MATRIX) + 0x3,
+};
+
+enum vx_pyramid_attribute_e {
+    VX_PYRAMID_LEVELS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PYRAMID) + 0x0,
+    VX_PYRAMID_SCALE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PYRAMID) + 0x1,
+    VX_PYRAMID_WIDTH = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PYRAMID) + 0x2,
+    VX_PYRAMID_HEIGHT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PYRAMID) + 0x3,
+    VX_PYRAMID_FORMAT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_PYRAMID) + 0x4,
+};
+
+enum vx_remap_attribute_e {
+    VX_REMAP_SOURCE_WIDTH = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_REMAP) + 0x0,
+    VX_REMAP_SOURCE_HEIGHT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_REMAP) + 0x1,
+    VX_REMAP_DESTINATION_WIDTH = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_REMAP) + 0x2,
+    VX_REMAP_DESTINATION_HEIGHT = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_REMAP) + 0x3,
+};
+
+enum vx_array_attribute_e {
+    VX_ARRAY_ITEMTYPE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_ARRAY) + 0x0,
+    VX_ARRAY_NUMITEMS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_ARRAY) + 0x1,
+    VX_ARRAY_CAPACITY = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_ARRAY) + 0x2,
+    VX_ARRAY_ITEMSIZE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_ARRAY) + 0x3,
+};
+
+/* ------------------------------------------------------------------ structures ------------ */
+typedef struct _vx_imagepatch_addressing_t {
+    vx_uint32 dim_x;
+    vx_uint32 dim_y;
+    vx_int32 stride_x;
+    vx_int32 stride_y;
+    vx_uint32 scale_x;
+    vx_uint32 scale_y;
+    vx_uint32 step_x;
+    vx_uint16 step_y;
+    vx_uint16 stride_x_bits;
+} vx_imagepatch_addressing_t;
+
+typedef struct _vx_perf_t {
+    vx_uint64 tmp;
+    vx_uint64 beg;
+    vx_uint64 end;
+    vx_uint64 sum;
+    vx_uint64 avg;
+    vx_uint64 min;
+    vx_uint64 num;
+    vx_uint64 max;
+} vx_perf_t;
+
+typedef struct _vx_rectangle_t {
+    vx_uint32 start_x;
+    vx_uint32 start_y;
+    vx_uint32 end_x;
+    vx_uint32 end_y;
+} vx_rectangle_t;
+
+typedef struct _vx_keypoint_t {
+    vx_int32 x;
+    vx_int32 y;
+    vx_float32 strength;
+    vx_float32 scale;
+    vx_float32 orientation;
+    vx_int32 tracking_status;
+    vx_float32 error;
+} vx_keypoint_t;
+
+typedef struct _vx_coordinates2d_t {
+    vx_uint32 x;
+    vx_uint32 y;
+} vx_coordinates2d_t;
+
+typedef struct _vx_coordinates2df_t {
+    vx_float32 x;
+    vx_float32 y;
+} vx_coordinates2df_t;
+
+typedef union _vx_pixel_value_t {
+    vx_uint8 RGB[3];
+    vx_uint8 RGBX[4];
+    vx_uint8 YUV[3];
+    vx_bool U1;
+    vx_uint8 U8;
+    vx_uint16 U16;
+    vx_int16 S16;
+    vx_uint32 U32;
+    vx_int32 S32;
+    vx_uint8 reserved[16];
+} vx_pixel_value_t;
+
+typedef struct _vx_border_t {
+    vx_enum mode;
+    vx_pixel_value_t constant_value;
+} vx_border_t;
+
+/* ------------------------------------------------------------------ context / reference --- */
+VX_API_ENTRY vx_context VX_API_CALL vxCreateContext(void);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseContext(vx_context *context);
+VX_API_ENTRY vx_context VX_API_CALL vxGetContext(vx_reference reference);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryContext(vx_context context, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxSetContextAttribute(vx_context context, vx_enum attribute, const void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxGetStatus(vx_reference reference);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryReference(vx_reference ref, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseReference(vx_reference *ref_ptr);
+VX_API_ENTRY vx_status VX_API_CALL vxRetainReference(vx_reference ref);
+VX_API_ENTRY vx_status VX_API_CALL vxSetReferenceName(vx_reference ref, const vx_char *name);
+VX_API_ENTRY vx_status VX_API_CALL vxLoadKernels(vx_context context, const vx_char *module);
+VX_API_ENTRY vx_kernel VX_API_CALL vxGetKernelByName(vx_context context, const vx_char *name);
+VX_API_ENTRY vx_kernel VX_API_CALL vxGetKernelByEnum(vx_context context, vx_enum kernel_enum);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryKernel(vx_kernel kernel, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseKernel(vx_kernel *kernel);
+VX_API_ENTRY vx_parameter VX_API_CALL vxGetKernelParameterByIndex(vx_kernel kernel, vx_uint32 index);
+VX_API_ENTRY vx_parameter VX_API_CALL vxGetParameterByIndex(vx_node node, vx_uint32 index);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryParameter(vx_parameter parameter, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseParameter(vx_parameter *param);
+VX_API_ENTRY vx_status VX_API_CALL vxSetParameterByIndex(vx_node node, vx_uint32 index, vx_reference value);
+
+typedef void(VX_CALLBACK *vx_log_callback_f)(vx_context context, vx_reference ref, vx_status status, const vx_char string[]);
+VX_API_ENTRY void VX_API_CALL vxRegisterLogCallback(vx_context context, vx_log_callback_f callback, vx_bool reentrant);
+
+/* ------------------------------------------------------------------ images ---------------- */
+VX_API_ENTRY vx_image VX_API_CALL vxCreateImage(vx_context context, vx_uint32 width, vx_uint32 height, vx_df_image color);
+VX_API_ENTRY vx_image VX_API_CALL vxCreateVirtualImage(vx_graph graph, vx_uint32 width, vx_uint32 height, vx_df_image color);
+VX_API_ENTRY vx_image VX_API_CALL vxCreateUniformImage(vx_context context, vx_uint32 width, vx_uint32 height, vx_df_image color, const vx_pixel_value_t *value);
+VX_API_ENTRY vx_image VX_API_CALL vxCreateImageFromHandle(vx_context context, vx_df_image color, const vx_imagepatch_addressing_t addrs[], void *const ptrs[], vx_enum memory_type);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryImage(vx_image image, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxSetImageAttribute(vx_image image, vx_enum attribute, const void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseImage(vx_image *image);
+VX_API_ENTRY vx_status VX_API_CALL vxCopyImagePatch(vx_image image, const vx_rectangle_t *image_rect, vx_uint32 image_plane_index,
+                                                    const vx_imagepatch_addressing_t *user_addr, void *user_ptr, vx_enum usage,
+                                                    vx_enum user_mem_type);
+VX_API_ENTRY vx_status VX_API_CALL vxMapImagePatch(vx_image image, const vx_rectangle_t *rect, vx_uint32 plane_index, vx_map_id *map_id,
+                                                   vx_imagepatch_addressing_t *addr, void **ptr, vx_enum usage, vx_enum mem_type,
+                                                   vx_uint32 flags);
+VX_API_ENTRY vx_status VX_API_CALL vxUnmapImagePatch(vx_image image, vx_map_id map_id);
+VX_API_ENTRY vx_status VX_API_CALL vxGetValidRegionImage(vx_image image, vx_rectangle_t *rect);
+
+/* ------------------------------------------------------------------ scalars --------------- */
+VX_API_ENTRY vx_scalar VX_API_CALL vxCreateScalar(vx_context context, vx_enum data_type, const void *ptr);
+VX_API_ENTRY vx_scalar VX_API_CALL vxCreateVirtualScalar(vx_graph graph, vx_enum data_type);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryScalar(vx_scalar scalar, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxCopyScalar(vx_scalar scalar, void *user_ptr, vx_enum usage, vx_enum user_mem_type);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseScalar(vx_scalar *scalar);
+
+/* ------------------------------------------------------------------ thresholds ------------ */
+VX_API_ENTRY vx_threshold VX_API_CALL vxCreateThreshold(vx_context c, vx_enum thresh_type, vx_enum data_type);
+VX_API_ENTRY vx_threshold VX_API_CALL vxCreateThresholdForImage(vx_context context, vx_enum thresh_type, vx_df_image input_format,
+                                                                vx_df_image output_format);
+VX_API_ENTRY vx_status VX_API_CALL vxSetThresholdAttribute(vx_threshold thresh, vx_enum attribute, const void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryThreshold(vx_threshold thresh, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxCopyThresholdValue(vx_threshold thresh, vx_pixel_value_t *value_ptr, vx_enum usage, vx_enum user_mem_type);
+VX_API_ENTRY vx_status VX_API_CALL vxCopyThresholdRange(vx_threshold thresh, vx_pixel_value_t *lower_value_ptr,
+                                                        vx_pixel_value_t *upper_value_ptr, vx_enum usage, vx_enum user_mem_type);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseThreshold(vx_threshold *thresh);
+
+/* ------------------------------------------------------------------ arrays ---------------- */
+VX_API_ENTRY vx_array VX_API_CALL vxCreateArray(vx_context context, vx_enum item_type, vx_size capacity);
+VX_API_ENTRY vx_array VX_API_CALL vxCreateVirtualArray(vx_graph graph, vx_enum item_type, vx_size capacity);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryArray(vx_array arr, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxAddArrayItems(vx_array arr, vx_size count, const void *ptr, vx_size stride);
+VX_API_ENTRY vx_status VX_API_CALL vxTruncateArray(vx_array arr, vx_size new_num_items);
+VX_API_ENTRY vx_status VX_API_CALL vxCopyArrayRange(vx_array array, vx_size range_start, vx_size range_end, vx_size user_stride,
+                                                    void *user_ptr, vx_enum usage, vx_enum user_mem_type);
+VX_API_ENTRY vx_status VX_API_CALL vxMapArrayRange(vx_array array, vx_size range_start, vx_size range_end, vx_map_id *map_id,
+                                                   vx_size *stride, void **ptr, vx_enum usage, vx_enum mem_type, vx_uint32 flags);
+VX_API_ENTRY vx_status VX_API_CALL vxUnmapArrayRange(vx_array array, vx_map_id map_id);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseArray(vx_array *arr);
+
+/* ------------------------------------------------------------------ pyramids -------------- */
+VX_API_ENTRY vx_pyramid VX_API_CALL vxCreatePyramid(vx_context context, vx_size levels, vx_float32 scale, vx_uint32 width,
+                                                    vx_uint32 height, vx_df_image format);
+VX_API_ENTRY vx_pyramid VX_API_CALL vxCreateVirtualPyramid(vx_graph graph, vx_size levels, vx_float32 scale, vx_uint32 width,
+                                                           vx_uint32 height, vx_df_image format);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryPyramid(vx_pyramid pyr, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_image VX_API_CALL vxGetPyramidLevel(vx_pyramid pyr, vx_uint32 index);
+VX_API_ENTRY vx_status VX_API_CALL vxReleasePyramid(vx_pyramid *pyr);
+
+/* ------------------------------------------------------------------ matrices / remaps ----- */
+VX_API_ENTRY vx_matrix VX_API_CALL vxCreateMatrix(vx_context c, vx_enum data_type, vx_size columns, vx_size rows);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryMatrix(vx_matrix mat, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxCopyMatrix(vx_matrix matrix, void *user_ptr, vx_enum usage, vx_enum user_mem_type);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseMatrix(vx_matrix *mat);
+VX_API_ENTRY vx_remap VX_API_CALL vxCreateRemap(vx_context context, vx_uint32 src_width, vx_uint32 src_height, vx_uint32 dst_width,
+                                                vx_uint32 dst_height);
+VX_API_ENTRY vx_status VX_API_CALL vxSetRemapPoint(vx_remap table, vx_uint32 dst_x, vx_uint32 dst_y, vx_float32 src_x, vx_float32 src_y);
+VX_API_ENTRY vx_status VX_API_CALL vxGetRemapPoint(vx_remap table, vx_uint32 dst_x, vx_uint32 dst_y, vx_float32 *src_x, vx_float32 *src_y);
+VX_API_ENTRY vx_status VX_API_CALL vxCopyRemapPatch(vx_remap remap, const vx_rectangle_t *rect, vx_size user_stride_y, void *user_ptr,
+                                                    vx_enum user_coordinate_type, vx_enum usage, vx_enum user_mem_type);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryRemap(vx_remap table, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseRemap(vx_remap *table);
+
+/* ------------------------------------------------------------------ graphs and nodes ------ */
+VX_API_ENTRY vx_graph VX_API_CALL vxCreateGraph(vx_context context);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseGraph(vx_graph *graph);
+VX_API_ENTRY vx_status VX_API_CALL vxVerifyGraph(vx_graph graph);
+VX_API_ENTRY vx_status VX_API_CALL vxProcessGraph(vx_graph graph);
+VX_API_ENTRY vx_status VX_API_CALL vxScheduleGraph(vx_graph graph);
+VX_API_ENTRY vx_status VX_API_CALL vxWaitGraph(vx_graph graph);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryGraph(vx_graph graph, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxSetGraphAttribute(vx_graph graph, vx_enum attribute, const void *ptr, vx_size size);
+VX_API_ENTRY vx_bool VX_API_CALL vxIsGraphVerified(vx_graph graph);
+VX_API_ENTRY vx_node VX_API_CALL vxCreateGenericNode(vx_graph graph, vx_kernel kernel);
+VX_API_ENTRY vx_status VX_API_CALL vxQueryNode(vx_node node, vx_enum attribute, void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxSetNodeAttribute(vx_node node, vx_enum attribute, const void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxReleaseNode(vx_node *node);
+VX_API_ENTRY void VX_API_CALL vxRemoveNode(vx_node *node);
+
+/* ------------------------------------------------------------------ node constructors ----- */
+VX_API_ENTRY vx_node VX_API_CALL vxColorConvertNode(vx_graph graph, vx_image input, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxChannelExtractNode(vx_graph graph, vx_image input, vx_enum channel, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxSobel3x3Node(vx_graph graph, vx_image input, vx_image output_x, vx_image output_y);
+VX_API_ENTRY vx_node VX_API_CALL vxMagnitudeNode(vx_graph graph, vx_image grad_x, vx_image grad_y, vx_image mag);
+VX_API_ENTRY vx_node VX_API_CALL vxPhaseNode(vx_graph graph, vx_image grad_x, vx_image grad_y, vx_image orientation);
+VX_API_ENTRY vx_node VX_API_CALL vxScaleImageNode(vx_graph graph, vx_image src, vx_image dst, vx_enum type);
+VX_API_ENTRY vx_node VX_API_CALL vxBox3x3Node(vx_graph graph, vx_image input, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxGaussian3x3Node(vx_graph graph, vx_image input, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxMedian3x3Node(vx_graph graph, vx_image input, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxGaussianPyramidNode(vx_graph graph, vx_image input, vx_pyramid gaussian);
+VX_API_ENTRY vx_node VX_API_CALL vxCannyEdgeDetectorNode(vx_graph graph, vx_image input, vx_threshold hyst, vx_int32 gradient_size,
+                                                         vx_enum norm_type, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxHarrisCornersNode(vx_graph graph, vx_image input, vx_scalar strength_thresh, vx_scalar min_distance,
+                                                     vx_scalar sensitivity, vx_int32 gradient_size, vx_int32 block_size,
+                                                     vx_array corners, vx_scalar num_corners);
+VX_API_ENTRY vx_node VX_API_CALL vxWarpAffineNode(vx_graph graph, vx_image input, vx_matrix matrix, vx_enum type, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxWarpPerspectiveNode(vx_graph graph, vx_image input, vx_matrix matrix, vx_enum type, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxRemapNode(vx_graph graph, vx_image input, vx_remap table, vx_enum policy, vx_image output);
+VX_API_ENTRY vx_node VX_API_CALL vxOpticalFlowPyrLKNode(vx_graph graph, vx_pyramid old_images, vx_pyramid new_images, vx_array old_points,
+                                                        vx_array new_points_estimates, vx_array new_points, vx_enum termination,
+                                                        vx_scalar epsilon, vx_scalar num_iterations, vx_scalar use_initial_estimate,
+                                                        vx_size window_dimension);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,42 @@
+/*
+ * VX/vxu.h -- OpenVX immediate-mode functions for the engine's node set (Khronos OpenVX 1.3
+ * signatures; the reference implements them in amd_openvx/openvx/api/vxu.cpp).  Each call builds a
+ * one-node graph, verifies it, runs it on the GPU and waits for it.
+ */
+#ifndef _OPENVX_UTILITY_H_
+#define _OPENVX_UTILITY_H_
+
+#include <VX/vx.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+VX_API_ENTRY vx_status VX_API_CALL vxuColorConvert(vx_context context, vx_image input, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuChannelExtract(vx_context context, vx_image input, vx_enum channel, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuSobel3x3(vx_context context, vx_image input, vx_image output_x, vx_image output_y);
+VX_API_ENTRY vx_status VX_API_CALL vxuMagnitude(vx_context context, vx_image grad_x, vx_image grad_y, vx_image mag);
+VX_API_ENTRY vx_status VX_API_CALL vxuPhase(vx_context context, vx_image grad_x, vx_image grad_y, vx_image orientation);
+VX_API_ENTRY vx_status VX_API_CALL vxuScaleImage(vx_context context, vx_image src, vx_image dst, vx_enum type);
+VX_API_ENTRY vx_status VX_API_CALL vxuBox3x3(vx_context context, vx_image input, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuGaussian3x3(vx_context context, vx_image input, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuMedian3x3(vx_context context, vx_image input, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuGaussianPyramid(vx_context context, vx_image input, vx_pyramid gaussian);
+VX_API_ENTRY vx_status VX_API_CALL vxuCannyEdgeDetector(vx_context context, vx_image input, vx_threshold hyst, vx_int32 gradient_size,
+                                                        vx_enum norm_type, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuHarrisCorners(vx_context context, vx_image input, vx_scalar strength_thresh, vx_scalar min_distance,
+                                                    vx_scalar sensitivity, vx_int32 gradient_size, vx_int32 block_size,
+                                                    vx_array corners, vx_scalar num_corners);
+VX_API_ENTRY vx_status VX_API_CALL vxuWarpAffine(vx_context context, vx_image input, vx_matrix matrix, vx_enum type, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuWarpPerspective(vx_context context, vx_image input, vx_matrix matrix, vx_enum type, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuRemap(vx_context context, vx_image input, vx_remap table, vx_enum policy, vx_image output);
+VX_API_ENTRY vx_status VX_API_CALL vxuOpticalFlowPyrLK(vx_context context, vx_pyramid old_images, vx_pyramid new_images, vx_array old_points,
+                                                       vx_array new_points_estimates, vx_array new_points, vx_enum termination,
+                                                       vx_scalar epsilon, vx_scalar num_iterations, vx_scalar use_initial_estimate,
+                                                       vx_size window_dimension);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

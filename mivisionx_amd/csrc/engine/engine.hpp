@@ -1,0 +1,260 @@
+// engine.hpp -- internal objects of the MI355X OpenVX engine (libmvx_openvx.so).
+//
+// The engine is the host side of the vision-node path: the vx*/vxu* API, the kernel table with its
+// validate / initialize / execute commands and target affinity, graph verification and execution,
+// and the GDF text import.  It mirrors the reference AGO engine's object model
+// (amd_openvx/openvx/ago/ago_internal.h: AgoData, AgoNode, AgoGraph, AgoKernel) with one
+// difference that shapes everything else: every image, array and scalar that a node reads or
+// writes lives in HBM, and every kernel runs on the GPU.  Host access (map / copy / query) is a
+// synchronous copy, graph execution enqueues on one HIP stream per graph and only vxProcessGraph /
+// vxWaitGraph synchronise.
+#pragma once
+
+#include <VX/vx.h>
+#include <VX/vxu.h>
+#include <vx_ext_amd.h>
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mvx_kernels.h"
+
+namespace mvxe {
+
+constexpr uint32_t MAGIC = 0x4558564du;  // "MVXE"
+constexpr size_t GUARD = 64;              // zero bytes before and after every device buffer
+constexpr uint32_t MAX_PARAMS = 12;
+
+inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
+inline size_t align256(size_t v) { return (v + 255u) & ~(size_t)255u; }
+
+void log(struct ::_vx_reference *ref, vx_status status, const char *fmt, ...);
+
+// device memory owned by the engine: zero-filled, GUARD zero bytes on both sides
+struct DevBuf {
+    uint8_t *base = nullptr;  // hipMalloc'd
+    size_t size = 0;          // usable bytes (after the front guard)
+    uint8_t *ptr() const { return base ? base + GUARD : nullptr; }
+    vx_status alloc(size_t bytes);
+    void release();
+};
+
+struct ParamSpec {
+    vx_enum dir;    // VX_INPUT / VX_OUTPUT / VX_BIDIRECTIONAL
+    vx_enum type;   // VX_TYPE_*
+    bool optional;
+};
+
+struct Node;
+using ValidateFn = vx_status (*)(::_vx_node *node);
+using InitFn = vx_status (*)(::_vx_node *node);
+using ExecFn = vx_status (*)(::_vx_node *node, hipStream_t stream);
+
+// one row of the kernel table (the role of ago_kernel_list.cpp's entries and the command
+// protocol of agoKernel_* functions: validate fills output meta data, initialize sizes the node's
+// device workspace, execute enqueues on the graph's stream and never synchronises)
+struct KernelDef {
+    vx_enum id;
+    const char *name;
+    std::vector<ParamSpec> params;
+    ValidateFn validate;
+    InitFn initialize;  // may be null (no workspace)
+    ExecFn execute;
+    uint32_t targets;   // AGO_TARGET_AFFINITY_GPU (this engine has no CPU target)
+};
+
+const std::vector<KernelDef> &kernel_table();
+
+}  // namespace mvxe
+
+struct _vx_reference {
+    uint32_t magic = mvxe::MAGIC;
+    vx_enum type;
+    struct _vx_context *context;
+    struct _vx_graph *scope = nullptr;  // owning graph of virtual objects
+    std::atomic<int> extCount{1};
+    std::atomic<int> intCount{0};
+    bool isVirtual = false;
+    std::string name;
+    _vx_reference(vx_enum t, _vx_context *c) : type(t), context(c) {}
+    virtual ~_vx_reference() {}
+};
+
+struct _vx_context : _vx_reference {
+    std::recursive_mutex lock;
+    std::vector<_vx_reference *> refs;  // every live object
+    AgoTargetAffinityInfo affinity{};
+    vx_border_t immediateBorder{VX_BORDER_UNDEFINED, {}};
+    vx_log_callback_f logCallback = nullptr;
+    int hipDevice = -1;
+    bool hipReady = false;
+    std::vector<struct _vx_kernel *> kernels;
+    _vx_context() : _vx_reference(VX_TYPE_CONTEXT, nullptr) { context = this; }
+    vx_status ensure_hip();
+};
+
+struct ImagePlane {
+    uint32_t width, height, bpp, stride;
+    size_t offset;  // from the image's device base
+};
+
+struct ImageMap {
+    vx_map_id id;
+    uint32_t plane;
+    vx_rectangle_t rect;
+    vx_enum usage;
+    std::vector<uint8_t> host;
+    vx_imagepatch_addressing_t addr;
+};
+
+struct _vx_image : _vx_reference {
+    uint32_t width = 0, height = 0;
+    vx_df_image format = VX_DF_IMAGE_VIRT;
+    vx_enum space = VX_COLOR_SPACE_BT709, range = VX_CHANNEL_RANGE_FULL;
+    std::vector<ImagePlane> planes;
+    mvxe::DevBuf dev;
+    uint8_t *external = nullptr;  // VX_MEMORY_TYPE_HIP handle: caller-owned device memory
+    bool uniform = false;
+    vx_pixel_value_t uniformValue{};
+    struct _vx_pyramid *parent = nullptr;
+    std::vector<ImageMap> maps;
+    vx_map_id nextMap = 1;
+    _vx_image(_vx_context *c) : _vx_reference(VX_TYPE_IMAGE, c) {}
+    bool layout();                       // derive planes from width/height/format
+    vx_status allocate();                // device memory (idempotent)
+    uint8_t *plane_ptr(uint32_t p);      // device address of plane p (null before allocation)
+    size_t bytes() const;
+};
+
+struct _vx_scalar : _vx_reference {
+    vx_enum dataType;
+    union { vx_uint64 u64; vx_int32 i32; vx_uint32 u32; vx_float32 f32; vx_float64 f64; vx_enum e; vx_size sz;
+            vx_uint8 u8; vx_bool b; vx_df_image df; } value{};
+    mvxe::DevBuf dev;        // device shadow for scalars that kernels write (8 bytes)
+    bool deviceWritten = false;
+    _vx_scalar(_vx_context *c, vx_enum t) : _vx_reference(VX_TYPE_SCALAR, c), dataType(t) {}
+    size_t item_size() const;
+};
+
+struct _vx_threshold : _vx_reference {
+    vx_enum threshType;
+    vx_enum dataType = VX_TYPE_UINT8;
+    vx_df_image inputFormat = VX_DF_IMAGE_U8, outputFormat = VX_DF_IMAGE_U8;
+    vx_pixel_value_t value{}, lower{}, upper{}, trueValue{}, falseValue{};
+    _vx_threshold(_vx_context *c, vx_enum t) : _vx_reference(VX_TYPE_THRESHOLD, c), threshType(t) {}
+};
+
+struct _vx_array : _vx_reference {
+    vx_enum itemType;
+    vx_size itemSize, capacity;
+    vx_size numItems = 0;     // host copy of the device count, refreshed after graphs write it
+    mvxe::DevBuf dev;         // [count u32 | pad to 256 | items]
+    std::vector<uint8_t> mapHost;
+    vx_map_id mapId = 0;
+    vx_size mapStart = 0, mapEnd = 0;
+    vx_enum mapUsage = 0;
+    _vx_array(_vx_context *c, vx_enum t, vx_size isz, vx_size cap)
+        : _vx_reference(VX_TYPE_ARRAY, c), itemType(t), itemSize(isz), capacity(cap) {}
+    vx_status allocate();
+    uint32_t *count_ptr() { return (uint32_t *)dev.ptr(); }
+    uint8_t *items_ptr() { return dev.ptr() ? dev.ptr() + 256 : nullptr; }
+    vx_status pull_count();   // numItems <- device count (synchronous)
+    vx_status push_count();   // device count <- numItems (synchronous)
+};
+
+struct _vx_pyramid : _vx_reference {
+    vx_size levels;
+    vx_float32 scale;
+    uint32_t width, height;
+    vx_df_image format;
+    std::vector<_vx_image *> images;
+    _vx_pyramid(_vx_context *c) : _vx_reference(VX_TYPE_PYRAMID, c) {}
+    bool build_levels();
+};
+
+struct _vx_matrix : _vx_reference {
+    vx_enum dataType;
+    vx_size columns, rows;
+    std::vector<uint8_t> data;
+    _vx_matrix(_vx_context *c) : _vx_reference(VX_TYPE_MATRIX, c) {}
+};
+
+struct _vx_remap : _vx_reference {
+    uint32_t srcWidth, srcHeight, dstWidth, dstHeight;
+    std::vector<float> map;  // dst h x w x {x, y}
+    _vx_remap(_vx_context *c) : _vx_reference(VX_TYPE_REMAP, c) {}
+};
+
+struct _vx_kernel : _vx_reference {
+    const mvxe::KernelDef *def;
+    _vx_kernel(_vx_context *c, const mvxe::KernelDef *d) : _vx_reference(VX_TYPE_KERNEL, c), def(d) {}
+};
+
+struct _vx_parameter : _vx_reference {
+    _vx_kernel *kernel;
+    struct _vx_node *node;
+    uint32_t index;
+    _vx_parameter(_vx_context *c) : _vx_reference(VX_TYPE_PARAMETER, c) {}
+};
+
+struct _vx_node : _vx_reference {
+    const mvxe::KernelDef *kernel;
+    struct _vx_graph *graph;
+    std::vector<_vx_reference *> params;
+    vx_border_t border{VX_BORDER_UNDEFINED, {}};
+    AgoTargetAffinityInfo affinity{};
+    vx_status status = VX_SUCCESS;
+    vx_perf_t perf{};
+    mvxe::DevBuf local;       // node workspace sized by the kernel's initialize command
+    size_t localSize = 0;
+    uint32_t target = AGO_TARGET_AFFINITY_GPU;
+    bool fused = false;       // executed by the graph's fused supernode
+    _vx_node(_vx_context *c) : _vx_reference(VX_TYPE_NODE, c) {}
+};
+
+namespace mvxe {
+struct FusedVision;  // the RGB vision supernode (vx_graph.cpp)
+}
+
+struct _vx_graph : _vx_reference {
+    std::vector<_vx_node *> nodes;
+    std::vector<_vx_node *> order;  // topological execution order after verification
+    bool verified = false;
+    vx_enum state = VX_GRAPH_STATE_UNVERIFIED;
+    hipStream_t stream = nullptr;
+    AgoTargetAffinityInfo affinity{};
+    uint32_t optimizerFlags = 0;
+    vx_perf_t perf{};
+    mvxe::FusedVision *fused = nullptr;
+    int fusedAt = -1;                 // position in `order` where the supernode runs
+    std::vector<_vx_array *> writtenArrays;
+    std::vector<_vx_scalar *> writtenScalars;
+    // vxScheduleGraph worker
+    std::thread worker;
+    std::mutex runLock;
+    bool scheduled = false;
+    vx_status lastStatus = VX_SUCCESS;
+    _vx_graph(_vx_context *c) : _vx_reference(VX_TYPE_GRAPH, c) {}
+};
+
+namespace mvxe {
+bool is_ref(const void *p, vx_enum type = 0);
+void add_ref(_vx_context *ctx, _vx_reference *r);
+void release_ref(_vx_reference *r, bool external);
+_vx_image *image_of(_vx_reference *r);
+vx_status import_graph_text(_vx_graph *graph, const AgoGraphImportInfo *info);
+_vx_node *create_node(_vx_graph *graph, const KernelDef *def, std::initializer_list<_vx_reference *> params);
+_vx_scalar *make_scalar(_vx_context *ctx, vx_enum type, const void *value);
+vx_status verify_graph(_vx_graph *graph);
+vx_status execute_graph(_vx_graph *graph);
+void invalidate_graph(_vx_graph *graph);
+const char *type_name(vx_enum type);
+vx_enum enum_from_name(const char *name, bool *ok);
+vx_df_image format_from_name(const char *name);
+}  // namespace mvxe

@@ -1,0 +1,562 @@
+// vx_kernels.cpp -- the engine's kernel table: Khronos vision nodes and the AMD low-level kernels
+// that GDF files may name directly, each with its validate / initialize / execute commands.
+//
+// Table rows follow ago_kernel_list.cpp:160-520 (names "org.khronos.openvx.<name>" and
+// "com.amd.openvx.<Name>", kernel enums, argument directions and order); the command protocol is
+// that of the agoKernel_* / ovxKernel_* functions in ago_kernel_api.cpp (validate :ago_kernel_cmd_
+// validate, initialize :ago_kernel_cmd_initialize, hip_execute :ago_kernel_cmd_hip_execute).  The
+// Khronos nodes are split into AMD kernels the way the reference's graph divider does it
+// (ago_drama_divide.cpp, cited per node below), but inside one execute command: every kernel is a
+// gfx950 launch behind mvx_kernels.h on the graph's stream, including the Canny trace and the
+// Harris sort-and-pick that the reference runs on the CPU.
+#include <cmath>
+
+#include "engine.hpp"
+
+using namespace mvxe;
+
+namespace {
+
+constexpr vx_enum IN = VX_INPUT, OUT = VX_OUTPUT;
+
+// AMD low-level kernel enums (ago_kernels.h, enum vx_kernel_amd_e)
+enum : vx_enum {
+    AMD_MAGNITUDE_S16_S16S16 = 0xd0006f,
+    AMD_PHASE_U8_S16S16 = 0xd00070,
+    AMD_CHANNEL_COPY_U8_U8 = 0xd00071,
+    AMD_COLOR_CONVERT_Y_RGB = 0xd000a2,
+    AMD_BOX_U8_U8_3x3 = 0xd000b0,
+    AMD_MEDIAN_U8_U8_3x3 = 0xd000b3,
+    AMD_GAUSSIAN_U8_U8_3x3 = 0xd000b4,
+    AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
+    AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,
+    AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM = 0xd000da,
+    AMD_SCALE_IMAGE_U8_U8_BILINEAR = 0xd000f1,
+};
+
+_vx_image *img(_vx_node *n, int i) { return image_of(n->params[i]); }
+_vx_scalar *scl(_vx_node *n, int i) { return is_ref(n->params[i], VX_TYPE_SCALAR) ? (_vx_scalar *)n->params[i] : nullptr; }
+uint8_t *P(_vx_image *im, uint32_t plane = 0) { return im->plane_ptr(plane); }
+uint32_t S(_vx_image *im, uint32_t plane = 0) { return im->planes[plane].stride; }
+
+vx_status fail(_vx_node *n, vx_status s, const char *what)
+{
+    log(n, s, "ERROR: %s: %s\n", n->kernel->name, what);
+    return s;
+}
+
+// an output image: virtual images with unset size/format take the given meta data; others must
+// already have it (ago_kernel_api.cpp ValidateArguments_* helpers)
+vx_status set_output(_vx_node *n, int i, uint32_t w, uint32_t h, vx_df_image fmt)
+{
+    _vx_image *o = img(n, i);
+    if (!o) return fail(n, VX_ERROR_INVALID_PARAMETERS, "output is not an image");
+    if (o->isVirtual && (o->width == 0 || o->height == 0 || o->format == VX_DF_IMAGE_VIRT)) {
+        if (o->width == 0) o->width = w;
+        if (o->height == 0) o->height = h;
+        if (o->format == VX_DF_IMAGE_VIRT) o->format = fmt;
+        if (!o->layout()) return fail(n, VX_ERROR_INVALID_FORMAT, "bad output format");
+    }
+    if (o->format != fmt) return fail(n, VX_ERROR_INVALID_FORMAT, "output format");
+    if (o->width != w || o->height != h) return fail(n, VX_ERROR_INVALID_DIMENSION, "output dimensions");
+    return VX_SUCCESS;
+}
+
+vx_status need_input(_vx_node *n, int i, vx_df_image fmt)
+{
+    _vx_image *in = img(n, i);
+    if (!in) return fail(n, VX_ERROR_INVALID_PARAMETERS, "input is not an image");
+    if (in->format != fmt) return fail(n, VX_ERROR_INVALID_FORMAT, "input format");
+    if (in->width == 0 || in->height == 0) return fail(n, VX_ERROR_INVALID_DIMENSION, "input dimensions");
+    return VX_SUCCESS;
+}
+
+vx_status border_undefined(_vx_node *n)
+{
+    if (n->border.mode != VX_BORDER_UNDEFINED)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "only the UNDEFINED border mode is provided for this node");
+    return VX_SUCCESS;
+}
+
+vx_status rc(_vx_node *n, int r)
+{
+    if (r == 0) return VX_SUCCESS;
+    log(n, r, "ERROR: %s: kernel launch returned %d\n", n->kernel->name, r);
+    return r;
+}
+
+// ---------------------------------------------------------------------- 3x3 filters (A4, A5) ----
+// gaussian_3x3 / box_3x3 / median_3x3 -> one AMD kernel each (ago_drama_divide.cpp:835-875)
+typedef int (*Filter3x3)(mvx_stream_t, uint32_t, uint32_t, uint8_t *, uint32_t, const uint8_t *, uint32_t);
+
+template <int IO>  // IO 0: Khronos order (in, out); 1: AMD order (out, in)
+vx_status v_filter(_vx_node *n)
+{
+    const int i = IO ? 1 : 0, o = IO ? 0 : 1;
+    vx_status s = need_input(n, i, VX_DF_IMAGE_U8);
+    if (!s) s = border_undefined(n);
+    if (!s) s = set_output(n, o, img(n, i)->width, img(n, i)->height, VX_DF_IMAGE_U8);
+    return s;
+}
+
+template <int IO, Filter3x3 F> vx_status x_filter(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, IO ? 1 : 0), *out = img(n, IO ? 0 : 1);
+    return rc(n, F(st, out->width, out->height, P(out), S(out), P(in), S(in)));
+}
+
+// ---------------------------------------------------------------------- Sobel (A6) ---------------
+// sobel_3x3 -> SOBEL_S16S16_U8_3x3_GXY, or _GX / _GY with one output (ago_drama_divide.cpp:509-531)
+vx_status v_sobel(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (s) return s;
+    if (!n->params[1] && !n->params[2]) return fail(n, VX_ERROR_INVALID_PARAMETERS, "no output");
+    if ((s = border_undefined(n))) return s;
+    for (int i = 1; i <= 2; i++)
+        if (n->params[i] && (s = set_output(n, i, img(n, 0)->width, img(n, 0)->height, VX_DF_IMAGE_S16))) return s;
+    return VX_SUCCESS;
+}
+
+vx_status x_sobel(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0), *gx = img(n, 1), *gy = img(n, 2);
+    return rc(n, mvx_Sobel_S16S16_U8_3x3_GXY(st, in->width, in->height, gx ? (int16_t *)P(gx) : nullptr, gx ? S(gx) : 0,
+                                             gy ? (int16_t *)P(gy) : nullptr, gy ? S(gy) : 0, P(in), S(in)));
+}
+
+vx_status v_amd_sobel(_vx_node *n)
+{
+    vx_status s = need_input(n, 2, VX_DF_IMAGE_U8);
+    if (!s) s = set_output(n, 0, img(n, 2)->width, img(n, 2)->height, VX_DF_IMAGE_S16);
+    if (!s) s = set_output(n, 1, img(n, 2)->width, img(n, 2)->height, VX_DF_IMAGE_S16);
+    return s;
+}
+
+vx_status x_amd_sobel(_vx_node *n, hipStream_t st)
+{
+    _vx_image *gx = img(n, 0), *gy = img(n, 1), *in = img(n, 2);
+    return rc(n, mvx_Sobel_S16S16_U8_3x3_GXY(st, in->width, in->height, (int16_t *)P(gx), S(gx), (int16_t *)P(gy), S(gy), P(in), S(in)));
+}
+
+// ---------------------------------------------------------------------- Magnitude / Phase (A7, A8)
+// magnitude -> MAGNITUDE_S16_S16S16, phase -> PHASE_U8_S16S16, output first (ago_drama_divide.cpp:533-561)
+template <int IO, vx_df_image OF> vx_status v_gradpair(_vx_node *n)
+{
+    const int a = IO ? 1 : 0, b = IO ? 2 : 1, o = IO ? 0 : 2;
+    vx_status s = need_input(n, a, VX_DF_IMAGE_S16);
+    if (!s) s = need_input(n, b, VX_DF_IMAGE_S16);
+    if (s) return s;
+    if (img(n, a)->width != img(n, b)->width || img(n, a)->height != img(n, b)->height)
+        return fail(n, VX_ERROR_INVALID_DIMENSION, "gradient sizes differ");
+    return set_output(n, o, img(n, a)->width, img(n, a)->height, OF);
+}
+
+template <int IO> vx_status x_magnitude(_vx_node *n, hipStream_t st)
+{
+    _vx_image *gx = img(n, IO ? 1 : 0), *gy = img(n, IO ? 2 : 1), *m = img(n, IO ? 0 : 2);
+    return rc(n, mvx_Magnitude_S16_S16S16(st, m->width, m->height, (int16_t *)P(m), S(m), (const int16_t *)P(gx), S(gx),
+                                          (const int16_t *)P(gy), S(gy)));
+}
+
+template <int IO> vx_status x_phase(_vx_node *n, hipStream_t st)
+{
+    _vx_image *gx = img(n, IO ? 1 : 0), *gy = img(n, IO ? 2 : 1), *p = img(n, IO ? 0 : 2);
+    return rc(n, mvx_Phase_U8_S16S16(st, p->width, p->height, P(p), S(p), (const int16_t *)P(gx), S(gx), (const int16_t *)P(gy), S(gy)));
+}
+
+// ---------------------------------------------------------------------- ColorConvert (A2) --------
+// RGB -> IYUV: the luma plane through ColorConvert_Y_RGB.  The reference keeps only this kernel when
+// the IYUV image is virtual and only its Y channel is read (ago_drama_divide.cpp:244-256 and the
+// unused-output removal, ago_drama_remove.cpp:989); chroma planes are not produced here.
+vx_status v_color(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_RGB);
+    if (s) return s;
+    _vx_image *o = img(n, 1);
+    if (!o) return fail(n, VX_ERROR_INVALID_PARAMETERS, "output is not an image");
+    const vx_df_image want = o->format == VX_DF_IMAGE_VIRT ? VX_DF_IMAGE_IYUV : o->format;
+    if (want != VX_DF_IMAGE_IYUV && want != VX_DF_IMAGE_NV12 && want != VX_DF_IMAGE_YUV4)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "RGB converts to the luma plane of IYUV / NV12 / YUV4 only");
+    if ((s = set_output(n, 1, img(n, 0)->width, img(n, 0)->height, want))) return s;
+    if (!o->isVirtual)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "chroma planes are not produced: the YUV output must be a virtual image read through "
+                                               "channel_extract(Y)");
+    return VX_SUCCESS;
+}
+
+vx_status x_color(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0), *o = img(n, 1);
+    return rc(n, mvx_ColorConvert_Y_RGB(st, o->width, o->height, P(o), S(o), P(in), S(in)));
+}
+
+vx_status v_amd_color_y(_vx_node *n)
+{
+    vx_status s = need_input(n, 1, VX_DF_IMAGE_RGB);
+    if (!s) s = set_output(n, 0, img(n, 1)->width, img(n, 1)->height, VX_DF_IMAGE_U8);
+    return s;
+}
+
+vx_status x_amd_color_y(_vx_node *n, hipStream_t st)
+{
+    _vx_image *o = img(n, 0), *in = img(n, 1);
+    return rc(n, mvx_ColorConvert_Y_RGB(st, o->width, o->height, P(o), S(o), P(in), S(in)));
+}
+
+// ---------------------------------------------------------------------- ChannelExtract (A3) ------
+// channel_extract(Y) of a YUV image -> CHANNEL_COPY_U8_U8 of plane 0 (ago_drama_divide.cpp:342-414)
+vx_status v_extract(_vx_node *n)
+{
+    _vx_image *in = img(n, 0);
+    _vx_scalar *ch = scl(n, 1);
+    if (!in || !ch) return fail(n, VX_ERROR_INVALID_PARAMETERS, "arguments");
+    if (in->format != VX_DF_IMAGE_IYUV && in->format != VX_DF_IMAGE_NV12 && in->format != VX_DF_IMAGE_NV21 && in->format != VX_DF_IMAGE_YUV4)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction is provided for the Y channel of YUV images");
+    if (ch->value.e != VX_CHANNEL_Y && ch->value.e != VX_CHANNEL_0)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "only the Y channel is provided");
+    return set_output(n, 2, in->width, in->height, VX_DF_IMAGE_U8);
+}
+
+vx_status x_extract(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0), *o = img(n, 2);
+    return rc(n, mvx_ChannelCopy_U8_U8(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0)));
+}
+
+vx_status v_amd_copy(_vx_node *n)
+{
+    vx_status s = need_input(n, 1, VX_DF_IMAGE_U8);
+    if (!s) s = set_output(n, 0, img(n, 1)->width, img(n, 1)->height, VX_DF_IMAGE_U8);
+    return s;
+}
+
+vx_status x_amd_copy(_vx_node *n, hipStream_t st)
+{
+    _vx_image *o = img(n, 0), *in = img(n, 1);
+    return rc(n, mvx_ChannelCopy_U8_U8(st, o->width, o->height, P(o), S(o), P(in), S(in)));
+}
+
+// ---------------------------------------------------------------------- ScaleImage (A10) ---------
+// scale_image bilinear -> SCALE_IMAGE_U8_U8_BILINEAR, same size -> CHANNEL_COPY (ago_drama_divide.cpp:563-616)
+vx_status v_scale(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (s) return s;
+    _vx_image *o = img(n, 1);
+    _vx_scalar *type = scl(n, 2);
+    if (!o || !type) return fail(n, VX_ERROR_INVALID_PARAMETERS, "arguments");
+    if (type->value.e != VX_INTERPOLATION_BILINEAR)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "only BILINEAR interpolation is provided");
+    if ((s = border_undefined(n))) return s;
+    if (o->width == 0 || o->height == 0) return fail(n, VX_ERROR_INVALID_DIMENSION, "output size must be set");
+    return set_output(n, 1, o->width, o->height, VX_DF_IMAGE_U8);
+}
+
+vx_status x_scale(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0), *o = img(n, 1);
+    if (in->width == o->width && in->height == o->height)
+        return rc(n, mvx_ChannelCopy_U8_U8(st, o->width, o->height, P(o), S(o), P(in), S(in)));
+    return rc(n, mvx_ScaleImage_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+}
+
+vx_status v_amd_scale(_vx_node *n)
+{
+    vx_status s = need_input(n, 1, VX_DF_IMAGE_U8);
+    if (s) return s;
+    _vx_image *o = img(n, 0);
+    if (!o || o->width == 0 || o->height == 0) return fail(n, VX_ERROR_INVALID_DIMENSION, "output size must be set");
+    return set_output(n, 0, o->width, o->height, VX_DF_IMAGE_U8);
+}
+
+vx_status x_amd_scale(_vx_node *n, hipStream_t st)
+{
+    _vx_image *o = img(n, 0), *in = img(n, 1);
+    return rc(n, mvx_ScaleImage_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+}
+
+// ---------------------------------------------------------------------- GaussianPyramid (A11) ----
+// level 0 CHANNEL_COPY, each further level SCALE_GAUSSIAN_HALF_U8_U8_5x5 (ago_drama_divide.cpp:900-922)
+vx_status v_pyramid(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (s) return s;
+    if (!is_ref(n->params[1], VX_TYPE_PYRAMID)) return fail(n, VX_ERROR_INVALID_PARAMETERS, "output is not a pyramid");
+    _vx_pyramid *p = (_vx_pyramid *)n->params[1];
+    if (p->scale != VX_SCALE_PYRAMID_HALF) return fail(n, VX_ERROR_NOT_SUPPORTED, "only HALF pyramids are provided");
+    if (p->format == VX_DF_IMAGE_VIRT) p->format = VX_DF_IMAGE_U8;
+    if (p->format != VX_DF_IMAGE_U8) return fail(n, VX_ERROR_INVALID_FORMAT, "pyramid format");
+    _vx_image *in = img(n, 0);
+    if (p->width == 0 || p->height == 0) {
+        if (!p->isVirtual) return fail(n, VX_ERROR_INVALID_DIMENSION, "pyramid size");
+        p->width = in->width;
+        p->height = in->height;
+        if (!p->build_levels()) return fail(n, VX_ERROR_INVALID_FORMAT, "pyramid levels");
+    }
+    if (p->width != in->width || p->height != in->height) return fail(n, VX_ERROR_INVALID_DIMENSION, "pyramid size");
+    for (size_t l = 1; l < p->images.size(); l++)
+        if (p->images[l - 1]->width < 5 || p->images[l - 1]->height < 5 || p->images[l]->width < 3 || p->images[l]->height < 3)
+            return fail(n, VX_ERROR_INVALID_DIMENSION, "pyramid level below the 5x5 filter's minimum size");
+    return border_undefined(n);
+}
+
+vx_status x_pyramid(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0);
+    _vx_pyramid *p = (_vx_pyramid *)n->params[1];
+    _vx_image *l0 = p->images[0];
+    vx_status s = rc(n, mvx_ChannelCopy_U8_U8(st, l0->width, l0->height, P(l0), S(l0), P(in), S(in)));
+    for (size_t l = 1; l < p->images.size() && !s; l++) {
+        _vx_image *a = p->images[l], *b = p->images[l - 1];
+        s = rc(n, mvx_ScaleGaussianHalf_U8_U8_5x5(st, a->width, a->height, P(a), S(a), b->width, b->height, P(b), S(b)));
+    }
+    return s;
+}
+
+vx_status v_amd_half(_vx_node *n)
+{
+    vx_status s = need_input(n, 1, VX_DF_IMAGE_U8);
+    if (s) return s;
+    _vx_image *in = img(n, 1);
+    return set_output(n, 0, (in->width + 1) / 2, (in->height + 1) / 2, VX_DF_IMAGE_U8);
+}
+
+vx_status x_amd_half(_vx_node *n, hipStream_t st)
+{
+    _vx_image *o = img(n, 0), *in = img(n, 1);
+    return rc(n, mvx_ScaleGaussianHalf_U8_U8_5x5(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+}
+
+// ---------------------------------------------------------------------- Canny (A12) --------------
+// CANNY_SOBEL_U16_U8_3x3_L1NORM -> CANNY_SUPP_THRESHOLD_U8XY_U16_3x3 -> CANNY_EDGE_TRACE_U8_U8XY with a
+// virtual stack of w*h items (ago_drama_divide.cpp:1618-1701).  Workspace: the U16 gradient plane and
+// the stack (the trace's union-find labels).
+vx_status v_canny(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (s) return s;
+    if (!is_ref(n->params[1], VX_TYPE_THRESHOLD)) return fail(n, VX_ERROR_INVALID_PARAMETERS, "hysteresis is not a threshold");
+    _vx_threshold *t = (_vx_threshold *)n->params[1];
+    if (t->threshType != VX_THRESHOLD_TYPE_RANGE) return fail(n, VX_ERROR_INVALID_TYPE, "hysteresis threshold must be RANGE");
+    _vx_scalar *g = scl(n, 2), *norm = scl(n, 3);
+    if (!g || !norm) return fail(n, VX_ERROR_INVALID_PARAMETERS, "gradient size / norm");
+    if (g->value.i32 != 3 && g->value.i32 != 5 && g->value.i32 != 7) return fail(n, VX_ERROR_INVALID_VALUE, "gradient size");
+    if (g->value.i32 != 3 || norm->value.e != VX_NORM_L1)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "the 3x3 L1-norm Canny is provided (gradient 5/7 and L2 are not)");
+    _vx_image *in = img(n, 0);
+    if (in->width < 3 || in->height < 3) return fail(n, VX_ERROR_INVALID_DIMENSION, "image below 3x3");
+    if ((s = border_undefined(n))) return s;
+    return set_output(n, 4, in->width, in->height, VX_DF_IMAGE_U8);
+}
+
+size_t canny_mag_bytes(uint32_t w, uint32_t h) { return align256((size_t)align16(2 * w) * h); }
+
+vx_status i_canny(_vx_node *n)
+{
+    _vx_image *in = img(n, 0);
+    n->localSize = canny_mag_bytes(in->width, in->height) + (size_t)4 * in->width * in->height;
+    return VX_SUCCESS;
+}
+
+vx_status x_canny(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0), *o = img(n, 4);
+    _vx_threshold *t = (_vx_threshold *)n->params[1];
+    const uint32_t w = in->width, h = in->height, magStride = align16(2 * w);
+    uint16_t *mag = (uint16_t *)n->local.ptr();
+    void *stack = n->local.ptr() + canny_mag_bytes(w, h);
+    // thresholds as the reference reads them: the low 32 bits of the pixel values (ago_kernel_api.cpp:17969)
+    const uint32_t lower = t->lower.U32, upper = t->upper.U32;
+    vx_status s = rc(n, mvx_CannySobel_U16_U8_3x3_L1NORM(st, w, h, mag, magStride, P(in), S(in)));
+    if (!s) s = rc(n, mvx_CannySuppThreshold_U8XY_U16_3x3(st, w, h, P(o), S(o), mag, magStride, stack, w * h, lower, upper));
+    if (!s) s = rc(n, mvx_CannyEdgeTrace_U8_U8XY(st, w, h, P(o), S(o), stack, w * h));
+    return s;
+}
+
+vx_status v_amd_canny_sobel(_vx_node *n)
+{
+    vx_status s = need_input(n, 1, VX_DF_IMAGE_U8);
+    if (!s) s = set_output(n, 0, img(n, 1)->width, img(n, 1)->height, VX_DF_IMAGE_U16);
+    return s;
+}
+
+vx_status x_amd_canny_sobel(_vx_node *n, hipStream_t st)
+{
+    _vx_image *o = img(n, 0), *in = img(n, 1);
+    return rc(n, mvx_CannySobel_U16_U8_3x3_L1NORM(st, o->width, o->height, (uint16_t *)P(o), S(o), P(in), S(in)));
+}
+
+// ---------------------------------------------------------------------- Harris (A13) -------------
+// HARRIS_SOBEL_HG3_U8_3x3 -> HARRIS_SCORE_HVC_HG3_3x3 -> NON_MAX_SUPP_XY_ANY_3x3 ->
+// HARRIS_MERGE_SORT_AND_PICK_XY_XYS (ago_drama_divide.cpp:1703-1787)
+struct HarrisWs {
+    size_t hg3, hvc, xys, cnt, nms, pick, total;
+    uint32_t maxList;
+    size_t nmsSize, pickSize;
+};
+
+HarrisWs harris_ws(uint32_t w, uint32_t h)
+{
+    HarrisWs L;
+    // non-maxima are never 8-adjacent: at most one per 2x2 block of the interior
+    L.maxList = std::max(1u, ((w - 1) / 2) * ((h - 1) / 2));
+    L.nmsSize = mvx_NonMaxSupp_XY_ANY_3x3_workspace_size(w, h);
+    L.pickSize = mvx_HarrisMergeSortAndPick_XY_XYS_workspace_size(L.maxList);
+    L.hg3 = 0;
+    L.hvc = align256(L.hg3 + (size_t)align16(12 * w) * h);
+    L.xys = align256(L.hvc + (size_t)align16(4 * w) * h);
+    L.cnt = align256(L.xys + (size_t)8 * L.maxList);
+    L.nms = align256(L.cnt + 4);
+    L.pick = align256(L.nms + L.nmsSize);
+    L.total = align256(L.pick + L.pickSize);
+    return L;
+}
+
+// (255 * 2^(g-1) * b)^4 in float, left to right (ago_kernel_api.cpp:16941-16942)
+float harris_norm(int gradient, int block)
+{
+    const float f = 255.0f * (float)(1 << (gradient - 1)) * (float)block;
+    return ((f * f) * f) * f;
+}
+
+vx_status v_harris(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (s) return s;
+    _vx_scalar *thr = scl(n, 1), *dist = scl(n, 2), *k = scl(n, 3), *g = scl(n, 4), *b = scl(n, 5);
+    if (!thr || !dist || !k || !g || !b) return fail(n, VX_ERROR_INVALID_PARAMETERS, "scalars");
+    if (thr->dataType != VX_TYPE_FLOAT32 || dist->dataType != VX_TYPE_FLOAT32 || k->dataType != VX_TYPE_FLOAT32)
+        return fail(n, VX_ERROR_INVALID_TYPE, "strength / distance / sensitivity must be FLOAT32");
+    if (g->value.i32 != 3 || b->value.i32 != 3)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "gradient size 3 and block size 3 are provided");
+    if (!is_ref(n->params[6], VX_TYPE_ARRAY) || ((_vx_array *)n->params[6])->itemType != VX_TYPE_KEYPOINT)
+        return fail(n, VX_ERROR_INVALID_PARAMETERS, "corners must be a KEYPOINT array");
+    _vx_array *corners = (_vx_array *)n->params[6];
+    if (!corners->capacity) return fail(n, VX_ERROR_INVALID_DIMENSION, "corner array capacity");
+    if (n->params[7] && (!scl(n, 7) || scl(n, 7)->dataType != VX_TYPE_SIZE))
+        return fail(n, VX_ERROR_INVALID_TYPE, "num_corners must be a SIZE scalar");
+    _vx_image *in = img(n, 0);
+    if (in->width < 3 || in->height < 3 || in->width >= 32768 || in->height >= 32768)
+        return fail(n, VX_ERROR_INVALID_DIMENSION, "image size");
+    return VX_SUCCESS;
+}
+
+vx_status i_harris(_vx_node *n)
+{
+    _vx_image *in = img(n, 0);
+    n->localSize = harris_ws(in->width, in->height).total;
+    return VX_SUCCESS;
+}
+
+vx_status x_harris(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0);
+    const uint32_t w = in->width, h = in->height;
+    const HarrisWs L = harris_ws(w, h);
+    uint8_t *ws = n->local.ptr();
+    _vx_array *corners = (_vx_array *)n->params[6];
+    _vx_scalar *num = scl(n, 7);
+    const float thr = scl(n, 1)->value.f32, dist = scl(n, 2)->value.f32, k = scl(n, 3)->value.f32;
+    float *hg3 = (float *)(ws + L.hg3), *hvc = (float *)(ws + L.hvc);
+    uint32_t *cnt = (uint32_t *)(ws + L.cnt);
+    vx_status s = rc(n, mvx_HarrisSobel_HG3_U8_3x3(st, w, h, hg3, align16(12 * w), P(in), S(in)));
+    if (!s) s = rc(n, mvx_HarrisScore_HVC_HG3_3x3(st, w, h, hvc, align16(4 * w), hg3, align16(12 * w), k, thr, harris_norm(3, 3)));
+    if (!s) s = rc(n, mvx_NonMaxSupp_XY_ANY_3x3(st, L.maxList, ws + L.xys, cnt, w, h, hvc, align16(4 * w), ws + L.nms, L.nmsSize));
+    // the count scalar lives in the node workspace's tail when the caller gave none
+    uint64_t *count = num ? (uint64_t *)num->dev.ptr() : (uint64_t *)(ws + L.cnt + 8);
+    if (!s) s = rc(n, mvx_HarrisMergeSortAndPick_XY_XYS(st, (uint32_t)corners->capacity, corners->items_ptr(), corners->count_ptr(),
+                                                        count, ws + L.xys, cnt, L.maxList, dist, w, h, ws + L.pick, L.pickSize));
+    return s;
+}
+
+// the pick workspace must be prepared once (hash slots empty)
+vx_status i_harris_prepare(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0);
+    const HarrisWs L = harris_ws(in->width, in->height);
+    return rc(n, mvx_HarrisMergeSortAndPick_workspace_init(st, L.maxList, n->local.ptr() + L.pick, L.pickSize));
+}
+
+}  // namespace
+
+namespace mvxe {
+
+// called by graph verification after the node's workspace is allocated
+vx_status prepare_node(_vx_node *n, hipStream_t st)
+{
+    if (n->kernel->id == VX_KERNEL_HARRIS_CORNERS) return i_harris_prepare(n, st);
+    return VX_SUCCESS;
+}
+
+float harris_norm_factor(int gradient, int block) { return harris_norm(gradient, block); }
+
+const std::vector<KernelDef> &kernel_table()
+{
+    static const std::vector<KernelDef> table = {
+        // Khronos vision nodes (ago_kernel_list.cpp:183-260)
+        {VX_KERNEL_COLOR_CONVERT, "org.khronos.openvx.color_convert", {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}},
+         v_color, nullptr, x_color, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_CHANNEL_EXTRACT, "org.khronos.openvx.channel_extract",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}}, v_extract, nullptr, x_extract,
+         AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_SOBEL_3x3, "org.khronos.openvx.sobel_3x3",
+         {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, true}, {OUT, VX_TYPE_IMAGE, true}}, v_sobel, nullptr, x_sobel,
+         AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_MAGNITUDE, "org.khronos.openvx.magnitude",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}}, v_gradpair<0, VX_DF_IMAGE_S16>, nullptr,
+         x_magnitude<0>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_PHASE, "org.khronos.openvx.phase",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}}, v_gradpair<0, VX_DF_IMAGE_U8>, nullptr,
+         x_phase<0>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_SCALE_IMAGE, "org.khronos.openvx.scale_image",
+         {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}}, v_scale, nullptr, x_scale,
+         AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_MEDIAN_3x3, "org.khronos.openvx.median_3x3", {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}},
+         v_filter<0>, nullptr, x_filter<0, mvx_Median_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_BOX_3x3, "org.khronos.openvx.box_3x3", {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}}, v_filter<0>,
+         nullptr, x_filter<0, mvx_Box_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_GAUSSIAN_3x3, "org.khronos.openvx.gaussian_3x3", {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}},
+         v_filter<0>, nullptr, x_filter<0, mvx_Gaussian_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_GAUSSIAN_PYRAMID, "org.khronos.openvx.gaussian_pyramid", {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_PYRAMID, false}},
+         v_pyramid, nullptr, x_pyramid, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_CANNY_EDGE_DETECTOR, "org.khronos.openvx.canny_edge_detector",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_THRESHOLD, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false},
+          {OUT, VX_TYPE_IMAGE, false}},
+         v_canny, i_canny, x_canny, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_HARRIS_CORNERS, "org.khronos.openvx.harris_corners",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false},
+          {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_ARRAY, false}, {OUT, VX_TYPE_SCALAR, true}},
+         v_harris, i_harris, x_harris, AGO_TARGET_AFFINITY_GPU},
+        // AMD low-level kernels, output first (ago_kernel_list.cpp:339-474)
+        {AMD_MAGNITUDE_S16_S16S16, "com.amd.openvx.Magnitude_S16_S16S16",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_gradpair<1, VX_DF_IMAGE_S16>, nullptr,
+         x_magnitude<1>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_PHASE_U8_S16S16, "com.amd.openvx.Phase_U8_S16S16",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_gradpair<1, VX_DF_IMAGE_U8>, nullptr,
+         x_phase<1>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_CHANNEL_COPY_U8_U8, "com.amd.openvx.ChannelCopy_U8_U8", {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}},
+         v_amd_copy, nullptr, x_amd_copy, AGO_TARGET_AFFINITY_GPU},
+        {AMD_COLOR_CONVERT_Y_RGB, "com.amd.openvx.ColorConvert_Y_RGB", {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}},
+         v_amd_color_y, nullptr, x_amd_color_y, AGO_TARGET_AFFINITY_GPU},
+        {AMD_BOX_U8_U8_3x3, "com.amd.openvx.Box_U8_U8_3x3", {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_filter<1>,
+         nullptr, x_filter<1, mvx_Box_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_MEDIAN_U8_U8_3x3, "com.amd.openvx.Median_U8_U8_3x3", {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}},
+         v_filter<1>, nullptr, x_filter<1, mvx_Median_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_GAUSSIAN_U8_U8_3x3, "com.amd.openvx.Gaussian_U8_U8_3x3", {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}},
+         v_filter<1>, nullptr, x_filter<1, mvx_Gaussian_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5, "com.amd.openvx.ScaleGaussianHalf_U8_U8_5x5",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_half, nullptr, x_amd_half, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SOBEL_S16S16_U8_3x3_GXY, "com.amd.openvx.Sobel_S16S16_U8_3x3_GXY",
+         {{OUT, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_sobel, nullptr, x_amd_sobel,
+         AGO_TARGET_AFFINITY_GPU},
+        {AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM, "com.amd.openvx.CannySobel_U16_U8_3x3_L1NORM",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel,
+         AGO_TARGET_AFFINITY_GPU},
+        {AMD_SCALE_IMAGE_U8_U8_BILINEAR, "com.amd.openvx.ScaleImage_U8_U8_Bilinear",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale, nullptr, x_amd_scale, AGO_TARGET_AFFINITY_GPU},
+    };
+    return table;
+}
+
+}  // namespace mvxe

@@ -1,0 +1,179 @@
+"""The OpenVX engine on the CPU side: exported API, header ABI, GDF import, graph validation.
+
+No kernel runs here; graphs that would need the GPU must fail loudly (there is no CPU fallback).
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(*headers):
+    names = set()
+    for h in headers:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"VX_API_CALL\s+(vx\w+)\s*\(", text))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def vx():
+    from mivisionx_amd import openvx
+    return openvx
+
+
+def test_engine_exports_every_declared_function(vx):
+    lib = vx.lib()
+    names = _declared("VX/vx.h", "VX/vxu.h")
+    assert len(names) >= 100
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_headers_compile_as_c99_with_openvx_abi(tmp_path):
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "c", "vx_abi_probe.c"), "-o", str(exe)])
+    got = dict(line.split() for line in subprocess.check_output([str(exe)]).decode().splitlines())
+    got = {k: int(v) for k, v in got.items()}
+    # structure layouts of the OpenVX 1.3 headers and vx_ext_amd.h
+    assert got["sizeof_keypoint"] == 28 and got["sizeof_addressing"] == 32 and got["sizeof_pixel_value"] == 16
+    assert got["sizeof_border"] == 20 and got["sizeof_affinity"] == 16 and got["sizeof_perf"] == 64
+    assert got["sizeof_rectangle"] == 16 and got["sizeof_coordinates2df"] == 8
+    # enumeration values: fourcc codes, kernel ids, enum / attribute bases
+    assert got["VX_DF_IMAGE_U8"] == 0x38303055 and got["VX_DF_IMAGE_RGB"] == 0x32424752 and got["VX_DF_IMAGE_IYUV"] == 0x56555949
+    assert (got["VX_KERNEL_GAUSSIAN_3x3"], got["VX_KERNEL_CANNY_EDGE_DETECTOR"], got["VX_KERNEL_HARRIS_CORNERS"],
+            got["VX_KERNEL_REMAP"]) == (0x13, 0x1B, 0x25, 0x28)
+    assert got["VX_NORM_L1"] == 0x10000 and got["VX_INTERPOLATION_BILINEAR"] == 0x4001
+    assert got["VX_THRESHOLD_TYPE_RANGE"] == 0xB001 and got["VX_BORDER_CONSTANT"] == 0xC001 and got["VX_READ_ONLY"] == 0x11001
+    assert got["VX_IMAGE_WIDTH"] == 0x80F00 and got["VX_ARRAY_NUMITEMS"] == 0x80E01 and got["VX_NODE_BORDER"] == 0x80302
+    assert got["VX_GRAPH_ATTRIBUTE_AMD_IMPORT_FROM_TEXT"] == 0xD80202 and got["VX_NODE_ATTRIBUTE_AMD_AFFINITY"] == 0xD80301
+    assert got["VX_IMAGE_ATTRIBUTE_AMD_HIP_BUFFER"] == 0xD80F06 and got["VX_MEMORY_TYPE_HIP"] == 0xE003
+    assert got["AGO_TARGET_AFFINITY_GPU"] == 0x20 and got["VX_ERROR_INVALID_GRAPH"] == -18
+
+
+def test_kernel_names_and_enums_resolve(vx):
+    lib = vx.lib()
+    ctx = lib.vxCreateContext()
+    try:
+        for name in ["org.khronos.openvx.gaussian_3x3", "org.khronos.openvx.canny_edge_detector",
+                     "org.khronos.openvx.harris_corners", "com.amd.openvx.Gaussian_U8_U8_3x3",
+                     "com.amd.openvx.ScaleGaussianHalf_U8_U8_5x5"]:
+            assert lib.vxGetKernelByName(ctx, name.encode()), name
+        assert lib.vxGetKernelByEnum(ctx, 0x13)               # VX_KERNEL_GAUSSIAN_3x3
+        assert lib.vxGetKernelByEnum(ctx, 0xd000b4)           # VX_KERNEL_AMD_GAUSSIAN_U8_U8_3x3
+    finally:
+        h = ctypes.c_void_p(ctx)
+        lib.vxReleaseContext(ctypes.byref(h))
+
+
+GDF = """
+# 4K vision graph (RGB -> luma -> scale / pyramid / Canny / Harris)
+data rgb = image:3840,2160,RGB2:READ,frame.rgb
+data yuv = image-virtual:3840,2160,IYUV
+data luma = virtual-image:3840,2160,U008
+data scaled = image:1920,1080,U008:WRITE,scaled.u8
+data pyr = pyramid:5,HALF,3840,2160,U008
+data edges = image:3840,2160,U008
+data hyst = threshold:RANGE,U008,U008:INIT,80,100
+data grad = scalar:INT32,3
+data strength = scalar:FLOAT32,0.0005
+data mind = scalar:FLOAT32,5.0
+data sens = scalar:FLOAT32,0.04
+data corners = array:KEYPOINT,65536
+data count = scalar:SIZE,0
+node org.khronos.openvx.color_convert rgb yuv
+node org.khronos.openvx.channel_extract yuv !CHANNEL_Y luma
+node org.khronos.openvx.scale_image luma scaled !BILINEAR
+node org.khronos.openvx.gaussian_pyramid luma pyr
+node org.khronos.openvx.canny_edge_detector luma hyst grad !NORM_L1 edges \\
+     attr:BORDER_MODE:UNDEFINED
+node org.khronos.openvx.harris_corners luma strength mind sens grad grad corners count attr:AFFINITY:GPU
+"""
+
+
+def test_gdf_import_builds_the_graph_and_reports_io(vx):
+    lib = vx.lib()
+    ctx = lib.vxCreateContext()
+    try:
+        g = lib.vxCreateGraph(ctx)
+        seen = []
+        assert vx.import_gdf(g, GDF, callback=lambda ref, name, params: seen.append((name, params))) == 0
+        n = ctypes.c_uint32()
+        assert lib.vxQueryGraph(g, vx.VX_GRAPH_NUMNODES, ctypes.byref(n), 4) == 0
+        assert n.value == 6
+        assert ("rgb", "READ,frame.rgb") in seen and ("scaled", "WRITE,scaled.u8") in seen
+        assert len(seen) == 2   # INIT is applied by the importer itself
+    finally:
+        h = ctypes.c_void_p(ctx)
+        lib.vxReleaseContext(ctypes.byref(h))
+
+
+@pytest.mark.parametrize("text,status", [
+    ("data a = image:64,64,U008\nnode org.khronos.openvx.no_such_kernel a a\n", -10),
+    ("data a = image:64,64,U008\nnode org.khronos.openvx.gaussian_3x3 a undefined_b\n", -10),
+    ("data a = image:64,64,U008\nfrobnicate a\n", -3),
+    ("data a = tensor:4,1,1,1,U008\n", -1),
+])
+def test_gdf_import_rejects_bad_text(vx, text, status):
+    lib = vx.lib()
+    ctx = lib.vxCreateContext()
+    try:
+        g = lib.vxCreateGraph(ctx)
+        assert vx.import_gdf(g, text) == status
+    finally:
+        h = ctypes.c_void_p(ctx)
+        lib.vxReleaseContext(ctypes.byref(h))
+
+
+def _verify(vx, text):
+    lib = vx.lib()
+    ctx = lib.vxCreateContext()
+    try:
+        g = lib.vxCreateGraph(ctx)
+        assert vx.import_gdf(g, text) == 0
+        return lib.vxVerifyGraph(g)
+    finally:
+        h = ctypes.c_void_p(ctx)
+        lib.vxReleaseContext(ctypes.byref(h))
+
+
+def test_validation_errors_come_before_any_device_work(vx):
+    # wrong input format
+    assert _verify(vx, "data a = image:64,64,S016\ndata b = image:64,64,U008\n"
+                       "node org.khronos.openvx.gaussian_3x3 a b\n") == vx.VX_ERROR_INVALID_FORMAT
+    # output size mismatch
+    assert _verify(vx, "data a = image:64,64,U008\ndata b = image:32,64,U008\n"
+                       "node org.khronos.openvx.box_3x3 a b\n") == vx.VX_ERROR_INVALID_DIMENSION
+    # two writers of one image
+    assert _verify(vx, "data a = image:64,64,U008\ndata b = image:64,64,U008\n"
+                       "node org.khronos.openvx.gaussian_3x3 a b\nnode org.khronos.openvx.box_3x3 a b\n") == vx.VX_ERROR_MULTIPLE_WRITERS
+    # a cycle
+    assert _verify(vx, "data a = image:64,64,U008\ndata b = image:64,64,U008\n"
+                       "node org.khronos.openvx.gaussian_3x3 a b\nnode org.khronos.openvx.box_3x3 b a\n") == vx.VX_ERROR_INVALID_GRAPH
+    # a variant this engine does not provide is refused, not emulated
+    assert _verify(vx, "data a = image:64,64,U008\ndata h = threshold:RANGE,U008,U008:INIT,10,20\n"
+                       "data g = scalar:INT32,5\ndata e = image:64,64,U008\n"
+                       "node org.khronos.openvx.canny_edge_detector a h g !NORM_L1 e\n") == vx.VX_ERROR_NOT_SUPPORTED
+
+
+def test_valid_graph_without_a_gpu_fails_loudly(vx):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    status = _verify(vx, "data a = image:64,64,U008\ndata b = image:64,64,U008\nnode org.khronos.openvx.gaussian_3x3 a b\n")
+    assert status == vx.VX_ERROR_NOT_SUPPORTED
+
+
+def test_runvx_cli_reports_usage_and_bad_files():
+    from mivisionx_amd import openvx
+    assert os.path.exists(openvx.RUNVX)
+    r = subprocess.run([openvx.RUNVX], capture_output=True)
+    assert r.returncode == 1 and b"usage" in r.stderr
+    r = subprocess.run([openvx.RUNVX, "/nonexistent.gdf"], capture_output=True)
+    assert r.returncode == 1
