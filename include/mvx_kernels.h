@@ -164,6 +164,51 @@ int mvx_HarrisMergeSortAndPick_XY_XYS(mvx_stream_t stream, uint32_t capacity, vo
                                       const uint32_t *pSrcCount, uint32_t maxList, float minDistance,
                                       uint32_t width, uint32_t height, void *workspace, size_t workspaceSize);
 
+/* ------------------------------------------------------------------ warps and remap -------- */
+/* Bilinear U8 warps and remap.  Rows are written up to ceil(w/4)*4 pixels (the reference's 4-pixel
+ * stores), so dstStride must be >= ceil(w/4)*4 and a multiple of 4; the source is read at linear
+ * byte addresses with reads outside [0, srcHeight*srcStride) returning 0.  `matrix` is host memory
+ * holding the VX matrix as stored: 6 floats (2 columns x 3 rows) for affine, 9 for perspective. */
+
+/* replaces HipExec_WarpAffine_U8_U8_Bilinear (hip_host_decls.h:762); HafCpu_WarpAffine_U8_U8_Bilinear
+ * (ago_haf_cpu_geometric.cpp:825-1099), border UNDEFINED */
+int mvx_WarpAffine_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                  uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                  uint32_t srcStride, const float *matrix);
+/* replaces HipExec_WarpAffine_U8_U8_Bilinear_Constant (hip_host_decls.h:768);
+ * HafCpu_WarpAffine_U8_U8_Bilinear_Constant (ago_haf_cpu_geometric.cpp:1100-1370) */
+int mvx_WarpAffine_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                           uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                           uint32_t srcStride, const float *matrix, uint8_t borderValue);
+/* replaces HipExec_WarpPerspective_U8_U8_Bilinear (hip_host_decls.h:788); HafCpu_WarpPerspective_U8_U8_Bilinear
+ * (ago_haf_cpu_geometric.cpp:1560-1577) */
+int mvx_WarpPerspective_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                       uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                       uint32_t srcStride, const float *matrix);
+/* replaces HipExec_WarpPerspective_U8_U8_Bilinear_Constant (hip_host_decls.h:794);
+ * HafCpu_WarpPerspective_U8_U8_Bilinear_Constant (ago_haf_cpu_geometric.cpp:1579-1873) */
+int mvx_WarpPerspective_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                                const uint8_t *pSrc, uint32_t srcStride, const float *matrix,
+                                                uint8_t borderValue);
+/* replaces HipExec_Remap_U8_U8_Bilinear (hip_host_decls.h:813); HafCpu_Remap_U8_U8_Bilinear
+ * (ago_haf_cpu_geometric.cpp:249-370).  pMap: device table of {u16 x, u16 y} in 13.3 fixed point,
+ * (0xffff, 0xffff) for points outside the source (mvx_Remap_FixedMap_F32 builds it), mapStride
+ * bytes per row (the reference uses dstWidth*4). */
+int mvx_Remap_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                             uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride,
+                             const void *pMap, uint32_t mapStride);
+/* replaces HipExec_Remap_U8_U8_Bilinear_Constant (hip_host_decls.h:819); HafCpu_Remap_U8_U8_Bilinear_Constant
+ * (ago_haf_cpu_geometric.cpp:372-504) */
+int mvx_Remap_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                      uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                      uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue);
+/* The fixed-point remap table of vxSetRemapPoint (vx_api.cpp:7865-7894, 3 fractional bits) from a
+ * device float table of {x, y} pairs, on the GPU.  Sources must be below 8192 pixels in each
+ * dimension (the reference uses fewer fractional bits beyond, ago_util.cpp:2450-2459). */
+int mvx_Remap_FixedMap_F32(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, void *pFixedMap, uint32_t fixedStride,
+                           const float *pFloatMap, uint32_t floatStride, uint32_t srcWidth, uint32_t srcHeight);
+
 /* ------------------------------------------------------------------ fused vision graph ----- */
 /* The RGB vision graph (vx nodes: ColorConvert RGB->IYUV, ChannelExtract Y, ScaleImage half bilinear,
  * GaussianPyramid HALF, CannyEdgeDetector 3/L1, HarrisCorners 3/3) over a batch of equally sized

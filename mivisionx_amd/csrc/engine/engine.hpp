@@ -187,8 +187,12 @@ struct _vx_matrix : _vx_reference {
 
 struct _vx_remap : _vx_reference {
     uint32_t srcWidth, srcHeight, dstWidth, dstHeight;
-    std::vector<float> map;  // dst h x w x {x, y}
+    std::vector<float> map;  // dst h x w x {x, y}, as set through the API
+    bool dirty = true;       // host table changed since the device copies were made
+    mvxe::DevBuf devFloat;   // the float table in HBM
+    mvxe::DevBuf devFixed;   // {u16 x, u16 y} 13.3 fixed point, dstWidth*4 bytes per row
     _vx_remap(_vx_context *c) : _vx_reference(VX_TYPE_REMAP, c) {}
+    vx_status sync(hipStream_t stream);  // refresh the device tables when dirty (enqueues the conversion)
 };
 
 struct _vx_kernel : _vx_reference {

@@ -21,6 +21,12 @@ constexpr vx_enum IN = VX_INPUT, OUT = VX_OUTPUT;
 
 // AMD low-level kernel enums (ago_kernels.h, enum vx_kernel_amd_e)
 enum : vx_enum {
+    AMD_REMAP_U8_U8_BILINEAR = 0xd000e6,
+    AMD_REMAP_U8_U8_BILINEAR_CONSTANT = 0xd000e7,
+    AMD_WARP_AFFINE_U8_U8_BILINEAR = 0xd000ea,
+    AMD_WARP_AFFINE_U8_U8_BILINEAR_CONSTANT = 0xd000eb,
+    AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR = 0xd000ee,
+    AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR_CONSTANT = 0xd000ef,
     AMD_MAGNITUDE_S16_S16S16 = 0xd0006f,
     AMD_PHASE_U8_S16S16 = 0xd00070,
     AMD_CHANNEL_COPY_U8_U8 = 0xd00071,
@@ -478,6 +484,134 @@ vx_status i_harris_prepare(_vx_node *n, hipStream_t st)
     return rc(n, mvx_HarrisMergeSortAndPick_workspace_init(st, L.maxList, n->local.ptr() + L.pick, L.pickSize));
 }
 
+// ---------------------------------------------------------------------- warps (A14, A15) ---------
+// warp_affine / warp_perspective bilinear -> WARP_*_U8_U8_BILINEAR (border UNDEFINED) or
+// WARP_*_U8_U8_BILINEAR_CONSTANT with the border value (ago_drama_divide.cpp:1542-1616); the matrix
+// goes to the kernel as stored (2x3 or 3x3 FLOAT32, ago_kernel_api.cpp:19143-19275).
+vx_status check_matrix(_vx_node *n, int i, vx_size cols)
+{
+    if (!is_ref(n->params[i], VX_TYPE_MATRIX)) return fail(n, VX_ERROR_INVALID_PARAMETERS, "matrix");
+    _vx_matrix *m = (_vx_matrix *)n->params[i];
+    if (m->dataType != VX_TYPE_FLOAT32) return fail(n, VX_ERROR_INVALID_TYPE, "matrix must be FLOAT32");
+    if (m->columns != cols || m->rows != 3) return fail(n, VX_ERROR_INVALID_DIMENSION, "matrix dimensions");
+    return VX_SUCCESS;
+}
+
+vx_status fixed_size_output(_vx_node *n, int i)
+{
+    _vx_image *o = img(n, i);
+    if (!o || o->width == 0 || o->height == 0) return fail(n, VX_ERROR_INVALID_DIMENSION, "output size must be set");
+    return set_output(n, i, o->width, o->height, VX_DF_IMAGE_U8);
+}
+
+vx_status bilinear_and_border(_vx_node *n, int interp)
+{
+    _vx_scalar *t = scl(n, interp);
+    if (!t) return fail(n, VX_ERROR_INVALID_PARAMETERS, "interpolation");
+    if (t->value.e != VX_INTERPOLATION_BILINEAR) return fail(n, VX_ERROR_NOT_SUPPORTED, "only BILINEAR interpolation is provided");
+    if (n->border.mode != VX_BORDER_UNDEFINED && n->border.mode != VX_BORDER_CONSTANT)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "border mode must be UNDEFINED or CONSTANT");
+    return VX_SUCCESS;
+}
+
+template <bool PERSP> vx_status v_warp(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (!s) s = check_matrix(n, 1, PERSP ? 3 : 2);
+    if (!s) s = bilinear_and_border(n, 2);
+    if (!s) s = fixed_size_output(n, 3);
+    return s;
+}
+
+template <bool PERSP>
+vx_status run_warp(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _vx_matrix *m, bool constant, uint8_t border)
+{
+    const float *mat = (const float *)m->data.data();
+    if (PERSP)
+        return rc(n, constant ? mvx_WarpPerspective_U8_U8_Bilinear_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height,
+                                                                            P(in), S(in), mat, border)
+                              : mvx_WarpPerspective_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in),
+                                                                   S(in), mat));
+    return rc(n, constant ? mvx_WarpAffine_U8_U8_Bilinear_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in),
+                                                                   S(in), mat, border)
+                          : mvx_WarpAffine_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in), mat));
+}
+
+template <bool PERSP> vx_status x_warp(_vx_node *n, hipStream_t st)
+{
+    const bool constant = n->border.mode == VX_BORDER_CONSTANT;
+    return run_warp<PERSP>(n, st, img(n, 3), img(n, 0), (_vx_matrix *)n->params[1], constant, n->border.constant_value.U8);
+}
+
+// AMD forms: (out, in, matrix[, border scalar])
+template <bool PERSP, bool CONST> vx_status v_amd_warp(_vx_node *n)
+{
+    vx_status s = need_input(n, 1, VX_DF_IMAGE_U8);
+    if (!s) s = check_matrix(n, 2, PERSP ? 3 : 2);
+    if (!s && CONST && (!scl(n, 3) || scl(n, 3)->dataType != VX_TYPE_UINT8)) s = fail(n, VX_ERROR_INVALID_FORMAT, "border scalar must be UINT8");
+    if (!s) s = fixed_size_output(n, 0);
+    return s;
+}
+
+template <bool PERSP, bool CONST> vx_status x_amd_warp(_vx_node *n, hipStream_t st)
+{
+    return run_warp<PERSP>(n, st, img(n, 0), img(n, 1), (_vx_matrix *)n->params[2], CONST, CONST ? scl(n, 3)->value.u8 : 0);
+}
+
+// ---------------------------------------------------------------------- Remap (A16) --------------
+// remap bilinear -> REMAP_U8_U8_BILINEAR or _BILINEAR_CONSTANT (ago_drama_divide.cpp:1490-1540) over
+// the fixed-point table vxSetRemapPoint builds (here: built on the GPU from the float table)
+vx_status check_remap(_vx_node *n, int i, _vx_image *in)
+{
+    if (!is_ref(n->params[i], VX_TYPE_REMAP)) return fail(n, VX_ERROR_INVALID_PARAMETERS, "table is not a remap");
+    _vx_remap *r = (_vx_remap *)n->params[i];
+    if (r->srcWidth != in->width || r->srcHeight != in->height) return fail(n, VX_ERROR_INVALID_DIMENSION, "remap source size");
+    if (r->srcWidth >= 8192 || r->srcHeight >= 8192)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "remap sources of 8192 pixels or more (fewer fractional bits) are not provided");
+    return VX_SUCCESS;
+}
+
+vx_status v_remap(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (!s) s = check_remap(n, 1, img(n, 0));
+    if (!s) s = bilinear_and_border(n, 2);
+    if (s) return s;
+    _vx_remap *r = (_vx_remap *)n->params[1];
+    return set_output(n, 3, r->dstWidth, r->dstHeight, VX_DF_IMAGE_U8);
+}
+
+vx_status run_remap(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _vx_remap *r, bool constant, uint8_t border)
+{
+    vx_status s = r->sync(st);
+    if (s) return fail(n, s, "remap table upload");
+    return rc(n, constant ? mvx_Remap_U8_U8_Bilinear_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
+                                                              r->devFixed.ptr(), r->dstWidth * 4, border)
+                          : mvx_Remap_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
+                                                     r->devFixed.ptr(), r->dstWidth * 4));
+}
+
+vx_status x_remap(_vx_node *n, hipStream_t st)
+{
+    return run_remap(n, st, img(n, 3), img(n, 0), (_vx_remap *)n->params[1], n->border.mode == VX_BORDER_CONSTANT,
+                     n->border.constant_value.U8);
+}
+
+template <bool CONST> vx_status v_amd_remap(_vx_node *n)
+{
+    vx_status s = need_input(n, 1, VX_DF_IMAGE_U8);
+    if (!s) s = check_remap(n, 2, img(n, 1));
+    if (!s && CONST && (!scl(n, 3) || scl(n, 3)->dataType != VX_TYPE_UINT8)) s = fail(n, VX_ERROR_INVALID_FORMAT, "border scalar must be UINT8");
+    if (s) return s;
+    _vx_remap *r = (_vx_remap *)n->params[2];
+    return set_output(n, 0, r->dstWidth, r->dstHeight, VX_DF_IMAGE_U8);
+}
+
+template <bool CONST> vx_status x_amd_remap(_vx_node *n, hipStream_t st)
+{
+    return run_remap(n, st, img(n, 0), img(n, 1), (_vx_remap *)n->params[2], CONST, CONST ? scl(n, 3)->value.u8 : 0);
+}
+
 }  // namespace
 
 namespace mvxe {
@@ -528,7 +662,34 @@ const std::vector<KernelDef> &kernel_table()
          {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false},
           {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_ARRAY, false}, {OUT, VX_TYPE_SCALAR, true}},
          v_harris, i_harris, x_harris, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_WARP_AFFINE, "org.khronos.openvx.warp_affine",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
+         v_warp<false>, nullptr, x_warp<false>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_WARP_PERSPECTIVE, "org.khronos.openvx.warp_perspective",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
+         v_warp<true>, nullptr, x_warp<true>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_REMAP, "org.khronos.openvx.remap",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
+         v_remap, nullptr, x_remap, AGO_TARGET_AFFINITY_GPU},
         // AMD low-level kernels, output first (ago_kernel_list.cpp:339-474)
+        {AMD_WARP_AFFINE_U8_U8_BILINEAR, "com.amd.openvx.WarpAffine_U8_U8_Bilinear",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}}, v_amd_warp<false, false>, nullptr,
+         x_amd_warp<false, false>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_WARP_AFFINE_U8_U8_BILINEAR_CONSTANT, "com.amd.openvx.WarpAffine_U8_U8_Bilinear_Constant",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_warp<false, true>, nullptr, x_amd_warp<false, true>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR, "com.amd.openvx.WarpPerspective_U8_U8_Bilinear",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}}, v_amd_warp<true, false>, nullptr,
+         x_amd_warp<true, false>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR_CONSTANT, "com.amd.openvx.WarpPerspective_U8_U8_Bilinear_Constant",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_warp<true, true>, nullptr, x_amd_warp<true, true>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_REMAP_U8_U8_BILINEAR, "com.amd.openvx.Remap_U8_U8_Bilinear",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}}, v_amd_remap<false>, nullptr,
+         x_amd_remap<false>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_REMAP_U8_U8_BILINEAR_CONSTANT, "com.amd.openvx.Remap_U8_U8_Bilinear_Constant",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_remap<true>, nullptr, x_amd_remap<true>, AGO_TARGET_AFFINITY_GPU},
         {AMD_MAGNITUDE_S16_S16S16, "com.amd.openvx.Magnitude_S16_S16S16",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_gradpair<1, VX_DF_IMAGE_S16>, nullptr,
          x_magnitude<1>, AGO_TARGET_AFFINITY_GPU},

@@ -104,6 +104,7 @@ static void destroy(_vx_reference *r)
     case VX_TYPE_IMAGE: ((_vx_image *)r)->dev.release(); break;
     case VX_TYPE_SCALAR: ((_vx_scalar *)r)->dev.release(); break;
     case VX_TYPE_ARRAY: ((_vx_array *)r)->dev.release(); break;
+    case VX_TYPE_REMAP: ((_vx_remap *)r)->devFloat.release(); ((_vx_remap *)r)->devFixed.release(); break;
     case VX_TYPE_PYRAMID:
         for (_vx_image *img : ((_vx_pyramid *)r)->images) release_ref(img, false);
         break;
@@ -1203,6 +1204,7 @@ VX_API_ENTRY vx_status VX_API_CALL vxSetRemapPoint(vx_remap table, vx_uint32 dst
     float *p = &table->map[((size_t)dst_y * table->dstWidth + dst_x) * 2];
     p[0] = src_x;
     p[1] = src_y;
+    table->dirty = true;
     return VX_SUCCESS;
 }
 
@@ -1232,6 +1234,7 @@ VX_API_ENTRY vx_status VX_API_CALL vxCopyRemapPatch(vx_remap remap, const vx_rec
         else if (usage == VX_WRITE_ONLY) memcpy(m, u, rowBytes);
         else return VX_ERROR_INVALID_PARAMETERS;
     }
+    if (usage == VX_WRITE_ONLY) remap->dirty = true;
     return VX_SUCCESS;
 }
 
@@ -1246,6 +1249,24 @@ VX_API_ENTRY vx_status VX_API_CALL vxQueryRemap(vx_remap table, vx_enum attribut
     case VX_REMAP_DESTINATION_HEIGHT: PUT(vx_uint32, table->dstHeight); break;
     default: return VX_ERROR_NOT_SUPPORTED;
     }
+    return VX_SUCCESS;
+}
+
+vx_status _vx_remap::sync(hipStream_t stream)
+{
+    if (!dirty && devFixed.base) return VX_SUCCESS;
+    vx_status s = context->ensure_hip();
+    if (!s) s = devFloat.alloc(map.size() * sizeof(float));
+    if (!s) s = devFixed.alloc((size_t)dstWidth * dstHeight * 4);
+    if (s) return s;
+    // the float table goes up once per change; the fixed-point table is derived from it on the GPU
+    if (hipMemcpyAsync(devFloat.ptr(), map.data(), map.size() * sizeof(float), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return VX_FAILURE;
+    const int r = mvx_Remap_FixedMap_F32(stream, dstWidth, dstHeight, devFixed.ptr(), dstWidth * 4, (const float *)devFloat.ptr(),
+                                         dstWidth * 8, srcWidth, srcHeight);
+    if (r) return r;
+    dirty = false;
     return VX_SUCCESS;
 }
 

@@ -5,7 +5,9 @@
 //   * neighbour reads are linear: byte index y*stride + x + dx, so the left neighbour of column 0
 //     is the last byte of the previous row's stride, and indices outside [0, h*stride) read 0
 //     (the reference's zero guard bytes).
-// Kernels never write padding bytes, so a plane allocated zeroed keeps zero padding forever.
+// Kernels never write padding bytes, so a plane allocated zeroed keeps zero padding, with one
+// exception copied from the reference: the warp and remap kernels store 4 pixels at a time and so
+// write pixels w..ceil4(w)-1 of every row, as the reference's SSE loops do.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -1,0 +1,303 @@
+// warp.hip -- bilinear WarpAffine, WarpPerspective and Remap (U8 -> U8, UNDEFINED and CONSTANT
+// borders) and the float -> fixed-point remap table conversion, for gfx950.
+//
+// Each thread produces four consecutive output pixels and writes them with one 32-bit store, the
+// reference's store unit: rows are written up to ceil(w/4)*4 pixels, so pixels w..ceil4(w)-1 land
+// in the row padding exactly as the reference's SSE loops leave them there (a 3x3 node reading
+// the output with linear-address neighbours then sees the same bytes).  The source is read at
+// linear byte addresses; addresses outside [0, h*stride) read 0, the reference's zero guard.
+// All float arithmetic is single precision, uncontracted and in the reference's order; the
+// x86 conversion semantics (truncation / round-to-nearest-even, 0x80000000 for values out of int32
+// range) are reproduced explicitly because the GPU's conversions saturate instead.
+#include "mvx_common.hpp"
+
+namespace mvx {
+
+namespace {
+
+MVX_DEV uint32_t ldg8(const uint8_t *p, uint64_t n, int64_t i) { return (i < 0 || (uint64_t)i >= n) ? 0u : (uint32_t)p[i]; }
+
+// _mm_cvttps_epi32
+MVX_DEV int32_t cvtt(float v)
+{
+    if (!(v > -2147483904.0f && v < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)v;
+}
+
+// _mm_cvtps_epi32 (MXCSR default: nearest, ties to even)
+MVX_DEV int32_t cvtn(float v)
+{
+    if (!(v > -2147483904.0f && v < 2147483648.0f)) return INT32_MIN;
+    return __float2int_rn(v);
+}
+
+MVX_DEV uint32_t sat8(int32_t v) { return (uint32_t)min(max(v, 0), 255); }
+
+struct Affine { float r00, r10, r01, r11, c1, c2; };
+struct Persp { float a, d, g, b, e, h, c, f, i; };
+
+}  // namespace
+
+// HafCpu_WarpAffine_U8_U8_Bilinear[_Constant] (ago_haf_cpu_geometric.cpp:825-1370).
+// BORDER: the corner test failed, run the masked loop with the offset clamped to [0, limit]
+// (limit = H*stride-1 for CONSTANT, (H-1)*stride-1 for UNDEFINED); otherwise the plain loop.
+template <bool BORDER>
+__global__ void __launch_bounds__(256)
+k_warp_affine(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+              const uint8_t *__restrict__ src, uint32_t srcStride, Affine m, int32_t limit, uint32_t border)
+{
+    const uint32_t x0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const uint64_t n = (uint64_t)sh * srcStride;
+    const float fw = (float)sw, fh = (float)sh;
+    const float xd = (float)y * m.r01 + m.c1, yd = (float)y * m.r11 + m.c2;
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float xs = m.r00 * (float)(x0 + k) + xd, ys = m.r10 * (float)(x0 + k) + yd;
+        const int32_t xi = cvtt(xs), yi = cvtt(ys);
+        const float fx = xs - (float)xi, fy = ys - (float)yi;
+        int32_t off = (int32_t)((uint32_t)yi * srcStride + (uint32_t)xi);
+        if (BORDER) off = max(min(off, limit), 0);
+        const float p0 = (float)ldg8(src, n, off), p1 = (float)ldg8(src, n, (int64_t)off + 1);
+        const float p2 = (float)ldg8(src, n, (int64_t)off + srcStride), p3 = (float)ldg8(src, n, (int64_t)off + srcStride + 1);
+        const float omx = 1.0f - fx, omy = 1.0f - fy;
+        const float a = (p0 * omx) * omy, b = (p1 * fx) * omy, c = (p2 * omx) * fy, d = (p3 * fx) * fy;
+        const float v = (a + b) + (c + d);
+        uint32_t px = sat8(cvtn(v));
+        if (BORDER && !(xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh)) px = border;
+        word |= px << (8 * k);
+    }
+    *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
+// HafCpu_WarpPerspective_U8_U8_Bilinear_Constant (ago_haf_cpu_geometric.cpp:1579-1873); the
+// UNDEFINED variant is the same with border 0 (:1560-1577).
+__global__ void __launch_bounds__(256)
+k_warp_perspective(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                   const uint8_t *__restrict__ src, uint32_t srcStride, Persp m, uint32_t border)
+{
+    const uint32_t x0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const uint64_t n = (uint64_t)sh * srcStride;
+    const int32_t limit = (int32_t)(sh * srcStride - 1u);
+    const float xd = (float)y * m.b + m.c, yd = (float)y * m.e + m.f, zd = (float)y * m.h + m.i;
+    const float bf = (float)border;
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float xf = (float)(x0 + k);
+        const float z = 1.0f / (m.g * xf + zd);
+        const float xs = (m.a * xf + xd) * z, ys = (m.d * xf + yd) * z;
+        const int32_t xi = (int32_t)((uint32_t)cvtt(xs) - (uint32_t)(xs < 0.0f));
+        const int32_t yi = (int32_t)((uint32_t)cvtt(ys) - (uint32_t)(ys < 0.0f));
+        const bool mask = xi >= 0 && xi < (int32_t)sw && yi >= 0 && yi < (int32_t)sh;
+        const bool mask1 = xi >= -1 && xi < (int32_t)sw && yi >= -1 && yi < (int32_t)sh;
+        const float fx = xs - (float)xi, fy = ys - (float)yi;
+        int32_t off = (int32_t)((uint32_t)yi * srcStride + (uint32_t)xi);
+        off = max(min(off, limit), 0);
+        const float p0 = mask ? (float)ldg8(src, n, off) : bf;
+        const float p1 = mask1 ? (float)ldg8(src, n, (int64_t)off + 1) : bf;
+        const float p2 = mask ? (float)ldg8(src, n, (int64_t)off + srcStride) : bf;
+        const float p3 = mask1 ? (float)ldg8(src, n, (int64_t)off + srcStride + 1) : bf;
+        const float omx = 1.0f - fx, omy = 1.0f - fy;
+        float v = (p0 * omx) * omy;
+        v = v + (p1 * fx) * omy;
+        v = v + (p2 * omx) * fy;
+        v = v + (p3 * fx) * fy;
+        word |= sat8(cvtt(v)) << (8 * k);
+    }
+    *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
+// HafCpu_Remap_U8_U8_Bilinear[_Constant] (ago_haf_cpu_geometric.cpp:249-504).  The map is read at
+// byte y*mapStride + 4x for every written pixel (pixels past w use the following entries, as the
+// reference's 16-byte map loads do); entries past the end of the map read as (0xffff, 0xffff).
+template <bool CONSTANT>
+__global__ void __launch_bounds__(256)
+k_remap(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sh,
+        const uint8_t *__restrict__ src, uint32_t srcStride, const uint8_t *__restrict__ map, uint32_t mapStride,
+        uint64_t mapBytes, uint32_t border)
+{
+    const uint32_t x0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const uint64_t n = (uint64_t)sh * srcStride;
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t mo = (uint64_t)y * mapStride + 4ull * (x0 + k);
+        const uint32_t e = (mo + 4 <= mapBytes) ? *reinterpret_cast<const uint32_t *>(map + mo) : 0xffffffffu;
+        uint32_t mx = e & 0xffffu, my = e >> 16;
+        const bool xbad = mx == 0xffffu;
+        if (mx == 0xffffu) mx = CONSTANT ? 0u : 8u;
+        if (my == 0xffffu) my = CONSTANT ? 0u : 8u;
+        const int32_t fx = (int32_t)(mx & 7u), fy = (int32_t)(my & 7u);
+        const int64_t o = (int64_t)(my >> 3) * srcStride + (int64_t)(mx >> 3);
+        const int32_t p00 = (int32_t)ldg8(src, n, o), p10 = (int32_t)ldg8(src, n, o + 1);
+        const int32_t p01 = (int32_t)ldg8(src, n, o + srcStride), p11 = (int32_t)ldg8(src, n, o + srcStride + 1);
+        int32_t v = (p00 * (8 - fx) * (8 - fy) + p10 * fx * (8 - fy) + p01 * (8 - fx) * fy + p11 * fx * fy + 32) >> 6;
+        if (CONSTANT && xbad) v = (int32_t)border;
+        word |= sat8(v) << (8 * k);
+    }
+    *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
+// vxSetRemapPoint (vx_api.cpp:7865-7894) for every entry: x = (u16)(sx*8 + 0.5f), y likewise,
+// (0xffff, 0xffff) when sx < 0, sy < 0, sx >= W-1 or sy >= H-1 (3 fractional bits: sources below
+// 8192 pixels, ago_util.cpp:2450-2459).
+__global__ void __launch_bounds__(256)
+k_remap_fixed(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dstMap, uint32_t dstStride, const uint8_t *__restrict__ srcMap,
+              uint32_t srcStride, float limX, float limY)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= dw || y >= dh) return;
+    const float2 s = *reinterpret_cast<const float2 *>(srcMap + (uint64_t)y * srcStride + 8ull * x);
+    uint32_t fx = (uint32_t)(uint16_t)(int32_t)(s.x * 8.0f + 0.5f), fy = (uint32_t)(uint16_t)(int32_t)(s.y * 8.0f + 0.5f);
+    if (s.x < 0.0f || s.y < 0.0f || s.x >= limX || s.y >= limY) fx = fy = 0xffffu;
+    *reinterpret_cast<uint32_t *>(dstMap + (uint64_t)y * dstStride + 4ull * x) = fx | (fy << 16);
+}
+
+namespace {
+
+bool warp_args_ok(uint32_t dw, uint32_t dh, const uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                  const uint8_t *src, uint32_t srcStride)
+{
+    if (!dst || !src || !dw || !dh || !sw || !sh) return false;
+    if (((uintptr_t)dst & 3u) || (dstStride & 3u) || dstStride < ((dw + 3u) & ~3u)) return false;
+    if (srcStride < sw || (uint64_t)sh * srcStride >= (1ull << 31)) return false;
+    return true;
+}
+
+dim3 warp_grid(uint32_t dw, uint32_t dh, dim3 block) { return dim3((((dw + 3) / 4) + block.x - 1) / block.x, (dh + block.y - 1) / block.y); }
+
+// the corner test of ago_haf_cpu_geometric.cpp:1142-1155 (host float arithmetic, no contraction)
+bool affine_needs_border(uint32_t dw, uint32_t dh, uint32_t sw, uint32_t sh, const Affine &m)
+{
+    const float fw = (float)sw, fh = (float)sh, w = (float)dw, h = (float)dh;
+    bool b = (m.c1 < 0) | (m.c2 < 0) | (m.c1 >= fw) | (m.c2 >= fh);
+    float x1 = m.r00 * w + m.c1, y1 = m.r10 * w + m.c2;
+    b |= (x1 < 0) | (y1 < 0) | (x1 >= fw) | (y1 >= fh);
+    x1 = m.r01 * h + m.c1;
+    y1 = m.r11 * h + m.c2;
+    b |= (x1 < 0) | (y1 < 0) | (x1 >= fw) | (y1 >= fh);
+    x1 = m.r00 * w + m.r01 * h + m.c1;
+    y1 = m.r10 * w + m.r11 * h + m.c2;
+    b |= (x1 < 0) | (y1 < 0) | (x1 >= fw) | (y1 >= fh);
+    return b;
+}
+
+int warp_affine(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                const uint8_t *src, uint32_t srcStride, const float *matrix, bool constant, uint32_t border)
+{
+    if (!matrix || !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride)) return MVX_ERROR_INVALID_PARAMETERS;
+    const Affine m{matrix[0], matrix[1], matrix[2], matrix[3], matrix[4], matrix[5]};
+    const int32_t limit = constant ? (int32_t)(sh * srcStride - 1u) : (int32_t)((sh - 1u) * srcStride - 1u);
+    const dim3 block(64, 4);
+    if (affine_needs_border(dw, dh, sw, sh, m))
+        hipLaunchKernelGGL(k_warp_affine<true>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw,
+                           sh, src, srcStride, m, limit, constant ? border : 0u);
+    else
+        hipLaunchKernelGGL(k_warp_affine<false>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw,
+                           sh, src, srcStride, m, limit, 0u);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+int warp_perspective(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                     const uint8_t *src, uint32_t srcStride, const float *matrix, uint32_t border)
+{
+    if (!matrix || !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride)) return MVX_ERROR_INVALID_PARAMETERS;
+    const Persp m{matrix[0], matrix[1], matrix[2], matrix[3], matrix[4], matrix[5], matrix[6], matrix[7], matrix[8]};
+    const dim3 block(64, 4);
+    hipLaunchKernelGGL(k_warp_perspective, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src,
+                       srcStride, m, border);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+int remap(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+          const uint8_t *src, uint32_t srcStride, const void *map, uint32_t mapStride, bool constant, uint32_t border)
+{
+    if (!map || ((uintptr_t)map & 3u) || (mapStride & 3u) || mapStride < 4u * dw ||
+        !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    const dim3 block(64, 4);
+    const uint64_t mapBytes = (uint64_t)dh * mapStride;
+    if (constant)
+        hipLaunchKernelGGL(k_remap<true>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sh, src,
+                           srcStride, (const uint8_t *)map, mapStride, mapBytes, border);
+    else
+        hipLaunchKernelGGL(k_remap<false>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sh, src,
+                           srcStride, (const uint8_t *)map, mapStride, mapBytes, 0u);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+}  // namespace
+}  // namespace mvx
+
+using namespace mvx;
+
+extern "C" int mvx_WarpAffine_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                             uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                             uint32_t srcStride, const float *matrix)
+{
+    return warp_affine(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, false, 0);
+}
+
+extern "C" int mvx_WarpAffine_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                      uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                                      const uint8_t *pSrc, uint32_t srcStride, const float *matrix,
+                                                      uint8_t borderValue)
+{
+    return warp_affine(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, true, borderValue);
+}
+
+extern "C" int mvx_WarpPerspective_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                  uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                                  uint32_t srcStride, const float *matrix)
+{
+    return warp_perspective(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, 0);
+}
+
+extern "C" int mvx_WarpPerspective_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                                           uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth,
+                                                           uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride,
+                                                           const float *matrix, uint8_t borderValue)
+{
+    return warp_perspective(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, borderValue);
+}
+
+extern "C" int mvx_Remap_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                        uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                        uint32_t srcStride, const void *pMap, uint32_t mapStride)
+{
+    return remap(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, false, 0);
+}
+
+extern "C" int mvx_Remap_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                 uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                                 uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue)
+{
+    return remap(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, true, borderValue);
+}
+
+extern "C" int mvx_Remap_FixedMap_F32(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, void *pFixedMap,
+                                      uint32_t fixedStride, const float *pFloatMap, uint32_t floatStride, uint32_t srcWidth,
+                                      uint32_t srcHeight)
+{
+    if (!pFixedMap || !pFloatMap || !dstWidth || !dstHeight || srcWidth < 2 || srcHeight < 2 || srcWidth >= 8192 || srcHeight >= 8192)
+        return MVX_ERROR_INVALID_PARAMETERS;
+    if (((uintptr_t)pFixedMap & 3u) || ((uintptr_t)pFloatMap & 7u) || (fixedStride & 3u) || (floatStride & 7u) ||
+        fixedStride < 4u * dstWidth || floatStride < 8u * dstWidth)
+        return MVX_ERROR_INVALID_PARAMETERS;
+    const dim3 block(64, 4), grid((dstWidth + 63) / 64, (dstHeight + 3) / 4);
+    hipLaunchKernelGGL(k_remap_fixed, grid, block, 0, stream_of(stream), dstWidth, dstHeight, (uint8_t *)pFixedMap, fixedStride,
+                       (const uint8_t *)pFloatMap, floatStride, (float)(srcWidth - 1), (float)(srcHeight - 1));
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}

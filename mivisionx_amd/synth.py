@@ -13,6 +13,11 @@ Workloads
   U(16,512), painted in order), then N(0, 8^2) noise by Box-Muller, rounded half-to-even and
   clipped to [0,255] (the 3840x2160 vision graph input).
 * ``scene_u8(w, h, seed)``: the same process for one channel.
+* ``warp_matrices(w, h, seed)``: a near-identity affine (2x3) and perspective (3x3) VX matrix about
+  the image centre -- rotation U(-5,5) degrees, scale U(0.9,1.1), translation U(-50,50) px,
+  perspective terms U(-1e-4,1e-4) -- as the float32 arrays the VX matrix objects store.
+* ``barrel_map(w, h, k1)``: the float remap table of a barrel distortion r' = r(1 + k1 r^2) about
+  the centre (r normalised by the half-diagonal): (h, w, 2) float32 source (x, y) per output pixel.
 """
 import numpy as np
 
@@ -107,3 +112,31 @@ def scene_rgb(w, h, seed=0):
 def scene_u8(w, h, seed=0):
     """(h, w) uint8 single-channel scene."""
     return _scene(w, h, seed, 1)[0]
+
+
+def warp_matrices(w, h, seed=0):
+    """(affine 6 floats, perspective 9 floats) in the VX storage order: x_src = m0*x + m2*y + m4,
+    y_src = m1*x + m3*y + m5; perspective (a d g / b e h / c f i) with z = g*x + h*y + i."""
+    p = uniform01(seed, 500, 8)
+    ang = np.deg2rad(-5.0 + 10.0 * p[0])
+    sc = 0.9 + 0.2 * p[1]
+    tx, ty = -50.0 + 100.0 * p[2], -50.0 + 100.0 * p[3]
+    g, hh = -1e-4 + 2e-4 * p[4], -1e-4 + 2e-4 * p[5]
+    c, s = np.cos(ang) * sc, np.sin(ang) * sc
+    cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+    # rotate/scale about the centre, then translate
+    a00, a01, a10, a11 = c, -s, s, c
+    bx = cx - (a00 * cx + a01 * cy) + tx
+    by = cy - (a10 * cx + a11 * cy) + ty
+    affine = np.array([a00, a10, a01, a11, bx, by], np.float32)
+    persp = np.array([a00, a10, g, a01, a11, hh, bx, by, 1.0 - g * cx - hh * cy], np.float32)
+    return affine, persp
+
+
+def barrel_map(w, h, k1):
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+    rn = np.hypot(cx, cy)
+    dx, dy = (xx - cx) / rn, (yy - cy) / rn
+    f = 1.0 + k1 * (dx * dx + dy * dy)
+    return np.ascontiguousarray(np.stack([cx + dx * f * rn, cy + dy * f * rn], -1).astype(np.float32))

@@ -67,6 +67,17 @@ int oracle_nonmax_xys(int w, int h, const float *img, int stride, oracle_xys_t *
 int oracle_harris_pick(oracle_xys_t *list, int n, float min_distance, int width, int height,
                        oracle_keypoint_t *out, int capacity);
 
+/* ---- warps and remap (warp.c); m is the VX matrix as stored (2x3 affine: 6 floats, 3x3: 9) ---- */
+int oracle_warp_affine_needs_border(int dw, int dh, int sw, int sh, const float m[6]);
+void oracle_warp_affine_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                 int src_stride, const float m[6], int constant, int border, uint8_t *unpinned,
+                                 int unpinned_stride);
+void oracle_warp_perspective_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                      int src_stride, const float m[9], int border, uint8_t *unpinned, int unpinned_stride);
+void oracle_remap_fixed(int dw, int dh, int sw, int sh, const float *map, uint16_t *fixed);
+void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                           int src_stride, const uint16_t *fixed, int constant, int border);
+
 /* ---- nodes and graphs on reference-layout planes (graphs.c); packed in/out ---- */
 void oracle_filter3x3(int which, int w, int h, const uint8_t *src, uint8_t *dst);
 void oracle_sobel_node(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy);

@@ -51,6 +51,19 @@ def split_levels(flat, dims):
     return out
 
 
+def _plane(src):
+    """a packed U8 image in the reference layout: stride ALIGN16(w), zero padding"""
+    h, w = src.shape
+    st = (w + 15) & ~15
+    out = np.zeros((h, st), np.uint8)
+    out[:, :w] = src
+    return out, st
+
+
+def _f32(m):
+    return np.ascontiguousarray(np.asarray(m, np.float32).reshape(-1))
+
+
 class Oracle:
     def __init__(self):
         if not os.path.exists(ORACLE_SO):
@@ -162,6 +175,45 @@ class Oracle:
     def harris_norm(self, gradient_size=3):
         return self.lib.oracle_harris_norm(gradient_size)
 
+    # ---- warps and remap ----
+    def warp_affine(self, src, dw, dh, m, constant=False, border=0, padded=False):
+        """(dst, unpinned); m = the 2x3 VX matrix as stored (6 floats); padded: return the full
+        ALIGN16(dw) rows, including the padding pixels the reference's 4-pixel stores write"""
+        sh, sw = src.shape
+        s, ss = _plane(src)
+        ds = (dw + 15) & ~15
+        d = np.zeros((dh, ds), np.uint8)
+        u = np.zeros((dh, dw), np.uint8)
+        self.lib.oracle_warp_affine_bilinear(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(_f32(m)), int(constant), border,
+                                             _p(u), dw)
+        return (d if padded else np.ascontiguousarray(d[:, :dw])), u.astype(bool)
+
+    def warp_perspective(self, src, dw, dh, m, border=0, padded=False):
+        sh, sw = src.shape
+        s, ss = _plane(src)
+        ds = (dw + 15) & ~15
+        d = np.zeros((dh, ds), np.uint8)
+        u = np.zeros((dh, dw), np.uint8)
+        self.lib.oracle_warp_perspective_bilinear(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(_f32(m)), border, _p(u), dw)
+        return (d if padded else np.ascontiguousarray(d[:, :dw])), u.astype(bool)
+
+    def remap_fixed(self, mapxy, sw, sh):
+        """vxSetRemapPoint's u16 (x, y) map from a float (dh, dw, 2) map"""
+        dh, dw = mapxy.shape[:2]
+        fixed = np.zeros((dh, dw, 2), np.uint16)
+        self.lib.oracle_remap_fixed(dw, dh, sw, sh, _p(_f32(mapxy)), _p(fixed))
+        return fixed
+
+    def remap(self, src, mapxy, constant=False, border=0, padded=False):
+        sh, sw = src.shape
+        dh, dw = mapxy.shape[:2]
+        fixed = self.remap_fixed(mapxy, sw, sh)
+        s, ss = _plane(src)
+        ds = (dw + 15) & ~15
+        d = np.zeros((dh, ds), np.uint8)
+        self.lib.oracle_remap_bilinear(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(fixed), int(constant), border)
+        return d if padded else np.ascontiguousarray(d[:, :dw])
+
     # ---- the 4K vision graph ----
     def graph2(self, rgb, dw, dh, levels, lower, upper, k, thr, dist, capacity):
         y = self.color_y(rgb)
@@ -264,3 +316,16 @@ class Reference:
         if time_iters:
             out["sec_per_frame"] = sec.value
         return out
+
+    def warp(self, kind, src, dw, dh, m, constant=False, border=0):
+        sh, sw = src.shape
+        d = np.zeros((dh, dw), np.uint8)
+        self._ok(self.lib.ref_warp(kind, sw, sh, _p(_c(src)), dw, dh, _p(_f32(m)), int(constant), border, _p(d)), "warp")
+        return d
+
+    def remap(self, src, mapxy, constant=False, border=0):
+        sh, sw = src.shape
+        dh, dw = mapxy.shape[:2]
+        d = np.zeros((dh, dw), np.uint8)
+        self._ok(self.lib.ref_remap(sw, sh, _p(_c(src)), dw, dh, _p(_f32(mapxy)), int(constant), border, _p(d)), "remap")
+        return d

@@ -284,4 +284,51 @@ int ref_graph2(int w, int h, const uint8_t *rgb, int dw, int dh, uint8_t *scaled
     return 0;
 }
 
+// WarpAffine (kind 0, m = 2x3 VX matrix, 6 floats) / WarpPerspective (kind 1, 3x3, 9 floats), bilinear;
+// border UNDEFINED (constant == 0) or CONSTANT with `border`.
+int ref_warp(int kind, int sw, int sh, const uint8_t *src, int dw, int dh, const float *m, int constant, int border,
+             uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    vx_matrix mat = vxCreateMatrix(c.ctx, VX_TYPE_FLOAT32, kind ? 3 : 2, 3);
+    CHECK(vxCopyMatrix(mat, (void *)m, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    vx_node n = kind ? vxWarpPerspectiveNode(c.graph, in, mat, VX_INTERPOLATION_BILINEAR, out)
+                     : vxWarpAffineNode(c.graph, in, mat, VX_INTERPOLATION_BILINEAR, out);
+    if (!n) return -1;
+    if (constant) {
+        vx_border_t b;
+        memset(&b, 0, sizeof(b));
+        b.mode = VX_BORDER_CONSTANT;
+        b.constant_value.U8 = (vx_uint8)border;
+        CHECK(vxSetNodeAttribute(n, VX_NODE_BORDER, &b, sizeof(b)));
+    }
+    int s = run(c); if (s) return s;
+    CHECK(get(out, dw, dh, dst, 1));
+    return 0;
+}
+
+// Remap bilinear from a float map (dw*dh {x, y} pairs, set through vxCopyRemapPatch); border as above.
+int ref_remap(int sw, int sh, const uint8_t *src, int dw, int dh, const float *map, int constant, int border,
+              uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    vx_remap rm = vxCreateRemap(c.ctx, sw, sh, dw, dh);
+    vx_rectangle_t r = {0, 0, (vx_uint32)dw, (vx_uint32)dh};
+    CHECK(vxCopyRemapPatch(rm, &r, (vx_size)dw * 8, (void *)map, VX_TYPE_COORDINATES2DF, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    vx_node n = vxRemapNode(c.graph, in, rm, VX_INTERPOLATION_BILINEAR, out);
+    if (!n) return -1;
+    if (constant) {
+        vx_border_t b;
+        memset(&b, 0, sizeof(b));
+        b.mode = VX_BORDER_CONSTANT;
+        b.constant_value.U8 = (vx_uint8)border;
+        CHECK(vxSetNodeAttribute(n, VX_NODE_BORDER, &b, sizeof(b)));
+    }
+    int s = run(c); if (s) return s;
+    CHECK(get(out, dw, dh, dst, 1));
+    return 0;
+}
+
 }  // extern "C"

@@ -1,0 +1,207 @@
+/*
+ * warp.c -- CPU restatement of the reference's bilinear WarpAffine, WarpPerspective and Remap
+ * (U8 -> U8, UNDEFINED and CONSTANT borders) and of vxSetRemapPoint's fixed-point map
+ * (TEST INFRASTRUCTURE: the checker, never the product).
+ *
+ * Memory model: see oracle.h.  Source bytes are read LINEARLY at y*stride + x; reads outside
+ * [0, h*stride) return 0 (the reference's zero guard).  The reference reads up to one row past the
+ * end of the source plane in some cases (the second bilinear row of the last source row); beyond
+ * its 64 guard bytes that is heap memory no restatement can know.  Where such a read carries a
+ * non-zero weight the pixel is flagged in `unpinned` (parity unpinned, masked in comparisons).
+ *
+ * Every output row writes ceil(w/4)*4 pixels, as the reference's 4-pixel stores do: pixels w..
+ * ceil4(w)-1 land in the row's padding (stride >= ALIGN16(w)).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <string.h>
+
+#define GUARD 64
+
+static inline int rd(const uint8_t *p, long n, long i) { return (i < 0 || i >= n) ? 0 : p[i]; }
+static inline int beyond(long n, long i) { return i < -GUARD || i >= n + GUARD; }
+static inline uint8_t sat8(int v) { return (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v); }
+
+/* cvttps_epi32: truncation; out-of-range and NaN give 0x80000000 */
+static inline int32_t cvtt(float v)
+{
+    if (!(v > -2147483904.0f && v < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)v;
+}
+
+/* cvtps_epi32 with the default rounding (to nearest, ties to even) */
+static inline int32_t cvtn(float v)
+{
+    if (!(v > -2147483904.0f && v < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)rintf(v);
+}
+
+static inline int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+static inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+
+/* The corner test of ago_haf_cpu_geometric.cpp:1142-1155: true when some corner of the output maps
+ * outside [0,W) x [0,H) (then the masked, clamped loop runs; otherwise the plain loop). */
+int oracle_warp_affine_needs_border(int dw, int dh, int sw, int sh, const float m[6])
+{
+    const float r00 = m[0], r10 = m[1], r01 = m[2], r11 = m[3], c1 = m[4], c2 = m[5];
+    const float fw = (float)sw, fh = (float)sh;
+    int b = (c1 < 0) | (c2 < 0) | (c1 >= fw) | (c2 >= fh);
+    float x1 = r00 * (float)dw + c1, y1 = r10 * (float)dw + c2;
+    b |= (x1 < 0) | (y1 < 0) | (x1 >= fw) | (y1 >= fh);
+    x1 = r01 * (float)dh + c1;
+    y1 = r11 * (float)dh + c2;
+    b |= (x1 < 0) | (y1 < 0) | (x1 >= fw) | (y1 >= fh);
+    x1 = r00 * (float)dw + r01 * (float)dh + c1;
+    y1 = r10 * (float)dw + r11 * (float)dh + c2;
+    b |= (x1 < 0) | (y1 < 0) | (x1 >= fw) | (y1 >= fh);
+    return b;
+}
+
+/* HafCpu_WarpAffine_U8_U8_Bilinear (ago_haf_cpu_geometric.cpp:825-1099) when constant == 0 and
+ * HafCpu_WarpAffine_U8_U8_Bilinear_Constant (:1100-1370) when constant != 0.  m is the VX 2x3
+ * matrix as stored (ago_affine_matrix_t, matrix[3][2] row-major): xs = m0*x + m2*y + m4,
+ * ys = m1*x + m3*y + m5, evaluated as fl(fl(m0*x) + fl(fl(y*m2) + m4)).
+ * Border loop (any corner outside): mask = 0 <= xs < W && 0 <= ys < H; xi = trunc(xs),
+ * fx = xs - xi; linear offset yi*stride + xi (32-bit wrap) clamped to [0, limit] with limit =
+ * H*stride - 1 (CONSTANT) or (H-1)*stride - 1 (UNDEFINED); pixels p0 = [off], p1 = [off+1],
+ * p2 = [off+stride], p3 = [off+stride+1];
+ *   v = ((p0*(1-fx))*(1-fy) + (p1*fx)*(1-fy)) + ((p2*(1-fx))*fy + (p3*fx)*fy), rounded to nearest
+ * even and saturated; outside the mask the border value (UNDEFINED: 0).  Plain loop: the same
+ * without mask and clamp.
+ * Pinned by: bit-exact against oracle/_ref (both borders, both loops, pixels flagged unpinned
+ * masked); identity and integer-translation closed forms. */
+void oracle_warp_affine_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                 int src_stride, const float m[6], int constant, int border, uint8_t *unpinned,
+                                 int unpinned_stride)
+{
+    const float r00 = m[0], r10 = m[1], r01 = m[2], r11 = m[3], c1 = m[4], c2 = m[5];
+    const long n = (long)sh * src_stride;
+    const int bb = oracle_warp_affine_needs_border(dw, dh, sw, sh, m);
+    const int32_t limit = constant ? (int32_t)((uint32_t)sh * (uint32_t)src_stride - 1u)
+                                   : (int32_t)((uint32_t)(sh - 1) * (uint32_t)src_stride - 1u);
+    const float fw = (float)sw, fh = (float)sh;
+    const int wq = (dw + 3) & ~3;
+    for (int y = 0; y < dh; y++) {
+        const float xd = (float)y * r01 + c1, yd = (float)y * r11 + c2;
+        for (int x = 0; x < wq; x++) {
+            const float xs = r00 * (float)x + xd, ys = r10 * (float)x + yd;
+            const int inside = xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh;
+            const int32_t xi = cvtt(xs), yi = cvtt(ys);
+            const float fx = xs - (float)xi, fy = ys - (float)yi;
+            int32_t off = (int32_t)((uint32_t)yi * (uint32_t)src_stride + (uint32_t)xi);
+            if (bb) off = imax(imin(off, limit), 0);
+            const long o = off;
+            const float p0 = (float)rd(src, n, o), p1 = (float)rd(src, n, o + 1);
+            const float p2 = (float)rd(src, n, o + src_stride), p3 = (float)rd(src, n, o + src_stride + 1);
+            const float omx = 1.0f - fx, omy = 1.0f - fy;
+            const float a = (p0 * omx) * omy, b = (p1 * fx) * omy, c = (p2 * omx) * fy, d = (p3 * fx) * fy;
+            const float v = (a + b) + (c + d);
+            const int keep = !bb || inside;
+            dst[(long)y * dst_stride + x] = keep ? sat8(cvtn(v)) : (uint8_t)(constant ? border : 0);
+            if (unpinned && x < dw)
+                unpinned[(long)y * unpinned_stride + x] =
+                    keep && ((beyond(n, o + 1) && fx != 0.0f) || (beyond(n, o + src_stride) && fy != 0.0f) ||
+                             (beyond(n, o + src_stride + 1) && fx != 0.0f && fy != 0.0f) || beyond(n, o));
+        }
+    }
+}
+
+/* HafCpu_WarpPerspective_U8_U8_Bilinear_Constant (ago_haf_cpu_geometric.cpp:1579-1873; the
+ * UNDEFINED variant :1560-1577 calls it with border 0).  m is the VX 3x3 matrix as stored
+ * (matrix[3][3] row-major; a=m0 d=m1 g=m2 b=m3 e=m4 h=m5 c=m6 f=m7 i=m8):
+ *   z = 1 / fl(g*x + fl(y*h + i)); xs = fl(a*x + fl(y*b + c)) * z; ys = fl(d*x + fl(y*e + f)) * z;
+ *   xi = trunc(xs) - (xs < 0) (so -1.0 gives -2, as the reference does), fx = xs - xi;
+ *   mask  = 0 <= xi < W && 0 <= yi < H  selects p0, p2 (else border);
+ *   mask1 = -1 <= xi < W && -1 <= yi < H selects p1, p3 (else border);
+ *   offset yi*stride + xi clamped to [0, H*stride - 1];
+ *   v = (((p0*(1-fx))*(1-fy) + (p1*fx)*(1-fy)) + (p2*(1-fx))*fy) + (p3*fx)*fy, TRUNCATED, saturated.
+ * The corner test is disabled in the reference (:1628-1640, bBoder = 1): the masked loop always runs.
+ * Pinned by: bit-exact against oracle/_ref (pixels flagged unpinned masked); the identity matrix
+ * reproduces the source (x + 0 weights). */
+void oracle_warp_perspective_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                      int src_stride, const float m[9], int border, uint8_t *unpinned, int unpinned_stride)
+{
+    const float a = m[0], d = m[1], g = m[2], b = m[3], e = m[4], hh = m[5], c = m[6], f = m[7], ii = m[8];
+    const long n = (long)sh * src_stride;
+    const int32_t limit = (int32_t)((uint32_t)sh * (uint32_t)src_stride - 1u);
+    const int wq = (dw + 3) & ~3;
+    for (int y = 0; y < dh; y++) {
+        const float xd = (float)y * b + c, yd = (float)y * e + f, zd = (float)y * hh + ii;
+        for (int x = 0; x < wq; x++) {
+            const float z = 1.0f / ((g * (float)x) + zd);
+            const float xs = ((a * (float)x) + xd) * z, ys = ((d * (float)x) + yd) * z;
+            const int32_t xi = (int32_t)((uint32_t)cvtt(xs) - (uint32_t)(xs < 0.0f));
+            const int32_t yi = (int32_t)((uint32_t)cvtt(ys) - (uint32_t)(ys < 0.0f));
+            const int mask = xi >= 0 && xi < sw && yi >= 0 && yi < sh;
+            const int mask1 = xi >= -1 && xi < sw && yi >= -1 && yi < sh;
+            const float fx = xs - (float)xi, fy = ys - (float)yi;
+            int32_t off = (int32_t)((uint32_t)yi * (uint32_t)src_stride + (uint32_t)xi);
+            off = imax(imin(off, limit), 0);
+            const long o = off;
+            const float p0 = (float)(mask ? rd(src, n, o) : border), p1 = (float)(mask1 ? rd(src, n, o + 1) : border);
+            const float p2 = (float)(mask ? rd(src, n, o + src_stride) : border);
+            const float p3 = (float)(mask1 ? rd(src, n, o + src_stride + 1) : border);
+            const float omx = 1.0f - fx, omy = 1.0f - fy;
+            float v = (p0 * omx) * omy;
+            v = v + (p1 * fx) * omy;
+            v = v + (p2 * omx) * fy;
+            v = v + (p3 * fx) * fy;
+            dst[(long)y * dst_stride + x] = sat8(cvtt(v));
+            if (unpinned && x < dw)
+                unpinned[(long)y * unpinned_stride + x] =
+                    (mask1 && beyond(n, o + 1) && fx != 0.0f) || (mask && beyond(n, o + src_stride) && fy != 0.0f) ||
+                    (mask1 && beyond(n, o + src_stride + 1) && fx != 0.0f && fy != 0.0f);
+        }
+    }
+}
+
+/* vxSetRemapPoint (vx_api.cpp:7865-7894) for source images below 8192 pixels (3 fractional bits,
+ * ago_util.cpp:2450-2459): x = (u16)(sx*8 + 0.5f), y likewise, and (0xffff, 0xffff) when
+ * sx < 0, sy < 0, sx >= W-1 or sy >= H-1.  map holds dw*dh {x, y} float pairs; fixed receives
+ * dw*dh {x, y} u16 pairs.  Pinned by: the GPU conversion and the remap outputs against oracle/_ref. */
+void oracle_remap_fixed(int dw, int dh, int sw, int sh, const float *map, uint16_t *fixed)
+{
+    for (long i = 0; i < (long)dw * dh; i++) {
+        const float sx = map[2 * i], sy = map[2 * i + 1];
+        uint16_t fx = (uint16_t)(int)(sx * 8.0f + 0.5f), fy = (uint16_t)(int)(sy * 8.0f + 0.5f);
+        if (sx < 0.0f || sy < 0.0f || sx >= (float)(sw - 1) || sy >= (float)(sh - 1)) fx = fy = 0xffff;
+        fixed[2 * i] = fx;
+        fixed[2 * i + 1] = fy;
+    }
+}
+
+/* HafCpu_Remap_U8_U8_Bilinear (ago_haf_cpu_geometric.cpp:249-370, constant == 0) and
+ * HafCpu_Remap_U8_U8_Bilinear_Constant (:372-504, constant != 0).  Each 16-bit map lane equal to
+ * 0xffff is replaced by 8 (UNDEFINED: the point (1,1) with zero fraction) or 0 (CONSTANT).
+ * fx = v & 7, xi = v >> 3 (rows alike); offset yi*stride + xi (no clamp);
+ *   out = (p00*(8-fx)*(8-fy) + p10*fx*(8-fy) + p01*(8-fx)*fy + p11*fx*fy + 32) >> 6.
+ * CONSTANT: the result is replaced by the border value when the X lane was 0xffff (the reference
+ * masks the 32-bit result with the 16-bit compare mask, so only the x lane counts).  The map has
+ * dw entries per row (stride dw*4 bytes); pixels w..ceil4(w)-1 of a row use the next row's
+ * entries; past the last entry the map is read as (0xffff, 0xffff).
+ * Pinned by: bit-exact against oracle/_ref; closed form for integer maps (fraction 0 copies the
+ * source pixel). */
+void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                           int src_stride, const uint16_t *fixed, int constant, int border)
+{
+    const long n = (long)sh * src_stride;
+    const long entries = (long)dw * dh;
+    const int wq = (dw + 3) & ~3;
+    (void)sw;
+    for (int y = 0; y < dh; y++) {
+        for (int x = 0; x < wq; x++) {
+            const long k = (long)y * dw + x;
+            uint32_t mx = k < entries ? fixed[2 * k] : 0xffff, my = k < entries ? fixed[2 * k + 1] : 0xffff;
+            const int xbad = mx == 0xffff;
+            if (mx == 0xffff) mx = constant ? 0 : 8;
+            if (my == 0xffff) my = constant ? 0 : 8;
+            const int fx = (int)(mx & 7), fy = (int)(my & 7);
+            const long o = (long)(my >> 3) * src_stride + (long)(mx >> 3);
+            const int p00 = rd(src, n, o), p10 = rd(src, n, o + 1), p01 = rd(src, n, o + src_stride),
+                      p11 = rd(src, n, o + src_stride + 1);
+            int v = (p00 * (8 - fx) * (8 - fy) + p10 * fx * (8 - fy) + p01 * (8 - fx) * fy + p11 * fx * fy + 32) >> 6;
+            if (constant && xbad) v = border;
+            dst[(long)y * dst_stride + x] = sat8(v);
+        }
+    }
+}
