@@ -209,6 +209,32 @@ int mvx_Remap_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, ui
 int mvx_Remap_FixedMap_F32(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, void *pFixedMap, uint32_t fixedStride,
                            const float *pFloatMap, uint32_t floatStride, uint32_t srcWidth, uint32_t srcHeight);
 
+/* ------------------------------------------------------------------ optical flow ----------- */
+/* One level of an image pyramid in device memory. */
+typedef struct mvx_image_level {
+    const uint8_t *ptr;
+    uint32_t width, height, stride;
+} mvx_image_level_t;
+
+/* Pyramidal Lucas-Kanade of HafCpu_OpticalFlowPyrLK_XY_XY_Generic (ago_haf_cpu_opticalflow.cpp:153-546),
+ * as called by agoKernel_OpticalFlowPyrLK_XY_XY (ago_kernel_api.cpp:20587-20650); CPU-only in the
+ * reference (ago_kernel_api.cpp:20642-20646).  oldPyramid / newPyramid: host arrays of `levels`
+ * U8 levels (level 0 first; equal sizes; widths >= 8).  Keypoint arrays are device vx_keypoint_t
+ * (28 bytes); *pOldCount (device) items of pOldKeypoints (and pEstimates when useInitialEstimate)
+ * are tracked, at most `capacity`; *pNewCount (device, may be NULL) receives the same count.
+ * termination: VX_TERM_CRITERIA_ITERATIONS / EPSILON / BOTH (EPSILON alone is capped at 10000
+ * iterations; the reference loops without bound).  windowDimension 3..15.
+ * The workspace (mvx_OpticalFlowPyrLK_workspace_size(level-0 width, height, capacity) bytes, device,
+ * 16-byte aligned) must be zero-filled before the first call and then kept: like the reference
+ * node's local data it carries state from call to call that the reference's results depend on. */
+size_t mvx_OpticalFlowPyrLK_workspace_size(uint32_t width0, uint32_t height0, uint32_t capacity);
+int mvx_OpticalFlowPyrLK_XY_XY(mvx_stream_t stream, void *pNewKeypoints, uint32_t *pNewCount,
+                               const mvx_image_level_t *oldPyramid, const mvx_image_level_t *newPyramid, uint32_t levels,
+                               float pyramidScale, const void *pOldKeypoints, const uint32_t *pOldCount,
+                               const void *pEstimates, uint32_t capacity, int32_t termination, float epsilon,
+                               uint32_t numIterations, int32_t useInitialEstimate, uint32_t windowDimension, void *workspace,
+                               size_t workspaceSize);
+
 /* ------------------------------------------------------------------ fused vision graph ----- */
 /* The RGB vision graph (vx nodes: ColorConvert RGB->IYUV, ChannelExtract Y, ScaleImage half bilinear,
  * GaussianPyramid HALF, CannyEdgeDetector 3/L1, HarrisCorners 3/3) over a batch of equally sized

@@ -612,6 +612,70 @@ template <bool CONST> vx_status x_amd_remap(_vx_node *n, hipStream_t st)
     return run_remap(n, st, img(n, 0), img(n, 1), (_vx_remap *)n->params[2], CONST, CONST ? scl(n, 3)->value.u8 : 0);
 }
 
+// ---------------------------------------------------------------------- OpticalFlowPyrLK (A17) --
+// optical_flow_pyr_lk -> OPTICAL_FLOW_PYR_LK_XY_XY (ago_drama_divide.cpp:1810-1891), validation of
+// ago_kernel_api.cpp:287-317 and :20617-20630; the node workspace keeps the tracker's scratch state
+// from run to run as the reference node's local data does (ago_kernel_api.cpp:20633-20640).
+vx_status v_lk(_vx_node *n)
+{
+    for (int i : {0, 1})
+        if (!is_ref(n->params[i], VX_TYPE_PYRAMID)) return fail(n, VX_ERROR_INVALID_PARAMETERS, "pyramids");
+    for (int i : {2, 3, 4})
+        if (!is_ref(n->params[i], VX_TYPE_ARRAY)) return fail(n, VX_ERROR_INVALID_PARAMETERS, "keypoint arrays");
+    _vx_pyramid *po = (_vx_pyramid *)n->params[0], *pn = (_vx_pyramid *)n->params[1];
+    _vx_array *ao = (_vx_array *)n->params[2], *ae = (_vx_array *)n->params[3], *an = (_vx_array *)n->params[4];
+    _vx_scalar *term = scl(n, 5), *eps = scl(n, 6), *iters = scl(n, 7), *use = scl(n, 8), *win = scl(n, 9);
+    if (!term || !eps || !iters || !use || !win) return fail(n, VX_ERROR_INVALID_PARAMETERS, "scalars");
+    if (ao->itemType != VX_TYPE_KEYPOINT || ae->itemType != VX_TYPE_KEYPOINT || term->dataType != VX_TYPE_ENUM ||
+        eps->dataType != VX_TYPE_FLOAT32 || iters->dataType != VX_TYPE_UINT32 || use->dataType != VX_TYPE_BOOL ||
+        win->dataType != VX_TYPE_SIZE)
+        return fail(n, VX_ERROR_INVALID_TYPE, "argument types");
+    if (po->format != VX_DF_IMAGE_U8 || pn->format != VX_DF_IMAGE_U8) return fail(n, VX_ERROR_INVALID_FORMAT, "pyramid format");
+    if (!po->width || !po->height || po->width != pn->width || po->height != pn->height || !ao->capacity ||
+        ao->capacity != ae->capacity)
+        return fail(n, VX_ERROR_INVALID_DIMENSION, "pyramid sizes / array capacities");
+    if (term->value.e != VX_TERM_CRITERIA_ITERATIONS && term->value.e != VX_TERM_CRITERIA_EPSILON && term->value.e != VX_TERM_CRITERIA_BOTH)
+        return fail(n, VX_ERROR_INVALID_VALUE, "termination");
+    if (po->scale != pn->scale || po->levels != pn->levels) return fail(n, VX_ERROR_INVALID_VALUE, "pyramid scale / levels");
+    if (win->value.sz < 3 || win->value.sz > 15) return fail(n, VX_ERROR_INVALID_VALUE, "window dimension");
+    for (_vx_image *l : po->images)
+        if (l->width < 8 || l->height < 3) return fail(n, VX_ERROR_NOT_SUPPORTED, "pyramid levels below 8x3 pixels");
+    if (an->itemType != VX_TYPE_KEYPOINT) return fail(n, VX_ERROR_INVALID_TYPE, "new points must be KEYPOINT");
+    if (an->isVirtual && !an->capacity) an->capacity = ao->capacity;
+    if (an->capacity < ao->capacity) return fail(n, VX_ERROR_INVALID_DIMENSION, "new points capacity");
+    return VX_SUCCESS;
+}
+
+vx_status i_lk(_vx_node *n)
+{
+    _vx_pyramid *po = (_vx_pyramid *)n->params[0];
+    n->localSize = mvx_OpticalFlowPyrLK_workspace_size(po->width, po->height, (uint32_t)((_vx_array *)n->params[2])->capacity);
+    return VX_SUCCESS;
+}
+
+vx_status x_lk(_vx_node *n, hipStream_t st)
+{
+    _vx_pyramid *po = (_vx_pyramid *)n->params[0], *pn = (_vx_pyramid *)n->params[1];
+    _vx_array *ao = (_vx_array *)n->params[2], *ae = (_vx_array *)n->params[3], *an = (_vx_array *)n->params[4];
+    // the reference refuses differing point counts (ago_kernel_api.cpp:20602); host counts are
+    // exact for arrays no node of this graph writes
+    auto hostKnown = [&](_vx_array *a) {
+        for (_vx_array *w : n->graph->writtenArrays)
+            if (w == a) return false;
+        return true;
+    };
+    if (hostKnown(ao) && hostKnown(ae) && ao->numItems != ae->numItems) return fail(n, VX_ERROR_INVALID_DIMENSION, "point counts differ");
+    std::vector<mvx_image_level_t> lo(po->levels), ln(pn->levels);
+    for (size_t l = 0; l < po->levels; l++) {
+        lo[l] = mvx_image_level_t{P(po->images[l]), po->images[l]->width, po->images[l]->height, S(po->images[l])};
+        ln[l] = mvx_image_level_t{P(pn->images[l]), pn->images[l]->width, pn->images[l]->height, S(pn->images[l])};
+    }
+    return rc(n, mvx_OpticalFlowPyrLK_XY_XY(st, an->items_ptr(), an->count_ptr(), lo.data(), ln.data(), (uint32_t)po->levels, po->scale,
+                                            ao->items_ptr(), ao->count_ptr(), ae->items_ptr(), (uint32_t)ao->capacity,
+                                            scl(n, 5)->value.e, scl(n, 6)->value.f32, scl(n, 7)->value.u32, scl(n, 8)->value.b ? 1 : 0,
+                                            (uint32_t)scl(n, 9)->value.sz, n->local.ptr(), n->localSize));
+}
+
 }  // namespace
 
 namespace mvxe {
@@ -668,6 +732,11 @@ const std::vector<KernelDef> &kernel_table()
         {VX_KERNEL_WARP_PERSPECTIVE, "org.khronos.openvx.warp_perspective",
          {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
          v_warp<true>, nullptr, x_warp<true>, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_OPTICAL_FLOW_PYR_LK, "org.khronos.openvx.optical_flow_pyr_lk",
+         {{IN, VX_TYPE_PYRAMID, false}, {IN, VX_TYPE_PYRAMID, false}, {IN, VX_TYPE_ARRAY, false}, {IN, VX_TYPE_ARRAY, false},
+          {OUT, VX_TYPE_ARRAY, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false},
+          {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_lk, i_lk, x_lk, AGO_TARGET_AFFINITY_GPU},
         {VX_KERNEL_REMAP, "org.khronos.openvx.remap",
          {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
          v_remap, nullptr, x_remap, AGO_TARGET_AFFINITY_GPU},

@@ -16,6 +16,10 @@ Workloads
 * ``warp_matrices(w, h, seed)``: a near-identity affine (2x3) and perspective (3x3) VX matrix about
   the image centre -- rotation U(-5,5) degrees, scale U(0.9,1.1), translation U(-50,50) px,
   perspective terms U(-1e-4,1e-4) -- as the float32 arrays the VX matrix objects store.
+* ``lk_pair(w, h, seed)``: the tracking workload -- a scene and a copy translated by a seeded
+  (tx, ty) with |t| in [2, 8] px (bilinear, edges clamped), and the translation.
+* ``lk_keypoints(w, h, n, seed, margin=16)``: n vx_keypoint_t records at seeded integer positions
+  inside the margin (tracking_status 1, other fields 0).
 * ``barrel_map(w, h, k1)``: the float remap table of a barrel distortion r' = r(1 + k1 r^2) about
   the centre (r normalised by the half-diagonal): (h, w, 2) float32 source (x, y) per output pixel.
 """
@@ -140,3 +144,40 @@ def barrel_map(w, h, k1):
     dx, dy = (xx - cx) / rn, (yy - cy) / rn
     f = 1.0 + k1 * (dx * dx + dy * dy)
     return np.ascontiguousarray(np.stack([cx + dx * f * rn, cy + dy * f * rn], -1).astype(np.float32))
+
+
+KEYPOINT = np.dtype([("x", np.int32), ("y", np.int32), ("strength", np.float32), ("scale", np.float32),
+                     ("orientation", np.float32), ("tracking_status", np.int32), ("error", np.float32)])
+
+
+def shift_image(img, tx, ty):
+    """out(x, y) = img(x - tx, y - ty), bilinear, coordinates clamped to the image, rounded"""
+    h, w = img.shape
+    xs = np.clip(np.arange(w, dtype=np.float64) - tx, 0, w - 1)
+    ys = np.clip(np.arange(h, dtype=np.float64) - ty, 0, h - 1)
+    x0 = np.minimum(np.floor(xs).astype(np.int64), w - 2)
+    y0 = np.minimum(np.floor(ys).astype(np.int64), h - 2)
+    fx, fy = xs - x0, ys - y0
+    f = img.astype(np.float64)
+    top = f[y0][:, x0] * (1 - fx) + f[y0][:, x0 + 1] * fx
+    bot = f[y0 + 1][:, x0] * (1 - fx) + f[y0 + 1][:, x0 + 1] * fx
+    out = top * (1 - fy)[:, None] + bot * fy[:, None]
+    return np.clip(np.rint(out), 0, 255).astype(np.uint8)
+
+
+def lk_pair(w, h, seed=1):
+    p = uniform01(seed, 600, 2)
+    mag = 2.0 + 6.0 * p[0]
+    ang = 2.0 * np.pi * p[1]
+    tx, ty = mag * np.cos(ang), mag * np.sin(ang)
+    old = scene_u8(w, h, seed)
+    return old, shift_image(old, tx, ty), (tx, ty)
+
+
+def lk_keypoints(w, h, n, seed=1, margin=16):
+    r = u32(seed, 700, 2 * n)
+    kp = np.zeros(n, KEYPOINT)
+    kp["x"] = margin + (r[0::2] % np.uint32(w - 2 * margin)).astype(np.int32)
+    kp["y"] = margin + (r[1::2] % np.uint32(h - 2 * margin)).astype(np.int32)
+    kp["tracking_status"] = 1
+    return kp

@@ -78,6 +78,13 @@ void oracle_remap_fixed(int dw, int dh, int sw, int sh, const float *map, uint16
 void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
                            int src_stride, const uint16_t *fixed, int constant, int border);
 
+/* ---- pyramidal Lucas-Kanade (lk.c); the state block plays the reference node's local data ---- */
+size_t oracle_lk_state_bytes(int w0, int h0, int capacity);
+int oracle_lk_run(void *state_mem, int levels, float pyr_scale, const uint8_t *const *old_pyr, const uint8_t *const *new_pyr,
+                  const int *ws, const int *hs, const int *strides, int npts, const oracle_keypoint_t *old_kp,
+                  const oracle_keypoint_t *est_kp, oracle_keypoint_t *new_kp, int termination, float epsilon,
+                  int num_iterations, int use_initial_estimate, int winsz);
+
 /* ---- nodes and graphs on reference-layout planes (graphs.c); packed in/out ---- */
 void oracle_filter3x3(int which, int w, int h, const uint8_t *src, uint8_t *dst);
 void oracle_sobel_node(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy);

@@ -214,6 +214,37 @@ class Oracle:
         self.lib.oracle_remap_bilinear(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(fixed), int(constant), border)
         return d if padded else np.ascontiguousarray(d[:, :dw])
 
+    # ---- pyramidal Lucas-Kanade ----
+    def lk_state(self, w0, h0, capacity):
+        """zeroed scratch state of one tracker node (the reference node's local data)"""
+        self.lib.oracle_lk_state_bytes.restype = ctypes.c_size_t
+        return np.zeros(self.lib.oracle_lk_state_bytes(w0, h0, capacity), np.uint8)
+
+    def lk(self, old_pyr, new_pyr, old_kp, est_kp, termination, epsilon, num_iterations, use_initial_estimate, window,
+           state=None, new_kp=None, scale=0.5):
+        """one OpticalFlowPyrLK run over lists of pyramid levels (packed U8 arrays); returns the new
+        keypoints.  Pass the same `state` to successive calls to reproduce a node run repeatedly."""
+        levels = len(old_pyr)
+        h0, w0 = old_pyr[0].shape
+        n = len(old_kp)
+        if state is None:
+            state = self.lk_state(w0, h0, max(n, 1))
+        planes_o = [_plane(a)[0] for a in old_pyr]
+        planes_n = [_plane(a)[0] for a in new_pyr]
+        ptr_t = ctypes.c_void_p * levels
+        int_t = ctypes.c_int * levels
+        po = ptr_t(*[a.ctypes.data for a in planes_o])
+        pn = ptr_t(*[a.ctypes.data for a in planes_n])
+        ws = int_t(*[a.shape[1] for a in old_pyr])
+        hs = int_t(*[a.shape[0] for a in old_pyr])
+        st = int_t(*[a.shape[1] for a in planes_o])
+        out = np.zeros(n, KEYPOINT) if new_kp is None else np.ascontiguousarray(new_kp).copy()
+        old_kp = np.ascontiguousarray(old_kp, KEYPOINT)
+        est_kp = np.ascontiguousarray(est_kp, KEYPOINT)
+        self.lib.oracle_lk_run(_p(state), levels, ctypes.c_float(scale), po, pn, ws, hs, st, n, _p(old_kp), _p(est_kp), _p(out),
+                               termination, ctypes.c_float(epsilon), num_iterations, int(use_initial_estimate), window)
+        return out
+
     # ---- the 4K vision graph ----
     def graph2(self, rgb, dw, dh, levels, lower, upper, k, thr, dist, capacity):
         y = self.color_y(rgb)
@@ -329,3 +360,14 @@ class Reference:
         d = np.zeros((dh, dw), np.uint8)
         self._ok(self.lib.ref_remap(sw, sh, _p(_c(src)), dw, dh, _p(_f32(mapxy)), int(constant), border, _p(d)), "remap")
         return d
+
+    def lk(self, old_img, new_img, levels, old_kp, est_kp, termination, epsilon, num_iterations, use_initial_estimate,
+           window, reps=1, timed=False):
+        h, w = old_img.shape
+        n = len(old_kp)
+        out = np.zeros(n, KEYPOINT)
+        sec = ctypes.c_double(0)
+        self._ok(self.lib.ref_lk(w, h, _p(_c(old_img)), _p(_c(new_img)), levels, n, _p(np.ascontiguousarray(old_kp, KEYPOINT)),
+                                 _p(np.ascontiguousarray(est_kp, KEYPOINT)), _p(out), termination, ctypes.c_float(epsilon),
+                                 num_iterations, int(use_initial_estimate), window, reps, ctypes.byref(sec)), "lk")
+        return (out, sec.value) if timed else out

@@ -331,4 +331,41 @@ int ref_remap(int sw, int sh, const uint8_t *src, int dw, int dh, const float *m
     return 0;
 }
 
+// OpticalFlowPyrLK between two images through HALF Gaussian pyramids built in the same graph
+// (GaussianPyramid nodes).  old_kp / est_kp: n vx_keypoint_t; new_kp receives n items.  The graph
+// is processed `reps` times (the node keeps scratch state between runs); new_kp holds the last run.
+int ref_lk(int w, int h, const uint8_t *old_img, const uint8_t *new_img, int levels, int n, const vx_keypoint_t *old_kp,
+           const vx_keypoint_t *est_kp, vx_keypoint_t *new_kp, int termination, float epsilon, int num_iterations,
+           int use_initial_estimate, int window, int reps, double *sec_per_run) {
+    Ctx c;
+    vx_image io = make_input(c, w, h, VX_DF_IMAGE_U8, old_img);
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, new_img);
+    vx_pyramid po = vxCreatePyramid(c.ctx, levels, VX_SCALE_PYRAMID_HALF, w, h, VX_DF_IMAGE_U8);
+    vx_pyramid pn = vxCreatePyramid(c.ctx, levels, VX_SCALE_PYRAMID_HALF, w, h, VX_DF_IMAGE_U8);
+    vx_array ao = vxCreateArray(c.ctx, VX_TYPE_KEYPOINT, n);
+    vx_array ae = vxCreateArray(c.ctx, VX_TYPE_KEYPOINT, n);
+    vx_array an = vxCreateArray(c.ctx, VX_TYPE_KEYPOINT, n);
+    CHECK(vxAddArrayItems(ao, n, old_kp, sizeof(vx_keypoint_t)));
+    CHECK(vxAddArrayItems(ae, n, est_kp, sizeof(vx_keypoint_t)));
+    vx_float32 eps = epsilon;
+    vx_uint32 iters = (vx_uint32)num_iterations;
+    vx_bool use = use_initial_estimate ? vx_true_e : vx_false_e;
+    vx_scalar s_eps = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &eps);
+    vx_scalar s_it = vxCreateScalar(c.ctx, VX_TYPE_UINT32, &iters);
+    vx_scalar s_use = vxCreateScalar(c.ctx, VX_TYPE_BOOL, &use);
+    if (!vxGaussianPyramidNode(c.graph, io, po)) return -1;
+    if (!vxGaussianPyramidNode(c.graph, in, pn)) return -1;
+    if (!vxOpticalFlowPyrLKNode(c.graph, po, pn, ao, ae, an, (vx_enum)termination, s_eps, s_it, s_use, (vx_size)window)) return -1;
+    CHECK(vxVerifyGraph(c.graph));
+    auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; r++) CHECK(vxProcessGraph(c.graph));
+    auto t1 = std::chrono::steady_clock::now();
+    if (sec_per_run) *sec_per_run = std::chrono::duration<double>(t1 - t0).count() / (reps > 0 ? reps : 1);
+    vx_size got = 0;
+    CHECK(vxQueryArray(an, VX_ARRAY_NUMITEMS, &got, sizeof(got)));
+    if ((int)got != n) return -2;
+    CHECK(vxCopyArrayRange(an, 0, n, sizeof(vx_keypoint_t), new_kp, VX_READ_ONLY, VX_MEMORY_TYPE_HOST));
+    return 0;
+}
+
 }  // extern "C"

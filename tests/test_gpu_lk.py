@@ -1,0 +1,158 @@
+"""GPU parity of OpticalFlowPyrLK: the HIP tracker through the C ABI (and through the vx API) versus
+the oracle, bit-exact on every keypoint field, including repeated calls on one workspace (the
+reference node's scratch state), lost points, initial estimates, EPSILON-only termination, the
+config-3 workload at 3840x2160 with 10,000 keypoints, and the reference engine's golden hash."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from mivisionx_amd import kernels, synth
+from mivisionx_amd.device import Plane, call
+from pyoracle import KEYPOINT
+from util import sha
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ITER, EPS, BOTH = 0xF000, 0xF001, 0xF002
+
+
+class Level(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("stride", ctypes.c_uint32)]
+
+
+class GpuLK:
+    """one tracker 'node': pyramids in HBM and a persistent zero-initialised workspace"""
+
+    def __init__(self, old_pyr, new_pyr, capacity):
+        K = kernels()
+        K.mvx_OpticalFlowPyrLK_workspace_size.restype = ctypes.c_size_t
+        self.levels = len(old_pyr)
+        self.po = [Plane.from_array(a) for a in old_pyr]
+        self.pn = [Plane.from_array(a) for a in new_pyr]
+        self.capacity = capacity
+        h0, w0 = old_pyr[0].shape
+        self.ws_size = K.mvx_OpticalFlowPyrLK_workspace_size(w0, h0, capacity)
+        self.ws = torch.zeros(self.ws_size, dtype=torch.uint8, device="cuda")
+        self.new = torch.zeros(28 * capacity, dtype=torch.uint8, device="cuda")
+        self.new_count = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+    def run(self, old_kp, est_kp, term, eps, iters, use_est, win):
+        n = len(old_kp)
+        oldt = torch.from_numpy(np.ascontiguousarray(old_kp).view(np.uint8).copy()).cuda()
+        estt = torch.from_numpy(np.ascontiguousarray(est_kp).view(np.uint8).copy()).cuda()
+        cnt = torch.tensor([n], dtype=torch.int32, device="cuda")
+        lo = (Level * self.levels)(*[Level(p.t.data_ptr(), p.width, p.height, p.stride) for p in self.po])
+        ln = (Level * self.levels)(*[Level(p.t.data_ptr(), p.width, p.height, p.stride) for p in self.pn])
+        call("OpticalFlowPyrLK_XY_XY", ctypes.c_void_p(self.new.data_ptr()), ctypes.c_void_p(self.new_count.data_ptr()), lo, ln,
+             self.levels, ctypes.c_float(0.5), ctypes.c_void_p(oldt.data_ptr()), ctypes.c_void_p(cnt.data_ptr()),
+             ctypes.c_void_p(estt.data_ptr()), self.capacity, ctypes.c_int32(term), ctypes.c_float(eps), iters,
+             ctypes.c_int32(int(use_est)), win, ctypes.c_void_p(self.ws.data_ptr()), ctypes.c_size_t(self.ws_size))
+        torch.cuda.synchronize()
+        assert int(self.new_count.item()) == n
+        return self.new[:28 * n].cpu().numpy().view(KEYPOINT).copy()
+
+
+def _estimate(kp):
+    est = kp.copy()
+    est["x"] += 2
+    est["y"] -= 1
+    return est
+
+
+def _same(got, want, what):
+    """every byte of every vx_keypoint_t record"""
+    assert len(got) == len(want), what
+    rows = (np.ascontiguousarray(got).view(np.uint8).reshape(-1, 28) != np.ascontiguousarray(want).view(np.uint8).reshape(-1, 28)).any(1)
+    bad = np.nonzero(rows)[0]
+    assert bad.size == 0, f"{what}: {bad.size} of {len(got)} keypoints differ; first: got {got[bad[:3]]} want {want[bad[:3]]}"
+
+
+@pytest.mark.parametrize("w,h,levels,n,win,term,use", [
+    (320, 240, 3, 400, 9, BOTH, 0), (640, 480, 4, 2000, 9, ITER, 0), (333, 200, 3, 300, 7, BOTH, 1),
+    (256, 128, 2, 300, 15, BOTH, 0), (640, 480, 4, 500, 5, BOTH, 0), (320, 240, 3, 300, 9, EPS, 0),
+    (200, 96, 2, 256, 3, BOTH, 1), (120, 64, 1, 100, 8, ITER, 0)])
+def test_lk_matches_oracle(gpu, orc, w, h, levels, n, win, term, use):
+    old, new, _ = synth.lk_pair(w, h, 21)
+    kp = synth.lk_keypoints(w, h, n, 22, margin=4)
+    kp["tracking_status"][::13] = 0
+    op, np_ = orc.pyramid(old, levels), orc.pyramid(new, levels)
+    g = GpuLK(op, np_, n)
+    got = g.run(kp, _estimate(kp), term, 0.01, 20, use, win)
+    want = orc.lk(op, np_, kp, _estimate(kp), term, 0.01, 20, use, win)
+    _same(got, want, f"lk {w}x{h} L{levels} win{win} term {term:x} est {use}")
+
+
+def test_lk_repeated_calls_keep_state(gpu, orc):
+    """three calls on one workspace against three oracle calls on one state"""
+    w, h, levels = 640, 480, 4
+    old, new, _ = synth.lk_pair(w, h, 23)
+    kp = synth.lk_keypoints(w, h, 1500, 24)
+    kp["x"][:500] = 16 + (kp["x"][:500] % 8)      # windows at the left edge of the coarse levels
+    op, np_ = orc.pyramid(old, levels), orc.pyramid(new, levels)
+    g = GpuLK(op, np_, len(kp))
+    st = orc.lk_state(w, h, len(kp))
+    for r in range(3):
+        got = g.run(kp, kp, BOTH, 0.01, 20, 0, 9)
+        want = orc.lk(op, np_, kp, kp, BOTH, 0.01, 20, 0, 9, state=st)
+        _same(got, want, f"run {r}")
+
+
+def test_lk_config3_4k_10000_points(gpu, orc):
+    w, h, levels = 3840, 2160, 4
+    old, new, (tx, ty) = synth.lk_pair(w, h, 1)
+    kp = synth.lk_keypoints(w, h, 10000, 1)
+    op, np_ = orc.pyramid(old, levels), orc.pyramid(new, levels)
+    got = GpuLK(op, np_, len(kp)).run(kp, kp, BOTH, 0.01, 20, 0, 9)
+    want = orc.lk(op, np_, kp, kp, BOTH, 0.01, 20, 0, 9)
+    _same(got, want, "4K 10000 points")
+    ok = got["tracking_status"] == 1
+    assert abs(np.median(got["x"][ok] - kp["x"][ok]) - tx) <= 0.5 and abs(np.median(got["y"][ok] - kp["y"][ok]) - ty) <= 0.5
+
+
+def test_lk_golden_1080p(gpu, orc):
+    with open(os.path.join(ROOT, "tests", "golden", "vision_golden_lk.json")) as f:
+        g = json.load(f)
+    c = next(c for c in g["cases"] if c["w"] == 1920)
+    old, new, _ = synth.lk_pair(c["w"], c["h"], c["pair_seed"])
+    kp = synth.lk_keypoints(c["w"], c["h"], c["points"], c["keypoint_seed"], c["margin"])
+    got = GpuLK(orc.pyramid(old, c["levels"]), orc.pyramid(new, c["levels"]), c["points"]).run(
+        kp, _estimate(kp), c["termination"], g["epsilon"], g["iterations"], c["use_estimate"], c["window"])
+    assert sha(got) == c["result_sha256"]
+
+
+def test_lk_through_vx_api(gpu, orc):
+    """GaussianPyramid x2 -> OpticalFlowPyrLK in one vx graph"""
+    from mivisionx_amd import openvx as vx
+    L = vx.lib()
+    ctx = L.vxCreateContext()
+    try:
+        w, h, levels, n = 640, 480, 4, 800
+        old, new, _ = synth.lk_pair(w, h, 25)
+        kp = synth.lk_keypoints(w, h, n, 26)
+        g = L.vxCreateGraph(ctx)
+        a, b = vx.create_image(ctx, w, h, "U8"), vx.create_image(ctx, w, h, "U8")
+        vx.write_image(a, old)
+        vx.write_image(b, new)
+        po = L.vxCreateVirtualPyramid(g, levels, vx.VX_SCALE_PYRAMID_HALF, w, h, vx.DF["U8"])
+        pn = L.vxCreateVirtualPyramid(g, levels, vx.VX_SCALE_PYRAMID_HALF, w, h, vx.DF["U8"])
+        ao, an = L.vxCreateArray(ctx, vx.VX_TYPE_KEYPOINT, n), L.vxCreateArray(ctx, vx.VX_TYPE_KEYPOINT, n)
+        assert L.vxAddArrayItems(ao, n, kp.ctypes.data_as(ctypes.c_void_p), 28) == 0
+        eps = L.vxCreateScalar(ctx, vx.VX_TYPE_FLOAT32, ctypes.byref(ctypes.c_float(0.01)))
+        its = L.vxCreateScalar(ctx, 0x007, ctypes.byref(ctypes.c_uint32(20)))          # VX_TYPE_UINT32
+        use = L.vxCreateScalar(ctx, 0x010, ctypes.byref(ctypes.c_int32(0)))            # VX_TYPE_BOOL
+        assert L.vxGaussianPyramidNode(g, a, po) and L.vxGaussianPyramidNode(g, b, pn)
+        assert L.vxOpticalFlowPyrLKNode(g, po, pn, ao, ao, an, BOTH, eps, its, use, 9)
+        assert L.vxVerifyGraph(g) == 0
+        for _ in range(2):
+            assert L.vxProcessGraph(g) == 0
+            got = vx.read_array(an, KEYPOINT)
+        want = orc.lk(orc.pyramid(old, levels), orc.pyramid(new, levels), kp, kp, BOTH, 0.01, 20, 0, 9)
+        _same(got, want, "vx graph lk")
+    finally:
+        hh = ctypes.c_void_p(ctx)
+        L.vxReleaseContext(ctypes.byref(hh))
