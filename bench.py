@@ -12,6 +12,11 @@ value = whole-job graph throughput in Mpix/s of 4K input.
 Also reported:
   roofline       -- the dominant stage's algorithmic bytes per launch / its mean launch time, timed with
                     HIP events on its launch stream inside the timed region;
+  nodes          -- the other node workloads, each timed alone with HIP events on its launch stream:
+                    the 1920x1080 Gaussian->Sobel->Magnitude+Phase chain (8 B/px), WarpAffine and
+                    WarpPerspective 3840x2160 bilinear constant-0 (2 B/px), Remap 3840x2160 through the
+                    fixed-point table of a barrel map (6 B/px), OpticalFlowPyrLK 4 levels / 9x9 / 20
+                    iterations / 10,000 keypoints on 3840x2160 (keypoints/s);
   graph_hbm      -- the whole graph's algorithmic bytes per frame / frame time;
   per_image      -- the same frames through one launch per node and frame (the reference's execution
                     model), and the batched/per-image ratio;
@@ -61,6 +66,99 @@ def stage_algorithmic_bytes(stage):
     }[stage]
 
 
+def measure_nodes(reps=10):
+    """GPU time of the node workloads beside the headline graph (HIP events on the launch stream)"""
+    import ctypes
+    import numpy as np
+    import torch
+    from mivisionx_amd import synth
+    from mivisionx_amd.device import Plane, call
+    from mivisionx_amd.pipelines import PyrLKTracker
+    from pyoracle import Oracle
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    def bw(name, ms, px, bpp, unit_value, unit):
+        gbs = px * bpp / (ms / 1e3) / 1e9
+        return {"ms": round(ms, 4), "value": round(unit_value, 2), "unit": unit, "algorithmic_bytes": int(px * bpp),
+                "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "bound": "hbm"}
+
+    out = {}
+    # config 1: the 1920x1080 chain, one launch per node
+    w1, h1 = 1920, 1080
+    s = Plane.from_array(synth.uniform_u8(w1, h1, 0))
+    g, gx, gy, m, p = Plane(w1, h1), Plane(w1, h1, "S16"), Plane(w1, h1, "S16"), Plane(w1, h1, "S16"), Plane(w1, h1)
+
+    def chain():
+        call("Gaussian_U8_U8_3x3", w1, h1, g, g.stride, s, s.stride)
+        call("Sobel_S16S16_U8_3x3_GXY", w1, h1, gx, gx.stride, gy, gy.stride, g, g.stride)
+        call("Magnitude_S16_S16S16", w1, h1, m, m.stride, gx, gx.stride, gy, gy.stride)
+        call("Phase_U8_S16S16", w1, h1, p, p.stride, gx, gx.stride, gy, gy.stride)
+    ms = timed(chain)
+    out["chain1_1920x1080"] = bw("chain1", ms, w1 * h1, 8, w1 * h1 / (ms / 1e3) / 1e6, "Mpix/s")
+    # config 3: warps and remap at 4K
+    src = synth.scene_u8(W, H, 1)
+    sp, dp = Plane.from_array(src), Plane(W, H)
+    aff, per = synth.warp_matrices(W, H, 1)
+    aff_c, per_c = np.ascontiguousarray(aff), np.ascontiguousarray(per)
+    ms = timed(lambda: call("WarpAffine_U8_U8_Bilinear_Constant", W, H, dp, dp.stride, W, H, sp, sp.stride,
+                            aff_c.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint8(0)))
+    out["warp_affine_3840x2160"] = bw("warp_affine", ms, W * H, 2, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    ms = timed(lambda: call("WarpPerspective_U8_U8_Bilinear_Constant", W, H, dp, dp.stride, W, H, sp, sp.stride,
+                            per_c.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint8(0)))
+    out["warp_perspective_3840x2160"] = bw("warp_perspective", ms, W * H, 2, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    fmap = torch.from_numpy(synth.barrel_map(W, H, 0.15)).cuda()
+    fixed = torch.zeros((H, W * 4), dtype=torch.uint8, device="cuda")
+    call("Remap_FixedMap_F32", W, H, ctypes.c_void_p(fixed.data_ptr()), W * 4, ctypes.c_void_p(fmap.data_ptr()), W * 8, W, H)
+    ms = timed(lambda: call("Remap_U8_U8_Bilinear", W, H, dp, dp.stride, W, H, sp, sp.stride, ctypes.c_void_p(fixed.data_ptr()), W * 4))
+    out["remap_3840x2160"] = bw("remap", ms, W * H, 6, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    # config 3: pyramidal LK, 10,000 keypoints
+    orc = Oracle()
+    old, new, _ = synth.lk_pair(W, H, 1)
+    lo = [Plane.from_array(a) for a in orc.pyramid(old, 4)]
+    ln = [Plane.from_array(a) for a in orc.pyramid(new, 4)]
+    tr = PyrLKTracker(lo, ln, 10000)
+    tr.set_points(synth.lk_keypoints(W, H, 10000, 1))
+    ms = timed(lambda: tr.run(0xF002, 0.01, 20, False, 9))
+    out["optical_flow_pyr_lk_4levels_10000pts"] = {"ms": round(ms, 4), "value": round(10000 / (ms / 1e3), 1), "unit": "keypoints/s",
+                                                   "bound": "latency (one 64-lane workgroup per keypoint, sequential Newton steps)"}
+    return out
+
+
+def cpu_nodes():
+    """the reference CPU engine (oracle/_ref, one thread) on the node workloads, a few frames each"""
+    from mivisionx_amd import synth
+    from pyoracle import Reference
+    ref = Reference()
+    import numpy as np
+    out = {}
+    _, sec = ref.chain1(synth.uniform_u8(1920, 1080, 0), time_iters=3)
+    out["chain1_1920x1080"] = {"ms": round(sec * 1e3, 2), "value": round(1920 * 1080 / sec / 1e6, 2), "unit": "Mpix/s"}
+    src = synth.scene_u8(W, H, 1)
+    aff, per = synth.warp_matrices(W, H, 1)
+    _, sec = ref.warp(0, src, W, H, aff, True, 0, time_iters=2)
+    out["warp_affine_3840x2160"] = {"ms": round(sec * 1e3, 2), "value": round(W * H / sec / 1e6, 2), "unit": "Mpix/s"}
+    _, sec = ref.warp(1, src, W, H, per, True, 0, time_iters=2)
+    out["warp_perspective_3840x2160"] = {"ms": round(sec * 1e3, 2), "value": round(W * H / sec / 1e6, 2), "unit": "Mpix/s"}
+    _, sec = ref.remap(src, synth.barrel_map(W, H, 0.15), False, 0, time_iters=2)
+    out["remap_3840x2160"] = {"ms": round(sec * 1e3, 2), "value": round(W * H / sec / 1e6, 2), "unit": "Mpix/s"}
+    old, new, _ = synth.lk_pair(W, H, 1)
+    kp = synth.lk_keypoints(W, H, 10000, 1)
+    _, sec = ref.lk(old, new, 4, kp, kp, 0xF002, 0.01, 20, 0, 9, reps=2, timed=True)
+    out["optical_flow_pyr_lk_4levels_10000pts"] = {"ms": round(sec * 1e3, 2), "value": round(10000 / sec, 1), "unit": "keypoints/s",
+                                                   "note": "includes the two 4-level GaussianPyramid nodes of the graph"}
+    return out
+
+
 def cpu_baseline(rgb, budget_s=10.0):
     from pyoracle import Oracle, Reference
     try:
@@ -95,6 +193,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-image", action="store_true")
+    ap.add_argument("--no-nodes", action="store_true", help="skip the config-1 / config-3 node measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the vision graph runs on a single MI355X here (asked for --gpus {args.gpus}); "
@@ -200,8 +299,15 @@ def main():
         out["per_image"] = {"value": round(pi_value, 2), "unit": "Mpix/s",
                             "what": "same frames, one launch per node and frame (no fusion, no batching)",
                             "batched_over_per_image": round(value / pi_value, 2)}
+    if not args.no_nodes:
+        out["nodes"] = measure_nodes()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(base[0].numpy(), args.cpu_budget)
+        if not args.no_nodes:
+            try:
+                out["cpu_baseline"]["nodes"] = cpu_nodes()
+            except FileNotFoundError:
+                pass
     print(json.dumps(out))
 
 

@@ -202,3 +202,40 @@ class FusedVisionGraph:
 
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class ImageLevel(ctypes.Structure):
+    """mvx_image_level_t"""
+    _fields_ = [("ptr", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("stride", ctypes.c_uint32)]
+
+
+class PyrLKTracker:
+    """OpticalFlowPyrLK between two device pyramids (lists of Planes) with a persistent workspace
+    (the reference node's local data); keypoints are device vx_keypoint_t arrays."""
+
+    def __init__(self, old_levels, new_levels, capacity):
+        K = kernels()
+        K.mvx_OpticalFlowPyrLK_workspace_size.restype = ctypes.c_size_t
+        self.old, self.new = old_levels, new_levels
+        self.capacity = capacity
+        self.ws_size = K.mvx_OpticalFlowPyrLK_workspace_size(old_levels[0].width, old_levels[0].height, capacity)
+        self.ws = _dev_bytes(self.ws_size)
+        self.out = _dev_bytes(28 * capacity)
+        self.out_count = torch.zeros(1, dtype=torch.int32, device="cuda")
+        n = len(old_levels)
+        self.lo = (ImageLevel * n)(*[ImageLevel(p.t.data_ptr(), p.width, p.height, p.stride) for p in old_levels])
+        self.ln = (ImageLevel * n)(*[ImageLevel(p.t.data_ptr(), p.width, p.height, p.stride) for p in new_levels])
+
+    def set_points(self, old_kp, est_kp=None):
+        self.n = len(old_kp)
+        self.kp = torch.from_numpy(np.ascontiguousarray(old_kp, KEYPOINT).view(np.uint8).copy()).cuda()
+        self.est = self.kp if est_kp is None else torch.from_numpy(np.ascontiguousarray(est_kp, KEYPOINT).view(np.uint8).copy()).cuda()
+        self.count = torch.tensor([self.n], dtype=torch.int32, device="cuda")
+
+    def run(self, termination=0xF002, epsilon=0.01, iterations=20, use_estimate=False, window=9, scale=0.5):
+        call("OpticalFlowPyrLK_XY_XY", _ptr(self.out), _ptr(self.out_count), self.lo, self.ln, len(self.old), ctypes.c_float(scale),
+             _ptr(self.kp), _ptr(self.count), _ptr(self.est), self.capacity, ctypes.c_int32(termination), ctypes.c_float(epsilon),
+             iterations, ctypes.c_int32(int(use_estimate)), window, _ptr(self.ws), ctypes.c_size_t(self.ws_size))
+
+    def result(self):
+        return self.out[:28 * self.n].cpu().numpy().view(KEYPOINT).copy()

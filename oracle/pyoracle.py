@@ -348,18 +348,22 @@ class Reference:
             out["sec_per_frame"] = sec.value
         return out
 
-    def warp(self, kind, src, dw, dh, m, constant=False, border=0):
+    def warp(self, kind, src, dw, dh, m, constant=False, border=0, time_iters=0):
         sh, sw = src.shape
         d = np.zeros((dh, dw), np.uint8)
-        self._ok(self.lib.ref_warp(kind, sw, sh, _p(_c(src)), dw, dh, _p(_f32(m)), int(constant), border, _p(d)), "warp")
-        return d
+        sec = ctypes.c_double(0)
+        self._ok(self.lib.ref_warp(kind, sw, sh, _p(_c(src)), dw, dh, _p(_f32(m)), int(constant), border, _p(d), time_iters,
+                                   ctypes.byref(sec)), "warp")
+        return (d, sec.value) if time_iters else d
 
-    def remap(self, src, mapxy, constant=False, border=0):
+    def remap(self, src, mapxy, constant=False, border=0, time_iters=0):
         sh, sw = src.shape
         dh, dw = mapxy.shape[:2]
         d = np.zeros((dh, dw), np.uint8)
-        self._ok(self.lib.ref_remap(sw, sh, _p(_c(src)), dw, dh, _p(_f32(mapxy)), int(constant), border, _p(d)), "remap")
-        return d
+        sec = ctypes.c_double(0)
+        self._ok(self.lib.ref_remap(sw, sh, _p(_c(src)), dw, dh, _p(_f32(mapxy)), int(constant), border, _p(d), time_iters,
+                                    ctypes.byref(sec)), "remap")
+        return (d, sec.value) if time_iters else d
 
     def lk(self, old_img, new_img, levels, old_kp, est_kp, termination, epsilon, num_iterations, use_initial_estimate,
            window, reps=1, timed=False):

@@ -287,7 +287,7 @@ int ref_graph2(int w, int h, const uint8_t *rgb, int dw, int dh, uint8_t *scaled
 // WarpAffine (kind 0, m = 2x3 VX matrix, 6 floats) / WarpPerspective (kind 1, 3x3, 9 floats), bilinear;
 // border UNDEFINED (constant == 0) or CONSTANT with `border`.
 int ref_warp(int kind, int sw, int sh, const uint8_t *src, int dw, int dh, const float *m, int constant, int border,
-             uint8_t *dst) {
+             uint8_t *dst, int time_iters, double *sec_per_frame) {
     Ctx c;
     vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
     vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
@@ -305,12 +305,18 @@ int ref_warp(int kind, int sw, int sh, const uint8_t *src, int dw, int dh, const
     }
     int s = run(c); if (s) return s;
     CHECK(get(out, dw, dh, dst, 1));
+    if (time_iters > 0 && sec_per_frame) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < time_iters; i++) CHECK(vxProcessGraph(c.graph));
+        auto t1 = std::chrono::steady_clock::now();
+        *sec_per_frame = std::chrono::duration<double>(t1 - t0).count() / time_iters;
+    }
     return 0;
 }
 
 // Remap bilinear from a float map (dw*dh {x, y} pairs, set through vxCopyRemapPatch); border as above.
 int ref_remap(int sw, int sh, const uint8_t *src, int dw, int dh, const float *map, int constant, int border,
-              uint8_t *dst) {
+              uint8_t *dst, int time_iters, double *sec_per_frame) {
     Ctx c;
     vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
     vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
@@ -328,6 +334,12 @@ int ref_remap(int sw, int sh, const uint8_t *src, int dw, int dh, const float *m
     }
     int s = run(c); if (s) return s;
     CHECK(get(out, dw, dh, dst, 1));
+    if (time_iters > 0 && sec_per_frame) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < time_iters; i++) CHECK(vxProcessGraph(c.graph));
+        auto t1 = std::chrono::steady_clock::now();
+        *sec_per_frame = std::chrono::duration<double>(t1 - t0).count() / time_iters;
+    }
     return 0;
 }
 
