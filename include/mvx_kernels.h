@@ -277,6 +277,57 @@ int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *params, 
 int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes,
                               void *workspace, size_t workspaceSize, int stage);
 
+
+/* ------------------------------------------------------------------ batched chain ---------- */
+/* Gaussian3x3 -> Sobel3x3 GXY -> Magnitude + Phase, and ScaleImage (bilinear) of the input to half
+ * size, for a batch of U8 images of uneven sizes in one persistent launch.  Per image the outputs
+ * equal those of the node sequence (the blurred image virtual):
+ *   mvx_Gaussian_U8_U8_3x3 -> mvx_Sobel_S16S16_U8_3x3_GXY -> mvx_Magnitude_S16_S16S16, mvx_Phase_U8_S16S16,
+ *   mvx_ScaleImage_U8_U8_Bilinear(width/2, height/2) of the input,
+ * i.e. the reference's HafCpu_Gaussian_U8_U8_3x3 (ago_haf_cpu_filter.cpp:1681-1861),
+ * HafCpu_Sobel_S16S16_U8_3x3_GXY (:2237), HafCpu_Magnitude_S16_S16S16 (ago_haf_cpu_arithmetic.cpp:4989),
+ * HafCpu_Phase_U8_S16S16 (:9592) and HafCpu_ScaleImage_U8_U8_Bilinear (ago_haf_cpu_geometric.cpp:1977)
+ * run one node at a time; the reference has no batched launch (its executor launches one kernel per
+ * node and frame, ago_util_hip.cpp:686-738).  Rows 0 and height-1 of gx/gy are not written.
+ * Widths and heights are even (2..65536), h*srcStride < 2^31; half may be NULL. */
+typedef struct mvx_chain_image {
+    const uint8_t *src; uint32_t width, height, srcStride;
+    int16_t *gx; uint32_t gxStride;
+    int16_t *gy; uint32_t gyStride;
+    int16_t *mag; uint32_t magStride;
+    uint8_t *phase; uint32_t phaseStride;
+    uint8_t *half; uint32_t halfStride;          /* (width/2) x (height/2) U8, or NULL */
+} mvx_chain_image_t;
+
+/* tile shapes (width x height): 16x16, 32x16, 32x32, 64x16, 64x32 */
+#define MVX_CHAIN_TILE_COUNT 5
+/* workgroups per CU tried by the autotuner: 1, 2, 4 and the occupancy limit */
+#define MVX_CHAIN_OCCUPANCY_STEPS 4
+
+typedef struct mvx_chain_plan {
+    void *table;                                 /* device plan (in the caller's workspace) */
+    uint32_t count, numTiles, tile, workgroups;  /* images, tiles, tile shape index, persistent grid */
+    uint32_t tileWidth, tileHeight;
+} mvx_chain_plan_t;
+
+size_t mvx_ChainBatch_workspace_size(uint32_t count);
+/* Validates the images and writes the device plan into `workspace` (device memory, 16-byte
+ * aligned); synchronous with respect to the host.  workgroupsPerCU 0 = the occupancy limit. */
+int mvx_ChainBatch_prepare(const mvx_chain_image_t *images, uint32_t count, uint32_t tile, uint32_t workgroupsPerCU,
+                           void *workspace, size_t workspaceSize, mvx_chain_plan_t *plan);
+/* One persistent launch over every tile of the plan; asynchronous, capturable. */
+int mvx_ChainBatch_run(mvx_stream_t stream, const mvx_chain_plan_t *plan);
+/* Times every tile shape x workgroups-per-CU step (reps launches each, HIP events on `stream`),
+ * prepares the fastest into `workspace` and returns its plan; msTable (optional) receives
+ * MVX_CHAIN_TILE_COUNT * MVX_CHAIN_OCCUPANCY_STEPS milliseconds per launch.  Synchronises `stream`;
+ * the outputs hold the result of the last timed launch. */
+int mvx_ChainBatch_autotune(mvx_stream_t stream, const mvx_chain_image_t *images, uint32_t count, uint32_t reps,
+                            void *workspace, size_t workspaceSize, mvx_chain_plan_t *best, float *msTable);
+/* Captures one mvx_ChainBatch_run of `plan` into a HIP graph and instantiates it (synchronous). */
+int mvx_ChainBatch_graph_create(const mvx_chain_plan_t *plan, void **graphExec);
+int mvx_graph_launch(void *graphExec, mvx_stream_t stream);
+int mvx_graph_destroy(void *graphExec);
+
 #ifdef __cplusplus
 }
 #endif

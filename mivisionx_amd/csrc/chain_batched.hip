@@ -1,0 +1,430 @@
+// chain_batched.hip -- Gaussian3x3 -> Sobel3x3 (gx, gy) -> Magnitude + Phase, plus ScaleImage to half
+// size, for a batch of U8 images of uneven sizes in ONE persistent launch.
+//
+// Per image the launch computes exactly what five reference nodes compute one after another
+// (HafCpu_Gaussian_U8_U8_3x3 ago_haf_cpu_filter.cpp:1681-1861 into a virtual image,
+// HafCpu_Sobel_S16S16_U8_3x3_GXY :2237, HafCpu_Magnitude_S16_S16S16 ago_haf_cpu_arithmetic.cpp:4989,
+// HafCpu_Phase_U8_S16S16 :9592, HafCpu_ScaleImage_U8_U8_Bilinear ago_haf_cpu_geometric.cpp:1977 on
+// the input), but the blurred image never leaves LDS.
+//
+// Tiles.  An output tile is TW x TH pixels of one image.  Its workgroup stages the input rows
+// y0-2 .. y0+TH+1 over columns x0-16 .. x0+TW+15 in LDS, each cell holding the byte at LINEAR
+// index r*stride + c of the plane (0 outside [0, h*stride)): the reference reads 3x3 neighbours by
+// linear address, so the left neighbour of column 0 is the previous row's last byte and the right
+// neighbour of column w-1 is padding or the next row's first byte, and this "extended coordinate"
+// tile reproduces all of that without a special case.  The blurred tile (rows y0-1 .. y0+TH,
+// columns x0-8 .. x0+TW+7) is then computed from it with the Gaussian node's row rule -- a
+// blurred pixel is nonzero only on rows 1..h-2 and columns < w, because the node writes nothing
+// else and the plane starts zeroed -- and Sobel, Magnitude and Phase run from the blurred tile.
+// Rows 0 and h-1 of gx/gy are not written (the Sobel node leaves them); Magnitude and Phase of
+// those rows are computed from zero gradients, which is what the nodes compute from the zeroed
+// rows.  The half-size image is computed from the same input tile.
+//
+// Schedule.  Tiles of all images form one sequence (image after image, row-major inside an
+// image).  The grid is persistent: it is a multiple of 8 workgroups, workgroup b runs on XCD b%8
+// (MI355X dispatches workgroups round-robin over its 8 XCDs), the sequence is cut into 8 equal
+// contiguous ranges, one per XCD, and the workgroups of an XCD stride through its range, so tiles
+// that share halo rows are processed at the same time on the same L2.  No atomics: the schedule is
+// static, so a captured hipGraph replays without any reset.
+#include "arith.hpp"
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace mvx {
+
+struct ChainRow {  // device plan: one 128-byte row per image
+    const uint8_t *src;
+    int16_t *gx, *gy, *mag;
+    uint8_t *phase, *half;
+    uint32_t w, h;
+    uint32_t srcStride, gxStride, gyStride, magStride, phaseStride, halfStride;
+    uint32_t tilesX, tileBegin;  // tiles per row; index of the image's first tile in the sequence
+    uint32_t pad[10];
+};
+static_assert(sizeof(ChainRow) == 128, "plan row is 128 bytes");
+
+template <int TW, int TH> struct ChainTile {
+    static constexpr int PIN = TW + 32;               // input tile pitch (columns x0-16 .. x0+TW+15)
+    static constexpr int RIN = TH + 4;                // input tile rows (y0-2 .. y0+TH+1)
+    static constexpr int PB = TW + 16;                // blurred tile pitch (columns x0-8 .. x0+TW+7)
+    static constexpr int RB = TH + 2;                 // blurred tile rows (y0-1 .. y0+TH)
+    static constexpr int CB = TW / 8 + 2;             // 8-pixel chunks per blurred row
+    static constexpr int NT = (TW * TH / 8 < 64) ? 64 : TW * TH / 8;   // threads: one 8-pixel chunk each
+    static_assert(TW % 16 == 0 && TH % 2 == 0, "tile width is a multiple of 16, height even");
+};
+
+MVX_DEV void unpack8(uint2 v, uint32_t *o)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) { o[i] = (v.x >> (8 * i)) & 0xffu; o[4 + i] = (v.y >> (8 * i)) & 0xffu; }
+}
+
+MVX_DEV void row24(const uint8_t *p, uint32_t (&b)[24])  // 24 bytes at an 8-byte aligned LDS address
+{
+    const uint2 *q = reinterpret_cast<const uint2 *>(p);
+    unpack8(q[0], b);
+    unpack8(q[1], b + 8);
+    unpack8(q[2], b + 16);
+}
+
+// blurred pixel validity at extended coordinate (r, c): the Gaussian node writes rows 1..h-2,
+// columns 0..w-1; every other byte of the virtual plane is zero.
+MVX_DEV bool blur_valid(int r, int c, int stride, int w, int h)
+{
+    if (c < 0) { r -= 1; c += stride; }
+    else if (c >= stride) { const int q = c / stride; r += q; c -= q * stride; }
+    return r >= 1 && r <= h - 2 && c < w;
+}
+
+template <int TW, int TH>
+__global__ void __launch_bounds__((ChainTile<TW, TH>::NT))
+k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTiles)
+{
+    using T = ChainTile<TW, TH>;
+    __shared__ __align__(16) uint8_t sIn[T::RIN * T::PIN];
+    __shared__ __align__(16) uint8_t sBlur[T::RB * T::PB];
+
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, perXcd = gridDim.x >> 3;
+    const uint32_t span = (numTiles + 7u) / 8u;
+    const uint32_t tBegin = xcd * span, tEnd = min(numTiles, tBegin + span);
+
+    for (uint32_t t = tBegin + slot; t < tEnd; t += perXcd) {
+        // image of tile t: last row with tileBegin <= t
+        uint32_t lo = 0, hi = count - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (plan[mid].tileBegin <= t) lo = mid; else hi = mid - 1;
+        }
+        const ChainRow &d = plan[lo];
+        const int w = (int)d.w, h = (int)d.h, stride = (int)d.srcStride;
+        const uint32_t local = t - d.tileBegin;
+        const int ty = (int)(local / d.tilesX), tx = (int)(local - (uint32_t)ty * d.tilesX);
+        const int x0 = tx * TW, y0 = ty * TH;
+        const long n = (long)h * stride;
+
+        // 1. input tile: 16-byte chunks at linear index (y0-2+row)*stride + x0-16+16k.  stride and x0
+        //    are multiples of 16, so a chunk lies wholly inside or wholly outside [0, n).
+        for (int i = threadIdx.x; i < T::RIN * (T::PIN / 16); i += T::NT) {
+            const int row = i / (T::PIN / 16), k = i - row * (T::PIN / 16);
+            const long L = (long)(y0 - 2 + row) * stride + (x0 - 16 + 16 * k);
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (L >= 0 && L < n) v = ld16(d.src + L);
+            *reinterpret_cast<uint4 *>(sIn + row * T::PIN + 16 * k) = v;
+        }
+        __syncthreads();
+
+        // 2. blurred tile, 8 pixels per chunk: (h(r-1) + 2 h(r) + h(r+1)) >> 4 with
+        //    h = p(c-1) + 2 p(c) + p(c+1), as the Gaussian node computes it.
+        for (int i = threadIdx.x; i < T::RB * T::CB; i += T::NT) {
+            const int rb = i / T::CB, cb = i - rb * T::CB;
+            uint32_t hs[8], b[24];
+            row24(sIn + rb * T::PIN + 8 * cb, b);
+#pragma unroll
+            for (int j = 0; j < 8; j++) hs[j] = b[7 + j] + 2 * b[8 + j] + b[9 + j];
+            row24(sIn + (rb + 1) * T::PIN + 8 * cb, b);
+#pragma unroll
+            for (int j = 0; j < 8; j++) hs[j] += 2 * (b[7 + j] + 2 * b[8 + j] + b[9 + j]);
+            row24(sIn + (rb + 2) * T::PIN + 8 * cb, b);
+#pragma unroll
+            for (int j = 0; j < 8; j++) hs[j] = (hs[j] + b[7 + j] + 2 * b[8 + j] + b[9 + j]) >> 4;
+            const int r = y0 - 1 + rb, c0 = x0 - 8 + 8 * cb;
+            if (c0 >= 0 && c0 + 8 <= stride) {
+                const bool rowOk = r >= 1 && r <= h - 2;
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (!rowOk || c0 + j >= w) hs[j] = 0;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (!blur_valid(r, c0 + j, stride, w, h)) hs[j] = 0;
+            }
+            uint2 o;
+            o.x = hs[0] | (hs[1] << 8) | (hs[2] << 16) | (hs[3] << 24);
+            o.y = hs[4] | (hs[5] << 8) | (hs[6] << 16) | (hs[7] << 24);
+            *reinterpret_cast<uint2 *>(sBlur + rb * T::PB + 8 * cb) = o;
+        }
+        __syncthreads();
+
+        // 3. Sobel, Magnitude, Phase: 8 pixels of one row per chunk
+        const int magBody = w & ~7;
+        for (int i = threadIdx.x; i < TH * (TW / 8); i += T::NT) {
+            const int ro = i / (TW / 8), co = i - ro * (TW / 8);
+            const int y = y0 + ro, x = x0 + 8 * co;
+            if (y >= h || x >= w) continue;
+            int32_t gx[8], gy[8];
+            const bool rowOk = y >= 1 && y <= h - 2;
+            if (rowOk) {
+                uint32_t b0[24], b1[24], b2[24];
+                row24(sBlur + ro * T::PB + 8 * co, b0);
+                row24(sBlur + (ro + 1) * T::PB + 8 * co, b1);
+                row24(sBlur + (ro + 2) * T::PB + 8 * co, b2);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    gx[j] = ((int)b0[9 + j] - (int)b0[7 + j]) + 2 * ((int)b1[9 + j] - (int)b1[7 + j]) +
+                            ((int)b2[9 + j] - (int)b2[7 + j]);
+                    gy[j] = (int)(b2[7 + j] + 2 * b2[8 + j] + b2[9 + j]) - (int)(b0[7 + j] + 2 * b0[8 + j] + b0[9 + j]);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) gx[j] = gy[j] = 0;
+            }
+            int32_t mg[8];
+            uint32_t ph[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                mg[j] = (x + j < magBody) ? mag_body(gx[j], gy[j]) : mag_tail(gx[j], gy[j]);
+                ph[j] = phase_px(gx[j], gy[j]);
+            }
+            int16_t *pgx = (int16_t *)((uint8_t *)d.gx + (long)y * d.gxStride) + x;
+            int16_t *pgy = (int16_t *)((uint8_t *)d.gy + (long)y * d.gyStride) + x;
+            int16_t *pmg = (int16_t *)((uint8_t *)d.mag + (long)y * d.magStride) + x;
+            uint8_t *pph = d.phase + (long)y * d.phaseStride + x;
+            if (x + 8 <= w) {
+                if (rowOk) {
+                    uint4 vx, vy;
+                    vx.x = ((uint32_t)gx[0] & 0xffffu) | ((uint32_t)gx[1] << 16);
+                    vx.y = ((uint32_t)gx[2] & 0xffffu) | ((uint32_t)gx[3] << 16);
+                    vx.z = ((uint32_t)gx[4] & 0xffffu) | ((uint32_t)gx[5] << 16);
+                    vx.w = ((uint32_t)gx[6] & 0xffffu) | ((uint32_t)gx[7] << 16);
+                    vy.x = ((uint32_t)gy[0] & 0xffffu) | ((uint32_t)gy[1] << 16);
+                    vy.y = ((uint32_t)gy[2] & 0xffffu) | ((uint32_t)gy[3] << 16);
+                    vy.z = ((uint32_t)gy[4] & 0xffffu) | ((uint32_t)gy[5] << 16);
+                    vy.w = ((uint32_t)gy[6] & 0xffffu) | ((uint32_t)gy[7] << 16);
+                    *reinterpret_cast<uint4 *>(pgx) = vx;
+                    *reinterpret_cast<uint4 *>(pgy) = vy;
+                }
+                uint4 vm;
+                vm.x = ((uint32_t)mg[0] & 0xffffu) | ((uint32_t)mg[1] << 16);
+                vm.y = ((uint32_t)mg[2] & 0xffffu) | ((uint32_t)mg[3] << 16);
+                vm.z = ((uint32_t)mg[4] & 0xffffu) | ((uint32_t)mg[5] << 16);
+                vm.w = ((uint32_t)mg[6] & 0xffffu) | ((uint32_t)mg[7] << 16);
+                *reinterpret_cast<uint4 *>(pmg) = vm;
+                uint2 vp;
+                vp.x = ph[0] | (ph[1] << 8) | (ph[2] << 16) | (ph[3] << 24);
+                vp.y = ph[4] | (ph[5] << 8) | (ph[6] << 16) | (ph[7] << 24);
+                *reinterpret_cast<uint2 *>(pph) = vp;
+            } else {
+                for (int j = 0; x + j < w; j++) {
+                    if (rowOk) { pgx[j] = (int16_t)gx[j]; pgy[j] = (int16_t)gy[j]; }
+                    pmg[j] = (int16_t)mg[j];
+                    pph[j] = (uint8_t)ph[j];
+                }
+            }
+        }
+
+        // 4. ScaleImage to (w/2, h/2).  With xscale = yscale = 2 the reference's fixed-point setup
+        //    (xinc = 2^19, xoffs = 2^17) gives source column 2x and row 2y with weight 128 on both
+        //    pixels, so the two-stage SIMD rule is ((p00+p01+1)>>1 + (p10+p11+1)>>1 + 1) >> 1 and
+        //    the single-rounding tail (columns >= (w/2) & ~7) is (p00+p01+p10+p11+2) >> 2; the row
+        //    and column clamps never engage.
+        if (d.half) {
+            const int dw = w >> 1, dh = h >> 1, halfBody = dw & ~7;
+            for (int i = threadIdx.x; i < (TH / 2) * (TW / 16); i += T::NT) {
+                const int rh = i / (TW / 16), k = i - rh * (TW / 16);
+                const int yh = (y0 >> 1) + rh, xh = (x0 >> 1) + 8 * k;
+                if (yh >= dh || xh >= dw) continue;
+                const uint4 r0 = *reinterpret_cast<const uint4 *>(sIn + (2 * rh + 2) * T::PIN + 16 + 16 * k);
+                const uint4 r1 = *reinterpret_cast<const uint4 *>(sIn + (2 * rh + 3) * T::PIN + 16 + 16 * k);
+                uint32_t a[16], c[16], o[8];
+                unpack16(r0, a);
+                unpack16(r1, c);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    if (xh + j < halfBody) {
+                        const uint32_t t0 = (a[2 * j] + a[2 * j + 1] + 1) >> 1, t1 = (c[2 * j] + c[2 * j + 1] + 1) >> 1;
+                        o[j] = (t0 + t1 + 1) >> 1;
+                    } else {
+                        o[j] = (a[2 * j] + a[2 * j + 1] + c[2 * j] + c[2 * j + 1] + 2) >> 2;
+                    }
+                }
+                uint8_t *ph = d.half + (long)yh * d.halfStride + xh;
+                if (xh + 8 <= dw) {
+                    uint2 v;
+                    v.x = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+                    v.y = o[4] | (o[5] << 8) | (o[6] << 16) | (o[7] << 24);
+                    *reinterpret_cast<uint2 *>(ph) = v;
+                } else {
+                    for (int j = 0; xh + j < dw; j++) ph[j] = (uint8_t)o[j];
+                }
+            }
+        }
+        __syncthreads();  // the next tile overwrites both LDS tiles
+    }
+}
+
+// ------------------------------------------------------------------ host side ------------------
+struct TileShape { int tw, th; };
+constexpr TileShape kShapes[MVX_CHAIN_TILE_COUNT] = {{16, 16}, {32, 16}, {32, 32}, {64, 16}, {64, 32}};
+
+typedef void (*ChainKernel)(const ChainRow *, uint32_t, uint32_t);
+constexpr ChainKernel kKernels[MVX_CHAIN_TILE_COUNT] = {k_chain_batch<16, 16>, k_chain_batch<32, 16>, k_chain_batch<32, 32>,
+                                                        k_chain_batch<64, 16>, k_chain_batch<64, 32>};
+constexpr int kThreads[MVX_CHAIN_TILE_COUNT] = {ChainTile<16, 16>::NT, ChainTile<32, 16>::NT, ChainTile<32, 32>::NT,
+                                                ChainTile<64, 16>::NT, ChainTile<64, 32>::NT};
+
+inline bool plane_ok_cb(const void *p, uint32_t w, uint32_t stride, uint32_t bpp)
+{
+    return p && aligned16(p) && (stride % 16u) == 0 && stride >= ((w * bpp + 15u) & ~15u);
+}
+
+bool image_ok(const mvx_chain_image_t &im)
+{
+    const uint32_t w = im.width, h = im.height;
+    if (w < 2 || h < 2 || (w & 1u) || (h & 1u) || w > 65536u || h > 65536u) return false;
+    if ((uint64_t)h * im.srcStride >= (1ull << 31)) return false;
+    if (!plane_ok_cb(im.src, w, im.srcStride, 1) || !plane_ok_cb(im.gx, w, im.gxStride, 2) ||
+        !plane_ok_cb(im.gy, w, im.gyStride, 2) || !plane_ok_cb(im.mag, w, im.magStride, 2) ||
+        !plane_ok_cb(im.phase, w, im.phaseStride, 1))
+        return false;
+    if (im.half && !plane_ok_cb(im.half, w / 2, im.halfStride, 1)) return false;
+    return true;
+}
+
+int device_cus()
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return cus;
+}
+
+}  // namespace mvx
+
+using namespace mvx;
+
+extern "C" size_t mvx_ChainBatch_workspace_size(uint32_t count) { return (size_t)count * sizeof(ChainRow); }
+
+extern "C" int mvx_ChainBatch_prepare(const mvx_chain_image_t *images, uint32_t count, uint32_t tile,
+                                      uint32_t workgroupsPerCU, void *workspace, size_t workspaceSize,
+                                      mvx_chain_plan_t *plan)
+{
+    if (!images || !plan || count == 0 || tile >= MVX_CHAIN_TILE_COUNT || !workspace || !aligned16(workspace) ||
+        workspaceSize < mvx_ChainBatch_workspace_size(count))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    const TileShape s = kShapes[tile];
+    std::vector<ChainRow> rows(count);
+    uint64_t tiles = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const mvx_chain_image_t &im = images[i];
+        if (!image_ok(im)) return MVX_ERROR_INVALID_PARAMETERS;
+        ChainRow &r = rows[i];
+        std::memset(&r, 0, sizeof(r));
+        r.src = im.src; r.gx = im.gx; r.gy = im.gy; r.mag = im.mag; r.phase = im.phase; r.half = im.half;
+        r.w = im.width; r.h = im.height;
+        r.srcStride = im.srcStride; r.gxStride = im.gxStride; r.gyStride = im.gyStride;
+        r.magStride = im.magStride; r.phaseStride = im.phaseStride; r.halfStride = im.halfStride;
+        r.tilesX = (im.width + s.tw - 1) / s.tw;
+        r.tileBegin = (uint32_t)tiles;
+        tiles += (uint64_t)r.tilesX * ((im.height + s.th - 1) / s.th);
+        if (tiles >= (1ull << 31)) return MVX_ERROR_INVALID_PARAMETERS;
+    }
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(kKernels[tile]), kThreads[tile], 0) !=
+            hipSuccess || occ <= 0)
+        return MVX_FAILURE;
+    const int cus = device_cus();
+    if (cus <= 0) return MVX_FAILURE;
+    const uint32_t perCu = workgroupsPerCU == 0 ? (uint32_t)occ : std::min<uint32_t>(workgroupsPerCU, (uint32_t)occ);
+    uint32_t wgs = (uint32_t)cus * perCu;
+    // no more workgroups than tiles (rounded to a multiple of the 8 XCDs)
+    wgs = std::min<uint32_t>(wgs, (uint32_t)((tiles + 7) / 8 * 8));
+    wgs = std::max<uint32_t>(8u, wgs & ~7u);
+    if (hipMemcpy(workspace, rows.data(), rows.size() * sizeof(ChainRow), hipMemcpyHostToDevice) != hipSuccess)
+        return MVX_FAILURE;
+    plan->table = workspace;
+    plan->count = count;
+    plan->numTiles = (uint32_t)tiles;
+    plan->tile = tile;
+    plan->workgroups = wgs;
+    plan->tileWidth = (uint32_t)s.tw;
+    plan->tileHeight = (uint32_t)s.th;
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_ChainBatch_run(mvx_stream_t stream, const mvx_chain_plan_t *plan)
+{
+    if (!plan || !plan->table || plan->tile >= MVX_CHAIN_TILE_COUNT || plan->count == 0 || plan->workgroups < 8 ||
+        (plan->workgroups & 7u))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    if (plan->numTiles == 0) return MVX_SUCCESS;
+    hipLaunchKernelGGL(kKernels[plan->tile], dim3(plan->workgroups), dim3(kThreads[plan->tile]), 0, stream_of(stream),
+                       (const ChainRow *)plan->table, plan->count, plan->numTiles);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_ChainBatch_autotune(mvx_stream_t stream, const mvx_chain_image_t *images, uint32_t count,
+                                       uint32_t reps, void *workspace, size_t workspaceSize, mvx_chain_plan_t *best,
+                                       float *msTable)
+{
+    if (!best || reps == 0) return MVX_ERROR_INVALID_PARAMETERS;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return MVX_FAILURE;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return MVX_FAILURE; }
+    static const uint32_t kPerCu[MVX_CHAIN_OCCUPANCY_STEPS] = {1, 2, 4, 0};  // 0 = the occupancy limit
+    float bestMs = -1.f;
+    uint32_t bestTile = 0, bestPerCu = 0;
+    int rc = MVX_SUCCESS;
+    for (uint32_t tile = 0; tile < MVX_CHAIN_TILE_COUNT && rc == MVX_SUCCESS; tile++) {
+        for (int o = 0; o < MVX_CHAIN_OCCUPANCY_STEPS && rc == MVX_SUCCESS; o++) {
+            mvx_chain_plan_t p;
+            rc = mvx_ChainBatch_prepare(images, count, tile, kPerCu[o], workspace, workspaceSize, &p);
+            if (rc != MVX_SUCCESS) break;
+            rc = mvx_ChainBatch_run(stream, &p);  // warm-up
+            if (rc != MVX_SUCCESS) break;
+            (void)hipEventRecord(e0, stream_of(stream));
+            for (uint32_t k = 0; k < reps && rc == MVX_SUCCESS; k++) rc = mvx_ChainBatch_run(stream, &p);
+            (void)hipEventRecord(e1, stream_of(stream));
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = MVX_FAILURE; break; }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)reps;
+            if (msTable) msTable[tile * MVX_CHAIN_OCCUPANCY_STEPS + o] = ms;
+            if (bestMs < 0.f || ms < bestMs) { bestMs = ms; bestTile = tile; bestPerCu = kPerCu[o]; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != MVX_SUCCESS) return rc;
+    // leave the workspace holding the winner's table
+    return mvx_ChainBatch_prepare(images, count, bestTile, bestPerCu, workspace, workspaceSize, best);
+}
+
+extern "C" int mvx_ChainBatch_graph_create(const mvx_chain_plan_t *plan, void **graphExec)
+{
+    if (!plan || !graphExec) return MVX_ERROR_INVALID_PARAMETERS;
+    *graphExec = nullptr;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return MVX_FAILURE;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    int rc = MVX_FAILURE;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const int launched = mvx_ChainBatch_run(s, plan);
+        const hipError_t ended = hipStreamEndCapture(s, &g);
+        if (launched == MVX_SUCCESS && ended == hipSuccess && g &&
+            hipGraphInstantiate(&e, g, nullptr, nullptr, 0) == hipSuccess) {
+            *graphExec = e;
+            rc = MVX_SUCCESS;
+        } else if (launched != MVX_SUCCESS) {
+            rc = launched;
+        }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+extern "C" int mvx_graph_launch(void *graphExec, mvx_stream_t stream)
+{
+    if (!graphExec) return MVX_ERROR_INVALID_PARAMETERS;
+    return hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graphExec), stream_of(stream)) == hipSuccess ? MVX_SUCCESS
+                                                                                                       : MVX_FAILURE;
+}
+
+extern "C" int mvx_graph_destroy(void *graphExec)
+{
+    if (!graphExec) return MVX_SUCCESS;
+    return hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graphExec)) == hipSuccess ? MVX_SUCCESS : MVX_FAILURE;
+}

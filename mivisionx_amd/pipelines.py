@@ -239,3 +239,129 @@ class PyrLKTracker:
 
     def result(self):
         return self.out[:28 * self.n].cpu().numpy().view(KEYPOINT).copy()
+
+
+class ChainImage(ctypes.Structure):
+    """mvx_chain_image_t"""
+    _fields_ = [("src", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("srcStride", ctypes.c_uint32),
+                ("gx", ctypes.c_void_p), ("gxStride", ctypes.c_uint32),
+                ("gy", ctypes.c_void_p), ("gyStride", ctypes.c_uint32),
+                ("mag", ctypes.c_void_p), ("magStride", ctypes.c_uint32),
+                ("phase", ctypes.c_void_p), ("phaseStride", ctypes.c_uint32),
+                ("half", ctypes.c_void_p), ("halfStride", ctypes.c_uint32)]
+
+
+class ChainPlan(ctypes.Structure):
+    """mvx_chain_plan_t"""
+    _fields_ = [("table", ctypes.c_void_p), ("count", ctypes.c_uint32), ("numTiles", ctypes.c_uint32),
+                ("tile", ctypes.c_uint32), ("workgroups", ctypes.c_uint32),
+                ("tileWidth", ctypes.c_uint32), ("tileHeight", ctypes.c_uint32)]
+
+
+CHAIN_TILES = [(16, 16), (32, 16), (32, 32), (64, 16), (64, 32)]
+CHAIN_OCCUPANCY_STEPS = [1, 2, 4, 0]
+
+
+class ChainBatch:
+    """Gaussian3x3 -> Sobel GXY -> Magnitude + Phase and ScaleImage to half size over a batch of U8
+    device planes of uneven sizes: one persistent launch (mvx_ChainBatch_*), or the same work as
+    plain per-node launches, image after image (``run_per_image``)."""
+
+    def __init__(self, sources, with_half=True):
+        K = kernels()
+        K.mvx_ChainBatch_workspace_size.restype = ctypes.c_size_t
+        self.src = list(sources)
+        self.gx, self.gy, self.mag, self.phase, self.half, self.blur = [], [], [], [], [], []
+        arr = (ChainImage * len(self.src))()
+        for i, s in enumerate(self.src):
+            w, h = s.width, s.height
+            gx, gy, mag, ph = Plane(w, h, "S16"), Plane(w, h, "S16"), Plane(w, h, "S16"), Plane(w, h, "U8")
+            hf = Plane(w // 2, h // 2, "U8") if with_half else None
+            self.gx.append(gx); self.gy.append(gy); self.mag.append(mag); self.phase.append(ph); self.half.append(hf)
+            arr[i] = ChainImage(s.t.data_ptr(), w, h, s.stride, gx.t.data_ptr(), gx.stride, gy.t.data_ptr(), gy.stride,
+                                mag.t.data_ptr(), mag.stride, ph.t.data_ptr(), ph.stride,
+                                hf.t.data_ptr() if hf else None, hf.stride if hf else 0)
+        self._arr = arr
+        self.ws_size = K.mvx_ChainBatch_workspace_size(len(self.src))
+        self.ws = _dev_bytes(self.ws_size)
+        self.plan = ChainPlan()
+        self.graph_exec = ctypes.c_void_p(None)
+
+    def prepare(self, tile=4, per_cu=0):
+        rc = kernels().mvx_ChainBatch_prepare(self._arr, len(self.src), tile, per_cu, _ptr(self.ws),
+                                              ctypes.c_size_t(self.ws_size), ctypes.byref(self.plan))
+        if rc != 0:
+            raise RuntimeError(f"mvx_ChainBatch_prepare returned {rc}")
+        self.destroy_graph()
+        return self
+
+    def autotune(self, reps=3):
+        """times every tile shape x workgroups-per-CU step; keeps the fastest plan; returns
+        {(tile_w, tile_h, per_cu): ms}"""
+        n = len(CHAIN_TILES) * len(CHAIN_OCCUPANCY_STEPS)
+        table = (ctypes.c_float * n)()
+        rc = kernels().mvx_ChainBatch_autotune(_stream(), self._arr, len(self.src), reps, _ptr(self.ws),
+                                               ctypes.c_size_t(self.ws_size), ctypes.byref(self.plan), table)
+        if rc != 0:
+            raise RuntimeError(f"mvx_ChainBatch_autotune returned {rc}")
+        self.destroy_graph()
+        out = {}
+        for t, (tw, th) in enumerate(CHAIN_TILES):
+            for o, pc in enumerate(CHAIN_OCCUPANCY_STEPS):
+                out[(tw, th, pc)] = table[t * len(CHAIN_OCCUPANCY_STEPS) + o]
+        return out
+
+    def run(self):
+        rc = kernels().mvx_ChainBatch_run(_stream(), ctypes.byref(self.plan))
+        if rc != 0:
+            raise RuntimeError(f"mvx_ChainBatch_run returned {rc}")
+
+    def capture(self):
+        """instantiates a HIP graph holding the batched launch of the current plan"""
+        self.destroy_graph()
+        rc = kernels().mvx_ChainBatch_graph_create(ctypes.byref(self.plan), ctypes.byref(self.graph_exec))
+        if rc != 0:
+            raise RuntimeError(f"mvx_ChainBatch_graph_create returned {rc}")
+
+    def launch_graph(self):
+        rc = kernels().mvx_graph_launch(self.graph_exec, _stream())
+        if rc != 0:
+            raise RuntimeError(f"mvx_graph_launch returned {rc}")
+
+    def destroy_graph(self):
+        if self.graph_exec.value:
+            kernels().mvx_graph_destroy(self.graph_exec)
+            self.graph_exec = ctypes.c_void_p(None)
+
+    def run_per_image(self):
+        """the reference executor's model: one launch per node and image"""
+        if not self.blur:
+            self.blur = [Plane(s.width, s.height, "U8") for s in self.src]
+        for s, b, gx, gy, mag, ph, hf in zip(self.src, self.blur, self.gx, self.gy, self.mag, self.phase, self.half):
+            w, h = s.width, s.height
+            call("Gaussian_U8_U8_3x3", w, h, b, b.stride, s, s.stride)
+            call("Sobel_S16S16_U8_3x3_GXY", w, h, gx, gx.stride, gy, gy.stride, b, b.stride)
+            call("Magnitude_S16_S16S16", w, h, mag, mag.stride, gx, gx.stride, gy, gy.stride)
+            call("Phase_U8_S16S16", w, h, ph, ph.stride, gx, gx.stride, gy, gy.stride)
+            if hf is not None:
+                call("ScaleImage_U8_U8_Bilinear", w // 2, h // 2, hf, hf.stride, w, h, s, s.stride)
+
+    def clear_outputs(self):
+        for planes in (self.gx, self.gy, self.mag, self.phase, self.half):
+            for p in planes:
+                if p is not None:
+                    p.t.fill_(0)
+
+    def results(self, i):
+        out = dict(gx=self.gx[i].download(), gy=self.gy[i].download(), mag=self.mag[i].download(),
+                   phase=self.phase[i].download())
+        if self.half[i] is not None:
+            out["half"] = self.half[i].download()
+        return out
+
+    def __del__(self):
+        try:
+            self.destroy_graph()
+        except Exception:
+            pass

@@ -181,3 +181,63 @@ def lk_keypoints(w, h, n, seed=1, margin=16):
     kp["y"] = margin + (r[1::2] % np.uint32(h - 2 * margin)).astype(np.int32)
     kp["tracking_status"] = 1
     return kp
+
+
+def batch_sizes(n=256, seed=0):
+    """The uneven-size batch: n (width, height) pairs, width uniform over {320, 336, ..., 4096} and
+    height uniform over {240, 256, ..., 2160} (stream 4 of `seed`)."""
+    u = u32(seed, 4, 2 * n).astype(np.uint64)
+    ws = 320 + 16 * ((u[0::2] * np.uint64(237)) >> np.uint64(32)).astype(np.int64)
+    hs = 240 + 16 * ((u[1::2] * np.uint64(121)) >> np.uint64(32)).astype(np.int64)
+    return [(int(a), int(b)) for a, b in zip(ws, hs)]
+
+
+def _signed64(v):
+    v = int(v) & 0xFFFFFFFFFFFFFFFF
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def _mix64_t(x):
+    import torch
+    x = x ^ ((x >> 30) & ((1 << 34) - 1))
+    x = x * _signed64(_M1)
+    x = x ^ ((x >> 27) & ((1 << 37) - 1))
+    x = x * _signed64(_M2)
+    return x ^ ((x >> 31) & ((1 << 33) - 1))
+
+
+def _uniform01_t(seed, stream, n, device):
+    """uniform01 computed with torch on `device` (int64 arithmetic wraps like uint64)"""
+    import torch
+    with np.errstate(over="ignore"):
+        key = _mix64(np.array([seed], dtype=np.uint64) * _GOLD + np.uint64(stream + 1))[0]
+    idx = torch.arange(n, dtype=torch.int64, device=device)
+    v = (_mix64_t(idx * _signed64(_GOLD) + _signed64(key)) >> 32) & 0xFFFFFFFF
+    return v.to(torch.float64) * (1.0 / 4294967296.0)
+
+
+def scene_u8_device(w, h, seed=0, device="cuda"):
+    """``scene_u8`` computed with torch on `device` (float64), for large batches: the same random
+    streams and formulas; transcendental functions may differ from numpy's in the last bit, which
+    moves a pixel by one level only where the noisy value sits on a rounding boundary."""
+    import torch
+    p = uniform01(seed, 100, 8 * 4)
+    xs = torch.arange(w, dtype=torch.float64, device=device)
+    ys = torch.arange(h, dtype=torch.float64, device=device)
+    img = torch.full((h, w), 128.0, dtype=torch.float64, device=device)
+    for k in range(8):
+        amp = 10.0 + 30.0 * p[4 * k]
+        period = 16.0 * np.exp(np.log(512.0 / 16.0) * p[4 * k + 1])
+        theta = np.pi * p[4 * k + 2]
+        phase = 2.0 * np.pi * p[4 * k + 3]
+        a = 2.0 * np.pi * np.cos(theta) / period
+        b = 2.0 * np.pi * np.sin(theta) / period
+        sx, cx = torch.sin(a * xs), torch.cos(a * xs)
+        sy, cy = torch.sin(b * ys + phase), torch.cos(b * ys + phase)
+        img += amp * (torch.outer(cy, sx) + torch.outer(sy, cx))
+    for (x0, y0, rw, rh, vals) in _rects(w, h, seed, 1):
+        img[y0:y0 + rh, x0:x0 + rw] = vals[0]
+    u1 = 1.0 - _uniform01_t(seed, 300, w * h, device)
+    u2 = _uniform01_t(seed, 400, w * h, device)
+    img += (8.0 * torch.sqrt(-2.0 * torch.log(u1)) * torch.cos(2.0 * np.pi * u2)).reshape(h, w)
+    return torch.clamp(torch.round(img), 0, 255).to(torch.uint8)
