@@ -18,6 +18,12 @@ Also reported:
                     fixed-point table of a barrel map (6 B/px), OpticalFlowPyrLK 4 levels / 9x9 / 20
                     iterations / 10,000 keypoints on 3840x2160 (keypoints/s);
   graph_hbm      -- the whole graph's algorithmic bytes per frame / frame time;
+  batched_uneven -- 256 U8 images of uneven sizes (W 320..4096, H 240..2160), each Gaussian3x3 ->
+                    Sobel3x3 -> Magnitude + Phase plus ScaleImage to half size, through the OpenVX
+                    engine: the chain-batch supernode (one persistent launch from a HIP graph) versus
+                    per-node launches (merge off), and the batch kernel's achieved GB/s (8.25 B/px);
+  alignment      -- 64 pairs of 1024x1024 F32 images, Adam on the 3-level pyramid L2 loss through the
+                    fused warp + loss + adjoint kernel: time per step and recovered homography error;
   per_image      -- the same frames through one launch per node and frame (the reference's execution
                     model), and the batched/per-image ratio;
   cpu_baseline   -- the reference's own CPU engine (oracle/_ref, one thread) on a bounded sample of the
@@ -134,6 +140,164 @@ def measure_nodes(reps=10):
     return out
 
 
+def measure_batched_uneven(reps=5):
+    """256 U8 images of uneven sizes, each Gaussian3x3 -> Sobel3x3 -> Magnitude + Phase plus ScaleImage
+    to half size, through the OpenVX engine: one graph holding all 256 chains, run (a) with the
+    chain-batch supernode -- one persistent launch replayed from a HIP graph -- and (b) with supernode
+    merging off -- 1280 per-node launches, the reference executor's model.  Wall time per
+    vxProcessGraph (it synchronises); the batch kernel also timed alone with HIP events."""
+    import ctypes as c
+    import numpy as np
+    import torch
+    from mivisionx_amd import openvx as vx, synth
+    from mivisionx_amd.device import Plane
+    from mivisionx_amd.pipelines import ChainBatch
+
+    L = vx.lib()
+    sizes = synth.batch_sizes()
+    px = sum(w * h for w, h in sizes)
+    planes = [Plane(w, h, "U8") for (w, h) in sizes]
+    for i, (p, (w, h)) in enumerate(zip(planes, sizes)):
+        p.t[:, :w].copy_(synth.scene_u8_device(w, h, seed=i))
+    torch.cuda.synchronize()
+    ctx = L.vxCreateContext()
+    messages = []
+    cb = vx.LOG_CB(lambda ctx_, ref, status, text: messages.append(text.decode()))
+    L.vxRegisterLogCallback(ctx, cb, 0)
+    srcs = []
+    for p in planes:
+        addr = vx.Addressing(p.width, p.height, 1, p.stride, 1024, 1024, 1, 1, 0)
+        srcs.append(L.vxCreateImageFromHandle(ctx, vx.DF["U8"], c.byref(addr), (c.c_void_p * 1)(p.t.data_ptr()), vx.VX_MEMORY_TYPE_HIP))
+
+    def build(merge):
+        g = L.vxCreateGraph(ctx)
+        if not merge:
+            flags = c.c_uint32(vx.AGO_GRAPH_OPTIMIZER_FLAG_NO_SUPERNODE_MERGE)
+            vx.check(L.vxSetGraphAttribute(g, vx.VX_GRAPH_ATTRIBUTE_AMD_OPTIMIZER_FLAGS, c.byref(flags), 4))
+        outs = []
+        for s, (w, h) in zip(srcs, sizes):
+            blur = L.vxCreateVirtualImage(g, 0, 0, vx.DF["VIRT"])
+            o = [vx.create_image(ctx, w, h, f) for f in ("S16", "S16", "S16", "U8")] + [vx.create_image(ctx, w // 2, h // 2, "U8")]
+            assert L.vxGaussian3x3Node(g, s, blur) and L.vxSobel3x3Node(g, blur, o[0], o[1])
+            assert L.vxMagnitudeNode(g, o[0], o[1], o[2]) and L.vxPhaseNode(g, o[0], o[1], o[3])
+            assert L.vxScaleImageNode(g, s, o[4], vx.VX_INTERPOLATION_BILINEAR)
+            outs.append(o)
+        t = time.perf_counter()
+        vx.check(L.vxVerifyGraph(g), "vxVerifyGraph")
+        return g, outs, time.perf_counter() - t
+
+    def wall(g):
+        for _ in range(2):
+            vx.check(L.vxProcessGraph(g))
+        t = time.perf_counter()
+        for _ in range(reps):
+            vx.check(L.vxProcessGraph(g))
+        return (time.perf_counter() - t) / reps * 1e3
+
+    gb, ob, verify_b = build(True)
+    note = next((m.strip() for m in messages if "chain batch supernode" in m), "")
+    ms_b = wall(gb)
+    gp, op, verify_p = build(False)
+    ms_p = wall(gp)
+    same = all(np.array_equal(vx.read_image(ob[i][k]), vx.read_image(op[i][k])) for i in (0, 255) for k in range(5))
+    # the batch kernel alone (C ABI, the engine's tile choice), HIP events on its launch stream
+    cbatch = ChainBatch(planes)
+    tile = next(i for i, t in enumerate(((16, 16), (32, 16), (32, 32), (64, 16), (64, 32))) if f"tile {t[0]}x{t[1]}" in note) \
+        if "tile" in note else 4
+    cbatch.prepare(tile)
+    cbatch.capture()
+    for _ in range(2):
+        cbatch.launch_graph()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        cbatch.launch_graph()
+    b.record()
+    torch.cuda.synchronize()
+    k_ms = a.elapsed_time(b) / reps
+    alg = int(px * 8.25)
+    for g in (gb, gp):
+        gg = c.c_void_p(g)
+        L.vxReleaseGraph(c.byref(gg))
+    cc = c.c_void_p(ctx)
+    L.vxReleaseContext(c.byref(cc))
+    return {
+        "workload": "256 U8 images, W uniform in {320..4096}, H in {240..2160} (multiples of 16), config-2 scene generator; "
+                    "Gaussian3x3 -> Sobel3x3 -> Magnitude + Phase (blurred image virtual) + ScaleImage bilinear to half size",
+        "images": len(sizes), "mpix": round(px / 1e6, 2),
+        "batched": {"ms": round(ms_b, 3), "value": round(px / (ms_b / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                    "how": "vxProcessGraph, chain-batch supernode: one persistent launch replayed from a HIP graph",
+                    "engine_log": note, "verify_s": round(verify_b, 2)},
+        "per_image": {"ms": round(ms_p, 3), "value": round(px / (ms_p / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                      "how": "vxProcessGraph with supernode merge off: 5 launches per image (1280 per run)",
+                      "verify_s": round(verify_p, 2)},
+        "batched_over_per_image": round(ms_p / ms_b, 2),
+        "outputs_identical": bool(same),
+        "kernel": {"name": "k_chain_batch", "ms": round(k_ms, 4), "algorithmic_bytes": alg,
+                   "achieved_GBps": round(alg / (k_ms / 1e3) / 1e9, 1), "frac": round(alg / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "bytes_per_px": "1 in + 4 gx/gy + 2 magnitude + 1 phase + 0.25 half = 8.25"},
+    }
+
+
+def measure_alignment(pairs=64, steps=200):
+    """64 pairs of 1024x1024 F32 images: Adam on the 3-level pyramid L2 loss through the fused
+    warp + loss + adjoint kernel; time per step and the recovered homography error"""
+    import torch
+    from mivisionx_amd import align
+    src, tgt, Ht = align.make_pairs(pairs, 1024, 1024)
+    al = align.Aligner(src, tgt)
+    p = torch.zeros((pairs, 8), dtype=torch.float64, device="cuda").requires_grad_(True)
+    opt = torch.optim.Adam([p], lr=2e-3)
+    px_step = pairs * sum(a.shape[1] * a.shape[2] for a in al.ps)
+    # fused kernel alone, HIP events on its stream (level 0)
+    Hl = align.level_H(align.H_from_p(p.detach(), 1024, 1024), 0)
+    lv = torch.zeros(pairs, dtype=torch.float64, device="cuda")
+    gH = torch.zeros((pairs, 9), dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        al.l2[0](Hl, 1.0, lv, gH, False)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(5):
+        al.l2[0](Hl, 1.0, lv, gH, False)
+    b.record()
+    torch.cuda.synchronize()
+    k_ms = a.elapsed_time(b) / 5
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        _, g = al.loss_and_grad(p.detach())
+        p.grad = g
+        opt.step()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t) / steps * 1e3
+    err = align.corner_error(align.H_from_p(p.detach(), 1024, 1024), Ht, 1024, 1024)
+    alg = pairs * 1024 * 1024 * 8
+    return {
+        "workload": f"{pairs} pairs of 1024x1024 F32, 3-level pyramid L2, {steps} Adam steps",
+        "ms_per_step": round(step_ms, 4), "value": round(px_step / (step_ms / 1e3) / 1e6, 1), "unit": "Mpix/s (pyramid pixels)",
+        "corner_error_px": {"mean": round(float(err.mean()), 4), "max": round(float(err.max()), 4),
+                            "pairs_below_0.5px": int((err < 0.5).sum())},
+        "kernel": {"name": "k_warp_f32_adjoint<L2> + k_reduce_partials, level 0", "ms": round(k_ms, 4), "algorithmic_bytes": alg,
+                   "achieved_GBps": round(alg / (k_ms / 1e3) / 1e9, 1), "frac": round(alg / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "bytes_per_px": "4 source (bilinear gather) + 4 target"},
+    }
+
+
+def cpu_batched_uneven(images=3):
+    """the reference CPU engine (one thread) on the first images of the uneven batch"""
+    from mivisionx_amd import synth
+    from pyoracle import Reference
+    ref = Reference()
+    tot_px, tot_s = 0, 0.0
+    for i, (w, h) in enumerate(synth.batch_sizes()[:images]):
+        _, sec = ref.chain_half(synth.scene_u8(w, h, seed=i), time_iters=2)
+        tot_px += w * h
+        tot_s += sec
+    return {"value": round(tot_px / tot_s / 1e6, 2), "unit": "Mpix/s", "cores": 1, "kind": "reference",
+            "sample": f"first {images} images of the uneven batch, Gaussian->Sobel->Magnitude+Phase + ScaleImage half, "
+                      "one graph each, 2 timed runs"}
+
+
 def cpu_nodes():
     """the reference CPU engine (oracle/_ref, one thread) on the node workloads, a few frames each"""
     from mivisionx_amd import synth
@@ -184,6 +348,18 @@ def cpu_baseline(rgb, budget_s=10.0):
             "ms_per_frame": round(spf * 1e3, 2)}
 
 
+def traffic_from_profiles(stage, frames):
+    """HBM bytes per launch of the dominant stage, from the committed rocprofv3 --pmc summary
+    (profiles/k_front_4k_traffic.md: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy
+    kernel's reading), per frame times the frames of one launch; None when no summary covers it"""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_front_4k_traffic.md")
+    if stage != "fused_tile_pass" or not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_frame:\s*(\d+)", open(path).read())
+    return int(m.group(1)) * frames if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +370,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-image", action="store_true")
     ap.add_argument("--no-nodes", action="store_true", help="skip the config-1 / config-3 node measurements")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
+    ap.add_argument("--no-align", action="store_true", help="skip the F32 alignment measurement")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run 3 launches of a plain 128 MiB device copy (ChannelCopy 16384x8192), the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings into bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the vision graph runs on a single MI355X here (asked for --gpus {args.gpus}); "
@@ -206,6 +387,14 @@ def main():
 
     torch.cuda.set_device(0)
     K = kernels()
+    if args.copy_calibration:
+        from mivisionx_amd.device import Plane, call
+        a, b = Plane(16384, 8192), Plane(16384, 8192)
+        a.t.random_(0, 256)
+        for _ in range(3):
+            call("ChannelCopy_U8_U8", 16384, 8192, b, b.stride, a, a.stride)
+        torch.cuda.synchronize()
+        del a, b
     base = [torch.from_numpy(synth.scene_rgb(W, H, s)) for s in (0, 1)]
     G = FusedVisionGraph(W, H, DW, DH, LEVELS, frames=args.frames, **PARAMS)
     for f, fr in enumerate(G.frames):
@@ -273,7 +462,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": STAGES[dominant], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(dom_bytes), "mean_launch_ms": round(dom_ms, 5),
-                     "traffic": None},
+                     "traffic": traffic_from_profiles(STAGES[dominant], len(G.frames))},
         "graph_hbm": {"algorithmic_bytes_per_frame": graph_bytes, "achieved": round(graph_gbs, 1), "unit": "GB/s",
                       "frac": round(graph_gbs / HBM_PEAK_GBS, 4)},
         "stage_ms_per_step": {STAGES[k]: round(v, 4) for k, v in stage_ms.items()},
@@ -301,11 +490,20 @@ def main():
                             "batched_over_per_image": round(value / pi_value, 2)}
     if not args.no_nodes:
         out["nodes"] = measure_nodes()
+    if not args.no_batch:
+        out["batched_uneven"] = measure_batched_uneven()
+    if not args.no_align:
+        out["alignment"] = measure_alignment()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(base[0].numpy(), args.cpu_budget)
         if not args.no_nodes:
             try:
                 out["cpu_baseline"]["nodes"] = cpu_nodes()
+            except FileNotFoundError:
+                pass
+        if not args.no_batch:
+            try:
+                out["cpu_baseline"]["batched_uneven"] = cpu_batched_uneven()
             except FileNotFoundError:
                 pass
     print(json.dumps(out))

@@ -28,11 +28,18 @@ k_magnitude(uint32_t w, uint32_t h, int16_t *__restrict__ dst, uint32_t dstStrid
     uint4 vy = *reinterpret_cast<const uint4 *>(py);
     const uint32_t ax[4] = {vx.x, vx.y, vx.z, vx.w}, ay[4] = {vy.x, vy.y, vy.z, vy.w};
     int32_t o[8];
+    int32_t a[8], b[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        int32_t a = (int16_t)(ax[i >> 1] >> (16 * (i & 1)));
-        int32_t b = (int16_t)(ay[i >> 1] >> (16 * (i & 1)));
-        o[i] = (x0 + i < body) ? mag_body(a, b) : mag_tail(a, b);
+        a[i] = (int16_t)(ax[i >> 1] >> (16 * (i & 1)));
+        b[i] = (int16_t)(ay[i >> 1] >> (16 * (i & 1)));
+    }
+    if (x0 + 8 <= body) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = mag_body(a[i], b[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = (x0 + i < body) ? mag_body(a[i], b[i]) : mag_tail(a[i], b[i]);
     }
     if (x0 + 8 <= w) {
         uint4 v;

@@ -77,6 +77,17 @@ MVX_DEV bool blur_valid(int r, int c, int stride, int w, int h)
     return r >= 1 && r <= h - 2 && c < w;
 }
 
+// image of tile t: the last plan row with tileBegin <= t (uniform across the workgroup)
+MVX_DEV uint32_t find_image(const ChainRow *__restrict__ plan, uint32_t count, uint32_t t)
+{
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (plan[mid].tileBegin <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <int TW, int TH>
 __global__ void __launch_bounds__((ChainTile<TW, TH>::NT))
 k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTiles)
@@ -89,30 +100,51 @@ k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTil
     const uint32_t span = (numTiles + 7u) / 8u;
     const uint32_t tBegin = xcd * span, tEnd = min(numTiles, tBegin + span);
 
-    for (uint32_t t = tBegin + slot; t < tEnd; t += perXcd) {
-        // image of tile t: last row with tileBegin <= t
-        uint32_t lo = 0, hi = count - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (plan[mid].tileBegin <= t) lo = mid; else hi = mid - 1;
+    // Loads of the next tile's input are issued into registers before the current tile is
+    // computed, so their latency hides behind the blur / gradient work of this tile.
+    constexpr int CHUNKS = T::RIN * (T::PIN / 16), LOADS = (CHUNKS + T::NT - 1) / T::NT;
+    uint4 pre[LOADS];
+    auto fetch = [&](uint32_t t) {
+        const uint32_t img = find_image(plan, count, t);
+        const ChainRow &d = plan[img];
+        const int stride = (int)d.srcStride;
+        const uint32_t local = t - d.tileBegin;
+        const int ty = (int)(local / d.tilesX), tx = (int)(local - (uint32_t)ty * d.tilesX);
+        const long n = (long)d.h * stride;
+#pragma unroll
+        for (int k = 0; k < LOADS; k++) {
+            const int i = threadIdx.x + k * T::NT;
+            pre[k] = make_uint4(0, 0, 0, 0);
+            if (i < CHUNKS) {
+                // 16-byte chunks at linear index (y0-2+row)*stride + x0-16+16c; stride and x0 are
+                // multiples of 16, so a chunk lies wholly inside or wholly outside [0, n)
+                const int row = i / (T::PIN / 16), c = i - row * (T::PIN / 16);
+                const long L = (long)(ty * TH - 2 + row) * stride + (tx * TW - 16 + 16 * c);
+                if (L >= 0 && L < n) pre[k] = ld16(d.src + L);
+            }
         }
-        const ChainRow &d = plan[lo];
+    };
+
+    uint32_t t = tBegin + slot;
+    if (t < tEnd) fetch(t);
+    for (; t < tEnd; t += perXcd) {
+        const ChainRow &d = plan[find_image(plan, count, t)];
         const int w = (int)d.w, h = (int)d.h, stride = (int)d.srcStride;
         const uint32_t local = t - d.tileBegin;
         const int ty = (int)(local / d.tilesX), tx = (int)(local - (uint32_t)ty * d.tilesX);
         const int x0 = tx * TW, y0 = ty * TH;
-        const long n = (long)h * stride;
 
-        // 1. input tile: 16-byte chunks at linear index (y0-2+row)*stride + x0-16+16k.  stride and x0
-        //    are multiples of 16, so a chunk lies wholly inside or wholly outside [0, n).
-        for (int i = threadIdx.x; i < T::RIN * (T::PIN / 16); i += T::NT) {
-            const int row = i / (T::PIN / 16), k = i - row * (T::PIN / 16);
-            const long L = (long)(y0 - 2 + row) * stride + (x0 - 16 + 16 * k);
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (L >= 0 && L < n) v = ld16(d.src + L);
-            *reinterpret_cast<uint4 *>(sIn + row * T::PIN + 16 * k) = v;
+        // 1. input tile (rows y0-2 .. y0+TH+1, columns x0-16 .. x0+TW+15) from the prefetched chunks
+#pragma unroll
+        for (int k = 0; k < LOADS; k++) {
+            const int i = threadIdx.x + k * T::NT;
+            if (i < CHUNKS) {
+                const int row = i / (T::PIN / 16), c = i - row * (T::PIN / 16);
+                *reinterpret_cast<uint4 *>(sIn + row * T::PIN + 16 * c) = pre[k];
+            }
         }
         __syncthreads();
+        if (t + perXcd < tEnd) fetch(t + perXcd);
 
         // 2. blurred tile, 8 pixels per chunk: (h(r-1) + 2 h(r) + h(r+1)) >> 4 with
         //    h = p(c-1) + 2 p(c) + p(c+1), as the Gaussian node computes it.
@@ -152,8 +184,13 @@ k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTil
             const int ro = i / (TW / 8), co = i - ro * (TW / 8);
             const int y = y0 + ro, x = x0 + 8 * co;
             if (y >= h || x >= w) continue;
-            int32_t gx[8], gy[8];
-            const bool rowOk = y >= 1 && y <= h - 2;
+            const bool rowOk = y >= 1 && y <= h - 2, full = x + 8 <= w;
+            int16_t *pgx = (int16_t *)((uint8_t *)d.gx + (long)y * d.gxStride) + x;
+            int16_t *pgy = (int16_t *)((uint8_t *)d.gy + (long)y * d.gyStride) + x;
+            int16_t *pmg = (int16_t *)((uint8_t *)d.mag + (long)y * d.magStride) + x;
+            uint8_t *pph = d.phase + (long)y * d.phaseStride + x;
+            // gx | gy << 16 per pixel (both in [-1020, 1020]); zero on rows the Sobel node skips
+            uint32_t g[8];
             if (rowOk) {
                 uint32_t b0[24], b1[24], b2[24];
                 row24(sBlur + ro * T::PB + 8 * co, b0);
@@ -161,55 +198,52 @@ k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTil
                 row24(sBlur + (ro + 2) * T::PB + 8 * co, b2);
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    gx[j] = ((int)b0[9 + j] - (int)b0[7 + j]) + 2 * ((int)b1[9 + j] - (int)b1[7 + j]) +
-                            ((int)b2[9 + j] - (int)b2[7 + j]);
-                    gy[j] = (int)(b2[7 + j] + 2 * b2[8 + j] + b2[9 + j]) - (int)(b0[7 + j] + 2 * b0[8 + j] + b0[9 + j]);
+                    const int gx = ((int)b0[9 + j] - (int)b0[7 + j]) + 2 * ((int)b1[9 + j] - (int)b1[7 + j]) +
+                                   ((int)b2[9 + j] - (int)b2[7 + j]);
+                    const int gy = (int)(b2[7 + j] + 2 * b2[8 + j] + b2[9 + j]) - (int)(b0[7 + j] + 2 * b0[8 + j] + b0[9 + j]);
+                    g[j] = ((uint32_t)gx & 0xffffu) | ((uint32_t)gy << 16);
                 }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; j++) gx[j] = gy[j] = 0;
-            }
-            int32_t mg[8];
-            uint32_t ph[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                mg[j] = (x + j < magBody) ? mag_body(gx[j], gy[j]) : mag_tail(gx[j], gy[j]);
-                ph[j] = phase_px(gx[j], gy[j]);
-            }
-            int16_t *pgx = (int16_t *)((uint8_t *)d.gx + (long)y * d.gxStride) + x;
-            int16_t *pgy = (int16_t *)((uint8_t *)d.gy + (long)y * d.gyStride) + x;
-            int16_t *pmg = (int16_t *)((uint8_t *)d.mag + (long)y * d.magStride) + x;
-            uint8_t *pph = d.phase + (long)y * d.phaseStride + x;
-            if (x + 8 <= w) {
-                if (rowOk) {
+                if (full) {
                     uint4 vx, vy;
-                    vx.x = ((uint32_t)gx[0] & 0xffffu) | ((uint32_t)gx[1] << 16);
-                    vx.y = ((uint32_t)gx[2] & 0xffffu) | ((uint32_t)gx[3] << 16);
-                    vx.z = ((uint32_t)gx[4] & 0xffffu) | ((uint32_t)gx[5] << 16);
-                    vx.w = ((uint32_t)gx[6] & 0xffffu) | ((uint32_t)gx[7] << 16);
-                    vy.x = ((uint32_t)gy[0] & 0xffffu) | ((uint32_t)gy[1] << 16);
-                    vy.y = ((uint32_t)gy[2] & 0xffffu) | ((uint32_t)gy[3] << 16);
-                    vy.z = ((uint32_t)gy[4] & 0xffffu) | ((uint32_t)gy[5] << 16);
-                    vy.w = ((uint32_t)gy[6] & 0xffffu) | ((uint32_t)gy[7] << 16);
+                    vx.x = (g[0] & 0xffffu) | (g[1] << 16);
+                    vx.y = (g[2] & 0xffffu) | (g[3] << 16);
+                    vx.z = (g[4] & 0xffffu) | (g[5] << 16);
+                    vx.w = (g[6] & 0xffffu) | (g[7] << 16);
+                    vy.x = (g[0] >> 16) | (g[1] & 0xffff0000u);
+                    vy.y = (g[2] >> 16) | (g[3] & 0xffff0000u);
+                    vy.z = (g[4] >> 16) | (g[5] & 0xffff0000u);
+                    vy.w = (g[6] >> 16) | (g[7] & 0xffff0000u);
                     *reinterpret_cast<uint4 *>(pgx) = vx;
                     *reinterpret_cast<uint4 *>(pgy) = vy;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        if (x + j < w) { pgx[j] = (int16_t)(g[j] & 0xffffu); pgy[j] = (int16_t)(g[j] >> 16); }
                 }
-                uint4 vm;
-                vm.x = ((uint32_t)mg[0] & 0xffffu) | ((uint32_t)mg[1] << 16);
-                vm.y = ((uint32_t)mg[2] & 0xffffu) | ((uint32_t)mg[3] << 16);
-                vm.z = ((uint32_t)mg[4] & 0xffffu) | ((uint32_t)mg[5] << 16);
-                vm.w = ((uint32_t)mg[6] & 0xffffu) | ((uint32_t)mg[7] << 16);
-                *reinterpret_cast<uint4 *>(pmg) = vm;
-                uint2 vp;
-                vp.x = ph[0] | (ph[1] << 8) | (ph[2] << 16) | (ph[3] << 24);
-                vp.y = ph[4] | (ph[5] << 8) | (ph[6] << 16) | (ph[7] << 24);
-                *reinterpret_cast<uint2 *>(pph) = vp;
             } else {
-                for (int j = 0; x + j < w; j++) {
-                    if (rowOk) { pgx[j] = (int16_t)gx[j]; pgy[j] = (int16_t)gy[j]; }
-                    pmg[j] = (int16_t)mg[j];
-                    pph[j] = (uint8_t)ph[j];
-                }
+#pragma unroll
+                for (int j = 0; j < 8; j++) g[j] = 0;
+            }
+            // Magnitude and Phase, packed as they are produced
+            uint32_t mw[4] = {0, 0, 0, 0}, pw[2] = {0, 0};
+            const bool body = x + 8 <= magBody;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int32_t gx = (int16_t)(g[j] & 0xffffu), gy = (int32_t)g[j] >> 16;
+                const int32_t m = (body || x + j < magBody) ? mag_body(gx, gy) : mag_tail(gx, gy);
+                mw[j >> 1] |= ((uint32_t)m & 0xffffu) << (16 * (j & 1));
+                pw[j >> 2] |= phase_px(gx, gy) << (8 * (j & 3));
+            }
+            if (full) {
+                *reinterpret_cast<uint4 *>(pmg) = make_uint4(mw[0], mw[1], mw[2], mw[3]);
+                *reinterpret_cast<uint2 *>(pph) = make_uint2(pw[0], pw[1]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (x + j < w) {
+                        pmg[j] = (int16_t)(mw[j >> 1] >> (16 * (j & 1)));
+                        pph[j] = (uint8_t)(pw[j >> 2] >> (8 * (j & 3)));
+                    }
             }
         }
 

@@ -71,6 +71,30 @@ struct KernelDef {
 
 const std::vector<KernelDef> &kernel_table();
 
+// AMD low-level kernel enums (ago_kernels.h, enum vx_kernel_amd_e)
+enum : vx_enum {
+    AMD_REMAP_U8_U8_BILINEAR = 0xd000e6,
+    AMD_REMAP_U8_U8_BILINEAR_CONSTANT = 0xd000e7,
+    AMD_WARP_AFFINE_U8_U8_BILINEAR = 0xd000ea,
+    AMD_WARP_AFFINE_U8_U8_BILINEAR_CONSTANT = 0xd000eb,
+    AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR = 0xd000ee,
+    AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR_CONSTANT = 0xd000ef,
+    AMD_MAGNITUDE_S16_S16S16 = 0xd0006f,
+    AMD_PHASE_U8_S16S16 = 0xd00070,
+    AMD_CHANNEL_COPY_U8_U8 = 0xd00071,
+    AMD_COLOR_CONVERT_Y_RGB = 0xd000a2,
+    AMD_BOX_U8_U8_3x3 = 0xd000b0,
+    AMD_MEDIAN_U8_U8_3x3 = 0xd000b3,
+    AMD_GAUSSIAN_U8_U8_3x3 = 0xd000b4,
+    AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
+    AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,
+    AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM = 0xd000da,
+    AMD_SCALE_IMAGE_U8_U8_BILINEAR = 0xd000f1,
+};
+
+// what executes a node: its own execute command, or one of the graph's supernodes
+enum FusedKind : int { FUSED_NONE = 0, FUSED_VISION = 1, FUSED_CHAIN = 2 };
+
 }  // namespace mvxe
 
 struct _vx_reference {
@@ -218,12 +242,13 @@ struct _vx_node : _vx_reference {
     mvxe::DevBuf local;       // node workspace sized by the kernel's initialize command
     size_t localSize = 0;
     uint32_t target = AGO_TARGET_AFFINITY_GPU;
-    bool fused = false;       // executed by the graph's fused supernode
+    int fused = mvxe::FUSED_NONE;  // executed by one of the graph's supernodes (FusedKind)
     _vx_node(_vx_context *c) : _vx_reference(VX_TYPE_NODE, c) {}
 };
 
 namespace mvxe {
 struct FusedVision;  // the RGB vision supernode (vx_graph.cpp)
+struct FusedChains;  // the batched Gaussian -> Sobel -> Magnitude/Phase (+ScaleImage) supernode (vx_graph.cpp)
 }
 
 struct _vx_graph : _vx_reference {
@@ -237,6 +262,8 @@ struct _vx_graph : _vx_reference {
     vx_perf_t perf{};
     mvxe::FusedVision *fused = nullptr;
     int fusedAt = -1;                 // position in `order` where the supernode runs
+    mvxe::FusedChains *chains = nullptr;
+    int chainsAt = -1;                // position in `order` where the chain batch runs
     std::vector<_vx_array *> writtenArrays;
     std::vector<_vx_scalar *> writtenScalars;
     // vxScheduleGraph worker

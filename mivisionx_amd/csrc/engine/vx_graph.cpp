@@ -8,7 +8,10 @@
 // every CPU/GPU level change, ago_interface.cpp:2294-2309), and when a graph contains the RGB
 // vision chain (ColorConvert -> ChannelExtract(Y) -> ScaleImage / GaussianPyramid / Canny / Harris)
 // that chain runs as one fused supernode (mvx_VisionGraph_*), which the reference reserves
-// (ago_util_hip.cpp:740-744) but never launches.
+// (ago_util_hip.cpp:740-744) but never launches.  Every Gaussian3x3 -> Sobel3x3 -> Magnitude + Phase
+// chain over a graph input (optionally with a half-size bilinear ScaleImage of the same input) is
+// collected into a second supernode that runs all such chains, of any image sizes, as one
+// persistent launch replayed from a HIP graph (mvx_ChainBatch_*).
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
@@ -29,6 +32,19 @@ struct FusedVision {
     DevBuf ws;
     size_t wsSize = 0;
     DevBuf count;  // corner count when the Harris node has no num_corners scalar
+};
+
+struct ChainNodes {
+    _vx_node *gauss, *sobel, *mag, *phase, *scale;  // scale may be null
+    _vx_image *in, *blur, *gx, *gy, *magOut, *phaseOut, *half;
+};
+
+struct FusedChains {
+    std::vector<ChainNodes> chains;
+    std::vector<mvx_chain_image_t> images;
+    mvx_chain_plan_t plan{};
+    DevBuf ws;
+    void *graphExec = nullptr;  // instantiated HIP graph holding the batched launch
 };
 }  // namespace mvxe
 
@@ -85,8 +101,15 @@ void invalidate_graph(_vx_graph *g)
     for (_vx_node *n : g->nodes) {
         n->local.release();
         n->localSize = 0;
-        n->fused = false;
+        n->fused = FUSED_NONE;
     }
+    if (g->chains) {
+        if (g->chains->graphExec) mvx_graph_destroy(g->chains->graphExec);
+        g->chains->ws.release();
+        delete g->chains;
+        g->chains = nullptr;
+    }
+    g->chainsAt = -1;
     if (g->fused) {
         g->fused->ws.release();
         g->fused->count.release();
@@ -211,6 +234,137 @@ static vx_status fused_prepare(_vx_graph *g)
     return VX_SUCCESS;
 }
 
+// -------------------------------------------------------------------- batched chains -----------
+// Node roles, Khronos form (inputs first) or AMD form (output first, ago_kernel_list.cpp:339-474).
+
+static bool gauss_io(_vx_node *n, _vx_image **in, _vx_image **out)
+{
+    if (n->kernel->id == VX_KERNEL_GAUSSIAN_3x3) { *in = image_of(n->params[0]); *out = image_of(n->params[1]); return true; }
+    if (n->kernel->id == AMD_GAUSSIAN_U8_U8_3x3) { *in = image_of(n->params[1]); *out = image_of(n->params[0]); return true; }
+    return false;
+}
+
+static bool sobel_io(_vx_node *n, _vx_image **in, _vx_image **gx, _vx_image **gy)
+{
+    if (n->kernel->id == VX_KERNEL_SOBEL_3x3) {
+        *in = image_of(n->params[0]); *gx = image_of(n->params[1]); *gy = image_of(n->params[2]);
+        return true;
+    }
+    if (n->kernel->id == AMD_SOBEL_S16S16_U8_3x3_GXY) {
+        *gx = image_of(n->params[0]); *gy = image_of(n->params[1]); *in = image_of(n->params[2]);
+        return true;
+    }
+    return false;
+}
+
+// Magnitude (phase = false) or Phase (phase = true) reading (gx, gy)
+static bool gradpair_io(_vx_node *n, bool phase, _vx_image **gx, _vx_image **gy, _vx_image **out)
+{
+    const vx_enum khr = phase ? VX_KERNEL_PHASE : VX_KERNEL_MAGNITUDE, amd = phase ? AMD_PHASE_U8_S16S16 : AMD_MAGNITUDE_S16_S16S16;
+    if (n->kernel->id == khr) {
+        *gx = image_of(n->params[0]); *gy = image_of(n->params[1]); *out = image_of(n->params[2]);
+        return true;
+    }
+    if (n->kernel->id == amd) {
+        *out = image_of(n->params[0]); *gx = image_of(n->params[1]); *gy = image_of(n->params[2]);
+        return true;
+    }
+    return false;
+}
+
+static bool scale_io(_vx_node *n, _vx_image **in, _vx_image **out)
+{
+    if (n->kernel->id == VX_KERNEL_SCALE_IMAGE) {
+        _vx_scalar *interp = is_ref(n->params[2], VX_TYPE_SCALAR) ? (_vx_scalar *)n->params[2] : nullptr;
+        if (!interp || interp->value.e != VX_INTERPOLATION_BILINEAR) return false;
+        *in = image_of(n->params[0]); *out = image_of(n->params[1]);
+        return true;
+    }
+    if (n->kernel->id == AMD_SCALE_IMAGE_U8_U8_BILINEAR) { *in = image_of(n->params[1]); *out = image_of(n->params[0]); return true; }
+    return false;
+}
+
+static bool written_in_graph(_vx_graph *g, _vx_reference *r)
+{
+    for (_vx_node *n : g->nodes)
+        for (size_t i = 0; i < n->params.size(); i++)
+            if (n->params[i] && n->kernel->params[i].dir != VX_INPUT && root_of(n->params[i]) == root_of(r)) return true;
+    return false;
+}
+
+// every Gaussian3x3 (input: a U8 graph input of even size; output: virtual, read only by one
+// Sobel3x3 with both outputs) whose gradients feed one Magnitude and one Phase node, plus a bilinear
+// ScaleImage of the same input to exactly half size when the graph has one
+static FusedChains *find_chains(_vx_graph *g)
+{
+    FusedChains fc;
+    std::set<_vx_node *> used;
+    for (_vx_node *gs : g->order) {
+        _vx_image *in = nullptr, *blur = nullptr;
+        if (gs->fused || !gauss_io(gs, &in, &blur) || !in || !blur || !blur->isVirtual) continue;
+        if (in->format != VX_DF_IMAGE_U8 || in->parent || in->uniform || written_in_graph(g, in)) continue;
+        if (in->width < 2 || in->height < 2 || (in->width & 1) || (in->height & 1)) continue;
+        std::vector<_vx_node *> c = consumers(g, blur);
+        _vx_image *sin = nullptr, *gx = nullptr, *gy = nullptr;
+        if (c.size() != 1 || c[0]->fused || !sobel_io(c[0], &sin, &gx, &gy) || sin != blur || !gx || !gy) continue;
+        ChainNodes ch{gs, c[0], nullptr, nullptr, nullptr, in, blur, gx, gy, nullptr, nullptr, nullptr};
+        for (_vx_node *n : consumers(g, gx)) {
+            _vx_image *a = nullptr, *b = nullptr, *o = nullptr;
+            if (n->fused) continue;
+            if (gradpair_io(n, false, &a, &b, &o) && a == gx && b == gy && !ch.mag) { ch.mag = n; ch.magOut = o; }
+            else if (gradpair_io(n, true, &a, &b, &o) && a == gx && b == gy && !ch.phase) { ch.phase = n; ch.phaseOut = o; }
+        }
+        if (!ch.mag || !ch.phase) continue;
+        for (_vx_node *n : consumers(g, in)) {
+            _vx_image *si = nullptr, *so = nullptr;
+            if (n->fused || used.count(n) || !scale_io(n, &si, &so) || si != in || !so) continue;
+            if (so->format == VX_DF_IMAGE_U8 && so->width == in->width / 2 && so->height == in->height / 2) {
+                ch.scale = n;
+                ch.half = so;
+                break;
+            }
+        }
+        for (_vx_node *n : {ch.gauss, ch.sobel, ch.mag, ch.phase, ch.scale}) if (n) used.insert(n);
+        fc.chains.push_back(ch);
+    }
+    if (fc.chains.empty()) return nullptr;
+    return new FusedChains(fc);
+}
+
+static vx_status chains_prepare(_vx_graph *g)
+{
+    FusedChains *f = g->chains;
+    f->images.clear();
+    for (const ChainNodes &c : f->chains) {
+        mvx_chain_image_t im{};
+        im.src = c.in->plane_ptr(0); im.width = c.in->width; im.height = c.in->height; im.srcStride = c.in->planes[0].stride;
+        im.gx = (int16_t *)c.gx->plane_ptr(0); im.gxStride = c.gx->planes[0].stride;
+        im.gy = (int16_t *)c.gy->plane_ptr(0); im.gyStride = c.gy->planes[0].stride;
+        im.mag = (int16_t *)c.magOut->plane_ptr(0); im.magStride = c.magOut->planes[0].stride;
+        im.phase = c.phaseOut->plane_ptr(0); im.phaseStride = c.phaseOut->planes[0].stride;
+        if (c.half) { im.half = c.half->plane_ptr(0); im.halfStride = c.half->planes[0].stride; }
+        f->images.push_back(im);
+    }
+    const uint32_t n = (uint32_t)f->images.size();
+    vx_status s = f->ws.alloc(mvx_ChainBatch_workspace_size(n));
+    if (s) return s;
+    // the autotuner times every tile shape and grid on these very images (MVX_CHAIN_AUTOTUNE=0: the
+    // default 64x32 tile at full occupancy)
+    const char *tune = getenv("MVX_CHAIN_AUTOTUNE");
+    int r = (tune && tune[0] == '0')
+                ? mvx_ChainBatch_prepare(f->images.data(), n, 4, 0, f->ws.ptr(), f->ws.size, &f->plan)
+                : mvx_ChainBatch_autotune(g->stream, f->images.data(), n, 2, f->ws.ptr(), f->ws.size, &f->plan, nullptr);
+    if (!r) r = mvx_ChainBatch_graph_create(&f->plan, &f->graphExec);
+    if (r) {
+        log(g, r, "ERROR: vxVerifyGraph: chain batch supernode prepare returned %d\n", r);
+        return r;
+    }
+    log(g, VX_SUCCESS, "INFO: vxVerifyGraph: chain batch supernode: %u chains (gaussian_3x3, sobel_3x3, magnitude, phase, "
+                       "scale_image) in one launch, tile %ux%u, %u workgroups\n", n, f->plan.tileWidth, f->plan.tileHeight,
+        f->plan.workgroups);
+    return VX_SUCCESS;
+}
+
 // ------------------------------------------------------------------------------ verification -----
 
 static const char *short_name(const KernelDef *k)
@@ -314,11 +468,18 @@ vx_status verify_graph(_vx_graph *g)
                           getenv("MVX_NO_FUSION");
     if (!noFusion && (g->fused = find_fusion(g))) {
         FusedVision *f = g->fused;
-        for (_vx_node *n : {f->cc, f->ex, f->scale, f->pyr, f->canny, f->harris}) n->fused = true;
+        for (_vx_node *n : {f->cc, f->ex, f->scale, f->pyr, f->canny, f->harris}) n->fused = FUSED_VISION;
         for (size_t i = 0; i < g->order.size(); i++)
             if (g->order[i] == f->cc) g->fusedAt = (int)i;
         log(g, VX_SUCCESS, "INFO: vxVerifyGraph: fused vision supernode (color_convert, channel_extract, scale_image, "
                            "gaussian_pyramid, canny_edge_detector, harris_corners) %ux%u\n", f->params.width, f->params.height);
+    }
+    if (!noFusion && (g->chains = find_chains(g))) {
+        for (const ChainNodes &c : g->chains->chains)
+            for (_vx_node *n : {c.gauss, c.sobel, c.mag, c.phase, c.scale})
+                if (n) n->fused = FUSED_CHAIN;
+        for (size_t i = 0; i < g->order.size() && g->chainsAt < 0; i++)
+            if (g->order[i]->fused == FUSED_CHAIN) g->chainsAt = (int)i;
     }
     // 6. device memory: every object a node touches, each node's workspace, one stream
     vx_status s = ctx->ensure_hip();
@@ -335,6 +496,12 @@ vx_status verify_graph(_vx_graph *g)
                 _vx_image *im = (_vx_image *)r;
                 // the fused supernode keeps the virtual YUV image and luma in LDS: nothing to allocate
                 if (g->fused && im->isVirtual && (r == g->fused->cc->params[1] || r == g->fused->ex->params[2])) continue;
+                // the chain batch keeps each blurred image in LDS
+                if (g->chains && im->isVirtual && n->fused == FUSED_CHAIN) {
+                    bool blur = false;
+                    for (const ChainNodes &c : g->chains->chains) blur = blur || r == c.blur;
+                    if (blur) continue;
+                }
                 if ((s = im->allocate())) return s;
             } else if (r->type == VX_TYPE_PYRAMID) {
                 for (_vx_image *im : ((_vx_pyramid *)r)->images)
@@ -355,11 +522,14 @@ vx_status verify_graph(_vx_graph *g)
         }
     }
     if (g->fused && (s = fused_prepare(g))) return s;
+    if (g->chains && (s = chains_prepare(g))) return s;
     if (hipStreamSynchronize(g->stream) != hipSuccess) return VX_FAILURE;
     g->verified = true;
     g->state = VX_GRAPH_STATE_VERIFIED;
     if (getenv("MVX_VERBOSE")) {
-        for (_vx_node *n : g->order) log(g, VX_SUCCESS, "  node %s%s\n", short_name(n->kernel), n->fused ? " (fused supernode)" : "");
+        for (_vx_node *n : g->order)
+            log(g, VX_SUCCESS, "  node %s%s\n", short_name(n->kernel),
+                n->fused == FUSED_VISION ? " (fused vision supernode)" : n->fused == FUSED_CHAIN ? " (chain batch supernode)" : "");
     }
     return VX_SUCCESS;
 }
@@ -375,6 +545,12 @@ vx_status execute_graph(_vx_graph *g)
     vx_status s = VX_SUCCESS;
     for (size_t i = 0; i < g->order.size() && !s; i++) {
         _vx_node *n = g->order[i];
+        if (n->fused == FUSED_CHAIN) {
+            if ((int)i != g->chainsAt) continue;
+            const int r = mvx_graph_launch(g->chains->graphExec, g->stream);
+            if (r) { log(g, r, "ERROR: vxProcessGraph: chain batch supernode returned %d\n", r); s = r; }
+            continue;
+        }
         if (n->fused) {
             if ((int)i != g->fusedAt) continue;
             FusedVision *f = g->fused;

@@ -19,26 +19,6 @@ namespace {
 
 constexpr vx_enum IN = VX_INPUT, OUT = VX_OUTPUT;
 
-// AMD low-level kernel enums (ago_kernels.h, enum vx_kernel_amd_e)
-enum : vx_enum {
-    AMD_REMAP_U8_U8_BILINEAR = 0xd000e6,
-    AMD_REMAP_U8_U8_BILINEAR_CONSTANT = 0xd000e7,
-    AMD_WARP_AFFINE_U8_U8_BILINEAR = 0xd000ea,
-    AMD_WARP_AFFINE_U8_U8_BILINEAR_CONSTANT = 0xd000eb,
-    AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR = 0xd000ee,
-    AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR_CONSTANT = 0xd000ef,
-    AMD_MAGNITUDE_S16_S16S16 = 0xd0006f,
-    AMD_PHASE_U8_S16S16 = 0xd00070,
-    AMD_CHANNEL_COPY_U8_U8 = 0xd00071,
-    AMD_COLOR_CONVERT_Y_RGB = 0xd000a2,
-    AMD_BOX_U8_U8_3x3 = 0xd000b0,
-    AMD_MEDIAN_U8_U8_3x3 = 0xd000b3,
-    AMD_GAUSSIAN_U8_U8_3x3 = 0xd000b4,
-    AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
-    AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,
-    AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM = 0xd000da,
-    AMD_SCALE_IMAGE_U8_U8_BILINEAR = 0xd000f1,
-};
 
 _vx_image *img(_vx_node *n, int i) { return image_of(n->params[i]); }
 _vx_scalar *scl(_vx_node *n, int i) { return is_ref(n->params[i], VX_TYPE_SCALAR) ? (_vx_scalar *)n->params[i] : nullptr; }

@@ -296,6 +296,18 @@ class Reference:
             return (gx, gy, mag, ph), sec.value
         return gx, gy, mag, ph
 
+    def chain_half(self, src, time_iters=0):
+        """Gaussian (virtual) -> Sobel -> Magnitude + Phase, and ScaleImage to half size, one graph"""
+        h, w = src.shape
+        gx = np.zeros((h, w), np.int16); gy = np.zeros((h, w), np.int16)
+        mag = np.zeros((h, w), np.int16); ph = np.zeros((h, w), np.uint8)
+        half = np.zeros((h // 2, w // 2), np.uint8)
+        sec = ctypes.c_double(0.0)
+        self._ok(self.lib.ref_chain_half(w, h, _p(_c(src)), _p(gx), _p(gy), _p(mag), _p(ph), _p(half),
+                                         time_iters, ctypes.byref(sec)), "chain_half")
+        out = dict(gx=gx, gy=gy, mag=mag, phase=ph, half=half)
+        return (out, sec.value) if time_iters else out
+
     def color_y(self, rgb):
         h, w = rgb.shape[:2]
         y = np.zeros((h, w), np.uint8)

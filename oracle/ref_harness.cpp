@@ -147,6 +147,38 @@ int ref_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16
     return 0;
 }
 
+// The uneven-batch chain on one image: Gaussian3x3 (virtual) -> Sobel3x3 -> Magnitude + Phase and
+// ScaleImage bilinear of the input to (w/2, h/2), one graph; timing as ref_chain1.
+int ref_chain_half(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16_t *mag, uint8_t *phase, uint8_t *half,
+                   int time_iters, double *sec_per_frame) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_image g = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_U8);
+    vx_image ox = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16);
+    vx_image oy = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16);
+    vx_image om = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16);
+    vx_image op = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    vx_image oh = vxCreateImage(c.ctx, w / 2, h / 2, VX_DF_IMAGE_U8);
+    if (!vxGaussian3x3Node(c.graph, in, g)) return -1;
+    if (!vxSobel3x3Node(c.graph, g, ox, oy)) return -1;
+    if (!vxMagnitudeNode(c.graph, ox, oy, om)) return -1;
+    if (!vxPhaseNode(c.graph, ox, oy, op)) return -1;
+    if (!vxScaleImageNode(c.graph, in, oh, VX_INTERPOLATION_BILINEAR)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(ox, w, h, gx, 2));
+    CHECK(get(oy, w, h, gy, 2));
+    CHECK(get(om, w, h, mag, 2));
+    CHECK(get(op, w, h, phase, 1));
+    CHECK(get(oh, w / 2, h / 2, half, 1));
+    if (time_iters > 0 && sec_per_frame) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < time_iters; i++) CHECK(vxProcessGraph(c.graph));
+        auto t1 = std::chrono::steady_clock::now();
+        *sec_per_frame = std::chrono::duration<double>(t1 - t0).count() / time_iters;
+    }
+    return 0;
+}
+
 // ColorConvert RGB -> IYUV (virtual) -> ChannelExtract Y.
 int ref_color_y(int w, int h, const uint8_t *rgb, uint8_t *y) {
     Ctx c;

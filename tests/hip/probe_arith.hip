@@ -1,0 +1,57 @@
+// probe_arith.hip -- test-only probes (libprobe_arith.so, loaded by tests/test_gpu_arith_exhaustive.py):
+// exhaustive device-side comparisons of the Magnitude / Phase building blocks of the product
+// kernels (mivisionx_amd/csrc/arith.hpp) against the operations they stand in for, over their
+// whole input domains.
+//   * div_small_rn(a, b) vs IEEE single division (__fdiv_rn), all integers 0 <= a <= b <= 32768;
+//   * mag_body(gx, gy) vs rint(sqrt((double)(gx^2 + gy^2))) saturated to 32767 -- the reference's
+//     cvtpd_epi32(sqrt_pd(.)) and packs (ago_haf_cpu_arithmetic.cpp:5031-5078) -- for all
+//     0 <= gy <= gx <= 32768 (the result depends on |gx|, |gy| only).
+#include "../../mivisionx_amd/csrc/arith.hpp"
+
+using namespace mvx;
+
+__global__ void k_div(uint32_t maxB, uint32_t *out)
+{
+    const uint32_t b = blockIdx.x + 1;
+    if (b > maxB) return;
+    for (uint32_t a = threadIdx.x; a <= b; a += blockDim.x) {
+        const float q = div_small_rn((float)a, (float)b);
+        const float r = __fdiv_rn((float)a, (float)b);
+        if (__float_as_uint(q) != __float_as_uint(r)) {
+            atomicAdd(out, 1u);
+            atomicCAS(out + 1, 0u, (b << 16) | a);
+        }
+    }
+}
+
+__global__ void k_mag(uint32_t maxA, uint32_t *out)
+{
+    const uint32_t gx = blockIdx.x;
+    if (gx > maxA) return;
+    for (uint32_t gy = threadIdx.x; gy <= gx; gy += blockDim.x) {
+        const int32_t m = mag_body((int32_t)gx, (int32_t)gy);
+        const double s = (double)gx * gx + (double)gy * gy;
+        const int32_t want = (int32_t)min(rint(sqrt(s)), 32767.0);
+        if (m != want) {
+            atomicAdd(out, 1u);
+            atomicCAS(out + 1, 0u, (gx << 16) | gy);
+        }
+    }
+}
+
+static int run(void (*kernel)(uint32_t, uint32_t *), uint32_t blocks, uint32_t arg, uint32_t *result)
+{
+    uint32_t *d = nullptr;
+    if (hipMalloc(&d, 8) != hipSuccess) return -1;
+    int rc = -1;
+    if (hipMemset(d, 0, 8) == hipSuccess) {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, 0, arg, d);
+        if (hipGetLastError() == hipSuccess && hipMemcpy(result, d, 8, hipMemcpyDeviceToHost) == hipSuccess) rc = 0;
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+// result[0] = mismatches, result[1] = first mismatch packed as (b << 16) | a (b, a <= 32768)
+extern "C" int probe_div_small(uint32_t maxB, uint32_t *result) { return run(k_div, maxB, maxB, result); }
+extern "C" int probe_mag_body(uint32_t maxA, uint32_t *result) { return run(k_mag, maxA + 1, maxA, result); }

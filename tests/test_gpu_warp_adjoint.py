@@ -1,0 +1,300 @@
+"""The differentiable F32 warp: forward kernels against float64 torch restatements of the same
+sampling rule, the hand-written adjoints against float64 autograd of those restatements and
+against FP64 central finite differences (eps 1e-3 px, relative error <= 1e-4), the adjoints wrapped
+as torch.autograd.Functions (including a warp of a pyramid level that recomputes the level in the
+backward pass instead of storing it), and homography recovery by Adam on the 3-level pyramid loss.
+
+Tolerances: forward values within 2e-5 absolute (values in [0, 1]; F32 coordinates); gradients
+within 1e-4 relative (norm over the 8 parameters, and per parameter where it is at least 1% of the
+largest)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from mivisionx_amd import align
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+
+
+# ------------------------------------------------------------------ float64 restatements --------
+def _sample64(src, u, v):
+    n, h, w = src.shape
+    x0, y0 = torch.floor(u), torch.floor(v)
+    fx, fy = u - x0, v - y0
+    flat = src.reshape(n, -1)
+
+    def px(xi, yi):
+        ok = (xi >= 0) & (xi < w) & (yi >= 0) & (yi < h)
+        idx = (yi.clamp(0, h - 1) * w + xi.clamp(0, w - 1)).long()
+        return torch.where(ok, flat.gather(1, idx), torch.zeros((), dtype=src.dtype, device=src.device))
+
+    p00, p01 = px(x0, y0), px(x0 + 1, y0)
+    p10, p11 = px(x0, y0 + 1), px(x0 + 1, y0 + 1)
+    return (1 - fy) * ((1 - fx) * p00 + fx * p01) + fy * ((1 - fx) * p10 + fx * p11)
+
+
+def warp64(src, H, out_hw=None):
+    n, h, w = src.shape
+    dh, dw = out_hw or (h, w)
+    ys, xs = torch.meshgrid(torch.arange(dh, dtype=F64, device=src.device), torch.arange(dw, dtype=F64, device=src.device),
+                            indexing="ij")
+    X, Y = xs.reshape(1, -1), ys.reshape(1, -1)
+    H = H.reshape(-1, 3, 3)
+    z = H[:, 2, 0, None] * X + H[:, 2, 1, None] * Y + H[:, 2, 2, None]
+    zs = torch.where(z > 0, z, torch.ones_like(z))
+    u = (H[:, 0, 0, None] * X + H[:, 0, 1, None] * Y + H[:, 0, 2, None]) / zs
+    v = (H[:, 1, 0, None] * X + H[:, 1, 1, None] * Y + H[:, 1, 2, None]) / zs
+    out = _sample64(src, u, v)
+    return torch.where(z > 0, out, torch.zeros_like(out)).reshape(n, dh, dw)
+
+
+def remap64(src, mapxy):
+    n, dh, dw, _ = mapxy.shape
+    return _sample64(src, mapxy[..., 0].reshape(n, -1), mapxy[..., 1].reshape(n, -1)).reshape(n, dh, dw)
+
+
+_W5 = [1.0, 4.0, 6.0, 4.0, 1.0]
+
+
+def pyr64(x):
+    n, h, w = x.shape
+    dh, dw = (h + 1) // 2, (w + 1) // 2
+    cols = [(2 * torch.arange(dw, device=x.device) + i - 2).clamp(0, w - 1) for i in range(5)]
+    rows = [(2 * torch.arange(dh, device=x.device) + j - 2).clamp(0, h - 1) for j in range(5)]
+    hs = sum(_W5[i] * x[:, :, cols[i]] for i in range(5))
+    return sum(_W5[j] * hs[:, rows[j], :] for j in range(5)) / 256.0
+
+
+def loss64(src_levels, tgt_levels, Hpix):
+    tot = torch.zeros(Hpix.shape[0], dtype=F64, device=Hpix.device)
+    for l, (s, t) in enumerate(zip(src_levels, tgt_levels)):
+        sc = 0.5 ** l
+        S = torch.diag(torch.tensor([sc, sc, 1.0], dtype=F64, device=Hpix.device))
+        Si = torch.diag(torch.tensor([1 / sc, 1 / sc, 1.0], dtype=F64, device=Hpix.device))
+        r = warp64(s, S @ Hpix @ Si) - t
+        tot = tot + (r * r).mean(dim=(1, 2))
+    return tot
+
+
+def _rel(a, b):
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
+def _check_grad(got, want, what, tol=1e-4):
+    assert _rel(got, want) <= tol, f"{what}: relative error {_rel(got, want):.3e}\n{got}\n{want}"
+    big = want.abs() >= 0.01 * want.abs().max()
+    per = ((got - want).abs() / want.abs().clamp_min(1e-300))[big]
+    assert float(per.max()) <= tol, f"{what}: per-parameter relative error {float(per.max()):.3e}\n{got}\n{want}"
+
+
+# ------------------------------------------------------------------ autograd wrappers ----------
+class WarpF32(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, src, H):
+        ctx.save_for_backward(src, H)
+        return align.warp(src, H)
+
+    @staticmethod
+    def backward(ctx, g):
+        src, H = ctx.saved_tensors
+        gsrc, gH = align.warp_adjoint(g.contiguous(), src, H, want_src=ctx.needs_input_grad[0])
+        return gsrc, gH.to(torch.float32)
+
+
+class WarpPyramidLevelF32(torch.autograd.Function):
+    """warp of pyramid level `level` of src: the level is recomputed in backward, never stored"""
+
+    @staticmethod
+    def forward(ctx, src, H, level):
+        ctx.save_for_backward(src, H)
+        ctx.level = level
+        return align.warp(align.pyramid(src, level + 1)[level], H)
+
+    @staticmethod
+    def backward(ctx, g):
+        src, H = ctx.saved_tensors
+        levels = align.pyramid(src, ctx.level + 1)
+        gl, gH = align.warp_adjoint(g.contiguous(), levels[-1], H, want_src=True)
+        for l in range(ctx.level, 0, -1):
+            gl = align.pyr_half_adjoint(gl, tuple(levels[l - 1].shape[1:]))
+        return gl, gH.to(torch.float32), None
+
+
+class RemapF32(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, src, mapxy):
+        ctx.save_for_backward(src, mapxy)
+        return align.remap(src, mapxy)
+
+    @staticmethod
+    def backward(ctx, g):
+        src, mapxy = ctx.saved_tensors
+        return align.remap_adjoint(g.contiguous(), src, mapxy, ctx.needs_input_grad[0], ctx.needs_input_grad[1])
+
+
+# ------------------------------------------------------------------ fixtures --------------------
+@pytest.fixture(scope="module")
+def pairs(gpu):
+    """2 pairs of the alignment workload at 256x256 and the parameters near which gradients are checked"""
+    w = h = 256
+    src, tgt, Ht = align.make_pairs(2, w, h, seed0=5000)
+    p_true = torch.as_tensor(align.true_params(2, w, h, 5100), device="cuda")
+    # 0.5 px-scale perturbation so that the gradient is far from zero
+    p = p_true + torch.tensor([[3e-3, -2e-3, 4e-3, 1e-3, 2e-3, -3e-3, 2e-3, -1e-3]], dtype=F64, device="cuda")
+    return src, tgt, p, w, h
+
+
+def test_forward_matches_float64(pairs):
+    src, tgt, p, w, h = pairs
+    Hp = align.H_from_p(p, w, h)
+    got = align.warp(src, Hp.reshape(-1, 9).float().contiguous())
+    want = warp64(src.double(), Hp)
+    assert float((got.double() - want).abs().max()) < 2e-5
+
+
+def test_forward_out_of_view_and_behind_camera(gpu):
+    src = torch.rand((1, 64, 80), device="cuda", dtype=torch.float32)
+    for H in ([1, 0, 500, 0, 1, 0, 0, 0, 1], [1, 0, 0, 0, 1, 0, 0, 0, -1], [0.5, 0.1, 10.3, -0.2, 1.7, -4.6, 1e-3, -2e-3, 1.1]):
+        Ht = torch.tensor([H], dtype=torch.float32, device="cuda")
+        got = align.warp(src, Ht, out_hw=(50, 70))
+        want = warp64(src.double(), Ht.double(), out_hw=(50, 70))
+        assert float((got.double() - want).abs().max()) < 2e-5, H
+
+
+def test_l2_gradient_vs_fd64_and_autograd64(pairs):
+    src, tgt, p, w, h = pairs
+    al = align.Aligner(src, tgt)
+    loss, g = al.loss_and_grad(p)
+    # float64 restatement on the same pyramid levels (built by the F32 kernel)
+    S = [x.double() for x in al.ps]
+    T = [x.double() for x in al.pt]
+    s = w / 2.0   # parameters in pixel units: one unit moves the border by about one pixel
+    q = (p * s).detach().requires_grad_(True)
+    L = loss64(S, T, align.H_from_p(q / s, w, h))
+    assert float((L - loss).abs().max() / L.abs().max()) < 1e-5
+    (g_auto,) = torch.autograd.grad(L.sum(), q)
+    g_kernel = g / s
+    eps = 1e-3
+    g_fd = torch.zeros_like(g_kernel)
+    with torch.no_grad():
+        for k in range(8):
+            e = torch.zeros_like(q)
+            e[:, k] = eps
+            g_fd[:, k] = (loss64(S, T, align.H_from_p((q + e) / s, w, h)) - loss64(S, T, align.H_from_p((q - e) / s, w, h))) / (2 * eps)
+    for i in range(g.shape[0]):
+        _check_grad(g_kernel[i], g_auto[i], f"pair {i} kernel vs float64 autograd")
+        _check_grad(g_kernel[i], g_fd[i], f"pair {i} kernel vs FP64 central differences")
+
+
+def test_adjoint_source_gradient_vs_autograd64(pairs):
+    src, tgt, p, w, h = pairs
+    H = align.H_from_p(p, w, h).reshape(-1, 9).float().contiguous()
+    out = align.warp(src, H)
+    gout = (2.0 * (out - tgt)).contiguous()
+    gsrc, gH = align.warp_adjoint(gout, src, H)
+    s64 = src.double().requires_grad_(True)
+    H64 = H.double().requires_grad_(True)
+    o64 = warp64(s64, H64)
+    gs64, gH64 = torch.autograd.grad((o64 * gout.double()).sum(), (s64, H64))
+    assert float((gsrc.double() - gs64).abs().max() / gs64.abs().max()) < 1e-4
+    for i in range(2):
+        _check_grad(gH[i, :8], gH64[i, :8], f"pair {i} dH through the generic adjoint")
+
+
+def test_autograd_function_matches_fused_l2(pairs):
+    src, tgt, p, w, h = pairs
+    H = align.H_from_p(p, w, h).reshape(-1, 9).float().contiguous().requires_grad_(True)
+    s = src.clone().requires_grad_(True)
+    out = WarpF32.apply(s, H)
+    loss = ((out - tgt) ** 2).mean(dim=(1, 2)).sum()
+    loss.backward()
+    l2 = align.L2Grad(src, tgt)
+    lv = torch.zeros(2, dtype=F64, device="cuda")
+    gH = torch.zeros((2, 9), dtype=F64, device="cuda")
+    gsrc = torch.zeros_like(src)
+    l2(H.detach(), 1.0 / (w * h), lv, gH, False, gsrc)
+    assert abs(float(lv.sum()) - float(loss)) / float(loss) < 1e-5
+    _check_grad(H.grad.double()[:, :8].reshape(-1), gH[:, :8].reshape(-1), "autograd.Function vs fused L2", tol=1e-3)
+    assert float((s.grad - gsrc).abs().max() / gsrc.abs().max()) < 1e-4
+
+
+def test_pyramid_forward_and_adjoint_vs_float64(gpu):
+    torch.manual_seed(0)
+    x = torch.rand((2, 37, 50), device="cuda", dtype=torch.float32)
+    y = align.pyr_half(x)
+    x64 = x.double().requires_grad_(True)
+    y64 = pyr64(x64)
+    assert float((y.double() - y64).abs().max()) < 1e-6
+    g = torch.rand_like(y)
+    (gx64,) = torch.autograd.grad((y64 * g.double()).sum(), x64)
+    gx = align.pyr_half_adjoint(g.contiguous(), (37, 50))
+    assert float((gx.double() - gx64).abs().max()) < 1e-6
+
+
+def test_warp_of_recomputed_pyramid_level(pairs):
+    """gradients through WarpPyramidLevelF32 (levels recomputed in backward) against float64 autograd
+    through the float64 pyramid"""
+    src, tgt, p, w, h = pairs
+    level = 2
+    Hl = align.level_H(align.H_from_p(p, w, h), level)
+    s = src.clone().requires_grad_(True)
+    H = Hl.clone().requires_grad_(True)
+    t = align.pyramid(tgt, level + 1)[level]
+    out = WarpPyramidLevelF32.apply(s, H, level)
+    ((out - t) ** 2).sum().backward()
+    s64 = src.double().requires_grad_(True)
+    H64 = Hl.double().requires_grad_(True)
+    lv = s64
+    for _ in range(level):
+        lv = pyr64(lv)
+    r = warp64(lv, H64) - t.double()
+    gs64, gH64 = torch.autograd.grad((r * r).sum(), (s64, H64))
+    assert float((s.grad.double() - gs64).abs().max() / gs64.abs().max()) < 1e-4
+    for i in range(2):
+        _check_grad(H.grad.double()[i, :8], gH64[i, :8], f"pair {i} dH at pyramid level {level}", tol=1e-3)
+
+
+def test_remap_adjoint_vs_float64_and_fd(gpu):
+    torch.manual_seed(1)
+    n, h, w = 2, 48, 64
+    src = align.gaussian_blur(torch.rand((n, h, w), device="cuda"), 1.5)
+    ys, xs = torch.meshgrid(torch.arange(h, device="cuda", dtype=torch.float32), torch.arange(w, device="cuda", dtype=torch.float32),
+                            indexing="ij")
+    m = torch.stack([xs + 3.3 * torch.sin(ys / 7.0) - 1.7, ys + 2.1 * torch.cos(xs / 9.0) + 0.6], dim=-1)
+    m = m.unsqueeze(0).repeat(n, 1, 1, 1).contiguous()
+    m[1] += 0.37
+    tgt = torch.rand((n, h, w), device="cuda")
+    out = align.remap(src, m)
+    assert float((out.double() - remap64(src.double(), m.double())).abs().max()) < 2e-5
+    s = src.clone().requires_grad_(True)
+    mm = m.clone().requires_grad_(True)
+    ((RemapF32.apply(s, mm) - tgt) ** 2).sum().backward()
+    s64, m64 = src.double().requires_grad_(True), m.double().requires_grad_(True)
+    r = remap64(s64, m64) - tgt.double()
+    gs64, gm64 = torch.autograd.grad((r * r).sum(), (s64, m64))
+    assert float((s.grad.double() - gs64).abs().max() / gs64.abs().max()) < 1e-4
+    assert float((mm.grad.double() - gm64).abs().max() / gm64.abs().max()) < 1e-4
+    # central differences in FP64 on single map entries
+    with torch.no_grad():
+        for (i, y, x, k) in [(0, 10, 20, 0), (1, 30, 5, 1), (0, 47, 63, 0)]:
+            e = torch.zeros_like(m64)
+            e[i, y, x, k] = 1e-3
+            lp = ((remap64(src.double(), m64 + e) - tgt.double()) ** 2).sum()
+            lm = ((remap64(src.double(), m64 - e) - tgt.double()) ** 2).sum()
+            fd = float((lp - lm) / 2e-3)
+            got = float(mm.grad[i, y, x, k])
+            assert abs(got - fd) <= 1e-4 * max(abs(fd), 1e-3), (i, y, x, k, got, fd)
+
+
+def test_alignment_recovers_homographies(gpu):
+    """Adam, 200 steps on the 3-level pyramid loss, 4 pairs of the 1024x1024 workload"""
+    src, tgt, Ht = align.make_pairs(4, 1024, 1024, seed0=5000)
+    al = align.Aligner(src, tgt)
+    p = al.run(steps=200)
+    err = align.corner_error(align.H_from_p(p, 1024, 1024), Ht, 1024, 1024)
+    print("corner error (px):", [round(float(e), 3) for e in err], flush=True)
+    assert float(err.max()) < 1.0
