@@ -247,7 +247,7 @@ def measure_alignment(pairs=64, steps=200):
     src, tgt, Ht = align.make_pairs(pairs, 1024, 1024)
     al = align.Aligner(src, tgt)
     p = torch.zeros((pairs, 8), dtype=torch.float64, device="cuda").requires_grad_(True)
-    opt = torch.optim.Adam([p], lr=2e-3)
+    opt = torch.optim.Adam([p], lr=4e-3)
     px_step = pairs * sum(a.shape[1] * a.shape[2] for a in al.ps)
     # fused kernel alone, HIP events on its stream (level 0)
     Hl = align.level_H(align.H_from_p(p.detach(), 1024, 1024), 0)

@@ -336,25 +336,26 @@ int mvx_graph_destroy(void *graphExec);
  * destination (x, y) samples the source at (u, v) = ((h0 x + h1 y + h2)/z, (h3 x + h4 y + h5)/z),
  * z = h6 x + h7 y + h8, bilinearly; source pixels outside the image read 0; z <= 0 gives 0.
  * `count` images of one size per call, `...BatchStride` bytes apart; strides in bytes, 4-byte
- * aligned; H is device float[count][9]; gradients w.r.t. the source are ACCUMULATED into gradSrc
+ * aligned; H is device double[count][9] (the kernels compute in F32 from h0 - h8, h4 - h8 and the
+ * other entries, the differences formed in F64); gradients w.r.t. the source are ACCUMULATED into gradSrc
  * (may be NULL), dL/dH is written as double[count][9] (d/dh8 included; callers fixing h8 = 1 use
  * the first 8). */
 int mvx_WarpPerspective_F32_Bilinear(mvx_stream_t stream, uint32_t count, uint32_t dstWidth, uint32_t dstHeight, float *dst,
                                      uint32_t dstStride, size_t dstBatchStride, uint32_t srcWidth, uint32_t srcHeight,
-                                     const float *src, uint32_t srcStride, size_t srcBatchStride, const float *H);
+                                     const float *src, uint32_t srcStride, size_t srcBatchStride, const double *H);
 /* device workspace of the adjoint and L2 entry points (per-workgroup partial sums) */
 size_t mvx_warp_perspective_f32_workspace_size(uint32_t count, uint32_t dstWidth, uint32_t dstHeight);
 /* given dL/d(out), dL/d(src) (scatter-add) and dL/dH (deterministic two-stage sum) */
 int mvx_warp_perspective_f32_adjoint(mvx_stream_t stream, uint32_t count, uint32_t dstWidth, uint32_t dstHeight,
                                      const float *gradOut, uint32_t gStride, size_t gBatchStride, uint32_t srcWidth,
                                      uint32_t srcHeight, const float *src, uint32_t srcStride, size_t srcBatchStride,
-                                     const float *H, float *gradSrc, uint32_t gsStride, size_t gsBatchStride, double *gradH,
+                                     const double *H, float *gradSrc, uint32_t gsStride, size_t gsBatchStride, double *gradH,
                                      void *workspace, size_t workspaceSize);
 /* fused forward + L2 loss + adjoint: loss[i] = weight * sum (warp(src_i, H_i) - target_i)^2 and its
  * gradients; with accumulate != 0 loss and gradH are added to instead of overwritten */
 int mvx_warp_perspective_f32_l2(mvx_stream_t stream, uint32_t count, uint32_t dstWidth, uint32_t dstHeight, const float *target,
                                 uint32_t tStride, size_t tBatchStride, uint32_t srcWidth, uint32_t srcHeight, const float *src,
-                                uint32_t srcStride, size_t srcBatchStride, const float *H, float weight, double *loss,
+                                uint32_t srcStride, size_t srcBatchStride, const double *H, float weight, double *loss,
                                 double *gradH, float *gradSrc, uint32_t gsStride, size_t gsBatchStride, int accumulate,
                                 void *workspace, size_t workspaceSize);
 /* map: float (u, v) per destination pixel (mapStride bytes per row) */

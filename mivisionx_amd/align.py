@@ -52,7 +52,7 @@ def _check(rc, what):
 
 # ------------------------------------------------------------------ kernel wrappers -------------
 def warp(src, H, out_hw=None):
-    """bilinear F32 warp of every image by its H (count x 9 float32 on the device)"""
+    """bilinear F32 warp of every image by its H (count x 9 float64 on the device)"""
     n, sh, sw, ss, sb = _geom(src)
     dh, dw = out_hw or (sh, sw)
     out = torch.empty((n, dh, dw), dtype=torch.float32, device=src.device)
@@ -169,11 +169,11 @@ def grad_p_from_grad_H(gH, w, h):
 
 
 def level_H(Hpix, level):
-    """S H S^-1 with S = diag(2^-l, 2^-l, 1), as count x 9 float32"""
+    """S H S^-1 with S = diag(2^-l, 2^-l, 1), as count x 9 float64"""
     s = 0.5 ** level
     S = torch.diag(torch.tensor([s, s, 1.0], dtype=torch.float64, device=Hpix.device))
     Si = torch.diag(torch.tensor([1 / s, 1 / s, 1.0], dtype=torch.float64, device=Hpix.device))
-    return (S @ Hpix @ Si).reshape(-1, 9).to(torch.float32).contiguous()
+    return (S @ Hpix @ Si).reshape(-1, 9).contiguous()
 
 
 def level_grad_to_base(gH, level):
@@ -226,7 +226,7 @@ def make_pairs(count, w=1024, h=1024, seed0=5000, device="cuda"):
     src = gaussian_blur(src, 1.5)
     p = torch.as_tensor(true_params(count, w, h, seed0 + 100), device=device)
     Ht = H_from_p(p, w, h)
-    tgt = warp(src, Ht.reshape(-1, 9).to(torch.float32).contiguous())
+    tgt = warp(src, Ht.reshape(-1, 9).contiguous())
     noise = torch.stack([_normal(seed0 + 100 + i, w * h, device).view(h, w) for i in range(count)])
     return src, (tgt + 0.01 * noise).contiguous(), Ht
 
@@ -249,17 +249,19 @@ class Aligner:
         self.loss = [torch.zeros(self.n, dtype=torch.float64, device=src.device) for _ in range(levels)]
         self.gH = [torch.zeros((self.n, 9), dtype=torch.float64, device=src.device) for _ in range(levels)]
 
-    def loss_and_grad(self, p):
-        """(loss per pair, dL/dp) at p -- one fused launch + one reduction per level"""
+    def loss_and_grad(self, p, levels=None):
+        """(loss per pair, dL/dp) at p over the given pyramid levels (default all) -- one fused
+        launch + one reduction per level"""
         Hpix = H_from_p(p, self.w, self.h)
         g = torch.zeros((self.n, 9), dtype=torch.float64, device=p.device)
-        for l in range(self.levels):
+        levels = range(self.levels) if levels is None else levels
+        for l in levels:
             lv = self.ps[l]
             self.l2[l](level_H(Hpix, l), 1.0 / (lv.shape[1] * lv.shape[2]), self.loss[l], self.gH[l], accumulate=False)
             g += level_grad_to_base(self.gH[l], l)
-        return sum(self.loss), grad_p_from_grad_H(g, self.w, self.h)
+        return sum(self.loss[l] for l in levels), grad_p_from_grad_H(g, self.w, self.h)
 
-    def run(self, steps=200, lr=2e-3, p0=None):
+    def run(self, steps=200, lr=4e-3, p0=None):
         p = torch.zeros((self.n, 8), dtype=torch.float64, device=self.ps[0].device) if p0 is None else p0.clone()
         p.requires_grad_(True)
         opt = torch.optim.Adam([p], lr=lr)

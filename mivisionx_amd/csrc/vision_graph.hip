@@ -156,15 +156,28 @@ MVX_DEV void gunion(uint32_t *par, uint32_t a, uint32_t b)
     }
 }
 
+// Arrays whose lifetimes do not overlap share LDS (every hand-over is separated by a barrier):
+// the gradients (steps 2-3) with the Canny states and row masks (steps 4-6), the Canny magnitude
+// (steps 3-4) with the border flags (step 6), the Harris score (steps 3-4 and the candidate append)
+// with the union-find labels (step 6).  37.8 KB -> 24.7 KB per workgroup: 6 instead of 4 resident
+// workgroups per CU.
 struct FrontShared {
     uint8_t Y[YH][TP];
-    int16_t Gx[GH][TP], Gy[GH][TP];
-    uint16_t M[MH][TP];
-    float V[MH][TP];
-    uint8_t S[TS];            // Canny states 0 / 127 / 255 of the tile's pixels
-    int L[TS];                // union-find labels of the weak pixels no strong pixel reaches in the tile
-    uint8_t FB[TS];           // root flag: the component touches the tile border
-    uint64_t Em[TH], Sm[TH], Rm[TH];   // per row: edge, strong, reached-from-strong pixels (bit = column)
+    union {
+        struct { int16_t Gx[GH][TP], Gy[GH][TP]; };
+        struct {
+            uint8_t S[TS];                     // Canny states 0 / 127 / 255 of the tile's pixels
+            uint64_t Em[TH], Sm[TH], Rm[TH];   // per row: edge, strong, reached-from-strong pixels (bit = column)
+        };
+    };
+    union {
+        uint16_t M[MH][TP];
+        uint8_t FB[TS];                        // root flag: the component touches the tile border
+    };
+    union {
+        float V[MH][TP];
+        int L[TS];                             // union-find labels of the weak pixels no strong pixel reaches
+    };
     uint32_t cnt, base, unres;
 };
 
@@ -304,7 +317,6 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
                    c > s.V[j][i + 1] && c > s.V[j + 1][i - 1] && c > s.V[j + 1][i] && c > s.V[j + 1][i + 1];
         }
         s.S[p] = st;
-        s.FB[p] = 0;
         if (cand) myCand |= 1u << q;
         // row masks of edge (127/255) and strong (255) pixels: bit = column
         const uint64_t em = __ballot(st != 0), sm = __ballot(st == 255);
@@ -407,6 +419,7 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
         const int p = r * TP + lane;
         const bool wk = s.S[p] && !((s.Rm[r] >> lane) & 1ull);
         s.L[p] = wk ? p : -1;
+        s.FB[p] = 0;
     }
     __syncthreads();
     for (int q = 0; q < TH / NW; q++) {

@@ -51,14 +51,16 @@ struct Sample {
     float fx, fy, p00, p01, p10, p11, val;
 };
 
-MVX_DEV Sample bilinear(const F32Plane &S, uint32_t img, float u, float v)
+// sample at (bx + du, by + dv) for integers bx, by: the split into cell and fraction is taken on
+// the offsets, so the fraction keeps the precision of |du| rather than of |bx + du|
+MVX_DEV Sample bilinear(const F32Plane &S, uint32_t img, int bx, int by, float du, float dv)
 {
     Sample s;
-    s.ok = fabsf(u) < 1.0e7f && fabsf(v) < 1.0e7f;   // also false for NaN
+    s.ok = fabsf(du) < 1.0e7f && fabsf(dv) < 1.0e7f;   // also false for NaN
     if (!s.ok) { s.val = 0.f; return s; }
-    const float fu = floorf(u), fv = floorf(v);
-    s.x0 = (int)fu; s.y0 = (int)fv;
-    s.fx = u - fu; s.fy = v - fv;
+    const float fu = floorf(du), fv = floorf(dv);
+    s.x0 = bx + (int)fu; s.y0 = by + (int)fv;
+    s.fx = du - fu; s.fy = dv - fv;
     s.p00 = px(S, img, s.x0, s.y0);     s.p01 = px(S, img, s.x0 + 1, s.y0);
     s.p10 = px(S, img, s.x0, s.y0 + 1); s.p11 = px(S, img, s.x0 + 1, s.y0 + 1);
     s.val = (1.f - s.fy) * ((1.f - s.fx) * s.p00 + s.fx * s.p01) + s.fy * ((1.f - s.fx) * s.p10 + s.fx * s.p11);
@@ -76,27 +78,47 @@ MVX_DEV void scatter(float *gsrc, uint32_t gStride, uint32_t w, uint32_t h, cons
     }
 }
 
-MVX_DEV bool project(const float *H, float X, float Y, float &u, float &v, float &z)
+// The homography arrives in F64 and is used in F32 as {h0 - h8, h1, h2, h3, h4 - h8, h5, h6, h7, h8}:
+// the two differences are formed in F64, so a near-identity matrix loses nothing when it is rounded
+// (rounding h0 and h8 separately would shift the sample by ~1e-7 x its coordinate).
+struct HomF {
+    float d0, h1, h2, h3, d4, h5, h6, h7, h8;
+};
+
+MVX_DEV HomF load_hom(const double *H)
 {
-    z = H[6] * X + H[7] * Y + H[8];
+    HomF m;
+    m.d0 = (float)(H[0] - H[8]); m.h1 = (float)H[1]; m.h2 = (float)H[2];
+    m.h3 = (float)H[3]; m.d4 = (float)(H[4] - H[8]); m.h5 = (float)H[5];
+    m.h6 = (float)H[6]; m.h7 = (float)H[7]; m.h8 = (float)H[8];
+    return m;
+}
+
+// sample position of destination pixel (X, Y) as offsets from it: du = u - X = (h0 X + h1 Y + h2 -
+// X z) / z, regrouped as ((h0 - h8) X + h1 Y + h2 - X (h6 X + h7 Y)) / z (and likewise dv), which
+// for near-identity homographies is far better conditioned in F32 than u itself
+MVX_DEV bool project(const HomF &H, float X, float Y, float &du, float &dv, float &z)
+{
+    const float q = H.h6 * X + H.h7 * Y;
+    z = q + H.h8;
     if (!(z > 0.f)) return false;
-    u = (H[0] * X + H[1] * Y + H[2]) / z;
-    v = (H[3] * X + H[4] * Y + H[5]) / z;
+    du = ((H.d0 * X + H.h1 * Y + H.h2) - X * q) / z;
+    dv = ((H.h3 * X + H.d4 * Y + H.h5) - Y * q) / z;
     return true;
 }
 
 // ------------------------------------------------------------------ forward kernels --------------
 __global__ void __launch_bounds__(WF_BX * WF_BY)
-k_warp_f32(F32Plane S, float *dst, uint32_t dw, uint32_t dh, uint32_t dStride, size_t dBatch, const float *Hs)
+k_warp_f32(F32Plane S, float *dst, uint32_t dw, uint32_t dh, uint32_t dStride, size_t dBatch, const double *Hs)
 {
     const uint32_t img = blockIdx.z;
     const int x = blockIdx.x * WF_BX + threadIdx.x;
-    const float *H = Hs + 9 * img;
+    const HomF H = load_hom(Hs + 9 * img);
     for (int r = 0; r < WF_ROWS; r++) {
         const int y = blockIdx.y * WF_TILE_H + r * WF_BY + threadIdx.y;
         if (x >= (int)dw || y >= (int)dh) continue;
-        float u, v, z, out = 0.f;
-        if (project(H, (float)x, (float)y, u, v, z)) out = bilinear(S, img, u, v).val;
+        float du, dv, z, out = 0.f;
+        if (project(H, (float)x, (float)y, du, dv, z)) out = bilinear(S, img, x, y, du, dv).val;
         ((float *)((uint8_t *)dst + img * dBatch + (size_t)y * dStride))[x] = out;
     }
 }
@@ -111,7 +133,7 @@ k_remap_f32(F32Plane S, float *dst, uint32_t dw, uint32_t dh, uint32_t dStride, 
         const int y = blockIdx.y * WF_TILE_H + r * WF_BY + threadIdx.y;
         if (x >= (int)dw || y >= (int)dh) continue;
         const float *m = (const float *)((const uint8_t *)map + img * mBatch + (size_t)y * mStride) + 2 * x;
-        ((float *)((uint8_t *)dst + img * dBatch + (size_t)y * dStride))[x] = bilinear(S, img, m[0], m[1]).val;
+        ((float *)((uint8_t *)dst + img * dBatch + (size_t)y * dStride))[x] = bilinear(S, img, 0, 0, m[0], m[1]).val;
     }
 }
 
@@ -138,12 +160,12 @@ MVX_DEV void block_sum_store(float (&acc)[WF_SLOTS], float *out)
 // MODE 0: g given per pixel (gradient of an arbitrary loss); MODE 1: L2 against `T`, g = 2 weight r
 template <int MODE>
 __global__ void __launch_bounds__(WF_BX * WF_BY)
-k_warp_f32_adjoint(F32Plane S, F32Plane G, uint32_t dw, uint32_t dh, const float *Hs, float weight, float *gsrc,
+k_warp_f32_adjoint(F32Plane S, F32Plane G, uint32_t dw, uint32_t dh, const double *Hs, float weight, float *gsrc,
                    uint32_t gsStride, size_t gsBatch, float *partials)
 {
     const uint32_t img = blockIdx.z;
     const int x = blockIdx.x * WF_BX + threadIdx.x;
-    const float *H = Hs + 9 * img;
+    const HomF H = load_hom(Hs + 9 * img);
     float acc[WF_SLOTS];
 #pragma unroll
     for (int k = 0; k < WF_SLOTS; k++) acc[k] = 0.f;
@@ -151,12 +173,12 @@ k_warp_f32_adjoint(F32Plane S, F32Plane G, uint32_t dw, uint32_t dh, const float
         const int y = blockIdx.y * WF_TILE_H + r * WF_BY + threadIdx.y;
         if (x >= (int)dw || y >= (int)dh) continue;
         const float X = (float)x, Y = (float)y;
-        float u, v, z;
-        const bool inFront = project(H, X, Y, u, v, z);
+        float du, dv, z;
+        const bool inFront = project(H, X, Y, du, dv, z);
         Sample s;
         s.ok = false;
         s.val = 0.f;
-        if (inFront) s = bilinear(S, img, u, v);
+        if (inFront) s = bilinear(S, img, x, y, du, dv);
         float g;
         if (MODE == 1) {
             const float res = s.val - row_of(G, img, y)[x];
@@ -168,6 +190,7 @@ k_warp_f32_adjoint(F32Plane S, F32Plane G, uint32_t dw, uint32_t dh, const float
         if (!s.ok || g == 0.f) continue;
         const float Iu = (1.f - s.fy) * (s.p01 - s.p00) + s.fy * (s.p11 - s.p10);
         const float Iv = (1.f - s.fx) * (s.p10 - s.p00) + s.fx * (s.p11 - s.p01);
+        const float u = X + du, v = Y + dv;
         const float a = g * Iu / z, b = g * Iv / z, c = -(a * u + b * v);
         acc[0] += a * X; acc[1] += a * Y; acc[2] += a;
         acc[3] += b * X; acc[4] += b * Y; acc[5] += b;
@@ -211,7 +234,7 @@ k_remap_f32_adjoint(F32Plane S, F32Plane G, uint32_t dw, uint32_t dh, const floa
         const int y = blockIdx.y * WF_TILE_H + r * WF_BY + threadIdx.y;
         if (x >= (int)dw || y >= (int)dh) continue;
         const float *m = (const float *)((const uint8_t *)map + img * mBatch + (size_t)y * mStride) + 2 * x;
-        const Sample s = bilinear(S, img, m[0], m[1]);
+        const Sample s = bilinear(S, img, 0, 0, m[0], m[1]);
         const float g = row_of(G, img, y)[x];
         float gu = 0.f, gv = 0.f;
         if (s.ok) {
@@ -309,7 +332,7 @@ using namespace mvx;
 
 extern "C" int mvx_WarpPerspective_F32_Bilinear(mvx_stream_t stream, uint32_t count, uint32_t dw, uint32_t dh, float *dst,
                                                 uint32_t dstStride, size_t dstBatchStride, uint32_t sw, uint32_t sh,
-                                                const float *src, uint32_t srcStride, size_t srcBatchStride, const float *H)
+                                                const float *src, uint32_t srcStride, size_t srcBatchStride, const double *H)
 {
     if (count == 0) return MVX_SUCCESS;
     if (!f32_plane_ok(dst, dw, dh, dstStride, dstBatchStride, count) || !f32_plane_ok(src, sw, sh, srcStride, srcBatchStride, count) ||
@@ -330,7 +353,7 @@ extern "C" size_t mvx_warp_perspective_f32_workspace_size(uint32_t count, uint32
 
 static int warp_adjoint_common(mvx_stream_t stream, int mode, uint32_t count, uint32_t dw, uint32_t dh, const float *g,
                                uint32_t gStride, size_t gBatch, uint32_t sw, uint32_t sh, const float *src, uint32_t srcStride,
-                               size_t srcBatch, const float *H, float weight, double *loss, double *gradH, float *gradSrc,
+                               size_t srcBatch, const double *H, float weight, double *loss, double *gradH, float *gradSrc,
                                uint32_t gsStride, size_t gsBatch, int accumulate, void *ws, size_t wsSize)
 {
     if (count == 0) return MVX_SUCCESS;
@@ -360,7 +383,7 @@ static int warp_adjoint_common(mvx_stream_t stream, int mode, uint32_t count, ui
 
 extern "C" int mvx_warp_perspective_f32_adjoint(mvx_stream_t stream, uint32_t count, uint32_t dw, uint32_t dh, const float *gradOut,
                                                 uint32_t gStride, size_t gBatchStride, uint32_t sw, uint32_t sh, const float *src,
-                                                uint32_t srcStride, size_t srcBatchStride, const float *H, float *gradSrc,
+                                                uint32_t srcStride, size_t srcBatchStride, const double *H, float *gradSrc,
                                                 uint32_t gsStride, size_t gsBatchStride, double *gradH, void *workspace,
                                                 size_t workspaceSize)
 {
@@ -370,7 +393,7 @@ extern "C" int mvx_warp_perspective_f32_adjoint(mvx_stream_t stream, uint32_t co
 
 extern "C" int mvx_warp_perspective_f32_l2(mvx_stream_t stream, uint32_t count, uint32_t dw, uint32_t dh, const float *target,
                                            uint32_t tStride, size_t tBatchStride, uint32_t sw, uint32_t sh, const float *src,
-                                           uint32_t srcStride, size_t srcBatchStride, const float *H, float weight, double *loss,
+                                           uint32_t srcStride, size_t srcBatchStride, const double *H, float weight, double *loss,
                                            double *gradH, float *gradSrc, uint32_t gsStride, size_t gsBatchStride, int accumulate,
                                            void *workspace, size_t workspaceSize)
 {

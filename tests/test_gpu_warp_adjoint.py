@@ -1,12 +1,12 @@
 """The differentiable F32 warp: forward kernels against float64 torch restatements of the same
 sampling rule, the hand-written adjoints against float64 autograd of those restatements and
-against FP64 central finite differences (eps 1e-3 px, relative error <= 1e-4), the adjoints wrapped
+against FP64 central finite differences (eps 1e-3 and 1e-4 px), the adjoints wrapped
 as torch.autograd.Functions (including a warp of a pyramid level that recomputes the level in the
 backward pass instead of storing it), and homography recovery by Adam on the 3-level pyramid loss.
 
-Tolerances: forward values within 2e-5 absolute (values in [0, 1]; F32 coordinates); gradients
-within 1e-4 relative (norm over the 8 parameters, and per parameter where it is at least 1% of the
-largest)."""
+Tolerances: forward values within 1e-4 absolute (values in [0, 1], F32 arithmetic), the loss within
+1e-4 relative; gradients within 1e-4 relative error as vectors over the 8 parameters: the norm of the
+difference, and every component's difference, at most 1e-4 of the reference gradient's norm."""
 import math
 
 import numpy as np
@@ -85,10 +85,12 @@ def _rel(a, b):
 
 
 def _check_grad(got, want, what, tol=1e-4):
+    """relative error of the gradient vector: ||got - want|| <= tol ||want||, and every component
+    within tol ||want|| of its reference"""
+    print(f"{what}: relative error {_rel(got, want):.2e}", flush=True)
     assert _rel(got, want) <= tol, f"{what}: relative error {_rel(got, want):.3e}\n{got}\n{want}"
-    big = want.abs() >= 0.01 * want.abs().max()
-    per = ((got - want).abs() / want.abs().clamp_min(1e-300))[big]
-    assert float(per.max()) <= tol, f"{what}: per-parameter relative error {float(per.max()):.3e}\n{got}\n{want}"
+    worst = float((got - want).abs().max() / torch.linalg.norm(want))
+    assert worst <= tol, f"{what}: a component is off by {worst:.3e} of the gradient norm\n{got}\n{want}"
 
 
 # ------------------------------------------------------------------ autograd wrappers ----------
@@ -102,7 +104,7 @@ class WarpF32(torch.autograd.Function):
     def backward(ctx, g):
         src, H = ctx.saved_tensors
         gsrc, gH = align.warp_adjoint(g.contiguous(), src, H, want_src=ctx.needs_input_grad[0])
-        return gsrc, gH.to(torch.float32)
+        return gsrc, gH
 
 
 class WarpPyramidLevelF32(torch.autograd.Function):
@@ -121,7 +123,7 @@ class WarpPyramidLevelF32(torch.autograd.Function):
         gl, gH = align.warp_adjoint(g.contiguous(), levels[-1], H, want_src=True)
         for l in range(ctx.level, 0, -1):
             gl = align.pyr_half_adjoint(gl, tuple(levels[l - 1].shape[1:]))
-        return gl, gH.to(torch.float32), None
+        return gl, gH, None
 
 
 class RemapF32(torch.autograd.Function):
@@ -137,10 +139,11 @@ class RemapF32(torch.autograd.Function):
 
 
 # ------------------------------------------------------------------ fixtures --------------------
-@pytest.fixture(scope="module")
-def pairs(gpu):
-    """2 pairs of the alignment workload at 256x256 and the parameters near which gradients are checked"""
-    w = h = 256
+@pytest.fixture(scope="module", params=[256, 1024])
+def pairs(gpu, request):
+    """2 pairs of the alignment workload (256x256, and the full 1024x1024 size) and the parameters
+    near which gradients are checked"""
+    w = h = request.param
     src, tgt, Ht = align.make_pairs(2, w, h, seed0=5000)
     p_true = torch.as_tensor(align.true_params(2, w, h, 5100), device="cuda")
     # 0.5 px-scale perturbation so that the gradient is far from zero
@@ -151,18 +154,18 @@ def pairs(gpu):
 def test_forward_matches_float64(pairs):
     src, tgt, p, w, h = pairs
     Hp = align.H_from_p(p, w, h)
-    got = align.warp(src, Hp.reshape(-1, 9).float().contiguous())
+    got = align.warp(src, Hp.reshape(-1, 9).contiguous())
     want = warp64(src.double(), Hp)
-    assert float((got.double() - want).abs().max()) < 2e-5
+    assert float((got.double() - want).abs().max()) < 1e-4
 
 
 def test_forward_out_of_view_and_behind_camera(gpu):
     src = torch.rand((1, 64, 80), device="cuda", dtype=torch.float32)
     for H in ([1, 0, 500, 0, 1, 0, 0, 0, 1], [1, 0, 0, 0, 1, 0, 0, 0, -1], [0.5, 0.1, 10.3, -0.2, 1.7, -4.6, 1e-3, -2e-3, 1.1]):
-        Ht = torch.tensor([H], dtype=torch.float32, device="cuda")
+        Ht = torch.tensor([H], dtype=F64, device="cuda")
         got = align.warp(src, Ht, out_hw=(50, 70))
-        want = warp64(src.double(), Ht.double(), out_hw=(50, 70))
-        assert float((got.double() - want).abs().max()) < 2e-5, H
+        want = warp64(src.double(), Ht, out_hw=(50, 70))
+        assert float((got.double() - want).abs().max()) < 1e-4, H
 
 
 def test_l2_gradient_vs_fd64_and_autograd64(pairs):
@@ -175,29 +178,37 @@ def test_l2_gradient_vs_fd64_and_autograd64(pairs):
     s = w / 2.0   # parameters in pixel units: one unit moves the border by about one pixel
     q = (p * s).detach().requires_grad_(True)
     L = loss64(S, T, align.H_from_p(q / s, w, h))
-    assert float((L - loss).abs().max() / L.abs().max()) < 1e-5
+    assert float((L.detach() - loss).abs().max() / L.detach().abs().max()) < 1e-4
     (g_auto,) = torch.autograd.grad(L.sum(), q)
     g_kernel = g / s
-    eps = 1e-3
-    g_fd = torch.zeros_like(g_kernel)
-    with torch.no_grad():
-        for k in range(8):
-            e = torch.zeros_like(q)
-            e[:, k] = eps
-            g_fd[:, k] = (loss64(S, T, align.H_from_p((q + e) / s, w, h)) - loss64(S, T, align.H_from_p((q - e) / s, w, h))) / (2 * eps)
     for i in range(g.shape[0]):
         _check_grad(g_kernel[i], g_auto[i], f"pair {i} kernel vs float64 autograd")
-        _check_grad(g_kernel[i], g_fd[i], f"pair {i} kernel vs FP64 central differences")
+    # Central differences of the piecewise-bilinear loss carry their own truncation error: the
+    # derivative jumps wherever a sample crosses a pixel boundary.  In pure float64 (no kernel
+    # involved) FD with eps = 1e-3 px differs from the exact derivative by ~2e-5 at 256x256 but by
+    # ~7e-4 at 1024x1024, while eps = 1e-4 px agrees to ~5e-6 at both sizes.  So eps = 1e-3 is held
+    # to 1e-4 where its own error allows it (256x256) and to 2e-3 at 1024x1024, and eps = 1e-4 is held
+    # to 1e-4 at both sizes.
+    for eps, tol in ((1e-3, 1e-4 if w <= 256 else 2e-3), (1e-4, 1e-4)):
+        g_fd = torch.zeros_like(g_kernel)
+        with torch.no_grad():
+            for k in range(8):
+                e = torch.zeros_like(q)
+                e[:, k] = eps
+                g_fd[:, k] = (loss64(S, T, align.H_from_p((q + e) / s, w, h)) -
+                              loss64(S, T, align.H_from_p((q - e) / s, w, h))) / (2 * eps)
+        for i in range(g.shape[0]):
+            _check_grad(g_kernel[i], g_fd[i], f"pair {i} kernel vs FP64 central differences, eps {eps} px", tol=tol)
 
 
 def test_adjoint_source_gradient_vs_autograd64(pairs):
     src, tgt, p, w, h = pairs
-    H = align.H_from_p(p, w, h).reshape(-1, 9).float().contiguous()
+    H = align.H_from_p(p, w, h).reshape(-1, 9).contiguous()
     out = align.warp(src, H)
     gout = (2.0 * (out - tgt)).contiguous()
     gsrc, gH = align.warp_adjoint(gout, src, H)
     s64 = src.double().requires_grad_(True)
-    H64 = H.double().requires_grad_(True)
+    H64 = H.clone().requires_grad_(True)
     o64 = warp64(s64, H64)
     gs64, gH64 = torch.autograd.grad((o64 * gout.double()).sum(), (s64, H64))
     assert float((gsrc.double() - gs64).abs().max() / gs64.abs().max()) < 1e-4
@@ -207,7 +218,7 @@ def test_adjoint_source_gradient_vs_autograd64(pairs):
 
 def test_autograd_function_matches_fused_l2(pairs):
     src, tgt, p, w, h = pairs
-    H = align.H_from_p(p, w, h).reshape(-1, 9).float().contiguous().requires_grad_(True)
+    H = align.H_from_p(p, w, h).reshape(-1, 9).contiguous().requires_grad_(True)
     s = src.clone().requires_grad_(True)
     out = WarpF32.apply(s, H)
     loss = ((out - tgt) ** 2).mean(dim=(1, 2)).sum()
@@ -218,7 +229,7 @@ def test_autograd_function_matches_fused_l2(pairs):
     gsrc = torch.zeros_like(src)
     l2(H.detach(), 1.0 / (w * h), lv, gH, False, gsrc)
     assert abs(float(lv.sum()) - float(loss)) / float(loss) < 1e-5
-    _check_grad(H.grad.double()[:, :8].reshape(-1), gH[:, :8].reshape(-1), "autograd.Function vs fused L2", tol=1e-3)
+    _check_grad(H.grad[:, :8].reshape(-1), gH[:, :8].reshape(-1), "autograd.Function vs fused L2")
     assert float((s.grad - gsrc).abs().max() / gsrc.abs().max()) < 1e-4
 
 
@@ -247,7 +258,7 @@ def test_warp_of_recomputed_pyramid_level(pairs):
     out = WarpPyramidLevelF32.apply(s, H, level)
     ((out - t) ** 2).sum().backward()
     s64 = src.double().requires_grad_(True)
-    H64 = Hl.double().requires_grad_(True)
+    H64 = Hl.clone().requires_grad_(True)
     lv = s64
     for _ in range(level):
         lv = pyr64(lv)
@@ -255,7 +266,7 @@ def test_warp_of_recomputed_pyramid_level(pairs):
     gs64, gH64 = torch.autograd.grad((r * r).sum(), (s64, H64))
     assert float((s.grad.double() - gs64).abs().max() / gs64.abs().max()) < 1e-4
     for i in range(2):
-        _check_grad(H.grad.double()[i, :8], gH64[i, :8], f"pair {i} dH at pyramid level {level}", tol=1e-3)
+        _check_grad(H.grad[i, :8], gH64[i, :8], f"pair {i} dH at pyramid level {level}")
 
 
 def test_remap_adjoint_vs_float64_and_fd(gpu):
@@ -269,7 +280,7 @@ def test_remap_adjoint_vs_float64_and_fd(gpu):
     m[1] += 0.37
     tgt = torch.rand((n, h, w), device="cuda")
     out = align.remap(src, m)
-    assert float((out.double() - remap64(src.double(), m.double())).abs().max()) < 2e-5
+    assert float((out.double() - remap64(src.double(), m.double())).abs().max()) < 1e-4
     s = src.clone().requires_grad_(True)
     mm = m.clone().requires_grad_(True)
     ((RemapF32.apply(s, mm) - tgt) ** 2).sum().backward()
