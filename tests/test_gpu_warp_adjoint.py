@@ -228,7 +228,7 @@ def test_autograd_function_matches_fused_l2(pairs):
     gH = torch.zeros((2, 9), dtype=F64, device="cuda")
     gsrc = torch.zeros_like(src)
     l2(H.detach(), 1.0 / (w * h), lv, gH, False, gsrc)
-    assert abs(float(lv.sum()) - float(loss)) / float(loss) < 1e-5
+    assert abs(float(lv.sum()) - loss.item()) / loss.item() < 1e-5
     _check_grad(H.grad[:, :8].reshape(-1), gH[:, :8].reshape(-1), "autograd.Function vs fused L2")
     assert float((s.grad - gsrc).abs().max() / gsrc.abs().max()) < 1e-4
 
