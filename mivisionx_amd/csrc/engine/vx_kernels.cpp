@@ -314,6 +314,25 @@ vx_status x_amd_half(_vx_node *n, hipStream_t st)
     return rc(n, mvx_ScaleGaussianHalf_U8_U8_5x5(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
 }
 
+// halfscale_gaussian -> SCALE_GAUSSIAN_HALF_U8_U8_5x5 for kernel size 5 (ago_drama_divide.cpp:1471-1488,
+// which picks ScaleImage nearest for 1 and ScaleGaussianHalf 3x3 for 3: those two are not built here)
+vx_status v_halfscale(_vx_node *n)
+{
+    vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
+    if (s) return s;
+    _vx_scalar *k = scl(n, 2);
+    if (!k || k->dataType != VX_TYPE_INT32) return fail(n, VX_ERROR_INVALID_PARAMETERS, "kernel_size is not an INT32 scalar");
+    if (k->value.i32 != 5) return fail(n, VX_ERROR_NOT_SUPPORTED, "kernel_size other than 5");
+    _vx_image *in = img(n, 0);
+    return set_output(n, 1, (in->width + 1) / 2, (in->height + 1) / 2, VX_DF_IMAGE_U8);
+}
+
+vx_status x_halfscale(_vx_node *n, hipStream_t st)
+{
+    _vx_image *in = img(n, 0), *o = img(n, 1);
+    return rc(n, mvx_ScaleGaussianHalf_U8_U8_5x5(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+}
+
 // ---------------------------------------------------------------------- Canny (A12) --------------
 // CANNY_SOBEL_U16_U8_3x3_L1NORM -> CANNY_SUPP_THRESHOLD_U8XY_U16_3x3 -> CANNY_EDGE_TRACE_U8_U8XY with a
 // virtual stack of w*h items (ago_drama_divide.cpp:1618-1701).  Workspace: the U16 gradient plane and
@@ -717,6 +736,9 @@ const std::vector<KernelDef> &kernel_table()
           {OUT, VX_TYPE_ARRAY, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false},
           {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}},
          v_lk, i_lk, x_lk, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_HALFSCALE_GAUSSIAN, "org.khronos.openvx.halfscale_gaussian",
+         {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}}, v_halfscale, nullptr, x_halfscale,
+         AGO_TARGET_AFFINITY_GPU},
         {VX_KERNEL_REMAP, "org.khronos.openvx.remap",
          {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
          v_remap, nullptr, x_remap, AGO_TARGET_AFFINITY_GPU},

@@ -130,6 +130,12 @@ VX_API_ENTRY vx_node VX_API_CALL vxRemapNode(vx_graph graph, vx_image input, vx_
     return make(graph, VX_KERNEL_REMAP, {input, table, p, output}, {p});
 }
 
+VX_API_ENTRY vx_node VX_API_CALL vxHalfScaleGaussianNode(vx_graph graph, vx_image input, vx_image output, vx_int32 kernel_size)
+{
+    _vx_scalar *k = sint(graph, kernel_size);
+    return make(graph, VX_KERNEL_HALFSCALE_GAUSSIAN, {input, output, k}, {k});
+}
+
 VX_API_ENTRY vx_node VX_API_CALL vxOpticalFlowPyrLKNode(vx_graph graph, vx_pyramid old_images, vx_pyramid new_images, vx_array old_points,
                                                        vx_array new_points_estimates, vx_array new_points, vx_enum termination,
                                                        vx_scalar epsilon, vx_scalar num_iterations, vx_scalar use_initial_estimate,
@@ -216,6 +222,11 @@ VX_API_ENTRY vx_status VX_API_CALL vxuWarpAffine(vx_context c, vx_image input, v
 VX_API_ENTRY vx_status VX_API_CALL vxuWarpPerspective(vx_context c, vx_image input, vx_matrix matrix, vx_enum type, vx_image output)
 {
     return immediate(c, [&](vx_graph g) { return vxWarpPerspectiveNode(g, input, matrix, type, output); });
+}
+
+VX_API_ENTRY vx_status VX_API_CALL vxuHalfScaleGaussian(vx_context c, vx_image input, vx_image output, vx_int32 kernel_size)
+{
+    return immediate(c, [&](vx_graph g) { return vxHalfScaleGaussianNode(g, input, output, kernel_size); });
 }
 
 VX_API_ENTRY vx_status VX_API_CALL vxuRemap(vx_context c, vx_image input, vx_remap table, vx_enum policy, vx_image output)

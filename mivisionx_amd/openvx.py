@@ -171,7 +171,7 @@ _PROTOS = {
     "vxCannyEdgeDetectorNode": (P, [P, P, P, c.c_int32, E, P]),
     "vxHarrisCornersNode": (P, [P, P, P, P, P, c.c_int32, c.c_int32, P, P]),
     "vxWarpAffineNode": (P, [P, P, P, E, P]), "vxWarpPerspectiveNode": (P, [P, P, P, E, P]),
-    "vxRemapNode": (P, [P, P, P, E, P]),
+    "vxRemapNode": (P, [P, P, P, E, P]), "vxHalfScaleGaussianNode": (P, [P, P, P, c.c_int32]),
     "vxOpticalFlowPyrLKNode": (P, [P, P, P, P, P, P, E, P, P, P, SZ]),
     "vxuGaussian3x3": (E, [P, P, P]), "vxuBox3x3": (E, [P, P, P]), "vxuMedian3x3": (E, [P, P, P]),
     "vxuSobel3x3": (E, [P, P, P, P]), "vxuMagnitude": (E, [P, P, P, P]), "vxuPhase": (E, [P, P, P, P]),
@@ -179,6 +179,7 @@ _PROTOS = {
     "vxuCannyEdgeDetector": (E, [P, P, P, c.c_int32, E, P]),
     "vxuHarrisCorners": (E, [P, P, P, P, P, c.c_int32, c.c_int32, P, P]),
     "vxuWarpAffine": (E, [P, P, P, E, P]), "vxuWarpPerspective": (E, [P, P, P, E, P]), "vxuRemap": (E, [P, P, P, E, P]),
+    "vxuHalfScaleGaussian": (E, [P, P, P, c.c_int32]),
     "vxuOpticalFlowPyrLK": (E, [P, P, P, P, P, P, E, P, P, P, SZ]),
 }
 

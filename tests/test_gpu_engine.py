@@ -114,6 +114,20 @@ def test_vxu_sobel_magnitude_phase_scale_pyramid(vx, ctx, orc):
         vx.release(lvl)
 
 
+@pytest.mark.parametrize("w,h", [(131, 67), (640, 480)])
+def test_vxu_halfscale_gaussian(vx, ctx, orc, w, h):
+    """halfscale_gaussian with kernel size 5 is ScaleGaussianHalf 5x5 (ago_drama_divide.cpp:1471-1488)"""
+    L = vx.lib()
+    img = synth.scene_u8(w, h, 13)
+    a = vx.create_image(ctx, w, h, "U8")
+    vx.write_image(a, img)
+    dw, dh = (w + 1) // 2, (h + 1) // 2
+    b = vx.create_image(ctx, dw, dh, "U8")
+    assert L.vxuHalfScaleGaussian(ctx, a, b, 5) == 0
+    assert_same(vx.read_image(b), orc.gaussian_half(img, dw, dh), f"halfscale_gaussian {w}x{h}")
+    assert L.vxuHalfScaleGaussian(ctx, a, b, 3) == vx.VX_ERROR_NOT_SUPPORTED
+
+
 def _graph2(vx, ctx, w, h, levels, lower, upper, k, thr, dist, capacity, fuse=True):
     L = vx.lib()
     g = L.vxCreateGraph(ctx)
