@@ -12,8 +12,17 @@
 //   -dump-profile             print per-graph and per-node timing
 //   -discard-compare-errors   keep running after a compare mismatch
 //   -v                        echo the GDF and the schedule
-// Object I/O: READ,<file> (raw planes / items, loaded before the first frame), WRITE,<file>
-// (after every frame), COMPARE,<file> (after every frame, exact bytes).
+// Object I/O (utilities/runvx/vxImage.cpp:560-640, 1185-1265):
+//   READ,<file>       raw planes / items; a file holding several frames is read frame after frame
+//                     (a single-frame file feeds every frame)
+//   WRITE,<file>      after every frame
+//   COMPARE,<file>[,rect{<sx>;<sy>;<ex>;<ey>}][,err{<min>;<max>}][,checksum|checksum-save-instead-of-test]
+//                     after every frame: raw frames read one after another from the file (a file of
+//                     exactly one frame serves every frame) (exact, or
+//                     pixel differences img - ref within [min, max] in the image's pixel type), or MD5
+//                     checksums of the region's rows (one 32-hex-digit line per frame; the -save- form
+//                     writes them instead).  A '%' in the file name makes it a per-frame printf pattern
+//                     of (frame, width, height).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -24,11 +33,23 @@
 #include <VX/vx.h>
 #include <vx_ext_amd.h>
 
+#include "md5.hpp"
+
 namespace {
+
+struct CompareSpec {
+    bool rect = false;
+    vx_rectangle_t region{0, 0, 0, 0};
+    float errMin = 0.f, errMax = 0.f;
+    bool checksum = false, checksumSave = false;
+};
 
 struct IoItem {
     vx_reference ref;
     std::string name, op, file;
+    CompareSpec cmp;
+    FILE *fp = nullptr;          // COMPARE / READ stream kept open across frames
+    std::vector<uint8_t> once;   // READ: a single-frame file
 };
 
 void VX_CALLBACK registry(void *obj, vx_reference ref, const char *name, const char *params)
@@ -36,8 +57,40 @@ void VX_CALLBACK registry(void *obj, vx_reference ref, const char *name, const c
     auto *list = (std::vector<IoItem> *)obj;
     std::string p = params;
     const size_t c = p.find(',');
-    IoItem it{ref, name, c == std::string::npos ? p : p.substr(0, c), c == std::string::npos ? "" : p.substr(c + 1)};
+    IoItem it;
+    it.ref = ref;
+    it.name = name;
+    it.op = c == std::string::npos ? p : p.substr(0, c);
+    it.file = c == std::string::npos ? "" : p.substr(c + 1);
     for (char &ch : it.op) ch = (char)toupper((unsigned char)ch);
+    if (it.op == "COMPARE") {
+        // options after the file name: ,rect{..} ,err{..} ,checksum ,checksum-save-instead-of-test
+        size_t k = it.file.find(',');
+        std::string opts = k == std::string::npos ? "" : it.file.substr(k);
+        if (k != std::string::npos) it.file.resize(k);
+        while (!opts.empty() && opts[0] == ',') {
+            size_t e = opts.find(',', 1);
+            std::string o = opts.substr(1, e == std::string::npos ? std::string::npos : e - 1);
+            opts = e == std::string::npos ? "" : opts.substr(e);
+            std::string lo = o;
+            for (char &ch : lo) ch = (char)tolower((unsigned char)ch);
+            if (!lo.compare(0, 4, "rect")) {
+                int v[4] = {0, 0, 0, 0};
+                if (sscanf(o.c_str() + 4, "{%d;%d;%d;%d}", &v[0], &v[1], &v[2], &v[3]) == 4) {
+                    it.cmp.rect = true;
+                    it.cmp.region = {(vx_uint32)v[0], (vx_uint32)v[1], (vx_uint32)v[2], (vx_uint32)v[3]};
+                }
+            } else if (!lo.compare(0, 3, "err")) {
+                sscanf(o.c_str() + 3, "{%f;%f}", &it.cmp.errMin, &it.cmp.errMax);
+            } else if (lo == "checksum") {
+                it.cmp.checksum = true;
+            } else if (lo == "checksum-save-instead-of-test") {
+                it.cmp.checksumSave = true;
+            } else {
+                fprintf(stderr, "runvx: invalid image compare option: %s\n", o.c_str());
+            }
+        }
+    }
     list->push_back(it);
 }
 
@@ -115,6 +168,191 @@ vx_status object_bytes(vx_reference ref, std::vector<uint8_t> &out, bool read)
     return VX_ERROR_NOT_SUPPORTED;
 }
 
+struct PlaneGeom {
+    uint32_t w, h, bpp;
+};
+
+// packed planes of an image (rows without padding) and each plane's geometry
+vx_status image_planes(vx_image img, std::vector<uint8_t> &data, std::vector<PlaneGeom> &geom)
+{
+    vx_size planes = 0;
+    vx_uint32 w = 0, h = 0;
+    vxQueryImage(img, VX_IMAGE_PLANES, &planes, sizeof(planes));
+    vxQueryImage(img, VX_IMAGE_WIDTH, &w, sizeof(w));
+    vxQueryImage(img, VX_IMAGE_HEIGHT, &h, sizeof(h));
+    data.clear();
+    geom.clear();
+    for (vx_uint32 p = 0; p < planes; p++) {
+        vx_rectangle_t rect{0, 0, w, h};
+        vx_map_id id;
+        vx_imagepatch_addressing_t addr;
+        void *ptr = nullptr;
+        vx_status s = vxMapImagePatch(img, &rect, p, &id, &addr, &ptr, VX_READ_ONLY, VX_MEMORY_TYPE_HOST, VX_NOGAP_X);
+        if (s) return s;
+        const size_t bytes = (size_t)addr.dim_x * addr.stride_x * addr.dim_y;
+        data.insert(data.end(), (uint8_t *)ptr, (uint8_t *)ptr + bytes);
+        geom.push_back({addr.dim_x, addr.dim_y, (uint32_t)addr.stride_x});
+        vxUnmapImagePatch(img, id);
+    }
+    return VX_SUCCESS;
+}
+
+std::string frame_file(const std::string &pattern, int frame, vx_reference ref)
+{
+    if (pattern.find('%') == std::string::npos) return pattern;
+    vx_uint32 w = 0, h = 0;
+    vx_enum type = 0;
+    vxQueryReference(ref, VX_REFERENCE_TYPE, &type, sizeof(type));
+    if (type == VX_TYPE_IMAGE) {
+        vxQueryImage((vx_image)ref, VX_IMAGE_WIDTH, &w, sizeof(w));
+        vxQueryImage((vx_image)ref, VX_IMAGE_HEIGHT, &h, sizeof(h));
+    }
+    char buf[4096];
+    snprintf(buf, sizeof(buf), pattern.c_str(), frame, w, h);
+    return buf;
+}
+
+// pixels of the region outside [errMin, errMax] (img - ref in the pixel type: S16 / U16 / S32 /
+// U32 / F32 planes compare as such, everything else byte by byte)
+size_t count_mismatches(vx_df_image fmt, const std::vector<uint8_t> &mine, const uint8_t *ref, const std::vector<PlaneGeom> &geom,
+                        uint32_t w, uint32_t h, const CompareSpec &c)
+{
+    size_t bad = 0, off = 0;
+    for (const PlaneGeom &g : geom) {
+        const uint32_t sx = c.rect ? c.region.start_x * g.w / w : 0, sy = c.rect ? c.region.start_y * g.h / h : 0;
+        const uint32_t ex = c.rect ? std::min(g.w, c.region.end_x * g.w / w) : g.w;
+        const uint32_t ey = c.rect ? std::min(g.h, c.region.end_y * g.h / h) : g.h;
+        for (uint32_t y = sy; y < ey; y++)
+            for (uint32_t x = sx; x < ex; x++) {
+                const size_t at = off + ((size_t)y * g.w + x) * g.bpp;
+                double d;
+                if (fmt == VX_DF_IMAGE_S16) d = (double)*(const int16_t *)&mine[at] - (double)*(const int16_t *)(ref + at);
+                else if (fmt == VX_DF_IMAGE_U16) d = (double)*(const uint16_t *)&mine[at] - (double)*(const uint16_t *)(ref + at);
+                else if (fmt == VX_DF_IMAGE_S32) d = (double)*(const int32_t *)&mine[at] - (double)*(const int32_t *)(ref + at);
+                else if (fmt == VX_DF_IMAGE_U32) d = (double)*(const uint32_t *)&mine[at] - (double)*(const uint32_t *)(ref + at);
+                else {
+                    for (uint32_t b = 0; b < g.bpp; b++) {
+                        const double e = (double)mine[at + b] - (double)ref[at + b];
+                        if (e < c.errMin || e > c.errMax) { bad++; break; }
+                    }
+                    continue;
+                }
+                if (d < c.errMin || d > c.errMax) bad++;
+            }
+        off += (size_t)g.w * g.h * g.bpp;
+    }
+    return bad;
+}
+
+std::string region_checksum(const std::vector<uint8_t> &mine, const std::vector<PlaneGeom> &geom, uint32_t w, uint32_t h,
+                            const CompareSpec &c)
+{
+    mvxe::Md5 md5;
+    size_t off = 0;
+    for (const PlaneGeom &g : geom) {
+        const uint32_t sx = c.rect ? c.region.start_x * g.w / w : 0, sy = c.rect ? c.region.start_y * g.h / h : 0;
+        const uint32_t ex = c.rect ? std::min(g.w, c.region.end_x * g.w / w) : g.w;
+        const uint32_t ey = c.rect ? std::min(g.h, c.region.end_y * g.h / h) : g.h;
+        for (uint32_t y = sy; y < ey; y++) md5.update(&mine[off + ((size_t)y * g.w + sx) * g.bpp], (size_t)(ex - sx) * g.bpp);
+        off += (size_t)g.w * g.h * g.bpp;
+    }
+    return md5.hex();
+}
+
+// one frame's COMPARE of an item; returns -1 on an I/O failure, else the number of mismatches
+long compare_frame(IoItem &it, int frame, bool verbose)
+{
+    std::vector<uint8_t> mine;
+    vx_enum type = 0;
+    vxQueryReference(it.ref, VX_REFERENCE_TYPE, &type, sizeof(type));
+    const bool perFrame = it.file.find('%') != std::string::npos;
+    const std::string fn = frame_file(it.file, frame, it.ref);
+    if (!it.fp && it.once.empty()) {
+        it.fp = fopen(fn.c_str(), it.cmp.checksumSave ? "w" : (it.cmp.checksum ? "r" : "rb"));
+        if (!it.fp) return -1;
+    }
+    // raw reference data: a file holding exactly one frame serves every frame
+    auto next_ref = [&](size_t bytes, std::vector<uint8_t> &ref) -> bool {
+        if (!it.once.empty()) { ref = it.once; return ref.size() == bytes; }
+        if (!perFrame && frame == 0) {
+            fseek(it.fp, 0, SEEK_END);
+            const long total = ftell(it.fp);
+            fseek(it.fp, 0, SEEK_SET);
+            if (total == (long)bytes) {
+                it.once.resize(bytes);
+                const bool ok = fread(it.once.data(), 1, bytes, it.fp) == bytes;
+                fclose(it.fp);
+                it.fp = nullptr;
+                ref = it.once;
+                return ok;
+            }
+        }
+        ref.resize(bytes);
+        return fread(ref.data(), 1, bytes, it.fp) == bytes;
+    };
+    long bad = 0;
+    if (type != VX_TYPE_IMAGE) {
+        // arrays and scalars: exact bytes, successive records of the same size
+        if (object_bytes(it.ref, mine, true)) return -1;
+        std::vector<uint8_t> ref;
+        if (!next_ref(mine.size(), ref)) return -1;
+        for (size_t k = 0; k < mine.size(); k++) bad += mine[k] != ref[k];
+        if (bad) printf("runvx: COMPARE %s: %ld byte(s) differ from frame %d of %s\n", it.name.c_str(), bad, frame, fn.c_str());
+    } else {
+        vx_image img = (vx_image)it.ref;
+        std::vector<PlaneGeom> geom;
+        vx_uint32 w = 0, h = 0;
+        vx_df_image fmt = 0;
+        vxQueryImage(img, VX_IMAGE_WIDTH, &w, sizeof(w));
+        vxQueryImage(img, VX_IMAGE_HEIGHT, &h, sizeof(h));
+        vxQueryImage(img, VX_IMAGE_FORMAT, &fmt, sizeof(fmt));
+        if (image_planes(img, mine, geom)) return -1;
+        if (it.cmp.checksumSave) {
+            fprintf(it.fp, "%s\n", region_checksum(mine, geom, w, h, it.cmp).c_str());
+        } else if (it.cmp.checksum) {
+            char want[64] = {0};
+            if (fscanf(it.fp, "%63s", want) != 1) return -1;
+            const std::string got = region_checksum(mine, geom, w, h, it.cmp);
+            if (got != want) {
+                printf("runvx: COMPARE %s: checksum %s instead of %s (frame %d of %s)\n", it.name.c_str(), got.c_str(), want, frame, fn.c_str());
+                bad = 1;
+            }
+        } else {
+            std::vector<uint8_t> ref;
+            if (!next_ref(mine.size(), ref)) return -1;
+            bad = (long)count_mismatches(fmt, mine, ref.data(), geom, w, h, it.cmp);
+            if (bad) printf("runvx: COMPARE %s: %ld pixel(s) differ from frame %d of %s\n", it.name.c_str(), bad, frame, fn.c_str());
+        }
+    }
+    if (!bad && verbose) printf("runvx: COMPARE %s: match (frame %d)\n", it.name.c_str(), frame);
+    if (perFrame && it.fp) { fclose(it.fp); it.fp = nullptr; }
+    return bad;
+}
+
+// one frame's READ of an item: the next frame of a multi-frame file, or the single frame again
+bool read_frame(IoItem &it, int frame)
+{
+    if (!it.once.empty()) return object_bytes(it.ref, it.once, false) == VX_SUCCESS;
+    const std::string fn = frame_file(it.file, frame, it.ref);
+    std::vector<uint8_t> cur;
+    if (object_bytes(it.ref, cur, true)) return false;   // the frame size
+    const size_t frameBytes = cur.size();
+    if (frame == 0 || fn != it.file) {
+        std::vector<uint8_t> all;
+        if (!read_file(fn, all)) return false;
+        if (fn == it.file && all.size() == frameBytes) {
+            it.once = all;
+            return object_bytes(it.ref, it.once, false) == VX_SUCCESS;
+        }
+        if (fn != it.file) return object_bytes(it.ref, all, false) == VX_SUCCESS;
+        it.fp = fopen(fn.c_str(), "rb");
+        if (!it.fp) return false;
+    }
+    std::vector<uint8_t> data(frameBytes);
+    if (!it.fp || fread(data.data(), 1, frameBytes, it.fp) != frameBytes) return false;
+    return object_bytes(it.ref, data, false) == VX_SUCCESS;
+}
+
 int usage()
 {
     fprintf(stderr, "usage: runvx [-frames:<n>] [-affinity:CPU|GPU[<n>]] [-dump-profile] [-discard-compare-errors] [-v] "
@@ -185,16 +423,18 @@ int main(int argc, char **argv)
     if (!rc && (s = vxVerifyGraph(graph))) { fprintf(stderr, "runvx: vxVerifyGraph failed (%d)\n", s); rc = 1; }
     auto t1 = std::chrono::steady_clock::now();
     if (!rc) printf("runvx: graph verified in %.3f ms\n", std::chrono::duration<double, std::milli>(t1 - t0).count());
-    for (IoItem &it : io) {
-        if (rc || it.op != "READ") continue;
-        std::vector<uint8_t> data;
-        if (!read_file(it.file, data) || object_bytes(it.ref, data, false)) {
-            fprintf(stderr, "runvx: READ %s from %s failed\n", it.name.c_str(), it.file.c_str());
-            rc = 1;
-        }
-    }
     double total = 0, best = 1e30;
+    int done = 0;
     for (int f = 0; f < frames && !rc; f++) {
+        bool eos = false;
+        for (IoItem &it : io) {
+            if (it.op != "READ") continue;
+            if (!read_frame(it, f)) {
+                if (f == 0) { fprintf(stderr, "runvx: READ %s from %s failed\n", it.name.c_str(), it.file.c_str()); rc = 1; }
+                eos = true;   // end of a multi-frame input: stop, as the reference does
+            }
+        }
+        if (rc || eos) break;
         auto a0 = std::chrono::steady_clock::now();
         if ((s = vxProcessGraph(graph))) { fprintf(stderr, "runvx: vxProcessGraph failed (%d) at frame %d\n", s, f); rc = 1; break; }
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a0).count();
@@ -211,24 +451,20 @@ int main(int argc, char **argv)
                 }
                 if (fp) fclose(fp);
             } else if (it.op == "COMPARE") {
-                std::vector<uint8_t> mine, ref;
-                if (object_bytes(it.ref, mine, true) || !read_file(it.file, ref)) {
+                const long bad = compare_frame(it, f, verbose);
+                if (bad < 0) {
                     fprintf(stderr, "runvx: COMPARE %s with %s failed\n", it.name.c_str(), it.file.c_str());
                     rc = 1;
-                    continue;
-                }
-                size_t diff = mine.size() == ref.size() ? 0 : std::max(mine.size(), ref.size());
-                for (size_t k = 0; k < std::min(mine.size(), ref.size()); k++) diff += mine[k] != ref[k];
-                if (diff) {
-                    printf("runvx: COMPARE %s: %zu bytes differ from %s\n", it.name.c_str(), diff, it.file.c_str());
-                    if (!discardCompare) rc = 2;
-                } else if (verbose) {
-                    printf("runvx: COMPARE %s: match\n", it.name.c_str());
+                } else if (bad && !discardCompare) {
+                    rc = 2;
                 }
             }
         }
+        done++;
     }
-    if (!rc) printf("runvx: %d frame(s): avg %.3f ms, min %.3f ms per frame\n", frames, total / frames, best);
+    for (IoItem &it : io)
+        if (it.fp) fclose(it.fp);
+    if (!rc) printf("runvx: %d frame(s): avg %.3f ms, min %.3f ms per frame\n", done, done ? total / done : 0.0, done ? best : 0.0);
     if (profile && !rc) {
         vx_perf_t perf;
         vxQueryGraph(graph, VX_GRAPH_PERFORMANCE, &perf, sizeof(perf));

@@ -177,3 +177,16 @@ def test_runvx_cli_reports_usage_and_bad_files():
     assert r.returncode == 1 and b"usage" in r.stderr
     r = subprocess.run([openvx.RUNVX, "/nonexistent.gdf"], capture_output=True)
     assert r.returncode == 1
+
+
+def test_runvx_md5_matches_rfc1321(tmp_path):
+    """runvx's image checksums are MD5 (the reference's CHasher); the engine's own implementation
+    against hashlib on RFC 1321's test suite and on lengths around the 64-byte block edges"""
+    import hashlib
+    exe = tmp_path / "md5_probe"
+    subprocess.run(["g++", "-std=c++17", "-O2", os.path.join(ROOT, "tests", "c", "md5_probe.cpp"), "-o", str(exe)], check=True)
+    msgs = ["", "a", "abc", "message digest", "abcdefghijklmnopqrstuvwxyz",
+            "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789", "1234567890" * 8] + \
+           ["x" * n for n in (55, 56, 57, 63, 64, 65, 119, 120, 128, 1000)]
+    out = subprocess.run([str(exe)] + msgs, capture_output=True, check=True).stdout.decode().split()
+    assert out == [hashlib.md5(m.encode()).hexdigest() for m in msgs]
