@@ -60,6 +60,22 @@ MVX_DEV void st16_partial(uint8_t *p, const uint32_t (&o)[16], int n)
     }
 }
 
+// a / b for a divisor b fixed per launch, given y = RN(1/b) from recip_rn(b): the product a*y, its
+// exact remainder r = a - b*(a*y) by FMA, and one FMA correction (Markstein's correction step).
+// Where it replaces an IEEE division the replacement is checked exhaustively over every float a
+// for the divisor used (tests/hip/probe_arith.hip); r == 0 returns a*y itself, which keeps the
+// sign of a zero quotient.
+MVX_DEV float div_by_const_rn(float a, float b, float y)
+{
+    const float q = a * y;
+    const float r = fmaf(-q, b, a);
+    return r == 0.0f ? q : fmaf(r, y, q);
+}
+
+// RN(1/b) in single precision: the double quotient is within half a double ulp of 1/b, and 1/b
+// (b a float) is never a single-precision rounding midpoint, so rounding it to float is exact
+inline float recip_rn(float b) { return (float)(1.0 / (double)b); }
+
 inline hipStream_t stream_of(mvx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }

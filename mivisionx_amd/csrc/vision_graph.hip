@@ -68,6 +68,7 @@ struct GraphParams {
     int lvlW[8], lvlH[8];
     uint32_t lower, upper;
     float k, thr, norm, dist;
+    float normRcp;           // RN(1 / norm)
     uint32_t capacity, candCap;
 };
 
@@ -113,6 +114,9 @@ MVX_DEV uint16_t canny_pack_f(int gx, int gyn)
     t = (int16_t)(t | (int16_t)(canny_sector_f((int16_t)gx, (int16_t)gyn) & 3u));
     return (uint16_t)t;
 }
+
+// the value of lane + 1 of the wave (DPP wave_shl:1; lane 63 reads 0)
+MVX_DEV int lane_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true); }
 
 // union-find with atomicMin linking (the larger root hangs under the smaller): the final root of
 // a component is its smallest index, whatever the order of the unions
@@ -230,65 +234,59 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
     //    sliding window of 3-column sums of the HG3 products {gx^2, gx gy, gy^2}.  HG3 rows 0 and h-1
     //    are never written (zero).  The products are integers whose 9-sums stay below 2^24, so int32
     //    sums equal the reference's float sums exactly.
-    if (lane < MW) {
+    //    Each lane forms the products of its own gradient column; the 3-column sums (columns
+    //    lane..lane+2) take the right neighbours' through DPP wave shifts, so this part runs with all
+    //    64 lanes active (lanes >= MW compute values nobody stores).
+    {
         const int x = tx0 - 1 + lane;
         const int r0 = wave * ((MH + NW - 1) / NW), r1 = min(r0 + (MH + NW - 1) / NW, MH);
         int hxx[3], hxy[3], hyy[3];
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int gr = r0 + q;
-            hxx[q] = hxy[q] = hyy[q] = 0;
+        auto hsum = [&](int gr, int &sxx, int &sxy, int &syy) {
+            int pxx = 0, pxy = 0, pyy = 0;
             if (INNER || (ty0 - 2 + gr >= 1 && ty0 - 2 + gr <= h - 2)) {
-#pragma unroll
-                for (int dx = 0; dx < 3; dx++) {
-                    const int gx = s.Gx[gr][lane + dx], gy = s.Gy[gr][lane + dx];
-                    hxx[q] += gx * gx; hxy[q] += gx * gy; hyy[q] += gy * gy;
-                }
+                const int gx = s.Gx[gr][lane], gy = s.Gy[gr][lane];
+                pxx = gx * gx; pxy = gx * gy; pyy = gy * gy;
             }
-        }
+            sxx = pxx + lane_next(pxx + lane_next(pxx));
+            sxy = pxy + lane_next(pxy + lane_next(pxy));
+            syy = pyy + lane_next(pyy + lane_next(pyy));
+        };
+        hsum(r0, hxx[0], hxy[0], hyy[0]);
+        hsum(r0 + 1, hxx[1], hxy[1], hyy[1]);
         for (int r = r0; r < r1; r++) {
             const int y = ty0 - 1 + r;
-            // Canny
-            uint16_t m = 0;
-            if (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w)) {
-                m = canny_pack_f(s.Gx[r + 1][lane + 1], -(int)s.Gy[r + 1][lane + 1]);
-            } else if (y >= 1 && y <= h - 2 && x == 0) {
-                // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
-                // r1[-1] = r0[ALIGN16(w)-1]); luma window column 3 is x = 0, rows r+1..r+3 are y-1..y+1
-                const int t0_1 = (int)s.Y[r + 1][4] + (int)s.Y[r + 3][4] + 2 * (int)s.Y[r + 2][4];
-                const int t1_0 = (int)s.Y[r + 3][3] - (int)s.Y[r + 1][3];
-                const int t1_1 = (int)s.Y[r + 3][4] - (int)s.Y[r + 1][4];
-                const int t1_m1 = (w % 16 == 0) ? ((int)yflat(F, P, y - 1, w - 1) + (int)yflat(F, P, y + 1, w - 1) +
-                                                   2 * (int)yflat(F, P, y, w - 1)) : 0;
-                m = canny_pack_f(t0_1, -(t1_m1 + 2 * t1_0 + t1_1));
-            }
-            s.M[r][lane] = m;
-            // Harris
-            const int gr = r + 2;
-            hxx[2] = hxy[2] = hyy[2] = 0;
-            if (INNER || (ty0 - 2 + gr >= 1 && ty0 - 2 + gr <= h - 2)) {
-#pragma unroll
-                for (int dx = 0; dx < 3; dx++) {
-                    const int gx = s.Gx[gr][lane + dx], gy = s.Gy[gr][lane + dx];
-                    hxx[2] += gx * gx; hxy[2] += gx * gy; hyy[2] += gy * gy;
+            hsum(r + 2, hxx[2], hxy[2], hyy[2]);
+            if (lane < MW) {
+                // Canny
+                uint16_t m = 0;
+                if (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w)) {
+                    m = canny_pack_f(s.Gx[r + 1][lane + 1], -(int)s.Gy[r + 1][lane + 1]);
+                } else if (y >= 1 && y <= h - 2 && x == 0) {
+                    // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
+                    // r1[-1] = r0[ALIGN16(w)-1]); luma window column 3 is x = 0, rows r+1..r+3 are y-1..y+1
+                    const int t0_1 = (int)s.Y[r + 1][4] + (int)s.Y[r + 3][4] + 2 * (int)s.Y[r + 2][4];
+                    const int t1_0 = (int)s.Y[r + 3][3] - (int)s.Y[r + 1][3];
+                    const int t1_1 = (int)s.Y[r + 3][4] - (int)s.Y[r + 1][4];
+                    const int t1_m1 = (w % 16 == 0) ? ((int)yflat(F, P, y - 1, w - 1) + (int)yflat(F, P, y + 1, w - 1) +
+                                                       2 * (int)yflat(F, P, y, w - 1)) : 0;
+                    m = canny_pack_f(t0_1, -(t1_m1 + 2 * t1_0 + t1_1));
                 }
-            }
-            float v = 0.0f;
-            if (INNER || (x >= 1 && x <= w - 2 && y >= 1 && y <= h - 2)) {
-                const float gx2 = (float)(hxx[0] + hxx[1] + hxx[2]);
-                const float gxy2 = (float)(hxy[0] + hxy[1] + hxy[2]);
-                const float gy2 = (float)(hyy[0] + hyy[1] + hyy[2]);
-                const float tr = gx2 + gy2;
-                const float det = (gx2 * gy2) - (gxy2 * gxy2);
-                const float mc = det - (P.k * tr * tr);
-                // Vc = mc / norm > T ? mc / norm : 0; with norm > 0 and T >= 0 a non-positive mc can
-                // never pass, so the IEEE divide runs only for positive mc (exactly the same result)
-                if (mc > 0.0f || P.thr < 0.0f) {
-                    const float q = __fdiv_rn(mc, P.norm);
+                s.M[r][lane] = m;
+                // Harris
+                float v = 0.0f;
+                if (INNER || (x >= 1 && x <= w - 2 && y >= 1 && y <= h - 2)) {
+                    const float gx2 = (float)(hxx[0] + hxx[1] + hxx[2]);
+                    const float gxy2 = (float)(hxy[0] + hxy[1] + hxy[2]);
+                    const float gy2 = (float)(hyy[0] + hyy[1] + hyy[2]);
+                    const float tr = gx2 + gy2;
+                    const float det = (gx2 * gy2) - (gxy2 * gxy2);
+                    const float mc = det - (P.k * tr * tr);
+                    // Vc = mc / norm > T ? mc / norm : 0, the quotient exactly the IEEE one
+                    const float q = div_by_const_rn(mc, P.norm, P.normRcp);
                     v = (q > P.thr) ? q : 0.0f;
                 }
+                s.V[r][lane] = v;
             }
-            s.V[r][lane] = v;
             hxx[0] = hxx[1]; hxy[0] = hxy[1]; hyy[0] = hyy[1];
             hxx[1] = hxx[2]; hxy[1] = hxy[2]; hyy[1] = hyy[2];
         }
@@ -678,6 +676,7 @@ inline GraphParams graph_params(const mvx_vision_params_t &p, const GraphLayout 
     // normFactor of ago_kernel_api.cpp:16941-16942 for gradient size 3, in float, left to right
     float nf = 255.0f * (float)(1 << 2) * 3;
     P.norm = nf * nf * nf * nf;
+    P.normRcp = recip_rn(P.norm);
     P.capacity = p.cornerCapacity;
     P.candCap = L.candCap;
     return P;

@@ -5,7 +5,9 @@
 //   * div_small_rn(a, b) vs IEEE single division (__fdiv_rn), all integers 0 <= a <= b <= 32768;
 //   * mag_body(gx, gy) vs rint(sqrt((double)(gx^2 + gy^2))) saturated to 32767 -- the reference's
 //     cvtpd_epi32(sqrt_pd(.)) and packs (ago_haf_cpu_arithmetic.cpp:5031-5078) -- for all
-//     0 <= gy <= gx <= 32768 (the result depends on |gx|, |gy| only).
+//     0 <= gy <= gx <= 32768 (the result depends on |gx|, |gy| only);
+//   * div_by_const_rn(a, b, RN(1/b)) vs IEEE single division for every finite float a, for a given b;
+//   * the lane mapping of the DPP wave_shl:1 control.
 #include "../../mivisionx_amd/csrc/arith.hpp"
 
 using namespace mvx;
@@ -52,6 +54,55 @@ static int run(void (*kernel)(uint32_t, uint32_t *), uint32_t blocks, uint32_t a
     return rc;
 }
 
+// a / norm by div_by_const_rn against IEEE division for every finite float a (2^32 - 2^24 inputs)
+__global__ void k_div_const(float b, float y, uint32_t *out)
+{
+    const uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 256u;
+    for (uint32_t k = 0; k < 256; k++) {
+        const uint32_t bits = (uint32_t)(base + k);
+        if (((bits >> 23) & 0xffu) == 0xffu) continue;   // Inf and NaN
+        const float a = __uint_as_float(bits);
+        const float q = div_by_const_rn(a, b, y), r = __fdiv_rn(a, b);
+        if (__float_as_uint(q) != __float_as_uint(r)) {
+            atomicAdd(out, 1u);
+            atomicCAS(out + 1, 0u, bits);
+        }
+    }
+}
+
+// lane i of wave_shl:1 (DPP control 0x130) reads lane i + 1; lane 63 reads nothing (0)
+__global__ void k_dpp(int32_t *out)
+{
+    const int v = (int)threadIdx.x + 1000;
+    out[threadIdx.x] = __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
+}
+
 // result[0] = mismatches, result[1] = first mismatch packed as (b << 16) | a (b, a <= 32768)
 extern "C" int probe_div_small(uint32_t maxB, uint32_t *result) { return run(k_div, maxB, maxB, result); }
 extern "C" int probe_mag_body(uint32_t maxA, uint32_t *result) { return run(k_mag, maxA + 1, maxA, result); }
+
+// result[0] = mismatches, result[1] = bits of the first mismatching dividend
+extern "C" int probe_div_const(float b, uint32_t *result)
+{
+    uint32_t *d = nullptr;
+    if (hipMalloc(&d, 8) != hipSuccess) return -1;
+    int rc = -1;
+    if (hipMemset(d, 0, 8) == hipSuccess) {
+        hipLaunchKernelGGL(k_div_const, dim3(65536), dim3(256), 0, 0, b, recip_rn(b), d);   // 2^32 / 256 threads
+        if (hipGetLastError() == hipSuccess && hipMemcpy(result, d, 8, hipMemcpyDeviceToHost) == hipSuccess) rc = 0;
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+// out[64]: what each lane of one wave reads through wave_shl:1
+extern "C" int probe_dpp_wave_shl1(int32_t *out)
+{
+    int32_t *d = nullptr;
+    if (hipMalloc(&d, 64 * sizeof(int32_t)) != hipSuccess) return -1;
+    int rc = -1;
+    hipLaunchKernelGGL(k_dpp, dim3(1), dim3(64), 0, 0, d);
+    if (hipGetLastError() == hipSuccess && hipMemcpy(out, d, 64 * sizeof(int32_t), hipMemcpyDeviceToHost) == hipSuccess) rc = 0;
+    (void)hipFree(d);
+    return rc;
+}

@@ -5,6 +5,8 @@ rounding used by Magnitude equals the reference's rint(sqrt(double)) for every |
 import ctypes
 import os
 
+import numpy as np
+
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -34,3 +36,20 @@ def test_division_all_pairs(probe):
 def test_magnitude_rounding_all_pairs(probe):
     bad, first = _run(probe.probe_mag_body, 32768)
     assert bad == 0, f"{bad} magnitudes differ; first (gx, gy) = {first}"
+
+
+@pytest.mark.parametrize("gradient", [3])
+def test_harris_norm_division_all_floats(probe, gradient):
+    """the Harris score divides by the fixed norm (255 * 2^(g-1) * 3)^4; the reciprocal-and-FMA
+    replacement equals IEEE division for every finite float dividend"""
+    f = np.float32(255.0) * np.float32(1 << (gradient - 1)) * np.float32(3.0)
+    norm = ((f * f) * f) * f
+    out = (ctypes.c_uint32 * 2)()
+    assert probe.probe_div_const(ctypes.c_float(float(norm)), out) == 0
+    assert out[0] == 0, f"{out[0]} quotients differ; first dividend bits {out[1]:#010x}"
+
+
+def test_dpp_wave_shift_left_lane_map(probe):
+    out = (ctypes.c_int32 * 64)()
+    assert probe.probe_dpp_wave_shl1(out) == 0
+    assert list(out)[:63] == [1001 + i for i in range(63)]
