@@ -12,10 +12,11 @@
 //   k_front     one 58x32 luma tile (+3 halo) per workgroup of four waves.  Every window row of the
 //               tile (64 luma, 62 gradient, 60 score columns) fits one 64-lane wave, so lane = column
 //               and a wave walks rows: RGB -> Y in LDS, then level 0, the 2:1 scaled image, level 1,
-//               Canny gradients/NMS/threshold with tile-local hysteresis (LDS union-find), Harris
-//               products / exact integer window sums / score / NMS with candidate append.  Only graph
-//               outputs and small per-tile tables reach HBM.  Tiles away from the image border take a
-//               branch-free instantiation.
+//               Canny gradients/NMS/threshold with tile-local hysteresis (row-mask dilation; union-find
+//               only over the weak components that touch the tile border), Harris products / exact
+//               integer window sums / score / NMS with candidate append.  Only graph outputs and small
+//               per-tile tables reach HBM.  Tiles away from the image border take a branch-free
+//               instantiation.
 //   k_border    unions the hysteresis components that touch tile borders, across tiles.  Strong
 //               components get ids below every weak one, so after atomicMin linking a set's root is
 //               strong iff the set holds a strong component -- no marking pass.
@@ -161,23 +162,18 @@ MVX_DEV void gunion(uint32_t *par, uint32_t a, uint32_t b)
 }
 
 // Arrays whose lifetimes do not overlap share LDS (every hand-over is separated by a barrier):
-// the gradients (steps 2-3) with the Canny states and row masks (steps 4-6), the Canny magnitude
-// (steps 3-4) with the border flags (step 6), the Harris score (steps 3-4 and the candidate append)
-// with the union-find labels (step 6).  37.8 KB -> 24.7 KB per workgroup: 6 instead of 4 resident
-// workgroups per CU.
+// the gradients (steps 2-3) with the row masks of the Canny states (steps 4-6), the Harris score
+// (steps 3-4 and the candidate append) with the union-find labels (step 6).  Under 25 KB per
+// workgroup: 6 resident workgroups per CU.
 struct FrontShared {
     uint8_t Y[YH][TP];
     union {
         struct { int16_t Gx[GH][TP], Gy[GH][TP]; };
-        struct {
-            uint8_t S[TS];                     // Canny states 0 / 127 / 255 of the tile's pixels
-            uint64_t Em[TH], Sm[TH], Rm[TH];   // per row: edge, strong, reached-from-strong pixels (bit = column)
-        };
+        // per row (bit = column): edge (127/255), strong (255), reached from a strong pixel, and
+        // weak pixels of the components that touch the tile border but hold no strong pixel
+        struct { uint64_t Em[TH], Sm[TH], Rm[TH], Bm[TH]; };
     };
-    union {
-        uint16_t M[MH][TP];
-        uint8_t FB[TS];                        // root flag: the component touches the tile border
-    };
+    uint16_t M[MH][TP];
     union {
         float V[MH][TP];
         int L[TS];                             // union-find labels of the weak pixels no strong pixel reaches
@@ -297,7 +293,6 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
     uint32_t myCand = 0;                 // bit q: row wave + q*NW is a candidate
     for (int q = 0; q < TH / NW; q++) {
         const int r = wave + q * NW;
-        const int p = r * TP + lane;
         uint8_t st = 0;
         bool cand = false;
         const int y = ty0 + r, x = tx0 + lane;
@@ -314,7 +309,6 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
             cand = c >= s.V[j - 1][i - 1] && c >= s.V[j - 1][i] && c >= s.V[j - 1][i + 1] && c >= s.V[j][i - 1] &&
                    c > s.V[j][i + 1] && c > s.V[j + 1][i - 1] && c > s.V[j + 1][i] && c > s.V[j + 1][i + 1];
         }
-        s.S[p] = st;
         if (cand) myCand |= 1u << q;
         // row masks of edge (127/255) and strong (255) pixels: bit = column
         const uint64_t em = __ballot(st != 0), sm = __ballot(st == 255);
@@ -391,16 +385,25 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
         }
     }
 
-    // 6. tile-local hysteresis.  (a) The edge pixels that a strong pixel reaches inside the tile:
-    //    one wave, lane = row, 64-bit row masks, dilate-and-mask until nothing changes.
+    // 6. tile-local hysteresis.  (a) One wave, lane = row, 64-bit row masks, dilate-and-mask until
+    //    nothing changes, two problems side by side: lanes 0..31 grow the edge pixels that a strong
+    //    pixel reaches inside the tile (the union of the 8-connected edge components that hold a
+    //    strong pixel), lanes 32..63 grow those that a tile-border edge pixel reaches (the components
+    //    that touch the tile border).  Edge pixels in neither are weak components inside the tile:
+    //    they settle to 0 here.  Border-touching components without a strong pixel are the only ones
+    //    left undecided.
+    static_assert(TH <= 32 && TW <= 63, "row masks: one lane per row in each half-wave");
     __syncthreads();
     if (wave == 0) {
-        const uint64_t e = lane < TH ? s.Em[lane] : 0ull;
-        uint64_t m = lane < TH ? s.Sm[lane] : 0ull;
+        const int r = lane & 31;
+        const uint64_t e = r < TH ? s.Em[r] : 0ull;
+        uint64_t m;
+        if (lane < 32) m = r < TH ? s.Sm[r] : 0ull;
+        else m = (r == 0 || r == th - 1) ? e : (e & (1ull | (1ull << (tw - 1))));
         while (true) {
             uint64_t up = __shfl(m, lane > 0 ? lane - 1 : 0), dn = __shfl(m, lane < 63 ? lane + 1 : 63);
-            if (lane == 0) up = 0ull;
-            if (lane >= TH - 1) dn = 0ull;
+            if (r == 0) up = 0ull;
+            if (r >= TH - 1) dn = 0ull;
             uint64_t v = m | up | dn;
             v |= (v << 1) | (v >> 1);
             const uint64_t n = (v & e) | m;
@@ -408,58 +411,52 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
             m = n;
             if (!__any(changed)) break;
         }
-        if (lane < TH) s.Rm[lane] = m;
-    }
-    __syncthreads();
-    //    (b) the weak pixels left over: union-find among themselves (8-connectivity in the tile)
-    for (int q = 0; q < TH / NW; q++) {
-        const int r = wave + q * NW;
-        const int p = r * TP + lane;
-        const bool wk = s.S[p] && !((s.Rm[r] >> lane) & 1ull);
-        s.L[p] = wk ? p : -1;
-        s.FB[p] = 0;
-    }
-    __syncthreads();
-    for (int q = 0; q < TH / NW; q++) {
-        const int r = wave + q * NW;
-        const int p = r * TP + lane;
-        if (s.L[p] < 0) continue;
-        if (lane + 1 < TW && s.L[p + 1] >= 0) lunion(s.L, p, p + 1);
-        if (r + 1 < TH) {
-            if (lane > 0 && s.L[p + TP - 1] >= 0) lunion(s.L, p, p + TP - 1);
-            if (s.L[p + TP] >= 0) lunion(s.L, p, p + TP);
-            if (lane + 1 < TW && s.L[p + TP + 1] >= 0) lunion(s.L, p, p + TP + 1);
+        const uint64_t other = __shfl(m, lane ^ 32);
+        if (r < TH) {
+            if (lane < 32) s.Rm[r] = m;
+            else s.Bm[r] = m & ~other;
         }
     }
     __syncthreads();
+    //    (b) union-find over the undecided pixels (8-connectivity in the tile).  Every 8-neighbour
+    //    of an undecided pixel that is an edge pixel lies in the same component, so the row masks
+    //    answer the neighbour tests and only undecided pixels are labelled.
     for (int q = 0; q < TH / NW; q++) {
         const int r = wave + q * NW;
-        const int p = r * TP + lane;
-        if (s.L[p] < 0) continue;
-        const int root = lfind(s.L, p);
-        if (lane == 0 || r == 0 || lane == tw - 1 || r == th - 1) s.FB[root] = 1;
+        if ((s.Bm[r] >> lane) & 1ull) s.L[r * TP + lane] = r * TP + lane;
     }
     __syncthreads();
-    //    (c) settle: reached -> 255; leftover weak components inside the tile -> 0; leftover ones
-    //    touching the tile border stay 127 for k_border / k_resolve under global id 1 + tile*TS + root.
-    //    Every reached border pixel maps to the single strong node 0 (the smallest id, so after
-    //    atomicMin linking a set is strong iff its root is 0).
+    for (int q = 0; q < TH / NW; q++) {
+        const int r = wave + q * NW;
+        const uint64_t b = s.Bm[r];
+        if (!b) continue;
+        if (!((b >> lane) & 1ull)) continue;
+        const int p = r * TP + lane;
+        if ((b >> (lane + 1)) & 1ull) lunion(s.L, p, p + 1);
+        if (r + 1 < TH) {
+            const uint64_t bd = s.Bm[r + 1];
+            if (lane > 0 && ((bd >> (lane - 1)) & 1ull)) lunion(s.L, p, p + TP - 1);
+            if ((bd >> lane) & 1ull) lunion(s.L, p, p + TP);
+            if ((bd >> (lane + 1)) & 1ull) lunion(s.L, p, p + TP + 1);
+        }
+    }
+    __syncthreads();
+    //    (c) settle: reached -> 255; undecided -> 127 for k_border / k_resolve under global id
+    //    1 + tile*TS + root; other edge pixels -> 0.  Every reached border pixel maps to the single
+    //    strong node 0 (the smallest id, so after atomicMin linking a set is strong iff its root is 0).
     bool unres = false;
     for (int q = 0; q < TH / NW; q++) {
         const int r = wave + q * NW;
-        const int p = r * TP + lane;
         const int y = ty0 + r, x = tx0 + lane;
         if (lane >= TW || !(INNER || (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2))) continue;
         uint8_t o = 0;
         if ((s.Rm[r] >> lane) & 1ull) {
             o = 255;
-        } else if (s.L[p] >= 0) {
-            const int root = lfind(s.L, p);
-            if (s.FB[root]) {
-                o = 127;
-                unres = true;
-                if (root == p) F.parent[1u + (uint32_t)tile * TS + (uint32_t)p] = 1u + (uint32_t)tile * TS + (uint32_t)p;
-            }
+        } else if ((s.Bm[r] >> lane) & 1ull) {
+            const int p = r * TP + lane;
+            o = 127;
+            unres = true;
+            if (lfind(s.L, p) == p) F.parent[1u + (uint32_t)tile * TS + (uint32_t)p] = 1u + (uint32_t)tile * TS + (uint32_t)p;
         }
         F.edges[(long)y * F.edgesStride + x] = o;
     }
@@ -474,9 +471,8 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
         else { lx = tw - 1; ly = n - 2 * TW - TH; }
         int32_t g = -1;
         if (lx < tw && ly < th) {
-            const int p = ly * TP + lx;
             if ((s.Rm[ly] >> lx) & 1ull) g = 0;
-            else if (s.L[p] >= 0) g = (int32_t)(1u + (uint32_t)tile * TS + (uint32_t)lfind(s.L, p));
+            else if ((s.Bm[ly] >> lx) & 1ull) g = (int32_t)(1u + (uint32_t)tile * TS + (uint32_t)lfind(s.L, ly * TP + lx));
         }
         bt[n] = g;
     }
