@@ -23,6 +23,13 @@
 
 namespace mvx {
 
+// Device global memory reached through a generic pointer (one read out of a descriptor table rather
+// than a kernel argument).  Accesses through the address_space(1) pointer issue as global_*
+// instructions; generic ones issue as flat_*, which also count against the LDS wait counter, so an
+// LDS read placed after a flat store waits until the store has reached memory.
+template <class T> using gptr_t = __attribute__((address_space(1))) T *;
+template <class T> MVX_DEV gptr_t<T> gmem(T *p) { return (gptr_t<T>)p; }
+
 MVX_DEV uint32_t ld_u8(const uint8_t *p, long n, long i) { return (i < 0 || i >= n) ? 0u : (uint32_t)p[i]; }
 
 // 16 bytes at a 16-byte aligned address inside the allocation

@@ -20,7 +20,7 @@
 //   k_border    unions the hysteresis components that touch tile borders, across tiles.  Strong
 //               components get ids below every weak one, so after atomicMin linking a set's root is
 //               strong iff the set holds a strong component -- no marking pass.
-//   k_resolve   only tiles left with undecided weak pixels re-read their edge tile and settle them.
+//   k_resolve   the undecided weak pixels k_front listed per tile take their global root's state.
 //   k_half      pyramid levels 2.. (one launch per level, all frames).
 //   k_pick      Harris sort and pick, one workgroup per frame (pick.hpp).
 //
@@ -55,8 +55,9 @@ struct FrameDesc {
     void *corners; uint32_t *numItems; uint64_t *count;
     // per-frame workspace
     int32_t *border;      // tiles x 2(TW+TH) component ids of tile-border pixels (-1: none)
-    uint32_t *parent;     // 2 * tiles * TS union-find slots (sparsely used)
-    uint8_t *tileFlag;    // tiles: 1 when the tile holds undecided weak pixels
+    uint32_t *parent;     // 1 + tiles * TS union-find slots (sparsely used)
+    uint32_t *ulist;      // tiles x TS: a tile's undecided pixels, p | tile-local root << 16
+    uint32_t *ucount;     // tiles: undecided pixels listed
     void *cand;           // Harris XYS candidates
     uint32_t *candCount;  // candidates appended (reset by k_pick)
     void *pickWs;         // harris pick workspace
@@ -75,7 +76,7 @@ struct GraphParams {
 
 // luma of one RGB pixel, HafCpu_ColorConvert_Y_RGB (ago_haf_cpu_color_convert.cpp:2913-2990):
 // ((R * 0.2126f) + G * 0.7152f) + B * 0.0722f in float, truncated, saturated
-MVX_DEV uint32_t luma(const uint8_t *p)
+template <class Ptr> MVX_DEV uint32_t luma(Ptr p)
 {
     float v = (float)p[0] * 0.2126f;
     v = v + (float)p[1] * 0.7152f;
@@ -87,13 +88,13 @@ MVX_DEV uint32_t luma(const uint8_t *p)
 // the virtual luma plane at linear position (r, c): stride yStride, padding and guard bytes zero
 MVX_DEV uint32_t yflat(const FrameDesc &F, const GraphParams &P, int r, int c)
 {
-    if (c >= 0 && c < P.w && r >= 0 && r < P.h) return luma(F.rgb + (long)r * F.rgbStride + 3L * c);
+    if (c >= 0 && c < P.w && r >= 0 && r < P.h) return luma(gmem(F.rgb) + (long)r * F.rgbStride + 3L * c);
     const long i = (long)r * P.yStride + c;
     if (i < 0 || i >= (long)P.h * P.yStride) return 0;
     r = (int)(i / P.yStride);
     c = (int)(i - (long)r * P.yStride);
     if (c >= P.w) return 0;
-    return luma(F.rgb + (long)r * F.rgbStride + 3L * c);
+    return luma(gmem(F.rgb) + (long)r * F.rgbStride + 3L * c);
 }
 
 // HafCpu_FastAtan2_Canny (ago_haf_cpu_arithmetic.cpp:9638-9656), statement order kept
@@ -141,21 +142,21 @@ MVX_DEV void lunion(int *L, int a, int b)
     }
 }
 
-MVX_DEV uint32_t gfind(const uint32_t *par, uint32_t p)
+MVX_DEV uint32_t gfind(gptr_t<const uint32_t> par, uint32_t p)
 {
     uint32_t l = par[p];
     while (l != p) { p = l; l = par[p]; }
     return p;
 }
 
-MVX_DEV void gunion(uint32_t *par, uint32_t a, uint32_t b)
+MVX_DEV void gunion(gptr_t<uint32_t> par, uint32_t a, uint32_t b)
 {
     while (true) {
         a = gfind(par, a);
         b = gfind(par, b);
         if (a == b) return;
         if (a < b) { uint32_t t = a; a = b; b = t; }
-        const uint32_t old = atomicMin(&par[a], b);
+        const uint32_t old = __hip_atomic_fetch_min(par + a, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == a) return;
         a = old;
     }
@@ -178,7 +179,7 @@ struct FrontShared {
         float V[MH][TP];
         int L[TS];                             // union-find labels of the weak pixels no strong pixel reaches
     };
-    uint32_t cnt, base, unres;
+    uint32_t cnt, base, unres;   // candidates, their global offset, undecided pixels listed
 };
 
 // INNER: the whole luma window lies inside the image and every Canny / Harris position of the tile
@@ -195,11 +196,11 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
     if (INNER) {
         constexpr int RPW = (YH + NW - 1) / NW;   // rows per wave
         uint8_t px[RPW][3];
-        const uint8_t *col = F.rgb + (long)(ty0 - HY + wave) * F.rgbStride + 3L * (tx0 - HY + lane);
+        gptr_t<const uint8_t> col = gmem(F.rgb) + (long)(ty0 - HY + wave) * F.rgbStride + 3L * (tx0 - HY + lane);
 #pragma unroll
         for (int k = 0; k < RPW; k++) {
             if (wave + k * NW < YH) {
-                const uint8_t *p = col + (long)(k * NW) * F.rgbStride;
+                gptr_t<const uint8_t> p = col + (long)(k * NW) * F.rgbStride;
                 px[k][0] = p[0]; px[k][1] = p[1]; px[k][2] = p[2];
             }
         }
@@ -319,20 +320,19 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
     uint32_t myOff = 0;
     if (nCand) myOff = atomicAdd(&s.cnt, nCand);
     __syncthreads();
-    if (t == 0 && s.cnt) s.base = atomicAdd(F.candCount, s.cnt);
+    if (t == 0 && s.cnt) s.base = __hip_atomic_fetch_add(gmem(F.candCount), s.cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (nCand) {
-        XYS *list = (XYS *)F.cand;
+        // one XYS record {int16 x, int16 y, float s} as a single 64-bit store
+        static_assert(sizeof(XYS) == 8, "XYS layout");
+        gptr_t<uint64_t> list = gmem((uint64_t *)F.cand);
         uint32_t pos = s.base + myOff;
         for (int q = 0; q < TH / NW; q++) {
             if (!(myCand & (1u << q))) continue;
             const int r = wave + q * NW;
             if (pos < P.candCap) {
-                XYS it;
-                it.x = (int16_t)(tx0 + lane);
-                it.y = (int16_t)(ty0 + r);
-                it.s = s.V[r + 1][lane + 1];
-                list[pos] = it;
+                list[pos] = (uint64_t)(uint16_t)(tx0 + lane) | ((uint64_t)(uint16_t)(ty0 + r) << 16) |
+                            ((uint64_t)__float_as_uint(s.V[r + 1][lane + 1]) << 32);
             }
             pos++;
         }
@@ -340,7 +340,7 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
 
     // 5. outputs that only need luma: level 0, the 2:1 scaled image, level 1
     if (lane < tw) {
-        uint8_t *l0 = F.lvl[0] + (long)ty0 * F.lvlStride[0] + tx0 + lane;
+        gptr_t<uint8_t> l0 = gmem(F.lvl[0]) + (long)ty0 * F.lvlStride[0] + tx0 + lane;
         for (int r = wave; r < th; r += NW) l0[(long)r * F.lvlStride[0]] = s.Y[r + HY][lane + HY];
     }
     if (lane < TW / 2) {
@@ -361,7 +361,7 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
                 } else {
                     v = (128 * 128 * (p00 + p01 + p10 + p11) + 0x8000) >> 16;
                 }
-                F.scaled[(long)Yo * F.scaledStride + X] = (uint8_t)v;
+                gmem(F.scaled)[(long)Yo * F.scaledStride + X] = (uint8_t)v;
             }
         }
         // ScaleGaussianHalf_5x5 level 1 (ago_haf_cpu_pyramid.cpp:94-305): centre (2X+ox, 2Y+oy)
@@ -380,7 +380,7 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
                     const uint32_t hr = rr[0] + 4u * rr[1] + 6u * rr[2] + 4u * rr[3] + rr[4];
                     acc += (jj == 0 || jj == 4 ? 1u : jj == 2 ? 6u : 4u) * hr;
                 }
-                F.lvl[1][(long)Yo * F.lvlStride[1] + X] = (uint8_t)((acc & 0xffffu) >> 8);
+                gmem(F.lvl[1])[(long)Yo * F.lvlStride[1] + X] = (uint8_t)((acc & 0xffffu) >> 8);
             }
         }
     }
@@ -441,10 +441,10 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
         }
     }
     __syncthreads();
-    //    (c) settle: reached -> 255; undecided -> 127 for k_border / k_resolve under global id
-    //    1 + tile*TS + root; other edge pixels -> 0.  Every reached border pixel maps to the single
-    //    strong node 0 (the smallest id, so after atomicMin linking a set is strong iff its root is 0).
-    bool unres = false;
+    //    (c) settle: reached -> 255; other edge pixels -> 0; undecided -> 127 and listed with their
+    //    tile-local root for k_border / k_resolve, under global id 1 + tile*TS + root.  Every reached
+    //    border pixel maps to the single strong node 0 (the smallest id, so after atomicMin linking a
+    //    set is strong iff its root is 0).
     for (int q = 0; q < TH / NW; q++) {
         const int r = wave + q * NW;
         const int y = ty0 + r, x = tx0 + lane;
@@ -454,15 +454,16 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
             o = 255;
         } else if ((s.Bm[r] >> lane) & 1ull) {
             const int p = r * TP + lane;
+            const int root = lfind(s.L, p);
             o = 127;
-            unres = true;
-            if (lfind(s.L, p) == p) F.parent[1u + (uint32_t)tile * TS + (uint32_t)p] = 1u + (uint32_t)tile * TS + (uint32_t)p;
+            if (root == p) gmem(F.parent)[1u + (uint32_t)tile * TS + (uint32_t)p] = 1u + (uint32_t)tile * TS + (uint32_t)p;
+            const uint32_t i = atomicAdd(&s.unres, 1u);
+            gmem(F.ulist)[(size_t)tile * TS + i] = (uint32_t)p | ((uint32_t)root << 16);
         }
-        F.edges[(long)y * F.edgesStride + x] = o;
+        gmem(F.edges)[(long)y * F.edgesStride + x] = o;
     }
-    if (unres) s.unres = 1;
     // border tables: top row, bottom row, left column, right column of the tile (actual extent)
-    int32_t *bt = F.border + (long)tile * 2 * (TW + TH);
+    gptr_t<int32_t> bt = gmem(F.border) + (long)tile * 2 * (TW + TH);
     for (int n = t; n < 2 * (TW + TH); n += FT) {
         int lx, ly;
         if (n < TW) { lx = n; ly = 0; }
@@ -477,7 +478,7 @@ MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P
         bt[n] = g;
     }
     __syncthreads();
-    if (t == 0) F.tileFlag[tile] = s.unres ? 1 : 0;
+    if (t == 0) gmem(F.ucount)[tile] = s.unres;
 }
 
 __global__ void __launch_bounds__(FT)
@@ -501,11 +502,12 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
     const int tile = (int)blockIdx.x;
     const int tX = tile % P.tilesX, tYi = tile / P.tilesX;
     const int tw = min(TW, P.w - tX * TW), th = min(TH, P.h - tYi * TH);
-    const int32_t *bt = F.border + (long)tile * 2 * (TW + TH);
+    gptr_t<const int32_t> bt = gmem((const int32_t *)F.border) + (long)tile * 2 * (TW + TH);
+    gptr_t<uint32_t> par = gmem(F.parent);
     const int t = (int)threadIdx.x;
     // right neighbour: my right column (rows 0..th-1) against its left column, |dr| <= 1
     if (tX + 1 < P.tilesX) {
-        const int32_t *nb = F.border + (long)(tile + 1) * 2 * (TW + TH);
+        gptr_t<const int32_t> nb = gmem((const int32_t *)F.border) + (long)(tile + 1) * 2 * (TW + TH);
         for (int r = t; r < th; r += 256) {
             const int32_t a = bt[2 * TW + TH + r];
             if (a < 0) continue;
@@ -513,13 +515,13 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
                 const int rr = r + d;
                 if (rr < 0 || rr >= th) continue;
                 const int32_t b = nb[2 * TW + rr];
-                if (b >= 0) gunion(F.parent, (uint32_t)a, (uint32_t)b);
+                if (b >= 0 && b != a) gunion(par, (uint32_t)a, (uint32_t)b);
             }
         }
     }
     // lower neighbour: my bottom row against its top row, |dc| <= 1
     if (tYi + 1 < P.tilesY) {
-        const int32_t *nb = F.border + (long)(tile + P.tilesX) * 2 * (TW + TH);
+        gptr_t<const int32_t> nb = gmem((const int32_t *)F.border) + (long)(tile + P.tilesX) * 2 * (TW + TH);
         for (int c = t; c < tw; c += 256) {
             const int32_t a = bt[TW + c];
             if (a < 0) continue;
@@ -527,7 +529,7 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
                 const int cc = c + d;
                 if (cc < 0 || cc >= tw) continue;   // the tile below has the same width
                 const int32_t b = nb[cc];
-                if (b >= 0) gunion(F.parent, (uint32_t)a, (uint32_t)b);
+                if (b >= 0 && b != a) gunion(par, (uint32_t)a, (uint32_t)b);
             }
         }
         if (t == 0) {
@@ -535,57 +537,36 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
             // bottom-left corner <-> lower-left tile's top-right (a full-width tile)
             if (tX + 1 < P.tilesX) {
                 const int32_t a = bt[TW + tw - 1];
-                const int32_t *nd = F.border + (long)(tile + P.tilesX + 1) * 2 * (TW + TH);
-                if (a >= 0 && nd[0] >= 0) gunion(F.parent, (uint32_t)a, (uint32_t)nd[0]);
+                gptr_t<const int32_t> nd = gmem((const int32_t *)F.border) + (long)(tile + P.tilesX + 1) * 2 * (TW + TH);
+                if (a >= 0 && nd[0] >= 0 && nd[0] != a) gunion(par, (uint32_t)a, (uint32_t)nd[0]);
             }
             if (tX > 0) {
                 const int32_t a = bt[TW + 0];
-                const int32_t *nd = F.border + (long)(tile + P.tilesX - 1) * 2 * (TW + TH);
-                if (a >= 0 && nd[TW - 1] >= 0) gunion(F.parent, (uint32_t)a, (uint32_t)nd[TW - 1]);
+                gptr_t<const int32_t> nd = gmem((const int32_t *)F.border) + (long)(tile + P.tilesX - 1) * 2 * (TW + TH);
+                if (a >= 0 && nd[TW - 1] >= 0 && nd[TW - 1] != a) gunion(par, (uint32_t)a, (uint32_t)nd[TW - 1]);
             }
         }
     }
 }
 
-// tiles holding undecided weak pixels: rebuild their components (the 127 pixels alone form exactly
-// the undecided components, and union-find roots are component minima, hence the same ids) and
-// settle each against its global root
-__global__ void __launch_bounds__(FT)
+// the undecided pixels k_front listed take the state of their component's global root (strong iff
+// the root is node 0); one wave per tile, four tiles per workgroup
+__global__ void __launch_bounds__(256)
 k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
 {
-    __shared__ int sL[TS];
-    __shared__ uint8_t sS[TS];
     const FrameDesc &F = frames[blockIdx.z];
-    const int tile = (int)blockIdx.y * P.tilesX + (int)blockIdx.x;
-    if (!F.tileFlag[tile]) return;
-    const int tx0 = (int)blockIdx.x * TW, ty0 = (int)blockIdx.y * TH;
-    const int tw = min(TW, P.w - tx0), th = min(TH, P.h - ty0);
-    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int r = wave; r < TH; r += NW) {
-        const int p = r * TP + lane;
-        uint8_t st = 0;
-        if (lane < tw && r < th) st = F.edges[(long)(ty0 + r) * F.edgesStride + tx0 + lane] == 127 ? 1 : 0;
-        sS[p] = st;
-        sL[p] = st ? p : -1;
-    }
-    __syncthreads();
-    for (int r = wave; r < TH; r += NW) {
-        const int p = r * TP + lane;
-        if (!sS[p]) continue;
-        if (lane + 1 < TW && sS[p + 1]) lunion(sL, p, p + 1);
-        if (r + 1 < TH) {
-            if (lane > 0 && sS[p + TP - 1]) lunion(sL, p, p + TP - 1);
-            if (sS[p + TP]) lunion(sL, p, p + TP);
-            if (lane + 1 < TW && sS[p + TP + 1]) lunion(sL, p, p + TP + 1);
-        }
-    }
-    __syncthreads();
-    for (int r = wave; r < TH; r += NW) {
-        const int p = r * TP + lane;
-        if (!sS[p]) continue;
-        const int root = lfind(sL, p);
-        const uint32_t g = gfind(F.parent, 1u + (uint32_t)tile * TS + (uint32_t)root);
-        F.edges[(long)(ty0 + r) * F.edgesStride + tx0 + lane] = (g == 0) ? 255 : 0;
+    const int lane = (int)threadIdx.x & 63;
+    const int tile = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    if (tile >= P.tilesX * P.tilesY) return;
+    const uint32_t n = gmem(F.ucount)[tile];
+    if (n == 0) return;
+    const int tx0 = (tile % P.tilesX) * TW, ty0 = (tile / P.tilesX) * TH;
+    gptr_t<const uint32_t> list = gmem((const uint32_t *)F.ulist) + (size_t)tile * TS;
+    for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
+        const uint32_t e = list[i];
+        const uint32_t p = e & 0xffffu, root = e >> 16;
+        const uint32_t g = gfind(gmem(F.parent), 1u + (uint32_t)tile * TS + root);
+        gmem(F.edges)[(long)(ty0 + (int)(p / TP)) * F.edgesStride + tx0 + (int)(p % TP)] = (g == 0) ? 255 : 0;
     }
 }
 
@@ -617,7 +598,7 @@ __global__ void k_fill_bytes(uint32_t *p, uint32_t v, size_t n)
 
 // ---------------------------------------------------------------- host side -----------------------
 struct GraphLayout {
-    size_t descs, frameBytes, border, parent, tileFlag, cand, candCount, pickHdr, pickWs, total;
+    size_t descs, frameBytes, border, parent, ulist, ucount, cand, candCount, pickHdr, pickWs, total;
     uint32_t tiles, candCap;
 };
 
@@ -630,8 +611,9 @@ inline GraphLayout graph_layout(const mvx_vision_params_t &p, uint32_t maxFrames
     L.descs = align256(sizeof(FrameDesc) * (size_t)maxFrames);
     size_t o = 0;
     L.border = o;    o = align256(o + (size_t)L.tiles * 2 * (TW + TH) * 4);
-    L.parent = o;    o = align256(o + (size_t)2 * L.tiles * TS * 4);
-    L.tileFlag = o;  o = align256(o + L.tiles);
+    L.parent = o;    o = align256(o + ((size_t)L.tiles * TS + 1) * 4);
+    L.ulist = o;     o = align256(o + (size_t)L.tiles * TS * 4);
+    L.ucount = o;    o = align256(o + (size_t)L.tiles * 4);
     L.cand = o;      o = align256(o + (size_t)L.candCap * sizeof(XYS));
     L.candCount = o; o = align256(o + 4);
     L.pickHdr = o;   o = align256(o + sizeof(PickWork));
@@ -725,7 +707,8 @@ extern "C" int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_par
         uint8_t *fb = base + (size_t)f * L.frameBytes;
         d.border = (int32_t *)(fb + L.border);
         d.parent = (uint32_t *)(fb + L.parent);
-        d.tileFlag = fb + L.tileFlag;
+        d.ulist = (uint32_t *)(fb + L.ulist);
+        d.ucount = (uint32_t *)(fb + L.ucount);
         d.cand = fb + L.cand;
         d.candCount = (uint32_t *)(fb + L.candCount);
         d.pickWs = fb + L.pickHdr;
@@ -762,7 +745,7 @@ extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_p
         hipLaunchKernelGGL(k_border, dim3(P.tilesX * P.tilesY, 1, nframes), dim3(256), 0, s, D, P);
         break;
     case 2:
-        hipLaunchKernelGGL(k_resolve, dim3(P.tilesX, P.tilesY, nframes), dim3(FT), 0, s, D, P);
+        hipLaunchKernelGGL(k_resolve, dim3((P.tilesX * P.tilesY + 3) / 4, 1, nframes), dim3(256), 0, s, D, P);
         break;
     case 3:
         for (int l = 2; l < P.levels; l++) {
