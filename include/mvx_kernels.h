@@ -268,11 +268,15 @@ size_t mvx_VisionGraph_workspace_size(const mvx_vision_params_t *params, uint32_
  * memory, 16-byte aligned); synchronous with respect to the host. */
 int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_params_t *params, const mvx_vision_frame_t *frames,
                             uint32_t nframes, void *workspace, size_t workspaceSize);
-/* every batch: enqueues the graph for the prepared frames; asynchronous, capturable in a hipGraph. */
+/* every batch: enqueues the graph for the prepared frames; asynchronous, capturable in a hipGraph.
+ * Stages 3 and 4 run on a library-owned stream of the current device (default priority) beside
+ * stages 1 and 2; the caller's stream waits for them before the call's work counts as done. */
 int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes, void *workspace,
                         size_t workspaceSize);
 /* the same, one stage at a time (0 fused tile pass, 1 hysteresis border union, 2 hysteresis resolve,
- * 3 pyramid levels 2.., 4 Harris sort and pick); stages must run in order. */
+ * 3 pyramid levels 2.., 4 Harris sort and pick).  Stage 0 comes first and 2 after 1; stages 3 and 4
+ * need only stage 0 and may run on another stream concurrently with 1 and 2.  The next batch's
+ * stage 0 must follow every stage of the previous batch. */
 #define MVX_VISION_STAGES 5
 int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes,
                               void *workspace, size_t workspaceSize, int stage);

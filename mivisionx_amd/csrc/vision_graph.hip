@@ -33,6 +33,7 @@
 #include "pyramid.hpp"
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace mvx {
@@ -494,55 +495,60 @@ k_front(const FrameDesc *__restrict__ frames, GraphParams P)
     else front_tile<false>(s, F, P, tx0, ty0, tile);
 }
 
-// union the border components of each tile with those of its right, lower and diagonal neighbours
+// union the border components of each tile with those of its right, lower and diagonal neighbours;
+// one wave per tile (lane = row of the right column, then column of the bottom row), four tiles per
+// workgroup
 __global__ void __launch_bounds__(256)
 k_border(const FrameDesc *__restrict__ frames, GraphParams P)
 {
+    static_assert(TW <= 64 && TH <= 64, "one lane per border pixel");
     const FrameDesc &F = frames[blockIdx.z];
-    const int tile = (int)blockIdx.x;
+    const int lane = (int)threadIdx.x & 63;
+    const int tile = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    if (tile >= P.tilesX * P.tilesY) return;
     const int tX = tile % P.tilesX, tYi = tile / P.tilesX;
     const int tw = min(TW, P.w - tX * TW), th = min(TH, P.h - tYi * TH);
-    gptr_t<const int32_t> bt = gmem((const int32_t *)F.border) + (long)tile * 2 * (TW + TH);
+    gptr_t<const int32_t> tables = gmem((const int32_t *)F.border);
+    gptr_t<const int32_t> bt = tables + (long)tile * 2 * (TW + TH);
     gptr_t<uint32_t> par = gmem(F.parent);
-    const int t = (int)threadIdx.x;
     // right neighbour: my right column (rows 0..th-1) against its left column, |dr| <= 1
-    if (tX + 1 < P.tilesX) {
-        gptr_t<const int32_t> nb = gmem((const int32_t *)F.border) + (long)(tile + 1) * 2 * (TW + TH);
-        for (int r = t; r < th; r += 256) {
-            const int32_t a = bt[2 * TW + TH + r];
-            if (a < 0) continue;
+    if (tX + 1 < P.tilesX && lane < th) {
+        gptr_t<const int32_t> nb = tables + (long)(tile + 1) * 2 * (TW + TH);
+        const int32_t a = bt[2 * TW + TH + lane];
+        if (a >= 0) {
             for (int d = -1; d <= 1; d++) {
-                const int rr = r + d;
+                const int rr = lane + d;
                 if (rr < 0 || rr >= th) continue;
                 const int32_t b = nb[2 * TW + rr];
                 if (b >= 0 && b != a) gunion(par, (uint32_t)a, (uint32_t)b);
             }
         }
     }
-    // lower neighbour: my bottom row against its top row, |dc| <= 1
+    // lower neighbour: my bottom row against its top row, |dc| <= 1 (the tile below has the same width)
     if (tYi + 1 < P.tilesY) {
-        gptr_t<const int32_t> nb = gmem((const int32_t *)F.border) + (long)(tile + P.tilesX) * 2 * (TW + TH);
-        for (int c = t; c < tw; c += 256) {
-            const int32_t a = bt[TW + c];
-            if (a < 0) continue;
-            for (int d = -1; d <= 1; d++) {
-                const int cc = c + d;
-                if (cc < 0 || cc >= tw) continue;   // the tile below has the same width
-                const int32_t b = nb[cc];
-                if (b >= 0 && b != a) gunion(par, (uint32_t)a, (uint32_t)b);
+        gptr_t<const int32_t> nb = tables + (long)(tile + P.tilesX) * 2 * (TW + TH);
+        if (lane < tw) {
+            const int32_t a = bt[TW + lane];
+            if (a >= 0) {
+                for (int d = -1; d <= 1; d++) {
+                    const int cc = lane + d;
+                    if (cc < 0 || cc >= tw) continue;
+                    const int32_t b = nb[cc];
+                    if (b >= 0 && b != a) gunion(par, (uint32_t)a, (uint32_t)b);
+                }
             }
         }
-        if (t == 0) {
+        if (lane == 0) {
             // diagonals: bottom-right corner <-> lower-right tile's top-left,
             // bottom-left corner <-> lower-left tile's top-right (a full-width tile)
             if (tX + 1 < P.tilesX) {
                 const int32_t a = bt[TW + tw - 1];
-                gptr_t<const int32_t> nd = gmem((const int32_t *)F.border) + (long)(tile + P.tilesX + 1) * 2 * (TW + TH);
+                gptr_t<const int32_t> nd = tables + (long)(tile + P.tilesX + 1) * 2 * (TW + TH);
                 if (a >= 0 && nd[0] >= 0 && nd[0] != a) gunion(par, (uint32_t)a, (uint32_t)nd[0]);
             }
             if (tX > 0) {
                 const int32_t a = bt[TW + 0];
-                gptr_t<const int32_t> nd = gmem((const int32_t *)F.border) + (long)(tile + P.tilesX - 1) * 2 * (TW + TH);
+                gptr_t<const int32_t> nd = tables + (long)(tile + P.tilesX - 1) * 2 * (TW + TH);
                 if (a >= 0 && nd[TW - 1] >= 0 && nd[TW - 1] != a) gunion(par, (uint32_t)a, (uint32_t)nd[TW - 1]);
             }
         }
@@ -742,7 +748,7 @@ extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_p
         hipLaunchKernelGGL(k_front, dim3(P.tilesX, P.tilesY, nframes), dim3(FT), 0, s, D, P);
         break;
     case 1:
-        hipLaunchKernelGGL(k_border, dim3(P.tilesX * P.tilesY, 1, nframes), dim3(256), 0, s, D, P);
+        hipLaunchKernelGGL(k_border, dim3((P.tilesX * P.tilesY + 3) / 4, 1, nframes), dim3(256), 0, s, D, P);
         break;
     case 2:
         hipLaunchKernelGGL(k_resolve, dim3((P.tilesX * P.tilesY + 3) / 4, 1, nframes), dim3(256), 0, s, D, P);
@@ -763,12 +769,60 @@ extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_p
     return MVX_SUCCESS;
 }
 
+namespace mvx {
+
+// The stream the pyramid levels 2.. and the Harris pick run on beside the hysteresis stages, with the
+// fork and join events: one set per device, default priority, created on first use (outside any
+// capture, normally) and kept until the process ends.  The mutex covers the enqueue sequence, so
+// calls from several host threads reuse the events safely.
+struct SideLane {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool ok = false, tried = false;
+};
+static std::mutex g_sideMu;
+static SideLane g_side[64];
+
+static SideLane *side_lane()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    SideLane &L = g_side[dev];
+    if (!L.tried) {
+        L.tried = true;
+        L.ok = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
+               hipEventCreateWithFlags(&L.fork, hipEventDisableTiming) == hipSuccess &&
+               hipEventCreateWithFlags(&L.join, hipEventDisableTiming) == hipSuccess;
+    }
+    return L.ok ? &L : nullptr;
+}
+
+}  // namespace mvx
+
+// Stage 0 on the caller's stream; then stages 3 and 4 (which need only stage 0) on the side stream
+// while stages 1 and 2 run on the caller's stream; the caller's stream waits for the side stream at
+// the end, so the call orders like a single-stream sequence.  Fork and join are event waits, which a
+// hipGraph capture of the caller's stream records as dependencies.
 extern "C" int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *p, uint32_t nframes, void *workspace,
                                    size_t workspaceSize)
 {
-    for (int st = 0; st < MVX_VISION_STAGES; st++) {
-        const int rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, st);
-        if (rc) return rc;
+    int rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 0);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_sideMu);
+    SideLane *L = side_lane();
+    if (!L) {
+        for (int st = 1; st < MVX_VISION_STAGES && !rc; st++)
+            rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, st);
+        return rc;
     }
-    return MVX_SUCCESS;
+    hipStream_t main = stream_of(stream);
+    const mvx_stream_t side = reinterpret_cast<mvx_stream_t>(L->stream);
+    if (hipEventRecord(L->fork, main) != hipSuccess || hipStreamWaitEvent(L->stream, L->fork, 0) != hipSuccess)
+        return MVX_FAILURE;
+    if ((rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 3))) return rc;
+    if ((rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 4))) return rc;
+    if (hipEventRecord(L->join, L->stream) != hipSuccess) return MVX_FAILURE;
+    if ((rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 1))) return rc;
+    if ((rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 2))) return rc;
+    return hipStreamWaitEvent(main, L->join, 0) == hipSuccess ? MVX_SUCCESS : MVX_FAILURE;
 }
