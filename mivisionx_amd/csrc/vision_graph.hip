@@ -9,18 +9,20 @@
 //                                         Harris 3x3/3x3 (HG3 -> score -> NMS -> sort and pick) }
 //
 // Launches per batch of frames (blockIdx.z = frame):
-//   k_front     one 58x32 luma tile (+3 halo) per workgroup of four waves.  Every window row of the
-//               tile (64 luma, 62 gradient, 60 score columns) fits one 64-lane wave, so lane = column
-//               and a wave walks rows: RGB -> Y in LDS, then level 0, the 2:1 scaled image, level 1,
-//               Canny gradients/NMS/threshold with tile-local hysteresis (row-mask dilation; union-find
-//               only over the weak components that touch the tile border), Harris products / exact
-//               integer window sums / score / NMS with candidate append.  Only graph outputs and small
+//   k_front     one 58x32 luma tile (+3 halo) per wave, four independent waves per workgroup.  Every
+//               window row of the tile (64 luma, 62 gradient, 60 score columns) fits the 64 lanes, so
+//               lane = column and the wave streams the tile's rows top to bottom with every quantity in
+//               rolling register windows (column neighbours through DPP lane shifts): RGB -> Y, level 0,
+//               the 2:1 scaled image, level 1, Canny gradients/NMS/threshold, Harris products / exact
+//               integer window sums / score / NMS; then tile-local hysteresis on 64-bit row masks
+//               (lane = row) with the border-touching weak components labelled by flood fill.  No
+//               barriers; LDS holds only the wave's Harris candidates.  Only graph outputs and small
 //               per-tile tables reach HBM.  Tiles away from the image border take a branch-free
 //               instantiation.
 //   k_border    unions the hysteresis components that touch tile borders, across tiles.  Strong
 //               components get ids below every weak one, so after atomicMin linking a set's root is
 //               strong iff the set holds a strong component -- no marking pass.
-//   k_resolve   the undecided weak pixels k_front listed per tile take their global root's state.
+//   k_resolve   the undecided weak pixels of each tile take their component's global root's state.
 //   k_half      pyramid levels 2.. (one launch per level, all frames).
 //   k_pick      Harris sort and pick, one workgroup per frame (pick.hpp).
 //
@@ -57,7 +59,7 @@ struct FrameDesc {
     // per-frame workspace
     int32_t *border;      // tiles x 2(TW+TH) component ids of tile-border pixels (-1: none)
     uint32_t *parent;     // 1 + tiles * TS union-find slots (sparsely used)
-    uint32_t *ulist;      // tiles x TS: a tile's undecided pixels, p | tile-local root << 16
+    uint64_t *umask;      // tiles x TH: a tile's undecided pixels, one 64-bit row mask per tile row
     uint32_t *ucount;     // tiles: undecided pixels listed
     void *cand;           // Harris XYS candidates
     uint32_t *candCount;  // candidates appended (reset by k_pick)
@@ -163,336 +165,302 @@ MVX_DEV void gunion(gptr_t<uint32_t> par, uint32_t a, uint32_t b)
     }
 }
 
-// Arrays whose lifetimes do not overlap share LDS (every hand-over is separated by a barrier):
-// the gradients (steps 2-3) with the row masks of the Canny states (steps 4-6), the Harris score
-// (steps 3-4 and the candidate append) with the union-find labels (step 6).  Under 25 KB per
-// workgroup: 6 resident workgroups per CU.
-struct FrontShared {
-    uint8_t Y[YH][TP];
-    union {
-        struct { int16_t Gx[GH][TP], Gy[GH][TP]; };
-        // per row (bit = column): edge (127/255), strong (255), reached from a strong pixel, and
-        // weak pixels of the components that touch the tile border but hold no strong pixel
-        struct { uint64_t Em[TH], Sm[TH], Rm[TH], Bm[TH]; };
-    };
-    uint16_t M[MH][TP];
-    union {
-        float V[MH][TP];
-        int L[TS];                             // union-find labels of the weak pixels no strong pixel reaches
-    };
-    uint32_t cnt, base, unres;   // candidates, their global offset, undecided pixels listed
-};
-
-// INNER: the whole luma window lies inside the image and every Canny / Harris position of the tile
-// window is an interior pixel, so no image-border rule applies.
-template <bool INNER>
-MVX_DEV void front_tile(FrontShared &s, const FrameDesc &F, const GraphParams &P, int tx0, int ty0, int tile)
+// the 64-bit value of lane l (l wave-uniform)
+MVX_DEV uint64_t readlane64(uint64_t v, int l)
 {
-    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+MVX_DEV float lane_next_f(float v) { return __int_as_float(lane_next(__float_as_int(v))); }
+
+// Row masks, lane = row (rows 0..31 of one problem in lanes 0..31, of a second in lanes 32..63):
+// grow m inside e by 8-connectivity until nothing changes.
+MVX_DEV uint64_t grow_rows(uint64_t m, uint64_t e, int lane)
+{
+    const int r = lane & 31;
+    while (true) {
+        uint64_t up = __shfl(m, lane > 0 ? lane - 1 : 0), dn = __shfl(m, lane < 63 ? lane + 1 : 63);
+        if (r == 0) up = 0ull;
+        if (r == 31) dn = 0ull;
+        uint64_t v = m | up | dn;
+        v |= (v << 1) | (v >> 1);
+        const uint64_t n = (v & e) | m;
+        const bool changed = n != m;
+        m = n;
+        if (!__any(changed)) break;
+    }
+    return m;
+}
+
+constexpr int FTW = 4;                                     // tiles (one wave each) per workgroup of k_front
+constexpr int CAND_MAX = ((TW + 1) / 2) * ((TH + 1) / 2);  // Harris NMS maxima of a tile are never 8-adjacent
+typedef uint32_t u32_any_align __attribute__((aligned(1)));
+
+// One wave streams one tile: luma window rows k = 0..YH-1 (image row ty0-3+k, lane = column tx0-3+lane)
+// top to bottom, every later quantity from rolling register windows of the rows it needs, column
+// neighbours by DPP lane shifts.  At step k:
+//   luma row k -> pyramid level 0 (row k-3), the 2:1 scaled image and level 1 when their rows complete;
+//   gradient row g = k-2 (image row ty0-2+g, column tx0-2+lane) -> HG3 products and 3-column sums;
+//   Canny magnitude|sector row k-3 and Harris score row k-4 (image row ty0-1+m, column tx0-1+lane);
+//   NMS of tile row r = k-6 (image row ty0+r, column tx0+lane): Canny double threshold into 64-bit row
+//   masks (edge, strong; lane r holds row r), Harris maxima into the per-wave candidate buffer.
+// Then the tile's hysteresis on the row masks and the settle pass.
+template <bool INNER>
+MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P, int tx0, int ty0, int tile)
+{
+    const int lane = (int)threadIdx.x & 63;
     const int w = P.w, h = P.h;
     const int tw = INNER ? TW : min(TW, w - tx0), th = INNER ? TH : min(TH, h - ty0);
+    const int x = tx0 - 1 + lane;       // Canny magnitude / Harris score column
+    const int ox = tx0 + lane;          // tile output column
 
-    // 1. luma window: lane = column (tx0-3+lane), linear-address semantics of the virtual Y plane.
-    //    Inner tiles issue all of a lane's RGB loads before converting, so their latencies overlap.
-    if (INNER) {
-        constexpr int RPW = (YH + NW - 1) / NW;   // rows per wave
-        uint8_t px[RPW][3];
-        gptr_t<const uint8_t> col = gmem(F.rgb) + (long)(ty0 - HY + wave) * F.rgbStride + 3L * (tx0 - HY + lane);
-#pragma unroll
-        for (int k = 0; k < RPW; k++) {
-            if (wave + k * NW < YH) {
-                gptr_t<const uint8_t> p = col + (long)(k * NW) * F.rgbStride;
-                px[k][0] = p[0]; px[k][1] = p[1]; px[k][2] = p[2];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < RPW; k++)
-            if (wave + k * NW < YH) s.Y[wave + k * NW][lane] = (uint8_t)luma(px[k]);
-    } else {
-        for (int r = wave; r < YH; r += NW) {
-            const int y = ty0 - HY + r, x = tx0 - HY + lane;
-            s.Y[r][lane] = (uint8_t)yflat(F, P, y, x);
-        }
-    }
-    __syncthreads();
+    // inner tiles read a pixel's R, G, B (and the next pixel's R) with one dword load; the window's
+    // last column is at most w-2, so the dword stays inside the row
+    gptr_t<const uint8_t> rgbCol = gmem(F.rgb) + (long)(ty0 - HY) * F.rgbStride + 3L * (tx0 - HY + lane);
+    auto fetch = [&](int k) -> uint32_t { return *(gptr_t<const u32_any_align>)(rgbCol + (long)k * F.rgbStride); };
+    gptr_t<uint8_t> l0 = gmem(F.lvl[0]) + (long)ty0 * F.lvlStride[0] + tx0 - HY + lane;
+    const int body = P.dw & ~7;
+    const int oxp = (w & 1) ? 0 : 1, oyp = (h & 1) ? 0 : 1;
 
-    // 2. Sobel gradients (linear-address reads) at (ty0-2+r, tx0-2+lane)
-    if (lane < GW) {
-        for (int r = wave; r < GH; r += NW) {
-            const int a0 = s.Y[r][lane], a1 = s.Y[r][lane + 1], a2 = s.Y[r][lane + 2];
-            const int b0 = s.Y[r + 1][lane], b2 = s.Y[r + 1][lane + 2];
-            const int c0 = s.Y[r + 2][lane], c1 = s.Y[r + 2][lane + 1], c2 = s.Y[r + 2][lane + 2];
-            s.Gx[r][lane] = (int16_t)((c2 - c0) + (a2 - a0) + ((b2 - b0) << 1));
-            s.Gy[r][lane] = (int16_t)((c0 + (c1 << 1) + c2) - (a0 + (a1 << 1) + a2));
-        }
-    }
-    __syncthreads();
+    int y0 = 0, y1 = 0, y2 = 0, y3 = 0;                   // luma rows k-4..k-1
+    int dA = 0, dB = 0, sA = 0, sB = 0;                   // row k-2, k-1: Y[l+2]-Y[l], Y[l]+2Y[l+1]+Y[l+2]
+    int hxx0 = 0, hxy0 = 0, hyy0 = 0, hxx1 = 0, hxy1 = 0, hyy1 = 0;   // HG3 3-column sums, gradient rows g-2, g-1
+    uint32_t Ma = 0, Mb = 0, Mc = 0;                      // Canny magnitude|sector rows k-6, k-5, k-4
+    float Va = 0.0f, Vb = 0.0f;                           // Harris score rows k-6, k-5
+    uint32_t emLo = 0, emHi = 0, smLo = 0, smHi = 0;      // lane r: edge / strong mask of tile row r
+    uint32_t nc = 0;                                      // candidates in the wave's buffer
+    uint32_t pf0 = 0, pf1 = 0;
+    if (INNER) { pf0 = fetch(0); pf1 = fetch(1); }
 
-    // 3. at (ty0-1+r, tx0-1+lane): Canny U16 magnitude|sector; Harris score from a per-column
-    //    sliding window of 3-column sums of the HG3 products {gx^2, gx gy, gy^2}.  HG3 rows 0 and h-1
-    //    are never written (zero).  The products are integers whose 9-sums stay below 2^24, so int32
-    //    sums equal the reference's float sums exactly.
-    //    Each lane forms the products of its own gradient column; the 3-column sums (columns
-    //    lane..lane+2) take the right neighbours' through DPP wave shifts, so this part runs with all
-    //    64 lanes active (lanes >= MW compute values nobody stores).
-    {
-        const int x = tx0 - 1 + lane;
-        const int r0 = wave * ((MH + NW - 1) / NW), r1 = min(r0 + (MH + NW - 1) / NW, MH);
-        int hxx[3], hxy[3], hyy[3];
-        auto hsum = [&](int gr, int &sxx, int &sxy, int &syy) {
-            int pxx = 0, pxy = 0, pyy = 0;
-            if (INNER || (ty0 - 2 + gr >= 1 && ty0 - 2 + gr <= h - 2)) {
-                const int gx = s.Gx[gr][lane], gy = s.Gy[gr][lane];
-                pxx = gx * gx; pxy = gx * gy; pyy = gy * gy;
-            }
-            sxx = pxx + lane_next(pxx + lane_next(pxx));
-            sxy = pxy + lane_next(pxy + lane_next(pxy));
-            syy = pyy + lane_next(pyy + lane_next(pyy));
-        };
-        hsum(r0, hxx[0], hxy[0], hyy[0]);
-        hsum(r0 + 1, hxx[1], hxy[1], hyy[1]);
-        for (int r = r0; r < r1; r++) {
-            const int y = ty0 - 1 + r;
-            hsum(r + 2, hxx[2], hxy[2], hyy[2]);
-            if (lane < MW) {
-                // Canny
-                uint16_t m = 0;
-                if (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w)) {
-                    m = canny_pack_f(s.Gx[r + 1][lane + 1], -(int)s.Gy[r + 1][lane + 1]);
-                } else if (y >= 1 && y <= h - 2 && x == 0) {
-                    // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
-                    // r1[-1] = r0[ALIGN16(w)-1]); luma window column 3 is x = 0, rows r+1..r+3 are y-1..y+1
-                    const int t0_1 = (int)s.Y[r + 1][4] + (int)s.Y[r + 3][4] + 2 * (int)s.Y[r + 2][4];
-                    const int t1_0 = (int)s.Y[r + 3][3] - (int)s.Y[r + 1][3];
-                    const int t1_1 = (int)s.Y[r + 3][4] - (int)s.Y[r + 1][4];
-                    const int t1_m1 = (w % 16 == 0) ? ((int)yflat(F, P, y - 1, w - 1) + (int)yflat(F, P, y + 1, w - 1) +
-                                                       2 * (int)yflat(F, P, y, w - 1)) : 0;
-                    m = canny_pack_f(t0_1, -(t1_m1 + 2 * t1_0 + t1_1));
+    for (int k = 0; k < YH; k++) {
+        // ---- luma row k (ColorConvert Y of the RGB pixel, or the virtual plane's linear-address value)
+        int yk;
+        if (INNER) {
+            const uint32_t cur = pf0;
+            pf0 = pf1;
+            if (k + 2 < YH) pf1 = fetch(k + 2);
+            float f = (float)(cur & 0xffu) * 0.2126f;
+            f = f + (float)((cur >> 8) & 0xffu) * 0.7152f;
+            f = f + (float)((cur >> 16) & 0xffu) * 0.0722f;
+            yk = min(max((int)f, 0), 255);
+        } else {
+            yk = (int)yflat(F, P, ty0 - HY + k, tx0 - HY + lane);
+        }
+        if (k >= HY && k < HY + th && lane >= HY && lane < HY + tw) l0[(long)(k - HY) * F.lvlStride[0]] = (uint8_t)yk;
+        const int n1 = lane_next(yk), n2 = lane_next(n1);
+        const int dk = n2 - yk, sk = yk + 2 * n1 + n2;
+
+        // ---- ScaleImage 2:1 (ago_haf_cpu_geometric.cpp:2000-2045 with xscale = yscale = 2, offsets 0.5):
+        //      output row oy from luma rows 2oy+3 (k-1) and 2oy+4 (k); lane 2X'+3 holds output column X'
+        if (k >= 4 && !(k & 1) && k <= HY + TH - 1) {
+            const int oy = (k - 4) >> 1, Yo = ty0 / 2 + oy;
+            const int lp = lane - HY;
+            const int Xo = tx0 / 2 + (lp >> 1);
+            const int p00 = y3, p01 = lane_next(y3), p10 = yk, p11 = n1;
+            if (Yo < P.dh && lp >= 0 && !(lp & 1) && (lp >> 1) < TW / 2 && Xo < P.dw) {
+                int v;
+                if (Xo < body) {
+                    const int t0 = (p00 + p01 + 1) >> 1, t1 = (p10 + p11 + 1) >> 1;   // (128 a + 128 b + 0x80) >> 8
+                    v = (t0 + t1 + 1) >> 1;
+                } else {
+                    v = (p00 + p01 + p10 + p11 + 2) >> 2;                           // (128*128 sum + 0x8000) >> 16
                 }
-                s.M[r][lane] = m;
-                // Harris
-                float v = 0.0f;
-                if (INNER || (x >= 1 && x <= w - 2 && y >= 1 && y <= h - 2)) {
-                    const float gx2 = (float)(hxx[0] + hxx[1] + hxx[2]);
-                    const float gxy2 = (float)(hxy[0] + hxy[1] + hxy[2]);
-                    const float gy2 = (float)(hyy[0] + hyy[1] + hyy[2]);
+                gmem(F.scaled)[(long)Yo * F.scaledStride + Xo] = (uint8_t)v;
+            }
+        }
+        // ---- ScaleGaussianHalf_5x5 level 1 (ago_haf_cpu_pyramid.cpp:94-305): centre (2X+oxp, 2Y+oyp),
+        //      luma rows k-4..k; lane 2X'+oxp+1 sums columns lane..lane+4 (centre lane+2)
+        if (P.levels > 1 && k >= 5 + oyp && !((k - 5 - oyp) & 1)) {
+            const int oy = (k - 5 - oyp) >> 1, Yo = ty0 / 2 + oy;
+            if (oy < TH / 2 && Yo >= 1 && Yo <= P.lvlH[1] - 2) {
+                const uint32_t vc = (uint32_t)(y0 + yk) + 4u * (uint32_t)(y1 + y3) + 6u * (uint32_t)y2;
+                const uint32_t v1 = (uint32_t)lane_next((int)vc), v2 = (uint32_t)lane_next((int)v1);
+                const uint32_t v3 = (uint32_t)lane_next((int)v2), v4 = (uint32_t)lane_next((int)v3);
+                const uint32_t acc = vc + 4u * v1 + 6u * v2 + 4u * v3 + v4;
+                const int lp = lane - oxp - 1;
+                const int Xo = tx0 / 2 + (lp >> 1);
+                if (lp >= 0 && !(lp & 1) && (lp >> 1) < TW / 2 && Xo < P.lvlW[1])
+                    gmem(F.lvl[1])[(long)Yo * F.lvlStride[1] + Xo] = (uint8_t)((acc & 0xffffu) >> 8);
+            }
+        }
+
+        if (k >= 2) {
+            // ---- Sobel gradient row g = k-2 (linear-address luma), HG3 products and 3-column sums;
+            //      HG3 rows 0 and h-1 are never written (zero).  The products are integers whose 9-sums
+            //      stay below 2^24, so int32 sums equal the reference's float sums exactly.
+            const int g = k - 2;
+            const int gx = (int)(int16_t)(dA + 2 * dB + dk), gy = (int)(int16_t)(sk - sA);
+            int pxx = 0, pxy = 0, pyy = 0;
+            if (INNER || (ty0 - 2 + g >= 1 && ty0 - 2 + g <= h - 2)) { pxx = gx * gx; pxy = gx * gy; pyy = gy * gy; }
+            const int hxx2 = pxx + lane_next(pxx + lane_next(pxx));
+            const int hxy2 = pxy + lane_next(pxy + lane_next(pxy));
+            const int hyy2 = pyy + lane_next(pyy + lane_next(pyy));
+
+            // ---- Canny magnitude|sector row m = k-3 from gradient row g at column lane+1
+            uint32_t Md = 0;
+            if (k >= 3) {
+                const int m = k - 3, y = ty0 - 1 + m;
+                const int gxn = lane_next(gx), gyn = lane_next(gy);
+                if (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w)) {
+                    Md = canny_pack_f(gxn, -gyn);
+                } else if (y >= 1 && y <= h - 2 && tx0 == 0) {
+                    // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
+                    // r1[-1] = r0[ALIGN16(w)-1]); luma lanes 3, 4 are x = 0, 1 and rows k-2..k are y-1..y+1
+                    const int a3 = __builtin_amdgcn_readlane(y2, 3), a4 = __builtin_amdgcn_readlane(y2, 4);
+                    const int b4 = __builtin_amdgcn_readlane(y3, 4);
+                    const int c3 = __builtin_amdgcn_readlane(yk, 3), c4 = __builtin_amdgcn_readlane(yk, 4);
+                    if (x == 0) {
+                        const int t0_1 = a4 + c4 + 2 * b4;
+                        const int t1_0 = c3 - a3, t1_1 = c4 - a4;
+                        const int t1_m1 = (w % 16 == 0) ? ((int)yflat(F, P, y - 1, w - 1) + (int)yflat(F, P, y + 1, w - 1) +
+                                                           2 * (int)yflat(F, P, y, w - 1)) : 0;
+                        Md = canny_pack_f(t0_1, -(t1_m1 + 2 * t1_0 + t1_1));
+                    }
+                }
+            }
+            // ---- Harris score row k-4: Mc = mc / norm > T ? mc / norm : 0, the quotient exactly the IEEE one
+            float Vc = 0.0f;
+            if (k >= 4) {
+                const int yv = ty0 - 1 + (k - 4);
+                if (INNER || (x >= 1 && x <= w - 2 && yv >= 1 && yv <= h - 2)) {
+                    const float gx2 = (float)(hxx0 + hxx1 + hxx2);
+                    const float gxy2 = (float)(hxy0 + hxy1 + hxy2);
+                    const float gy2 = (float)(hyy0 + hyy1 + hyy2);
                     const float tr = gx2 + gy2;
                     const float det = (gx2 * gy2) - (gxy2 * gxy2);
                     const float mc = det - (P.k * tr * tr);
-                    // Vc = mc / norm > T ? mc / norm : 0, the quotient exactly the IEEE one
                     const float q = div_by_const_rn(mc, P.norm, P.normRcp);
-                    v = (q > P.thr) ? q : 0.0f;
+                    Vc = (q > P.thr) ? q : 0.0f;
                 }
-                s.V[r][lane] = v;
             }
-            hxx[0] = hxx[1]; hxy[0] = hxy[1]; hyy[0] = hyy[1];
-            hxx[1] = hxx[2]; hxy[1] = hxy[2]; hyy[1] = hyy[2];
+            // ---- NMS of tile row r = k-6 (image row ty0+r): magnitude rows r..r+2 = Ma, Mb, Mc and
+            //      score rows r..r+2 = Va, Vb, Vc, columns lane..lane+2, centre lane+1
+            if (k >= 6) {
+                const int r = k - 6, y = ty0 + r;
+                const bool valid = lane < TW && (INNER || (y >= 1 && y <= h - 2 && ox >= 1 && ox <= w - 2));
+                const uint32_t a1 = (uint32_t)lane_next((int)Ma), a2 = (uint32_t)lane_next((int)a1);
+                const uint32_t b1 = (uint32_t)lane_next((int)Mb), b2 = (uint32_t)lane_next((int)b1);
+                const uint32_t c1 = (uint32_t)lane_next((int)Mc), c2 = (uint32_t)lane_next((int)c1);
+                const uint32_t sec = b1 & 3u;
+                const uint32_t n0 = sec == 0 ? Mb : sec == 1 ? a2 : sec == 2 ? a1 : Ma;
+                const uint32_t nn = sec == 0 ? b2 : sec == 1 ? Mc : sec == 2 ? c1 : c2;
+                const int mag = (int)(b1 >> 2);
+                const int edge = (mag > (int)(n0 >> 2) && mag > (int)(nn >> 2)) ? mag : 0;
+                const uint32_t st = !valid ? 0u : edge > (int)P.upper ? 255u : (edge <= (int)P.lower ? 0u : 127u);
+                const uint64_t eb = __ballot(st != 0u), sb = __ballot(st == 255u);
+                if (lane == r) { emLo = (uint32_t)eb; emHi = (uint32_t)(eb >> 32); smLo = (uint32_t)sb; smHi = (uint32_t)(sb >> 32); }
+                // Harris 3x3 NMS: >= towards the upper-left half of the window, > towards the lower-right
+                const float c = lane_next_f(Vb), vr = lane_next_f(c);
+                const float va1 = lane_next_f(Va), va2 = lane_next_f(va1);
+                const float vc1 = lane_next_f(Vc), vc2 = lane_next_f(vc1);
+                const float upl = fmaxf(fmaxf(Va, va1), fmaxf(va2, Vb));
+                const float lwr = fmaxf(fmaxf(vr, Vc), fmaxf(vc1, vc2));
+                const bool isMax = valid && c >= upl && c > lwr;
+                const uint64_t cb = __ballot(isMax);
+                if (isMax) {
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cb, 0u));
+                    cand[nc + below] = (uint64_t)(uint16_t)ox | ((uint64_t)(uint16_t)y << 16) | ((uint64_t)__float_as_uint(c) << 32);
+                }
+                nc += (uint32_t)__popcll(cb);
+            }
+            Ma = Mb; Mb = Mc; Mc = Md;
+            Va = Vb; Vb = Vc;
+            hxx0 = hxx1; hxy0 = hxy1; hyy0 = hyy1;
+            hxx1 = hxx2; hxy1 = hxy2; hyy1 = hyy2;
         }
+        y0 = y1; y1 = y2; y2 = y3; y3 = yk;
+        dA = dB; dB = dk; sA = sB; sB = sk;
     }
-    __syncthreads();
 
-    // 4. tile pixels (ty0+r, tx0+lane): Canny NMS + double threshold, Harris NMS -> candidates
-    uint32_t myCand = 0;                 // bit q: row wave + q*NW is a candidate
-    for (int q = 0; q < TH / NW; q++) {
-        const int r = wave + q * NW;
-        uint8_t st = 0;
-        bool cand = false;
-        const int y = ty0 + r, x = tx0 + lane;
-        if (lane < TW && (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2))) {
-            const int j = r + 1, i = lane + 1;
-            const uint32_t mv = s.M[j][i];
-            const int m = (int)(mv >> 2), a = (int)(mv & 3u);
-            const int dx0 = (a == 0) ? -1 : (a == 1) ? 1 : (a == 2) ? 0 : -1;
-            const int dy0 = (a == 0) ? 0 : -1;
-            const int n0 = (int)(s.M[j + dy0][i + dx0] >> 2), n1 = (int)(s.M[j - dy0][i - dx0] >> 2);
-            const int edge = (m > n0 && m > n1) ? m : 0;
-            st = edge > (int)P.upper ? 255 : (edge <= (int)P.lower ? 0 : 127);
-            const float c = s.V[j][i];
-            cand = c >= s.V[j - 1][i - 1] && c >= s.V[j - 1][i] && c >= s.V[j - 1][i + 1] && c >= s.V[j][i - 1] &&
-                   c > s.V[j][i + 1] && c > s.V[j + 1][i - 1] && c > s.V[j + 1][i] && c > s.V[j + 1][i + 1];
-        }
-        if (cand) myCand |= 1u << q;
-        // row masks of edge (127/255) and strong (255) pixels: bit = column
-        const uint64_t em = __ballot(st != 0), sm = __ballot(st == 255);
-        if (lane == 0) { s.Em[r] = em; s.Sm[r] = sm; }
-    }
-    // candidate append: one LDS atomic per thread with candidates, one global atomic per workgroup
-    const uint32_t nCand = (uint32_t)__popc(myCand);
-    uint32_t myOff = 0;
-    if (nCand) myOff = atomicAdd(&s.cnt, nCand);
-    __syncthreads();
-    if (t == 0 && s.cnt) s.base = __hip_atomic_fetch_add(gmem(F.candCount), s.cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (nCand) {
-        // one XYS record {int16 x, int16 y, float s} as a single 64-bit store
-        static_assert(sizeof(XYS) == 8, "XYS layout");
+    // ---- Harris candidates: one global atomic per tile, then the buffer copied out
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (nc) {
+        uint32_t base = 0;
+        if (lane == 0) base = __hip_atomic_fetch_add(gmem(F.candCount), nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
         gptr_t<uint64_t> list = gmem((uint64_t *)F.cand);
-        uint32_t pos = s.base + myOff;
-        for (int q = 0; q < TH / NW; q++) {
-            if (!(myCand & (1u << q))) continue;
-            const int r = wave + q * NW;
-            if (pos < P.candCap) {
-                list[pos] = (uint64_t)(uint16_t)(tx0 + lane) | ((uint64_t)(uint16_t)(ty0 + r) << 16) |
-                            ((uint64_t)__float_as_uint(s.V[r + 1][lane + 1]) << 32);
-            }
-            pos++;
-        }
+        for (uint32_t i = (uint32_t)lane; i < nc; i += 64)
+            if (base + i < P.candCap) list[base + i] = cand[i];
     }
 
-    // 5. outputs that only need luma: level 0, the 2:1 scaled image, level 1
-    if (lane < tw) {
-        gptr_t<uint8_t> l0 = gmem(F.lvl[0]) + (long)ty0 * F.lvlStride[0] + tx0 + lane;
-        for (int r = wave; r < th; r += NW) l0[(long)r * F.lvlStride[0]] = s.Y[r + HY][lane + HY];
-    }
-    if (lane < TW / 2) {
-        // ScaleImage 2:1: xm = 2X, ym = 2Y, xf = yf = 128 (ago_haf_cpu_geometric.cpp:2000-2045 with
-        // xscale = yscale = 2, offsets 0.5): two-stage rounded averages; tail columns single rounding
-        const int X = tx0 / 2 + lane;
-        const int body = P.dw & ~7;
-        if (X < P.dw) {
-            for (int oy = wave; oy < TH / 2; oy += NW) {
-                const int Yo = ty0 / 2 + oy;
-                if (Yo >= P.dh) break;
-                const int sy = 2 * oy + HY, sx = 2 * lane + HY;
-                const int p00 = s.Y[sy][sx], p01 = s.Y[sy][sx + 1], p10 = s.Y[sy + 1][sx], p11 = s.Y[sy + 1][sx + 1];
-                int v;
-                if (X < body) {
-                    const int t0 = (128 * p00 + 128 * p01 + 0x80) >> 8, t1 = (128 * p10 + 128 * p11 + 0x80) >> 8;
-                    v = (128 * t0 + 128 * t1 + 0x80) >> 8;
-                } else {
-                    v = (128 * 128 * (p00 + p01 + p10 + p11) + 0x8000) >> 16;
-                }
-                gmem(F.scaled)[(long)Yo * F.scaledStride + X] = (uint8_t)v;
-            }
-        }
-        // ScaleGaussianHalf_5x5 level 1 (ago_haf_cpu_pyramid.cpp:94-305): centre (2X+ox, 2Y+oy)
-        if (P.levels > 1 && X < P.lvlW[1]) {
-            const int oxp = (w & 1) ? 0 : 1, oyp = (h & 1) ? 0 : 1;
-            const int l1h = P.lvlH[1];
-            for (int oy = wave; oy < TH / 2; oy += NW) {
-                const int Yo = ty0 / 2 + oy;
-                if (Yo > l1h - 2) break;
-                if (Yo < 1) continue;
-                const int cy = 2 * oy + oyp + HY, cx = 2 * lane + oxp + HY;
-                uint32_t acc = 0;
-#pragma unroll
-                for (int jj = 0; jj < 5; jj++) {
-                    const uint8_t *rr = &s.Y[cy - 2 + jj][cx - 2];
-                    const uint32_t hr = rr[0] + 4u * rr[1] + 6u * rr[2] + 4u * rr[3] + rr[4];
-                    acc += (jj == 0 || jj == 4 ? 1u : jj == 2 ? 6u : 4u) * hr;
-                }
-                gmem(F.lvl[1])[(long)Yo * F.lvlStride[1] + X] = (uint8_t)((acc & 0xffffu) >> 8);
-            }
-        }
-    }
-
-    // 6. tile-local hysteresis.  (a) One wave, lane = row, 64-bit row masks, dilate-and-mask until
-    //    nothing changes, two problems side by side: lanes 0..31 grow the edge pixels that a strong
-    //    pixel reaches inside the tile (the union of the 8-connected edge components that hold a
-    //    strong pixel), lanes 32..63 grow those that a tile-border edge pixel reaches (the components
-    //    that touch the tile border).  Edge pixels in neither are weak components inside the tile:
-    //    they settle to 0 here.  Border-touching components without a strong pixel are the only ones
-    //    left undecided.
+    // ---- hysteresis inside the tile.  (a) Lanes 0..31 grow the edge pixels that a strong pixel reaches
+    //      (the union of the 8-connected edge components holding a strong pixel), lanes 32..63 those a
+    //      tile-border edge pixel reaches.  Edge pixels in neither are weak components inside the tile
+    //      (-> 0); border-touching components without a strong pixel are undecided.
     static_assert(TH <= 32 && TW <= 63, "row masks: one lane per row in each half-wave");
-    __syncthreads();
-    if (wave == 0) {
-        const int r = lane & 31;
-        const uint64_t e = r < TH ? s.Em[r] : 0ull;
-        uint64_t m;
-        if (lane < 32) m = r < TH ? s.Sm[r] : 0ull;
-        else m = (r == 0 || r == th - 1) ? e : (e & (1ull | (1ull << (tw - 1))));
-        while (true) {
-            uint64_t up = __shfl(m, lane > 0 ? lane - 1 : 0), dn = __shfl(m, lane < 63 ? lane + 1 : 63);
-            if (r == 0) up = 0ull;
-            if (r >= TH - 1) dn = 0ull;
-            uint64_t v = m | up | dn;
-            v |= (v << 1) | (v >> 1);
-            const uint64_t n = (v & e) | m;
-            const bool changed = n != m;
-            m = n;
-            if (!__any(changed)) break;
-        }
-        const uint64_t other = __shfl(m, lane ^ 32);
-        if (r < TH) {
-            if (lane < 32) s.Rm[r] = m;
-            else s.Bm[r] = m & ~other;
-        }
+    const int rr = lane & 31;
+    const uint64_t emine = ((uint64_t)emHi << 32) | emLo, smine = ((uint64_t)smHi << 32) | smLo;
+    const uint64_t e = __shfl(emine, rr);
+    uint64_t seeds;
+    if (lane < 32) seeds = __shfl(smine, rr);
+    else seeds = (rr == 0 || rr == th - 1) ? e : (e & (1ull | (1ull << (tw - 1))));
+    const uint64_t grown = grow_rows(seeds, e, lane);
+    const uint64_t other = __shfl(grown, lane ^ 32);
+    const uint64_t rm = lane < 32 ? grown : other;                  // lane r (both halves): reached from strong
+    const uint64_t bmAll = lane < 32 ? (other & ~grown) : 0ull;     // lane r < 32: undecided
+
+    //  (b) settle: reached -> 255; undecided -> 127 (k_resolve decides them); other edge pixels -> 0
+    gptr_t<uint8_t> eo = gmem(F.edges) + (long)ty0 * F.edgesStride + ox;
+    for (int r = 0; r < th; r++) {
+        const uint64_t R = readlane64(rm, r), B = readlane64(bmAll, r);
+        const int y = ty0 + r;
+        if (lane < TW && (INNER || (y >= 1 && y <= h - 2 && ox >= 1 && ox <= w - 2)))
+            eo[(long)r * F.edgesStride] = ((R >> lane) & 1ull) ? 255 : (((B >> lane) & 1ull) ? 127 : 0);
     }
-    __syncthreads();
-    //    (b) union-find over the undecided pixels (8-connectivity in the tile).  Every 8-neighbour
-    //    of an undecided pixel that is an edge pixel lies in the same component, so the row masks
-    //    answer the neighbour tests and only undecided pixels are labelled.
-    for (int q = 0; q < TH / NW; q++) {
-        const int r = wave + q * NW;
-        if ((s.Bm[r] >> lane) & 1ull) s.L[r * TP + lane] = r * TP + lane;
+
+    //  (c) the undecided components, one at a time: the first set bit in row-major order is the
+    //      component's smallest index p = row*TP + column, its id 1 + tile*TS + p; flood it inside the
+    //      undecided mask.  Border tables: top row, bottom row, left column, right column of the tile
+    //      (actual extent): 0 for reached pixels (the strong node), the component id for undecided
+    //      ones, -1 otherwise.
+    int32_t topv = -1, botv = -1, leftv = -1, rightv = -1;
+    {
+        const uint64_t R0 = readlane64(rm, 0), Rb = readlane64(rm, th - 1);
+        if (lane < tw && ((R0 >> lane) & 1ull)) topv = 0;
+        if (lane < tw && ((Rb >> lane) & 1ull)) botv = 0;
+        if (lane < th && (rm & 1ull)) leftv = 0;
+        if (lane < th && ((rm >> (tw - 1)) & 1ull)) rightv = 0;
     }
-    __syncthreads();
-    for (int q = 0; q < TH / NW; q++) {
-        const int r = wave + q * NW;
-        const uint64_t b = s.Bm[r];
-        if (!b) continue;
-        if (!((b >> lane) & 1ull)) continue;
-        const int p = r * TP + lane;
-        if ((b >> (lane + 1)) & 1ull) lunion(s.L, p, p + 1);
-        if (r + 1 < TH) {
-            const uint64_t bd = s.Bm[r + 1];
-            if (lane > 0 && ((bd >> (lane - 1)) & 1ull)) lunion(s.L, p, p + TP - 1);
-            if ((bd >> lane) & 1ull) lunion(s.L, p, p + TP);
-            if ((bd >> (lane + 1)) & 1ull) lunion(s.L, p, p + TP + 1);
-        }
+    const bool anyU = __ballot(bmAll != 0ull) != 0ull;
+    uint64_t bm = bmAll;
+    while (true) {
+        const uint64_t rows = __ballot(bm != 0ull);
+        if (!rows) break;
+        const int r0 = __builtin_ctzll(rows);
+        const int c0 = __builtin_ctzll(readlane64(bm, r0));
+        const uint32_t gid = 1u + (uint32_t)tile * TS + (uint32_t)(r0 * TP + c0);
+        const uint64_t c = grow_rows(lane == r0 ? (1ull << c0) : 0ull, bm, lane);
+        if (lane == 0) gmem(F.parent)[gid] = gid;
+        const uint64_t C0 = readlane64(c, 0), Cb = readlane64(c, th - 1);
+        if (lane < tw && ((C0 >> lane) & 1ull)) topv = (int32_t)gid;
+        if (lane < tw && ((Cb >> lane) & 1ull)) botv = (int32_t)gid;
+        if (lane < th && (c & 1ull)) leftv = (int32_t)gid;
+        if (lane < th && ((c >> (tw - 1)) & 1ull)) rightv = (int32_t)gid;
+        bm &= ~c;
     }
-    __syncthreads();
-    //    (c) settle: reached -> 255; other edge pixels -> 0; undecided -> 127 and listed with their
-    //    tile-local root for k_border / k_resolve, under global id 1 + tile*TS + root.  Every reached
-    //    border pixel maps to the single strong node 0 (the smallest id, so after atomicMin linking a
-    //    set is strong iff its root is 0).
-    for (int q = 0; q < TH / NW; q++) {
-        const int r = wave + q * NW;
-        const int y = ty0 + r, x = tx0 + lane;
-        if (lane >= TW || !(INNER || (y >= 1 && y <= h - 2 && x >= 1 && x <= w - 2))) continue;
-        uint8_t o = 0;
-        if ((s.Rm[r] >> lane) & 1ull) {
-            o = 255;
-        } else if ((s.Bm[r] >> lane) & 1ull) {
-            const int p = r * TP + lane;
-            const int root = lfind(s.L, p);
-            o = 127;
-            if (root == p) gmem(F.parent)[1u + (uint32_t)tile * TS + (uint32_t)p] = 1u + (uint32_t)tile * TS + (uint32_t)p;
-            const uint32_t i = atomicAdd(&s.unres, 1u);
-            gmem(F.ulist)[(size_t)tile * TS + i] = (uint32_t)p | ((uint32_t)root << 16);
-        }
-        gmem(F.edges)[(long)y * F.edgesStride + x] = o;
-    }
-    // border tables: top row, bottom row, left column, right column of the tile (actual extent)
     gptr_t<int32_t> bt = gmem(F.border) + (long)tile * 2 * (TW + TH);
-    for (int n = t; n < 2 * (TW + TH); n += FT) {
-        int lx, ly;
-        if (n < TW) { lx = n; ly = 0; }
-        else if (n < 2 * TW) { lx = n - TW; ly = th - 1; }
-        else if (n < 2 * TW + TH) { lx = 0; ly = n - 2 * TW; }
-        else { lx = tw - 1; ly = n - 2 * TW - TH; }
-        int32_t g = -1;
-        if (lx < tw && ly < th) {
-            if ((s.Rm[ly] >> lx) & 1ull) g = 0;
-            else if ((s.Bm[ly] >> lx) & 1ull) g = (int32_t)(1u + (uint32_t)tile * TS + (uint32_t)lfind(s.L, ly * TP + lx));
-        }
-        bt[n] = g;
-    }
-    __syncthreads();
-    if (t == 0) gmem(F.ucount)[tile] = s.unres;
+    if (lane < TW) { bt[lane] = topv; bt[TW + lane] = botv; }
+    if (lane < TH) { bt[2 * TW + lane] = leftv; bt[2 * TW + TH + lane] = rightv; }
+    if (anyU && lane < TH) gmem(F.umask)[(size_t)tile * TH + lane] = bmAll;
+    if (lane == 0) gmem(F.ucount)[tile] = anyU ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(FT)
+__global__ void __launch_bounds__(64 * FTW)
 k_front(const FrameDesc *__restrict__ frames, GraphParams P)
 {
-    __shared__ FrontShared s;
+    __shared__ uint64_t candLds[FTW][CAND_MAX];
     const FrameDesc F = frames[blockIdx.z];
-    const int tx0 = (int)blockIdx.x * TW, ty0 = (int)blockIdx.y * TH;
-    const int tile = (int)blockIdx.y * P.tilesX + (int)blockIdx.x;
-    if (threadIdx.x == 0) { s.cnt = 0; s.unres = 0; }
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int tile = (int)blockIdx.x * FTW + wave;
+    if (tile >= P.tilesX * P.tilesY) return;
+    const int tx0 = (tile % P.tilesX) * TW, ty0 = (tile / P.tilesX) * TH;
     const bool inner = tx0 >= 4 && tx0 + TW + 4 <= P.w && ty0 >= 4 && ty0 + TH + 4 <= P.h;
-    if (inner) front_tile<true>(s, F, P, tx0, ty0, tile);
-    else front_tile<false>(s, F, P, tx0, ty0, tile);
+    if (inner) front_tile<true>(candLds[wave], F, P, tx0, ty0, tile);
+    else front_tile<false>(candLds[wave], F, P, tx0, ty0, tile);
 }
 
 // union the border components of each tile with those of its right, lower and diagonal neighbours;
@@ -555,24 +523,36 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
     }
 }
 
-// the undecided pixels k_front listed take the state of their component's global root (strong iff
-// the root is node 0); one wave per tile, four tiles per workgroup
+// The undecided pixels of each tile take the state of their component's global root (strong iff the
+// root is node 0).  The components are rebuilt from the tile's row masks exactly as k_front labelled
+// them (first set bit in row-major order, flood inside the mask), so each gets the same id.  One wave
+// per tile, four tiles per workgroup.
 __global__ void __launch_bounds__(256)
 k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
 {
     const FrameDesc &F = frames[blockIdx.z];
     const int lane = (int)threadIdx.x & 63;
-    const int tile = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     if (tile >= P.tilesX * P.tilesY) return;
-    const uint32_t n = gmem(F.ucount)[tile];
-    if (n == 0) return;
+    if (!gmem(F.ucount)[tile]) return;
     const int tx0 = (tile % P.tilesX) * TW, ty0 = (tile / P.tilesX) * TH;
-    gptr_t<const uint32_t> list = gmem((const uint32_t *)F.ulist) + (size_t)tile * TS;
-    for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
-        const uint32_t e = list[i];
-        const uint32_t p = e & 0xffffu, root = e >> 16;
-        const uint32_t g = gfind(gmem(F.parent), 1u + (uint32_t)tile * TS + root);
-        gmem(F.edges)[(long)(ty0 + (int)(p / TP)) * F.edgesStride + tx0 + (int)(p % TP)] = (g == 0) ? 255 : 0;
+    uint64_t bm = lane < TH ? gmem((const uint64_t *)F.umask)[(size_t)tile * TH + lane] : 0ull;
+    gptr_t<uint8_t> eo = gmem(F.edges) + (long)ty0 * F.edgesStride + tx0 + lane;
+    while (true) {
+        const uint64_t rows = __ballot(bm != 0ull);
+        if (!rows) break;
+        const int r0 = __builtin_ctzll(rows);
+        const int c0 = __builtin_ctzll(readlane64(bm, r0));
+        const uint32_t gid = 1u + (uint32_t)tile * TS + (uint32_t)(r0 * TP + c0);
+        const uint64_t c = grow_rows(lane == r0 ? (1ull << c0) : 0ull, bm, lane);
+        const uint8_t val = gfind(gmem(F.parent), gid) == 0 ? 255 : 0;
+        uint64_t crow = __ballot(c != 0ull);
+        while (crow) {
+            const int r = __builtin_ctzll(crow);
+            crow &= crow - 1;
+            if ((readlane64(c, r) >> lane) & 1ull) eo[(long)r * F.edgesStride] = val;
+        }
+        bm &= ~c;
     }
 }
 
@@ -604,7 +584,7 @@ __global__ void k_fill_bytes(uint32_t *p, uint32_t v, size_t n)
 
 // ---------------------------------------------------------------- host side -----------------------
 struct GraphLayout {
-    size_t descs, frameBytes, border, parent, ulist, ucount, cand, candCount, pickHdr, pickWs, total;
+    size_t descs, frameBytes, border, parent, umask, ucount, cand, candCount, pickHdr, pickWs, total;
     uint32_t tiles, candCap;
 };
 
@@ -618,7 +598,7 @@ inline GraphLayout graph_layout(const mvx_vision_params_t &p, uint32_t maxFrames
     size_t o = 0;
     L.border = o;    o = align256(o + (size_t)L.tiles * 2 * (TW + TH) * 4);
     L.parent = o;    o = align256(o + ((size_t)L.tiles * TS + 1) * 4);
-    L.ulist = o;     o = align256(o + (size_t)L.tiles * TS * 4);
+    L.umask = o;     o = align256(o + (size_t)L.tiles * TH * 8);
     L.ucount = o;    o = align256(o + (size_t)L.tiles * 4);
     L.cand = o;      o = align256(o + (size_t)L.candCap * sizeof(XYS));
     L.candCount = o; o = align256(o + 4);
@@ -713,7 +693,7 @@ extern "C" int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_par
         uint8_t *fb = base + (size_t)f * L.frameBytes;
         d.border = (int32_t *)(fb + L.border);
         d.parent = (uint32_t *)(fb + L.parent);
-        d.ulist = (uint32_t *)(fb + L.ulist);
+        d.umask = (uint64_t *)(fb + L.umask);
         d.ucount = (uint32_t *)(fb + L.ucount);
         d.cand = fb + L.cand;
         d.candCount = (uint32_t *)(fb + L.candCount);
@@ -745,7 +725,7 @@ extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_p
     hipStream_t s = stream_of(stream);
     switch (stage) {
     case 0:
-        hipLaunchKernelGGL(k_front, dim3(P.tilesX, P.tilesY, nframes), dim3(FT), 0, s, D, P);
+        hipLaunchKernelGGL(k_front, dim3((P.tilesX * P.tilesY + FTW - 1) / FTW, 1, nframes), dim3(64 * FTW), 0, s, D, P);
         break;
     case 1:
         hipLaunchKernelGGL(k_border, dim3((P.tilesX * P.tilesY + 3) / 4, 1, nframes), dim3(256), 0, s, D, P);
