@@ -22,7 +22,7 @@
 //   k_border    unions the hysteresis components that touch tile borders, across tiles.  Strong
 //               components get ids below every weak one, so after atomicMin linking a set's root is
 //               strong iff the set holds a strong component -- no marking pass.
-//   k_resolve   the undecided weak pixels of each tile take their component's global root's state.
+//   k_resolve   the undecided weak pixels k_front listed per tile take their global root's state.
 //   k_half      pyramid levels 2.. (one launch per level, all frames).
 //   k_pick      Harris sort and pick, one workgroup per frame (pick.hpp).
 //
@@ -59,7 +59,7 @@ struct FrameDesc {
     // per-frame workspace
     int32_t *border;      // tiles x 2(TW+TH) component ids of tile-border pixels (-1: none)
     uint32_t *parent;     // 1 + tiles * TS union-find slots (sparsely used)
-    uint64_t *umask;      // tiles x TH: a tile's undecided pixels, one 64-bit row mask per tile row
+    uint32_t *ulist;      // tiles x TS: a tile's undecided pixels, p | p of the component's id << 16
     uint32_t *ucount;     // tiles: undecided pixels listed
     void *cand;           // Harris XYS candidates
     uint32_t *candCount;  // candidates appended (reset by k_pick)
@@ -425,7 +425,8 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
         if (lane < th && (rm & 1ull)) leftv = 0;
         if (lane < th && ((rm >> (tw - 1)) & 1ull)) rightv = 0;
     }
-    const bool anyU = __ballot(bmAll != 0ull) != 0ull;
+    gptr_t<uint32_t> ulist = gmem(F.ulist) + (size_t)tile * TS;
+    uint32_t nU = 0;
     uint64_t bm = bmAll;
     while (true) {
         const uint64_t rows = __ballot(bm != 0ull);
@@ -440,13 +441,24 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
         if (lane < tw && ((Cb >> lane) & 1ull)) botv = (int32_t)gid;
         if (lane < th && (c & 1ull)) leftv = (int32_t)gid;
         if (lane < th && ((c >> (tw - 1)) & 1ull)) rightv = (int32_t)gid;
+        // list the component's pixels for k_resolve: p | p of the component's smallest pixel << 16
+        uint64_t crow = __ballot(c != 0ull);
+        while (crow) {
+            const int r = __builtin_ctzll(crow);
+            crow &= crow - 1;
+            const uint64_t m = readlane64(c, r);
+            if ((m >> lane) & 1ull) {
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                ulist[nU + below] = (uint32_t)(r * TP + lane) | ((uint32_t)(r0 * TP + c0) << 16);
+            }
+            nU += (uint32_t)__popcll(m);
+        }
         bm &= ~c;
     }
     gptr_t<int32_t> bt = gmem(F.border) + (long)tile * 2 * (TW + TH);
     if (lane < TW) { bt[lane] = topv; bt[TW + lane] = botv; }
     if (lane < TH) { bt[2 * TW + lane] = leftv; bt[2 * TW + TH + lane] = rightv; }
-    if (anyU && lane < TH) gmem(F.umask)[(size_t)tile * TH + lane] = bmAll;
-    if (lane == 0) gmem(F.ucount)[tile] = anyU ? 1u : 0u;
+    if (lane == 0) gmem(F.ucount)[tile] = nU;
 }
 
 __global__ void __launch_bounds__(64 * FTW)
@@ -523,10 +535,8 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
     }
 }
 
-// The undecided pixels of each tile take the state of their component's global root (strong iff the
-// root is node 0).  The components are rebuilt from the tile's row masks exactly as k_front labelled
-// them (first set bit in row-major order, flood inside the mask), so each gets the same id.  One wave
-// per tile, four tiles per workgroup.
+// the undecided pixels k_front listed take the state of their component's global root (strong iff
+// the root is node 0); one wave per tile, four tiles per workgroup
 __global__ void __launch_bounds__(256)
 k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
 {
@@ -534,25 +544,15 @@ k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
     const int lane = (int)threadIdx.x & 63;
     const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     if (tile >= P.tilesX * P.tilesY) return;
-    if (!gmem(F.ucount)[tile]) return;
+    const uint32_t n = gmem(F.ucount)[tile];
+    if (n == 0) return;
     const int tx0 = (tile % P.tilesX) * TW, ty0 = (tile / P.tilesX) * TH;
-    uint64_t bm = lane < TH ? gmem((const uint64_t *)F.umask)[(size_t)tile * TH + lane] : 0ull;
-    gptr_t<uint8_t> eo = gmem(F.edges) + (long)ty0 * F.edgesStride + tx0 + lane;
-    while (true) {
-        const uint64_t rows = __ballot(bm != 0ull);
-        if (!rows) break;
-        const int r0 = __builtin_ctzll(rows);
-        const int c0 = __builtin_ctzll(readlane64(bm, r0));
-        const uint32_t gid = 1u + (uint32_t)tile * TS + (uint32_t)(r0 * TP + c0);
-        const uint64_t c = grow_rows(lane == r0 ? (1ull << c0) : 0ull, bm, lane);
-        const uint8_t val = gfind(gmem(F.parent), gid) == 0 ? 255 : 0;
-        uint64_t crow = __ballot(c != 0ull);
-        while (crow) {
-            const int r = __builtin_ctzll(crow);
-            crow &= crow - 1;
-            if ((readlane64(c, r) >> lane) & 1ull) eo[(long)r * F.edgesStride] = val;
-        }
-        bm &= ~c;
+    gptr_t<const uint32_t> list = gmem((const uint32_t *)F.ulist) + (size_t)tile * TS;
+    for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
+        const uint32_t e = list[i];
+        const uint32_t p = e & 0xffffu, root = e >> 16;
+        const uint32_t g = gfind(gmem(F.parent), 1u + (uint32_t)tile * TS + root);
+        gmem(F.edges)[(long)(ty0 + (int)(p / TP)) * F.edgesStride + tx0 + (int)(p % TP)] = (g == 0) ? 255 : 0;
     }
 }
 
@@ -584,7 +584,7 @@ __global__ void k_fill_bytes(uint32_t *p, uint32_t v, size_t n)
 
 // ---------------------------------------------------------------- host side -----------------------
 struct GraphLayout {
-    size_t descs, frameBytes, border, parent, umask, ucount, cand, candCount, pickHdr, pickWs, total;
+    size_t descs, frameBytes, border, parent, ulist, ucount, cand, candCount, pickHdr, pickWs, total;
     uint32_t tiles, candCap;
 };
 
@@ -598,7 +598,7 @@ inline GraphLayout graph_layout(const mvx_vision_params_t &p, uint32_t maxFrames
     size_t o = 0;
     L.border = o;    o = align256(o + (size_t)L.tiles * 2 * (TW + TH) * 4);
     L.parent = o;    o = align256(o + ((size_t)L.tiles * TS + 1) * 4);
-    L.umask = o;     o = align256(o + (size_t)L.tiles * TH * 8);
+    L.ulist = o;     o = align256(o + (size_t)L.tiles * TS * 4);
     L.ucount = o;    o = align256(o + (size_t)L.tiles * 4);
     L.cand = o;      o = align256(o + (size_t)L.candCap * sizeof(XYS));
     L.candCount = o; o = align256(o + 4);
@@ -693,7 +693,7 @@ extern "C" int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_par
         uint8_t *fb = base + (size_t)f * L.frameBytes;
         d.border = (int32_t *)(fb + L.border);
         d.parent = (uint32_t *)(fb + L.parent);
-        d.umask = (uint64_t *)(fb + L.umask);
+        d.ulist = (uint32_t *)(fb + L.ulist);
         d.ucount = (uint32_t *)(fb + L.ucount);
         d.cand = fb + L.cand;
         d.candCount = (uint32_t *)(fb + L.candCount);
