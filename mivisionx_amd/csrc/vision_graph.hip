@@ -225,17 +225,24 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
     const int oxp = (w & 1) ? 0 : 1, oyp = (h & 1) ? 0 : 1;
 
     int y0 = 0, y1 = 0, y2 = 0, y3 = 0;                   // luma rows k-4..k-1
-    int dA = 0, dB = 0, sA = 0, sB = 0;                   // row k-2, k-1: Y[l+2]-Y[l], Y[l]+2Y[l+1]+Y[l+2]
+    int dA = 0, dB = 0, sA = 0, sB = 0;                   // rows k-2, k-1: Y[l+2]-Y[l], Y[l]+2Y[l+1]+Y[l+2]
     int hxx0 = 0, hxy0 = 0, hyy0 = 0, hxx1 = 0, hxy1 = 0, hyy1 = 0;   // HG3 3-column sums, gradient rows g-2, g-1
     uint32_t Ma = 0, Mb = 0, Mc = 0;                      // Canny magnitude|sector rows k-6, k-5, k-4
-    float Va = 0.0f, Vb = 0.0f;                           // Harris score rows k-6, k-5
+    int Va = 0, Vb = 0, Ha = 0, Hb = 0;                   // Harris score keys, rows k-6, k-5; their 3-column maxima
     uint32_t emLo = 0, emHi = 0, smLo = 0, smHi = 0;      // lane r: edge / strong mask of tile row r
     uint32_t nc = 0;                                      // candidates in the wave's buffer
+    // Harris scores compare as int32 keys: a non-negative float's bits order like the float; with a
+    // negative threshold negative scores occur, and the key flips their magnitude bits (an involution).
+    // Scores are never -0 (mc is never -0), so equal keys are equal scores.
+    const bool negKeys = P.thr < 0.0f;
+    const int U = (int)P.upper, Lo = (int)P.lower;
     uint32_t pf0 = 0, pf1 = 0;
     if (INNER) { pf0 = fetch(0); pf1 = fetch(1); }
 
+#pragma unroll 2
     for (int k = 0; k < YH; k++) {
-        // ---- luma row k (ColorConvert Y of the RGB pixel, or the virtual plane's linear-address value)
+        // ---- luma row k (ColorConvert Y of the RGB pixel, or the virtual plane's linear-address value);
+        //      the float sum lies in [0, 256), so the truncation needs no clamp
         int yk;
         if (INNER) {
             const uint32_t cur = pf0;
@@ -244,13 +251,14 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
             float f = (float)(cur & 0xffu) * 0.2126f;
             f = f + (float)((cur >> 8) & 0xffu) * 0.7152f;
             f = f + (float)((cur >> 16) & 0xffu) * 0.0722f;
-            yk = min(max((int)f, 0), 255);
+            yk = (int)f;
         } else {
             yk = (int)yflat(F, P, ty0 - HY + k, tx0 - HY + lane);
         }
         if (k >= HY && k < HY + th && lane >= HY && lane < HY + tw) l0[(long)(k - HY) * F.lvlStride[0]] = (uint8_t)yk;
-        const int n1 = lane_next(yk), n2 = lane_next(n1);
-        const int dk = n2 - yk, sk = yk + 2 * n1 + n2;
+        // binomial column sums through one-lane shifts: t = Y[l]+Y[l+1], s = Y[l]+2Y[l+1]+Y[l+2], d = Y[l+2]-Y[l]
+        const int tk = yk + lane_next(yk);
+        const int sk = tk + lane_next(tk), dk = lane_next(tk) - tk;
 
         // ---- ScaleImage 2:1 (ago_haf_cpu_geometric.cpp:2000-2045 with xscale = yscale = 2, offsets 0.5):
         //      output row oy from luma rows 2oy+3 (k-1) and 2oy+4 (k); lane 2X'+3 holds output column X'
@@ -258,53 +266,54 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
             const int oy = (k - 4) >> 1, Yo = ty0 / 2 + oy;
             const int lp = lane - HY;
             const int Xo = tx0 / 2 + (lp >> 1);
-            const int p00 = y3, p01 = lane_next(y3), p10 = yk, p11 = n1;
+            const int top = y3 + lane_next(y3);                 // p00 + p01; p10 + p11 = tk
             if (Yo < P.dh && lp >= 0 && !(lp & 1) && (lp >> 1) < TW / 2 && Xo < P.dw) {
                 int v;
-                if (Xo < body) {
-                    const int t0 = (p00 + p01 + 1) >> 1, t1 = (p10 + p11 + 1) >> 1;   // (128 a + 128 b + 0x80) >> 8
-                    v = (t0 + t1 + 1) >> 1;
-                } else {
-                    v = (p00 + p01 + p10 + p11 + 2) >> 2;                           // (128*128 sum + 0x8000) >> 16
-                }
+                if (Xo < body) v = (((top + 1) >> 1) + ((tk + 1) >> 1) + 1) >> 1;   // (128 a + 128 b + 0x80) >> 8, twice
+                else v = (top + tk + 2) >> 2;                                         // (128*128 sum + 0x8000) >> 16
                 gmem(F.scaled)[(long)Yo * F.scaledStride + Xo] = (uint8_t)v;
             }
         }
         // ---- ScaleGaussianHalf_5x5 level 1 (ago_haf_cpu_pyramid.cpp:94-305): centre (2X+oxp, 2Y+oyp),
-        //      luma rows k-4..k; lane 2X'+oxp+1 sums columns lane..lane+4 (centre lane+2)
+        //      luma rows k-4..k; the horizontal 1 4 6 4 1 is four "v + v[l+1]" steps, so lane 2X'+oxp+1
+        //      holds the sum over columns lane..lane+4 (centre lane+2)
         if (P.levels > 1 && k >= 5 + oyp && !((k - 5 - oyp) & 1)) {
             const int oy = (k - 5 - oyp) >> 1, Yo = ty0 / 2 + oy;
             if (oy < TH / 2 && Yo >= 1 && Yo <= P.lvlH[1] - 2) {
-                const uint32_t vc = (uint32_t)(y0 + yk) + 4u * (uint32_t)(y1 + y3) + 6u * (uint32_t)y2;
-                const uint32_t v1 = (uint32_t)lane_next((int)vc), v2 = (uint32_t)lane_next((int)v1);
-                const uint32_t v3 = (uint32_t)lane_next((int)v2), v4 = (uint32_t)lane_next((int)v3);
-                const uint32_t acc = vc + 4u * v1 + 6u * v2 + 4u * v3 + v4;
+                int acc = (y0 + yk) + 4 * (y1 + y3) + 6 * y2;
+                acc = acc + lane_next(acc);
+                acc = acc + lane_next(acc);
+                acc = acc + lane_next(acc);
+                acc = acc + lane_next(acc);
                 const int lp = lane - oxp - 1;
                 const int Xo = tx0 / 2 + (lp >> 1);
                 if (lp >= 0 && !(lp & 1) && (lp >> 1) < TW / 2 && Xo < P.lvlW[1])
-                    gmem(F.lvl[1])[(long)Yo * F.lvlStride[1] + Xo] = (uint8_t)((acc & 0xffffu) >> 8);
+                    gmem(F.lvl[1])[(long)Yo * F.lvlStride[1] + Xo] = (uint8_t)(((uint32_t)acc & 0xffffu) >> 8);
             }
         }
 
         if (k >= 2) {
-            // ---- Sobel gradient row g = k-2 (linear-address luma), HG3 products and 3-column sums;
-            //      HG3 rows 0 and h-1 are never written (zero).  The products are integers whose 9-sums
-            //      stay below 2^24, so int32 sums equal the reference's float sums exactly.
+            // ---- Sobel gradient row g = k-2 (linear-address luma; |gx|, |gy| <= 1020, so the reference's
+            //      S16 values are these), HG3 products and 3-column sums.  HG3 rows 0 and h-1 are never
+            //      written (zero).  The products are integers whose 9-sums stay below 2^24, so int32 sums
+            //      equal the reference's float sums exactly.
             const int g = k - 2;
-            const int gx = (int)(int16_t)(dA + 2 * dB + dk), gy = (int)(int16_t)(sk - sA);
+            const int gx = dA + 2 * dB + dk, gy = sk - sA;
             int pxx = 0, pxy = 0, pyy = 0;
             if (INNER || (ty0 - 2 + g >= 1 && ty0 - 2 + g <= h - 2)) { pxx = gx * gx; pxy = gx * gy; pyy = gy * gy; }
+            asm("" : "+v"(pxx), "+v"(pxy), "+v"(pyy));   // keep the window sums as shifted adds (not mads)
             const int hxx2 = pxx + lane_next(pxx + lane_next(pxx));
             const int hxy2 = pxy + lane_next(pxy + lane_next(pxy));
             const int hyy2 = pyy + lane_next(pyy + lane_next(pyy));
 
-            // ---- Canny magnitude|sector row m = k-3 from gradient row g at column lane+1
+            // ---- Canny magnitude|sector row m = k-3: packed at the gradient's column, then moved to the
+            //      magnitude column (lane+1 -> lane)
             uint32_t Md = 0;
             if (k >= 3) {
                 const int m = k - 3, y = ty0 - 1 + m;
-                const int gxn = lane_next(gx), gyn = lane_next(gy);
+                const uint32_t packed = (uint32_t)lane_next((int)canny_pack_f(gx, -gy));
                 if (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w)) {
-                    Md = canny_pack_f(gxn, -gyn);
+                    Md = packed;
                 } else if (y >= 1 && y <= h - 2 && tx0 == 0) {
                     // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
                     // r1[-1] = r0[ALIGN16(w)-1]); luma lanes 3, 4 are x = 0, 1 and rows k-2..k are y-1..y+1
@@ -320,10 +329,12 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
                     }
                 }
             }
-            // ---- Harris score row k-4: Mc = mc / norm > T ? mc / norm : 0, the quotient exactly the IEEE one
-            float Vc = 0.0f;
+            // ---- Harris score row k-4: Vc = mc / norm > T ? mc / norm : 0, the quotient exactly the IEEE
+            //      one; kept as its compare key, with the maximum of the key over columns lane..lane+2
+            int Vc = 0;
             if (k >= 4) {
                 const int yv = ty0 - 1 + (k - 4);
+                float v = 0.0f;
                 if (INNER || (x >= 1 && x <= w - 2 && yv >= 1 && yv <= h - 2)) {
                     const float gx2 = (float)(hxx0 + hxx1 + hxx2);
                     const float gxy2 = (float)(hxy0 + hxy1 + hxy2);
@@ -332,41 +343,45 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
                     const float det = (gx2 * gy2) - (gxy2 * gxy2);
                     const float mc = det - (P.k * tr * tr);
                     const float q = div_by_const_rn(mc, P.norm, P.normRcp);
-                    Vc = (q > P.thr) ? q : 0.0f;
+                    v = (q > P.thr) ? q : 0.0f;
                 }
+                Vc = __float_as_int(v);
+                if (negKeys) Vc ^= (Vc >> 31) & 0x7fffffff;
             }
-            // ---- NMS of tile row r = k-6 (image row ty0+r): magnitude rows r..r+2 = Ma, Mb, Mc and
-            //      score rows r..r+2 = Va, Vb, Vc, columns lane..lane+2, centre lane+1
+            int Hc = max(Vc, lane_next(Vc));
+            Hc = max(Vc, lane_next(Hc));
+            // ---- NMS of tile row r = k-6 (image row ty0+r): magnitude rows r..r+2 = Ma, Mb, Mc and score
+            //      rows r..r+2 = Va, Vb, Vc, columns lane..lane+2, centre lane+1
             if (k >= 6) {
                 const int r = k - 6, y = ty0 + r;
                 const bool valid = lane < TW && (INNER || (y >= 1 && y <= h - 2 && ox >= 1 && ox <= w - 2));
-                const uint32_t a1 = (uint32_t)lane_next((int)Ma), a2 = (uint32_t)lane_next((int)a1);
-                const uint32_t b1 = (uint32_t)lane_next((int)Mb), b2 = (uint32_t)lane_next((int)b1);
-                const uint32_t c1 = (uint32_t)lane_next((int)Mc), c2 = (uint32_t)lane_next((int)c1);
+                // Canny: the larger of the two neighbours along the sector's direction, packed values
+                // compared through (v | 3): mag > (n >> 2) <=> (centre | 3) > (n | 3)
+                const uint32_t a1 = (uint32_t)lane_next((int)Ma), b1 = (uint32_t)lane_next((int)Mb);
+                const uint32_t c1 = (uint32_t)lane_next((int)Mc);
                 const uint32_t sec = b1 & 3u;
-                const uint32_t n0 = sec == 0 ? Mb : sec == 1 ? a2 : sec == 2 ? a1 : Ma;
-                const uint32_t nn = sec == 0 ? b2 : sec == 1 ? Mc : sec == 2 ? c1 : c2;
-                const int mag = (int)(b1 >> 2);
-                const int edge = (mag > (int)(n0 >> 2) && mag > (int)(nn >> 2)) ? mag : 0;
-                const uint32_t st = !valid ? 0u : edge > (int)P.upper ? 255u : (edge <= (int)P.lower ? 0u : 127u);
-                const uint64_t eb = __ballot(st != 0u), sb = __ballot(st == 255u);
+                const uint32_t d0 = max(Mb, (uint32_t)lane_next((int)b1));     // (r+1, l), (r+1, l+2)
+                const uint32_t d1 = max(Mc, (uint32_t)lane_next((int)a1));     // (r, l+2), (r+2, l)
+                const uint32_t d2 = max(a1, c1);                               // (r, l+1), (r+2, l+1)
+                const uint32_t d3 = max(Ma, (uint32_t)lane_next((int)c1));     // (r, l), (r+2, l+2)
+                const uint32_t nmax = sec == 0 ? d0 : sec == 1 ? d1 : sec == 2 ? d2 : d3;
+                const int edge = ((b1 | 3u) > (nmax | 3u)) ? (int)(b1 >> 2) : 0;
+                const bool strongPx = valid && edge > U, edgePx = valid && (edge > U || edge > Lo);
+                const uint64_t eb = __ballot(edgePx), sb = __ballot(strongPx);
                 if (lane == r) { emLo = (uint32_t)eb; emHi = (uint32_t)(eb >> 32); smLo = (uint32_t)sb; smHi = (uint32_t)(sb >> 32); }
                 // Harris 3x3 NMS: >= towards the upper-left half of the window, > towards the lower-right
-                const float c = lane_next_f(Vb), vr = lane_next_f(c);
-                const float va1 = lane_next_f(Va), va2 = lane_next_f(va1);
-                const float vc1 = lane_next_f(Vc), vc2 = lane_next_f(vc1);
-                const float upl = fmaxf(fmaxf(Va, va1), fmaxf(va2, Vb));
-                const float lwr = fmaxf(fmaxf(vr, Vc), fmaxf(vc1, vc2));
-                const bool isMax = valid && c >= upl && c > lwr;
+                const int c = lane_next(Vb);
+                const bool isMax = valid && c >= max(Ha, Vb) && c > max(Hc, lane_next(c));
                 const uint64_t cb = __ballot(isMax);
                 if (isMax) {
                     const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cb, 0u));
-                    cand[nc + below] = (uint64_t)(uint16_t)ox | ((uint64_t)(uint16_t)y << 16) | ((uint64_t)__float_as_uint(c) << 32);
+                    const uint32_t bits = negKeys ? (uint32_t)(c ^ ((c >> 31) & 0x7fffffff)) : (uint32_t)c;
+                    cand[nc + below] = (uint64_t)(uint16_t)ox | ((uint64_t)(uint16_t)y << 16) | ((uint64_t)bits << 32);
                 }
                 nc += (uint32_t)__popcll(cb);
             }
             Ma = Mb; Mb = Mc; Mc = Md;
-            Va = Vb; Vb = Vc;
+            Va = Vb; Vb = Vc; Ha = Hb; Hb = Hc;
             hxx0 = hxx1; hxy0 = hxy1; hyy0 = hyy1;
             hxx1 = hxx2; hxy1 = hxy2; hyy1 = hyy2;
         }
