@@ -175,6 +175,16 @@ MVX_DEV uint64_t readlane64(uint64_t v, int l)
 
 MVX_DEV float lane_next_f(float v) { return __int_as_float(lane_next(__float_as_int(v))); }
 
+// Raw buffer access to one plane: a scalar resource holding the plane's base, the lane's column byte
+// offset in a VGPR and the wave-uniform row offset in an SGPR, so a row's load or store needs no
+// per-lane 64-bit address arithmetic.  Word 3 is the gfx9 raw-buffer format.  The record limit is the
+// largest one, 2^32 - 1 bytes: planes of at most 32767 rows of at most 3 * 32767 + 15 bytes keep
+// every row + column offset below it, so no access is ever clipped.
+MVX_DEV __amdgpu_buffer_rsrc_t plane_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xffffffffu, 0x00020000);
+}
+
 // Row masks, lane = row (rows 0..31 of one problem in lanes 0..31, of a second in lanes 32..63):
 // grow m inside e by 8-connectivity until nothing changes.
 MVX_DEV uint64_t grow_rows(uint64_t m, uint64_t e, int lane)
@@ -196,7 +206,6 @@ MVX_DEV uint64_t grow_rows(uint64_t m, uint64_t e, int lane)
 
 constexpr int FTW = 4;                                     // tiles (one wave each) per workgroup of k_front
 constexpr int CAND_MAX = ((TW + 1) / 2) * ((TH + 1) / 2);  // Harris NMS maxima of a tile are never 8-adjacent
-typedef uint32_t u32_any_align __attribute__((aligned(1)));
 
 // One wave streams one tile: luma window rows k = 0..YH-1 (image row ty0-3+k, lane = column tx0-3+lane)
 // top to bottom, every later quantity from rolling register windows of the rows it needs, column
@@ -218,36 +227,45 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
 
     // inner tiles read a pixel's R, G, B (and the next pixel's R) with one dword load; the window's
     // last column is at most w-2, so the dword stays inside the row
-    gptr_t<const uint8_t> rgbCol = gmem(F.rgb) + (long)(ty0 - HY) * F.rgbStride + 3L * (tx0 - HY + lane);
-    auto fetch = [&](int k) -> uint32_t { return *(gptr_t<const u32_any_align>)(rgbCol + (long)k * F.rgbStride); };
-    gptr_t<uint8_t> l0 = gmem(F.lvl[0]) + (long)ty0 * F.lvlStride[0] + tx0 - HY + lane;
+    const __amdgpu_buffer_rsrc_t rgbR = plane_rsrc(F.rgb), l0R = plane_rsrc(F.lvl[0]), scR = plane_rsrc(F.scaled);
+    const __amdgpu_buffer_rsrc_t l1R = plane_rsrc(F.lvl[1]), edR = plane_rsrc(F.edges);
+    const int rgbCol = 3 * (tx0 - HY + lane);
+    auto fetch = [&](int k) -> uint32_t {
+        return __builtin_amdgcn_raw_buffer_load_b32(rgbR, rgbCol, (int)((uint32_t)(ty0 - HY + k) * F.rgbStride), 0);
+    };
     const int body = P.dw & ~7;
     const int oxp = (w & 1) ? 0 : 1, oyp = (h & 1) ? 0 : 1;
-
-    int y0 = 0, y1 = 0, y2 = 0, y3 = 0;                   // luma rows k-4..k-1
-    int dA = 0, dB = 0, sA = 0, sB = 0;                   // rows k-2, k-1: Y[l+2]-Y[l], Y[l]+2Y[l+1]+Y[l+2]
-    int hxx0 = 0, hxy0 = 0, hyy0 = 0, hxx1 = 0, hxy1 = 0, hyy1 = 0;   // HG3 3-column sums, gradient rows g-2, g-1
-    uint32_t Ma = 0, Mb = 0, Mc = 0;                      // Canny magnitude|sector rows k-6, k-5, k-4
-    int Va = 0, Vb = 0, Ha = 0, Hb = 0;                   // Harris score keys, rows k-6, k-5; their 3-column maxima
-    uint32_t emLo = 0, emHi = 0, smLo = 0, smHi = 0;      // lane r: edge / strong mask of tile row r
-    uint32_t nc = 0;                                      // candidates in the wave's buffer
     // Harris scores compare as int32 keys: a non-negative float's bits order like the float; with a
     // negative threshold negative scores occur, and the key flips their magnitude bits (an involution).
     // Scores are never -0 (mc is never -0), so equal keys are equal scores.
     const bool negKeys = P.thr < 0.0f;
     const int U = (int)P.upper, Lo = (int)P.lower;
-    uint32_t pf0 = 0, pf1 = 0;
-    if (INNER) { pf0 = fetch(0); pf1 = fetch(1); }
+    // output columns of this lane: level 0 (luma lane), 2:1 scaled (lane 2X'+3), level 1 (lane 2X'+oxp+1)
+    const int sp = lane - HY, lp1 = lane - oxp - 1;
+    const bool scOut = sp >= 0 && !(sp & 1) && (sp >> 1) < TW / 2 && tx0 / 2 + (sp >> 1) < P.dw;
+    const bool l1Out = lp1 >= 0 && !(lp1 & 1) && (lp1 >> 1) < TW / 2 && tx0 / 2 + (lp1 >> 1) < P.lvlW[1];
+    const bool scBody = tx0 / 2 + (sp >> 1) < body;
 
-#pragma unroll 2
-    for (int k = 0; k < YH; k++) {
-        // ---- luma row k (ColorConvert Y of the RGB pixel, or the virtual plane's linear-address value);
-        //      the float sum lies in [0, 256), so the truncation needs no clamp
+    uint32_t emLo = 0, emHi = 0, smLo = 0, smHi = 0;      // lane r: edge / strong mask of tile row r
+    uint32_t nc = 0;                                      // candidates in the wave's buffer
+    uint32_t packedPrev = 0;                              // Canny magnitude|sector of gradient row k-3, its own column
+    int Vb = 0;                                           // Harris score key row k-5
+
+    // One step of the stream (luma row k).  Windows deeper than one row are passed in age order and
+    // the step writes its new row over the oldest one, so the caller rotates the arguments instead of
+    // the values (no register copies): yA..yD = luma rows k-4..k-1 (yA <- row k); dO, dN / sO, sN = rows
+    // k-2, k-1 (dO <- row k); xx0.. = HG3 sums of gradient rows g-2, g-1 (<- row g); MO, MN = Canny
+    // rows k-6, k-5 (MO <- row k-4); HO, HN = score key 3-column maxima rows k-6, k-5 (HO <- row k-4);
+    // pO, pN = RGB of rows k, k+1 (pO <- row k+2).
+    auto step = [&](const int k, uint32_t &pO, uint32_t &pN, int &yA, int &yB, int &yC, int &yD, int &dO, int &dN,
+                    int &sO, int &sN, int &xx0, int &xx1, int &xy0, int &xy1, int &yy0, int &yy1, uint32_t &MO,
+                    uint32_t &MN, int &HO, int &HN) {
+        // ---- luma row k (ColorConvert Y of the RGB pixel, or the virtual plane's linear-address
+        //      value); the float sum lies in [0, 256), so the truncation needs no clamp
         int yk;
         if (INNER) {
-            const uint32_t cur = pf0;
-            pf0 = pf1;
-            if (k + 2 < YH) pf1 = fetch(k + 2);
+            const uint32_t cur = pO;
+            if (k + 2 < YH) pO = fetch(k + 2);
             float f = (float)(cur & 0xffu) * 0.2126f;
             f = f + (float)((cur >> 8) & 0xffu) * 0.7152f;
             f = f + (float)((cur >> 16) & 0xffu) * 0.0722f;
@@ -255,7 +273,8 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
         } else {
             yk = (int)yflat(F, P, ty0 - HY + k, tx0 - HY + lane);
         }
-        if (k >= HY && k < HY + th && lane >= HY && lane < HY + tw) l0[(long)(k - HY) * F.lvlStride[0]] = (uint8_t)yk;
+        if (k >= HY && k < HY + th && lane >= HY && lane < HY + tw)
+            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)yk, l0R, tx0 - HY + lane, (int)((uint32_t)(ty0 + k - HY) * F.lvlStride[0]), 0);
         // binomial column sums through one-lane shifts: t = Y[l]+Y[l+1], s = Y[l]+2Y[l+1]+Y[l+2], d = Y[l+2]-Y[l]
         const int tk = yk + lane_next(yk);
         const int sk = tk + lane_next(tk), dk = lane_next(tk) - tk;
@@ -263,15 +282,12 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
         // ---- ScaleImage 2:1 (ago_haf_cpu_geometric.cpp:2000-2045 with xscale = yscale = 2, offsets 0.5):
         //      output row oy from luma rows 2oy+3 (k-1) and 2oy+4 (k); lane 2X'+3 holds output column X'
         if (k >= 4 && !(k & 1) && k <= HY + TH - 1) {
-            const int oy = (k - 4) >> 1, Yo = ty0 / 2 + oy;
-            const int lp = lane - HY;
-            const int Xo = tx0 / 2 + (lp >> 1);
-            const int top = y3 + lane_next(y3);                 // p00 + p01; p10 + p11 = tk
-            if (Yo < P.dh && lp >= 0 && !(lp & 1) && (lp >> 1) < TW / 2 && Xo < P.dw) {
-                int v;
-                if (Xo < body) v = (((top + 1) >> 1) + ((tk + 1) >> 1) + 1) >> 1;   // (128 a + 128 b + 0x80) >> 8, twice
-                else v = (top + tk + 2) >> 2;                                         // (128*128 sum + 0x8000) >> 16
-                gmem(F.scaled)[(long)Yo * F.scaledStride + Xo] = (uint8_t)v;
+            const int Yo = ty0 / 2 + ((k - 4) >> 1);
+            const int top = yD + lane_next(yD);                 // p00 + p01; p10 + p11 = tk
+            if (Yo < P.dh && scOut) {
+                const int v = scBody ? ((((top + 1) >> 1) + ((tk + 1) >> 1) + 1) >> 1)   // (128 a + 128 b + 0x80) >> 8, twice
+                                     : ((top + tk + 2) >> 2);                             // (128*128 sum + 0x8000) >> 16
+                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, scR, tx0 / 2 + (sp >> 1), (int)((uint32_t)Yo * F.scaledStride), 0);
             }
         }
         // ---- ScaleGaussianHalf_5x5 level 1 (ago_haf_cpu_pyramid.cpp:94-305): centre (2X+oxp, 2Y+oyp),
@@ -280,114 +296,128 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
         if (P.levels > 1 && k >= 5 + oyp && !((k - 5 - oyp) & 1)) {
             const int oy = (k - 5 - oyp) >> 1, Yo = ty0 / 2 + oy;
             if (oy < TH / 2 && Yo >= 1 && Yo <= P.lvlH[1] - 2) {
-                int acc = (y0 + yk) + 4 * (y1 + y3) + 6 * y2;
+                int acc = (yA + yk) + 4 * (yB + yD) + 6 * yC;
                 acc = acc + lane_next(acc);
                 acc = acc + lane_next(acc);
                 acc = acc + lane_next(acc);
                 acc = acc + lane_next(acc);
-                const int lp = lane - oxp - 1;
-                const int Xo = tx0 / 2 + (lp >> 1);
-                if (lp >= 0 && !(lp & 1) && (lp >> 1) < TW / 2 && Xo < P.lvlW[1])
-                    gmem(F.lvl[1])[(long)Yo * F.lvlStride[1] + Xo] = (uint8_t)(((uint32_t)acc & 0xffffu) >> 8);
+                if (l1Out)
+                    __builtin_amdgcn_raw_buffer_store_b8((uint8_t)(((uint32_t)acc & 0xffffu) >> 8), l1R, tx0 / 2 + (lp1 >> 1),
+                                                         (int)((uint32_t)Yo * F.lvlStride[1]), 0);
             }
         }
 
-        if (k >= 2) {
-            // ---- Sobel gradient row g = k-2 (linear-address luma; |gx|, |gy| <= 1020, so the reference's
-            //      S16 values are these), HG3 products and 3-column sums.  HG3 rows 0 and h-1 are never
-            //      written (zero).  The products are integers whose 9-sums stay below 2^24, so int32 sums
-            //      equal the reference's float sums exactly.
-            const int g = k - 2;
-            const int gx = dA + 2 * dB + dk, gy = sk - sA;
-            int pxx = 0, pxy = 0, pyy = 0;
-            if (INNER || (ty0 - 2 + g >= 1 && ty0 - 2 + g <= h - 2)) { pxx = gx * gx; pxy = gx * gy; pyy = gy * gy; }
-            asm("" : "+v"(pxx), "+v"(pxy), "+v"(pyy));   // keep the window sums as shifted adds (not mads)
-            const int hxx2 = pxx + lane_next(pxx + lane_next(pxx));
-            const int hxy2 = pxy + lane_next(pxy + lane_next(pxy));
-            const int hyy2 = pyy + lane_next(pyy + lane_next(pyy));
+        // ---- Sobel gradient row g = k-2 (linear-address luma; |gx|, |gy| <= 1020, so the reference's S16
+        //      values are these), HG3 products and 3-column sums.  HG3 rows 0 and h-1 are never written
+        //      (zero).  The products are integers whose 9-sums stay below 2^24, so int32 sums equal the
+        //      reference's float sums exactly.
+        const int g = k - 2;
+        const int gx = dO + 2 * dN + dk, gy = sk - sO;
+        int pxx = 0, pxy = 0, pyy = 0;
+        if (INNER || (ty0 - 2 + g >= 1 && ty0 - 2 + g <= h - 2)) { pxx = __mul24(gx, gx); pxy = __mul24(gx, gy); pyy = __mul24(gy, gy); }
+        asm("" : "+v"(pxx), "+v"(pxy), "+v"(pyy));   // keep the window sums as shifted adds (not mads)
+        const int hxx = pxx + lane_next(pxx + lane_next(pxx));
+        const int hxy = pxy + lane_next(pxy + lane_next(pxy));
+        const int hyy = pyy + lane_next(pyy + lane_next(pyy));
 
-            // ---- Canny magnitude|sector row m = k-3: packed at the gradient's column, then moved to the
-            //      magnitude column (lane+1 -> lane)
-            uint32_t Md = 0;
-            if (k >= 3) {
-                const int m = k - 3, y = ty0 - 1 + m;
-                const uint32_t packed = (uint32_t)lane_next((int)canny_pack_f(gx, -gy));
-                if (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w)) {
-                    Md = packed;
-                } else if (y >= 1 && y <= h - 2 && tx0 == 0) {
-                    // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
-                    // r1[-1] = r0[ALIGN16(w)-1]); luma lanes 3, 4 are x = 0, 1 and rows k-2..k are y-1..y+1
-                    const int a3 = __builtin_amdgcn_readlane(y2, 3), a4 = __builtin_amdgcn_readlane(y2, 4);
-                    const int b4 = __builtin_amdgcn_readlane(y3, 4);
-                    const int c3 = __builtin_amdgcn_readlane(yk, 3), c4 = __builtin_amdgcn_readlane(yk, 4);
-                    if (x == 0) {
-                        const int t0_1 = a4 + c4 + 2 * b4;
-                        const int t1_0 = c3 - a3, t1_1 = c4 - a4;
-                        const int t1_m1 = (w % 16 == 0) ? ((int)yflat(F, P, y - 1, w - 1) + (int)yflat(F, P, y + 1, w - 1) +
-                                                           2 * (int)yflat(F, P, y, w - 1)) : 0;
-                        Md = canny_pack_f(t0_1, -(t1_m1 + 2 * t1_0 + t1_1));
-                    }
-                }
+        // ---- Harris score row k-4: Vc = mc / norm > T ? mc / norm : 0, the quotient exactly the IEEE one;
+        //      kept as its compare key, with the maximum of the key over columns lane..lane+2
+        int Vc = 0;
+        {
+            const int yv = ty0 - 1 + (k - 4);
+            float v = 0.0f;
+            if (k >= 4 && (INNER || (x >= 1 && x <= w - 2 && yv >= 1 && yv <= h - 2))) {
+                const float gx2 = (float)(xx0 + xx1 + hxx);
+                const float gxy2 = (float)(xy0 + xy1 + hxy);
+                const float gy2 = (float)(yy0 + yy1 + hyy);
+                const float tr = gx2 + gy2;
+                const float det = (gx2 * gy2) - (gxy2 * gxy2);
+                const float mc = det - (P.k * tr * tr);
+                const float q = div_by_const_rn(mc, P.norm, P.normRcp);
+                v = (q > P.thr) ? q : 0.0f;
             }
-            // ---- Harris score row k-4: Vc = mc / norm > T ? mc / norm : 0, the quotient exactly the IEEE
-            //      one; kept as its compare key, with the maximum of the key over columns lane..lane+2
-            int Vc = 0;
-            if (k >= 4) {
-                const int yv = ty0 - 1 + (k - 4);
-                float v = 0.0f;
-                if (INNER || (x >= 1 && x <= w - 2 && yv >= 1 && yv <= h - 2)) {
-                    const float gx2 = (float)(hxx0 + hxx1 + hxx2);
-                    const float gxy2 = (float)(hxy0 + hxy1 + hxy2);
-                    const float gy2 = (float)(hyy0 + hyy1 + hyy2);
-                    const float tr = gx2 + gy2;
-                    const float det = (gx2 * gy2) - (gxy2 * gxy2);
-                    const float mc = det - (P.k * tr * tr);
-                    const float q = div_by_const_rn(mc, P.norm, P.normRcp);
-                    v = (q > P.thr) ? q : 0.0f;
-                }
-                Vc = __float_as_int(v);
-                if (negKeys) Vc ^= (Vc >> 31) & 0x7fffffff;
-            }
-            int Hc = max(Vc, lane_next(Vc));
-            Hc = max(Vc, lane_next(Hc));
-            // ---- NMS of tile row r = k-6 (image row ty0+r): magnitude rows r..r+2 = Ma, Mb, Mc and score
-            //      rows r..r+2 = Va, Vb, Vc, columns lane..lane+2, centre lane+1
-            if (k >= 6) {
-                const int r = k - 6, y = ty0 + r;
-                const bool valid = lane < TW && (INNER || (y >= 1 && y <= h - 2 && ox >= 1 && ox <= w - 2));
-                // Canny: the larger of the two neighbours along the sector's direction, packed values
-                // compared through (v | 3): mag > (n >> 2) <=> (centre | 3) > (n | 3)
-                const uint32_t a1 = (uint32_t)lane_next((int)Ma), b1 = (uint32_t)lane_next((int)Mb);
-                const uint32_t c1 = (uint32_t)lane_next((int)Mc);
-                const uint32_t sec = b1 & 3u;
-                const uint32_t d0 = max(Mb, (uint32_t)lane_next((int)b1));     // (r+1, l), (r+1, l+2)
-                const uint32_t d1 = max(Mc, (uint32_t)lane_next((int)a1));     // (r, l+2), (r+2, l)
-                const uint32_t d2 = max(a1, c1);                               // (r, l+1), (r+2, l+1)
-                const uint32_t d3 = max(Ma, (uint32_t)lane_next((int)c1));     // (r, l), (r+2, l+2)
-                const uint32_t nmax = sec == 0 ? d0 : sec == 1 ? d1 : sec == 2 ? d2 : d3;
-                const int edge = ((b1 | 3u) > (nmax | 3u)) ? (int)(b1 >> 2) : 0;
-                const bool strongPx = valid && edge > U, edgePx = valid && (edge > U || edge > Lo);
-                const uint64_t eb = __ballot(edgePx), sb = __ballot(strongPx);
-                if (lane == r) { emLo = (uint32_t)eb; emHi = (uint32_t)(eb >> 32); smLo = (uint32_t)sb; smHi = (uint32_t)(sb >> 32); }
-                // Harris 3x3 NMS: >= towards the upper-left half of the window, > towards the lower-right
-                const int c = lane_next(Vb);
-                const bool isMax = valid && c >= max(Ha, Vb) && c > max(Hc, lane_next(c));
-                const uint64_t cb = __ballot(isMax);
-                if (isMax) {
-                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cb, 0u));
-                    const uint32_t bits = negKeys ? (uint32_t)(c ^ ((c >> 31) & 0x7fffffff)) : (uint32_t)c;
-                    cand[nc + below] = (uint64_t)(uint16_t)ox | ((uint64_t)(uint16_t)y << 16) | ((uint64_t)bits << 32);
-                }
-                nc += (uint32_t)__popcll(cb);
-            }
-            Ma = Mb; Mb = Mc; Mc = Md;
-            Va = Vb; Vb = Vc; Ha = Hb; Hb = Hc;
-            hxx0 = hxx1; hxy0 = hxy1; hyy0 = hyy1;
-            hxx1 = hxx2; hxy1 = hxy2; hyy1 = hyy2;
+            Vc = __float_as_int(v);
+            if (negKeys) Vc ^= (Vc >> 31) & 0x7fffffff;
         }
-        y0 = y1; y1 = y2; y2 = y3; y3 = yk;
-        dA = dB; dB = dk; sA = sB; sB = sk;
+        int Hc = max(Vc, lane_next(Vc));
+        Hc = max(Vc, lane_next(Hc));
+
+        // ---- Canny magnitude|sector row m = k-4 from gradient row k-3 (packed at its own column, moved
+        //      to the magnitude column: lane+1 -> lane)
+        uint32_t Mc = 0;
+        {
+            const int m = k - 4, y = ty0 - 1 + m;
+            const uint32_t shifted = (uint32_t)lane_next((int)packedPrev);
+            if (k >= 4 && (INNER || (y >= 1 && y <= h - 2 && x >= 1 && x < w))) {
+                Mc = shifted;
+            } else if (!INNER && k >= 4 && y >= 1 && y <= h - 2 && tx0 == 0) {
+                // CannySobel column 0 (ago_haf_cpu_canny.cpp:83-116 with the staged rows r0[-1] = 0 and
+                // r1[-1] = r0[ALIGN16(w)-1]); luma lanes 3, 4 are x = 0, 1 and rows k-3..k-1 are y-1..y+1
+                const int a3 = __builtin_amdgcn_readlane(yB, 3), a4 = __builtin_amdgcn_readlane(yB, 4);
+                const int b4 = __builtin_amdgcn_readlane(yC, 4);
+                const int c3 = __builtin_amdgcn_readlane(yD, 3), c4 = __builtin_amdgcn_readlane(yD, 4);
+                if (x == 0) {
+                    const int t0_1 = a4 + c4 + 2 * b4;
+                    const int t1_0 = c3 - a3, t1_1 = c4 - a4;
+                    const int t1_m1 = (w % 16 == 0) ? ((int)yflat(F, P, y - 1, w - 1) + (int)yflat(F, P, y + 1, w - 1) +
+                                                       2 * (int)yflat(F, P, y, w - 1)) : 0;
+                    Mc = canny_pack_f(t0_1, -(t1_m1 + 2 * t1_0 + t1_1));
+                }
+            }
+        }
+        packedPrev = canny_pack_f(gx, -gy);
+
+        // ---- NMS of tile row r = k-6 (image row ty0+r): magnitude rows r..r+2 = MO, MN, Mc and score
+        //      rows r..r+2 (maxima HO, raw Vb, maxima Hc), columns lane..lane+2, centre lane+1
+        if (k >= 6) {
+            const int r = k - 6, y = ty0 + r;
+            const bool valid = lane < TW && (INNER || (y >= 1 && y <= h - 2 && ox >= 1 && ox <= w - 2));
+            // Canny: the larger of the two neighbours along the sector's direction, packed values compared
+            // through (v | 3): mag > (n >> 2) <=> (centre | 3) > (n | 3)
+            const uint32_t a1 = (uint32_t)lane_next((int)MO), b1 = (uint32_t)lane_next((int)MN);
+            const uint32_t c1 = (uint32_t)lane_next((int)Mc);
+            const uint32_t sec = b1 & 3u;
+            const uint32_t d0 = max(MN, (uint32_t)lane_next((int)b1));     // (r+1, l), (r+1, l+2)
+            const uint32_t d1 = max(Mc, (uint32_t)lane_next((int)a1));     // (r, l+2), (r+2, l)
+            const uint32_t d2 = max(a1, c1);                               // (r, l+1), (r+2, l+1)
+            const uint32_t d3 = max(MO, (uint32_t)lane_next((int)c1));     // (r, l), (r+2, l+2)
+            const uint32_t nmax = sec == 0 ? d0 : sec == 1 ? d1 : sec == 2 ? d2 : d3;
+            const int edge = ((b1 | 3u) > (nmax | 3u)) ? (int)(b1 >> 2) : 0;
+            const bool strongPx = valid && edge > U, edgePx = valid && (edge > U || edge > Lo);
+            const uint64_t eb = __ballot(edgePx), sb = __ballot(strongPx);
+            if (lane == r) { emLo = (uint32_t)eb; emHi = (uint32_t)(eb >> 32); smLo = (uint32_t)sb; smHi = (uint32_t)(sb >> 32); }
+            // Harris 3x3 NMS: >= towards the upper-left half of the window, > towards the lower-right
+            const int c = lane_next(Vb);
+            const bool isMax = valid && c >= max(HO, Vb) && c > max(Hc, lane_next(c));
+            const uint64_t cb = __ballot(isMax);
+            if (isMax) {
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cb, 0u));
+                const uint32_t bits = negKeys ? (uint32_t)(c ^ ((c >> 31) & 0x7fffffff)) : (uint32_t)c;
+                cand[nc + below] = (uint64_t)(uint16_t)ox | ((uint64_t)(uint16_t)y << 16) | ((uint64_t)bits << 32);
+            }
+            nc += (uint32_t)__popcll(cb);
+        }
+        // the new rows over the oldest ones
+        yA = yk; dO = dk; sO = sk;
+        xx0 = hxx; xy0 = hxy; yy0 = hyy;
+        MO = Mc; HO = Hc; Vb = Vc;
+    };
+
+    uint32_t pf0 = 0, pf1 = 0;
+    if (INNER) { pf0 = fetch(0); pf1 = fetch(1); }
+    int y0 = 0, y1 = 0, y2 = 0, y3 = 0, dA = 0, dB = 0, sA = 0, sB = 0;
+    int xx0 = 0, xx1 = 0, xy0 = 0, xy1 = 0, yy0 = 0, yy1 = 0, Ha = 0, Hb = 0;
+    uint32_t Ma = 0, Mb = 0;
+    static_assert(YH % 4 == 2, "four steps per loop trip, two after it");
+    int k = 0;
+    for (; k + 4 <= YH; k += 4) {
+        step(k, pf0, pf1, y0, y1, y2, y3, dA, dB, sA, sB, xx0, xx1, xy0, xy1, yy0, yy1, Ma, Mb, Ha, Hb);
+        step(k + 1, pf1, pf0, y1, y2, y3, y0, dB, dA, sB, sA, xx1, xx0, xy1, xy0, yy1, yy0, Mb, Ma, Hb, Ha);
+        step(k + 2, pf0, pf1, y2, y3, y0, y1, dA, dB, sA, sB, xx0, xx1, xy0, xy1, yy0, yy1, Ma, Mb, Ha, Hb);
+        step(k + 3, pf1, pf0, y3, y0, y1, y2, dB, dA, sB, sA, xx1, xx0, xy1, xy0, yy1, yy0, Mb, Ma, Hb, Ha);
     }
+    step(k, pf0, pf1, y0, y1, y2, y3, dA, dB, sA, sB, xx0, xx1, xy0, xy1, yy0, yy1, Ma, Mb, Ha, Hb);
+    step(k + 1, pf1, pf0, y1, y2, y3, y0, dB, dA, sB, sA, xx1, xx0, xy1, xy0, yy1, yy0, Mb, Ma, Hb, Ha);
 
     // ---- Harris candidates: one global atomic per tile, then the buffer copied out
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -419,12 +449,12 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
     const uint64_t bmAll = lane < 32 ? (other & ~grown) : 0ull;     // lane r < 32: undecided
 
     //  (b) settle: reached -> 255; undecided -> 127 (k_resolve decides them); other edge pixels -> 0
-    gptr_t<uint8_t> eo = gmem(F.edges) + (long)ty0 * F.edgesStride + ox;
     for (int r = 0; r < th; r++) {
         const uint64_t R = readlane64(rm, r), B = readlane64(bmAll, r);
         const int y = ty0 + r;
         if (lane < TW && (INNER || (y >= 1 && y <= h - 2 && ox >= 1 && ox <= w - 2)))
-            eo[(long)r * F.edgesStride] = ((R >> lane) & 1ull) ? 255 : (((B >> lane) & 1ull) ? 127 : 0);
+            __builtin_amdgcn_raw_buffer_store_b8(((R >> lane) & 1ull) ? 255 : (((B >> lane) & 1ull) ? 127 : 0), edR, ox,
+                                                 (int)((uint32_t)y * F.edgesStride), 0);
     }
 
     //  (c) the undecided components, one at a time: the first set bit in row-major order is the
@@ -482,9 +512,10 @@ k_front(const FrameDesc *__restrict__ frames, GraphParams P)
     __shared__ uint64_t candLds[FTW][CAND_MAX];
     const FrameDesc F = frames[blockIdx.z];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int tile = (int)blockIdx.x * FTW + wave;
-    if (tile >= P.tilesX * P.tilesY) return;
-    const int tx0 = (tile % P.tilesX) * TW, ty0 = (tile / P.tilesX) * TH;
+    const int tX = (int)blockIdx.x * FTW + wave, tY = (int)blockIdx.y;
+    if (tX >= P.tilesX) return;
+    const int tile = tY * P.tilesX + tX;
+    const int tx0 = tX * TW, ty0 = tY * TH;
     const bool inner = tx0 >= 4 && tx0 + TW + 4 <= P.w && ty0 >= 4 && ty0 + TH + 4 <= P.h;
     if (inner) front_tile<true>(candLds[wave], F, P, tx0, ty0, tile);
     else front_tile<false>(candLds[wave], F, P, tx0, ty0, tile);
@@ -740,7 +771,7 @@ extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_p
     hipStream_t s = stream_of(stream);
     switch (stage) {
     case 0:
-        hipLaunchKernelGGL(k_front, dim3((P.tilesX * P.tilesY + FTW - 1) / FTW, 1, nframes), dim3(64 * FTW), 0, s, D, P);
+        hipLaunchKernelGGL(k_front, dim3((P.tilesX + FTW - 1) / FTW, P.tilesY, nframes), dim3(64 * FTW), 0, s, D, P);
         break;
     case 1:
         hipLaunchKernelGGL(k_border, dim3((P.tilesX * P.tilesY + 3) / 4, 1, nframes), dim3(256), 0, s, D, P);
