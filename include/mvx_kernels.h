@@ -35,6 +35,7 @@ typedef void *mvx_stream_t;
 #define MVX_SUCCESS                   0
 #define MVX_FAILURE                 (-1)   /* VX_FAILURE */
 #define MVX_ERROR_NOT_IMPLEMENTED   (-2)   /* VX_ERROR_NOT_IMPLEMENTED */
+#define MVX_ERROR_NOT_SUPPORTED     (-3)   /* VX_ERROR_NOT_SUPPORTED */
 #define MVX_ERROR_NO_MEMORY         (-8)   /* VX_ERROR_NO_MEMORY */
 #define MVX_ERROR_INVALID_PARAMETERS (-10) /* VX_ERROR_INVALID_PARAMETERS */
 #define MVX_ERROR_LAUNCH            MVX_FAILURE
@@ -97,6 +98,36 @@ int mvx_ChannelCopy_U8_U8(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHe
 int mvx_ScaleImage_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
                                   uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
                                   const uint8_t *pSrc, uint32_t srcStride);
+
+/* replaces HipExec_ScaleImage_U8_U8_Nearest (hip_host_decls.h:723); HafCpu_ScaleImage_U8_U8_Nearest
+ * (ago_haf_cpu_geometric.cpp:1874-1976) with offsets ratio/2 (ago_kernel_api.cpp:19790-19793).
+ * Sizes up to 65535 (the reference's 16-bit maps); dstStride a multiple of 16. */
+int mvx_ScaleImage_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                 uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                 const uint8_t *pSrc, uint32_t srcStride);
+
+/* replace HipExec_ScaleImage_U8_U8_Bilinear_Replicate / _Constant (hip_host_decls.h:733, :738);
+ * HafCpu_ScaleImage_U8_U8_Bilinear_Replicate (ago_haf_cpu_geometric.cpp:2108-2258) and _Constant
+ * (:2260-2391), scale matrix of ago_kernel_api.cpp:20017-20020 / :20147-20150, including the
+ * reference's split between its two-stage-rounded SIMD columns and single-rounded tail columns.
+ * srcWidth >= 2; dstStride a multiple of 16. */
+int mvx_ScaleImage_U8_U8_Bilinear_Replicate(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                            uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                            const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ScaleImage_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                           uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                           const uint8_t *pSrc, uint32_t srcStride, uint8_t borderValue);
+
+/* replaces HipExec_ScaleImage_U8_U8_Area (hip_host_decls.h:744); HafCpu_ScaleImage_U8_U8_Area
+ * (ago_haf_cpu_geometric.cpp:2491-2617), offsets -1/2 (ago_kernel_api.cpp:20276-20277), as a
+ * reference graph runs it after the optimizer's rewrite (ago_drama_remove.cpp:1270-1311): a plane
+ * copy at the same size, nearest neighbour unless both dimensions shrink, and the 2x2 average
+ * ((a+b+c+d+2)>>2, columns below dstWidth & ~7 only; dstWidth >= 8) at an exact 2x.  Other
+ * shrinking ratios return MVX_ERROR_NOT_SUPPORTED: the reference's generic path stores its row sums
+ * past the end of its node buffer, so its result is not defined. */
+int mvx_ScaleImage_U8_U8_Area(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                              uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                              const uint8_t *pSrc, uint32_t srcStride);
 
 /* replaces HipExec_ScaleGaussianHalf_U8_U8_5x5 (hip_host_decls.h:681); HafCpu_ScaleGaussianHalf_U8_U8_5x5
  * (ago_haf_cpu_pyramid.cpp:94-305) as called by ago_kernel_api.cpp:15420-15436 (rows 1..h-2, parity

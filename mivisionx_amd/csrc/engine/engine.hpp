@@ -89,7 +89,11 @@ enum : vx_enum {
     AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
     AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,
     AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM = 0xd000da,
+    AMD_SCALE_IMAGE_U8_U8_NEAREST = 0xd000f0,
     AMD_SCALE_IMAGE_U8_U8_BILINEAR = 0xd000f1,
+    AMD_SCALE_IMAGE_U8_U8_BILINEAR_REPLICATE = 0xd000f2,
+    AMD_SCALE_IMAGE_U8_U8_BILINEAR_CONSTANT = 0xd000f3,
+    AMD_SCALE_IMAGE_U8_U8_AREA = 0xd000f4,
 };
 
 // what executes a node: its own execute command, or one of the graph's supernodes

@@ -186,6 +186,11 @@ static FusedVision *find_fusion(_vx_graph *g)
             *slot = n;
         }
         if (!ok || !f.scale || !f.pyr || !f.canny || !f.harris) continue;
+        {
+            // the tile pass computes the bilinear, border-UNDEFINED scaling only
+            _vx_scalar *interp = is_ref(f.scale->params[2], VX_TYPE_SCALAR) ? (_vx_scalar *)f.scale->params[2] : nullptr;
+            if (!interp || interp->value.e != VX_INTERPOLATION_BILINEAR || f.scale->border.mode != VX_BORDER_UNDEFINED) continue;
+        }
         if (((_vx_pyramid *)f.pyr->params[1])->levels > 8) continue;
         fused_params(&f);
         if (mvx_VisionGraph_supported(&f.params) != 1) continue;
@@ -276,7 +281,7 @@ static bool scale_io(_vx_node *n, _vx_image **in, _vx_image **out)
 {
     if (n->kernel->id == VX_KERNEL_SCALE_IMAGE) {
         _vx_scalar *interp = is_ref(n->params[2], VX_TYPE_SCALAR) ? (_vx_scalar *)n->params[2] : nullptr;
-        if (!interp || interp->value.e != VX_INTERPOLATION_BILINEAR) return false;
+        if (!interp || interp->value.e != VX_INTERPOLATION_BILINEAR || n->border.mode != VX_BORDER_UNDEFINED) return false;
         *in = image_of(n->params[0]); *out = image_of(n->params[1]);
         return true;
     }

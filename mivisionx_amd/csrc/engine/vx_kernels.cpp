@@ -224,18 +224,48 @@ vx_status x_amd_copy(_vx_node *n, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------- ScaleImage (A10) ---------
-// scale_image bilinear -> SCALE_IMAGE_U8_U8_BILINEAR, same size -> CHANNEL_COPY (ago_drama_divide.cpp:563-616)
+// scale_image (ago_drama_divide.cpp:563-616): same size -> CHANNEL_COPY; otherwise by interpolation
+// and the node's border mode: NEAREST_NEIGHBOR -> SCALE_IMAGE_U8_U8_NEAREST (any border); BILINEAR ->
+// _BILINEAR (UNDEFINED), _BILINEAR_REPLICATE, _BILINEAR_CONSTANT (with the border value); AREA ->
+// _AREA (any border).  The reference's optimizer then makes a same-size area scale a copy and one
+// that does not shrink both dimensions nearest (ago_drama_remove.cpp:1270-1311); of the shrinking
+// ratios only an exact 2x has a defined result there
+// the sizes each ScaleImage kernel provides: kind 0 nearest, 1 bilinear, 2 bilinear replicate,
+// 3 bilinear constant, 4 area
+static vx_status scale_supported(_vx_node *n, const _vx_image *in, const _vx_image *o, int kind)
+{
+    if (in->width == o->width && in->height == o->height) return VX_SUCCESS;   // a plane copy
+    if (kind == 4 && in->width > o->width && in->height > o->height) {
+        const float xs = (float)((double)in->width / (double)o->width), ys = (float)((double)in->height / (double)o->height);
+        if (!(xs == 2.0f && ys == 2.0f && o->width >= 8))
+            return fail(n, VX_ERROR_NOT_SUPPORTED, "a shrinking AREA scale is provided for an exact 2x (output width >= 8) "
+                                                   "only: the reference's generic area path has no defined result");
+        return VX_SUCCESS;
+    }
+    if ((kind == 0 || kind == 4) && (in->width > 65535 || in->height > 65535))
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "nearest-neighbour sources above 65535 pixels (16-bit maps)");
+    if ((kind == 2 || kind == 3) && in->width < 2)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "bilinear border forms need a source at least 2 pixels wide");
+    return VX_SUCCESS;
+}
+
 vx_status v_scale(_vx_node *n)
 {
     vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
     if (s) return s;
-    _vx_image *o = img(n, 1);
+    _vx_image *in = img(n, 0), *o = img(n, 1);
     _vx_scalar *type = scl(n, 2);
     if (!o || !type) return fail(n, VX_ERROR_INVALID_PARAMETERS, "arguments");
-    if (type->value.e != VX_INTERPOLATION_BILINEAR)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "only BILINEAR interpolation is provided");
-    if ((s = border_undefined(n))) return s;
+    const vx_enum t = type->value.e;
+    if (t != VX_INTERPOLATION_NEAREST_NEIGHBOR && t != VX_INTERPOLATION_BILINEAR && t != VX_INTERPOLATION_AREA)
+        return fail(n, VX_ERROR_INVALID_VALUE, "interpolation type");
+    if (n->border.mode != VX_BORDER_UNDEFINED && n->border.mode != VX_BORDER_REPLICATE && n->border.mode != VX_BORDER_CONSTANT)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "border mode");
     if (o->width == 0 || o->height == 0) return fail(n, VX_ERROR_INVALID_DIMENSION, "output size must be set");
+    const int kind = t == VX_INTERPOLATION_NEAREST_NEIGHBOR ? 0 : t == VX_INTERPOLATION_AREA ? 4
+                   : n->border.mode == VX_BORDER_REPLICATE ? 2 : n->border.mode == VX_BORDER_CONSTANT ? 3 : 1;
+    s = scale_supported(n, in, o, kind);
+    if (s) return s;
     return set_output(n, 1, o->width, o->height, VX_DF_IMAGE_U8);
 }
 
@@ -244,6 +274,17 @@ vx_status x_scale(_vx_node *n, hipStream_t st)
     _vx_image *in = img(n, 0), *o = img(n, 1);
     if (in->width == o->width && in->height == o->height)
         return rc(n, mvx_ChannelCopy_U8_U8(st, o->width, o->height, P(o), S(o), P(in), S(in)));
+    const vx_enum t = scl(n, 2)->value.e;
+    if (t == VX_INTERPOLATION_NEAREST_NEIGHBOR)
+        return rc(n, mvx_ScaleImage_U8_U8_Nearest(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+    if (t == VX_INTERPOLATION_AREA)
+        return rc(n, mvx_ScaleImage_U8_U8_Area(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+    if (n->border.mode == VX_BORDER_REPLICATE)
+        return rc(n, mvx_ScaleImage_U8_U8_Bilinear_Replicate(st, o->width, o->height, P(o), S(o), in->width, in->height,
+                                                             P(in), S(in)));
+    if (n->border.mode == VX_BORDER_CONSTANT)
+        return rc(n, mvx_ScaleImage_U8_U8_Bilinear_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height,
+                                                            P(in), S(in), n->border.constant_value.U8));
     return rc(n, mvx_ScaleImage_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
 }
 
@@ -256,10 +297,29 @@ vx_status v_amd_scale(_vx_node *n)
     return set_output(n, 0, o->width, o->height, VX_DF_IMAGE_U8);
 }
 
-vx_status x_amd_scale(_vx_node *n, hipStream_t st)
+// AMD forms (out, in[, border scalar]): one ScaleImage kernel each
+template <int KIND> vx_status x_amd_scale(_vx_node *n, hipStream_t st)
 {
     _vx_image *o = img(n, 0), *in = img(n, 1);
-    return rc(n, mvx_ScaleImage_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+    const uint32_t dw = o->width, dh = o->height, sw = in->width, sh = in->height;
+    // every AMD ScaleImage kernel runs as a plane copy at the same size (ago_drama_remove.cpp:1283-1288)
+    if (dw == sw && dh == sh) return rc(n, mvx_ChannelCopy_U8_U8(st, dw, dh, P(o), S(o), P(in), S(in)));
+    switch (KIND) {
+    case 0: return rc(n, mvx_ScaleImage_U8_U8_Nearest(st, dw, dh, P(o), S(o), sw, sh, P(in), S(in)));
+    case 1: return rc(n, mvx_ScaleImage_U8_U8_Bilinear(st, dw, dh, P(o), S(o), sw, sh, P(in), S(in)));
+    case 2: return rc(n, mvx_ScaleImage_U8_U8_Bilinear_Replicate(st, dw, dh, P(o), S(o), sw, sh, P(in), S(in)));
+    case 3: return rc(n, mvx_ScaleImage_U8_U8_Bilinear_Constant(st, dw, dh, P(o), S(o), sw, sh, P(in), S(in), scl(n, 2)->value.u8));
+    default: return rc(n, mvx_ScaleImage_U8_U8_Area(st, dw, dh, P(o), S(o), sw, sh, P(in), S(in)));
+    }
+}
+
+template <int KIND> vx_status v_amd_scale_k(_vx_node *n)
+{
+    vx_status s = v_amd_scale(n);
+    if (!s && KIND == 3 && (!scl(n, 2) || scl(n, 2)->dataType != VX_TYPE_UINT8))
+        s = fail(n, VX_ERROR_INVALID_FORMAT, "border scalar must be UINT8");
+    if (!s) s = scale_supported(n, img(n, 1), img(n, 0), KIND);
+    return s;
 }
 
 // ---------------------------------------------------------------------- GaussianPyramid (A11) ----
@@ -785,8 +845,17 @@ const std::vector<KernelDef> &kernel_table()
         {AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM, "com.amd.openvx.CannySobel_U16_U8_3x3_L1NORM",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel,
          AGO_TARGET_AFFINITY_GPU},
+        {AMD_SCALE_IMAGE_U8_U8_NEAREST, "com.amd.openvx.ScaleImage_U8_U8_Nearest",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale_k<0>, nullptr, x_amd_scale<0>, AGO_TARGET_AFFINITY_GPU},
         {AMD_SCALE_IMAGE_U8_U8_BILINEAR, "com.amd.openvx.ScaleImage_U8_U8_Bilinear",
-         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale, nullptr, x_amd_scale, AGO_TARGET_AFFINITY_GPU},
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale_k<1>, nullptr, x_amd_scale<1>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SCALE_IMAGE_U8_U8_BILINEAR_REPLICATE, "com.amd.openvx.ScaleImage_U8_U8_Bilinear_Replicate",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale_k<2>, nullptr, x_amd_scale<2>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SCALE_IMAGE_U8_U8_BILINEAR_CONSTANT, "com.amd.openvx.ScaleImage_U8_U8_Bilinear_Constant",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}}, v_amd_scale_k<3>, nullptr,
+         x_amd_scale<3>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SCALE_IMAGE_U8_U8_AREA, "com.amd.openvx.ScaleImage_U8_U8_Area",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale_k<4>, nullptr, x_amd_scale<4>, AGO_TARGET_AFFINITY_GPU},
     };
     return table;
 }

@@ -121,18 +121,144 @@ k_scale_bilinear(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t d
     }
 }
 
-ScaleParams scale_params(uint32_t dw, uint32_t dh, uint32_t sw, uint32_t sh)
+// the scale matrix of ago_kernel_api.cpp:19897-19916 (half = 0.5: offsets ratio/2 - 1/2) or of the
+// nearest form :19790-19793 (half = 0: offsets ratio/2), in fixed point with 18 fractional bits
+ScaleParams scale_params(uint32_t dw, uint32_t dh, uint32_t sw, uint32_t sh, double half = 0.5)
 {
     const float xscale = (float)((double)sw / (double)dw);
     const float yscale = (float)((double)sh / (double)dh);
-    const float xoffset = (float)((double)sw / (double)dw * 0.5 - 0.5);
-    const float yoffset = (float)((double)sh / (double)dh * 0.5 - 0.5);
+    const float xoffset = (float)((double)sw / (double)dw * 0.5 - half);
+    const float yoffset = (float)((double)sh / (double)dh * 0.5 - half);
     ScaleParams p;
     p.xinc = (int)((float)(1 << 18) * xscale);
     p.yinc = (int)((float)(1 << 18) * yscale);
     p.xoffs = (int)((float)(1 << 18) * xoffset);
     p.yoffs = (int)((float)(1 << 18) * yoffset);
     return p;
+}
+
+// ------------------------------------------------------------------ ScaleImage nearest ---------
+// HafCpu_ScaleImage_U8_U8_Nearest (ago_haf_cpu_geometric.cpp:1874-1976), offsets ratio/2
+// (ago_kernel_api.cpp:19790-19793): map = pos >> 18 clamped to the source, per axis.
+__global__ void __launch_bounds__(256)
+k_scale_nearest(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                const uint8_t *__restrict__ src, uint32_t srcStride, ScaleParams sp)
+{
+    const uint32_t x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
+    const uint32_t y = blockIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const int ym = min(max((int)((uint32_t)sp.yoffs + y * (uint32_t)sp.yinc) >> 18, 0), (int)sh - 1);
+    const uint8_t *row = src + (long)ym * srcStride;
+    uint32_t o[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int xm = min(max((int)((uint32_t)sp.xoffs + (x0 + i) * (uint32_t)sp.xinc) >> 18, 0), (int)sw - 1);
+        o[i] = (x0 + i < dw) ? row[xm] : 0u;
+    }
+    uint8_t *d = dst + (long)y * dstStride + x0;
+    if (x0 + 8 <= dw) {
+        uint2 v;
+        v.x = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+        v.y = o[4] | (o[5] << 8) | (o[6] << 16) | (o[7] << 24);
+        *reinterpret_cast<uint2 *>(d) = v;
+    } else {
+        for (uint32_t i = 0; x0 + i < dw; i++) d[i] = (uint8_t)o[i];
+    }
+}
+
+// ------------------------------------------------------------------ ScaleImage bilinear, borders -
+// HafCpu_ScaleImage_U8_U8_Bilinear_Replicate (ago_haf_cpu_geometric.cpp:2108-2258) and _Constant
+// (:2260-2391).  Rows: ym < 0 -> (row 0, row 0) | (border, row 0); ym >= sh-1 -> (sh-1, sh-1) |
+// (sh-1, border).  Columns below `split` (computed on the host exactly as the reference's
+// newDstWidth) take the SIMD two-stage rounding; the others the single rounding with the right
+// neighbour at the last source column replaced (p0 | border); on an upscale with split > 0 column 0
+// is recomputed with the left neighbour replaced.
+template <bool CONSTANT>
+__global__ void __launch_bounds__(256)
+k_scale_bilinear_border(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw,
+                        uint32_t sh, const uint8_t *__restrict__ src, uint32_t srcStride, ScaleParams sp, int split,
+                        int upscale, int border)
+{
+    const uint32_t x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
+    const uint32_t y = blockIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const int ypos = (int)((uint32_t)sp.yoffs + y * (uint32_t)sp.yinc);
+    const int ym = ypos >> 18;
+    const int yf = ((ypos & 0x3ffff) + 0x200) >> 10, omy = 0x100 - yf;
+    const uint8_t *r1, *r2;   // nullptr: the border row
+    if (ym < 0) { r1 = CONSTANT ? nullptr : src; r2 = src; }
+    else if (ym >= (int)sh - 1) { r1 = src + (long)(sh - 1) * srcStride; r2 = CONSTANT ? nullptr : r1; }
+    else { r1 = src + (long)ym * srcStride; r2 = r1 + srcStride; }
+    auto px = [&](const uint8_t *r, int c) -> int { return (CONSTANT && !r) ? border : (int)r[c]; };
+    uint32_t o[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int x = (int)x0 + i;
+        const int xpos = (int)((uint32_t)sp.xoffs + (uint32_t)x * (uint32_t)sp.xinc);
+        int xm = xpos >> 18;
+        xm = xm >= (int)sw - 1 ? (int)sw - 1 : max(xm, 0);
+        const int xf = ((xpos & 0x3ffff) + 0x200) >> 10, omx = 0x100 - xf;
+        int v = 0;
+        if ((uint32_t)x < dw) {
+            if (x == 0 && upscale && split > 0) {
+                // column 0 recomputed after the SIMD pass (column 0 of both rows; xm is 0 here)
+                int p0, p1, p2, p3;
+                if (CONSTANT) { p0 = p2 = border; p1 = px(r1, 0); p3 = px(r2, 0); }
+                else { p0 = p1 = px(r1, 0); p2 = p3 = px(r2, 0); }
+                v = (int)(((uint32_t)(omx * omy * p0) + (uint32_t)(xf * omy * p1) + (uint32_t)(omx * yf * p2) +
+                           (uint32_t)(xf * yf * p3) + 0x8000u) >> 16);
+            } else if (x < split) {
+                // below the split every map is below sw-1, so xm + 1 stays inside the row
+                const int t1 = (omx * px(r1, xm) + xf * px(r1, xm + 1) + 0x80) >> 8;
+                const int t2 = (omx * px(r2, xm) + xf * px(r2, xm + 1) + 0x80) >> 8;
+                v = (omy * t1 + yf * t2 + 0x80) >> 8;
+            } else {
+                const int p0 = px(r1, xm), p2 = px(r2, xm);
+                const bool last = xm >= (int)sw - 1;
+                const int p1 = last ? (CONSTANT ? border : p0) : px(r1, xm + 1);
+                const int p3 = last ? (CONSTANT ? border : p2) : px(r2, xm + 1);
+                v = (int)(((uint32_t)(omx * omy * p0) + (uint32_t)(xf * omy * p1) + (uint32_t)(omx * yf * p2) +
+                           (uint32_t)(xf * yf * p3) + 0x8000u) >> 16);
+            }
+        }
+        o[i] = (uint32_t)v;
+    }
+    uint8_t *d = dst + (long)y * dstStride + x0;
+    if (x0 + 8 <= dw) {
+        uint2 v;
+        v.x = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+        v.y = o[4] | (o[5] << 8) | (o[6] << 16) | (o[7] << 24);
+        *reinterpret_cast<uint2 *>(d) = v;
+    } else {
+        for (uint32_t i = 0; x0 + i < dw; i++) d[i] = (uint8_t)o[i];
+    }
+}
+
+// ------------------------------------------------------------------ ScaleImage area (2x) --------
+// HafCpu_ScaleImage_U8_U8_Area, exact 2x form (ago_haf_cpu_geometric.cpp:2518-2545, offsets -1/2):
+// output row y averages source rows S = (int)(2y - 0.5f) clamped to sh-2, and S+1, in 2x2 blocks,
+// (a + b + c + d + 2) >> 2, columns below dw & ~7 (the rest are not written).  One thread: 8
+// outputs from 16 source bytes of each row (16-byte aligned: rows are, and 2x is a multiple of 16).
+__global__ void __launch_bounds__(256)
+k_scale_area2(uint32_t dw8, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sh,
+              const uint8_t *__restrict__ src, uint32_t srcStride)
+{
+    const uint32_t x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
+    const uint32_t y = blockIdx.y;
+    if (x0 >= dw8 || y >= dh) return;
+    int r = (int)(-0.5f + (float)(2u * y));
+    r = min(max(r, 0), (int)sh - 2);
+    const uint4 a = ld16(src + (long)r * srcStride + 2 * x0), b = ld16(src + (long)(r + 1) * srcStride + 2 * x0);
+    uint32_t w0 = 0, w1 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t v = (byte_of(a, 2 * i) + byte_of(a, 2 * i + 1) + byte_of(b, 2 * i) + byte_of(b, 2 * i + 1) + 2u) >> 2;
+        if (i < 4) w0 |= v << (8 * i);
+        else w1 |= v << (8 * (i - 4));
+    }
+    uint2 v;
+    v.x = w0; v.y = w1;
+    *reinterpret_cast<uint2 *>(dst + (long)y * dstStride + x0) = v;
 }
 
 // ------------------------------------------------------------------ ScaleGaussianHalf 5x5 ------
@@ -185,6 +311,96 @@ extern "C" int mvx_ScaleImage_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dw, u
                        srcStride, sp);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
+}
+
+extern "C" int mvx_ScaleImage_U8_U8_Nearest(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst,
+                                           uint32_t dstStride, uint32_t sw, uint32_t sh, const uint8_t *src,
+                                           uint32_t srcStride)
+{
+    if (dw == 0 || dh == 0) return MVX_SUCCESS;
+    if (sw == 0 || sh == 0 || sw > 65535 || sh > 65535 || !plane_ok_g(dst, dw, dstStride, 1) ||
+        !plane_ok_g(src, sw, srcStride, 1) || (dstStride & 7u))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    ScaleParams sp = scale_params(dw, dh, sw, sh, 0.0);
+    dim3 grid(((dw + 7) / 8 + 255) / 256, dh);
+    hipLaunchKernelGGL(k_scale_nearest, grid, dim3(256), 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src,
+                       srcStride, sp);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+namespace {
+// newDstWidth of the bilinear border forms (ago_haf_cpu_geometric.cpp:2133-2155): (first column whose
+// map reaches sw-1) - 1, the value 0 counting as unset so a later column sets it again; 0 when none
+// does; rounded down to a multiple of 8 when dw is not one
+int border_split(uint32_t dw, uint32_t sw, const ScaleParams &sp)
+{
+    uint32_t nd = 0;
+    for (uint32_t x = 0; x < dw; x++)
+        if (((int)((uint32_t)sp.xoffs + x * (uint32_t)sp.xinc) >> 18) >= (int)sw - 1 && !nd) nd = x - 1u;
+    if (dw & 7u) nd &= ~7u;
+    return (int)nd;
+}
+
+int scale_bilinear_border(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw,
+                          uint32_t sh, const uint8_t *src, uint32_t srcStride, bool constant, uint8_t border)
+{
+    if (dw == 0 || dh == 0) return MVX_SUCCESS;
+    // a source of one column makes the reference's column split wrap below zero (it then runs its
+    // 8-wide loop past the row); widths of at least 2 keep every access inside the source row
+    if (sw < 2 || sh == 0 || !plane_ok_g(dst, dw, dstStride, 1) || !plane_ok_g(src, sw, srcStride, 1) || (dstStride & 7u))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    const ScaleParams sp = scale_params(dw, dh, sw, sh);
+    const int split = border_split(dw, sw, sp);
+    const int upscale = (float)((double)sw / (double)dw) < 1.0f ? 1 : 0;
+    dim3 grid(((dw + 7) / 8 + 255) / 256, dh);
+    if (constant)
+        hipLaunchKernelGGL(k_scale_bilinear_border<true>, grid, dim3(256), 0, stream_of(stream), dw, dh, dst, dstStride,
+                           sw, sh, src, srcStride, sp, split, upscale, (int)border);
+    else
+        hipLaunchKernelGGL(k_scale_bilinear_border<false>, grid, dim3(256), 0, stream_of(stream), dw, dh, dst, dstStride,
+                           sw, sh, src, srcStride, sp, split, upscale, 0);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+}  // namespace
+
+extern "C" int mvx_ScaleImage_U8_U8_Bilinear_Replicate(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst,
+                                                       uint32_t dstStride, uint32_t sw, uint32_t sh, const uint8_t *src,
+                                                       uint32_t srcStride)
+{
+    return scale_bilinear_border(stream, dw, dh, dst, dstStride, sw, sh, src, srcStride, false, 0);
+}
+
+extern "C" int mvx_ScaleImage_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst,
+                                                      uint32_t dstStride, uint32_t sw, uint32_t sh, const uint8_t *src,
+                                                      uint32_t srcStride, uint8_t border)
+{
+    return scale_bilinear_border(stream, dw, dh, dst, dstStride, sw, sh, src, srcStride, true, border);
+}
+
+extern "C" int mvx_ScaleImage_U8_U8_Area(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst,
+                                         uint32_t dstStride, uint32_t sw, uint32_t sh, const uint8_t *src,
+                                         uint32_t srcStride)
+{
+    if (dw == 0 || dh == 0) return MVX_SUCCESS;
+    if (sw == 0 || sh == 0 || !plane_ok_g(dst, dw, dstStride, 1) || !plane_ok_g(src, sw, srcStride, 1) || (dstStride & 7u))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    // the reference's graph optimizer turns a same-size scale into a plane copy and an area scale that
+    // does not shrink both dimensions into nearest neighbour (ago_drama_remove.cpp:1270-1311)
+    if (sw == dw && sh == dh) return mvx_ChannelCopy_U8_U8(stream, dw, dh, dst, dstStride, src, srcStride);
+    if (!(sw > dw && sh > dh)) return mvx_ScaleImage_U8_U8_Nearest(stream, dw, dh, dst, dstStride, sw, sh, src, srcStride);
+    const float xs = (float)((double)sw / (double)dw), ys = (float)((double)sh / (double)dh);
+    if (xs == 2.0f && ys == 2.0f && dw >= 8 && sh >= 2) {
+        const uint32_t dw8 = dw & ~7u;
+        dim3 grid((dw8 / 8 + 255) / 256, dh);
+        hipLaunchKernelGGL(k_scale_area2, grid, dim3(256), 0, stream_of(stream), dw8, dh, dst, dstStride, sh, src, srcStride);
+        MVX_LAUNCH_CHECK();
+        return MVX_SUCCESS;
+    }
+    // the reference's generic area path keeps its row sums past the end of its node buffer, so its
+    // result is not defined (ago_kernel_api.cpp:20269 vs ago_haf_cpu_geometric.cpp:2553)
+    return MVX_ERROR_NOT_SUPPORTED;
 }
 
 extern "C" int mvx_ScaleGaussianHalf_U8_U8_5x5(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst,

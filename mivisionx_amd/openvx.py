@@ -53,6 +53,7 @@ VX_MEMORY_TYPE_HOST = _enum(0, 0x0E) + 1
 VX_MEMORY_TYPE_HIP = _enum(0, 0x0E) + 3
 VX_INTERPOLATION_NEAREST_NEIGHBOR = _enum(0, 0x04) + 0
 VX_INTERPOLATION_BILINEAR = _enum(0, 0x04) + 1
+VX_INTERPOLATION_AREA = _enum(0, 0x04) + 2
 VX_CHANNEL_Y = _enum(0, 0x09) + 0x14
 VX_THRESHOLD_TYPE_BINARY = _enum(0, 0x0B) + 0
 VX_THRESHOLD_TYPE_RANGE = _enum(0, 0x0B) + 1

@@ -83,6 +83,162 @@ void oracle_scale_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw,
     }
 }
 
+/* The scale matrix of ago_kernel_api.cpp:19757-19800 (nearest: offset = ratio/2), :19897-19916
+ * (bilinear forms: offset = ratio/2 - 1/2) and :20242-20275 (area: offset = -1/2), in float from
+ * double arithmetic; the fixed-point forms truncate 2^18 * value toward zero.  Positions advance by
+ * repeated 32-bit additions (ago_haf_cpu_geometric.cpp:1897-1918), i.e. modulo 2^32. */
+static void scale_matrix(int dw, int dh, int sw, int sh, double half, int *xinc, int *yinc, int *xoffs, int *yoffs,
+                         float *xscale, float *yscale)
+{
+    *xscale = (float)((double)sw / (double)dw);
+    *yscale = (float)((double)sh / (double)dh);
+    const float xoffset = (float)((double)sw / (double)dw * 0.5 - half);
+    const float yoffset = (float)((double)sh / (double)dh * 0.5 - half);
+    *xinc = (int)((float)(1 << 18) * *xscale);
+    *yinc = (int)((float)(1 << 18) * *yscale);
+    *xoffs = (int)((float)(1 << 18) * xoffset);
+    *yoffs = (int)((float)(1 << 18) * yoffset);
+}
+
+static inline int pos_at(int offs, int inc, int i) { return (int)((uint32_t)offs + (uint32_t)inc * (uint32_t)i); }
+
+/* ScaleImage_U8_U8_Nearest: ago_haf_cpu_geometric.cpp:1874-1976 (offset ratio/2, no -1/2).
+ *   map = pos >> 18 clamped to [0, size-1] per axis; out = src[ymap][xmap].
+ * Pinned by: bit-exact against oracle/_ref (up- and downscale, odd sizes); integer ratios give the
+ * closed form out(x,y) = src(k x + k/2, k y + k/2) for an exact downscale by k. */
+void oracle_scale_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                          int src_stride)
+{
+    int xinc, yinc, xoffs, yoffs;
+    float xs, ys;
+    scale_matrix(dw, dh, sw, sh, 0.0, &xinc, &yinc, &xoffs, &yoffs, &xs, &ys);
+    for (int y = 0; y < dh; y++) {
+        int ym = pos_at(yoffs, yinc, y) >> 18;
+        if (ym > sh - 1) ym = sh - 1;
+        if (ym < 0) ym = 0;
+        for (int x = 0; x < dw; x++) {
+            int xm = pos_at(xoffs, xinc, x) >> 18;
+            if (xm > sw - 1) xm = sw - 1;
+            if (xm < 0) xm = 0;
+            dst[(long)y * dst_stride + x] = src[(long)ym * src_stride + xm];
+        }
+    }
+}
+
+/* The column split of the bilinear border forms (ago_haf_cpu_geometric.cpp:2133-2155, :2297-2319):
+ * newDstWidth = (first x whose map reaches sw-1) - 1, where the value 0 counts as unset (a later x
+ * sets it again); 0 when no column reaches sw-1; rounded down to a multiple of 8 when dw is not one.
+ * Columns below it take the two-stage (SIMD) rounding, the rest the single rounding. */
+int oracle_scale_border_split(int dw, int sw, int xoffs, int xinc)
+{
+    uint32_t nd = 0;
+    for (int x = 0; x < dw; x++)
+        if ((pos_at(xoffs, xinc, x) >> 18) >= sw - 1 && !nd) nd = (uint32_t)x - 1u;
+    if (dw & 7) nd &= ~7u;
+    return (int)nd;
+}
+
+/* ScaleImage_U8_U8_Bilinear_Replicate (ago_haf_cpu_geometric.cpp:2108-2258) and _Constant
+ * (:2260-2391), scale matrix as the UNDEFINED form.  Per output row: ym = ypos >> 18, yf =
+ * ((ypos & 0x3ffff) + 0x200) >> 10; rows (R1, R2) = (ym, ym+1), or for ym < 0 (0, 0) | (border, 0),
+ * for ym >= sh-1 (sh-1, sh-1) | (sh-1, border).  Per column: Xmap = xpos >> 18 clamped to [0, sw-1],
+ * xf likewise from xpos.
+ *   x < split: t1 = ((256-xf) R1[Xmap] + xf R1[Xmap+1] + 0x80) >> 8, t2 from R2, then
+ *              ((256-yf) t1 + yf t2 + 0x80) >> 8 (16-bit lanes; no term reaches 2^16);
+ *   x >= split: ((256-xf)(256-yf) p0 + xf(256-yf) p1 + (256-xf) yf p2 + xf yf p3 + 0x8000) >> 16 with
+ *              p1 (p3) = the right neighbour, or at Xmap = sw-1 p0 (p2) | the border value.
+ *   Upscale (xscale < 1) with split > 0: column 0 again with the left neighbour replaced:
+ *              replicate p0 = p1 = R1[0], p2 = p3 = R2[0]; constant p0 = p2 = border, p1 = R1[0],
+ *              p3 = R2[0] (with split 0 the single-rounding pass rewrites column 0 afterwards).
+ * Pinned by: bit-exact against oracle/_ref for up/downscale, sizes on and off multiples of 8, border
+ * values 0 and 200; constant image => constant output (replicate), impulse responses. */
+void oracle_scale_bilinear_border(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                  int src_stride, int constant, int border)
+{
+    int xinc, yinc, xoffs, yoffs;
+    float xs, ys;
+    scale_matrix(dw, dh, sw, sh, 0.5, &xinc, &yinc, &xoffs, &yoffs, &xs, &ys);
+    const int split = oracle_scale_border_split(dw, sw, xoffs, xinc);
+    for (int y = 0; y < dh; y++) {
+        const int ypos = pos_at(yoffs, yinc, y);
+        const int ym = ypos >> 18;
+        const int yf = ((ypos & 0x3ffff) + 0x200) >> 10, omy = 0x100 - yf;
+        int r1, r2;   /* -1: the border row */
+        if (ym < 0) { r1 = constant ? -1 : 0; r2 = 0; }
+        else if (ym >= sh - 1) { r1 = sh - 1; r2 = constant ? -1 : sh - 1; }
+        else { r1 = ym; r2 = ym + 1; }
+#define PX(r, c) ((r) < 0 ? border : (int)src[(long)(r) * src_stride + (c)])
+        for (int x = 0; x < dw; x++) {
+            const int xpos = pos_at(xoffs, xinc, x);
+            int xm = xpos >> 18;
+            if (xm >= sw - 1) xm = sw - 1;
+            else if (xm < 0) xm = 0;
+            const int xf = ((xpos & 0x3ffff) + 0x200) >> 10, omx = 0x100 - xf;
+            int v;
+            if (x < split) {
+                const int t1 = (omx * PX(r1, xm) + xf * PX(r1, xm + 1) + 0x80) >> 8;
+                const int t2 = (omx * PX(r2, xm) + xf * PX(r2, xm + 1) + 0x80) >> 8;
+                v = (omy * t1 + yf * t2 + 0x80) >> 8;
+            } else {
+                const int p0 = PX(r1, xm), p2 = PX(r2, xm);
+                const int p1 = xm < sw - 1 ? PX(r1, xm + 1) : (constant ? border : p0);
+                const int p3 = xm < sw - 1 ? PX(r2, xm + 1) : (constant ? border : p2);
+                v = (int)(((unsigned)(omx * omy * p0) + (unsigned)(xf * omy * p1) + (unsigned)(omx * yf * p2) +
+                           (unsigned)(xf * yf * p3) + 0x8000u) >> 16);
+            }
+            dst[(long)y * dst_stride + x] = (uint8_t)v;
+        }
+        if (xs < 1.0f && split > 0) {
+            const int xpos = xoffs;
+            const int xf = ((xpos & 0x3ffff) + 0x200) >> 10, omx = 0x100 - xf;
+            int p0, p1, p2, p3;
+            if (constant) { p0 = p2 = border; p1 = PX(r1, 0); p3 = PX(r2, 0); }
+            else { p0 = p1 = PX(r1, 0); p2 = p3 = PX(r2, 0); }
+            dst[(long)y * dst_stride] = (uint8_t)(((unsigned)(omx * omy * p0) + (unsigned)(xf * omy * p1) +
+                                                   (unsigned)(omx * yf * p2) + (unsigned)(xf * yf * p3) + 0x8000u) >> 16);
+        }
+#undef PX
+    }
+}
+
+/* ScaleImage_U8_U8_Area as a graph runs it.  The reference's graph optimizer rewrites every AMD
+ * ScaleImage node before it runs (ago_drama_remove.cpp:1270-1311): the same size becomes
+ * CHANNEL_COPY_U8_U8, and an AREA node that does not shrink both dimensions becomes
+ * SCALE_IMAGE_U8_U8_NEAREST.  What reaches HafCpu_ScaleImage_U8_U8_Area (ago_haf_cpu_geometric.cpp:
+ * 2491-2617, offsets -1/2 from ago_kernel_api.cpp:20276-20277) has a defined result for an exact 2x
+ * only: row y averages source rows S = (int)(2y - 0.5f) clamped to [0, sh-2] and S+1,
+ * (a + b + c + d + 2) >> 2 over 2x2 blocks, columns 0..(dw & ~7)-1 only (the rest are not written).
+ * Returns -1 for other shrinking ratios: the generic path keeps its row sums past the end of its node
+ * buffer (localDataSize = sizeof(matrix) + 2*alignedWidth + 16 ends where Ymap starts), so its result
+ * is not defined; also for a 2x destination narrower than 8 (the loop bound dw - 8 wraps).
+ * Pinned by: bit-exact against oracle/_ref (the named AMD kernel and vxScaleImageNode) on the same
+ * size, upscales, mixed ratios and exact 2x sizes; constant image => constant output. */
+int oracle_scale_area(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                      int src_stride)
+{
+    if (sw == dw && sh == dh) {
+        for (int y = 0; y < dh; y++) memcpy(dst + (long)y * dst_stride, src + (long)y * src_stride, (size_t)dw);
+        return 0;
+    }
+    if (!(sw > dw && sh > dh)) {
+        oracle_scale_nearest(dw, dh, dst, dst_stride, sw, sh, src, src_stride);
+        return 0;
+    }
+    const float xs = (float)((double)sw / (double)dw), ys = (float)((double)sh / (double)dh);
+    if (xs == 2.0f && ys == 2.0f && dw >= 8 && sh >= 2) {
+        for (int y = 0; y < dh; y++) {
+            int r = (int)(-0.5f + (float)(unsigned)(2 * y));
+            if (r < 0) r = 0;
+            if (r > sh - 2) r = sh - 2;
+            const uint8_t *s0 = src + (long)r * src_stride, *s1 = s0 + src_stride;
+            for (int x = 0; x < (dw & ~7); x++)
+                dst[(long)y * dst_stride + x] = (uint8_t)((s0[2 * x] + s0[2 * x + 1] + s1[2 * x] + s1[2 * x + 1] + 2) >> 2);
+        }
+        return 0;
+    }
+    return -1;
+}
+
 /* ScaleGaussianHalf_U8_U8_5x5: ago_haf_cpu_pyramid.cpp:94-305, wrapper ago_kernel_api.cpp:15420-15436.
  * Output rows 1..dh-2, all columns:
  *   cy = 2y + oy (oy = 1 if srcH is even), cx = 2x + ox (ox = 1 if srcW is even);

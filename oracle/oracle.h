@@ -50,6 +50,13 @@ void oracle_scale_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw,
                            int src_stride);
 int oracle_scale_gaussian_half_5x5(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh,
                                    const uint8_t *src, int src_stride);
+void oracle_scale_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                          int src_stride);
+int oracle_scale_border_split(int dw, int sw, int xoffs, int xinc);
+void oracle_scale_bilinear_border(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                  int src_stride, int constant, int border);
+int oracle_scale_area(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                      int src_stride);
 
 /* ---- Canny and Harris (vision.c) ---- */
 typedef struct { int16_t x, y; float s; } oracle_xys_t;                 /* ago_keypoint_xys_t */

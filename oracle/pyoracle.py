@@ -60,6 +60,11 @@ def _plane(src):
     return out, st
 
 
+# OpenVX enums (VX/vx.h): interpolation types and border modes
+INTERP_NEAREST, INTERP_BILINEAR, INTERP_AREA = 0x4000, 0x4001, 0x4002
+BORDER_UNDEFINED, BORDER_CONSTANT, BORDER_REPLICATE = 0xC000, 0xC001, 0xC002
+
+
 def _f32(m):
     return np.ascontiguousarray(np.asarray(m, np.float32).reshape(-1))
 
@@ -117,6 +122,42 @@ class Oracle:
         sh, sw = src.shape
         d = np.zeros((dh, dw), np.uint8)
         self.lib.oracle_scale_node(sw, sh, _p(_c(src)), dw, dh, _p(d))
+        return d
+
+    def scale_mode(self, src, dw, dh, interp, border_mode=BORDER_UNDEFINED, border=0):
+        """vxScaleImageNode semantics (ago_drama_divide.cpp:563-616 and the rewrite of
+        ago_drama_remove.cpp:1270-1311): same size -> plane copy; otherwise nearest for
+        NEAREST_NEIGHBOR, the border form of bilinear for BILINEAR (UNDEFINED / REPLICATE / CONSTANT),
+        and for AREA (any border) nearest unless both dimensions shrink, else area.  Unwritten pixels
+        stay 0."""
+        sh, sw = src.shape
+        s = _c(src)
+        d = np.zeros((dh, dw), np.uint8)
+        if (sw, sh) == (dw, dh):
+            d[:] = src
+        elif interp == INTERP_NEAREST:
+            self.lib.oracle_scale_nearest(dw, dh, _p(d), dw, sw, sh, _p(s), sw)
+        elif interp == INTERP_BILINEAR:
+            if border_mode == BORDER_UNDEFINED:
+                # reads one byte past the row at the right edge: the node wrapper models the reference's
+                # zero-padded ALIGN16 rows
+                self.lib.oracle_scale_node(sw, sh, _p(s), dw, dh, _p(d))
+            else:
+                self.lib.oracle_scale_bilinear_border(dw, dh, _p(d), dw, sw, sh, _p(s), sw,
+                                                      int(border_mode == BORDER_CONSTANT), int(border))
+        elif interp == INTERP_AREA:
+            d = self.scale_area(src, dw, dh)
+        else:
+            raise ValueError(interp)
+        return d
+
+    def scale_area(self, src, dw, dh):
+        """ScaleImage_U8_U8_Area as a graph runs it: copy at the same size, nearest unless both
+        dimensions shrink, the 2x2 average at an exact 2x; other shrinking ratios have no defined result"""
+        sh, sw = src.shape
+        d = np.zeros((dh, dw), np.uint8)
+        if self.lib.oracle_scale_area(dw, dh, _p(d), dw, sw, sh, _p(_c(src)), sw) != 0:
+            raise ValueError("a shrinking area scale is defined for an exact 2x (width >= 8) only")
         return d
 
     def pyramid(self, src, levels):
@@ -318,6 +359,18 @@ class Reference:
         sh, sw = src.shape
         d = np.zeros((dh, dw), np.uint8)
         self._ok(self.lib.ref_scale(sw, sh, _p(_c(src)), dw, dh, _p(d)), "scale")
+        return d
+
+    def scale_mode(self, src, dw, dh, interp, border_mode=BORDER_UNDEFINED, border=0):
+        sh, sw = src.shape
+        d = np.zeros((dh, dw), np.uint8)
+        self._ok(self.lib.ref_scale_mode(sw, sh, _p(_c(src)), dw, dh, _p(d), interp, border_mode, border), "scale_mode")
+        return d
+
+    def amd_scale_area(self, src, dw, dh):
+        sh, sw = src.shape
+        d = np.zeros((dh, dw), np.uint8)
+        self._ok(self.lib.ref_amd_scale_area(sw, sh, _p(_c(src)), dw, dh, _p(d)), "amd_scale_area")
         return d
 
     def pyramid(self, src, levels):

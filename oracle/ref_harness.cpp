@@ -203,6 +203,41 @@ int ref_scale(int sw, int sh, const uint8_t *src, int dw, int dh, uint8_t *dst) 
     return 0;
 }
 
+// ScaleImage with any interpolation (VX_INTERPOLATION_*) and node border mode (VX_BORDER_*; the
+// constant value for VX_BORDER_CONSTANT).  The output image is fresh (zeros where nothing writes).
+int ref_scale_mode(int sw, int sh, const uint8_t *src, int dw, int dh, uint8_t *dst, int interp, int border_mode,
+                   int border_value) {
+    Ctx c;
+    vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    vx_node n = vxScaleImageNode(c.graph, in, out, (vx_enum)interp);
+    if (!n) return -1;
+    vx_border_t b = {};
+    b.mode = (vx_enum)border_mode;
+    b.constant_value.U8 = (vx_uint8)border_value;
+    CHECK(vxSetNodeAttribute(n, VX_NODE_BORDER, &b, sizeof(b)));
+    int s = run(c); if (s) return s;
+    CHECK(get(out, dw, dh, dst, 1));
+    return 0;
+}
+
+// The AMD kernel com.amd.openvx.ScaleImage_U8_U8_Area run by name (parameters: output, input), which
+// reaches its same-size form -- vxScaleImageNode turns a same-size scale into a plane copy instead.
+int ref_amd_scale_area(int sw, int sh, const uint8_t *src, int dw, int dh, uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    vx_kernel k = vxGetKernelByName(c.ctx, "com.amd.openvx.ScaleImage_U8_U8_Area");
+    if (!k) return -1;
+    vx_node n = vxCreateGenericNode(c.graph, k);
+    if (!n) return -1;
+    CHECK(vxSetParameterByIndex(n, 0, (vx_reference)out));
+    CHECK(vxSetParameterByIndex(n, 1, (vx_reference)in));
+    int s = run(c); if (s) return s;
+    CHECK(get(out, dw, dh, dst, 1));
+    return 0;
+}
+
 static int get_pyramid(vx_pyramid pyr, int levels, uint8_t *out) {
     size_t off = 0;
     for (int l = 0; l < levels; l++) {

@@ -104,7 +104,8 @@ def test_vxu_sobel_magnitude_phase_scale_pyramid(vx, ctx, orc):
     s = vx.create_image(ctx, 77, 50, "U8")
     assert L.vxuScaleImage(ctx, a, s, vx.VX_INTERPOLATION_BILINEAR) == 0
     assert_same(vx.read_image(s), orc.scale(img, 77, 50), "scale")
-    assert L.vxuScaleImage(ctx, a, s, vx.VX_INTERPOLATION_NEAREST_NEIGHBOR) == vx.VX_ERROR_NOT_SUPPORTED
+    assert L.vxuScaleImage(ctx, a, s, vx.VX_INTERPOLATION_NEAREST_NEIGHBOR) == 0
+    assert_same(vx.read_image(s), orc.scale_mode(img, 77, 50, 0x4000), "scale nearest")
     pyr = L.vxCreatePyramid(ctx, 3, vx.VX_SCALE_PYRAMID_HALF, w, h, vx.DF["U8"])
     assert L.vxuGaussianPyramid(ctx, a, pyr) == 0
     want = orc.pyramid(img, 3)

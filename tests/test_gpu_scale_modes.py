@@ -1,0 +1,121 @@
+"""GPU parity of the ScaleImage forms beside bilinear/UNDEFINED -- nearest neighbour, bilinear with
+REPLICATE and CONSTANT borders, area (same size and exact 2x) -- through the C ABI and through
+vxScaleImageNode with the node border attribute, against the oracle, bit-exact with untouched zero
+row padding."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from mivisionx_amd import synth
+from mivisionx_amd.device import MvxError, Plane, call
+from pyoracle import BORDER_CONSTANT, BORDER_REPLICATE, BORDER_UNDEFINED, INTERP_AREA, INTERP_BILINEAR, INTERP_NEAREST
+from util import assert_same
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(200, 120, 100, 60), (200, 120, 97, 61), (64, 48, 160, 100), (37, 29, 90, 71), (640, 480, 1280, 960),
+         (100, 80, 33, 17), (331, 211, 128, 96), (16, 16, 48, 40), (24, 8, 5, 3), (3840, 2160, 1920, 1080),
+         (1920, 1080, 3840, 2160)]
+
+
+def _src(sw, sh):
+    return synth.scene_u8(sw, sh, sw * 7 + sh) if sw >= 16 and sh >= 16 else synth.uniform_u8(sw, sh, sw + sh)
+
+
+def _check(d, want, what):
+    assert_same(d.download(), want, what)
+    assert not d.t[:, d.width:].any().item(), f"{what}: row padding written"
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh", SIZES)
+def test_scale_modes_c_abi(gpu, orc, sw, sh, dw, dh):
+    src = _src(sw, sh)
+    s = Plane.from_array(src)
+    d = Plane(dw, dh)
+    call("ScaleImage_U8_U8_Nearest", dw, dh, d, d.stride, sw, sh, s, s.stride)
+    _check(d, orc.scale_mode(src, dw, dh, INTERP_NEAREST), f"nearest {sw}x{sh}->{dw}x{dh}")
+    d = Plane(dw, dh)
+    call("ScaleImage_U8_U8_Bilinear_Replicate", dw, dh, d, d.stride, sw, sh, s, s.stride)
+    _check(d, orc.scale_mode(src, dw, dh, INTERP_BILINEAR, BORDER_REPLICATE), f"replicate {sw}x{sh}->{dw}x{dh}")
+    for border in (0, 200):
+        d = Plane(dw, dh)
+        call("ScaleImage_U8_U8_Bilinear_Constant", dw, dh, d, d.stride, sw, sh, s, s.stride, ctypes.c_uint8(border))
+        _check(d, orc.scale_mode(src, dw, dh, INTERP_BILINEAR, BORDER_CONSTANT, border),
+               f"constant {border} {sw}x{sh}->{dw}x{dh}")
+
+
+@pytest.mark.parametrize("sw,sh", [(200, 120), (64, 48), (34, 22), (1920, 1080), (3840, 2160), (50, 10)])
+def test_scale_area_c_abi(gpu, orc, sw, sh):
+    """2x, same size (a copy), upscale and a mixed ratio (nearest neighbour)"""
+    src = _src(sw, sh)
+    s = Plane.from_array(src)
+    for dw, dh in ((sw // 2, sh // 2), (sw, sh), (sw * 2 + 3, sh + 5), (sw // 2, sh)):
+        d = Plane(dw, dh)
+        call("ScaleImage_U8_U8_Area", dw, dh, d, d.stride, sw, sh, s, s.stride)
+        _check(d, orc.scale_mode(src, dw, dh, INTERP_AREA), f"area {sw}x{sh}->{dw}x{dh}")
+
+
+def test_scale_area_undefined_ratio_refused(gpu):
+    s, d = Plane(90, 60), Plane(30, 20)
+    with pytest.raises(MvxError, match="-3"):
+        call("ScaleImage_U8_U8_Area", 30, 20, d, d.stride, 90, 60, s, s.stride)
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh", [(200, 120, 97, 61), (64, 48, 160, 100), (640, 480, 320, 240), (640, 480, 640, 480)])
+def test_scale_node_interpolation_and_border(gpu, orc, sw, sh, dw, dh):
+    """vxScaleImageNode routes by interpolation and node border (ago_drama_divide.cpp:563-616)"""
+    from mivisionx_amd import openvx as vx
+    L = vx.lib()
+    ctx = L.vxCreateContext()
+    try:
+        src = _src(sw, sh)
+        a = vx.create_image(ctx, sw, sh, "U8")
+        vx.write_image(a, src)
+        cases = [(vx.VX_INTERPOLATION_NEAREST_NEIGHBOR, INTERP_NEAREST), (vx.VX_INTERPOLATION_BILINEAR, INTERP_BILINEAR)]
+        if not (sw > dw and sh > dh) or (sw, sh) == (2 * dw, 2 * dh):
+            cases.append((vx.VX_INTERPOLATION_AREA, INTERP_AREA))
+        for vxi, oi in cases:
+            for bm, obm, bv in ((None, BORDER_UNDEFINED, 0), (vx.VX_BORDER_REPLICATE, BORDER_REPLICATE, 0),
+                                (vx.VX_BORDER_CONSTANT, BORDER_CONSTANT, 77)):
+                out = vx.create_image(ctx, dw, dh, "U8")
+                g = L.vxCreateGraph(ctx)
+                node = L.vxScaleImageNode(g, a, out, vxi)
+                assert node
+                if bm is not None:
+                    b = vx.Border(bm)
+                    b.constant_value.U8 = bv
+                    assert L.vxSetNodeAttribute(node, vx.VX_NODE_BORDER, ctypes.byref(b), ctypes.sizeof(b)) == 0
+                assert L.vxVerifyGraph(g) == 0
+                assert L.vxProcessGraph(g) == 0
+                assert_same(vx.read_image(out), orc.scale_mode(src, dw, dh, oi, obm, bv),
+                            f"node interp {vxi:#x} border {obm:#x} {sw}x{sh}->{dw}x{dh}")
+                gg = ctypes.c_void_p(g)
+                L.vxReleaseGraph(ctypes.byref(gg))
+                vx.release(out)
+        # the AMD area kernel by name routes as the node does
+        k = L.vxGetKernelByName(ctx, b"com.amd.openvx.ScaleImage_U8_U8_Area")
+        assert k
+        for ow, oh in ((dw, dh), (sw + 7, sh // 2 + 1)) if (sw, sh) != (dw, dh) else ((sw, sh),):
+            if sw > ow and sh > oh and (sw, sh) != (2 * ow, 2 * oh):
+                continue
+            out = vx.create_image(ctx, ow, oh, "U8")
+            g = L.vxCreateGraph(ctx)
+            node = L.vxCreateGenericNode(g, k)
+            assert L.vxSetParameterByIndex(node, 0, out) == 0 and L.vxSetParameterByIndex(node, 1, a) == 0
+            assert L.vxProcessGraph(g) == 0
+            assert_same(vx.read_image(out), orc.scale_area(src, ow, oh), f"amd area {sw}x{sh}->{ow}x{oh}")
+            gg = ctypes.c_void_p(g)
+            L.vxReleaseGraph(ctypes.byref(gg))
+            vx.release(out)
+        # AREA at a ratio the reference leaves undefined is refused at verification
+        out = vx.create_image(ctx, sw // 3, sh // 3, "U8")
+        g = L.vxCreateGraph(ctx)
+        assert L.vxScaleImageNode(g, a, out, vx.VX_INTERPOLATION_AREA)
+        assert L.vxVerifyGraph(g) == vx.VX_ERROR_NOT_SUPPORTED
+        gg = ctypes.c_void_p(g)
+        L.vxReleaseGraph(ctypes.byref(gg))
+        vx.release(out)
+    finally:
+        cc = ctypes.c_void_p(ctx)
+        L.vxReleaseContext(ctypes.byref(cc))
