@@ -129,6 +129,15 @@ int mvx_ScaleImage_U8_U8_Area(mvx_stream_t stream, uint32_t dstWidth, uint32_t d
                               uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
                               const uint8_t *pSrc, uint32_t srcStride);
 
+/* replaces HipExec_ScaleGaussianHalf_U8_U8_3x3 (hip_host_decls.h:676); HafCpu_ScaleGaussianHalf_U8_U8_3x3
+ * (ago_haf_cpu_geometric.cpp:2772-2829) as its wrapper calls it (ago_kernel_api.cpp:15341-15352): output
+ * rows 1..dstHeight-2, row y centred on source row 2y, [1 2 1]^2 >> 4 with a zero left neighbour at
+ * column 0.  Returns -15 (VX_ERROR_INVALID_DIMENSION) for dstHeight == 1 or an output larger than half
+ * the source rounded up; dstStride a multiple of 8. */
+int mvx_ScaleGaussianHalf_U8_U8_3x3(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                    uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                    const uint8_t *pSrc, uint32_t srcStride);
+
 /* replaces HipExec_ScaleGaussianHalf_U8_U8_5x5 (hip_host_decls.h:681); HafCpu_ScaleGaussianHalf_U8_U8_5x5
  * (ago_haf_cpu_pyramid.cpp:94-305) as called by ago_kernel_api.cpp:15420-15436 (rows 1..h-2, parity
  * of the source size picks the sampled row/column).  Returns -15 (VX_ERROR_INVALID_DIMENSION) when the

@@ -86,6 +86,7 @@ enum : vx_enum {
     AMD_BOX_U8_U8_3x3 = 0xd000b0,
     AMD_MEDIAN_U8_U8_3x3 = 0xd000b3,
     AMD_GAUSSIAN_U8_U8_3x3 = 0xd000b4,
+    AMD_SCALE_GAUSSIAN_HALF_U8_U8_3x3 = 0xd000b5,
     AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
     AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,
     AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM = 0xd000da,

@@ -360,38 +360,59 @@ vx_status x_pyramid(_vx_node *n, hipStream_t st)
     return s;
 }
 
-vx_status v_amd_half(_vx_node *n)
+// the sizes each half-scale kernel provides: 3 -> ScaleGaussianHalf 3x3 (an output at least 2 rows high:
+// the reference's dstHeight - 2 wraps below that), 5 -> ScaleGaussianHalf 5x5 (source >= 5 and output
+// >= 3 in each dimension, ago_kernel_api.cpp:15428-15430), 1 -> ScaleImage nearest
+static vx_status half_supported(_vx_node *n, const _vx_image *in, int ksize)
+{
+    const uint32_t dw = (in->width + 1) / 2, dh = (in->height + 1) / 2;
+    if (ksize == 3 && dh < 2) return fail(n, VX_ERROR_INVALID_DIMENSION, "ScaleGaussianHalf 3x3 needs an output 2 rows high");
+    if (ksize == 5 && (in->width < 5 || in->height < 5 || dw < 3 || dh < 3))
+        return fail(n, VX_ERROR_INVALID_DIMENSION, "ScaleGaussianHalf 5x5 needs a source of at least 5x5");
+    if (ksize == 1 && (in->width > 65535 || in->height > 65535))
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "nearest-neighbour sources above 65535 pixels (16-bit maps)");
+    return VX_SUCCESS;
+}
+
+static vx_status run_half(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, int ksize)
+{
+    if (ksize == 1)
+        return rc(n, mvx_ScaleImage_U8_U8_Nearest(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+    if (ksize == 3)
+        return rc(n, mvx_ScaleGaussianHalf_U8_U8_3x3(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+    return rc(n, mvx_ScaleGaussianHalf_U8_U8_5x5(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
+}
+
+// AMD forms (out, in): KSIZE 3 or 5
+template <int KSIZE> vx_status v_amd_half(_vx_node *n)
 {
     vx_status s = need_input(n, 1, VX_DF_IMAGE_U8);
     if (s) return s;
     _vx_image *in = img(n, 1);
+    s = half_supported(n, in, KSIZE);
+    if (s) return s;
     return set_output(n, 0, (in->width + 1) / 2, (in->height + 1) / 2, VX_DF_IMAGE_U8);
 }
 
-vx_status x_amd_half(_vx_node *n, hipStream_t st)
-{
-    _vx_image *o = img(n, 0), *in = img(n, 1);
-    return rc(n, mvx_ScaleGaussianHalf_U8_U8_5x5(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
-}
+template <int KSIZE> vx_status x_amd_half(_vx_node *n, hipStream_t st) { return run_half(n, st, img(n, 0), img(n, 1), KSIZE); }
 
-// halfscale_gaussian -> SCALE_GAUSSIAN_HALF_U8_U8_5x5 for kernel size 5 (ago_drama_divide.cpp:1471-1488,
-// which picks ScaleImage nearest for 1 and ScaleGaussianHalf 3x3 for 3: those two are not built here)
+// halfscale_gaussian (ago_drama_divide.cpp:1471-1488): kernel size 1 -> SCALE_IMAGE_U8_U8_NEAREST,
+// 3 -> SCALE_GAUSSIAN_HALF_U8_U8_3x3, 5 -> SCALE_GAUSSIAN_HALF_U8_U8_5x5; output (w+1)/2 x (h+1)/2
 vx_status v_halfscale(_vx_node *n)
 {
     vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
     if (s) return s;
     _vx_scalar *k = scl(n, 2);
     if (!k || k->dataType != VX_TYPE_INT32) return fail(n, VX_ERROR_INVALID_PARAMETERS, "kernel_size is not an INT32 scalar");
-    if (k->value.i32 != 5) return fail(n, VX_ERROR_NOT_SUPPORTED, "kernel_size other than 5");
+    const int ks = k->value.i32;
+    if (ks != 1 && ks != 3 && ks != 5) return fail(n, VX_ERROR_INVALID_VALUE, "kernel_size must be 1, 3 or 5");
     _vx_image *in = img(n, 0);
+    s = half_supported(n, in, ks);
+    if (s) return s;
     return set_output(n, 1, (in->width + 1) / 2, (in->height + 1) / 2, VX_DF_IMAGE_U8);
 }
 
-vx_status x_halfscale(_vx_node *n, hipStream_t st)
-{
-    _vx_image *in = img(n, 0), *o = img(n, 1);
-    return rc(n, mvx_ScaleGaussianHalf_U8_U8_5x5(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in)));
-}
+vx_status x_halfscale(_vx_node *n, hipStream_t st) { return run_half(n, st, img(n, 1), img(n, 0), scl(n, 2)->value.i32); }
 
 // ---------------------------------------------------------------------- Canny (A12) --------------
 // CANNY_SOBEL_U16_U8_3x3_L1NORM -> CANNY_SUPP_THRESHOLD_U8XY_U16_3x3 -> CANNY_EDGE_TRACE_U8_U8XY with a
@@ -837,8 +858,10 @@ const std::vector<KernelDef> &kernel_table()
          v_filter<1>, nullptr, x_filter<1, mvx_Median_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
         {AMD_GAUSSIAN_U8_U8_3x3, "com.amd.openvx.Gaussian_U8_U8_3x3", {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}},
          v_filter<1>, nullptr, x_filter<1, mvx_Gaussian_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SCALE_GAUSSIAN_HALF_U8_U8_3x3, "com.amd.openvx.ScaleGaussianHalf_U8_U8_3x3",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_half<3>, nullptr, x_amd_half<3>, AGO_TARGET_AFFINITY_GPU},
         {AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5, "com.amd.openvx.ScaleGaussianHalf_U8_U8_5x5",
-         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_half, nullptr, x_amd_half, AGO_TARGET_AFFINITY_GPU},
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_half<5>, nullptr, x_amd_half<5>, AGO_TARGET_AFFINITY_GPU},
         {AMD_SOBEL_S16S16_U8_3x3_GXY, "com.amd.openvx.Sobel_S16S16_U8_3x3_GXY",
          {{OUT, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_sobel, nullptr, x_amd_sobel,
          AGO_TARGET_AFFINITY_GPU},

@@ -272,6 +272,42 @@ k_gaussian_half_5x5(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_
     half_strip(dw, dh, dst, dstStride, sw, sh, src, srcStride, x0, y0);
 }
 
+// ------------------------------------------------------------------ ScaleGaussianHalf 3x3 ------
+// HafCpu_ScaleGaussianHalf_U8_U8_3x3 (ago_haf_cpu_geometric.cpp:2772-2829) over output rows 1..dh-2
+// (ago_kernel_api.cpp:15348): output row y is centred on source row 2y; V(c) = [1 2 1] down the three
+// rows, V(-1) = 0 (the reference's zeroed scratch word), out = (V(2x-1) + 2V(2x) + V(2x+1)) >> 4.
+// One thread: 8 outputs of one row from one aligned 16-byte load per source row (the host checks
+// dw <= (sw+1)/2 and dh <= (sh+1)/2, which keeps the loads inside the row strides).
+__global__ void __launch_bounds__(256)
+k_gaussian_half_3x3(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride,
+                    const uint8_t *__restrict__ src, uint32_t srcStride)
+{
+    const uint32_t x0 = (blockIdx.x * 256 + threadIdx.x) * 8;
+    const uint32_t y = blockIdx.y + 1;
+    if (x0 >= dw || y + 1 >= dh) return;
+    const uint8_t *r0 = src + (long)(2 * y - 1) * srcStride + 2 * x0;
+    uint32_t a[16], b[16], c[16];
+    unpack16(ld16(r0), a);
+    unpack16(ld16(r0 + srcStride), b);
+    unpack16(ld16(r0 + 2 * (long)srcStride), c);
+    const uint32_t vm1 = x0 ? (uint32_t)r0[-1] + 2u * r0[(long)srcStride - 1] + r0[2 * (long)srcStride - 1] : 0u;
+    uint32_t v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = a[i] + 2u * b[i] + c[i];
+    uint32_t o[16];
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = ((k ? v[2 * k - 1] : vm1) + 2u * v[2 * k] + v[2 * k + 1]) >> 4;
+    uint8_t *d = dst + (long)y * dstStride + x0;
+    if (x0 + 8 <= dw) {
+        uint2 w;
+        w.x = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+        w.y = o[4] | (o[5] << 8) | (o[6] << 16) | (o[7] << 24);
+        *reinterpret_cast<uint2 *>(d) = w;
+    } else {
+        for (uint32_t k = 0; k < dw - x0; k++) d[k] = (uint8_t)o[k];
+    }
+}
+
 }  // namespace mvx
 
 using namespace mvx;
@@ -401,6 +437,22 @@ extern "C" int mvx_ScaleImage_U8_U8_Area(mvx_stream_t stream, uint32_t dw, uint3
     // the reference's generic area path keeps its row sums past the end of its node buffer, so its
     // result is not defined (ago_kernel_api.cpp:20269 vs ago_haf_cpu_geometric.cpp:2553)
     return MVX_ERROR_NOT_SUPPORTED;
+}
+
+extern "C" int mvx_ScaleGaussianHalf_U8_U8_3x3(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst,
+                                               uint32_t dstStride, uint32_t sw, uint32_t sh, const uint8_t *src,
+                                               uint32_t srcStride)
+{
+    // an output row count below 2 wraps the reference's dstHeight - 2; a larger output than half the
+    // source (rounded up) would read past the source rows
+    if (dh == 1 || dw > (sw + 1) / 2 || dh > (sh + 1) / 2) return -15;
+    if (dw == 0 || dh < 3) return MVX_SUCCESS;
+    if (!plane_ok_g(dst, dw, dstStride, 1) || !plane_ok_g(src, sw, srcStride, 1) || (dstStride & 7u))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    dim3 grid(((dw + 7) / 8 + 255) / 256, dh - 2);
+    hipLaunchKernelGGL(k_gaussian_half_3x3, grid, dim3(256), 0, stream_of(stream), dw, dh, dst, dstStride, src, srcStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
 }
 
 extern "C" int mvx_ScaleGaussianHalf_U8_U8_5x5(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst,

@@ -34,6 +34,7 @@ VX_SUCCESS = 0
 VX_ERROR_NOT_SUPPORTED = -3
 VX_ERROR_INVALID_FORMAT = -14
 VX_ERROR_INVALID_DIMENSION = -15
+VX_ERROR_INVALID_VALUE = -16
 VX_ERROR_INVALID_TYPE = -17
 VX_ERROR_INVALID_GRAPH = -18
 VX_ERROR_MULTIPLE_WRITERS = -23

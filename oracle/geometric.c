@@ -239,6 +239,30 @@ int oracle_scale_area(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int 
     return -1;
 }
 
+/* ScaleGaussianHalf_U8_U8_3x3: ago_haf_cpu_geometric.cpp:2772-2829, wrapper ago_kernel_api.cpp:15341-15352
+ * (rows start one row in on both images; zeroed node scratch, ago_interface.cpp:1859-1866).
+ * Output rows 1..dh-2, all columns; output row y is centred on source row 2y whatever the parity:
+ *   V(c) = p(c, 2y-1) + 2 p(c, 2y) + p(c, 2y+1) for c >= 0, V(-1) = 0 (the zeroed scratch word
+ *          before the row buffer), reads inside the source row stride (columns up to sw read 0 padding)
+ *   out  = (V(2x-1) + 2 V(2x) + V(2x+1)) >> 4                             (no rounding)
+ * Returns -1 when dh < 2 (the reference's dstHeight - 2 wraps) or the output is larger than half the
+ * source rounded up.
+ * Pinned by: bit-exact against oracle/_ref (halfscale_gaussian with kernel size 3) on even/odd
+ * sizes; constant image => constant interior with column 0 at 3/4; impulse => [1 2 1]^2 / 16. */
+int oracle_scale_gaussian_half_3x3(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh,
+                                   const uint8_t *src, int src_stride)
+{
+    if (dh < 2 || dw > (sw + 1) / 2 || dh > (sh + 1) / 2) return -1;
+    for (int y = 1; y < dh - 1; y++) {
+        const uint8_t *r0 = src + (long)(2 * y - 1) * src_stride, *r1 = r0 + src_stride, *r2 = r1 + src_stride;
+#define V(c) ((c) < 0 ? 0u : (unsigned)r0[c] + 2u * (unsigned)r1[c] + (unsigned)r2[c])
+        for (int x = 0; x < dw; x++)
+            dst[(long)y * dst_stride + x] = (uint8_t)((V(2 * x - 1) + 2u * V(2 * x) + V(2 * x + 1)) >> 4);
+#undef V
+    }
+    return 0;
+}
+
 /* ScaleGaussianHalf_U8_U8_5x5: ago_haf_cpu_pyramid.cpp:94-305, wrapper ago_kernel_api.cpp:15420-15436.
  * Output rows 1..dh-2, all columns:
  *   cy = 2y + oy (oy = 1 if srcH is even), cx = 2x + ox (ox = 1 if srcW is even);

@@ -178,6 +178,25 @@ class Oracle:
             raise ValueError("ScaleGaussianHalf needs src >= 5 and dst >= 3")
         return np.ascontiguousarray(d[:, :dw])
 
+    def halfscale(self, src, ksize):
+        """halfscale_gaussian (ago_drama_divide.cpp:1471-1488): kernel size 1 -> ScaleImage nearest,
+        3 -> ScaleGaussianHalf 3x3, 5 -> ScaleGaussianHalf 5x5, output (w+1)/2 x (h+1)/2 on
+        reference-layout planes; unwritten rows stay 0"""
+        sh, sw = src.shape
+        dw, dh = (sw + 1) // 2, (sh + 1) // 2
+        if ksize == 1:
+            return self.scale_mode(src, dw, dh, INTERP_NEAREST)
+        if ksize == 5:
+            return self.gaussian_half(src, dw, dh)
+        if ksize != 3:
+            raise ValueError(ksize)
+        ss, ds = (sw + 15) & ~15, (dw + 15) & ~15
+        s = np.zeros((sh, ss), np.uint8); s[:, :sw] = src
+        d = np.zeros((dh, ds), np.uint8)
+        if self.lib.oracle_scale_gaussian_half_3x3(dw, dh, _p(d), ds, sw, sh, _p(s), ss) != 0:
+            raise ValueError("ScaleGaussianHalf 3x3 needs an output at least 2 rows high")
+        return np.ascontiguousarray(d[:, :dw])
+
     # ---- Canny ----
     def canny(self, src, lower, upper):
         h, w = src.shape
@@ -371,6 +390,12 @@ class Reference:
         sh, sw = src.shape
         d = np.zeros((dh, dw), np.uint8)
         self._ok(self.lib.ref_amd_scale_area(sw, sh, _p(_c(src)), dw, dh, _p(d)), "amd_scale_area")
+        return d
+
+    def halfscale(self, src, ksize):
+        sh, sw = src.shape
+        d = np.zeros(((sh + 1) // 2, (sw + 1) // 2), np.uint8)
+        self._ok(self.lib.ref_halfscale(sw, sh, _p(_c(src)), ksize, _p(d)), "halfscale")
         return d
 
     def pyramid(self, src, levels):

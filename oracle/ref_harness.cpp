@@ -238,6 +238,18 @@ int ref_amd_scale_area(int sw, int sh, const uint8_t *src, int dw, int dh, uint8
     return 0;
 }
 
+// halfscale_gaussian with kernel size 1, 3 or 5 (output (w+1)/2 x (h+1)/2).
+int ref_halfscale(int w, int h, const uint8_t *src, int ksize, uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    const int dw = (w + 1) / 2, dh = (h + 1) / 2;
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    if (!vxHalfScaleGaussianNode(c.graph, in, out, ksize)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(out, dw, dh, dst, 1));
+    return 0;
+}
+
 static int get_pyramid(vx_pyramid pyr, int levels, uint8_t *out) {
     size_t off = 0;
     for (int l = 0; l < levels; l++) {

@@ -117,7 +117,8 @@ def test_vxu_sobel_magnitude_phase_scale_pyramid(vx, ctx, orc):
 
 @pytest.mark.parametrize("w,h", [(131, 67), (640, 480)])
 def test_vxu_halfscale_gaussian(vx, ctx, orc, w, h):
-    """halfscale_gaussian with kernel size 5 is ScaleGaussianHalf 5x5 (ago_drama_divide.cpp:1471-1488)"""
+    """halfscale_gaussian: kernel size 5 is ScaleGaussianHalf 5x5, 3 is ScaleGaussianHalf 3x3, 1 is ScaleImage
+    nearest (ago_drama_divide.cpp:1471-1488)"""
     L = vx.lib()
     img = synth.scene_u8(w, h, 13)
     a = vx.create_image(ctx, w, h, "U8")
@@ -126,7 +127,11 @@ def test_vxu_halfscale_gaussian(vx, ctx, orc, w, h):
     b = vx.create_image(ctx, dw, dh, "U8")
     assert L.vxuHalfScaleGaussian(ctx, a, b, 5) == 0
     assert_same(vx.read_image(b), orc.gaussian_half(img, dw, dh), f"halfscale_gaussian {w}x{h}")
-    assert L.vxuHalfScaleGaussian(ctx, a, b, 3) == vx.VX_ERROR_NOT_SUPPORTED
+    for k in (1, 3):
+        b = vx.create_image(ctx, dw, dh, "U8")   # kernel size 3 leaves rows 0 and dh-1 as they were
+        assert L.vxuHalfScaleGaussian(ctx, a, b, k) == 0
+        assert_same(vx.read_image(b), orc.halfscale(img, k), f"halfscale_gaussian {k} {w}x{h}")
+    assert L.vxuHalfScaleGaussian(ctx, a, b, 7) == vx.VX_ERROR_INVALID_VALUE
 
 
 def _graph2(vx, ctx, w, h, levels, lower, upper, k, thr, dist, capacity, fuse=True):

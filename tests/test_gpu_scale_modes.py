@@ -1,7 +1,7 @@
 """GPU parity of the ScaleImage forms beside bilinear/UNDEFINED -- nearest neighbour, bilinear with
 REPLICATE and CONSTANT borders, area (same size and exact 2x) -- through the C ABI and through
 vxScaleImageNode with the node border attribute, against the oracle, bit-exact with untouched zero
-row padding."""
+row padding; and ScaleGaussianHalf 3x3 (halfscale_gaussian kernel size 3)."""
 import ctypes
 
 import numpy as np
@@ -116,6 +116,45 @@ def test_scale_node_interpolation_and_border(gpu, orc, sw, sh, dw, dh):
         gg = ctypes.c_void_p(g)
         L.vxReleaseGraph(ctypes.byref(gg))
         vx.release(out)
+    finally:
+        cc = ctypes.c_void_p(ctx)
+        L.vxReleaseContext(ctypes.byref(cc))
+
+
+@pytest.mark.parametrize("w,h", [(640, 480), (131, 67), (37, 29), (16, 16), (5, 5), (33, 4), (17, 9), (3840, 2160), (1921, 1081)])
+def test_gaussian_half_3x3_c_abi(gpu, orc, w, h):
+    src = _src(w, h)
+    s = Plane.from_array(src)
+    dw, dh = (w + 1) // 2, (h + 1) // 2
+    d = Plane(dw, dh)
+    call("ScaleGaussianHalf_U8_U8_3x3", dw, dh, d, d.stride, w, h, s, s.stride)
+    _check(d, orc.halfscale(src, 3), f"half 3x3 {w}x{h}")
+
+
+def test_gaussian_half_3x3_refuses_sizes(gpu):
+    s, d = Plane(64, 48), Plane(40, 24)
+    with pytest.raises(MvxError, match="-15"):
+        call("ScaleGaussianHalf_U8_U8_3x3", 40, 24, d, d.stride, 64, 48, s, s.stride)
+    d1 = Plane(32, 1)
+    with pytest.raises(MvxError, match="-15"):
+        call("ScaleGaussianHalf_U8_U8_3x3", 32, 1, d1, d1.stride, 64, 2, s, s.stride)
+
+
+def test_gaussian_half_3x3_amd_kernel_by_name(gpu, orc):
+    from mivisionx_amd import openvx as vx
+    L = vx.lib()
+    ctx = L.vxCreateContext()
+    try:
+        w, h = 333, 201
+        src = _src(w, h)
+        a = vx.create_image(ctx, w, h, "U8")
+        vx.write_image(a, src)
+        out = vx.create_image(ctx, (w + 1) // 2, (h + 1) // 2, "U8")
+        g = L.vxCreateGraph(ctx)
+        node = L.vxCreateGenericNode(g, L.vxGetKernelByName(ctx, b"com.amd.openvx.ScaleGaussianHalf_U8_U8_3x3"))
+        assert L.vxSetParameterByIndex(node, 0, out) == 0 and L.vxSetParameterByIndex(node, 1, a) == 0
+        assert L.vxProcessGraph(g) == 0
+        assert_same(vx.read_image(out), orc.halfscale(src, 3), "amd ScaleGaussianHalf 3x3")
     finally:
         cc = ctypes.c_void_p(ctx)
         L.vxReleaseContext(ctypes.byref(cc))

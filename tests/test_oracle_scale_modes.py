@@ -99,3 +99,31 @@ def test_area_undefined_ratios_are_refused(orc):
         orc.scale_area(synth.uniform_u8(90, 60, 1), 30, 20)
     with pytest.raises(ValueError):
         orc.scale_area(synth.uniform_u8(12, 8, 1), 6, 4)   # 2x but narrower than 8
+
+
+@pytest.mark.parametrize("w,h", [(640, 480), (131, 67), (37, 29), (16, 16), (5, 5), (33, 4), (2, 3), (17, 9), (48, 3)])
+@pytest.mark.parametrize("ksize", [1, 3, 5])
+def test_halfscale_gaussian_matches_reference(orc, ref, w, h, ksize):
+    """halfscale_gaussian kernel sizes 1 (nearest), 3 (ScaleGaussianHalf 3x3), 5 (5x5)"""
+    if ksize == 5 and (w < 5 or h < 5):
+        pytest.skip("ScaleGaussianHalf 5x5 needs a 5x5 source")
+    src = _src(w, h)
+    assert np.array_equal(orc.halfscale(src, ksize), ref.halfscale(src, ksize))
+
+
+def test_gaussian_half_3x3_closed_forms(orc):
+    """constant image: interior rows constant, column 0 at 3/4 (its left neighbour is the zero word);
+    impulse at an even (row, column) => the [1 2 1]^2 / 16 kernel on the half grid, truncated"""
+    c = np.full((40, 64), 160, np.uint8)
+    got = orc.halfscale(c, 3)
+    assert (got[0] == 0).all() and (got[-1] == 0).all()
+    assert (got[1:-1, 1:] == 160).all() and (got[1:-1, 0] == 120).all()
+    imp = np.zeros((40, 64), np.uint8)
+    imp[20, 30] = 255
+    got = orc.halfscale(imp, 3).astype(int)
+    assert got[10, 15] == (4 * 255) >> 4
+    imp[20, 30] = 0
+    imp[21, 31] = 255   # odd row and column: reaches the four half-grid neighbours with weight 1
+    got = orc.halfscale(imp, 3).astype(int)
+    assert got[10, 15] == 255 >> 4 and got[11, 16] == 255 >> 4 and got[10, 16] == 255 >> 4 and got[11, 15] == 255 >> 4
+    assert got.sum() == 4 * (255 >> 4)
