@@ -206,7 +206,8 @@ int mvx_HarrisMergeSortAndPick_XY_XYS(mvx_stream_t stream, uint32_t capacity, vo
 
 /* ------------------------------------------------------------------ warps and remap -------- */
 /* Bilinear U8 warps and remap.  Rows are written up to ceil(w/4)*4 pixels (the reference's 4-pixel
- * stores), so dstStride must be >= ceil(w/4)*4 and a multiple of 4; the source is read at linear
+ * stores; padding pixels hold what the reference leaves there: column 0's value for the warps, the
+ * next map entries' for remap), so dstStride must be >= ceil(w/4)*4 and a multiple of 4; the source is read at linear
  * byte addresses with reads outside [0, srcHeight*srcStride) returning 0.  `matrix` is host memory
  * holding the VX matrix as stored: 6 floats (2 columns x 3 rows) for affine, 9 for perspective. */
 
@@ -243,6 +244,44 @@ int mvx_Remap_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t ds
 int mvx_Remap_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
                                       uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
                                       uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue);
+/* Nearest-neighbour forms: the same conventions (the nearest remap writes no padding). */
+/* replaces HipExec_WarpAffine_U8_U8_Nearest (hip_host_decls.h:749); HafCpu_WarpAffine_U8_U8_Nearest
+ * (ago_haf_cpu_geometric.cpp:508-656): truncated coordinates, the corner test choosing the masked
+ * loop (0 outside).  The reference's column tables fault on widths that are not a multiple of 4;
+ * here every width is evaluated by the same formula. */
+int mvx_WarpAffine_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                 uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                 uint32_t srcStride, const float *matrix);
+/* replaces HipExec_WarpAffine_U8_U8_Nearest_Constant (hip_host_decls.h:755);
+ * HafCpu_WarpAffine_U8_U8_Nearest_Constant (ago_haf_cpu_geometric.cpp:658-823).  validRect: host
+ * {start_x, start_y, end_x, end_y} of the source's valid region (the reference's rect_valid), NULL for
+ * the whole image. */
+int mvx_WarpAffine_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                          uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                          uint32_t srcStride, const float *matrix, uint8_t borderValue,
+                                          const uint32_t *validRect);
+/* replaces HipExec_WarpPerspective_U8_U8_Nearest (hip_host_decls.h:775); HafCpu_WarpPerspective_U8_U8_Nearest
+ * (ago_haf_cpu_geometric.cpp:1372-1385) */
+int mvx_WarpPerspective_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                      uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                      uint32_t srcStride, const float *matrix);
+/* replaces HipExec_WarpPerspective_U8_U8_Nearest_Constant (hip_host_decls.h:781);
+ * HafCpu_WarpPerspective_U8_U8_Nearest_Constant (ago_haf_cpu_geometric.cpp:1406-1558) */
+int mvx_WarpPerspective_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                               uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                               const uint8_t *pSrc, uint32_t srcStride, const float *matrix,
+                                               uint8_t borderValue);
+/* replaces HipExec_Remap_U8_U8_Nearest (hip_host_decls.h:801); HafCpu_Remap_U8_U8_Nearest
+ * (ago_haf_cpu_geometric.cpp:72-154), including its different reading of (0xffff) lanes in the last
+ * dstWidth % 4 columns */
+int mvx_Remap_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                            uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride,
+                            const void *pMap, uint32_t mapStride);
+/* replaces HipExec_Remap_U8_U8_Nearest_Constant (hip_host_decls.h:807); HafCpu_Remap_U8_U8_Nearest_Constant
+ * (ago_haf_cpu_geometric.cpp:156-247) */
+int mvx_Remap_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                     uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                     uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue);
 /* The fixed-point remap table of vxSetRemapPoint (vx_api.cpp:7865-7894, 3 fractional bits) from a
  * device float table of {x, y} pairs, on the GPU.  Sources must be below 8192 pixels in each
  * dimension (the reference uses fewer fractional bits beyond, ago_util.cpp:2450-2459). */

@@ -73,10 +73,16 @@ const std::vector<KernelDef> &kernel_table();
 
 // AMD low-level kernel enums (ago_kernels.h, enum vx_kernel_amd_e)
 enum : vx_enum {
+    AMD_REMAP_U8_U8_NEAREST = 0xd000e4,
+    AMD_REMAP_U8_U8_NEAREST_CONSTANT = 0xd000e5,
     AMD_REMAP_U8_U8_BILINEAR = 0xd000e6,
     AMD_REMAP_U8_U8_BILINEAR_CONSTANT = 0xd000e7,
+    AMD_WARP_AFFINE_U8_U8_NEAREST = 0xd000e8,
+    AMD_WARP_AFFINE_U8_U8_NEAREST_CONSTANT = 0xd000e9,
     AMD_WARP_AFFINE_U8_U8_BILINEAR = 0xd000ea,
     AMD_WARP_AFFINE_U8_U8_BILINEAR_CONSTANT = 0xd000eb,
+    AMD_WARP_PERSPECTIVE_U8_U8_NEAREST = 0xd000ec,
+    AMD_WARP_PERSPECTIVE_U8_U8_NEAREST_CONSTANT = 0xd000ed,
     AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR = 0xd000ee,
     AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR_CONSTANT = 0xd000ef,
     AMD_MAGNITUDE_S16_S16S16 = 0xd0006f,

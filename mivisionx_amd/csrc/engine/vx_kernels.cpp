@@ -584,11 +584,12 @@ vx_status fixed_size_output(_vx_node *n, int i)
     return set_output(n, i, o->width, o->height, VX_DF_IMAGE_U8);
 }
 
-vx_status bilinear_and_border(_vx_node *n, int interp)
+vx_status interp_and_border(_vx_node *n, int interp)
 {
     _vx_scalar *t = scl(n, interp);
     if (!t) return fail(n, VX_ERROR_INVALID_PARAMETERS, "interpolation");
-    if (t->value.e != VX_INTERPOLATION_BILINEAR) return fail(n, VX_ERROR_NOT_SUPPORTED, "only BILINEAR interpolation is provided");
+    if (t->value.e != VX_INTERPOLATION_BILINEAR && t->value.e != VX_INTERPOLATION_NEAREST_NEIGHBOR)
+        return fail(n, VX_ERROR_INVALID_VALUE, "interpolation must be NEAREST_NEIGHBOR or BILINEAR");
     if (n->border.mode != VX_BORDER_UNDEFINED && n->border.mode != VX_BORDER_CONSTANT)
         return fail(n, VX_ERROR_NOT_SUPPORTED, "border mode must be UNDEFINED or CONSTANT");
     return VX_SUCCESS;
@@ -598,15 +599,30 @@ template <bool PERSP> vx_status v_warp(_vx_node *n)
 {
     vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
     if (!s) s = check_matrix(n, 1, PERSP ? 3 : 2);
-    if (!s) s = bilinear_and_border(n, 2);
+    if (!s) s = interp_and_border(n, 2);
     if (!s) s = fixed_size_output(n, 3);
     return s;
 }
 
+// warp_affine / warp_perspective -> WARP_*_U8_U8_{NEAREST,BILINEAR}[_CONSTANT] by interpolation and node
+// border (ago_drama_divide.cpp:1542-1616).  This engine keeps no valid-region rectangles: the nearest
+// constant affine form is given the whole source as its valid region.
 template <bool PERSP>
-vx_status run_warp(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _vx_matrix *m, bool constant, uint8_t border)
+vx_status run_warp(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _vx_matrix *m, bool constant, uint8_t border,
+                   bool nearest)
 {
     const float *mat = (const float *)m->data.data();
+    if (nearest) {
+        if (PERSP)
+            return rc(n, constant ? mvx_WarpPerspective_U8_U8_Nearest_Constant(st, o->width, o->height, P(o), S(o), in->width,
+                                                                               in->height, P(in), S(in), mat, border)
+                                  : mvx_WarpPerspective_U8_U8_Nearest(st, o->width, o->height, P(o), S(o), in->width, in->height,
+                                                                      P(in), S(in), mat));
+        return rc(n, constant ? mvx_WarpAffine_U8_U8_Nearest_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height,
+                                                                      P(in), S(in), mat, border, nullptr)
+                              : mvx_WarpAffine_U8_U8_Nearest(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in),
+                                                             S(in), mat));
+    }
     if (PERSP)
         return rc(n, constant ? mvx_WarpPerspective_U8_U8_Bilinear_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height,
                                                                             P(in), S(in), mat, border)
@@ -620,7 +636,8 @@ vx_status run_warp(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _vx
 template <bool PERSP> vx_status x_warp(_vx_node *n, hipStream_t st)
 {
     const bool constant = n->border.mode == VX_BORDER_CONSTANT;
-    return run_warp<PERSP>(n, st, img(n, 3), img(n, 0), (_vx_matrix *)n->params[1], constant, n->border.constant_value.U8);
+    return run_warp<PERSP>(n, st, img(n, 3), img(n, 0), (_vx_matrix *)n->params[1], constant, n->border.constant_value.U8,
+                           scl(n, 2)->value.e == VX_INTERPOLATION_NEAREST_NEIGHBOR);
 }
 
 // AMD forms: (out, in, matrix[, border scalar])
@@ -633,13 +650,13 @@ template <bool PERSP, bool CONST> vx_status v_amd_warp(_vx_node *n)
     return s;
 }
 
-template <bool PERSP, bool CONST> vx_status x_amd_warp(_vx_node *n, hipStream_t st)
+template <bool PERSP, bool CONST, bool NN> vx_status x_amd_warp(_vx_node *n, hipStream_t st)
 {
-    return run_warp<PERSP>(n, st, img(n, 0), img(n, 1), (_vx_matrix *)n->params[2], CONST, CONST ? scl(n, 3)->value.u8 : 0);
+    return run_warp<PERSP>(n, st, img(n, 0), img(n, 1), (_vx_matrix *)n->params[2], CONST, CONST ? scl(n, 3)->value.u8 : 0, NN);
 }
 
 // ---------------------------------------------------------------------- Remap (A16) --------------
-// remap bilinear -> REMAP_U8_U8_BILINEAR or _BILINEAR_CONSTANT (ago_drama_divide.cpp:1490-1540) over
+// remap -> REMAP_U8_U8_{NEAREST,BILINEAR}[_CONSTANT] (ago_drama_divide.cpp:1490-1540) over
 // the fixed-point table vxSetRemapPoint builds (here: built on the GPU from the float table)
 vx_status check_remap(_vx_node *n, int i, _vx_image *in)
 {
@@ -655,16 +672,22 @@ vx_status v_remap(_vx_node *n)
 {
     vx_status s = need_input(n, 0, VX_DF_IMAGE_U8);
     if (!s) s = check_remap(n, 1, img(n, 0));
-    if (!s) s = bilinear_and_border(n, 2);
+    if (!s) s = interp_and_border(n, 2);
     if (s) return s;
     _vx_remap *r = (_vx_remap *)n->params[1];
     return set_output(n, 3, r->dstWidth, r->dstHeight, VX_DF_IMAGE_U8);
 }
 
-vx_status run_remap(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _vx_remap *r, bool constant, uint8_t border)
+vx_status run_remap(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _vx_remap *r, bool constant, uint8_t border,
+                    bool nearest)
 {
     vx_status s = r->sync(st);
     if (s) return fail(n, s, "remap table upload");
+    if (nearest)
+        return rc(n, constant ? mvx_Remap_U8_U8_Nearest_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in),
+                                                                 S(in), r->devFixed.ptr(), r->dstWidth * 4, border)
+                              : mvx_Remap_U8_U8_Nearest(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
+                                                        r->devFixed.ptr(), r->dstWidth * 4));
     return rc(n, constant ? mvx_Remap_U8_U8_Bilinear_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
                                                               r->devFixed.ptr(), r->dstWidth * 4, border)
                           : mvx_Remap_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
@@ -674,7 +697,7 @@ vx_status run_remap(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _v
 vx_status x_remap(_vx_node *n, hipStream_t st)
 {
     return run_remap(n, st, img(n, 3), img(n, 0), (_vx_remap *)n->params[1], n->border.mode == VX_BORDER_CONSTANT,
-                     n->border.constant_value.U8);
+                     n->border.constant_value.U8, scl(n, 2)->value.e == VX_INTERPOLATION_NEAREST_NEIGHBOR);
 }
 
 template <bool CONST> vx_status v_amd_remap(_vx_node *n)
@@ -687,9 +710,9 @@ template <bool CONST> vx_status v_amd_remap(_vx_node *n)
     return set_output(n, 0, r->dstWidth, r->dstHeight, VX_DF_IMAGE_U8);
 }
 
-template <bool CONST> vx_status x_amd_remap(_vx_node *n, hipStream_t st)
+template <bool CONST, bool NN> vx_status x_amd_remap(_vx_node *n, hipStream_t st)
 {
-    return run_remap(n, st, img(n, 0), img(n, 1), (_vx_remap *)n->params[2], CONST, CONST ? scl(n, 3)->value.u8 : 0);
+    return run_remap(n, st, img(n, 0), img(n, 1), (_vx_remap *)n->params[2], CONST, CONST ? scl(n, 3)->value.u8 : 0, NN);
 }
 
 // ---------------------------------------------------------------------- OpticalFlowPyrLK (A17) --
@@ -824,24 +847,42 @@ const std::vector<KernelDef> &kernel_table()
          {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
          v_remap, nullptr, x_remap, AGO_TARGET_AFFINITY_GPU},
         // AMD low-level kernels, output first (ago_kernel_list.cpp:339-474)
+        {AMD_WARP_AFFINE_U8_U8_NEAREST, "com.amd.openvx.WarpAffine_U8_U8_Nearest",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}}, v_amd_warp<false, false>, nullptr,
+         x_amd_warp<false, false, true>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_WARP_AFFINE_U8_U8_NEAREST_CONSTANT, "com.amd.openvx.WarpAffine_U8_U8_Nearest_Constant",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_warp<false, true>, nullptr, x_amd_warp<false, true, true>, AGO_TARGET_AFFINITY_GPU},
         {AMD_WARP_AFFINE_U8_U8_BILINEAR, "com.amd.openvx.WarpAffine_U8_U8_Bilinear",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}}, v_amd_warp<false, false>, nullptr,
-         x_amd_warp<false, false>, AGO_TARGET_AFFINITY_GPU},
+         x_amd_warp<false, false, false>, AGO_TARGET_AFFINITY_GPU},
         {AMD_WARP_AFFINE_U8_U8_BILINEAR_CONSTANT, "com.amd.openvx.WarpAffine_U8_U8_Bilinear_Constant",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}},
-         v_amd_warp<false, true>, nullptr, x_amd_warp<false, true>, AGO_TARGET_AFFINITY_GPU},
+         v_amd_warp<false, true>, nullptr, x_amd_warp<false, true, false>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_WARP_PERSPECTIVE_U8_U8_NEAREST, "com.amd.openvx.WarpPerspective_U8_U8_Nearest",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}}, v_amd_warp<true, false>, nullptr,
+         x_amd_warp<true, false, true>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_WARP_PERSPECTIVE_U8_U8_NEAREST_CONSTANT, "com.amd.openvx.WarpPerspective_U8_U8_Nearest_Constant",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_warp<true, true>, nullptr, x_amd_warp<true, true, true>, AGO_TARGET_AFFINITY_GPU},
         {AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR, "com.amd.openvx.WarpPerspective_U8_U8_Bilinear",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}}, v_amd_warp<true, false>, nullptr,
-         x_amd_warp<true, false>, AGO_TARGET_AFFINITY_GPU},
+         x_amd_warp<true, false, false>, AGO_TARGET_AFFINITY_GPU},
         {AMD_WARP_PERSPECTIVE_U8_U8_BILINEAR_CONSTANT, "com.amd.openvx.WarpPerspective_U8_U8_Bilinear_Constant",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}},
-         v_amd_warp<true, true>, nullptr, x_amd_warp<true, true>, AGO_TARGET_AFFINITY_GPU},
+         v_amd_warp<true, true>, nullptr, x_amd_warp<true, true, false>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_REMAP_U8_U8_NEAREST, "com.amd.openvx.Remap_U8_U8_Nearest",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}}, v_amd_remap<false>, nullptr,
+         x_amd_remap<false, true>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_REMAP_U8_U8_NEAREST_CONSTANT, "com.amd.openvx.Remap_U8_U8_Nearest_Constant",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_remap<true>, nullptr, x_amd_remap<true, true>, AGO_TARGET_AFFINITY_GPU},
         {AMD_REMAP_U8_U8_BILINEAR, "com.amd.openvx.Remap_U8_U8_Bilinear",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}}, v_amd_remap<false>, nullptr,
-         x_amd_remap<false>, AGO_TARGET_AFFINITY_GPU},
+         x_amd_remap<false, false>, AGO_TARGET_AFFINITY_GPU},
         {AMD_REMAP_U8_U8_BILINEAR_CONSTANT, "com.amd.openvx.Remap_U8_U8_Bilinear_Constant",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_REMAP, false}, {IN, VX_TYPE_SCALAR, false}},
-         v_amd_remap<true>, nullptr, x_amd_remap<true>, AGO_TARGET_AFFINITY_GPU},
+         v_amd_remap<true>, nullptr, x_amd_remap<true, false>, AGO_TARGET_AFFINITY_GPU},
         {AMD_MAGNITUDE_S16_S16S16, "com.amd.openvx.Magnitude_S16_S16S16",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_gradpair<1, VX_DF_IMAGE_S16>, nullptr,
          x_magnitude<1>, AGO_TARGET_AFFINITY_GPU},

@@ -1,10 +1,13 @@
-// warp.hip -- bilinear WarpAffine, WarpPerspective and Remap (U8 -> U8, UNDEFINED and CONSTANT
-// borders) and the float -> fixed-point remap table conversion, for gfx950.
+// warp.hip -- bilinear and nearest-neighbour WarpAffine, WarpPerspective and Remap (U8 -> U8,
+// UNDEFINED and CONSTANT borders) and the float -> fixed-point remap table conversion, for gfx950.
 //
 // Each thread produces four consecutive output pixels and writes them with one 32-bit store, the
-// reference's store unit: rows are written up to ceil(w/4)*4 pixels, so pixels w..ceil4(w)-1 land
-// in the row padding exactly as the reference's SSE loops leave them there (a 3x3 node reading
-// the output with linear-address neighbours then sees the same bytes).  The source is read at
+// reference's store unit: warp rows are written up to ceil(w/4)*4 pixels, so pixels w..ceil4(w)-1
+// land in the row padding with the values the reference's SSE loops leave there (a 3x3 node reading
+// the output with linear-address neighbours then sees the same bytes).  The reference evaluates the
+// warps from per-column coefficient tables (a*x for x < w) whose slots past w are zero, so a padding
+// pixel is column 0's with a coefficient of +0.0f; bilinear remap pads from the next map entries;
+// nearest remap writes its last w % 4 pixels one byte at a time and no padding.  The source is read at
 // linear byte addresses; addresses outside [0, h*stride) read 0, the reference's zero guard.
 // All float arithmetic is single precision, uncontracted and in the reference's order; the
 // x86 conversion semantics (truncation / round-to-nearest-even, 0x80000000 for values out of int32
@@ -55,7 +58,8 @@ k_warp_affine(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstS
     uint32_t word = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const float xs = m.r00 * (float)(x0 + k) + xd, ys = m.r10 * (float)(x0 + k) + yd;
+        const bool col = x0 + k < dw;
+        const float xs = (col ? m.r00 * (float)(x0 + k) : 0.0f) + xd, ys = (col ? m.r10 * (float)(x0 + k) : 0.0f) + yd;
         const int32_t xi = cvtt(xs), yi = cvtt(ys);
         const float fx = xs - (float)xi, fy = ys - (float)yi;
         int32_t off = (int32_t)((uint32_t)yi * srcStride + (uint32_t)xi);
@@ -88,9 +92,10 @@ k_warp_perspective(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t
     uint32_t word = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
+        const bool col = x0 + k < dw;
         const float xf = (float)(x0 + k);
-        const float z = 1.0f / (m.g * xf + zd);
-        const float xs = (m.a * xf + xd) * z, ys = (m.d * xf + yd) * z;
+        const float z = 1.0f / ((col ? m.g * xf : 0.0f) + zd);
+        const float xs = ((col ? m.a * xf : 0.0f) + xd) * z, ys = ((col ? m.d * xf : 0.0f) + yd) * z;
         const int32_t xi = (int32_t)((uint32_t)cvtt(xs) - (uint32_t)(xs < 0.0f));
         const int32_t yi = (int32_t)((uint32_t)cvtt(ys) - (uint32_t)(ys < 0.0f));
         const bool mask = xi >= 0 && xi < (int32_t)sw && yi >= 0 && yi < (int32_t)sh;
@@ -114,7 +119,8 @@ k_warp_perspective(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t
 
 // HafCpu_Remap_U8_U8_Bilinear[_Constant] (ago_haf_cpu_geometric.cpp:249-504).  The map is read at
 // byte y*mapStride + 4x for every written pixel (pixels past w use the following entries, as the
-// reference's 16-byte map loads do); entries past the end of the map read as (0xffff, 0xffff).
+// reference's 16-byte map loads do); entries past the end of the map read as (0, 0), the zeroed
+// tail of the reference's map allocation.
 template <bool CONSTANT>
 __global__ void __launch_bounds__(256)
 k_remap(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sh,
@@ -129,7 +135,7 @@ k_remap(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride,
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const uint64_t mo = (uint64_t)y * mapStride + 4ull * (x0 + k);
-        const uint32_t e = (mo + 4 <= mapBytes) ? *reinterpret_cast<const uint32_t *>(map + mo) : 0xffffffffu;
+        const uint32_t e = (mo + 4 <= mapBytes) ? *reinterpret_cast<const uint32_t *>(map + mo) : 0u;
         uint32_t mx = e & 0xffffu, my = e >> 16;
         const bool xbad = mx == 0xffffu;
         if (mx == 0xffffu) mx = CONSTANT ? 0u : 8u;
@@ -143,6 +149,118 @@ k_remap(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride,
         word |= sat8(v) << (8 * k);
     }
     *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
+// HafCpu_WarpAffine_U8_U8_Nearest[_Constant] (ago_haf_cpu_geometric.cpp:508-823): trunc(xs), trunc(ys),
+// offset yi*stride + xi.  BORDER (the corner test failed): mask 0 <= xs < W, 0 <= ys < H and inside
+// the valid rectangle [vl, vr) x [vt, vb), offset clamped to [0, H*stride - 1], the border value
+// outside the mask (0 for UNDEFINED); otherwise the plain read.
+template <bool BORDER>
+__global__ void __launch_bounds__(256)
+k_warp_affine_nn(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                 const uint8_t *__restrict__ src, uint32_t srcStride, Affine m, uint32_t border, float vl, float vt, float vr,
+                 float vb)
+{
+    const uint32_t x0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const uint64_t n = (uint64_t)sh * srcStride;
+    const int32_t limit = (int32_t)(sh * srcStride - 1u);
+    const float fw = (float)sw, fh = (float)sh;
+    const float xd = (float)y * m.r01 + m.c1, yd = (float)y * m.r11 + m.c2;
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool col = x0 + k < dw;
+        const float xs = (col ? m.r00 * (float)(x0 + k) : 0.0f) + xd, ys = (col ? m.r10 * (float)(x0 + k) : 0.0f) + yd;
+        const int32_t xi = cvtt(xs), yi = cvtt(ys);
+        int32_t off = (int32_t)((uint32_t)yi * srcStride + (uint32_t)xi);
+        uint32_t px;
+        if (BORDER) {
+            const bool mask = xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh && xs >= vl && xs < vr && ys >= vt && ys < vb;
+            off = max(min(off, limit), 0);
+            px = mask ? ldg8(src, n, off) : border;
+        } else {
+            px = ldg8(src, n, off);
+        }
+        word |= px << (8 * k);
+    }
+    *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
+// HafCpu_WarpPerspective_U8_U8_Nearest_Constant (ago_haf_cpu_geometric.cpp:1406-1558; UNDEFINED is
+// border 0, :1372-1385): always the masked loop, mask 0 <= xs < W, 0 <= ys < H, offset
+// trunc(ys)*stride + trunc(xs) clamped to [0, H*stride - 1].
+__global__ void __launch_bounds__(256)
+k_warp_perspective_nn(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                      const uint8_t *__restrict__ src, uint32_t srcStride, Persp m, uint32_t border)
+{
+    const uint32_t x0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const uint64_t n = (uint64_t)sh * srcStride;
+    const int32_t limit = (int32_t)(sh * srcStride - 1u);
+    const float fw = (float)sw, fh = (float)sh;
+    const float xd = (float)y * m.b + m.c, yd = (float)y * m.e + m.f, zd = (float)y * m.h + m.i;
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool col = x0 + k < dw;
+        const float xf = (float)(x0 + k);
+        const float z = 1.0f / ((col ? m.g * xf : 0.0f) + zd);
+        const float xs = ((col ? m.a * xf : 0.0f) + xd) * z, ys = ((col ? m.d * xf : 0.0f) + yd) * z;
+        const bool mask = xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh;
+        int32_t off = (int32_t)((uint32_t)cvtt(ys) * srcStride + (uint32_t)cvtt(xs));
+        off = max(min(off, limit), 0);
+        word |= (mask ? ldg8(src, n, off) : border) << (8 * k);
+    }
+    *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
+// HafCpu_Remap_U8_U8_Nearest[_Constant] (ago_haf_cpu_geometric.cpp:72-247).  Columns below w & ~3 (the
+// reference's 4-pixel body), per 16-bit lane v: coordinate (v >> 3) + ((v & 7) > 3); a lane of 0xffff
+// reads as 7 (UNDEFINED: coordinate 1) or 0 (CONSTANT, where the pixel is then the border value, as
+// it is when the offset passes H*stride - 1).  The last w % 4 columns (its byte loop) take each lane
+// as a signed 16-bit s: (s >> 3) + ((s & 7) >> 2), and s == -1 gives 0 (UNDEFINED) or the border value
+// itself as the coordinate (CONSTANT); no bounds test there (offsets outside the plane read 0).
+template <bool CONSTANT>
+__global__ void __launch_bounds__(256)
+k_remap_nn(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sh,
+           const uint8_t *__restrict__ src, uint32_t srcStride, const uint8_t *__restrict__ map, uint32_t mapStride,
+           uint32_t border)
+{
+    const uint32_t x0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const uint64_t n = (uint64_t)sh * srcStride;
+    const int32_t limit = (int32_t)(sh * srcStride - 1u);
+    const uint32_t *row = reinterpret_cast<const uint32_t *>(map + (uint64_t)y * mapStride);
+    uint8_t *d = dst + (uint64_t)y * dstStride + x0;
+    if (x0 + 4 <= dw) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t e = row[x0 + k];
+            uint32_t mx = e & 0xffffu, my = e >> 16;
+            const bool bad = mx == 0xffffu || my == 0xffffu;
+            if (mx == 0xffffu) mx = CONSTANT ? 0u : 7u;
+            if (my == 0xffffu) my = CONSTANT ? 0u : 7u;
+            const uint32_t xi = (mx >> 3) + ((mx & 7u) > 3u), yi = (my >> 3) + ((my & 7u) > 3u);
+            const int32_t off = (int32_t)(yi * srcStride + xi);
+            const uint32_t px = (CONSTANT && (bad || off > limit)) ? border : ldg8(src, n, off);
+            word |= px << (8 * k);
+        }
+        *reinterpret_cast<uint32_t *>(d) = word;
+    } else {
+        for (uint32_t k = 0; k < dw - x0; k++) {
+            const uint32_t e = row[x0 + k];
+            const int32_t sx = (int16_t)(e & 0xffffu), sy = (int16_t)(e >> 16);
+            const int32_t xi = sx != -1 ? (sx >> 3) + ((sx & 7) >> 2) : (CONSTANT ? (int32_t)border : 0);
+            const int32_t yi = sy != -1 ? (sy >> 3) + ((sy & 7) >> 2) : (CONSTANT ? (int32_t)border : 0);
+            const uint32_t off = (uint32_t)yi * srcStride + (uint32_t)xi;
+            d[k] = (uint8_t)ldg8(src, n, (int64_t)off);
+        }
+    }
 }
 
 // vxSetRemapPoint (vx_api.cpp:7865-7894) for every entry: x = (u16)(sx*8 + 0.5f), y likewise,
@@ -219,6 +337,53 @@ int warp_perspective(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst
     return MVX_SUCCESS;
 }
 
+int warp_affine_nn(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                   const uint8_t *src, uint32_t srcStride, const float *matrix, uint32_t border, const uint32_t *valid)
+{
+    if (!matrix || !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride)) return MVX_ERROR_INVALID_PARAMETERS;
+    const Affine m{matrix[0], matrix[1], matrix[2], matrix[3], matrix[4], matrix[5]};
+    const dim3 block(64, 4);
+    const float vl = valid ? (float)valid[0] : 0.0f, vt = valid ? (float)valid[1] : 0.0f;
+    const float vr = valid ? (float)valid[2] : (float)sw, vb = valid ? (float)valid[3] : (float)sh;
+    if (affine_needs_border(dw, dh, sw, sh, m))
+        hipLaunchKernelGGL(k_warp_affine_nn<true>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw,
+                           sh, src, srcStride, m, border, vl, vt, vr, vb);
+    else
+        hipLaunchKernelGGL(k_warp_affine_nn<false>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw,
+                           sh, src, srcStride, m, 0u, vl, vt, vr, vb);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+int warp_perspective_nn(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                        const uint8_t *src, uint32_t srcStride, const float *matrix, uint32_t border)
+{
+    if (!matrix || !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride)) return MVX_ERROR_INVALID_PARAMETERS;
+    const Persp m{matrix[0], matrix[1], matrix[2], matrix[3], matrix[4], matrix[5], matrix[6], matrix[7], matrix[8]};
+    const dim3 block(64, 4);
+    hipLaunchKernelGGL(k_warp_perspective_nn, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh,
+                       src, srcStride, m, border);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+int remap_nn(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+             const uint8_t *src, uint32_t srcStride, const void *map, uint32_t mapStride, bool constant, uint32_t border)
+{
+    if (!map || ((uintptr_t)map & 3u) || (mapStride & 3u) || mapStride < 4u * dw ||
+        !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    const dim3 block(64, 4);
+    if (constant)
+        hipLaunchKernelGGL(k_remap_nn<true>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sh, src,
+                           srcStride, (const uint8_t *)map, mapStride, border);
+    else
+        hipLaunchKernelGGL(k_remap_nn<false>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sh, src,
+                           srcStride, (const uint8_t *)map, mapStride, 0u);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
 int remap(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
           const uint8_t *src, uint32_t srcStride, const void *map, uint32_t mapStride, bool constant, uint32_t border)
 {
@@ -284,6 +449,53 @@ extern "C" int mvx_Remap_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t d
                                                  uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue)
 {
     return remap(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, true, borderValue);
+}
+
+extern "C" int mvx_WarpAffine_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                            uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                            uint32_t srcStride, const float *matrix)
+{
+    return warp_affine_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, 0, nullptr);
+}
+
+extern "C" int mvx_WarpAffine_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                     uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight,
+                                                     const uint8_t *pSrc, uint32_t srcStride, const float *matrix,
+                                                     uint8_t borderValue, const uint32_t *validRect)
+{
+    return warp_affine_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, borderValue,
+                          validRect);
+}
+
+extern "C" int mvx_WarpPerspective_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                 uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                                 uint32_t srcStride, const float *matrix)
+{
+    return warp_perspective_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, 0);
+}
+
+extern "C" int mvx_WarpPerspective_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                                          uint8_t *pDst, uint32_t dstStride, uint32_t srcWidth,
+                                                          uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride,
+                                                          const float *matrix, uint8_t borderValue)
+{
+    return warp_perspective_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix,
+                               borderValue);
+}
+
+extern "C" int mvx_Remap_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                       uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                       uint32_t srcStride, const void *pMap, uint32_t mapStride)
+{
+    return remap_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, false, 0);
+}
+
+extern "C" int mvx_Remap_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                                uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue)
+{
+    return remap_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, true,
+                    borderValue);
 }
 
 extern "C" int mvx_Remap_FixedMap_F32(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, void *pFixedMap,

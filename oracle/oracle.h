@@ -84,6 +84,14 @@ void oracle_warp_affine_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, i
 void oracle_warp_perspective_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
                                       int src_stride, const float m[9], int border, uint8_t *unpinned, int unpinned_stride);
 void oracle_remap_fixed(int dw, int dh, int sw, int sh, const float *map, uint16_t *fixed);
+void oracle_warp_affine_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                int src_stride, const float m[6], int border, const int valid[4]);
+void oracle_warp_perspective_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                     int src_stride, const float m[9], int border);
+void oracle_remap_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                          int src_stride, const uint16_t *fixed, int constant, int border, uint8_t *unpinned,
+                          int unpinned_stride);
+
 void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
                            int src_stride, const uint16_t *fixed, int constant, int border);
 

@@ -274,6 +274,37 @@ class Oracle:
         self.lib.oracle_remap_bilinear(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(fixed), int(constant), border)
         return d if padded else np.ascontiguousarray(d[:, :dw])
 
+    def warp_affine_nearest(self, src, dw, dh, m, border=None, valid=None, padded=False):
+        """border None: UNDEFINED (0 outside the mask); valid: the input's valid rectangle (start_x,
+        start_y, end_x, end_y), the whole image by default"""
+        sh, sw = src.shape
+        s, ss = _plane(src)
+        ds = (dw + 15) & ~15
+        d = np.zeros((dh, ds), np.uint8)
+        v = np.ascontiguousarray(valid if valid is not None else (0, 0, sw, sh), np.int32)
+        self.lib.oracle_warp_affine_nearest(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(_f32(m)), border or 0, _p(v))
+        return d if padded else np.ascontiguousarray(d[:, :dw])
+
+    def warp_perspective_nearest(self, src, dw, dh, m, border=None, padded=False):
+        sh, sw = src.shape
+        s, ss = _plane(src)
+        ds = (dw + 15) & ~15
+        d = np.zeros((dh, ds), np.uint8)
+        self.lib.oracle_warp_perspective_nearest(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(_f32(m)), border or 0)
+        return d if padded else np.ascontiguousarray(d[:, :dw])
+
+    def remap_nearest(self, src, mapxy, constant=False, border=0, padded=False):
+        """(dst, unpinned)"""
+        sh, sw = src.shape
+        dh, dw = mapxy.shape[:2]
+        fixed = self.remap_fixed(mapxy, sw, sh)
+        s, ss = _plane(src)
+        ds = (dw + 15) & ~15
+        d = np.zeros((dh, ds), np.uint8)
+        u = np.zeros((dh, dw), np.uint8)
+        self.lib.oracle_remap_nearest(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(fixed), int(constant), border, _p(u), dw)
+        return (d if padded else np.ascontiguousarray(d[:, :dw])), u.astype(bool)
+
     # ---- pyramidal Lucas-Kanade ----
     def lk_state(self, w0, h0, capacity):
         """zeroed scratch state of one tracker node (the reference node's local data)"""
@@ -454,6 +485,22 @@ class Reference:
         self._ok(self.lib.ref_remap(sw, sh, _p(_c(src)), dw, dh, _p(_f32(mapxy)), int(constant), border, _p(d), time_iters,
                                     ctypes.byref(sec)), "remap")
         return (d, sec.value) if time_iters else d
+
+    def warp_padded(self, kind, interp, src, dw, dh, m, constant=False, border=0):
+        """WarpAffine (kind 0) / WarpPerspective (kind 1) output rows of ALIGN16(dw) bytes, padding included"""
+        sh, sw = src.shape
+        d = np.zeros((dh, (dw + 15) & ~15), np.uint8)
+        self._ok(self.lib.ref_warp_padded(kind, interp, sw, sh, _p(_c(src)), dw, dh, _p(_f32(m)), int(constant), border,
+                                          _p(d)), "warp_padded")
+        return d
+
+    def remap_padded(self, interp, src, mapxy, constant=False, border=0):
+        sh, sw = src.shape
+        dh, dw = mapxy.shape[:2]
+        d = np.zeros((dh, (dw + 15) & ~15), np.uint8)
+        self._ok(self.lib.ref_remap_padded(interp, sw, sh, _p(_c(src)), dw, dh, _p(_f32(mapxy)), int(constant), border,
+                                           _p(d)), "remap_padded")
+        return d
 
     def lk(self, old_img, new_img, levels, old_kp, est_kp, termination, epsilon, num_iterations, use_initial_estimate,
            window, reps=1, timed=False):

@@ -54,6 +54,22 @@ vx_status get(vx_image img, int w, int h, void *dst, int bpp, int plane = 0) {
     return vxCopyImagePatch(img, &r, plane, &a, dst, VX_READ_ONLY, VX_MEMORY_TYPE_HOST);
 }
 
+// U8 rows including their padding: the image's own buffer (vxMapImagePatch returns a pointer into it)
+// read over ALIGN16(w) bytes per row, so the bytes the reference's 4-pixel stores leave past the
+// width are seen too
+vx_status get_padded(vx_image img, int w, int h, uint8_t *dst) {
+    vx_rectangle_t r = {0, 0, (vx_uint32)w, (vx_uint32)h};
+    vx_imagepatch_addressing_t a = {};
+    vx_map_id id;
+    void *ptr = nullptr;
+    vx_status s = vxMapImagePatch(img, &r, 0, &id, &a, &ptr, VX_READ_ONLY, VX_MEMORY_TYPE_HOST, 0);
+    if (s != VX_SUCCESS) return s;
+    const int ds = (w + 15) & ~15;
+    if (a.stride_y < ds) { vxUnmapImagePatch(img, id); return VX_FAILURE; }
+    for (int y = 0; y < h; y++) memcpy(dst + (size_t)y * ds, (const uint8_t *)ptr + (size_t)y * a.stride_y, (size_t)ds);
+    return vxUnmapImagePatch(img, id);
+}
+
 vx_image make_input(Ctx &c, int w, int h, vx_df_image f, const void *src) {
     vx_image img = vxCreateImage(c.ctx, w, h, f);
     if (put(img, w, h, src, bpp_of(f)) != VX_SUCCESS) return nullptr;
@@ -390,6 +406,52 @@ int ref_warp(int kind, int sw, int sh, const uint8_t *src, int dw, int dh, const
         auto t1 = std::chrono::steady_clock::now();
         *sec_per_frame = std::chrono::duration<double>(t1 - t0).count() / time_iters;
     }
+    return 0;
+}
+
+// WarpAffine / WarpPerspective with any interpolation; `dst` receives ALIGN16(dw)-byte rows, padding included.
+int ref_warp_padded(int kind, int interp, int sw, int sh, const uint8_t *src, int dw, int dh, const float *m,
+                    int constant, int border, uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    vx_matrix mat = vxCreateMatrix(c.ctx, VX_TYPE_FLOAT32, kind ? 3 : 2, 3);
+    CHECK(vxCopyMatrix(mat, (void *)m, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    vx_node n = kind ? vxWarpPerspectiveNode(c.graph, in, mat, (vx_enum)interp, out)
+                     : vxWarpAffineNode(c.graph, in, mat, (vx_enum)interp, out);
+    if (!n) return -1;
+    if (constant) {
+        vx_border_t b;
+        memset(&b, 0, sizeof(b));
+        b.mode = VX_BORDER_CONSTANT;
+        b.constant_value.U8 = (vx_uint8)border;
+        CHECK(vxSetNodeAttribute(n, VX_NODE_BORDER, &b, sizeof(b)));
+    }
+    int s = run(c); if (s) return s;
+    CHECK(get_padded(out, dw, dh, dst));
+    return 0;
+}
+
+// Remap with any interpolation from a float map; `dst` receives ALIGN16(dw)-byte rows, padding included.
+int ref_remap_padded(int interp, int sw, int sh, const uint8_t *src, int dw, int dh, const float *map, int constant,
+                     int border, uint8_t *dst) {
+    Ctx c;
+    vx_image in = make_input(c, sw, sh, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, dw, dh, VX_DF_IMAGE_U8);
+    vx_remap rm = vxCreateRemap(c.ctx, sw, sh, dw, dh);
+    vx_rectangle_t r = {0, 0, (vx_uint32)dw, (vx_uint32)dh};
+    CHECK(vxCopyRemapPatch(rm, &r, (vx_size)dw * 8, (void *)map, VX_TYPE_COORDINATES2DF, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    vx_node n = vxRemapNode(c.graph, in, rm, (vx_enum)interp, out);
+    if (!n) return -1;
+    if (constant) {
+        vx_border_t b;
+        memset(&b, 0, sizeof(b));
+        b.mode = VX_BORDER_CONSTANT;
+        b.constant_value.U8 = (vx_uint8)border;
+        CHECK(vxSetNodeAttribute(n, VX_NODE_BORDER, &b, sizeof(b)));
+    }
+    int s = run(c); if (s) return s;
+    CHECK(get_padded(out, dw, dh, dst));
     return 0;
 }
 

@@ -9,8 +9,11 @@
  * its 64 guard bytes that is heap memory no restatement can know.  Where such a read carries a
  * non-zero weight the pixel is flagged in `unpinned` (parity unpinned, masked in comparisons).
  *
- * Every output row writes ceil(w/4)*4 pixels, as the reference's 4-pixel stores do: pixels w..
- * ceil4(w)-1 land in the row's padding (stride >= ALIGN16(w)).
+ * The warps write every output row up to ceil(w/4)*4 pixels, as the reference's 4-pixel stores do:
+ * pixels w..ceil4(w)-1 land in the row's padding (stride >= ALIGN16(w)).  The reference evaluates
+ * those from its per-column coefficient tables (a*x, ... for x < w), whose slots past w are the
+ * zeroed gap before the next 16-byte aligned table: padding pixels are computed as column 0 with
+ * a table value of +0.0f.  Bilinear remap pads from the following map entries.
  */
 #include "oracle.h"
 #include <math.h>
@@ -84,7 +87,8 @@ void oracle_warp_affine_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, i
     for (int y = 0; y < dh; y++) {
         const float xd = (float)y * r01 + c1, yd = (float)y * r11 + c2;
         for (int x = 0; x < wq; x++) {
-            const float xs = r00 * (float)x + xd, ys = r10 * (float)x + yd;
+            const float tx = x < dw ? r00 * (float)x : 0.0f, ty = x < dw ? r10 * (float)x : 0.0f;
+            const float xs = tx + xd, ys = ty + yd;
             const int inside = xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh;
             const int32_t xi = cvtt(xs), yi = cvtt(ys);
             const float fx = xs - (float)xi, fy = ys - (float)yi;
@@ -128,8 +132,9 @@ void oracle_warp_perspective_bilinear(int dw, int dh, uint8_t *dst, int dst_stri
     for (int y = 0; y < dh; y++) {
         const float xd = (float)y * b + c, yd = (float)y * e + f, zd = (float)y * hh + ii;
         for (int x = 0; x < wq; x++) {
-            const float z = 1.0f / ((g * (float)x) + zd);
-            const float xs = ((a * (float)x) + xd) * z, ys = ((d * (float)x) + yd) * z;
+            const float ax = x < dw ? a * (float)x : 0.0f, dx = x < dw ? d * (float)x : 0.0f, gx = x < dw ? g * (float)x : 0.0f;
+            const float z = 1.0f / (gx + zd);
+            const float xs = (ax + xd) * z, ys = (dx + yd) * z;
             const int32_t xi = (int32_t)((uint32_t)cvtt(xs) - (uint32_t)(xs < 0.0f));
             const int32_t yi = (int32_t)((uint32_t)cvtt(ys) - (uint32_t)(ys < 0.0f));
             const int mask = xi >= 0 && xi < sw && yi >= 0 && yi < sh;
@@ -178,7 +183,7 @@ void oracle_remap_fixed(int dw, int dh, int sw, int sh, const float *map, uint16
  * CONSTANT: the result is replaced by the border value when the X lane was 0xffff (the reference
  * masks the 32-bit result with the 16-bit compare mask, so only the x lane counts).  The map has
  * dw entries per row (stride dw*4 bytes); pixels w..ceil4(w)-1 of a row use the next row's
- * entries; past the last entry the map is read as (0xffff, 0xffff).
+ * entries; past the last entry the map reads as (0, 0), the zeroed tail of its allocation.
  * Pinned by: bit-exact against oracle/_ref; closed form for integer maps (fraction 0 copies the
  * source pixel). */
 void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
@@ -191,7 +196,7 @@ void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw,
     for (int y = 0; y < dh; y++) {
         for (int x = 0; x < wq; x++) {
             const long k = (long)y * dw + x;
-            uint32_t mx = k < entries ? fixed[2 * k] : 0xffff, my = k < entries ? fixed[2 * k + 1] : 0xffff;
+            uint32_t mx = k < entries ? fixed[2 * k] : 0, my = k < entries ? fixed[2 * k + 1] : 0;
             const int xbad = mx == 0xffff;
             if (mx == 0xffff) mx = constant ? 0 : 8;
             if (my == 0xffff) my = constant ? 0 : 8;
@@ -202,6 +207,128 @@ void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw,
             int v = (p00 * (8 - fx) * (8 - fy) + p10 * fx * (8 - fy) + p01 * (8 - fx) * fy + p11 * fx * fy + 32) >> 6;
             if (constant && xbad) v = border;
             dst[(long)y * dst_stride + x] = sat8(v);
+        }
+    }
+}
+
+/* HafCpu_WarpAffine_U8_U8_Nearest (ago_haf_cpu_geometric.cpp:508-656; border 0) and
+ * HafCpu_WarpAffine_U8_U8_Nearest_Constant (:658-823).  xs, ys and the corner test as for bilinear;
+ * xi = trunc(xs), yi = trunc(ys), offset yi*stride + xi (32-bit wrap).
+ * Masked loop (some corner outside): mask = 0 <= xs < W && 0 <= ys < H (and, for CONSTANT, inside
+ * the input's valid rectangle valid = {start_x, start_y, end_x, end_y}); offset clamped to
+ * [0, H*stride - 1]; the border value outside the mask.  Plain loop: the source byte at the offset.
+ * The reference's y-coefficient table starts right after the dw x-coefficients, so its 16-byte
+ * loads fault unless dw % 4 == 0; for other widths this restatement evaluates every column by the
+ * same formula and the padding pixels as column 0 ("parity unpinned" for dw % 4 != 0).
+ * Pinned by: bit-exact against oracle/_ref (widths multiple of 4, both loops, both borders);
+ * identity and integer translations copy the source. */
+void oracle_warp_affine_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                int src_stride, const float m[6], int border, const int valid[4])
+{
+    const float r00 = m[0], r10 = m[1], r01 = m[2], r11 = m[3], c1 = m[4], c2 = m[5];
+    const long n = (long)sh * src_stride;
+    const int bb = oracle_warp_affine_needs_border(dw, dh, sw, sh, m);
+    const int32_t limit = (int32_t)((uint32_t)sh * (uint32_t)src_stride - 1u);
+    const float fw = (float)sw, fh = (float)sh;
+    const float vl = (float)valid[0], vt = (float)valid[1], vr = (float)valid[2], vb = (float)valid[3];
+    const int wq = (dw + 3) & ~3;
+    for (int y = 0; y < dh; y++) {
+        const float xd = (float)y * r01 + c1, yd = (float)y * r11 + c2;
+        for (int x = 0; x < wq; x++) {
+            const float tx = x < dw ? r00 * (float)x : 0.0f, ty = x < dw ? r10 * (float)x : 0.0f;
+            const float xs = tx + xd, ys = ty + yd;
+            const int32_t xi = cvtt(xs), yi = cvtt(ys);
+            int32_t off = (int32_t)((uint32_t)yi * (uint32_t)src_stride + (uint32_t)xi);
+            int v;
+            if (bb) {
+                const int mask = xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh && xs >= vl && xs < vr && ys >= vt && ys < vb;
+                off = imax(imin(off, limit), 0);
+                v = mask ? rd(src, n, off) : border;
+            } else {
+                v = rd(src, n, off);
+            }
+            dst[(long)y * dst_stride + x] = (uint8_t)v;
+        }
+    }
+}
+
+/* HafCpu_WarpPerspective_U8_U8_Nearest_Constant (ago_haf_cpu_geometric.cpp:1406-1558; the UNDEFINED
+ * form :1372-1385 calls it with border 0).  z = 1 / fl(g*x + fl(y*h + i)), xs = fl(a*x + fl(y*b + c))*z,
+ * ys = fl(d*x + fl(y*e + f))*z (matrix as for bilinear); always the masked loop:
+ * mask = 0 <= xs < W && 0 <= ys < H; offset trunc(ys)*stride + trunc(xs) clamped to [0, H*stride - 1];
+ * the border value outside the mask.  Padding pixels: column 0 (zeroed table slots).
+ * Pinned by: bit-exact against oracle/_ref, padding included; the identity matrix copies the source. */
+void oracle_warp_perspective_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                                     int src_stride, const float m[9], int border)
+{
+    const float a = m[0], d = m[1], g = m[2], b = m[3], e = m[4], hh = m[5], c = m[6], f = m[7], ii = m[8];
+    const long n = (long)sh * src_stride;
+    const int32_t limit = (int32_t)((uint32_t)sh * (uint32_t)src_stride - 1u);
+    const float fw = (float)sw, fh = (float)sh;
+    const int wq = (dw + 3) & ~3;
+    for (int y = 0; y < dh; y++) {
+        const float xd = (float)y * b + c, yd = (float)y * e + f, zd = (float)y * hh + ii;
+        for (int x = 0; x < wq; x++) {
+            const float ax = x < dw ? a * (float)x : 0.0f, dx = x < dw ? d * (float)x : 0.0f, gx = x < dw ? g * (float)x : 0.0f;
+            const float z = 1.0f / (gx + zd);
+            const float xs = (ax + xd) * z, ys = (dx + yd) * z;
+            const int mask = xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh;
+            const int32_t xi = cvtt(xs), yi = cvtt(ys);
+            int32_t off = (int32_t)((uint32_t)yi * (uint32_t)src_stride + (uint32_t)xi);
+            off = imax(imin(off, limit), 0);
+            dst[(long)y * dst_stride + x] = (uint8_t)(mask ? rd(src, n, off) : border);
+        }
+    }
+}
+
+/* HafCpu_Remap_U8_U8_Nearest (ago_haf_cpu_geometric.cpp:72-154, constant == 0) and
+ * HafCpu_Remap_U8_U8_Nearest_Constant (:156-247, constant != 0) over vxSetRemapPoint's 13.3 map.
+ * Columns below dw & ~3 (the 4-pixel body), per 16-bit lane v: coordinate (v >> 3) + ((v & 7) > 3);
+ *   UNDEFINED: a lane of 0xffff reads as 7, i.e. coordinate 1; the source byte at y*stride + x;
+ *   CONSTANT: a lane of 0xffff reads as 0, and the pixel is the border value when either lane was
+ *   0xffff or the offset exceeds H*stride - 1.
+ * The last dw % 4 columns (byte loop), per lane taken as a signed 16-bit s: coordinate
+ *   (s >> 3) + ((s & 7) >> 2) unless s == -1, which gives 0 (UNDEFINED) or the border value itself
+ *   as a coordinate (CONSTANT); the source byte at y*stride + x, no bounds test (offsets outside the
+ *   plane read 0 here and are flagged in `unpinned`).
+ * No padding bytes are written.
+ * Pinned by: bit-exact against oracle/_ref (both borders, body and tail columns); integer maps copy
+ * the source pixel. */
+void oracle_remap_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
+                          int src_stride, const uint16_t *fixed, int constant, int border, uint8_t *unpinned,
+                          int unpinned_stride)
+{
+    const long n = (long)sh * src_stride;
+    const int32_t limit = (int32_t)((uint32_t)sh * (uint32_t)src_stride - 1u);
+    const int body = dw & ~3;
+    (void)sw;
+    for (int y = 0; y < dh; y++) {
+        for (int x = 0; x < dw; x++) {
+            const long k = (long)y * dw + x;
+            uint32_t mx = fixed[2 * k], my = fixed[2 * k + 1];
+            int v, far = 0;
+            if (x < body) {
+                const int bad = mx == 0xffff || my == 0xffff;
+                if (mx == 0xffff) mx = constant ? 0 : 7;
+                if (my == 0xffff) my = constant ? 0 : 7;
+                const uint32_t xi = (mx >> 3) + ((mx & 7) > 3), yi = (my >> 3) + ((my & 7) > 3);
+                const int32_t off = (int32_t)(yi * (uint32_t)src_stride + xi);
+                if (constant && (bad || off > limit)) {
+                    v = border;
+                } else {
+                    v = rd(src, n, off);
+                    far = beyond(n, off);
+                }
+            } else {
+                const int16_t sx = (int16_t)mx, sy = (int16_t)my;
+                const int xi = sx != -1 ? (sx >> 3) + ((sx & 7) >> 2) : (constant ? border : 0);
+                const int yi = sy != -1 ? (sy >> 3) + ((sy & 7) >> 2) : (constant ? border : 0);
+                const uint32_t off = (uint32_t)yi * (uint32_t)src_stride + (uint32_t)xi;
+                v = rd(src, n, (long)off);
+                far = beyond(n, (long)off);
+            }
+            dst[(long)y * dst_stride + x] = (uint8_t)v;
+            if (unpinned) unpinned[(long)y * unpinned_stride + x] = (uint8_t)far;
         }
     }
 }

@@ -1,7 +1,7 @@
-"""GPU parity of WarpAffine, WarpPerspective and Remap (bilinear, UNDEFINED and CONSTANT borders)
-and of the GPU-built fixed-point remap table: HIP kernels through the C ABI (and through the vx
-API) versus the oracle, bit-exact including the padding pixels the 4-pixel stores write, plus the
-reference engine's golden hashes."""
+"""GPU parity of WarpAffine, WarpPerspective and Remap (bilinear and nearest neighbour, UNDEFINED and
+CONSTANT borders) and of the GPU-built fixed-point remap table: HIP kernels through the C ABI (and
+through the vx API) versus the oracle, bit-exact including the padding pixels the 4-pixel stores
+write, plus the reference engine's golden hashes."""
 import ctypes
 import json
 import os
@@ -83,18 +83,19 @@ def test_warp_edge_matrices(gpu, orc):
     assert_same(_padded(d), want, "perspective with z = 0 column")
 
 
-def _remap(src_plane, mp, border):
+def _remap(src_plane, mp, border, nearest=False):
     dh, dw = mp.shape[:2]
     fmap = torch.from_numpy(np.ascontiguousarray(mp)).cuda()
     fixed = torch.zeros((dh, dw * 4), dtype=torch.uint8, device="cuda")
     call("Remap_FixedMap_F32", dw, dh, ctypes.c_void_p(fixed.data_ptr()), dw * 4, ctypes.c_void_p(fmap.data_ptr()), dw * 8,
          src_plane.width, src_plane.height)
     d = Plane(dw, dh)
+    name = "Remap_U8_U8_Nearest" if nearest else "Remap_U8_U8_Bilinear"
     if border is None:
-        call("Remap_U8_U8_Bilinear", dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride,
+        call(name, dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride,
              ctypes.c_void_p(fixed.data_ptr()), dw * 4)
     else:
-        call("Remap_U8_U8_Bilinear_Constant", dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride,
+        call(name + "_Constant", dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride,
              ctypes.c_void_p(fixed.data_ptr()), dw * 4, ctypes.c_uint8(border))
     return d, fixed.cpu().numpy().view(np.uint16).reshape(dh, dw, 2)
 
@@ -112,6 +113,59 @@ def test_remap_matches_oracle(gpu, orc, w, h):
             assert_same(fixed, orc.remap_fixed(mp, w, h), f"fixed map {w}x{h} k1 {k1}")
             assert_same(_padded(d), orc.remap(src, mp, border is not None, border or 0, padded=True),
                         f"remap {w}x{h} k1 {k1} border {border}")
+
+
+def _run_warp_nn(kind, src_plane, dw, dh, m, border, valid=None):
+    d = Plane(dw, dh)
+    keep, mp = _fptr(m)
+    name = ("WarpAffine" if kind == 0 else "WarpPerspective") + "_U8_U8_Nearest"
+    args = (dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride, mp)
+    if border is None:
+        call(name, *args)
+    elif kind == 0:
+        rect = (ctypes.c_uint32 * 4)(*valid) if valid is not None else None
+        call(name + "_Constant", *args, ctypes.c_uint8(border), rect)
+    else:
+        call(name + "_Constant", *args, ctypes.c_uint8(border))
+    return d
+
+
+@pytest.mark.parametrize("w,h", SIZES)
+def test_nearest_warps_match_oracle(gpu, orc, w, h):
+    src = synth.scene_u8(w, h, 44)
+    s = Plane.from_array(src)
+    for seed in ((1, 2) if w * h < 4_000_000 else (1,)):
+        aff, per = synth.warp_matrices(w, h, seed)
+        for (dw, dh) in ((w, h), (w + 5, max(3, h - 3)), (w + 4, h + 1)):
+            for border in (None, 0, 77):
+                d = _run_warp_nn(0, s, dw, dh, aff, border)
+                assert_same(_padded(d), orc.warp_affine_nearest(src, dw, dh, aff, border, padded=True),
+                            f"affine nearest {w}x{h}->{dw}x{dh} seed {seed} border {border}")
+                d = _run_warp_nn(1, s, dw, dh, per, border)
+                assert_same(_padded(d), orc.warp_perspective_nearest(src, dw, dh, per, border, padded=True),
+                            f"perspective nearest {w}x{h}->{dw}x{dh} seed {seed} border {border}")
+    # a valid rectangle narrower than the source masks the CONSTANT form further
+    aff = np.array([0.9, 0.05, -0.04, 0.92, 3.3, 2.7], np.float32)
+    valid = (2, 3, max(3, w - 5), max(4, h - 4))
+    d = _run_warp_nn(0, s, w, h, aff, 9, valid)
+    assert_same(_padded(d), orc.warp_affine_nearest(src, w, h, aff, 9, valid=valid, padded=True), f"affine nearest valid rect {w}x{h}")
+
+
+@pytest.mark.parametrize("w,h", SIZES)
+def test_remap_nearest_matches_oracle(gpu, orc, w, h):
+    src = synth.scene_u8(w, h, 45)
+    s = Plane.from_array(src)
+    for k1 in ((-0.2, 0.15) if w * h < 4_000_000 else (-0.2,)):
+        mp = synth.barrel_map(w, h, k1)
+        if w * h < 4_000_000:
+            mp[::5, ::3] = [-2.0, 1.0]
+        for border in (None, 3, 200):
+            d, _ = _remap(s, mp, border, nearest=True)
+            want, unp = orc.remap_nearest(src, mp, border is not None, border or 0, padded=True)
+            got = _padded(d)
+            got[:, :w][unp] = 0
+            want[:, :w][unp] = 0
+            assert_same(got, want, f"remap nearest {w}x{h} k1 {k1} border {border}")
 
 
 def test_warp_and_remap_golden(gpu, orc):
@@ -153,19 +207,22 @@ def test_warp_remap_through_vx_api(gpu, orc):
             mat = L.vxCreateMatrix(ctx, vx.VX_TYPE_FLOAT32, 2 if kind == 0 else 3, 3)
             keep = np.ascontiguousarray(m)
             assert L.vxCopyMatrix(mat, keep.ctypes.data_as(ctypes.c_void_p), vx.VX_WRITE_ONLY, vx.VX_MEMORY_TYPE_HOST) == 0
-            for border in (None, 90):
+            for border, interp in ((None, vx.VX_INTERPOLATION_BILINEAR), (90, vx.VX_INTERPOLATION_BILINEAR),
+                                   (None, vx.VX_INTERPOLATION_NEAREST_NEIGHBOR), (90, vx.VX_INTERPOLATION_NEAREST_NEIGHBOR)):
                 g = L.vxCreateGraph(ctx)
-                node = (L.vxWarpAffineNode if kind == 0 else L.vxWarpPerspectiveNode)(g, a, mat, vx.VX_INTERPOLATION_BILINEAR, out)
+                node = (L.vxWarpAffineNode if kind == 0 else L.vxWarpPerspectiveNode)(g, a, mat, interp, out)
                 if border is not None:
                     b = vx.Border(vx.VX_BORDER_CONSTANT)
                     b.constant_value.U8 = border
                     assert L.vxSetNodeAttribute(node, vx.VX_NODE_BORDER, ctypes.byref(b), ctypes.sizeof(b)) == 0
                 assert L.vxProcessGraph(g) == 0
-                if kind == 0:
+                if interp == vx.VX_INTERPOLATION_NEAREST_NEIGHBOR:
+                    want = (orc.warp_affine_nearest if kind == 0 else orc.warp_perspective_nearest)(src, w, h, m, border)
+                elif kind == 0:
                     want, _ = orc.warp_affine(src, w, h, m, border is not None, border or 0)
                 else:
                     want, _ = orc.warp_perspective(src, w, h, m, border or 0)
-                assert_same(vx.read_image(out), want, f"vx warp {kind} border {border}")
+                assert_same(vx.read_image(out), want, f"vx warp {kind} border {border} interp {interp:#x}")
                 gg = ctypes.c_void_p(g)
                 L.vxReleaseGraph(ctypes.byref(gg))
         # remap: the table set through vxCopyRemapPatch, converted on the GPU
@@ -182,6 +239,8 @@ def test_warp_remap_through_vx_api(gpu, orc):
                                   vx.VX_WRITE_ONLY, vx.VX_MEMORY_TYPE_HOST) == 0
         assert L.vxuRemap(ctx, a, rm, vx.VX_INTERPOLATION_BILINEAR, out) == 0
         assert_same(vx.read_image(out), orc.remap(src, mp2), "vxuRemap after table update")
+        assert L.vxuRemap(ctx, a, rm, vx.VX_INTERPOLATION_NEAREST_NEIGHBOR, out) == 0
+        assert_same(vx.read_image(out), orc.remap_nearest(src, mp2)[0], "vxuRemap nearest")
     finally:
         h_ = ctypes.c_void_p(ctx)
         L.vxReleaseContext(ctypes.byref(h_))

@@ -100,3 +100,81 @@ def test_remap_fixed_point_table(orc):
     fx = orc.remap_fixed(mp, 10, 10)
     # round(v * 8) with +0.5, and (0xffff, 0xffff) for sx < 0, sy < 0, sx >= W-1 or sy >= H-1
     assert fx[0].tolist() == [[0, 0], [8, 20], [0xffff, 0xffff], [0xffff, 0xffff], [72, 72], [27, 0]]
+
+
+INTERIOR = (np.array([0.9, 0.05, -0.04, 0.92, 3.3, 2.7], np.float32),
+            np.array([0.9, 0.05, 0.0001, -0.04, 0.92, 0.0002, 3.3, 2.7, 1.0], np.float32))
+
+
+def _cmp_rows(got, want, dw, what, unp=None):
+    """the first ceil4(dw) bytes of every row (the pixels and the padding the 4-pixel stores write)"""
+    q = (dw + 3) & ~3
+    diff = got[:, :q] != want[:, :q]
+    if unp is not None:
+        diff[:, :dw] &= ~unp
+    assert not diff.any(), f"{what}: {int(diff.sum())} differ, first {np.argwhere(diff)[:4].tolist()}"
+
+
+@pytest.mark.parametrize("w,h", [(333, 77), (37, 29), (44, 30)])
+def test_padding_and_nearest_match_live_reference(orc, ref, w, h):
+    """row padding of the bilinear warps/remap and every nearest-neighbour form, against the reference
+    engine's own buffers (read with their padding); nearest affine only at widths that are a multiple of
+    4, where the reference does not fault"""
+    from pyoracle import INTERP_BILINEAR, INTERP_NEAREST
+    src = synth.scene_u8(w, h, 40)
+    for seed in (1, 2, 0):
+        aff, per = synth.warp_matrices(w, h, seed) if seed else INTERIOR
+        for dw, dh in ((w, h), (w + 5, max(3, h - 3)), (w + 4, h + 2)):
+            for border in (None, 77):
+                cst, bv = border is not None, border or 0
+                tag = f"{w}x{h}->{dw}x{dh} seed {seed} border {border}"
+                got, unp = orc.warp_affine(src, dw, dh, aff, cst, bv, padded=True)
+                _cmp_rows(got, ref.warp_padded(0, INTERP_BILINEAR, src, dw, dh, aff, cst, bv), dw, "affine bilinear " + tag, unp)
+                got, unp = orc.warp_perspective(src, dw, dh, per, bv, padded=True)
+                _cmp_rows(got, ref.warp_padded(1, INTERP_BILINEAR, src, dw, dh, per, cst, bv), dw, "perspective bilinear " + tag, unp)
+                if dw % 4 == 0:
+                    got = orc.warp_affine_nearest(src, dw, dh, aff, border, padded=True)
+                    _cmp_rows(got, ref.warp_padded(0, INTERP_NEAREST, src, dw, dh, aff, cst, bv), dw, "affine nearest " + tag)
+                got = orc.warp_perspective_nearest(src, dw, dh, per, border, padded=True)
+                _cmp_rows(got, ref.warp_padded(1, INTERP_NEAREST, src, dw, dh, per, cst, bv), dw, "perspective nearest " + tag)
+    for k in (0.12, -0.1):
+        mp = synth.barrel_map(w, h, k)
+        mp[::7, ::5] = -3.0
+        for border in (None, 5, 20):
+            cst, bv = border is not None, border or 0
+            got = orc.remap(src, mp, cst, bv, padded=True)
+            _cmp_rows(got, ref.remap_padded(INTERP_BILINEAR, src, mp, cst, bv), w, f"remap bilinear k {k} border {border}")
+            got, unp = orc.remap_nearest(src, mp, cst, bv, padded=True)
+            want = ref.remap_padded(INTERP_NEAREST, src, mp, cst, bv)
+            _cmp_rows(got, want, w, f"remap nearest k {k} border {border}", unp)
+            assert not want[:, w:].any() and not got[:, w:].any()   # the nearest remap writes no padding
+
+
+def test_nearest_closed_forms(orc):
+    w, h = 52, 20
+    src = synth.uniform_u8(w, h, 32)
+    ident = np.array([1, 0, 0, 1, 0, 0], np.float32)
+    assert_same(orc.warp_affine_nearest(src, w, h, ident, 7), src, "affine nearest identity")
+    assert_same(orc.warp_perspective_nearest(src, w, h, np.array([1, 0, 0, 0, 1, 0, 0, 0, 1], np.float32), 7), src,
+                "perspective nearest identity")
+    # x_src = x + 3.6, y_src = y + 2.2: truncation picks (x + 3, y + 2); CONSTANT border outside
+    got = orc.warp_affine_nearest(src, w, h, np.array([1, 0, 0, 1, 3.6, 2.2], np.float32), 7)
+    want = np.full((h, w), 7, np.uint8)
+    want[:h - 2, :w - 4] = src[2:, 3:w - 1]
+    want[:h - 2, w - 4] = src[2:, w - 1]
+    assert_same(got, want, "affine nearest shift")
+    # padding pixels of a warp repeat column 0
+    got = orc.warp_perspective_nearest(src, 50, h, np.array([1, 0, 0, 0, 1, 0, 1.5, 0.5, 1], np.float32), None, padded=True)
+    assert (got[:, 50] == got[:, 0]).all() and (got[:, 51] == got[:, 0]).all()
+    # remap: fraction > 3/8 rounds up in the 4-pixel body and in the tail alike; out-of-source points
+    # read coordinate 1 (body) or 0 (tail) under UNDEFINED
+    yy, xx = np.mgrid[0:h, 0:w - 1]
+    mp = np.stack([np.minimum(xx + 0.5, w - 3), np.minimum(yy + 0.25, h - 3)], -1).astype(np.float32)
+    got, _ = orc.remap_nearest(src, mp)
+    assert_same(got, src[np.minimum(yy, h - 3), np.minimum(xx + 1, w - 3)], "remap nearest rounding")
+    mp[:, :] = -1.0
+    got, _ = orc.remap_nearest(src, mp)
+    body = (w - 1) & ~3
+    assert (got[:, :body] == src[1, 1]).all() and (got[:, body:] == src[0, 0]).all()
+    got, _ = orc.remap_nearest(src, mp, True, 9)
+    assert (got[:, :body] == 9).all() and (got[:, body:] == src[9, 9]).all()   # the tail reads at (border, border)
