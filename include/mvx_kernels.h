@@ -153,6 +153,24 @@ int mvx_ScaleGaussianHalf_U8_U8_5x5(mvx_stream_t stream, uint32_t dstWidth, uint
 int mvx_CannySobel_U16_U8_3x3_L1NORM(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
                                      uint16_t *pDst, uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
 
+/* replace HipExec_CannySobel_U16_U8_5x5_L1NORM, _7x7_L1NORM, _3x3_L2NORM, _5x5_L2NORM, _7x7_L2NORM
+ * (hip_host_decls.h:832, 836, 840, 844, 848); HafCpu_CannySobel_U16_U8_* (ago_haf_cpu_canny.cpp:145-427,
+ * 646-1056): rows R..h-R-1 (R = gradient/2), ((magnitude << 2) | sector) in 16 bits, the reference's
+ * 16-bit wrapping/saturating lane arithmetic, its scratch-row reads at the ends of the SSE columns and
+ * its scalar loop from column (w & ~15) - 1 (from -1 below 16 pixels of width, writing the slot before
+ * each row).  The suppression stage is mvx_CannySuppThreshold_U8XY_U16_3x3 for every size (for
+ * gradient 7 with the thresholds divided by 4, as the reference's 7x7 stage does). */
+int mvx_CannySobel_U16_U8_5x5_L1NORM(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint16_t *pDst,
+                                     uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_CannySobel_U16_U8_7x7_L1NORM(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint16_t *pDst,
+                                     uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_CannySobel_U16_U8_3x3_L2NORM(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint16_t *pDst,
+                                     uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_CannySobel_U16_U8_5x5_L2NORM(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint16_t *pDst,
+                                     uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_CannySobel_U16_U8_7x7_L2NORM(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint16_t *pDst,
+                                     uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+
 /* replaces HipExec_CannySuppThreshold_U8XY_U16_3x3 (hip_host_decls.h:852); HafCpu_CannySuppThreshold_U8XY_U16_3x3
  * (ago_haf_cpu_canny.cpp:549-596).  Writes 0/127/255 over [1,w-2]x[1,h-2].  xyStack is the
  * ago-canny-stack buffer (capacity >= w*h items of 4 bytes, ago_drama_divide.cpp:1633); it is

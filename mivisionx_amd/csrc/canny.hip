@@ -1,5 +1,5 @@
-// canny.hip -- CannyEdgeDetector (gradient 3, L1 norm) for gfx950: Sobel -> non-max suppression with
-// double threshold -> hysteresis.
+// canny.hip -- CannyEdgeDetector (gradient 3, 5, 7; L1 and L2 norms) for gfx950: Sobel -> non-max
+// suppression with double threshold -> hysteresis.
 //
 // The reference traces hysteresis with a sequential LIFO stack on the CPU (its HIP backend has no
 // trace kernel).  The traced image only depends on connectivity: a weak pixel ends at 255 iff it is
@@ -99,6 +99,194 @@ __global__ void k_canny_sobel_narrow(uint32_t w, uint32_t h, uint16_t *dst, uint
     for (int y = 1; y < (int)h - 1; y++)
         for (int x = -1; x < (int)w; x++)
             *(uint16_t *)((uint8_t *)dst + (long)y * dstStride + 2L * x) = canny_flat(src, n, srcStride, x, y);
+}
+
+// ---- gradient 5 / 7 and L2 Sobel planes (ago_haf_cpu_canny.cpp:145-427, 646-1056) -------------------
+// 16-bit lane arithmetic of the reference's SSE code
+MVX_DEV int w16(int v) { return (int16_t)(uint16_t)(uint32_t)v; }
+MVX_DEV int s16(int v) { return min(max(v, -32768), 32767); }
+MVX_DEV int sll16(int v, int k) { return w16((int)((uint32_t)(uint16_t)v << k)); }
+MVX_DEV int abs16(int v) { return w16(v < 0 ? -v : v); }
+MVX_DEV uint16_t canny_out(int mag16, uint32_t sector) { return (uint16_t)(sll16(mag16, 2) | (int)(sector & 3u)); }
+
+// floor(sqrt(F)) for F < 2^32, exact: a float estimate corrected in integer arithmetic
+MVX_DEV uint32_t isqrt_u64(uint64_t F)
+{
+    uint32_t k = (uint32_t)sqrtf((float)F);
+    while ((uint64_t)k * k > F) k--;
+    while ((uint64_t)(k + 1) * (k + 1) <= F) k++;
+    return k;
+}
+
+// _mm_sqrt_ps then _mm_cvtps_epi32 (nearest even) of an integer-valued float f in [0, 2^31], computed
+// exactly: the nearest integer to sqrt(f), unless the float-rounded root lands on the half-integer
+// k + 1/2 itself (sqrt(f) within half a float ulp of it), which then rounds to the even neighbour.
+// The bounds (k + 1/2 -/+ ulp/2)^2 have at most 50 significant bits, so double products are exact.
+MVX_DEV int l2_pack_simd(float f)
+{
+    const uint64_t F = (uint64_t)f;
+    const uint32_t k = isqrt_u64(F);
+    uint32_t K = (F >= (uint64_t)k * k + k + 1) ? k + 1 : k;
+    if (k > 0) {
+        const double m = (double)k + 0.5;
+        const double u = ldexp(1.0, ilogb(m) - 23);
+        const double lo = m - 0.5 * u, hi = m + 0.5 * u;
+        if (lo * lo < (double)F && (double)F < hi * hi) K = (k & 1u) ? k + 1 : k;
+    }
+    return K > 65535u ? 65535 : (int)K;
+}
+
+// (vx_int16)sqrt((Gx*Gx) + (Gy*Gy)): 32-bit int sum, correctly rounded double sqrt truncated -- the
+// integer square root -- with 0x80000000 for a negative (wrapped) sum, then 16 bits
+MVX_DEV int l2_scalar(int gx, int gy)
+{
+    const int32_t q = (int32_t)((uint32_t)(gx * gx) + (uint32_t)(gy * gy));
+    const int32_t t = q < 0 ? INT32_MIN : (int32_t)isqrt_u64((uint64_t)q);
+    return (int16_t)(uint16_t)(uint32_t)t;
+}
+
+template <int G> struct Taps;
+template <> struct Taps<3> { static constexpr int sm[3] = {1, 2, 1}, dv[3] = {-1, 0, 1}; };
+template <> struct Taps<5> { static constexpr int sm[5] = {1, 4, 6, 4, 1}, dv[5] = {-1, -2, 0, 2, 1}; };
+template <> struct Taps<7> { static constexpr int sm[7] = {1, 6, 15, 20, 15, 6, 1}, dv[7] = {-1, -4, -5, 0, 5, 4, 1}; };
+
+// the vertical pass of the SSE loop at column c (< aligned): r0 = smoothing rows, r1 = derivative
+// rows (gradient 7: both >> 2)
+template <int G> MVX_DEV void canny_vertical(const uint8_t *src, long stride, int c, int y, int &t0, int &t1)
+{
+    constexpr int R = G / 2;
+    int s[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) s[k] = src[(long)(y - R + k) * stride + c];
+    if (G == 3) {
+        t0 = s[0] + s[2] + 2 * s[1];
+        t1 = s[2] - s[0];
+    } else if (G == 5) {
+        t0 = 4 * (s[1] + s[3]) + 6 * s[2] + (s[0] + s[4]);
+        t1 = 2 * (s[3] - s[1]) + (s[4] - s[0]);
+    } else {
+        t0 = (6 * (s[1] + s[5]) + 20 * s[3] + (s[0] + s[6]) + 15 * (s[2] + s[4])) >> 2;
+        t1 = (4 * (s[5] - s[1]) + (s[6] - s[0]) + 5 * (s[4] - s[2])) >> 2;
+    }
+}
+
+// the scalar loop's value at (x, y), linear-address reads
+template <int G, bool L2> MVX_DEV uint16_t canny_scalar(const uint8_t *src, long n, long stride, int x, int y)
+{
+    constexpr int R = G / 2;
+    int gxs = 0, gys = 0;
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+        const long b = (long)(y - R + j) * stride + x;
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int p = (int)ld_u8(src, n, b + i - R);
+            gxs += Taps<G>::sm[j] * Taps<G>::dv[i] * p;
+            // 5x5 L2: the first row's term lacks its parentheses (ago_haf_cpu_canny.cpp:880)
+            const int wy = (G == 5 && L2 && j == 0 && i > 0) ? Taps<G>::sm[i] : Taps<G>::dv[j] * Taps<G>::sm[i];
+            gys += wy * p;
+        }
+    }
+    const int gx = w16(gxs);
+    int gy = w16(gys);
+    if (G == 5 && !L2) gy = w16(-gy);
+    const int mag = L2 ? l2_scalar(gx, gy) : w16(abs(gx) + abs(gy));
+    return canny_out(mag, canny_sector(gx, gy));
+}
+
+// The SSE passes over [0, aligned), aligned = w & ~15 >= 16: scratch rows r0, r1 (r1 = r0 + ALIGN16(w))
+// with zeros before r0, after r1 and in their unwritten tails; a read past the end of r0 lands in r1
+// (and one before r1 in r0) when w is a multiple of 16.  Row y of the block stages the scratch words
+// its 256 columns read (x-R .. x+255+R) in LDS, then each thread finishes one column; columns
+// aligned-1 .. w-1 take the scalar loop's value.
+template <int G, bool L2>
+__global__ void __launch_bounds__(256)
+k_canny_sobel_g(uint32_t w, uint32_t h, uint16_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
+                uint32_t srcStride)
+{
+    constexpr int R = G / 2;
+    __shared__ int16_t l0[256 + 2 * R], l1[256 + 2 * R];
+    const int x0 = (int)blockIdx.x * 256, y = R + (int)blockIdx.y;
+    const int aligned = (int)(w & ~15u), a16 = (int)((w + 15u) & ~15u);
+    for (int c = (int)threadIdx.x; c < 256 + 2 * R; c += 256) {
+        const int col = x0 - R + c;
+        int v0 = 0, v1 = 0, t0, t1;
+        // r0[col]
+        if (col >= 0 && col < aligned) {
+            canny_vertical<G>(src, srcStride, col, y, t0, t1);
+            v0 = t0;
+        } else if (col >= a16 && col - a16 < aligned) {
+            canny_vertical<G>(src, srcStride, col - a16, y, t0, t1);
+            v0 = t1;
+        }
+        // r1[col]
+        if (col >= 0 && col < aligned) {
+            canny_vertical<G>(src, srcStride, col, y, t0, t1);
+            v1 = t1;
+        } else if (col < 0 && a16 + col < aligned) {
+            canny_vertical<G>(src, srcStride, a16 + col, y, t0, t1);
+            v1 = t0;
+        }
+        l0[c] = (int16_t)v0;
+        l1[c] = (int16_t)v1;
+    }
+    __syncthreads();
+    const int x = x0 + (int)threadIdx.x;
+    if (x >= (int)w) return;
+    uint16_t out;
+    if (x < aligned - 1) {
+        const int c = (int)threadIdx.x + R;
+#define A0(d) ((int)l0[c + (d)])
+#define A1(d) ((int)l1[c + (d)])
+        int gx, gy;
+        if (G == 3) {
+            gx = w16(A0(1) - A0(-1));
+            gy = w16(-w16(w16(A1(-1) + A1(1)) + sll16(A1(0), 1)));
+        } else if (G == 5) {
+            gx = s16(sll16(w16(A0(1) - A0(-1)), 1) + w16(A0(2) - A0(-2)));
+            int t = w16(sll16(w16(A1(-1) + A1(1)), 2) + w16(A1(-2) + A1(2)));
+            t = L2 ? s16(t + w16(A1(0) * 6)) : w16(t + w16(A1(0) * 6));
+            gy = w16(-t);
+        } else {
+            int t = sll16(s16(A0(2) - A0(-2)), 2);
+            const int u = w16(s16(A0(1) - A0(-1)) * 5);
+            t = s16(t + s16(A0(3) - A0(-3)));
+            gx = s16(t + u);
+            int v = s16(w16(w16(A1(-2) + A1(2)) * 6) + w16(A1(0) * 20));
+            const int q = w16(w16(A1(-1) + A1(1)) * 15);
+            v = s16(v + s16(A1(-3) + A1(3)));
+            v = s16(v + q);
+            gy = s16(-v);
+        }
+#undef A0
+#undef A1
+        int mag;
+        if (!L2) {
+            mag = w16(abs16(gx) + abs16(gy));
+            if (G == 7 && mag > 0x3fff) mag = 0x3fff;
+        } else if (G == 3) {
+            mag = l2_pack_simd((float)(uint16_t)w16(gx * gx) + (float)(uint16_t)w16(gy * gy));
+        } else {
+            mag = l2_pack_simd((float)(gx * gx + gy * gy));
+        }
+        out = canny_out(mag, canny_sector(gx, gy));
+    } else {
+        out = canny_scalar<G, L2>(src, (long)h * srcStride, srcStride, x, y);
+    }
+    *(uint16_t *)((uint8_t *)dst + (long)y * dstStride + 2L * x) = out;
+}
+
+// widths below 16: only the scalar loop runs, from x = -1 (the slot before each row), rows in order
+template <int G, bool L2>
+__global__ void k_canny_sobel_g_narrow(uint32_t w, uint32_t h, uint16_t *dst, uint32_t dstStride, const uint8_t *src,
+                                       uint32_t srcStride)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    constexpr int R = G / 2;
+    const long n = (long)h * srcStride;
+    for (int y = R; y < (int)h - R; y++)
+        for (int x = -1; x < (int)w; x++)
+            *(uint16_t *)((uint8_t *)dst + (long)y * dstStride + 2L * x) = canny_scalar<G, L2>(src, n, srcStride, x, y);
 }
 
 // CannySuppThreshold_U8XY_U16_3x3 (ago_haf_cpu_canny.cpp:549-596): x in [1, w-2], y in [1, h-2].
@@ -204,6 +392,37 @@ k_canny_final(uint32_t w, uint32_t h, uint8_t *__restrict__ img, uint32_t stride
 }  // namespace mvx
 
 using namespace mvx;
+
+namespace {
+template <int G, bool L2>
+int canny_sobel_g(mvx_stream_t stream, uint32_t w, uint32_t h, uint16_t *dst, uint32_t dstStride, const uint8_t *src,
+                  uint32_t srcStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok_c(dst, w, dstStride, 2) || !plane_ok_c(src, w, srcStride, 1)) return MVX_ERROR_INVALID_PARAMETERS;
+    if ((int)h < G) return MVX_SUCCESS;
+    if (w < 16)
+        hipLaunchKernelGGL((k_canny_sobel_g_narrow<G, L2>), dim3(1), dim3(64), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride);
+    else
+        hipLaunchKernelGGL((k_canny_sobel_g<G, L2>), dim3((w + 255) / 256, h - 2 * (G / 2)), dim3(256), 0, stream_of(stream), w, h, dst,
+                           dstStride, src, srcStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+}  // namespace
+
+#define MVX_CANNY_SOBEL(NAME, G, L2)                                                                                      \
+    extern "C" int NAME(mvx_stream_t stream, uint32_t w, uint32_t h, uint16_t *dst, uint32_t dstStride, const uint8_t *src, \
+                        uint32_t srcStride)                                                                               \
+    {                                                                                                                     \
+        return canny_sobel_g<G, L2>(stream, w, h, dst, dstStride, src, srcStride);                                         \
+    }
+MVX_CANNY_SOBEL(mvx_CannySobel_U16_U8_5x5_L1NORM, 5, false)
+MVX_CANNY_SOBEL(mvx_CannySobel_U16_U8_7x7_L1NORM, 7, false)
+MVX_CANNY_SOBEL(mvx_CannySobel_U16_U8_3x3_L2NORM, 3, true)
+MVX_CANNY_SOBEL(mvx_CannySobel_U16_U8_5x5_L2NORM, 5, true)
+MVX_CANNY_SOBEL(mvx_CannySobel_U16_U8_7x7_L2NORM, 7, true)
+#undef MVX_CANNY_SOBEL
 
 extern "C" int mvx_CannySobel_U16_U8_3x3_L1NORM(mvx_stream_t stream, uint32_t w, uint32_t h, uint16_t *dst,
                                                 uint32_t dstStride, const uint8_t *src, uint32_t srcStride)

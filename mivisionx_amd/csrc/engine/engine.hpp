@@ -96,6 +96,11 @@ enum : vx_enum {
     AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
     AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,
     AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM = 0xd000da,
+    AMD_CANNY_SOBEL_U16_U8_3x3_L2NORM = 0xd000db,
+    AMD_CANNY_SOBEL_U16_U8_5x5_L1NORM = 0xd000dc,
+    AMD_CANNY_SOBEL_U16_U8_5x5_L2NORM = 0xd000dd,
+    AMD_CANNY_SOBEL_U16_U8_7x7_L1NORM = 0xd000de,
+    AMD_CANNY_SOBEL_U16_U8_7x7_L2NORM = 0xd000df,
     AMD_SCALE_IMAGE_U8_U8_NEAREST = 0xd000f0,
     AMD_SCALE_IMAGE_U8_U8_BILINEAR = 0xd000f1,
     AMD_SCALE_IMAGE_U8_U8_BILINEAR_REPLICATE = 0xd000f2,

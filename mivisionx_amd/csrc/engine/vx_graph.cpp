@@ -190,6 +190,10 @@ static FusedVision *find_fusion(_vx_graph *g)
             // the tile pass computes the bilinear, border-UNDEFINED scaling only
             _vx_scalar *interp = is_ref(f.scale->params[2], VX_TYPE_SCALAR) ? (_vx_scalar *)f.scale->params[2] : nullptr;
             if (!interp || interp->value.e != VX_INTERPOLATION_BILINEAR || f.scale->border.mode != VX_BORDER_UNDEFINED) continue;
+            // and the gradient-3, L1-norm Canny only
+            _vx_scalar *gs = is_ref(f.canny->params[2], VX_TYPE_SCALAR) ? (_vx_scalar *)f.canny->params[2] : nullptr;
+            _vx_scalar *nt = is_ref(f.canny->params[3], VX_TYPE_SCALAR) ? (_vx_scalar *)f.canny->params[3] : nullptr;
+            if (!gs || !nt || gs->value.i32 != 3 || nt->value.e != VX_NORM_L1) continue;
         }
         if (((_vx_pyramid *)f.pyr->params[1])->levels > 8) continue;
         fused_params(&f);

@@ -428,8 +428,7 @@ vx_status v_canny(_vx_node *n)
     _vx_scalar *g = scl(n, 2), *norm = scl(n, 3);
     if (!g || !norm) return fail(n, VX_ERROR_INVALID_PARAMETERS, "gradient size / norm");
     if (g->value.i32 != 3 && g->value.i32 != 5 && g->value.i32 != 7) return fail(n, VX_ERROR_INVALID_VALUE, "gradient size");
-    if (g->value.i32 != 3 || norm->value.e != VX_NORM_L1)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "the 3x3 L1-norm Canny is provided (gradient 5/7 and L2 are not)");
+    if (norm->value.e != VX_NORM_L1 && norm->value.e != VX_NORM_L2) return fail(n, VX_ERROR_INVALID_VALUE, "norm type");
     _vx_image *in = img(n, 0);
     if (in->width < 3 || in->height < 3) return fail(n, VX_ERROR_INVALID_DIMENSION, "image below 3x3");
     if ((s = border_undefined(n))) return s;
@@ -452,9 +451,24 @@ vx_status x_canny(_vx_node *n, hipStream_t st)
     const uint32_t w = in->width, h = in->height, magStride = align16(2 * w);
     uint16_t *mag = (uint16_t *)n->local.ptr();
     void *stack = n->local.ptr() + canny_mag_bytes(w, h);
+    const int g = scl(n, 2)->value.i32;
+    const bool l2 = scl(n, 3)->value.e == VX_NORM_L2;
+    // gradient 7 runs CANNY_SUPP_THRESHOLD_U8XY_U16_7x7, which divides the threshold object's values by
+    // 4 in place on every run before using them (ago_kernel_api.cpp:18033-18035)
+    if (g == 7) {
+        t->lower.S32 /= 4;
+        t->upper.S32 /= 4;
+    }
     // thresholds as the reference reads them: the low 32 bits of the pixel values (ago_kernel_api.cpp:17969)
     const uint32_t lower = t->lower.U32, upper = t->upper.U32;
-    vx_status s = rc(n, mvx_CannySobel_U16_U8_3x3_L1NORM(st, w, h, mag, magStride, P(in), S(in)));
+    const uint8_t *ip = P(in);
+    const uint32_t is = S(in);
+    vx_status s = rc(n, g == 3 ? (l2 ? mvx_CannySobel_U16_U8_3x3_L2NORM(st, w, h, mag, magStride, ip, is)
+                                     : mvx_CannySobel_U16_U8_3x3_L1NORM(st, w, h, mag, magStride, ip, is))
+                     : g == 5 ? (l2 ? mvx_CannySobel_U16_U8_5x5_L2NORM(st, w, h, mag, magStride, ip, is)
+                                    : mvx_CannySobel_U16_U8_5x5_L1NORM(st, w, h, mag, magStride, ip, is))
+                              : (l2 ? mvx_CannySobel_U16_U8_7x7_L2NORM(st, w, h, mag, magStride, ip, is)
+                                    : mvx_CannySobel_U16_U8_7x7_L1NORM(st, w, h, mag, magStride, ip, is)));
     if (!s) s = rc(n, mvx_CannySuppThreshold_U8XY_U16_3x3(st, w, h, P(o), S(o), mag, magStride, stack, w * h, lower, upper));
     if (!s) s = rc(n, mvx_CannyEdgeTrace_U8_U8XY(st, w, h, P(o), S(o), stack, w * h));
     return s;
@@ -467,10 +481,16 @@ vx_status v_amd_canny_sobel(_vx_node *n)
     return s;
 }
 
-vx_status x_amd_canny_sobel(_vx_node *n, hipStream_t st)
+// AMD forms CannySobel_U16_U8_<g>x<g>_<norm> (out U16, in U8)
+template <int G, bool L2> vx_status x_amd_canny_sobel(_vx_node *n, hipStream_t st)
 {
     _vx_image *o = img(n, 0), *in = img(n, 1);
-    return rc(n, mvx_CannySobel_U16_U8_3x3_L1NORM(st, o->width, o->height, (uint16_t *)P(o), S(o), P(in), S(in)));
+    uint16_t *d = (uint16_t *)P(o);
+    const uint32_t w = o->width, h = o->height, ds = S(o), ss = S(in);
+    const uint8_t *sp = P(in);
+    if (G == 3) return rc(n, L2 ? mvx_CannySobel_U16_U8_3x3_L2NORM(st, w, h, d, ds, sp, ss) : mvx_CannySobel_U16_U8_3x3_L1NORM(st, w, h, d, ds, sp, ss));
+    if (G == 5) return rc(n, L2 ? mvx_CannySobel_U16_U8_5x5_L2NORM(st, w, h, d, ds, sp, ss) : mvx_CannySobel_U16_U8_5x5_L1NORM(st, w, h, d, ds, sp, ss));
+    return rc(n, L2 ? mvx_CannySobel_U16_U8_7x7_L2NORM(st, w, h, d, ds, sp, ss) : mvx_CannySobel_U16_U8_7x7_L1NORM(st, w, h, d, ds, sp, ss));
 }
 
 // ---------------------------------------------------------------------- Harris (A13) -------------
@@ -907,7 +927,22 @@ const std::vector<KernelDef> &kernel_table()
          {{OUT, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_sobel, nullptr, x_amd_sobel,
          AGO_TARGET_AFFINITY_GPU},
         {AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM, "com.amd.openvx.CannySobel_U16_U8_3x3_L1NORM",
-         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel,
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel<3, false>,
+         AGO_TARGET_AFFINITY_GPU},
+        {AMD_CANNY_SOBEL_U16_U8_3x3_L2NORM, "com.amd.openvx.CannySobel_U16_U8_3x3_L2NORM",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel<3, true>,
+         AGO_TARGET_AFFINITY_GPU},
+        {AMD_CANNY_SOBEL_U16_U8_5x5_L1NORM, "com.amd.openvx.CannySobel_U16_U8_5x5_L1NORM",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel<5, false>,
+         AGO_TARGET_AFFINITY_GPU},
+        {AMD_CANNY_SOBEL_U16_U8_5x5_L2NORM, "com.amd.openvx.CannySobel_U16_U8_5x5_L2NORM",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel<5, true>,
+         AGO_TARGET_AFFINITY_GPU},
+        {AMD_CANNY_SOBEL_U16_U8_7x7_L1NORM, "com.amd.openvx.CannySobel_U16_U8_7x7_L1NORM",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel<7, false>,
+         AGO_TARGET_AFFINITY_GPU},
+        {AMD_CANNY_SOBEL_U16_U8_7x7_L2NORM, "com.amd.openvx.CannySobel_U16_U8_7x7_L2NORM",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel<7, true>,
          AGO_TARGET_AFFINITY_GPU},
         {AMD_SCALE_IMAGE_U8_U8_NEAREST, "com.amd.openvx.ScaleImage_U8_U8_Nearest",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale_k<0>, nullptr, x_amd_scale<0>, AGO_TARGET_AFFINITY_GPU},

@@ -270,12 +270,20 @@ def read_scalar(s, vx_type):
     return v.value
 
 
-def range_threshold(ctx, lower, upper):
-    t = _ok(lib().vxCreateThresholdForImage(ctx, VX_THRESHOLD_TYPE_RANGE, DF["U8"], DF["U8"]), "vxCreateThresholdForImage")
+def range_threshold(ctx, lower, upper, fmt="U8"):
+    """a RANGE threshold for `fmt` input images ("U8" or "S16", whose values may exceed 255)"""
+    t = _ok(lib().vxCreateThresholdForImage(ctx, VX_THRESHOLD_TYPE_RANGE, DF[fmt], DF["U8"]), "vxCreateThresholdForImage")
     lo, hi = PixelValue(), PixelValue()
-    lo.U8, hi.U8 = lower, upper
+    setattr(lo, fmt, lower)
+    setattr(hi, fmt, upper)
     check(lib().vxCopyThresholdRange(t, c.byref(lo), c.byref(hi), VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST), "vxCopyThresholdRange")
     return t
+
+
+def threshold_range(t, fmt="U8"):
+    lo, hi = PixelValue(), PixelValue()
+    check(lib().vxCopyThresholdRange(t, c.byref(lo), c.byref(hi), VX_READ_ONLY, VX_MEMORY_TYPE_HOST), "vxCopyThresholdRange")
+    return getattr(lo, fmt), getattr(hi, fmt)
 
 
 def read_array(arr, dtype):

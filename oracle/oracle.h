@@ -68,6 +68,9 @@ void oracle_canny_sobel_3x3_l1(int w, int h, uint16_t *dst, int dst_stride, cons
 int oracle_canny_supp_threshold(int w, int h, uint8_t *dst, int dst_stride, const uint16_t *mag, int mag_stride,
                                 int lower, int upper);
 void oracle_canny_trace(int w, int h, uint8_t *img, int stride);
+void oracle_canny_sobel_ex(int w, int h, uint16_t *dst, int dst_stride, const uint8_t *src, int src_stride, int gsize,
+                           int l2);
+void oracle_canny_node_ex(int w, int h, const uint8_t *src, int gsize, int l2, int lower, int upper, uint8_t *dst);
 void oracle_harris_sobel_hg3(int w, int h, float *dst, int dst_stride, const uint8_t *src, int src_stride);
 void oracle_harris_score(int w, int h, float *dst, int dst_stride, const float *hg3, int hg3_stride,
                          float k, float thr, float norm);

@@ -204,6 +204,23 @@ class Oracle:
         self.lib.oracle_canny_node(w, h, _p(_c(src)), lower, upper, _p(d))
         return d
 
+    def canny_ex(self, src, gsize, l2, lower, upper):
+        """CannyEdgeDetector for gradient 3/5/7 and L1/L2 with the thresholds the suppression reads"""
+        h, w = src.shape
+        d = np.zeros((h, w), np.uint8)
+        self.lib.oracle_canny_node_ex(w, h, _p(_c(src)), gsize, int(l2), lower, upper, _p(d))
+        return d
+
+    def canny_sobel_ex(self, src, gsize, l2, padded=False):
+        """the U16 gradient plane (rows of ALIGN16(2w) bytes when padded, so the slot before row y --
+        the previous row's last -- is visible)"""
+        h, w = src.shape
+        s, ss = _plane(src)
+        ds = (2 * w + 15) & ~15
+        d = np.zeros((h, ds // 2), np.uint16)
+        self.lib.oracle_canny_sobel_ex(w, h, _p(d), ds, _p(s), ss, gsize, int(l2))
+        return d if padded else np.ascontiguousarray(d[:, :w])
+
     def canny_sobel(self, src):
         h, w = src.shape
         d = np.zeros((h, w), np.uint16)
@@ -441,6 +458,15 @@ class Reference:
         d = np.zeros((h, w), np.uint8)
         self._ok(self.lib.ref_canny(w, h, _p(_c(src)), lower, upper, _p(d)), "canny")
         return d
+
+    def canny_ex(self, src, gsize, l2, lower, upper, reps=1):
+        """(edges, threshold values after the runs)"""
+        h, w = src.shape
+        d = np.zeros((h, w), np.uint8)
+        lo, hi = ctypes.c_int(0), ctypes.c_int(0)
+        self._ok(self.lib.ref_canny_ex(w, h, _p(_c(src)), gsize, int(l2), lower, upper, reps, _p(d), ctypes.byref(lo),
+                                       ctypes.byref(hi)), "canny_ex")
+        return d, (lo.value, hi.value)
 
     def harris(self, src, k, thr, dist, capacity):
         h, w = src.shape

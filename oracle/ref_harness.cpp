@@ -311,6 +311,29 @@ int ref_canny(int w, int h, const uint8_t *src, int lower, int upper, uint8_t *d
     return 0;
 }
 
+// CannyEdgeDetector with any gradient size / norm; the range threshold is created for S16 input so
+// that values above 255 fit.  The graph runs `reps` times (the gradient-7 path divides the threshold
+// object's values by 4 on every run); *lo_after / *hi_after receive the threshold values afterwards.
+int ref_canny_ex(int w, int h, const uint8_t *src, int gsize, int l2, int lower, int upper, int reps, uint8_t *dst,
+                 int *lo_after, int *hi_after) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_image out = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    vx_threshold thr = vxCreateThresholdForImage(c.ctx, VX_THRESHOLD_TYPE_RANGE, VX_DF_IMAGE_S16, VX_DF_IMAGE_U8);
+    vx_pixel_value_t lo, hi;
+    memset(&lo, 0, sizeof(lo)); memset(&hi, 0, sizeof(hi));
+    lo.S16 = (vx_int16)lower; hi.S16 = (vx_int16)upper;
+    CHECK(vxCopyThresholdRange(thr, &lo, &hi, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    if (!vxCannyEdgeDetectorNode(c.graph, in, thr, gsize, l2 ? VX_NORM_L2 : VX_NORM_L1, out)) return -1;
+    CHECK(vxVerifyGraph(c.graph));
+    for (int r = 0; r < reps; r++) CHECK(vxProcessGraph(c.graph));
+    CHECK(get(out, w, h, dst, 1));
+    CHECK(vxCopyThresholdRange(thr, &lo, &hi, VX_READ_ONLY, VX_MEMORY_TYPE_HOST));
+    *lo_after = lo.S16;
+    *hi_after = hi.S16;
+    return 0;
+}
+
 static vx_node harris_node(Ctx &c, vx_image in, float k, float thr, float dist, vx_array corners, vx_scalar num) {
     vx_scalar sthr = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &thr);
     vx_scalar sdist = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &dist);

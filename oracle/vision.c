@@ -89,6 +89,167 @@ void oracle_canny_sobel_3x3_l1(int w, int h, uint16_t *dst, int dst_stride, cons
     }
 }
 
+/* 16-bit lane arithmetic of the SSE code: wrapping and saturating forms */
+static inline int w16(long v) { return (int16_t)(uint16_t)(unsigned long)v; }
+static inline int s16(long v) { return v > 32767 ? 32767 : v < -32768 ? -32768 : (int)v; }
+static inline int sll16(int v, int k) { return w16((long)((unsigned)(uint16_t)v << k)); }
+static inline int abs16(int v) { return w16(v < 0 ? -(long)v : v); }
+
+/* _mm_sqrt_ps of a float, _mm_cvtps_epi32 (nearest even), _mm_packus_epi32 */
+static inline int l2_pack_simd(float f)
+{
+    float r = sqrtf(f);
+    long v = (r > -2147483904.0f && r < 2147483648.0f) ? (long)rintf(r) : (long)INT32_MIN;
+    return v < 0 ? 0 : v > 65535 ? 65535 : (int)v;
+}
+
+/* (vx_int16)sqrt((Gx*Gx) + (Gy*Gy)) of the scalar loops: int products (32-bit wrap), double sqrt,
+ * truncation to 32 bits (0x80000000 for NaN), then to 16 */
+static inline int l2_scalar(int gx, int gy)
+{
+    const int32_t q = (int32_t)(uint32_t)((uint64_t)((int64_t)gx * gx) + (uint64_t)((int64_t)gy * gy));
+    const double r = sqrt((double)q);
+    const int32_t t = (r == r && r < 2147483648.0) ? (int32_t)r : INT32_MIN;
+    return (int16_t)(uint16_t)(uint32_t)t;
+}
+
+static inline uint16_t canny_out(int mag16, int sector) { return (uint16_t)(sll16(mag16, 2) | (sector & 3)); }
+
+/* CannySobel_U16_U8_{5x5,7x7}_L1NORM and _{3x3,5x5,7x7}_L2NORM: ago_haf_cpu_canny.cpp:145-427 and
+ * :646-1056 (the 3x3 L1 form is oracle_canny_sobel_3x3_l1).  Rows R..h-R-1 (R = gsize/2) are written.
+ * Each row runs, as the reference does (destination rows 16-byte aligned, so no prefix):
+ *   - over [0, aligned) with aligned = w & ~15: a vertical pass into two scratch rows r0, r1 of int16
+ *     (r1 = r0 + ALIGN16(w), 8 zero words before r0, zero words after r1), then a horizontal pass that
+ *     reads r0/r1 at x-R..x+R -- so column 0 sees zeros on the left and, when w is a multiple of 16,
+ *     the end of one scratch row runs into the start of the other.  Vertical / horizontal filters:
+ *       5x5: r0 = [1 4 6 4 1] rows, r1 = [-1 -2 0 2 1] rows; Gx = [-1 -2 0 2 1] of r0 (the last add
+ *            saturating), Gy = -[1 4 6 4 1] of r1 (L2: the *6 term added saturating);
+ *       7x7: r0 = ([1 6 15 20 15 6 1] rows) >> 2, r1 = ([-1 -4 -5 0 5 4 1] rows) >> 2; Gx, Gy with
+ *            saturating adds, Gy negated by a saturating subtract;
+ *       3x3 L2: r0 = [1 2 1] rows, r1 = [-1 0 1] rows; Gx = r0[x+1] - r0[x-1], Gy = -[1 2 1] of r1;
+ *     value: L1 = |Gx| + |Gy| (16-bit wrap; 7x7: signed min with 0x3FFF); L2 = sqrt of the float sum
+ *     of squares (3x3: of the low 16 bits of each square) rounded to nearest, saturated to 16 bits;
+ *     stored as (value << 2) | sector(Gx, Gy) in 16 bits;
+ *   - then over [aligned-1, w) (from -1 when aligned == 0, writing the slot before the row) the
+ *     scalar formulas on linear-address reads, in 16-bit wrapping arithmetic: Gx, Gy of the full
+ *     kernels (Gy negated for 5x5 L1 only; the 7x7 forms are not divided by 4; the 5x5 L2 Gy adds
+ *     the first row's pixels x-1..x+2 where it means to subtract them), L1 = |Gx| + |Gy|,
+ *     L2 = (int16)sqrt(Gx*Gx + Gy*Gy).
+ * Pinned by: bit-exact against oracle/_ref for each size and norm at widths that are / are not
+ * multiples of 16 and below 16; a constant image gives magnitude 0 in every written pixel. */
+void oracle_canny_sobel_ex(int w, int h, uint16_t *dst, int dst_stride, const uint8_t *src, int src_stride, int gsize,
+                           int l2)
+{
+    if (gsize == 3 && !l2) {
+        oracle_canny_sobel_3x3_l1(w, h, dst, dst_stride, src, src_stride);
+        return;
+    }
+    static const int smooth5[5] = {1, 4, 6, 4, 1}, deriv5[5] = {-1, -2, 0, 2, 1};
+    static const int smooth7[7] = {1, 6, 15, 20, 15, 6, 1}, deriv7[7] = {-1, -4, -5, 0, 5, 4, 1};
+    static const int smooth3[3] = {1, 2, 1}, deriv3[3] = {-1, 0, 1};
+    const int R = gsize / 2;
+    const int *sm = gsize == 3 ? smooth3 : gsize == 5 ? smooth5 : smooth7;
+    const int *dv = gsize == 3 ? deriv3 : gsize == 5 ? deriv5 : deriv7;
+    const long n = (long)h * src_stride;
+    const int aligned = w & ~15, a16 = (w + 15) & ~15;
+    int16_t *buf = (int16_t *)calloc((size_t)(8 + 2 * a16 + 64), sizeof(int16_t));
+    int16_t *r0 = buf + 8, *r1 = r0 + a16;
+    uint8_t *db = (uint8_t *)dst;
+    for (int y = R; y < h - R; y++) {
+        uint16_t *drow = (uint16_t *)(db + (long)y * dst_stride);
+        for (int x = 0; x < aligned; x++) {
+            int s[7];
+            for (int k = 0; k <= 2 * R; k++) s[k] = src[(long)(y - R + k) * src_stride + x];
+            int t0, t1;
+            if (gsize == 3) {
+                t0 = s[0] + s[2] + 2 * s[1];
+                t1 = s[2] - s[0];
+            } else if (gsize == 5) {
+                t0 = 4 * (s[1] + s[3]) + 6 * s[2] + (s[0] + s[4]);
+                t1 = 2 * (s[3] - s[1]) + (s[4] - s[0]);
+            } else {
+                t0 = (6 * (s[1] + s[5]) + 20 * s[3] + (s[0] + s[6]) + 15 * (s[2] + s[4])) >> 2;
+                t1 = (4 * (s[5] - s[1]) + (s[6] - s[0]) + 5 * (s[4] - s[2])) >> 2;
+            }
+            r0[x] = (int16_t)t0;
+            r1[x] = (int16_t)t1;
+        }
+        for (int x = 0; x < aligned; x++) {
+#define A0(d) ((int)r0[x + (d)])
+#define A1(d) ((int)r1[x + (d)])
+            int gx, gy;
+            if (gsize == 3) {
+                gx = w16(A0(1) - A0(-1));
+                gy = w16(-(long)w16(w16(A1(-1) + A1(1)) + sll16(A1(0), 1)));
+            } else if (gsize == 5) {
+                gx = s16((long)sll16(w16(A0(1) - A0(-1)), 1) + w16(A0(2) - A0(-2)));
+                int t = w16((long)sll16(w16(A1(-1) + A1(1)), 2) + w16(A1(-2) + A1(2)));
+                t = l2 ? s16((long)t + w16(A1(0) * 6)) : w16((long)t + w16(A1(0) * 6));
+                gy = w16(-(long)t);
+            } else {
+                int t = sll16(s16((long)A0(2) - A0(-2)), 2);
+                const int u = w16((long)s16((long)A0(1) - A0(-1)) * 5);
+                t = s16((long)t + s16((long)A0(3) - A0(-3)));
+                gx = s16((long)t + u);
+                int v = s16((long)w16((long)w16(A1(-2) + A1(2)) * 6) + w16((long)A1(0) * 20));
+                const int q = w16((long)w16(A1(-1) + A1(1)) * 15);
+                v = s16((long)v + s16((long)A1(-3) + A1(3)));
+                v = s16((long)v + q);
+                gy = s16(-(long)v);
+            }
+#undef A0
+#undef A1
+            const int sector = canny_sector(gx, gy);
+            int mag;
+            if (!l2) {
+                mag = w16((long)abs16(gx) + abs16(gy));
+                if (gsize == 7 && mag > 0x3fff) mag = 0x3fff;
+            } else if (gsize == 3) {
+                mag = l2_pack_simd((float)(uint16_t)w16((long)gx * gx) + (float)(uint16_t)w16((long)gy * gy));
+            } else {
+                mag = l2_pack_simd((float)((long)gx * gx + (long)gy * gy));
+            }
+            drow[x] = canny_out(mag, sector);
+        }
+        for (int x = aligned - 1; x < w; x++) {
+            long gxs = 0, gys = 0;
+            for (int j = 0; j <= 2 * R; j++) {
+                const long b = (long)(y - R + j) * src_stride + x;
+                for (int i = 0; i <= 2 * R; i++) {
+                    const int p = rd8(src, n, b + i - R);
+                    gxs += (long)sm[j] * dv[i] * p;
+                    /* 5x5 L2: the first row's term lacks its parentheses, so only its first pixel is
+                     * subtracted (ago_haf_cpu_canny.cpp:880) */
+                    const int wy = (gsize == 5 && l2 && j == 0 && i > 0) ? sm[i] : dv[j] * sm[i];
+                    gys += (long)wy * p;
+                }
+            }
+            int gx = (int16_t)(uint16_t)(unsigned long)gxs, gy = (int16_t)(uint16_t)(unsigned long)gys;
+            if (gsize == 5 && !l2) gy = (int16_t)(uint16_t)(~(unsigned)gy + 1u);
+            int mag = l2 ? l2_scalar(gx, gy) : (int16_t)(uint16_t)(unsigned)(abs(gx) + abs(gy));
+            *(uint16_t *)(db + (long)y * dst_stride + 2L * x) = canny_out(mag, canny_sector(gx, gy));
+        }
+    }
+    free(buf);
+}
+
+/* CannyEdgeDetector for any gradient size and norm: CANNY_SOBEL_U16_U8_<g>x<g>_<norm> ->
+ * CANNY_SUPP_THRESHOLD_U8XY_U16_3x3 (gradient 7: _7x7, which first divides the threshold object's
+ * values by 4 in place, ago_kernel_api.cpp:18033-18035) -> CANNY_EDGE_TRACE_U8_U8XY
+ * (ago_drama_divide.cpp:1618-1701).  lower/upper are the values the suppression reads. */
+void oracle_canny_node_ex(int w, int h, const uint8_t *src, int gsize, int l2, int lower, int upper, uint8_t *dst)
+{
+    const int s16w = ((w + 15) & ~15), s8w = ((2 * w + 15) & ~15);
+    uint8_t *in = (uint8_t *)calloc((size_t)s16w * h + 64, 1), *mag = (uint8_t *)calloc((size_t)s8w * h + 64, 1);
+    uint8_t *out = (uint8_t *)calloc((size_t)s16w * h + 64, 1);
+    for (int y = 0; y < h; y++) memcpy(in + (long)y * s16w, src + (long)y * w, (size_t)w);
+    oracle_canny_sobel_ex(w, h, (uint16_t *)mag, s8w, in, s16w, gsize, l2);
+    oracle_canny_supp_threshold(w, h, out, s16w, (const uint16_t *)mag, s8w, lower, upper);
+    oracle_canny_trace(w, h, out, s16w);
+    for (int y = 0; y < h; y++) memcpy(dst + (long)y * w, out + (long)y * s16w, (size_t)w);
+    free(in); free(mag); free(out);
+}
+
 /* CannySuppThreshold_U8XY_U16_3x3: ago_haf_cpu_canny.cpp:549-596, over x in [1, w-2], y in
  * [1, h-2]; neighbours by sector {(-1,0),(1,0)}, {(1,-1),(-1,1)}, {(0,-1),(0,1)}, {(-1,-1),(1,1)}
  * (:26-31); strict '>' against both; out 255 (and pushed) if > upper, 0 if <= lower, else 127.
