@@ -207,6 +207,24 @@ int mvx_NonMaxSupp_XY_ANY_3x3(mvx_stream_t stream, uint32_t capacity, void *pDst
                               uint32_t srcWidth, uint32_t srcHeight, const float *pSrc, uint32_t srcStride,
                               void *workspace, size_t workspaceSize);
 
+/* replace HipExec_HarrisSobel_HG3_U8_5x5 / _7x7 (hip_host_decls.h:872, :876); HafCpu_HarrisSobel_HG3_U8_5x5
+ * / _7x7 (ago_haf_cpu_harris.cpp:333-594): rows R..h-R-1 (R = 2, 3), written from byte R*stride as packed
+ * rows of w pixels, as the reference writes them. */
+int mvx_HarrisSobel_HG3_U8_5x5(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                               float *pDstGxy, uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_HarrisSobel_HG3_U8_7x7(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                               float *pDstGxy, uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
+/* replace HipExec_HarrisScore_HVC_HG3_5x5 / _7x7 (hip_host_decls.h:886, :892); HafCpu_HarrisScore_HVC_HG3_5x5
+ * / _7x7 (ago_haf_cpu_harris.cpp:661-802), including the 7x7 form's one-column left shift of its
+ * outputs; normFactor = (255 * 2^(g-1) * block)^4 (ago_kernel_api.cpp:17012-17013, :17083-17084).  Planes
+ * narrower or lower than the block return -15 (the reference's border stores would overrun them). */
+int mvx_HarrisScore_HVC_HG3_5x5(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                float *pDstVc, uint32_t dstStride, const float *pSrcGxy, uint32_t srcStride,
+                                float sensitivity, float strengthThreshold, float normFactor);
+int mvx_HarrisScore_HVC_HG3_7x7(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                float *pDstVc, uint32_t dstStride, const float *pSrcGxy, uint32_t srcStride,
+                                float sensitivity, float strengthThreshold, float normFactor);
+
 /* Sort and pick of HafCpu_HarrisMergeSortAndPick_XY_XYS (ago_haf_cpu_harris.cpp:923-1005; CPU-only in
  * the reference, ago_kernel_list.cpp:483), grid of ago_kernel_api.cpp:20808-20822.  Reads
  * *pSrcCount (device) items of pSrcList (at most maxList); writes vx_keypoint_t items (28 bytes) to

@@ -194,6 +194,10 @@ static FusedVision *find_fusion(_vx_graph *g)
             _vx_scalar *gs = is_ref(f.canny->params[2], VX_TYPE_SCALAR) ? (_vx_scalar *)f.canny->params[2] : nullptr;
             _vx_scalar *nt = is_ref(f.canny->params[3], VX_TYPE_SCALAR) ? (_vx_scalar *)f.canny->params[3] : nullptr;
             if (!gs || !nt || gs->value.i32 != 3 || nt->value.e != VX_NORM_L1) continue;
+            // and the gradient-3, block-3 Harris only
+            _vx_scalar *hg = is_ref(f.harris->params[4], VX_TYPE_SCALAR) ? (_vx_scalar *)f.harris->params[4] : nullptr;
+            _vx_scalar *hb = is_ref(f.harris->params[5], VX_TYPE_SCALAR) ? (_vx_scalar *)f.harris->params[5] : nullptr;
+            if (!hg || !hb || hg->value.i32 != 3 || hb->value.i32 != 3) continue;
         }
         if (((_vx_pyramid *)f.pyr->params[1])->levels > 8) continue;
         fused_params(&f);

@@ -534,8 +534,8 @@ vx_status v_harris(_vx_node *n)
     if (!thr || !dist || !k || !g || !b) return fail(n, VX_ERROR_INVALID_PARAMETERS, "scalars");
     if (thr->dataType != VX_TYPE_FLOAT32 || dist->dataType != VX_TYPE_FLOAT32 || k->dataType != VX_TYPE_FLOAT32)
         return fail(n, VX_ERROR_INVALID_TYPE, "strength / distance / sensitivity must be FLOAT32");
-    if (g->value.i32 != 3 || b->value.i32 != 3)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "gradient size 3 and block size 3 are provided");
+    for (int v : {g->value.i32, b->value.i32})
+        if (v != 3 && v != 5 && v != 7) return fail(n, VX_ERROR_INVALID_VALUE, "gradient and block sizes must be 3, 5 or 7");
     if (!is_ref(n->params[6], VX_TYPE_ARRAY) || ((_vx_array *)n->params[6])->itemType != VX_TYPE_KEYPOINT)
         return fail(n, VX_ERROR_INVALID_PARAMETERS, "corners must be a KEYPOINT array");
     _vx_array *corners = (_vx_array *)n->params[6];
@@ -543,7 +543,8 @@ vx_status v_harris(_vx_node *n)
     if (n->params[7] && (!scl(n, 7) || scl(n, 7)->dataType != VX_TYPE_SIZE))
         return fail(n, VX_ERROR_INVALID_TYPE, "num_corners must be a SIZE scalar");
     _vx_image *in = img(n, 0);
-    if (in->width < 3 || in->height < 3 || in->width >= 32768 || in->height >= 32768)
+    const uint32_t minSide = (uint32_t)std::max(3, std::max(g->value.i32, b->value.i32));
+    if (in->width < minSide || in->height < minSide || in->width >= 32768 || in->height >= 32768)
         return fail(n, VX_ERROR_INVALID_DIMENSION, "image size");
     return VX_SUCCESS;
 }
@@ -566,8 +567,16 @@ vx_status x_harris(_vx_node *n, hipStream_t st)
     const float thr = scl(n, 1)->value.f32, dist = scl(n, 2)->value.f32, k = scl(n, 3)->value.f32;
     float *hg3 = (float *)(ws + L.hg3), *hvc = (float *)(ws + L.hvc);
     uint32_t *cnt = (uint32_t *)(ws + L.cnt);
-    vx_status s = rc(n, mvx_HarrisSobel_HG3_U8_3x3(st, w, h, hg3, align16(12 * w), P(in), S(in)));
-    if (!s) s = rc(n, mvx_HarrisScore_HVC_HG3_3x3(st, w, h, hvc, align16(4 * w), hg3, align16(12 * w), k, thr, harris_norm(3, 3)));
+    // HARRIS_SOBEL_HG3_U8_<g>x<g> -> HARRIS_SCORE_HVC_HG3_<b>x<b> (ago_drama_divide.cpp:1703-1787)
+    const int g = scl(n, 4)->value.i32, b = scl(n, 5)->value.i32;
+    const uint32_t hs = align16(12 * w), vs = align16(4 * w);
+    vx_status s = rc(n, g == 3 ? mvx_HarrisSobel_HG3_U8_3x3(st, w, h, hg3, hs, P(in), S(in))
+                     : g == 5 ? mvx_HarrisSobel_HG3_U8_5x5(st, w, h, hg3, hs, P(in), S(in))
+                              : mvx_HarrisSobel_HG3_U8_7x7(st, w, h, hg3, hs, P(in), S(in)));
+    const float norm = harris_norm(g, b);
+    if (!s) s = rc(n, b == 3 ? mvx_HarrisScore_HVC_HG3_3x3(st, w, h, hvc, vs, hg3, hs, k, thr, norm)
+                      : b == 5 ? mvx_HarrisScore_HVC_HG3_5x5(st, w, h, hvc, vs, hg3, hs, k, thr, norm)
+                               : mvx_HarrisScore_HVC_HG3_7x7(st, w, h, hvc, vs, hg3, hs, k, thr, norm));
     if (!s) s = rc(n, mvx_NonMaxSupp_XY_ANY_3x3(st, L.maxList, ws + L.xys, cnt, w, h, hvc, align16(4 * w), ws + L.nms, L.nmsSize));
     // the count scalar lives in the node workspace's tail when the caller gave none
     uint64_t *count = num ? (uint64_t *)num->dev.ptr() : (uint64_t *)(ws + L.cnt + 8);

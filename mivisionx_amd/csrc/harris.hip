@@ -1,4 +1,4 @@
-// harris.hip -- HarrisCorners (gradient 3, block 3) for gfx950: HG3 Sobel products -> score ->
+// harris.hip -- HarrisCorners (gradient and block sizes 3, 5, 7) for gfx950: HG3 Sobel products -> score ->
 // 3x3 non-max suppression with raster-ordered compaction -> sort-and-pick with a minimum distance.
 //
 // The reference sorts and picks on the CPU (HafCpu_HarrisMergeSortAndPick_XY_XYS has no HIP
@@ -73,6 +73,76 @@ k_harris_score(uint32_t w, uint32_t h, float *__restrict__ dst, uint32_t dstStri
     float mc = det - (k * tr * tr);
     mc = __fdiv_rn(mc, norm);
     d[x] = (mc > thr) ? mc : 0.0f;
+}
+
+// HarrisSobel_HG3_U8_5x5 / _7x7 (ago_haf_cpu_harris.cpp:333-594): rows R..h-R-1, all columns, linear
+// reads; gx = column smoothing of the row derivatives, gy = column derivative of the row smoothings.
+// The writer starts at byte R*stride and stores packed rows of w pixels (no row padding).
+template <int G> struct HTaps;
+template <> struct HTaps<5> { static constexpr int d[5] = {-1, -2, 0, 2, 1}, s[5] = {1, 4, 6, 4, 1}; };
+template <> struct HTaps<7> { static constexpr int d[7] = {-1, -4, -5, 0, 5, 4, 1}, s[7] = {1, 6, 15, 20, 15, 6, 1}; };
+
+template <int G>
+__global__ void __launch_bounds__(256)
+k_harris_sobel_g(uint32_t w, uint32_t h, float *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
+                 uint32_t srcStride)
+{
+    constexpr int R = G / 2;
+    const int x = (int)(blockIdx.x * 256 + threadIdx.x);
+    const int y = R + (int)blockIdx.y;
+    if (x >= (int)w || y >= (int)h - R) return;
+    const long n = (long)h * srcStride;
+    int gx = 0, gy = 0;
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+        const long b = (long)(y - R + j) * srcStride + x;
+        int hd = 0, hs = 0;
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int p = (int)ld_u8(src, n, b + i - R);
+            hd += HTaps<G>::d[i] * p;
+            hs += HTaps<G>::s[i] * p;
+        }
+        gx += HTaps<G>::s[j] * hd;
+        gy += HTaps<G>::d[j] * hs;
+    }
+    float *d = dst + (long)R * (dstStride / 4) + 3L * ((long)(y - R) * w + x);
+    d[0] = (float)gx * (float)gx;
+    d[1] = (float)gx * (float)gy;
+    d[2] = (float)gy * (float)gy;
+}
+
+// HarrisScore_HVC_HG3_5x5 / _7x7 (ago_haf_cpu_harris.cpp:661-802): rows R..h-R-1, window sums j-major.
+// 5x5 writes columns 2..w-3; 7x7 stores the value of column x at x-1 (x in 3..w-4) because its third
+// leading zero does not advance the output pointer.  Every other pixel of the plane is 0.
+template <int B>
+__global__ void __launch_bounds__(256)
+k_harris_score_b(uint32_t w, uint32_t h, float *__restrict__ dst, uint32_t dstStride, const float *__restrict__ hg3,
+                 uint32_t hg3Stride, float k, float thr, float norm)
+{
+    constexpr int R = B / 2;
+    const int c = (int)(blockIdx.x * 256 + threadIdx.x);
+    const int y = (int)blockIdx.y;
+    if (c >= (int)w || y >= (int)h) return;
+    float *d = (float *)((uint8_t *)dst + (long)y * dstStride);
+    const int x = B == 7 ? c + 1 : c;   // the column whose window this output holds
+    if (y < R || y >= (int)h - R || x < R || x >= (int)w - R) { d[c] = 0.0f; return; }
+    float gx2 = 0.0f, gy2 = 0.0f, gxy2 = 0.0f;
+#pragma unroll
+    for (int j = -R; j <= R; j++) {
+#pragma unroll
+        for (int i = -R; i <= R; i++) {
+            const float *s = hg3 + hg3_read_off(y + j, x + i, hg3Stride);
+            gx2 += s[0];
+            gxy2 += s[1];
+            gy2 += s[2];
+        }
+    }
+    const float tr = gx2 + gy2;
+    const float det = (gx2 * gy2) - (gxy2 * gxy2);
+    float mc = det - (k * tr * tr);
+    mc = __fdiv_rn(mc, norm);
+    d[c] = (mc > thr) ? mc : 0.0f;
 }
 
 // ---------------------------------------------------------------- non-max suppression -------------
@@ -182,6 +252,61 @@ extern "C" int mvx_HarrisSobel_HG3_U8_3x3(mvx_stream_t stream, uint32_t w, uint3
     hipLaunchKernelGGL(k_harris_sobel, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
+}
+
+namespace {
+template <int G>
+int harris_sobel_g(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride, const uint8_t *src,
+                   uint32_t srcStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok_h(dst, w, dstStride, 12) || !plane_ok_h(src, w, srcStride, 1)) return MVX_ERROR_INVALID_PARAMETERS;
+    if ((int)h < G) return MVX_SUCCESS;
+    hipLaunchKernelGGL(k_harris_sobel_g<G>, dim3((w + 255) / 256, h - 2 * (G / 2)), dim3(256), 0, stream_of(stream), w, h, dst,
+                       dstStride, src, srcStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+template <int B>
+int harris_score_b(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride, const float *hg3,
+                   uint32_t hg3Stride, float k, float thr, float norm)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok_h(dst, w, dstStride, 4) || !plane_ok_h(hg3, w, hg3Stride, 12)) return MVX_ERROR_INVALID_PARAMETERS;
+    // the reference writes its zero border columns and rows unconditionally: narrower planes overrun
+    if (w < (uint32_t)B || h < (uint32_t)B) return -15;
+    hipLaunchKernelGGL(k_harris_score_b<B>, dim3((w + 255) / 256, h), dim3(256), 0, stream_of(stream), w, h, dst, dstStride, hg3,
+                       hg3Stride, k, thr, norm);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+}  // namespace
+
+extern "C" int mvx_HarrisSobel_HG3_U8_5x5(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride,
+                                          const uint8_t *src, uint32_t srcStride)
+{
+    return harris_sobel_g<5>(stream, w, h, dst, dstStride, src, srcStride);
+}
+
+extern "C" int mvx_HarrisSobel_HG3_U8_7x7(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride,
+                                          const uint8_t *src, uint32_t srcStride)
+{
+    return harris_sobel_g<7>(stream, w, h, dst, dstStride, src, srcStride);
+}
+
+extern "C" int mvx_HarrisScore_HVC_HG3_5x5(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride,
+                                           const float *hg3, uint32_t hg3Stride, float sensitivity, float strengthThreshold,
+                                           float normFactor)
+{
+    return harris_score_b<5>(stream, w, h, dst, dstStride, hg3, hg3Stride, sensitivity, strengthThreshold, normFactor);
+}
+
+extern "C" int mvx_HarrisScore_HVC_HG3_7x7(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride,
+                                           const float *hg3, uint32_t hg3Stride, float sensitivity, float strengthThreshold,
+                                           float normFactor)
+{
+    return harris_score_b<7>(stream, w, h, dst, dstStride, hg3, hg3Stride, sensitivity, strengthThreshold, normFactor);
 }
 
 extern "C" int mvx_HarrisScore_HVC_HG3_3x3(mvx_stream_t stream, uint32_t w, uint32_t h, float *dst, uint32_t dstStride,

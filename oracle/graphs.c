@@ -160,6 +160,28 @@ void oracle_canny_sobel_node(int w, int h, const uint8_t *src, uint16_t *dst)
 /* HarrisCorners (gradient 3, block 3): HARRIS_SOBEL_HG3 -> HARRIS_SCORE_HVC -> NON_MAX_SUPP_XY_ANY_3x3
  * (XYS capacity w*h) -> HARRIS_MERGE_SORT_AND_PICK (ago_drama_divide.cpp:1703-1787).  Returns the
  * uncapped corner count; `hvc` (w*h floats, may be NULL) receives the score plane. */
+/* HarrisCorners with any gradient size g and block size b in {3, 5, 7}: HARRIS_SOBEL_HG3_U8_<g>x<g> ->
+ * HARRIS_SCORE_HVC_HG3_<b>x<b> -> NON_MAX_SUPP_XY_ANY_3x3 -> HARRIS_MERGE_SORT_AND_PICK_XY_XYS
+ * (ago_drama_divide.cpp:1703-1787). */
+int oracle_harris_node_ex(int w, int h, const uint8_t *src, int g, int b, float k, float thr, float min_distance,
+                          oracle_keypoint_t *out, int capacity, float *hvc)
+{
+    plane_t in = plane_new(w, h, 1), hg3 = plane_new(w, h, 12), vc = plane_new(w, h, 4);
+    plane_put(&in, w, h, 1, src);
+    if (g == 3) oracle_harris_sobel_hg3(w, h, (float *)hg3.p, hg3.stride, in.p, in.stride);
+    else oracle_harris_sobel_g(w, h, (float *)hg3.p, hg3.stride, in.p, in.stride, g);
+    const float norm = oracle_harris_norm_gb(g, b);
+    if (b == 3) oracle_harris_score(w, h, (float *)vc.p, vc.stride, (const float *)hg3.p, hg3.stride, k, thr, norm);
+    else oracle_harris_score_b(w, h, (float *)vc.p, vc.stride, (const float *)hg3.p, hg3.stride, k, thr, norm, b);
+    if (hvc) plane_get(&vc, w, h, 4, hvc);
+    int cap = w * h;
+    oracle_xys_t *list = (oracle_xys_t *)malloc(sizeof(oracle_xys_t) * (size_t)(cap > 0 ? cap : 1));
+    int n = oracle_nonmax_xys(w, h, (const float *)vc.p, vc.stride, list, cap);
+    int count = oracle_harris_pick(list, n, min_distance, w, h, out, capacity);
+    free(list); free(in.p); free(hg3.p); free(vc.p);
+    return count;
+}
+
 int oracle_harris_node(int w, int h, const uint8_t *src, float k, float thr, float min_distance,
                        oracle_keypoint_t *out, int capacity, float *hvc)
 {

@@ -75,6 +75,10 @@ void oracle_harris_sobel_hg3(int w, int h, float *dst, int dst_stride, const uin
 void oracle_harris_score(int w, int h, float *dst, int dst_stride, const float *hg3, int hg3_stride,
                          float k, float thr, float norm);
 float oracle_harris_norm(int gradient_size);
+void oracle_harris_sobel_g(int w, int h, float *dst, int dst_stride, const uint8_t *src, int src_stride, int g);
+void oracle_harris_score_b(int w, int h, float *dst, int dst_stride, const float *hg3, int hg3_stride, float k,
+                           float thr, float norm, int b);
+float oracle_harris_norm_gb(int gradient_size, int block_size);
 int oracle_nonmax_xys(int w, int h, const float *img, int stride, oracle_xys_t *list, int capacity);
 int oracle_harris_pick(oracle_xys_t *list, int n, float min_distance, int width, int height,
                        oracle_keypoint_t *out, int capacity);
@@ -117,6 +121,8 @@ void oracle_pyramid_dims(int w, int h, int levels, int *ws, int *hs);
 int oracle_pyramid_node(int w, int h, const uint8_t *src, int levels, uint8_t *out);
 void oracle_canny_node(int w, int h, const uint8_t *src, int lower, int upper, uint8_t *dst);
 void oracle_canny_sobel_node(int w, int h, const uint8_t *src, uint16_t *dst);
+int oracle_harris_node_ex(int w, int h, const uint8_t *src, int g, int b, float k, float thr, float min_distance,
+                          oracle_keypoint_t *out, int capacity, float *hvc);
 int oracle_harris_node(int w, int h, const uint8_t *src, float k, float thr, float min_distance,
                        oracle_keypoint_t *out, int capacity, float *hvc);
 

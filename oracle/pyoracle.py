@@ -77,6 +77,7 @@ class Oracle:
         L.oracle_magnitude_px.restype = ctypes.c_int16
         L.oracle_phase_px.restype = ctypes.c_uint8
         L.oracle_harris_norm.restype = ctypes.c_float
+        L.oracle_harris_norm_gb.restype = ctypes.c_float
 
     # ---- 3x3 filters, Sobel, Magnitude, Phase ----
     def filter3x3(self, which, src):
@@ -251,6 +252,36 @@ class Oracle:
 
     def harris_norm(self, gradient_size=3):
         return self.lib.oracle_harris_norm(gradient_size)
+
+    def harris_planes(self, src, g, b, k, thr):
+        """(HG3 plane as the raw floats of its ALIGN16(12w)-byte rows, HVC plane (h, w)) of one gradient
+        / block size pair"""
+        h, w = src.shape
+        s, ss = _plane(src)
+        hs = (12 * w + 15) & ~15
+        hg3 = np.zeros((h, hs // 4), np.float32)
+        if g == 3:
+            self.lib.oracle_harris_sobel_hg3(w, h, _p(hg3), hs, _p(s), ss)
+        else:
+            self.lib.oracle_harris_sobel_g(w, h, _p(hg3), hs, _p(s), ss, g)
+        vs = (4 * w + 15) & ~15
+        vc = np.zeros((h, vs // 4), np.float32)
+        norm = ctypes.c_float(self.lib.oracle_harris_norm_gb(g, b))
+        if b == 3:
+            self.lib.oracle_harris_score(w, h, _p(vc), vs, _p(hg3), hs, ctypes.c_float(k), ctypes.c_float(thr), norm)
+        else:
+            self.lib.oracle_harris_score_b(w, h, _p(vc), vs, _p(hg3), hs, ctypes.c_float(k), ctypes.c_float(thr), norm, b)
+        return hg3, np.ascontiguousarray(vc[:, :w])
+
+    def harris_ex(self, src, g, b, k, thr, dist, capacity, with_score=False):
+        """HarrisCorners with gradient size g and block size b (3, 5, 7)"""
+        h, w = src.shape
+        out = np.zeros(max(capacity, 1), KEYPOINT)
+        hvc = np.zeros((h, w), np.float32)
+        n = self.lib.oracle_harris_node_ex(w, h, _p(_c(src)), g, b, ctypes.c_float(k), ctypes.c_float(thr),
+                                           ctypes.c_float(dist), _p(out), capacity, _p(hvc))
+        res = (out[:min(n, capacity)], n)
+        return res + (hvc,) if with_score else res
 
     # ---- warps and remap ----
     def warp_affine(self, src, dw, dh, m, constant=False, border=0, padded=False):
@@ -475,6 +506,15 @@ class Reference:
         cnt = ctypes.c_long(0)
         self._ok(self.lib.ref_harris(w, h, _p(_c(src)), ctypes.c_float(k), ctypes.c_float(thr), ctypes.c_float(dist),
                                      capacity, _p(out), ctypes.byref(n), ctypes.byref(cnt)), "harris")
+        return out[:n.value], cnt.value
+
+    def harris_ex(self, src, g, b, k, thr, dist, capacity):
+        h, w = src.shape
+        out = np.zeros(max(capacity, 1), KEYPOINT)
+        n = ctypes.c_int(0)
+        cnt = ctypes.c_long(0)
+        self._ok(self.lib.ref_harris_ex(w, h, _p(_c(src)), ctypes.c_float(k), ctypes.c_float(thr), ctypes.c_float(dist), g, b,
+                                        capacity, _p(out), ctypes.byref(n), ctypes.byref(cnt)), "harris_ex")
         return out[:n.value], cnt.value
 
     def graph2(self, rgb, dw, dh, levels, lower, upper, k, thr, dist, capacity, time_iters=0):

@@ -365,6 +365,22 @@ int ref_harris(int w, int h, const uint8_t *src, float k, float thr, float dist,
     return get_corners(corners, num, out, numitems, count);
 }
 
+// HarrisCorners with gradient size g and block size b.
+int ref_harris_ex(int w, int h, const uint8_t *src, float k, float thr, float dist, int g, int b, int capacity,
+                  vx_keypoint_t *out, int *numitems, long *count) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_array corners = vxCreateArray(c.ctx, VX_TYPE_KEYPOINT, capacity);
+    vx_size zero = 0;
+    vx_scalar num = vxCreateScalar(c.ctx, VX_TYPE_SIZE, &zero);
+    vx_scalar sthr = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &thr);
+    vx_scalar sdist = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &dist);
+    vx_scalar sk = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &k);
+    if (!vxHarrisCornersNode(c.graph, in, sthr, sdist, sk, g, b, corners, num)) return -1;
+    int s = run(c); if (s) return s;
+    return get_corners(corners, num, out, numitems, count);
+}
+
 // The 4K vision graph: RGB -> IYUV (virtual) -> Y (virtual); Y -> ScaleImage (dw x dh, bilinear);
 // Y -> GaussianPyramid (levels, HALF); Y -> Canny (3, L1, lower/upper); Y -> Harris (k, thr, dist, 3, 3).
 // If time_iters > 0 the graph is processed that many more times; *sec_per_frame gets the mean.

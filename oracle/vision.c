@@ -374,6 +374,83 @@ void oracle_harris_score(int w, int h, float *dst, int dst_stride, const float *
     }
 }
 
+/* HarrisSobel_HG3_U8_5x5 / _7x7: ago_haf_cpu_harris.cpp:333-453 / 455-594.  Per source row r the
+ * horizontal passes (linear reads) hd(r) = derivative taps, hs(r) = smoothing taps (5: [-1 -2 0 2 1],
+ * [1 4 6 4 1] in int16; 7: [-1 -4 -5 0 5 4 1], [1 6 15 20 15 6 1] in int32); for output rows
+ * y = R..h-R-1: gx = smoothing of hd down the rows, gy = derivative of hs down the rows;
+ * out = {(float)gx*gx, (float)gx*gy, (float)gy*gy}.  The writer starts at byte R*stride and then
+ * advances one pixel per pixel with no row padding (packed rows of w pixels), unlike the 3x3 writer.
+ * Pinned by: bit-exact against oracle/_ref through the whole Harris node (gradient 5 and 7 with every
+ * block size, at widths with 12w % 16 == 0 and != 0). */
+void oracle_harris_sobel_g(int w, int h, float *dst, int dst_stride, const uint8_t *src, int src_stride, int g)
+{
+    static const int d5[5] = {-1, -2, 0, 2, 1}, s5[5] = {1, 4, 6, 4, 1};
+    static const int d7[7] = {-1, -4, -5, 0, 5, 4, 1}, s7[7] = {1, 6, 15, 20, 15, 6, 1};
+    const int R = g / 2;
+    const int *dv = g == 5 ? d5 : d7, *sm = g == 5 ? s5 : s7;
+    const long n = (long)h * src_stride;
+    for (int y = R; y < h - R; y++) {
+        for (int x = 0; x < w; x++) {
+            long gx = 0, gy = 0;
+            for (int j = 0; j < g; j++) {
+                const long b = (long)(y - R + j) * src_stride + x;
+                long hd = 0, hs = 0;
+                for (int i = 0; i < g; i++) {
+                    const int p = rd8(src, n, b + i - R);
+                    hd += (long)dv[i] * p;
+                    hs += (long)sm[i] * p;
+                }
+                gx += (long)sm[j] * hd;
+                gy += (long)dv[j] * hs;
+            }
+            float *d = dst + (long)R * (dst_stride / 4) + 3L * ((long)(y - R) * w + x);
+            d[0] = (float)gx * (float)gx;
+            d[1] = (float)gx * (float)gy;
+            d[2] = (float)gy * (float)gy;
+        }
+    }
+}
+
+/* HarrisScore_HVC_HG3_5x5 / _7x7: ago_haf_cpu_harris.cpp:661-729 / 731-802, reading HG3 at pixel
+ * y*P + x (P = floor(stride/12)); window sums j-major over -R..R from 0, then as for 3x3.  5x5: output
+ * rows 2..h-3, columns 2..w-3; the rest 0.  7x7: rows 3..h-4, and the value computed at x (3..w-4) is
+ * stored at column x-1 (its third leading zero does not advance the pointer), columns 0..1 and
+ * w-4..w-2 are 0 and column w-1 is never written (0 in a zeroed plane).
+ * Pinned by: bit-exact against oracle/_ref through the whole Harris node. */
+void oracle_harris_score_b(int w, int h, float *dst, int dst_stride, const float *hg3, int hg3_stride, float k,
+                           float thr, float norm, int b)
+{
+    const int R = b / 2;
+    for (int y = 0; y < h; y++) {
+        float *d = (float *)((uint8_t *)dst + (long)y * dst_stride);
+        for (int x = 0; x < w; x++) d[x] = 0.0f;
+        if (y < R || y >= h - R) continue;
+        for (int x = R; x < w - R; x++) {
+            float gx2 = 0, gy2 = 0, gxy2 = 0;
+            for (int j = -R; j <= R; j++) {
+                for (int i = -R; i <= R; i++) {
+                    const float *s = hg3 + hg3_read_off(y + j, x + i, hg3_stride);
+                    gx2 += s[0];
+                    gxy2 += s[1];
+                    gy2 += s[2];
+                }
+            }
+            float tr = gx2 + gy2;
+            float det = (gx2 * gy2) - (gxy2 * gxy2);
+            float mc = det - (k * tr * tr);
+            mc /= norm;
+            d[b == 7 ? x - 1 : x] = (mc > thr) ? mc : 0.0f;
+        }
+    }
+}
+
+/* normFactor of ago_kernel_api.cpp:16941-16942, :17012-17013, :17083-17084: (255 * 2^(g-1) * b)^4 */
+float oracle_harris_norm_gb(int gradient_size, int block_size)
+{
+    float f = 255.0f * (float)(1 << (gradient_size - 1)) * (float)block_size;
+    return f * f * f * f;
+}
+
 /* normFactor of ago_kernel_api.cpp:16941-16942: (255 * 2^(g-1) * 3)^4 in float, left to right */
 float oracle_harris_norm(int gradient_size)
 {
