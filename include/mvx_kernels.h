@@ -185,6 +185,23 @@ int mvx_CannySuppThreshold_U8XY_U16_3x3(mvx_stream_t stream, uint32_t dstWidth, 
 int mvx_CannyEdgeTrace_U8_U8XY(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pImg,
                                uint32_t stride, void *xyStack, uint32_t capacityOfXY);
 
+/* ------------------------------------------------------------------ FAST corners ----------- */
+/* replace HipExec_FastCorners_XY_U8_NoSupression / _Supression (hip_host_decls.h:858, :863), with the
+ * results of HafCpu_FastCorners_XY_U8_NoSupression / _Supression (ago_haf_cpu_fast_corners.cpp:414-790):
+ * FAST-9 on the radius-3 circle with t = (short)floorf(threshold) and the reference's early exit; without
+ * suppression candidates at every x of rows 3..h-4 with strength = threshold, with suppression x in
+ * [3, w-4] and the bisected strength, then the 3x3 test (>= upper and left, > right and lower).
+ * Keypoints (vx_keypoint_t, 28 bytes) in raster order up to `capacity`; *dstCount (device) receives
+ * min(count, capacity) and *numCorners (device, optional) the uncapped count.  Images narrower or
+ * lower than 7 return -15.  The workspace holds mvx_FastCorners_workspace_size(w, h) bytes. */
+size_t mvx_FastCorners_workspace_size(uint32_t width, uint32_t height);
+int mvx_FastCorners_XY_U8_NoSupression(mvx_stream_t stream, uint32_t capacity, void *dstCorners, uint32_t *dstCount,
+                                       uint64_t *numCorners, uint32_t width, uint32_t height, const uint8_t *pSrc,
+                                       uint32_t srcStride, float threshold, void *workspace, size_t workspaceSize);
+int mvx_FastCorners_XY_U8_Supression(mvx_stream_t stream, uint32_t capacity, void *dstCorners, uint32_t *dstCount,
+                                     uint64_t *numCorners, uint32_t width, uint32_t height, const uint8_t *pSrc,
+                                     uint32_t srcStride, float threshold, void *workspace, size_t workspaceSize);
+
 /* ------------------------------------------------------------------ Harris (3x3, 3x3) ------ */
 /* replaces HipExec_HarrisSobel_HG3_U8_3x3 (hip_host_decls.h:868); HafCpu_HarrisSobel_HG3_U8_3x3
  * (ago_haf_cpu_harris.cpp:85-169).  HG3 is an F32x3 plane (stride >= ALIGN16(12w)); rows are laid

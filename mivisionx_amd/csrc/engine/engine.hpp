@@ -92,6 +92,8 @@ enum : vx_enum {
     AMD_BOX_U8_U8_3x3 = 0xd000b0,
     AMD_MEDIAN_U8_U8_3x3 = 0xd000b3,
     AMD_GAUSSIAN_U8_U8_3x3 = 0xd000b4,
+    AMD_FAST_CORNERS_XY_U8_SUPRESSION = 0xd000cc,
+    AMD_FAST_CORNERS_XY_U8_NOSUPRESSION = 0xd000cd,
     AMD_SCALE_GAUSSIAN_HALF_U8_U8_3x3 = 0xd000b5,
     AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
     AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,

@@ -593,6 +593,72 @@ vx_status i_harris_prepare(_vx_node *n, hipStream_t st)
     return rc(n, mvx_HarrisMergeSortAndPick_workspace_init(st, L.maxList, n->local.ptr() + L.pick, L.pickSize));
 }
 
+// ---------------------------------------------------------------------- FAST corners -------------
+// fast_corners -> FAST_CORNERS_XY_U8_SUPRESSION, or _NOSUPRESSION when nonmax_suppression is false
+// (ago_drama_divide.cpp:1789-1808); array count = min(corners, capacity), num_corners = all corners
+// (ago_kernel_api.cpp:16597-16604)
+static vx_status fast_check(_vx_node *n, int img_i, int thr_i, int arr_i, int num_i)
+{
+    vx_status s = need_input(n, img_i, VX_DF_IMAGE_U8);
+    if (s) return s;
+    _vx_scalar *thr = scl(n, thr_i);
+    if (!thr || thr->dataType != VX_TYPE_FLOAT32) return fail(n, VX_ERROR_INVALID_TYPE, "strength_thresh must be FLOAT32");
+    if (!is_ref(n->params[arr_i], VX_TYPE_ARRAY) || ((_vx_array *)n->params[arr_i])->itemType != VX_TYPE_KEYPOINT)
+        return fail(n, VX_ERROR_INVALID_PARAMETERS, "corners must be a KEYPOINT array");
+    if (!((_vx_array *)n->params[arr_i])->capacity) return fail(n, VX_ERROR_INVALID_DIMENSION, "corner array capacity");
+    if (n->params[num_i] && (!scl(n, num_i) || scl(n, num_i)->dataType != VX_TYPE_SIZE))
+        return fail(n, VX_ERROR_INVALID_TYPE, "num_corners must be a SIZE scalar");
+    _vx_image *in = img(n, img_i);
+    if (in->width < 7 || in->height < 7 || in->width > 32767 || in->height > 32767)
+        return fail(n, VX_ERROR_INVALID_DIMENSION, "FAST needs an image of at least 7x7");
+    return VX_SUCCESS;
+}
+
+vx_status v_fast(_vx_node *n)
+{
+    vx_status s = fast_check(n, 0, 1, 3, 4);
+    if (!s && (!scl(n, 2) || scl(n, 2)->dataType != VX_TYPE_BOOL)) s = fail(n, VX_ERROR_INVALID_TYPE, "nonmax_suppression must be BOOL");
+    return s;
+}
+
+vx_status i_fast(_vx_node *n)
+{
+    _vx_image *in = img(n, 0);
+    n->localSize = mvx_FastCorners_workspace_size(in->width, in->height) + 256;
+    return VX_SUCCESS;
+}
+
+static vx_status run_fast(_vx_node *n, hipStream_t st, _vx_image *in, float thr, bool nonmax, _vx_array *corners, _vx_scalar *num)
+{
+    const size_t wsz = mvx_FastCorners_workspace_size(in->width, in->height);
+    uint8_t *ws = n->local.ptr();
+    // the count scalar lives in the node workspace's tail when the caller gave none
+    uint64_t *count = num ? (uint64_t *)num->dev.ptr() : (uint64_t *)(ws + ((wsz + 7) & ~(size_t)7));
+    return rc(n, (nonmax ? mvx_FastCorners_XY_U8_Supression : mvx_FastCorners_XY_U8_NoSupression)(
+                     st, (uint32_t)corners->capacity, corners->items_ptr(), corners->count_ptr(), count, in->width, in->height, P(in), S(in),
+                     thr, ws, wsz));
+}
+
+vx_status x_fast(_vx_node *n, hipStream_t st)
+{
+    return run_fast(n, st, img(n, 0), scl(n, 1)->value.f32, scl(n, 2)->value.b != vx_false_e, (_vx_array *)n->params[3], scl(n, 4));
+}
+
+// AMD forms: (corners out, num_corners out, image, strength_thresh)
+vx_status v_amd_fast(_vx_node *n) { return fast_check(n, 2, 3, 0, 1); }
+
+vx_status i_amd_fast(_vx_node *n)
+{
+    _vx_image *in = img(n, 2);
+    n->localSize = mvx_FastCorners_workspace_size(in->width, in->height) + 256;
+    return VX_SUCCESS;
+}
+
+template <bool NONMAX> vx_status x_amd_fast(_vx_node *n, hipStream_t st)
+{
+    return run_fast(n, st, img(n, 2), scl(n, 3)->value.f32, NONMAX, (_vx_array *)n->params[0], scl(n, 1));
+}
+
 // ---------------------------------------------------------------------- warps (A14, A15) ---------
 // warp_affine / warp_perspective bilinear -> WARP_*_U8_U8_BILINEAR (border UNDEFINED) or
 // WARP_*_U8_U8_BILINEAR_CONSTANT with the border value (ago_drama_divide.cpp:1542-1616); the matrix
@@ -858,6 +924,16 @@ const std::vector<KernelDef> &kernel_table()
          {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false},
           {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_ARRAY, false}, {OUT, VX_TYPE_SCALAR, true}},
          v_harris, i_harris, x_harris, AGO_TARGET_AFFINITY_GPU},
+        {VX_KERNEL_FAST_CORNERS, "org.khronos.openvx.fast_corners",
+         {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_ARRAY, false},
+          {OUT, VX_TYPE_SCALAR, true}},
+         v_fast, i_fast, x_fast, AGO_TARGET_AFFINITY_GPU},
+        {AMD_FAST_CORNERS_XY_U8_SUPRESSION, "com.amd.openvx.FastCorners_XY_U8_Supression",
+         {{OUT, VX_TYPE_ARRAY, false}, {OUT, VX_TYPE_SCALAR, true}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_fast, i_amd_fast, x_amd_fast<true>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_FAST_CORNERS_XY_U8_NOSUPRESSION, "com.amd.openvx.FastCorners_XY_U8_NoSupression",
+         {{OUT, VX_TYPE_ARRAY, false}, {OUT, VX_TYPE_SCALAR, true}, {IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_SCALAR, false}},
+         v_amd_fast, i_amd_fast, x_amd_fast<false>, AGO_TARGET_AFFINITY_GPU},
         {VX_KERNEL_WARP_AFFINE, "org.khronos.openvx.warp_affine",
          {{IN, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_MATRIX, false}, {IN, VX_TYPE_SCALAR, false}, {OUT, VX_TYPE_IMAGE, false}},
          v_warp<false>, nullptr, x_warp<false>, AGO_TARGET_AFFINITY_GPU},

@@ -112,6 +112,14 @@ VX_API_ENTRY vx_node VX_API_CALL vxHarrisCornersNode(vx_graph graph, vx_image in
     return make(graph, VX_KERNEL_HARRIS_CORNERS, {input, strength_thresh, min_distance, sensitivity, g, b, corners, num_corners}, {g, b});
 }
 
+VX_API_ENTRY vx_node VX_API_CALL vxFastCornersNode(vx_graph graph, vx_image input, vx_scalar strength_thresh,
+                                                  vx_bool nonmax_suppression, vx_array corners, vx_scalar num_corners)
+{
+    vx_bool nm = nonmax_suppression;
+    _vx_scalar *b = is_ref(graph, VX_TYPE_GRAPH) ? make_scalar(graph->context, VX_TYPE_BOOL, &nm) : nullptr;
+    return make(graph, VX_KERNEL_FAST_CORNERS, {input, strength_thresh, b, corners, num_corners}, {b});
+}
+
 VX_API_ENTRY vx_node VX_API_CALL vxWarpAffineNode(vx_graph graph, vx_image input, vx_matrix matrix, vx_enum type, vx_image output)
 {
     _vx_scalar *t = senum(graph, type);
@@ -203,6 +211,12 @@ VX_API_ENTRY vx_status VX_API_CALL vxuCannyEdgeDetector(vx_context c, vx_image i
                                                        vx_enum norm_type, vx_image output)
 {
     return immediate(c, [&](vx_graph g) { return vxCannyEdgeDetectorNode(g, input, hyst, gradient_size, norm_type, output); });
+}
+
+VX_API_ENTRY vx_status VX_API_CALL vxuFastCorners(vx_context c, vx_image input, vx_scalar strength_thresh, vx_bool nonmax_suppression,
+                                                 vx_array corners, vx_scalar num_corners)
+{
+    return immediate(c, [&](vx_graph g) { return vxFastCornersNode(g, input, strength_thresh, nonmax_suppression, corners, num_corners); });
 }
 
 VX_API_ENTRY vx_status VX_API_CALL vxuHarrisCorners(vx_context c, vx_image input, vx_scalar strength_thresh, vx_scalar min_distance,

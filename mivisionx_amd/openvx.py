@@ -172,6 +172,7 @@ _PROTOS = {
     "vxMedian3x3Node": (P, [P, P, P]), "vxGaussianPyramidNode": (P, [P, P, P]),
     "vxCannyEdgeDetectorNode": (P, [P, P, P, c.c_int32, E, P]),
     "vxHarrisCornersNode": (P, [P, P, P, P, P, c.c_int32, c.c_int32, P, P]),
+    "vxFastCornersNode": (P, [P, P, P, E, P, P]), "vxuFastCorners": (E, [P, P, P, E, P, P]),
     "vxWarpAffineNode": (P, [P, P, P, E, P]), "vxWarpPerspectiveNode": (P, [P, P, P, E, P]),
     "vxRemapNode": (P, [P, P, P, E, P]), "vxHalfScaleGaussianNode": (P, [P, P, P, c.c_int32]),
     "vxOpticalFlowPyrLKNode": (P, [P, P, P, P, P, P, E, P, P, P, SZ]),

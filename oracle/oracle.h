@@ -75,6 +75,8 @@ void oracle_harris_sobel_hg3(int w, int h, float *dst, int dst_stride, const uin
 void oracle_harris_score(int w, int h, float *dst, int dst_stride, const float *hg3, int hg3_stride,
                          float k, float thr, float norm);
 float oracle_harris_norm(int gradient_size);
+int oracle_fast_corners(int w, int h, const uint8_t *src, int stride, float threshold, int nonmax,
+                        oracle_keypoint_t *out, int capacity);
 void oracle_harris_sobel_g(int w, int h, float *dst, int dst_stride, const uint8_t *src, int src_stride, int g);
 void oracle_harris_score_b(int w, int h, float *dst, int dst_stride, const float *hg3, int hg3_stride, float k,
                            float thr, float norm, int b);

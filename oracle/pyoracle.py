@@ -253,6 +253,14 @@ class Oracle:
     def harris_norm(self, gradient_size=3):
         return self.lib.oracle_harris_norm(gradient_size)
 
+    def fast(self, src, thr, nonmax, capacity):
+        """FastCorners: (keypoints, uncapped count)"""
+        h, w = src.shape
+        s, ss = _plane(src)
+        out = np.zeros(max(capacity, 1), KEYPOINT)
+        n = self.lib.oracle_fast_corners(w, h, _p(s), ss, ctypes.c_float(thr), int(nonmax), _p(out), capacity)
+        return out[:min(n, capacity)], n
+
     def harris_planes(self, src, g, b, k, thr):
         """(HG3 plane as the raw floats of its ALIGN16(12w)-byte rows, HVC plane (h, w)) of one gradient
         / block size pair"""
@@ -506,6 +514,15 @@ class Reference:
         cnt = ctypes.c_long(0)
         self._ok(self.lib.ref_harris(w, h, _p(_c(src)), ctypes.c_float(k), ctypes.c_float(thr), ctypes.c_float(dist),
                                      capacity, _p(out), ctypes.byref(n), ctypes.byref(cnt)), "harris")
+        return out[:n.value], cnt.value
+
+    def fast(self, src, thr, nonmax, capacity):
+        h, w = src.shape
+        out = np.zeros(max(capacity, 1), KEYPOINT)
+        n = ctypes.c_int(0)
+        cnt = ctypes.c_long(0)
+        self._ok(self.lib.ref_fast(w, h, _p(_c(src)), ctypes.c_float(thr), int(nonmax), capacity, _p(out), ctypes.byref(n),
+                                   ctypes.byref(cnt)), "fast")
         return out[:n.value], cnt.value
 
     def harris_ex(self, src, g, b, k, thr, dist, capacity):

@@ -365,6 +365,20 @@ int ref_harris(int w, int h, const uint8_t *src, float k, float thr, float dist,
     return get_corners(corners, num, out, numitems, count);
 }
 
+// FastCorners with and without non-maximum suppression.
+int ref_fast(int w, int h, const uint8_t *src, float thr, int nonmax, int capacity, vx_keypoint_t *out, int *numitems,
+             long *count) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_array corners = vxCreateArray(c.ctx, VX_TYPE_KEYPOINT, capacity);
+    vx_size zero = 0;
+    vx_scalar num = vxCreateScalar(c.ctx, VX_TYPE_SIZE, &zero);
+    vx_scalar sthr = vxCreateScalar(c.ctx, VX_TYPE_FLOAT32, &thr);
+    if (!vxFastCornersNode(c.graph, in, sthr, nonmax ? vx_true_e : vx_false_e, corners, num)) return -1;
+    int s = run(c); if (s) return s;
+    return get_corners(corners, num, out, numitems, count);
+}
+
 // HarrisCorners with gradient size g and block size b.
 int ref_harris_ex(int w, int h, const uint8_t *src, float k, float thr, float dist, int g, int b, int capacity,
                   vx_keypoint_t *out, int *numitems, long *count) {

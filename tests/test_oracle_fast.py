@@ -1,0 +1,43 @@
+"""The oracle's FastCorners (HafCpu_FastCorners_XY_U8_NoSupression / _Supression,
+ago_haf_cpu_fast_corners.cpp:414-790), pinned against the reference engine (oracle/_ref, vxFastCornersNode)
+and against closed forms."""
+import numpy as np
+import pytest
+
+from mivisionx_amd import synth
+
+
+@pytest.mark.parametrize("w,h", [(64, 48), (101, 61), (37, 29), (333, 77), (8, 8), (15, 9)])
+@pytest.mark.parametrize("thr", [0.0, 10.0, 25.5, 60.0])
+@pytest.mark.parametrize("nonmax", [0, 1])
+def test_fast_matches_reference(orc, ref, w, h, thr, nonmax):
+    src = synth.scene_u8(w, h, 21) if w >= 16 and h >= 16 else synth.uniform_u8(w, h, 21)
+    want, wc = ref.fast(src, thr, nonmax, 5000)
+    got, gc = orc.fast(src, thr, nonmax, 5000)
+    assert wc == gc and np.array_equal(want.view(np.uint8), got.view(np.uint8))
+
+
+def test_fast_capacity_keeps_raster_prefix(orc):
+    src = synth.scene_u8(200, 120, 3)
+    full, n = orc.fast(src, 15.0, 0, 100000)
+    part, m = orc.fast(src, 15.0, 0, 37)
+    assert m == n > 37 and np.array_equal(part.view(np.uint8), full[:37].view(np.uint8))
+    assert np.all(np.diff(full["y"].astype(np.int64) * 1000 + full["x"]) > 0)   # raster order
+
+
+def test_fast_closed_forms(orc):
+    """a bright 10x10 square on black: no corner on flat ground; the corner pixels of the square are FAST
+    corners (a run of nine dark circle pixels below c - t); with suppression one corner survives near each
+    square corner (ties resolved by the >= up/left, > right/down rule), with strength 199: the largest t
+    for which 0 < 200 - t"""
+    img = np.zeros((40, 40), np.uint8)
+    img[15:25, 15:25] = 200
+    kp, n = orc.fast(img, 50.0, 0, 100)
+    pts = {(int(k["x"]), int(k["y"])) for k in kp}
+    assert {(15, 15), (24, 15), (15, 24), (24, 24)} <= pts and all(15 <= x <= 24 and 15 <= y <= 24 for x, y in pts)
+    kp, n = orc.fast(img, 50.0, 1, 100)
+    assert n == 4 and all(k["strength"] == 199 for k in kp)
+    quads = {(int(k["x"]) < 20, int(k["y"]) < 20) for k in kp}
+    assert len(quads) == 4
+    kp, n = orc.fast(np.full((40, 40), 77, np.uint8), 1.0, 0, 100)
+    assert n == 0
