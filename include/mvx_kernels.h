@@ -92,6 +92,15 @@ int mvx_ColorConvert_Y_RGB(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstH
 int mvx_ChannelCopy_U8_U8(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
                           uint8_t *pDst, uint32_t dstStride, const uint8_t *pSrc, uint32_t srcStride);
 
+/* replace HipExec_ColorConvert_RGB_IYUV / _RGB_NV12 (hip_host_decls.h:523, :529); HafCpu_ColorConvert_RGB_IYUV /
+ * _RGB_NV12 (ago_haf_cpu_color_convert.cpp:472-768): BT.709 with one chroma sample per 2x2 block, truncated
+ * and clamped per channel.  Width and height must be even (-15 otherwise); pDst is packed RGB. */
+int mvx_ColorConvert_RGB_IYUV(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pY, uint32_t yStride, const uint8_t *pU, uint32_t uStride, const uint8_t *pV,
+                              uint32_t vStride);
+int mvx_ColorConvert_RGB_NV12(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pY, uint32_t yStride, const uint8_t *pUV, uint32_t uvStride);
+
 /* replaces HipExec_ScaleImage_U8_U8_Bilinear (hip_host_decls.h:728); HafCpu_ScaleImage_U8_U8_Bilinear
  * (ago_haf_cpu_geometric.cpp:1977-2105) with the scale matrix of ago_kernel_api.cpp:19897-19916.
  * dstStride must be a multiple of 16 (8-byte stores). */

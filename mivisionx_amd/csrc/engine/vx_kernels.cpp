@@ -155,8 +155,19 @@ template <int IO> vx_status x_phase(_vx_node *n, hipStream_t st)
 // RGB -> IYUV: the luma plane through ColorConvert_Y_RGB.  The reference keeps only this kernel when
 // the IYUV image is virtual and only its Y channel is read (ago_drama_divide.cpp:244-256 and the
 // unused-output removal, ago_drama_remove.cpp:989); chroma planes are not produced here.
+// IYUV / NV12 -> RGB: COLOR_CONVERT_RGB_IYUV / _RGB_NV12 (ago_drama_divide.cpp:80-113).  (The reference
+// CPU has no RGB -> IYUV chroma kernels -- ago_haf_cpu.cpp:26-77 return NOT_IMPLEMENTED.)
 vx_status v_color(_vx_node *n)
 {
+    _vx_image *src = img(n, 0);
+    if (src && (src->format == VX_DF_IMAGE_IYUV || src->format == VX_DF_IMAGE_NV12)) {
+        _vx_image *out = img(n, 1);
+        if (!out) return fail(n, VX_ERROR_INVALID_PARAMETERS, "output is not an image");
+        if (out->format != VX_DF_IMAGE_RGB && out->format != VX_DF_IMAGE_VIRT)
+            return fail(n, VX_ERROR_NOT_SUPPORTED, "YUV converts to RGB here");
+        if ((src->width & 1u) || (src->height & 1u)) return fail(n, VX_ERROR_INVALID_DIMENSION, "YUV 4:2:0 sizes must be even");
+        return set_output(n, 1, src->width, src->height, VX_DF_IMAGE_RGB);
+    }
     vx_status s = need_input(n, 0, VX_DF_IMAGE_RGB);
     if (s) return s;
     _vx_image *o = img(n, 1);
@@ -174,6 +185,10 @@ vx_status v_color(_vx_node *n)
 vx_status x_color(_vx_node *n, hipStream_t st)
 {
     _vx_image *in = img(n, 0), *o = img(n, 1);
+    if (in->format == VX_DF_IMAGE_IYUV)
+        return rc(n, mvx_ColorConvert_RGB_IYUV(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1), P(in, 2), S(in, 2)));
+    if (in->format == VX_DF_IMAGE_NV12)
+        return rc(n, mvx_ColorConvert_RGB_NV12(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1)));
     return rc(n, mvx_ColorConvert_Y_RGB(st, o->width, o->height, P(o), S(o), P(in), S(in)));
 }
 
@@ -191,7 +206,8 @@ vx_status x_amd_color_y(_vx_node *n, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------- ChannelExtract (A3) ------
-// channel_extract(Y) of a YUV image -> CHANNEL_COPY_U8_U8 of plane 0 (ago_drama_divide.cpp:342-414)
+// channel_extract(Y) of a YUV image, and U / V of IYUV and YUV4 -> CHANNEL_COPY_U8_U8 of that plane
+// (ago_drama_divide.cpp:342-414)
 vx_status v_extract(_vx_node *n)
 {
     _vx_image *in = img(n, 0);
@@ -199,15 +215,21 @@ vx_status v_extract(_vx_node *n)
     if (!in || !ch) return fail(n, VX_ERROR_INVALID_PARAMETERS, "arguments");
     if (in->format != VX_DF_IMAGE_IYUV && in->format != VX_DF_IMAGE_NV12 && in->format != VX_DF_IMAGE_NV21 && in->format != VX_DF_IMAGE_YUV4)
         return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction is provided for the Y channel of YUV images");
-    if (ch->value.e != VX_CHANNEL_Y && ch->value.e != VX_CHANNEL_0)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "only the Y channel is provided");
-    return set_output(n, 2, in->width, in->height, VX_DF_IMAGE_U8);
+    const vx_enum c = ch->value.e;
+    if (c == VX_CHANNEL_Y || c == VX_CHANNEL_0) return set_output(n, 2, in->width, in->height, VX_DF_IMAGE_U8);
+    if ((c == VX_CHANNEL_U || c == VX_CHANNEL_V) && (in->format == VX_DF_IMAGE_IYUV || in->format == VX_DF_IMAGE_YUV4)) {
+        const uint32_t d = in->format == VX_DF_IMAGE_IYUV ? 2 : 1;
+        return set_output(n, 2, in->width / d, in->height / d, VX_DF_IMAGE_U8);
+    }
+    return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction: Y of YUV images, U and V of IYUV and YUV4");
 }
 
 vx_status x_extract(_vx_node *n, hipStream_t st)
 {
     _vx_image *in = img(n, 0), *o = img(n, 2);
-    return rc(n, mvx_ChannelCopy_U8_U8(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0)));
+    const vx_enum c = scl(n, 1)->value.e;
+    const int plane = c == VX_CHANNEL_U ? 1 : c == VX_CHANNEL_V ? 2 : 0;
+    return rc(n, mvx_ChannelCopy_U8_U8(st, o->width, o->height, P(o), S(o), P(in, plane), S(in, plane)));
 }
 
 vx_status v_amd_copy(_vx_node *n)

@@ -308,9 +308,75 @@ k_gaussian_half_3x3(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_
     }
 }
 
+// ------------------------------------------------------------------ ColorConvert RGB <- YUV ------
+// HafCpu_ColorConvert_RGB_IYUV / _RGB_NV12 (ago_haf_cpu_color_convert.cpp:472-768), BT.709: one chroma
+// sample per 2x2 block, u = U - 128, v = V - 128; R = Y + v*1.5748, G = Y - (u*0.1873 + v*0.4681),
+// B = Y + u*1.8556, each truncated and clamped to [0, 255].  One thread: 4 consecutive pixels of a row.
+template <bool NV12>
+__global__ void __launch_bounds__(256)
+k_rgb_from_yuv(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ ys, uint32_t yStride,
+               const uint8_t *__restrict__ us, uint32_t uStride, const uint8_t *__restrict__ vs, uint32_t vStride)
+{
+    const uint32_t x0 = (blockIdx.x * 256 + threadIdx.x) * 4, r = blockIdx.y;
+    if (x0 >= w || r >= h) return;
+    uint8_t *d = dst + (size_t)r * dstStride + 3ull * x0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+        const uint32_t x = x0 + k;
+        if (x >= w) break;
+        const float Y = (float)ys[(size_t)r * yStride + x];
+        float U, V;
+        if (NV12) {
+            const uint8_t *c = us + (size_t)(r / 2) * uStride + 2 * (x / 2);
+            U = (float)c[0] - 128.0f;
+            V = (float)c[1] - 128.0f;
+        } else {
+            U = (float)us[(size_t)(r / 2) * uStride + x / 2] - 128.0f;
+            V = (float)vs[(size_t)(r / 2) * vStride + x / 2] - 128.0f;
+        }
+        const float G = (U * 0.1873f) + (V * 0.4681f), R = V * 1.5748f, B = U * 1.8556f;
+        d[3 * k + 0] = (uint8_t)fminf(fmaxf(Y + R, 0.0f), 255.0f);
+        d[3 * k + 1] = (uint8_t)fminf(fmaxf(Y - G, 0.0f), 255.0f);
+        d[3 * k + 2] = (uint8_t)fminf(fmaxf(Y + B, 0.0f), 255.0f);
+    }
+}
+
 }  // namespace mvx
 
 using namespace mvx;
+
+namespace {
+int rgb_from_yuv(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *dst, uint32_t dstStride, const uint8_t *y, uint32_t yStride,
+                 const uint8_t *u, uint32_t uStride, const uint8_t *v, uint32_t vStride, bool nv12)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if ((w & 1u) || (h & 1u)) return -15;   // the reference converts 2x2 blocks and writes past odd sizes
+    if (!dst || !y || !u || (!nv12 && !v) || dstStride < 3 * w || yStride < w || uStride < (nv12 ? w : w / 2) ||
+        (!nv12 && vStride < w / 2))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    dim3 grid(((w + 3) / 4 + 255) / 256, h);
+    if (nv12)
+        hipLaunchKernelGGL(k_rgb_from_yuv<true>, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, y, yStride, u, uStride, u,
+                           uStride);
+    else
+        hipLaunchKernelGGL(k_rgb_from_yuv<false>, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, y, yStride, u, uStride, v,
+                           vStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+}  // namespace
+
+extern "C" int mvx_ColorConvert_RGB_IYUV(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *dst, uint32_t dstStride, const uint8_t *y,
+                                         uint32_t yStride, const uint8_t *u, uint32_t uStride, const uint8_t *v, uint32_t vStride)
+{
+    return rgb_from_yuv(stream, w, h, dst, dstStride, y, yStride, u, uStride, v, vStride, false);
+}
+
+extern "C" int mvx_ColorConvert_RGB_NV12(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *dst, uint32_t dstStride, const uint8_t *y,
+                                         uint32_t yStride, const uint8_t *uv, uint32_t uvStride)
+{
+    return rgb_from_yuv(stream, w, h, dst, dstStride, y, yStride, uv, uvStride, nullptr, 0, true);
+}
 
 extern "C" int mvx_ColorConvert_Y_RGB(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *dst, uint32_t dstStride,
                                       const uint8_t *rgb, uint32_t rgbStride)

@@ -56,6 +56,8 @@ VX_INTERPOLATION_NEAREST_NEIGHBOR = _enum(0, 0x04) + 0
 VX_INTERPOLATION_BILINEAR = _enum(0, 0x04) + 1
 VX_INTERPOLATION_AREA = _enum(0, 0x04) + 2
 VX_CHANNEL_Y = _enum(0, 0x09) + 0x14
+VX_CHANNEL_U = _enum(0, 0x09) + 0x15
+VX_CHANNEL_V = _enum(0, 0x09) + 0x16
 VX_THRESHOLD_TYPE_BINARY = _enum(0, 0x0B) + 0
 VX_THRESHOLD_TYPE_RANGE = _enum(0, 0x0B) + 1
 VX_NORM_L1 = _enum(0, 0x10) + 0
@@ -167,6 +169,7 @@ _PROTOS = {
     "vxRegisterLogCallback": (None, [P, LOG_CB, E]),
     "vxCreateGenericNode": (P, [P, P]), "vxSetParameterByIndex": (E, [P, U32, P]),
     "vxColorConvertNode": (P, [P, P, P]), "vxChannelExtractNode": (P, [P, P, E, P]),
+    "vxuColorConvert": (E, [P, P, P]), "vxuChannelExtract": (E, [P, P, E, P]),
     "vxSobel3x3Node": (P, [P, P, P, P]), "vxMagnitudeNode": (P, [P, P, P, P]), "vxPhaseNode": (P, [P, P, P, P]),
     "vxScaleImageNode": (P, [P, P, P, E]), "vxBox3x3Node": (P, [P, P, P]), "vxGaussian3x3Node": (P, [P, P, P]),
     "vxMedian3x3Node": (P, [P, P, P]), "vxGaussianPyramidNode": (P, [P, P, P]),
@@ -231,13 +234,14 @@ def create_image(ctx, w, h, fmt):
     return _ok(lib().vxCreateImage(ctx, w, h, DF[fmt]), "vxCreateImage")
 
 
-def write_image(img, arr, plane=0):
-    """copy a host array (h, w[, channels]) into plane `plane` of an image"""
+def write_image(img, arr, plane=0, sub=1):
+    """copy a host array (h, w[, channels]) into plane `plane` of an image; `sub` is the plane's
+    subsampling (2 for the chroma planes of IYUV / NV12: the rectangle is in image coordinates)"""
     arr = np.ascontiguousarray(arr)
     h, w = arr.shape[:2]
     bpp = arr.dtype.itemsize * (arr.shape[2] if arr.ndim == 3 else 1)
     addr = Addressing(w, h, bpp, w * bpp, 1024, 1024, 1, 1, 0)
-    check(lib().vxCopyImagePatch(img, c.byref(Rect(0, 0, w, h)), plane, c.byref(addr), arr.ctypes.data_as(P),
+    check(lib().vxCopyImagePatch(img, c.byref(Rect(0, 0, w * sub, h * sub)), plane, c.byref(addr), arr.ctypes.data_as(P),
                                  VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST), "vxCopyImagePatch(write)")
 
 

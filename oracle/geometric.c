@@ -239,6 +239,37 @@ int oracle_scale_area(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int 
     return -1;
 }
 
+/* ColorConvert_RGB_IYUV / _RGB_NV12: ago_haf_cpu_color_convert.cpp:472-623 / 624-768 (BT.709).  Each 2x2
+ * block shares one chroma sample (IYUV: separate U, V planes; NV12: interleaved U,V); u = U - 128,
+ * v = V - 128 in float; per pixel, truncated and clamped to [0, 255]:
+ *   R = Y + v*1.5748, G = Y - (u*0.1873 + v*0.4681), B = Y + u*1.8556
+ * (the SSE body adds u*0 or v*0 terms, which do not change the sums, and truncates before saturating;
+ * the scalar tail clamps before truncating -- the same bytes).  w and h even.
+ * Pinned by: bit-exact against oracle/_ref (color_convert IYUV -> RGB, NV12 -> RGB); grey chroma
+ * (128) gives R = G = B = Y. */
+void oracle_color_rgb_yuv(int w, int h, uint8_t *dst, int dst_stride, const uint8_t *y, int y_stride, const uint8_t *u,
+                          int u_stride, const uint8_t *v, int v_stride, int nv12)
+{
+    for (int r = 0; r < h; r++)
+        for (int x = 0; x < w; x++) {
+            const float Y = (float)y[(long)r * y_stride + x];
+            float U, V;
+            if (nv12) {
+                U = (float)u[(long)(r / 2) * u_stride + 2 * (x / 2)] - 128.0f;
+                V = (float)u[(long)(r / 2) * u_stride + 2 * (x / 2) + 1] - 128.0f;
+            } else {
+                U = (float)u[(long)(r / 2) * u_stride + x / 2] - 128.0f;
+                V = (float)v[(long)(r / 2) * v_stride + x / 2] - 128.0f;
+            }
+            const float G = (U * 0.1873f) + (V * 0.4681f), R = V * 1.5748f, B = U * 1.8556f;
+            const float c[3] = {Y + R, Y - G, Y + B};
+            for (int k = 0; k < 3; k++) {
+                const float f = fminf(fmaxf(c[k], 0.0f), 255.0f);
+                dst[(long)r * dst_stride + 3L * x + k] = (uint8_t)f;
+            }
+        }
+}
+
 /* ScaleGaussianHalf_U8_U8_3x3: ago_haf_cpu_geometric.cpp:2772-2829, wrapper ago_kernel_api.cpp:15341-15352
  * (rows start one row in on both images; zeroed node scratch, ago_interface.cpp:1859-1866).
  * Output rows 1..dh-2, all columns; output row y is centred on source row 2y whatever the parity:

@@ -48,6 +48,8 @@ uint8_t oracle_phase_px(int gx, int gy);
 void oracle_color_y_rgb(int w, int h, uint8_t *dst, int dst_stride, const uint8_t *rgb, int rgb_stride);
 void oracle_scale_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
                            int src_stride);
+void oracle_color_rgb_yuv(int w, int h, uint8_t *dst, int dst_stride, const uint8_t *y, int y_stride, const uint8_t *u,
+                          int u_stride, const uint8_t *v, int v_stride, int nv12);
 int oracle_scale_gaussian_half_3x3(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh,
                                    const uint8_t *src, int src_stride);
 int oracle_scale_gaussian_half_5x5(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh,

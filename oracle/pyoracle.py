@@ -179,6 +179,15 @@ class Oracle:
             raise ValueError("ScaleGaussianHalf needs src >= 5 and dst >= 3")
         return np.ascontiguousarray(d[:, :dw])
 
+    def color_rgb_yuv(self, y, u, v=None):
+        """color_convert IYUV -> RGB (u, v planes) or NV12 -> RGB (u = the interleaved (h/2, w) UV plane)"""
+        h, w = y.shape
+        nv12 = v is None
+        d = np.zeros((h, w, 3), np.uint8)
+        vv = _c(v) if v is not None else _c(u)
+        self.lib.oracle_color_rgb_yuv(w, h, _p(d), 3 * w, _p(_c(y)), w, _p(_c(u)), u.shape[1], _p(vv), vv.shape[1], int(nv12))
+        return d
+
     def halfscale(self, src, ksize):
         """halfscale_gaussian (ago_drama_divide.cpp:1471-1488): kernel size 1 -> ScaleImage nearest,
         3 -> ScaleGaussianHalf 3x3, 5 -> ScaleGaussianHalf 5x5, output (w+1)/2 x (h+1)/2 on
@@ -477,6 +486,13 @@ class Reference:
         sh, sw = src.shape
         d = np.zeros((dh, dw), np.uint8)
         self._ok(self.lib.ref_amd_scale_area(sw, sh, _p(_c(src)), dw, dh, _p(d)), "amd_scale_area")
+        return d
+
+    def color_rgb_yuv(self, y, u, v=None):
+        h, w = y.shape
+        d = np.zeros((h, w, 3), np.uint8)
+        vv = _c(v) if v is not None else _c(u)
+        self._ok(self.lib.ref_color_rgb_yuv(w, h, _p(_c(y)), _p(_c(u)), _p(vv), int(v is None), _p(d)), "color_rgb_yuv")
         return d
 
     def halfscale(self, src, ksize):

@@ -254,6 +254,28 @@ int ref_amd_scale_area(int sw, int sh, const uint8_t *src, int dw, int dh, uint8
     return 0;
 }
 
+// color_convert IYUV -> RGB (nv12 == 0: y, u, v planes of w x h, w/2 x h/2, w/2 x h/2) or NV12 -> RGB
+// (nv12 != 0: y plane and the interleaved w/2 x h/2 UV plane passed as u).
+int ref_color_rgb_yuv(int w, int h, const uint8_t *y, const uint8_t *u, const uint8_t *v, int nv12, uint8_t *rgb) {
+    Ctx c;
+    vx_image in = vxCreateImage(c.ctx, w, h, nv12 ? VX_DF_IMAGE_NV12 : VX_DF_IMAGE_IYUV);
+    vx_rectangle_t r = {0, 0, (vx_uint32)w, (vx_uint32)h};
+    vx_imagepatch_addressing_t a = {};
+    a.dim_x = w; a.dim_y = h; a.stride_x = 1; a.stride_y = w;
+    a.scale_x = VX_SCALE_UNITY; a.scale_y = VX_SCALE_UNITY; a.step_x = 1; a.step_y = 1;
+    CHECK(vxCopyImagePatch(in, &r, 0, &a, (void *)y, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    vx_imagepatch_addressing_t b = {};
+    b.dim_x = w; b.dim_y = h; b.stride_x = nv12 ? 2 : 1; b.stride_y = nv12 ? w : w / 2;
+    b.scale_x = VX_SCALE_UNITY; b.scale_y = VX_SCALE_UNITY; b.step_x = 2; b.step_y = 2;
+    CHECK(vxCopyImagePatch(in, &r, 1, &b, (void *)u, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    if (!nv12) CHECK(vxCopyImagePatch(in, &r, 2, &b, (void *)v, VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST));
+    vx_image out = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_RGB);
+    if (!vxColorConvertNode(c.graph, in, out)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(out, w, h, rgb, 3));
+    return 0;
+}
+
 // halfscale_gaussian with kernel size 1, 3 or 5 (output (w+1)/2 x (h+1)/2).
 int ref_halfscale(int w, int h, const uint8_t *src, int ksize, uint8_t *dst) {
     Ctx c;
