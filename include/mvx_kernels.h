@@ -462,6 +462,20 @@ int mvx_ChainBatch_prepare(const mvx_chain_image_t *images, uint32_t count, uint
                            void *workspace, size_t workspaceSize, mvx_chain_plan_t *plan);
 /* One persistent launch over every tile of the plan; asynchronous, capturable. */
 int mvx_ChainBatch_run(mvx_stream_t stream, const mvx_chain_plan_t *plan);
+/* The same plan from the amd_rpp batchPD layout (amd_rpp/source/image/*batchPD.cpp, e.g.
+ * AbsoluteDifferencebatchPD.cpp:52-61, and the rppi_*_batchPD_gpu calls at :146-161): nbatch images
+ * stacked in one tall plane per role, image i in rows [i*maxHeight, i*maxHeight + heights[i]) and
+ * columns [0, widths[i]) of the plane, with the plane's stride; widths / heights are host arrays (the
+ * srcBatch_width / srcBatch_height arrays).  The outputs gx, gy, mag, phase use the same rows per
+ * image; half (optional) uses halfMaxHeight rows per image.  Neighbour reads follow the linear memory
+ * model with the tall plane's stride (column 0 and w-1 see the plane's bytes beside the image).  No
+ * reference counterpart: it lets batchPD callers (RPP-style plug-ins) reach the batched chain without
+ * building descriptors. */
+int mvx_ChainBatch_prepare_batchPD(const uint8_t *pSrc, uint32_t srcStride, uint32_t maxHeight, const uint32_t *widths,
+                                   const uint32_t *heights, uint32_t nbatch, int16_t *pGx, uint32_t gxStride, int16_t *pGy,
+                                   uint32_t gyStride, int16_t *pMag, uint32_t magStride, uint8_t *pPhase, uint32_t phaseStride,
+                                   uint8_t *pHalf, uint32_t halfStride, uint32_t halfMaxHeight, uint32_t tile,
+                                   uint32_t workgroupsPerCU, void *workspace, size_t workspaceSize, mvx_chain_plan_t *plan);
 /* Times every tile shape x workgroups-per-CU step (reps launches each, HIP events on `stream`),
  * prepares the fastest into `workspace` and returns its plan; msTable (optional) receives
  * MVX_CHAIN_TILE_COUNT * MVX_CHAIN_OCCUPANCY_STEPS milliseconds per launch.  Synchronises `stream`;

@@ -329,6 +329,37 @@ using namespace mvx;
 
 extern "C" size_t mvx_ChainBatch_workspace_size(uint32_t count) { return (size_t)count * sizeof(ChainRow); }
 
+// batchPD front end: image i of every tall plane starts at row i * (the plane's rows per image)
+extern "C" int mvx_ChainBatch_prepare_batchPD(const uint8_t *src, uint32_t srcStride, uint32_t maxHeight, const uint32_t *widths,
+                                              const uint32_t *heights, uint32_t nbatch, int16_t *gx, uint32_t gxStride, int16_t *gy,
+                                              uint32_t gyStride, int16_t *mag, uint32_t magStride, uint8_t *phase,
+                                              uint32_t phaseStride, uint8_t *half, uint32_t halfStride, uint32_t halfMaxHeight,
+                                              uint32_t tile, uint32_t workgroupsPerCU, void *workspace, size_t workspaceSize,
+                                              mvx_chain_plan_t *plan)
+{
+    if (!src || !widths || !heights || nbatch == 0 || !gx || !gy || !mag || !phase || !plan) return MVX_ERROR_INVALID_PARAMETERS;
+    std::vector<mvx_chain_image_t> images(nbatch);
+    for (uint32_t i = 0; i < nbatch; i++) {
+        if (heights[i] > maxHeight || (half && heights[i] / 2 > halfMaxHeight)) return MVX_ERROR_INVALID_PARAMETERS;
+        mvx_chain_image_t &im = images[i];
+        im.src = src + (size_t)i * maxHeight * srcStride;
+        im.width = widths[i];
+        im.height = heights[i];
+        im.srcStride = srcStride;
+        im.gx = (int16_t *)((uint8_t *)gx + (size_t)i * maxHeight * gxStride);
+        im.gxStride = gxStride;
+        im.gy = (int16_t *)((uint8_t *)gy + (size_t)i * maxHeight * gyStride);
+        im.gyStride = gyStride;
+        im.mag = (int16_t *)((uint8_t *)mag + (size_t)i * maxHeight * magStride);
+        im.magStride = magStride;
+        im.phase = phase + (size_t)i * maxHeight * phaseStride;
+        im.phaseStride = phaseStride;
+        im.half = half ? half + (size_t)i * halfMaxHeight * halfStride : nullptr;
+        im.halfStride = halfStride;
+    }
+    return mvx_ChainBatch_prepare(images.data(), nbatch, tile, workgroupsPerCU, workspace, workspaceSize, plan);
+}
+
 extern "C" int mvx_ChainBatch_prepare(const mvx_chain_image_t *images, uint32_t count, uint32_t tile,
                                       uint32_t workgroupsPerCU, void *workspace, size_t workspaceSize,
                                       mvx_chain_plan_t *plan)

@@ -119,3 +119,51 @@ def test_autotune_then_batch_vs_per_node(gpu):
         want = b.results(i)
         for k in want:
             assert_same(got[i][k], want[k], f"image {i} {sizes[i]} {k}")
+
+
+def test_batchpd_layout(gpu, orc):
+    """the batchPD front end (images stacked in tall planes, per-image widths/heights arrays) gives the
+    same per-image results as the oracle's node chain.  Neighbour reads are linear in the tall plane's
+    stride, so the widths here are not multiples of 16: then the bytes left of column 0 and right of
+    column w-1 are zero padding in both layouts"""
+    import ctypes
+    from mivisionx_amd import kernels
+    from mivisionx_amd.pipelines import ChainPlan
+    dims = [(34, 18), (322, 242), (18, 4), (130, 66), (2, 6), (98, 100)]
+    n = len(dims)
+    maxW, maxH = max(w for w, _ in dims), max(h for _, h in dims)
+    al = lambda b: (b + 15) & ~15
+    src = torch.zeros((n * maxH, al(maxW)), dtype=torch.uint8, device="cuda")
+    imgs = []
+    for i, (w, h) in enumerate(dims):
+        a = synth.scene_u8(w, h, seed=70 + i) if w >= 16 and h >= 16 else synth.uniform_u8(w, h, seed=70 + i)
+        imgs.append(a)
+        src[i * maxH:i * maxH + h, :w] = torch.from_numpy(a).cuda()
+    gx = torch.zeros((n * maxH, al(2 * maxW)), dtype=torch.uint8, device="cuda")
+    gy, mag = torch.zeros_like(gx), torch.zeros_like(gx)
+    ph = torch.zeros((n * maxH, al(maxW)), dtype=torch.uint8, device="cuda")
+    half = torch.zeros((n * (maxH // 2), al(maxW // 2)), dtype=torch.uint8, device="cuda")
+    K = kernels()
+    K.mvx_ChainBatch_workspace_size.restype = ctypes.c_size_t
+    wsz = K.mvx_ChainBatch_workspace_size(n)
+    ws = torch.zeros(int(wsz) + 256, dtype=torch.uint8, device="cuda")
+    U32 = ctypes.c_uint32 * n
+    plan = ChainPlan()
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    rc = K.mvx_ChainBatch_prepare_batchPD(p(src), src.shape[1], maxH, U32(*[w for w, _ in dims]), U32(*[h for _, h in dims]), n,
+                                          p(gx), gx.shape[1], p(gy), gy.shape[1], p(mag), mag.shape[1], p(ph), ph.shape[1],
+                                          p(half), half.shape[1], maxH // 2, 4, 0, p(ws), ctypes.c_size_t(int(wsz)),
+                                          ctypes.byref(plan))
+    assert rc == 0
+    assert K.mvx_ChainBatch_run(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), ctypes.byref(plan)) == 0
+    torch.cuda.synchronize()
+    G, Y, M, P_, H = (t.cpu().numpy() for t in (gx, gy, mag, ph, half))
+    for i, (w, h) in enumerate(dims):
+        want = _oracle(orc, imgs[i])
+        r0 = i * maxH
+        assert_same(G[r0:r0 + h].view(np.int16)[:, :w], want["gx"], f"batchPD gx {i}")
+        assert_same(Y[r0:r0 + h].view(np.int16)[:, :w], want["gy"], f"batchPD gy {i}")
+        assert_same(M[r0:r0 + h].view(np.int16)[:, :w], want["mag"], f"batchPD mag {i}")
+        assert_same(P_[r0:r0 + h, :w], want["phase"], f"batchPD phase {i}")
+        hr = i * (maxH // 2)
+        assert_same(H[hr:hr + h // 2, :w // 2], want["half"], f"batchPD half {i}")
