@@ -344,6 +344,24 @@ int mvx_Remap_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dst
 int mvx_Remap_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
                                      uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
                                      uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue);
+/* RGB (packed U24) bilinear remap for the stitch path.  The reference registers REMAP_U24_U24_BILINEAR
+ * with an empty execute (ago_kernel_api.cpp:18535-18539); here each channel equals
+ * mvx_Remap_U8_U8_Bilinear[_Constant] of that channel over the same 13.3 table (valid entries only
+ * read inside the image, so the layout does not matter).  Exactly dstWidth pixels per row. */
+int mvx_Remap_U24_U24_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                               uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride,
+                               const void *pMap, uint32_t mapStride);
+int mvx_Remap_U24_U24_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                        uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                        uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue);
+/* Stitch merge of up to MVX_STITCH_MAX_CAMERAS remapped RGB camera images with per-camera U8 weight
+ * planes (the role of Loom's merge, amd_loomsl/kernels/merge.cpp:223-290, OpenCL-only in the
+ * reference): per channel the in-order float sum of (w_k / 255) * p_k, rounded to nearest and
+ * saturated.  pImages / pWeights / strides are host arrays of device pointers and pitches. */
+#define MVX_STITCH_MAX_CAMERAS 16
+int mvx_StitchMerge_RGB(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pDst, uint32_t dstStride, uint32_t count,
+                        const uint8_t *const *pImages, const uint32_t *imageStrides, const uint8_t *const *pWeights,
+                        const uint32_t *weightStrides);
 /* The fixed-point remap table of vxSetRemapPoint (vx_api.cpp:7865-7894, 3 fractional bits) from a
  * device float table of {x, y} pairs, on the GPU.  Sources must be below 8192 pixels in each
  * dimension (the reference uses fewer fractional bits beyond, ago_util.cpp:2450-2459). */

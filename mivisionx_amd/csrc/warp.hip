@@ -263,6 +263,67 @@ k_remap_nn(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStri
     }
 }
 
+// RGB remap for the stitch path: the reference registers REMAP_U24_U24_BILINEAR but executes nothing
+// (ago_kernel_api.cpp:18535-18539), so each channel is defined as the U8 bilinear remap of that channel
+// (HafCpu_Remap_U8_U8_Bilinear[_Constant], ago_haf_cpu_geometric.cpp:249-504) over the same 13.3 table:
+// every read of a valid entry stays inside the image (the table marks x >= W-1 and y >= H-1 invalid),
+// so packed and planar layouts give the same bytes.  Exactly w pixels per row are written.
+template <bool CONSTANT>
+__global__ void __launch_bounds__(256)
+k_remap_u24(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+            const uint8_t *__restrict__ src, uint32_t srcStride, const uint8_t *__restrict__ map, uint32_t mapStride, uint32_t border)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= dw || y >= dh) return;
+    const uint32_t e = *reinterpret_cast<const uint32_t *>(map + (uint64_t)y * mapStride + 4ull * x);
+    uint32_t mx = e & 0xffffu, my = e >> 16;
+    const bool xbad = mx == 0xffffu;
+    if (mx == 0xffffu) mx = CONSTANT ? 0u : 8u;
+    if (my == 0xffffu) my = CONSTANT ? 0u : 8u;
+    const int32_t fx = (int32_t)(mx & 7u), fy = (int32_t)(my & 7u);
+    const uint32_t xi = min(mx >> 3, sw - 1), yi = min(my >> 3, sh - 1);
+    const uint32_t x1 = min(xi + 1, sw - 1), y1 = min(yi + 1, sh - 1);
+    const uint8_t *r0 = src + (uint64_t)yi * srcStride, *r1 = src + (uint64_t)y1 * srcStride;
+    uint8_t *d = dst + (uint64_t)y * dstStride + 3ull * x;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int32_t p00 = r0[3 * xi + c], p10 = r0[3 * x1 + c], p01 = r1[3 * xi + c], p11 = r1[3 * x1 + c];
+        int32_t v = (p00 * (8 - fx) * (8 - fy) + p10 * fx * (8 - fy) + p01 * (8 - fx) * fy + p11 * fx * fy + 32) >> 6;
+        if (CONSTANT && xbad) v = (int32_t)border;
+        d[c] = (uint8_t)sat8(v);
+    }
+}
+
+// Stitch merge: the weighted sum of the remapped camera images (the role of Loom's merge kernel,
+// amd_loomsl/kernels/merge.cpp:223-290, which scales U8 weights by 1/255 and accumulates the cameras in
+// order; Loom is OpenCL-only and has no CPU form).  Per channel acc = sum over k in order of
+// ((float)w_k * (1/255)) * (float)p_k, each product and sum rounded separately; out = acc rounded to
+// nearest (ties to even), saturated to U8.
+struct MergeArgs {
+    const uint8_t *img[MVX_STITCH_MAX_CAMERAS];
+    const uint8_t *wt[MVX_STITCH_MAX_CAMERAS];
+    uint32_t imgStride[MVX_STITCH_MAX_CAMERAS], wtStride[MVX_STITCH_MAX_CAMERAS];
+    uint32_t count;
+};
+
+__global__ void __launch_bounds__(256)
+k_stitch_merge(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStride, MergeArgs a)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const float k255 = 1.0f / 255.0f;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < a.count; k++) {
+        const float wk = (float)a.wt[k][(uint64_t)y * a.wtStride[k] + x] * k255;
+        const uint8_t *p = a.img[k] + (uint64_t)y * a.imgStride[k] + 3ull * x;
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] = acc[c] + wk * (float)p[c];
+    }
+    uint8_t *d = dst + (uint64_t)y * dstStride + 3ull * x;
+#pragma unroll
+    for (int c = 0; c < 3; c++) d[c] = (uint8_t)min(max(__float2int_rn(acc[c]), 0), 255);
+}
+
 // vxSetRemapPoint (vx_api.cpp:7865-7894) for every entry: x = (u16)(sx*8 + 0.5f), y likewise,
 // (0xffff, 0xffff) when sx < 0, sy < 0, sx >= W-1 or sy >= H-1 (3 fractional bits: sources below
 // 8192 pixels, ago_util.cpp:2450-2459).
@@ -496,6 +557,61 @@ extern "C" int mvx_Remap_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t ds
 {
     return remap_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, true,
                     borderValue);
+}
+
+namespace {
+int remap_u24(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uint32_t dstStride, uint32_t sw, uint32_t sh, const uint8_t *src,
+              uint32_t srcStride, const void *map, uint32_t mapStride, bool constant, uint32_t border)
+{
+    if (!dst || !src || !map || !dw || !dh || sw < 2 || sh < 2 || sw >= 8192 || sh >= 8192 || dstStride < 3 * dw || srcStride < 3 * sw ||
+        ((uintptr_t)map & 3u) || (mapStride & 3u) || mapStride < 4u * dw)
+        return MVX_ERROR_INVALID_PARAMETERS;
+    const dim3 block(64, 4), grid((dw + 63) / 64, (dh + 3) / 4);
+    if (constant)
+        hipLaunchKernelGGL(k_remap_u24<true>, grid, block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src, srcStride,
+                           (const uint8_t *)map, mapStride, border);
+    else
+        hipLaunchKernelGGL(k_remap_u24<false>, grid, block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src, srcStride,
+                           (const uint8_t *)map, mapStride, 0u);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+}  // namespace
+
+extern "C" int mvx_Remap_U24_U24_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                                          uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride, const void *pMap,
+                                          uint32_t mapStride)
+{
+    return remap_u24(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, false, 0);
+}
+
+extern "C" int mvx_Remap_U24_U24_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
+                                                   uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
+                                                   uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue)
+{
+    return remap_u24(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, true, borderValue);
+}
+
+extern "C" int mvx_StitchMerge_RGB(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pDst, uint32_t dstStride, uint32_t count,
+                                  const uint8_t *const *pImages, const uint32_t *imageStrides, const uint8_t *const *pWeights,
+                                  const uint32_t *weightStrides)
+{
+    if (!pDst || !pImages || !imageStrides || !pWeights || !weightStrides || count == 0 || count > MVX_STITCH_MAX_CAMERAS || !width ||
+        !height || dstStride < 3 * width)
+        return MVX_ERROR_INVALID_PARAMETERS;
+    MergeArgs a{};
+    a.count = count;
+    for (uint32_t k = 0; k < count; k++) {
+        if (!pImages[k] || !pWeights[k] || imageStrides[k] < 3 * width || weightStrides[k] < width) return MVX_ERROR_INVALID_PARAMETERS;
+        a.img[k] = pImages[k];
+        a.wt[k] = pWeights[k];
+        a.imgStride[k] = imageStrides[k];
+        a.wtStride[k] = weightStrides[k];
+    }
+    const dim3 block(64, 4), grid((width + 63) / 64, (height + 3) / 4);
+    hipLaunchKernelGGL(k_stitch_merge, grid, block, 0, stream_of(stream), width, height, pDst, dstStride, a);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
 }
 
 extern "C" int mvx_Remap_FixedMap_F32(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, void *pFixedMap,

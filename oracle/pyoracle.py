@@ -370,6 +370,17 @@ class Oracle:
         self.lib.oracle_remap_nearest(dw, dh, _p(d), ds, sw, sh, _p(s), ss, _p(fixed), int(constant), border, _p(u), dw)
         return (d if padded else np.ascontiguousarray(d[:, :dw])), u.astype(bool)
 
+    def stitch_merge(self, images, weights):
+        """the stitch merge: per channel the in-order float32 sum of (w_k * (1/255)) * p_k, rounded to
+        nearest even and saturated (Loom's merge, amd_loomsl/kernels/merge.cpp:223-290, has no CPU form:
+        parity unpinned against the reference; pinned by closed forms in the tests)"""
+        k255 = np.float32(1.0) / np.float32(255.0)
+        acc = np.zeros(images[0].shape, np.float32)
+        for img, wt in zip(images, weights):
+            wk = (wt.astype(np.float32) * k255)[..., None]
+            acc = (acc + (wk * img.astype(np.float32)).astype(np.float32)).astype(np.float32)
+        return np.clip(np.rint(acc), 0, 255).astype(np.uint8)
+
     # ---- pyramidal Lucas-Kanade ----
     def lk_state(self, w0, h0, capacity):
         """zeroed scratch state of one tracker node (the reference node's local data)"""

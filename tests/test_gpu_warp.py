@@ -244,3 +244,27 @@ def test_warp_remap_through_vx_api(gpu, orc):
     finally:
         h_ = ctypes.c_void_p(ctx)
         L.vxReleaseContext(ctypes.byref(h_))
+
+
+@pytest.mark.parametrize("w,h", [(333, 77), (640, 480), (37, 29), (1920, 1080)])
+def test_remap_rgb_is_three_u8_remaps(gpu, orc, w, h):
+    """the stitch path's packed-RGB remap equals the U8 bilinear remap of each channel"""
+    rgb = np.stack([synth.scene_u8(w, h, 60 + c) for c in range(3)], -1)
+    s = Plane.from_array(rgb, "RGB")
+    for k1 in (-0.2, 0.15):
+        mp = synth.barrel_map(w, h, k1)
+        mp[::5, ::3] = [-2.0, 1.0]
+        fmap = torch.from_numpy(np.ascontiguousarray(mp)).cuda()
+        fixed = torch.zeros((h, w * 4), dtype=torch.uint8, device="cuda")
+        call("Remap_FixedMap_F32", w, h, ctypes.c_void_p(fixed.data_ptr()), w * 4, ctypes.c_void_p(fmap.data_ptr()), w * 8, w, h)
+        for border in (None, 90):
+            d = Plane(w, h, "RGB")
+            if border is None:
+                call("Remap_U24_U24_Bilinear", w, h, d, d.stride, w, h, s, s.stride, ctypes.c_void_p(fixed.data_ptr()), w * 4)
+            else:
+                call("Remap_U24_U24_Bilinear_Constant", w, h, d, d.stride, w, h, s, s.stride, ctypes.c_void_p(fixed.data_ptr()), w * 4,
+                     ctypes.c_uint8(border))
+            got = d.download()
+            for c in range(3):
+                want = orc.remap(np.ascontiguousarray(rgb[..., c]), mp, border is not None, border or 0)
+                assert_same(got[..., c], want, f"RGB remap {w}x{h} k1 {k1} border {border} channel {c}")
