@@ -21,6 +21,15 @@ NODE_SET = [
     "vision_profile/25_median.gdf", "vision_profile/31_phase.gdf", "vision_profile/32_gaussianPyramid.gdf",
     "vision_profile/34_scaleImage.gdf", "vision_profile/35_sobel.gdf", "vision_profile/39_warpAffine.gdf",
     "vision_profile/40_warpPerspective.gdf",
+    # the further kernel families
+    "geometric/Remap_U8_U8_Nearest.gdf", "geometric/Remap_U8_U8_Nearest_Constant.gdf", "geometric/ScaleImage_U8_U8_Area.gdf",
+    "geometric/ScaleImage_U8_U8_Bilinear_Constant.gdf", "geometric/ScaleImage_U8_U8_Bilinear_Replicate.gdf",
+    "geometric/ScaleImage_U8_U8_Nearest.gdf", "geometric/WarpAffine_U8_U8_Nearest.gdf", "geometric/WarpAffine_U8_U8_Nearest_Constant.gdf",
+    "geometric/WarpPerspective_U8_U8_Nearest_Constant.gdf", "vision/Canny_3x3_L2NORM.gdf", "vision/Canny_5x5_L1NORM.gdf",
+    "vision/Canny_5x5_L2NORM.gdf", "vision/Canny_7x7_L1NORM.gdf", "vision/Canny_7x7_L2NORM.gdf",
+    "vision/FastCorners_XY_U8_NoSupression.gdf", "vision/FastCorners_XY_U8_Supression.gdf", "vision/Harris_5x5.gdf",
+    "vision/Harris_7x7.gdf", "color/ColorConvert_RGB_IYUV.gdf", "color/ColorConvert_RGB_NV12.gdf", "vision_profile/11_colorConvert.gdf",
+    "vision_profile/17_fastCorners.gdf", "vision_profile/29_opticalFlowLK.gdf", "vision_profile/33_remap.gdf",
 ]
 
 
