@@ -217,17 +217,22 @@ vx_status v_extract(_vx_node *n)
         return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction is provided for the Y channel of YUV images");
     const vx_enum c = ch->value.e;
     if (c == VX_CHANNEL_Y || c == VX_CHANNEL_0) return set_output(n, 2, in->width, in->height, VX_DF_IMAGE_U8);
-    if ((c == VX_CHANNEL_U || c == VX_CHANNEL_V) && (in->format == VX_DF_IMAGE_IYUV || in->format == VX_DF_IMAGE_YUV4)) {
-        const uint32_t d = in->format == VX_DF_IMAGE_IYUV ? 2 : 1;
+    if ((c == VX_CHANNEL_U || c == VX_CHANNEL_V) && in->format != VX_DF_IMAGE_RGB) {
+        const uint32_t d = in->format == VX_DF_IMAGE_YUV4 ? 1 : 2;
         return set_output(n, 2, in->width / d, in->height / d, VX_DF_IMAGE_U8);
     }
-    return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction: Y of YUV images, U and V of IYUV and YUV4");
+    return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction: Y, U and V of YUV images");
 }
 
 vx_status x_extract(_vx_node *n, hipStream_t st)
 {
     _vx_image *in = img(n, 0), *o = img(n, 2);
     const vx_enum c = scl(n, 1)->value.e;
+    // NV12 / NV21 chroma: CHANNEL_EXTRACT_U8_U16_POS0 / _POS1 of the interleaved plane (ago_drama_divide.cpp:342-414)
+    if ((in->format == VX_DF_IMAGE_NV12 || in->format == VX_DF_IMAGE_NV21) && c != VX_CHANNEL_Y && c != VX_CHANNEL_0) {
+        const uint32_t pos = (c == VX_CHANNEL_U) == (in->format == VX_DF_IMAGE_NV12) ? 0u : 1u;
+        return rc(n, mvx_ChannelExtract_U8_U16(st, o->width, o->height, P(o), S(o), P(in, 1), S(in, 1), pos));
+    }
     const int plane = c == VX_CHANNEL_U ? 1 : c == VX_CHANNEL_V ? 2 : 0;
     return rc(n, mvx_ChannelCopy_U8_U8(st, o->width, o->height, P(o), S(o), P(in, plane), S(in, plane)));
 }

@@ -49,6 +49,16 @@ def test_yuv_nodes(gpu, orc):
             out = vx.create_image(ctx, want.shape[1], want.shape[0], "U8")
             assert L.vxuChannelExtract(ctx, iyuv, ch, out) == 0
             assert_same(vx.read_image(out), want, f"channel_extract {ch:#x}")
+        # NV12: the interleaved chroma plane, U at byte 0 and V at byte 1 of each pair
+        nv12 = vx.create_image(ctx, w, h, "NV12")
+        vx.write_image(nv12, y, plane=0)
+        vx.write_image(nv12, np.stack([u, v], -1).reshape(h // 2, w // 2, 2), plane=1, sub=2)
+        assert L.vxuColorConvert(ctx, nv12, rgb) == 0
+        assert_same(vx.read_image(rgb), orc.color_rgb_yuv(y, np.stack([u, v], -1).reshape(h // 2, w)), "vxuColorConvert NV12->RGB")
+        for ch, want in ((vx.VX_CHANNEL_U, u), (vx.VX_CHANNEL_V, v)):
+            out = vx.create_image(ctx, want.shape[1], want.shape[0], "U8")
+            assert L.vxuChannelExtract(ctx, nv12, ch, out) == 0
+            assert_same(vx.read_image(out), want, f"NV12 channel_extract {ch:#x}")
     finally:
         cc = ctypes.c_void_p(ctx)
         L.vxReleaseContext(ctypes.byref(cc))
