@@ -96,6 +96,10 @@ int mvx_ChannelCopy_U8_U8(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHe
  * _Pos1 (ago_haf_cpu_ch_extract_combine.cpp:663-760): byte `pos` (0 or 1) of each 2-byte pixel. */
 int mvx_ChannelExtract_U8_U16(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
                               const uint8_t *pSrc, uint32_t srcStride, uint32_t pos);
+/* replaces HipExec_ChannelExtract_U8_U24_Pos0 / _Pos1 / _Pos2 (hip_host_decls.h:421, :425, :429);
+ * HafCpu_ChannelExtract_U8_U24_Pos0..2 (ago_haf_cpu_ch_extract_combine.cpp:759 on): byte `pos` (0..2) of each RGB pixel. */
+int mvx_ChannelExtract_U8_U24(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pSrc, uint32_t srcStride, uint32_t pos);
 
 /* replace HipExec_ColorConvert_RGB_IYUV / _RGB_NV12 (hip_host_decls.h:523, :529); HafCpu_ColorConvert_RGB_IYUV /
  * _RGB_NV12 (ago_haf_cpu_color_convert.cpp:472-768): BT.709 with one chroma sample per 2x2 block, truncated

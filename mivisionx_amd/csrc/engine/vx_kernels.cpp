@@ -213,9 +213,15 @@ vx_status v_extract(_vx_node *n)
     _vx_image *in = img(n, 0);
     _vx_scalar *ch = scl(n, 1);
     if (!in || !ch) return fail(n, VX_ERROR_INVALID_PARAMETERS, "arguments");
-    if (in->format != VX_DF_IMAGE_IYUV && in->format != VX_DF_IMAGE_NV12 && in->format != VX_DF_IMAGE_NV21 && in->format != VX_DF_IMAGE_YUV4)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction is provided for the Y channel of YUV images");
     const vx_enum c = ch->value.e;
+    // RGB -> R / G / B: CHANNEL_EXTRACT_U8_U24_POS0..2
+    if (in->format == VX_DF_IMAGE_RGB) {
+        if (c != VX_CHANNEL_R && c != VX_CHANNEL_G && c != VX_CHANNEL_B && c != VX_CHANNEL_0 && c != VX_CHANNEL_1 && c != VX_CHANNEL_2)
+            return fail(n, VX_ERROR_INVALID_VALUE, "RGB channels are R, G, B");
+        return set_output(n, 2, in->width, in->height, VX_DF_IMAGE_U8);
+    }
+    if (in->format != VX_DF_IMAGE_IYUV && in->format != VX_DF_IMAGE_NV12 && in->format != VX_DF_IMAGE_NV21 && in->format != VX_DF_IMAGE_YUV4)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "channel extraction is provided for RGB and YUV images");
     if (c == VX_CHANNEL_Y || c == VX_CHANNEL_0) return set_output(n, 2, in->width, in->height, VX_DF_IMAGE_U8);
     if ((c == VX_CHANNEL_U || c == VX_CHANNEL_V) && in->format != VX_DF_IMAGE_RGB) {
         const uint32_t d = in->format == VX_DF_IMAGE_YUV4 ? 1 : 2;
@@ -228,6 +234,10 @@ vx_status x_extract(_vx_node *n, hipStream_t st)
 {
     _vx_image *in = img(n, 0), *o = img(n, 2);
     const vx_enum c = scl(n, 1)->value.e;
+    if (in->format == VX_DF_IMAGE_RGB) {
+        const uint32_t pos = (c == VX_CHANNEL_R || c == VX_CHANNEL_0) ? 0u : (c == VX_CHANNEL_G || c == VX_CHANNEL_1) ? 1u : 2u;
+        return rc(n, mvx_ChannelExtract_U8_U24(st, o->width, o->height, P(o), S(o), P(in), S(in), pos));
+    }
     // NV12 / NV21 chroma: CHANNEL_EXTRACT_U8_U16_POS0 / _POS1 of the interleaved plane (ago_drama_divide.cpp:342-414)
     if ((in->format == VX_DF_IMAGE_NV12 || in->format == VX_DF_IMAGE_NV21) && c != VX_CHANNEL_Y && c != VX_CHANNEL_0) {
         const uint32_t pos = (c == VX_CHANNEL_U) == (in->format == VX_DF_IMAGE_NV12) ? 0u : 1u;

@@ -308,19 +308,19 @@ k_gaussian_half_3x3(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_
     }
 }
 
-// ------------------------------------------------------------------ ChannelExtract U8 <- U16 ------
-// HafCpu_ChannelExtract_U8_U16_Pos0 / _Pos1 (ago_haf_cpu_ch_extract_combine.cpp:663-760): byte `pos` of
-// every 2-byte pixel (the U or V samples of an NV12 / NV21 chroma plane).  One thread: 16 outputs.
+// ------------------------------------------------------------------ ChannelExtract U8 <- U16 / U24 ---
+// HafCpu_ChannelExtract_U8_U16_Pos0/1 and _U8_U24_Pos0/1/2 (ago_haf_cpu_ch_extract_combine.cpp:663 on):
+// byte `pos` of every `bpp`-byte pixel (NV12 / NV21 chroma samples, RGB channels).  One thread: 16 outputs.
 __global__ void __launch_bounds__(256)
-k_extract_u8_u16(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
-                 uint32_t srcStride, uint32_t pos)
+k_extract_u8(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
+             uint32_t srcStride, uint32_t bpp, uint32_t pos)
 {
     const uint32_t x0 = (blockIdx.x * 256 + threadIdx.x) * 16, y = blockIdx.y;
     if (x0 >= w || y >= h) return;
-    const uint8_t *s = src + (size_t)y * srcStride + 2 * x0 + pos;
+    const uint8_t *s = src + (size_t)y * srcStride + (size_t)bpp * x0 + pos;
     uint8_t *d = dst + (size_t)y * dstStride + x0;
     const uint32_t n = min(16u, w - x0);
-    for (uint32_t k = 0; k < n; k++) d[k] = s[2 * k];
+    for (uint32_t k = 0; k < n; k++) d[k] = s[bpp * k];
 }
 
 // ------------------------------------------------------------------ ColorConvert RGB <- YUV ------
@@ -387,7 +387,18 @@ extern "C" int mvx_ChannelExtract_U8_U16(mvx_stream_t stream, uint32_t w, uint32
     if (w == 0 || h == 0) return MVX_SUCCESS;
     if (!dst || !src || pos > 1 || dstStride < w || srcStride < 2 * w) return MVX_ERROR_INVALID_PARAMETERS;
     dim3 grid(((w + 15) / 16 + 255) / 256, h);
-    hipLaunchKernelGGL(k_extract_u8_u16, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride, pos);
+    hipLaunchKernelGGL(k_extract_u8, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride, 2u, pos);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_ChannelExtract_U8_U24(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *dst, uint32_t dstStride, const uint8_t *src,
+                                         uint32_t srcStride, uint32_t pos)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!dst || !src || pos > 2 || dstStride < w || srcStride < 3 * w) return MVX_ERROR_INVALID_PARAMETERS;
+    dim3 grid(((w + 15) / 16 + 255) / 256, h);
+    hipLaunchKernelGGL(k_extract_u8, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride, 3u, pos);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }

@@ -55,6 +55,9 @@ VX_MEMORY_TYPE_HIP = _enum(0, 0x0E) + 3
 VX_INTERPOLATION_NEAREST_NEIGHBOR = _enum(0, 0x04) + 0
 VX_INTERPOLATION_BILINEAR = _enum(0, 0x04) + 1
 VX_INTERPOLATION_AREA = _enum(0, 0x04) + 2
+VX_CHANNEL_R = _enum(0, 0x09) + 0x10
+VX_CHANNEL_G = _enum(0, 0x09) + 0x11
+VX_CHANNEL_B = _enum(0, 0x09) + 0x12
 VX_CHANNEL_Y = _enum(0, 0x09) + 0x14
 VX_CHANNEL_U = _enum(0, 0x09) + 0x15
 VX_CHANNEL_V = _enum(0, 0x09) + 0x16

@@ -49,6 +49,11 @@ def test_yuv_nodes(gpu, orc):
             out = vx.create_image(ctx, want.shape[1], want.shape[0], "U8")
             assert L.vxuChannelExtract(ctx, iyuv, ch, out) == 0
             assert_same(vx.read_image(out), want, f"channel_extract {ch:#x}")
+        # RGB channels
+        for ch, k in ((vx.VX_CHANNEL_R, 0), (vx.VX_CHANNEL_G, 1), (vx.VX_CHANNEL_B, 2)):
+            out = vx.create_image(ctx, w, h, "U8")
+            assert L.vxuChannelExtract(ctx, rgb, ch, out) == 0
+            assert_same(vx.read_image(out), vx.read_image(rgb)[..., k], f"RGB channel_extract {k}")
         # NV12: the interleaved chroma plane, U at byte 0 and V at byte 1 of each pair
         nv12 = vx.create_image(ctx, w, h, "NV12")
         vx.write_image(nv12, y, plane=0)

@@ -30,6 +30,8 @@ NODE_SET = [
     "vision/FastCorners_XY_U8_NoSupression.gdf", "vision/FastCorners_XY_U8_Supression.gdf", "vision/Harris_5x5.gdf",
     "vision/Harris_7x7.gdf", "color/ColorConvert_RGB_IYUV.gdf", "color/ColorConvert_RGB_NV12.gdf", "vision_profile/11_colorConvert.gdf",
     "vision_profile/17_fastCorners.gdf", "vision_profile/29_opticalFlowLK.gdf", "vision_profile/33_remap.gdf",
+    "color/ChannelExtract_U8_U24_Pos0.gdf", "color/ChannelExtract_U8_U24_Pos1.gdf", "color/ChannelExtract_U8_U24_Pos2.gdf",
+    "color/ChannelExtract_U8_U16_Pos0.gdf", "color/ChannelExtract_U8_U16_Pos1.gdf", "vision_profile/10_channelExtract.gdf",
 ]
 
 
