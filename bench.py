@@ -128,6 +128,28 @@ def measure_nodes(reps=10):
     call("Remap_FixedMap_F32", W, H, ctypes.c_void_p(fixed.data_ptr()), W * 4, ctypes.c_void_p(fmap.data_ptr()), W * 8, W, H)
     ms = timed(lambda: call("Remap_U8_U8_Bilinear", W, H, dp, dp.stride, W, H, sp, sp.stride, ctypes.c_void_p(fixed.data_ptr()), W * 4))
     out["remap_3840x2160"] = bw("remap", ms, W * H, 6, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    # the further kernel families at 4K
+    ms = timed(lambda: call("WarpPerspective_U8_U8_Nearest_Constant", W, H, dp, dp.stride, W, H, sp, sp.stride,
+                            per_c.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint8(0)))
+    out["warp_perspective_nearest_3840x2160"] = bw("warp_perspective_nearest", ms, W * H, 2, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    u16 = Plane(W, H, "U16")
+    ms = timed(lambda: call("CannySobel_U16_U8_5x5_L2NORM", W, H, u16, u16.stride, sp, sp.stride))
+    out["canny_sobel_5x5_l2_3840x2160"] = bw("canny_sobel_5x5_l2", ms, W * H, 3, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    K = __import__("mivisionx_amd").kernels()
+    K.mvx_FastCorners_workspace_size.restype = ctypes.c_size_t
+    fws = int(K.mvx_FastCorners_workspace_size(W, H))
+    fbuf = torch.zeros(fws + 20000 * 28 + 64, dtype=torch.uint8, device="cuda")
+    fp = fbuf.data_ptr()
+    ms = timed(lambda: call("FastCorners_XY_U8_Supression", 20000, ctypes.c_void_p(fp + fws), ctypes.c_void_p(fp + fws + 20000 * 28),
+                            None, W, H, sp, sp.stride, 20.0, ctypes.c_void_p(fp), ctypes.c_size_t(fws)))
+    out["fast_corners_nonmax_3840x2160"] = bw("fast_corners", ms, W * H, 2, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    yp, up, vp_, rgbp = Plane(W, H), Plane(W // 2, H // 2), Plane(W // 2, H // 2), Plane(W, H, "RGB")
+    ms = timed(lambda: call("ColorConvert_RGB_IYUV", W, H, rgbp, rgbp.stride, yp, yp.stride, up, up.stride, vp_, vp_.stride))
+    out["color_rgb_from_iyuv_3840x2160"] = bw("rgb_from_iyuv", ms, W * H, 4.5, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    rgbs = Plane(W, H, "RGB")
+    ms = timed(lambda: call("Remap_U24_U24_Bilinear", W, H, rgbp, rgbp.stride, W, H, rgbs, rgbs.stride, ctypes.c_void_p(fixed.data_ptr()),
+                            W * 4))
+    out["remap_rgb_3840x2160"] = bw("remap_rgb", ms, W * H, 10, W * H / (ms / 1e3) / 1e6, "Mpix/s")
     # config 3: pyramidal LK, 10,000 keypoints
     orc = Oracle()
     old, new, _ = synth.lk_pair(W, H, 1)
