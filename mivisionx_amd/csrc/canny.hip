@@ -211,19 +211,14 @@ k_canny_sobel_g(uint32_t w, uint32_t h, uint16_t *__restrict__ dst, uint32_t dst
     for (int c = (int)threadIdx.x; c < 256 + 2 * R; c += 256) {
         const int col = x0 - R + c;
         int v0 = 0, v1 = 0, t0, t1;
-        // r0[col]
-        if (col >= 0 && col < aligned) {
+        if (col >= 0 && col < aligned) {               // r0[col], r1[col]: this column's two sums
             canny_vertical<G>(src, srcStride, col, y, t0, t1);
             v0 = t0;
-        } else if (col >= a16 && col - a16 < aligned) {
+            v1 = t1;
+        } else if (col >= a16 && col - a16 < aligned) { // past the end of r0: the start of r1
             canny_vertical<G>(src, srcStride, col - a16, y, t0, t1);
             v0 = t1;
-        }
-        // r1[col]
-        if (col >= 0 && col < aligned) {
-            canny_vertical<G>(src, srcStride, col, y, t0, t1);
-            v1 = t1;
-        } else if (col < 0 && a16 + col < aligned) {
+        } else if (col < 0 && a16 + col < aligned) {    // before the start of r1: the end of r0
             canny_vertical<G>(src, srcStride, a16 + col, y, t0, t1);
             v1 = t0;
         }
