@@ -67,23 +67,32 @@ MVX_DEV bool is_corner(const Circle &k, int t)
     return !early_exit(k, t) && (nine_run(plus_mask(k, t)) || nine_run(minus_mask(k, t)));
 }
 
-// checkForCornerAndGetStrength (:216-412): per sign with a nine-run, the largest threshold in [t, 254]
-// that still passes (bisection between t and 255, the early exit repeated at each probe); the strength
-// is the larger of the two, 0 when neither sign has a run (a map value of 0 is "no corner")
+// checkForCornerAndGetStrength (:216-412): per sign with a nine-run at t, the largest threshold m in
+// [t, 254] for which the corner test still passes (the reference bisects between t and 255, repeating the
+// early exit at each probe); 0 for a sign without a run; the strength is the larger of the two, and a
+// map value of 0 means no corner.  The test at m passes iff some nine-pixel arc has every difference
+// d_i (b_i - c, or c - b_i) above m, i.e. m < R = max over arcs of min d_i, and the early exit does not
+// fire, i.e. m <= E = min(max(|c-b0|, |c-b8|), max(|c-b4|, |c-b12|)).  Both conditions only get harder as
+// m grows, so the bisection's answer is min(R - 1, E, 254), computed here directly.
 MVX_DEV int strength(const Circle &k, int t)
 {
     if (early_exit(k, t)) return 0;
+    const int E = min(max(abs(k.c - k.b[0]), abs(k.c - k.b[8])), max(abs(k.c - k.b[4]), abs(k.c - k.b[12])));
     int best = 0;
 #pragma unroll
     for (int sign = 0; sign < 2; sign++) {
-        if (!nine_run(sign ? minus_mask(k, t) : plus_mask(k, t))) continue;
-        int lo = t, hi = 255;
-        while (hi - lo > 1) {
-            const int mid = (hi + lo) >> 1;
-            if (!early_exit(k, mid) && nine_run(sign ? minus_mask(k, mid) : plus_mask(k, mid))) lo = mid;
-            else hi = mid;
+        int d[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) d[i] = sign ? k.c - k.b[i] : k.b[i] - k.c;
+        int R = -1024;
+#pragma unroll
+        for (int s0 = 0; s0 < 16; s0++) {
+            int m = d[s0];
+#pragma unroll
+            for (int j = 1; j < 9; j++) m = min(m, d[(s0 + j) & 15]);
+            R = max(R, m);
         }
-        best = max(best, lo);
+        if (R > t) best = max(best, min(min(R - 1, E), 254));
     }
     return best;
 }
