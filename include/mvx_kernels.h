@@ -110,6 +110,14 @@ int mvx_ColorConvert_RGB_IYUV(mvx_stream_t stream, uint32_t dstWidth, uint32_t d
 int mvx_ColorConvert_RGB_NV12(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
                               const uint8_t *pY, uint32_t yStride, const uint8_t *pUV, uint32_t uvStride);
 
+/* replaces HafCpu_ColorConvert_IYUV_RGB (ago_haf_cpu_color_convert.cpp:2492-2705; AGO kernel
+ * COLOR_CONVERT_IYUV_RGB, ago_kernel_api.cpp:13447) and HipExec_ColorConvert_IYUV_RGB (hip_host_decls.h:565):
+ * all three planes of IYUV from packed RGB, BT.709, chroma averaged over each 2x2 block with the reference's
+ * rounding.  Width and height must be even (-15 otherwise); U and V are (w/2) x (h/2). */
+int mvx_ColorConvert_IYUV_RGB(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstY, uint32_t yStride,
+                              uint8_t *pDstU, uint32_t uStride, uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrc,
+                              uint32_t srcStride);
+
 /* replaces HipExec_ScaleImage_U8_U8_Bilinear (hip_host_decls.h:728); HafCpu_ScaleImage_U8_U8_Bilinear
  * (ago_haf_cpu_geometric.cpp:1977-2105) with the scale matrix of ago_kernel_api.cpp:19897-19916.
  * dstStride must be a multiple of 16 (8-byte stores). */

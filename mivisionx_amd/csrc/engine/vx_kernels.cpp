@@ -152,11 +152,12 @@ template <int IO> vx_status x_phase(_vx_node *n, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------- ColorConvert (A2) --------
-// RGB -> IYUV: the luma plane through ColorConvert_Y_RGB.  The reference keeps only this kernel when
-// the IYUV image is virtual and only its Y channel is read (ago_drama_divide.cpp:244-256 and the
-// unused-output removal, ago_drama_remove.cpp:989); chroma planes are not produced here.
-// IYUV / NV12 -> RGB: COLOR_CONVERT_RGB_IYUV / _RGB_NV12 (ago_drama_divide.cpp:80-113).  (The reference
-// CPU has no RGB -> IYUV chroma kernels -- ago_haf_cpu.cpp:26-77 return NOT_IMPLEMENTED.)
+// RGB -> IYUV: a non-virtual IYUV output gets all three planes (COLOR_CONVERT_IYUV_RGB,
+// ago_kernel_api.cpp:13447; HafCpu_ColorConvert_IYUV_RGB).  A virtual IYUV image read only through its Y
+// channel keeps only the luma kernel, ColorConvert_Y_RGB, as the reference's graph optimizer does
+// (ago_drama_divide.cpp:244-256 and the unused-output removal, ago_drama_remove.cpp:989).  RGB -> NV12 /
+// YUV4 chroma planes are not produced.
+// IYUV / NV12 -> RGB: COLOR_CONVERT_RGB_IYUV / _RGB_NV12 (ago_drama_divide.cpp:80-113).
 vx_status v_color(_vx_node *n)
 {
     _vx_image *src = img(n, 0);
@@ -176,9 +177,11 @@ vx_status v_color(_vx_node *n)
     if (want != VX_DF_IMAGE_IYUV && want != VX_DF_IMAGE_NV12 && want != VX_DF_IMAGE_YUV4)
         return fail(n, VX_ERROR_NOT_SUPPORTED, "RGB converts to the luma plane of IYUV / NV12 / YUV4 only");
     if ((s = set_output(n, 1, img(n, 0)->width, img(n, 0)->height, want))) return s;
-    if (!o->isVirtual)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "chroma planes are not produced: the YUV output must be a virtual image read through "
-                                               "channel_extract(Y)");
+    if (!o->isVirtual && want != VX_DF_IMAGE_IYUV)
+        return fail(n, VX_ERROR_NOT_SUPPORTED, "RGB -> NV12 / YUV4 chroma planes are not produced: the output must be IYUV, or a "
+                                               "virtual image read through channel_extract(Y)");
+    if (!o->isVirtual && ((o->width & 1u) || (o->height & 1u)))
+        return fail(n, VX_ERROR_INVALID_DIMENSION, "YUV 4:2:0 sizes must be even");
     return VX_SUCCESS;
 }
 
@@ -189,6 +192,9 @@ vx_status x_color(_vx_node *n, hipStream_t st)
         return rc(n, mvx_ColorConvert_RGB_IYUV(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1), P(in, 2), S(in, 2)));
     if (in->format == VX_DF_IMAGE_NV12)
         return rc(n, mvx_ColorConvert_RGB_NV12(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1)));
+    if (!o->isVirtual && o->format == VX_DF_IMAGE_IYUV)
+        return rc(n, mvx_ColorConvert_IYUV_RGB(st, o->width, o->height, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in),
+                                               S(in)));
     return rc(n, mvx_ColorConvert_Y_RGB(st, o->width, o->height, P(o), S(o), P(in), S(in)));
 }
 

@@ -50,6 +50,79 @@ k_color_y_rgb(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStr
     st16_partial(dst + (long)y * dstStride + x0, o, nv);
 }
 
+// ------------------------------------------------------------------ ColorConvert IYUV <- RGB ---
+// HafCpu_ColorConvert_IYUV_RGB (ago_haf_cpu_color_convert.cpp:2492-2705).  One thread per 4 columns of a
+// row pair.  Columns below w & ~3 follow the SSE body: per pixel, in float (explicit round-to-nearest
+// operations, no contraction), Y = ((R*0.2126f) + G*0.7152f) + B*0.0722f, U = (((R*-0.1146f) +
+// G*-0.3854f) + B*0.5f) + 128, V = (((R*0.5f) + G*-0.4542f) + B*-0.0458f) + 128, each truncated; each
+// chroma sample averages vertically with rounding, then horizontally with rounding.  The last two columns
+// when w % 4 == 2 follow the scalar tail: Y adds B*0.0722 in double, and U / V are the float sum of the four
+// values (row-major order) divided by 4, truncated.
+MVX_DEV float yuv_dot(float r, float g, float b, float wr, float wg, float wb)
+{
+    return __fadd_rn(__fadd_rn(__fmul_rn(r, wr), __fmul_rn(g, wg)), __fmul_rn(b, wb));
+}
+
+__global__ void __launch_bounds__(256)
+k_iyuv_from_rgb(uint32_t w, uint32_t h, uint8_t *__restrict__ dy, uint32_t ys, uint8_t *__restrict__ du, uint32_t us,
+                uint8_t *__restrict__ dv, uint32_t vs, const uint8_t *__restrict__ rgb, uint32_t rs)
+{
+    const uint32_t x0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    const uint32_t r = blockIdx.y * 2;
+    if (x0 >= w || r >= h) return;
+    const bool body = x0 < (w & ~3u);
+    const int n = body ? 4 : 2;
+    float R[2][4], G[2][4], B[2][4];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint8_t *p = rgb + (long)(r + q) * rs + 3L * (x0 + (i < n ? i : 0));
+            R[q][i] = p[0];
+            G[q][i] = p[1];
+            B[q][i] = p[2];
+        }
+    if (body) {
+        int uu[2][4], vv[2][4];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            uint32_t yw = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int yi = min(max((int)yuv_dot(R[q][i], G[q][i], B[q][i], 0.2126f, 0.7152f, 0.0722f), 0), 255);
+                yw |= (uint32_t)yi << (8 * i);
+                uu[q][i] = min(max((int)__fadd_rn(yuv_dot(R[q][i], G[q][i], B[q][i], -0.1146f, -0.3854f, 0.5f), 128.0f), 0), 65535);
+                vv[q][i] = min(max((int)__fadd_rn(yuv_dot(R[q][i], G[q][i], B[q][i], 0.5f, -0.4542f, -0.0458f), 128.0f), 0), 65535);
+            }
+            uint8_t *py = dy + (long)(r + q) * ys + x0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) py[i] = (uint8_t)(yw >> (8 * i));
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int uo = (((uu[0][2 * j] + uu[1][2 * j] + 1) >> 1) + ((uu[0][2 * j + 1] + uu[1][2 * j + 1] + 1) >> 1) + 1) >> 1;
+            const int vo = (((vv[0][2 * j] + vv[1][2 * j] + 1) >> 1) + ((vv[0][2 * j + 1] + vv[1][2 * j + 1] + 1) >> 1) + 1) >> 1;
+            du[(long)blockIdx.y * us + (x0 >> 1) + j] = (uint8_t)min(uo, 255);
+            dv[(long)blockIdx.y * vs + (x0 >> 1) + j] = (uint8_t)min(vo, 255);
+        }
+    } else {
+        float U = 0.0f, V = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int q = k >> 1, i = k & 1;
+            const float rg = __fadd_rn(__fmul_rn(R[q][i], 0.2126f), __fmul_rn(G[q][i], 0.7152f));
+            const double yd = __dadd_rn((double)rg, __dmul_rn((double)B[q][i], 0.0722));
+            dy[(long)(r + q) * ys + x0 + i] = (uint8_t)(int)yd;
+            const float tu = __fadd_rn(yuv_dot(R[q][i], G[q][i], B[q][i], -0.1146f, -0.3854f, 0.5f), 128.0f);
+            const float tv = __fadd_rn(yuv_dot(R[q][i], G[q][i], B[q][i], 0.5f, -0.4542f, -0.0458f), 128.0f);
+            U = k ? __fadd_rn(U, tu) : tu;
+            V = k ? __fadd_rn(V, tv) : tv;
+        }
+        du[(long)blockIdx.y * us + (x0 >> 1)] = (uint8_t)(int)__fmul_rn(U, 0.25f);
+        dv[(long)blockIdx.y * vs + (x0 >> 1)] = (uint8_t)(int)__fmul_rn(V, 0.25f);
+    }
+}
+
 // ------------------------------------------------------------------ ChannelCopy ----------------
 __global__ void __launch_bounds__(256)
 k_copy_u8(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
@@ -422,6 +495,21 @@ extern "C" int mvx_ColorConvert_Y_RGB(mvx_stream_t stream, uint32_t w, uint32_t 
     if (!plane_ok_g(dst, w, dstStride, 1) || !plane_ok_g(rgb, w, rgbStride, 3)) return MVX_ERROR_INVALID_PARAMETERS;
     dim3 grid(((w + 15) / 16 + 255) / 256, h);
     hipLaunchKernelGGL(k_color_y_rgb, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, rgb, rgbStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_ColorConvert_IYUV_RGB(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *dstY, uint32_t yStride, uint8_t *dstU,
+                                         uint32_t uStride, uint8_t *dstV, uint32_t vStride, const uint8_t *rgb, uint32_t rgbStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if ((w & 1u) || (h & 1u)) return -15;
+    if (!plane_ok_g(dstY, w, yStride, 1) || !plane_ok_g(dstU, w / 2, uStride, 1) || !plane_ok_g(dstV, w / 2, vStride, 1) ||
+        !plane_ok_g(rgb, w, rgbStride, 3))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    dim3 grid(((w + 3) / 4 + 255) / 256, h / 2);
+    hipLaunchKernelGGL(k_iyuv_from_rgb, grid, dim3(256), 0, stream_of(stream), w, h, dstY, yStride, dstU, uStride, dstV, vStride, rgb,
+                       rgbStride);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }

@@ -325,3 +325,56 @@ int oracle_scale_gaussian_half_5x5(int dw, int dh, uint8_t *dst, int dst_stride,
     }
     return 0;
 }
+
+/* ColorConvert RGB -> IYUV: HafCpu_ColorConvert_IYUV_RGB (ago_haf_cpu_color_convert.cpp:2492-2705), per
+ * 2x2 block.  Columns below w & ~3 follow the SSE body: per pixel, in float, Y = ((R*0.2126f) + G*0.7152f)
+ * + B*0.0722f, U = (((R*-0.1146f) + G*-0.3854f) + B*0.5f) + 128, V = (((R*0.5f) + G*-0.4542f) +
+ * B*-0.0458f) + 128, each truncated; chroma = ((u00+u10+1)>>1 + (u01+u11+1)>>1 + 1) >> 1 (a vertical
+ * rounding average, then a horizontal one).  The last two columns when w % 4 == 2 follow the scalar tail:
+ * Y adds B*0.0722 in double, and U / V are the float sum of the four pixel values (row-major order)
+ * divided by 4, truncated. */
+static float yuv_u_px(float r, float g, float b) { return ((r * -0.1146f) + (g * -0.3854f)) + (b * 0.5f) + 128.0f; }
+static float yuv_v_px(float r, float g, float b) { return ((r * 0.5f) + (g * -0.4542f)) + (b * -0.0458f) + 128.0f; }
+static int sat_trunc(float f, int hi) { int i = (int)f; return i < 0 ? 0 : (i > hi ? hi : i); }
+
+void oracle_color_iyuv_from_rgb(int w, int h, const uint8_t *rgb, int rgbStride, uint8_t *y, int yStride, uint8_t *u,
+                                int uStride, uint8_t *v, int vStride)
+{
+    const int aligned = w & ~3;
+    for (int r = 0; r < h; r += 2)
+        for (int x = 0; x < w; x += 2) {
+            float R[4], G[4], B[4];
+            for (int k = 0; k < 4; k++) {
+                const uint8_t *p = rgb + (long)(r + (k >> 1)) * rgbStride + 3L * (x + (k & 1));
+                R[k] = p[0]; G[k] = p[1]; B[k] = p[2];
+            }
+            int uo, vo;
+            if (x < aligned) {
+                int uu[4], vv[4];
+                for (int k = 0; k < 4; k++) {
+                    const float yf = ((R[k] * 0.2126f) + (G[k] * 0.7152f)) + (B[k] * 0.0722f);
+                    y[(long)(r + (k >> 1)) * yStride + x + (k & 1)] = (uint8_t)sat_trunc(yf, 255);
+                    uu[k] = sat_trunc(yuv_u_px(R[k], G[k], B[k]), 65535);
+                    vv[k] = sat_trunc(yuv_v_px(R[k], G[k], B[k]), 65535);
+                }
+                uo = ((((uu[0] + uu[2] + 1) >> 1) + ((uu[1] + uu[3] + 1) >> 1)) + 1) >> 1;
+                vo = ((((vv[0] + vv[2] + 1) >> 1) + ((vv[1] + vv[3] + 1) >> 1)) + 1) >> 1;
+                if (uo > 255) uo = 255;
+                if (vo > 255) vo = 255;
+            } else {
+                float U = 0.0f, V = 0.0f;
+                for (int k = 0; k < 4; k++) {
+                    const double yd = (double)((R[k] * 0.2126f) + (G[k] * 0.7152f)) + (double)B[k] * 0.0722;
+                    y[(long)(r + (k >> 1)) * yStride + x + (k & 1)] = (uint8_t)yd;
+                    U = k ? U + yuv_u_px(R[k], G[k], B[k]) : yuv_u_px(R[k], G[k], B[k]);
+                    V = k ? V + yuv_v_px(R[k], G[k], B[k]) : yuv_v_px(R[k], G[k], B[k]);
+                }
+                U /= 4.0f;
+                V /= 4.0f;
+                uo = (uint8_t)U;
+                vo = (uint8_t)V;
+            }
+            u[(long)(r >> 1) * uStride + (x >> 1)] = (uint8_t)uo;
+            v[(long)(r >> 1) * vStride + (x >> 1)] = (uint8_t)vo;
+        }
+}

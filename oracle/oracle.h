@@ -130,6 +130,8 @@ int oracle_harris_node_ex(int w, int h, const uint8_t *src, int g, int b, float 
 int oracle_harris_node(int w, int h, const uint8_t *src, float k, float thr, float min_distance,
                        oracle_keypoint_t *out, int capacity, float *hvc);
 
+void oracle_color_iyuv_from_rgb(int w, int h, const uint8_t *rgb, int rgbStride, uint8_t *y, int yStride, uint8_t *u,
+                                int uStride, uint8_t *v, int vStride);
 #ifdef __cplusplus
 }
 #endif

@@ -188,6 +188,15 @@ class Oracle:
         self.lib.oracle_color_rgb_yuv(w, h, _p(d), 3 * w, _p(_c(y)), w, _p(_c(u)), u.shape[1], _p(vv), vv.shape[1], int(nv12))
         return d
 
+    def color_iyuv_from_rgb(self, rgb):
+        """color_convert RGB -> IYUV, all three planes (ago_haf_cpu_color_convert.cpp:2492-2705)"""
+        h, w = rgb.shape[:2]
+        y = np.zeros((h, w), np.uint8)
+        u = np.zeros((h // 2, w // 2), np.uint8)
+        v = np.zeros((h // 2, w // 2), np.uint8)
+        self.lib.oracle_color_iyuv_from_rgb(w, h, _p(_c(rgb)), 3 * w, _p(y), w, _p(u), w // 2, _p(v), w // 2)
+        return y, u, v
+
     def halfscale(self, src, ksize):
         """halfscale_gaussian (ago_drama_divide.cpp:1471-1488): kernel size 1 -> ScaleImage nearest,
         3 -> ScaleGaussianHalf 3x3, 5 -> ScaleGaussianHalf 5x5, output (w+1)/2 x (h+1)/2 on
@@ -505,6 +514,14 @@ class Reference:
         vv = _c(v) if v is not None else _c(u)
         self._ok(self.lib.ref_color_rgb_yuv(w, h, _p(_c(y)), _p(_c(u)), _p(vv), int(v is None), _p(d)), "color_rgb_yuv")
         return d
+
+    def color_iyuv_from_rgb(self, rgb):
+        h, w = rgb.shape[:2]
+        y = np.zeros((h, w), np.uint8)
+        u = np.zeros((h // 2, w // 2), np.uint8)
+        v = np.zeros((h // 2, w // 2), np.uint8)
+        self._ok(self.lib.ref_color_iyuv_from_rgb(w, h, _p(_c(rgb)), _p(y), _p(u), _p(v)), "color_iyuv_from_rgb")
+        return y, u, v
 
     def halfscale(self, src, ksize):
         sh, sw = src.shape

@@ -276,6 +276,26 @@ int ref_color_rgb_yuv(int w, int h, const uint8_t *y, const uint8_t *u, const ui
     return 0;
 }
 
+// RGB -> IYUV (non-virtual output, all three planes): ColorConvert_IYUV_RGB.
+int ref_color_iyuv_from_rgb(int w, int h, const uint8_t *rgb, uint8_t *y, uint8_t *u, uint8_t *v) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_RGB, rgb);
+    vx_image out = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_IYUV);
+    if (!vxColorConvertNode(c.graph, in, out)) return -1;
+    int s = run(c); if (s) return s;
+    vx_rectangle_t r = {0, 0, (vx_uint32)w, (vx_uint32)h};
+    vx_imagepatch_addressing_t a = {};
+    a.dim_x = w; a.dim_y = h; a.stride_x = 1; a.stride_y = w;
+    a.scale_x = VX_SCALE_UNITY; a.scale_y = VX_SCALE_UNITY; a.step_x = 1; a.step_y = 1;
+    CHECK(vxCopyImagePatch(out, &r, 0, &a, (void *)y, VX_READ_ONLY, VX_MEMORY_TYPE_HOST));
+    vx_imagepatch_addressing_t b = {};
+    b.dim_x = w; b.dim_y = h; b.stride_x = 1; b.stride_y = w / 2;
+    b.scale_x = VX_SCALE_UNITY; b.scale_y = VX_SCALE_UNITY; b.step_x = 2; b.step_y = 2;
+    CHECK(vxCopyImagePatch(out, &r, 1, &b, (void *)u, VX_READ_ONLY, VX_MEMORY_TYPE_HOST));
+    CHECK(vxCopyImagePatch(out, &r, 2, &b, (void *)v, VX_READ_ONLY, VX_MEMORY_TYPE_HOST));
+    return 0;
+}
+
 // halfscale_gaussian with kernel size 1, 3 or 5 (output (w+1)/2 x (h+1)/2).
 int ref_halfscale(int w, int h, const uint8_t *src, int ksize, uint8_t *dst) {
     Ctx c;
