@@ -66,6 +66,7 @@ typedef struct _vx_array *vx_array;
 typedef struct _vx_pyramid *vx_pyramid;
 typedef struct _vx_matrix *vx_matrix;
 typedef struct _vx_remap *vx_remap;
+typedef struct _vx_meta_format *vx_meta_format;
 
 /* ------------------------------------------------------------------ enumeration bases ----- */
 #define VX_ID_KHRONOS 0x000
@@ -78,6 +79,7 @@ typedef struct _vx_remap *vx_remap;
 #define VX_FMT_REF "%p"
 #define VX_FMT_SIZE "%zu"
 #define VX_SCALE_UNITY (1024u)
+#define VX_ID_USER 0xFFE
 #define VX_SCALE_PYRAMID_HALF (0.5f)
 #define VX_SCALE_PYRAMID_ORB ((vx_float32)0.8408964f)
 
@@ -338,9 +340,17 @@ enum vx_node_attribute_e {
     VX_NODE_PERFORMANCE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x1,
     VX_NODE_BORDER = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x2,
     VX_NODE_LOCAL_DATA_SIZE = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x3,
+    VX_NODE_LOCAL_DATA_PTR = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x4,
     VX_NODE_PARAMETERS = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_NODE) + 0x5,
 };
 #define VX_NODE_ATTRIBUTE_BORDER_MODE VX_NODE_BORDER
+#define VX_NODE_ATTRIBUTE_LOCAL_DATA_SIZE VX_NODE_LOCAL_DATA_SIZE
+#define VX_NODE_ATTRIBUTE_LOCAL_DATA_PTR VX_NODE_LOCAL_DATA_PTR
+#define VX_KERNEL_ATTRIBUTE_LOCAL_DATA_SIZE VX_KERNEL_LOCAL_DATA_SIZE
+
+enum vx_meta_format_attribute_e {
+    VX_VALID_RECT_CALLBACK = VX_ATTRIBUTE_BASE(VX_ID_KHRONOS, VX_TYPE_META_FORMAT) + 0x1,
+};
 #define VX_NODE_ATTRIBUTE_STATUS VX_NODE_STATUS
 
 enum vx_parameter_attribute_e {
@@ -499,6 +509,32 @@ VX_API_ENTRY vx_parameter VX_API_CALL vxGetParameterByIndex(vx_node node, vx_uin
 VX_API_ENTRY vx_status VX_API_CALL vxQueryParameter(vx_parameter parameter, vx_enum attribute, void *ptr, vx_size size);
 VX_API_ENTRY vx_status VX_API_CALL vxReleaseParameter(vx_parameter *param);
 VX_API_ENTRY vx_status VX_API_CALL vxSetParameterByIndex(vx_node node, vx_uint32 index, vx_reference value);
+
+/* ------------------------------------------------------------------ user kernels ---------- */
+/* The Khronos user-kernel interface (vx_types.h:1760-1848, vx_api.h), so that plug-in modules written
+ * against the reference load unchanged: vxLoadKernels dlopens lib<module>.so and calls its
+ * vxPublishKernels, which registers kernels with vxAddUserKernel / vxAddParameterToKernel /
+ * vxSetKernelAttribute / vxFinalizeKernel. */
+typedef vx_status(VX_API_CALL *vx_publish_kernels_f)(vx_context context);
+typedef vx_status(VX_API_CALL *vx_unpublish_kernels_f)(vx_context context);
+typedef vx_status(VX_CALLBACK *vx_kernel_f)(vx_node node, const vx_reference *parameters, vx_uint32 num);
+typedef vx_status(VX_CALLBACK *vx_kernel_initialize_f)(vx_node node, const vx_reference *parameters, vx_uint32 num);
+typedef vx_status(VX_CALLBACK *vx_kernel_deinitialize_f)(vx_node node, const vx_reference *parameters, vx_uint32 num);
+typedef vx_status(VX_CALLBACK *vx_kernel_validate_f)(vx_node node, const vx_reference parameters[], vx_uint32 num, vx_meta_format metas[]);
+typedef vx_status(VX_CALLBACK *vx_kernel_image_valid_rectangle_f)(vx_node node, vx_uint32 index, const vx_rectangle_t *const input_valid[],
+                                                                  vx_rectangle_t *const output_valid[]);
+VX_API_ENTRY vx_status VX_API_CALL vxUnloadKernels(vx_context context, const vx_char *module);
+VX_API_ENTRY vx_status VX_API_CALL vxAllocateUserKernelId(vx_context context, vx_enum *pKernelEnumId);
+VX_API_ENTRY vx_status VX_API_CALL vxAllocateUserKernelLibraryId(vx_context context, vx_enum *pLibraryId);
+VX_API_ENTRY vx_kernel VX_API_CALL vxAddUserKernel(vx_context context, const vx_char name[VX_MAX_KERNEL_NAME], vx_enum enumeration,
+                                                   vx_kernel_f func_ptr, vx_uint32 numParams, vx_kernel_validate_f validate,
+                                                   vx_kernel_initialize_f init, vx_kernel_deinitialize_f deinit);
+VX_API_ENTRY vx_status VX_API_CALL vxFinalizeKernel(vx_kernel kernel);
+VX_API_ENTRY vx_status VX_API_CALL vxAddParameterToKernel(vx_kernel kernel, vx_uint32 index, vx_enum dir, vx_enum data_type, vx_enum state);
+VX_API_ENTRY vx_status VX_API_CALL vxRemoveKernel(vx_kernel kernel);
+VX_API_ENTRY vx_status VX_API_CALL vxSetKernelAttribute(vx_kernel kernel, vx_enum attribute, const void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxSetMetaFormatAttribute(vx_meta_format meta, vx_enum attribute, const void *ptr, vx_size size);
+VX_API_ENTRY vx_status VX_API_CALL vxSetMetaFormatFromReference(vx_meta_format meta, vx_reference exemplar);
 
 typedef void(VX_CALLBACK *vx_log_callback_f)(vx_context context, vx_reference ref, vx_status status, const vx_char string[]);
 VX_API_ENTRY void VX_API_CALL vxRegisterLogCallback(vx_context context, vx_log_callback_f callback, vx_bool reentrant);

@@ -67,6 +67,25 @@ int mvx_Sobel_S16S16_U8_3x3_GXY(mvx_stream_t stream, uint32_t dstWidth, uint32_t
                                 int16_t *pDstGx, uint32_t dstGxStride, int16_t *pDstGy, uint32_t dstGyStride,
                                 const uint8_t *pSrc, uint32_t srcStride);
 
+/* Merged Sobel -> Magnitude / Phase: the kernels the reference's graph optimizer substitutes for
+ * Sobel_S16S16_U8_3x3_GXY + Magnitude_S16_S16S16 [+ Phase_U8_S16S16] when gx and gy are virtual and no
+ * other node reads them (merge rules ago_drama_remove.cpp:155-182).  CPU-only in the reference
+ * (ago_kernel_api.cpp:15958-16110 have no hip_execute branch), so there is no HipExec_* launcher to
+ * replace; the argument order follows the HafCpu_* functions (output planes first).  Rows 1..h-2 are
+ * written.  Magnitude: columns [0, w & ~15) round half to even, the last w mod 16 columns truncate. */
+/* HafCpu_SobelMagnitudePhase_S16U8_U8_3x3 (ago_haf_cpu_filter.cpp:3972); pPhase may be NULL */
+int mvx_SobelMagnitudePhase_S16U8_U8_3x3(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                         int16_t *pMag, uint32_t magStride, uint8_t *pPhase, uint32_t phaseStride,
+                                         const uint8_t *pSrc, uint32_t srcStride);
+/* HafCpu_SobelMagnitude_S16_U8_3x3 (ago_haf_cpu_filter.cpp:2468) */
+int mvx_SobelMagnitude_S16_U8_3x3(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                                  int16_t *pMag, uint32_t magStride, const uint8_t *pSrc, uint32_t srcStride);
+/* HafCpu_SobelPhase_U8_U8_3x3 (ago_haf_cpu_filter.cpp:3946): the reference's Gx/Gy scratch rows are
+ * ALIGN16(w) bytes apart for w int16 values, so pixel (x, y) takes the gradient of (x - ALIGN16(w)/2,
+ * y + 1) for x >= ALIGN16(w)/2 on every row but the last; reproduced without a scratch buffer. */
+int mvx_SobelPhase_U8_U8_3x3(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight,
+                             uint8_t *pPhase, uint32_t phaseStride, const uint8_t *pSrc, uint32_t srcStride);
+
 /* ------------------------------------------------------------------ arithmetic ------------- */
 /* replaces HipExec_Magnitude_S16_S16S16 (hip_host_decls.h:219); HafCpu_Magnitude_S16_S16S16
  * (ago_haf_cpu_arithmetic.cpp:4989-5097), all rows */
@@ -449,6 +468,16 @@ int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_params_t *para
  * stages 1 and 2; the caller's stream waits for them before the call's work counts as done. */
 int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes, void *workspace,
                         size_t workspaceSize);
+/* A lane: a default-priority side stream and its fork / join events, on which mvx_VisionGraph_run_lane
+ * runs the pyramid levels and the Harris pick beside the hysteresis stages.  mvx_VisionGraph_run uses one
+ * lane per device shared by all its callers; a caller that runs several graphs concurrently gives each
+ * its own lane.  Create and destroy outside stream capture. */
+typedef struct mvx_vision_lane mvx_vision_lane_t;
+int mvx_VisionGraph_lane_create(mvx_vision_lane_t **lane);
+int mvx_VisionGraph_lane_destroy(mvx_vision_lane_t *lane);
+/* mvx_VisionGraph_run with the side stages on `lane` (NULL: every stage on `stream`) */
+int mvx_VisionGraph_run_lane(mvx_stream_t stream, mvx_vision_lane_t *lane, const mvx_vision_params_t *params,
+                             uint32_t nframes, void *workspace, size_t workspaceSize);
 /* the same, one stage at a time (0 fused tile pass, 1 hysteresis border union, 2 hysteresis resolve,
  * 3 pyramid levels 2.., 4 Harris sort and pick).  Stage 0 comes first and 2 after 1; stages 3 and 4
  * need only stage 0 and may run on another stream concurrently with 1 and 2.  The next batch's
@@ -469,6 +498,10 @@ int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_params_t *pa
  * HafCpu_Phase_U8_S16S16 (:9592) and HafCpu_ScaleImage_U8_U8_Bilinear (ago_haf_cpu_geometric.cpp:1977)
  * run one node at a time; the reference has no batched launch (its executor launches one kernel per
  * node and frame, ago_util_hip.cpp:686-738).  Rows 0 and height-1 of gx/gy are not written.
+ * gx and gy both NULL: the gradients are virtual and the image takes the merged kernel
+ * mvx_SobelMagnitudePhase_S16U8_U8_3x3 instead of the three gradient nodes (the reference's merge rule,
+ * ago_drama_remove.cpp:155-163): magnitude columns from width & ~15 on truncate, and rows 0 and height-1
+ * of magnitude and phase are not written.
  * Widths and heights are even (2..65536), h*srcStride < 2^31; half may be NULL. */
 typedef struct mvx_chain_image {
     const uint8_t *src; uint32_t width, height, srcStride;
@@ -497,7 +530,7 @@ int mvx_ChainBatch_prepare(const mvx_chain_image_t *images, uint32_t count, uint
                            void *workspace, size_t workspaceSize, mvx_chain_plan_t *plan);
 /* One persistent launch over every tile of the plan; asynchronous, capturable. */
 int mvx_ChainBatch_run(mvx_stream_t stream, const mvx_chain_plan_t *plan);
-/* The same plan from the amd_rpp batchPD layout (amd_rpp/source/image/*batchPD.cpp, e.g.
+/* The same plan from the amd_rpp batchPD layout (amd_rpp/source/image/<name>batchPD.cpp, e.g.
  * AbsoluteDifferencebatchPD.cpp:52-61, and the rppi_*_batchPD_gpu calls at :146-161): nbatch images
  * stacked in one tall plane per role, image i in rows [i*maxHeight, i*maxHeight + heights[i]) and
  * columns [0, widths[i]) of the plane, with the plane's stride; widths / heights are host arrays (the

@@ -39,6 +39,12 @@ typedef struct {
     void *data_registry_callback_obj;
 } AgoGraphImportInfo;
 
+/* user-kernel attributes (vx_ext_amd.h:149-160 of the reference) */
+enum vx_kernel_attribute_amd_e {
+    VX_KERNEL_ATTRIBUTE_AMD_QUERY_TARGET_SUPPORT = VX_ATTRIBUTE_BASE(VX_ID_AMD, VX_TYPE_KERNEL) + 0x01,
+    VX_KERNEL_ATTRIBUTE_AMD_GPU_BUFFER_ACCESS_ENABLE = VX_ATTRIBUTE_BASE(VX_ID_AMD, VX_TYPE_KERNEL) + 0x05,
+};
+
 enum vx_context_attribute_amd_e {
     VX_CONTEXT_ATTRIBUTE_AMD_AFFINITY = VX_ATTRIBUTE_BASE(VX_ID_AMD, VX_TYPE_CONTEXT) + 0x02,
     VX_CONTEXT_ATTRIBUTE_AMD_HIP_DEVICE = VX_ATTRIBUTE_BASE(VX_ID_AMD, VX_TYPE_CONTEXT) + 0x07,
@@ -71,6 +77,16 @@ enum vx_memory_type_amd_e {
 
 #ifdef __cplusplus
 }
+
+/* A user kernel's target-support callback (vx_ext_amd.h:436-438 of the reference): sets
+ * supported_target_affinity to AGO_TARGET_AFFINITY_CPU, _GPU or both.  A kernel without one runs on the
+ * CPU: its execute callback is called on the host after the graph's stream has drained, and reads and
+ * writes its objects through the map / copy API.  A GPU kernel's execute callback runs on the host too,
+ * without a drain, and enqueues device work on VX_NODE_ATTRIBUTE_AMD_HIP_STREAM over
+ * VX_IMAGE_ATTRIBUTE_AMD_HIP_BUFFER pointers; so does a CPU kernel with
+ * VX_KERNEL_ATTRIBUTE_AMD_GPU_BUFFER_ACCESS_ENABLE set. */
+typedef vx_status(VX_CALLBACK *amd_kernel_query_target_support_f)(vx_graph graph, vx_node node, vx_bool use_opencl_1_2,
+                                                                  vx_uint32 &supported_target_affinity);
 #endif
 
 #endif

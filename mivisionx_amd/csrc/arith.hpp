@@ -34,6 +34,17 @@ MVX_DEV int32_t mag_tail(int32_t gx, int32_t gy)
     return (int32_t)(int16_t)(int32_t)roundf(t);
 }
 
+// Magnitude of the merged Sobel kernels' scalar prefix/postfix columns (ago_haf_cpu_filter.cpp:2496-2499,
+// :4004-4007): (vx_int16)sqrtf((float)(gx*gx) + (float)(gy*gy)), truncated rather than rounded.  The sum is
+// an integer below 2^24, so the float adds are exact and the correctly rounded square root is the same
+// float the reference computes.
+MVX_DEV int32_t mag_trunc(int32_t gx, int32_t gy)
+{
+    float t = (float)(gx * gx) + (float)(gy * gy);
+    t = __fsqrt_rn(t);
+    return (int32_t)(int16_t)(int32_t)t;
+}
+
 // atan2 polynomial of ago_haf_cpu.h:32-36: each constant is a product of two float literals,
 // rounded to float, and the polynomial runs in float.
 constexpr float ATAN_P1 = 0.9997878412794807f * 57.29577951308232f;

@@ -178,13 +178,17 @@ k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTil
         }
         __syncthreads();
 
-        // 3. Sobel, Magnitude, Phase: 8 pixels of one row per chunk
-        const int magBody = w & ~7;
+        // 3. Sobel, Magnitude, Phase: 8 pixels of one row per chunk.  With virtual gradients (gx == gy ==
+        //    NULL) the image takes the merged SobelMagnitudePhase kernel instead: no gradient stores, rows
+        //    1..h-2 only, and magnitude columns from w & ~15 on truncate (ago_haf_cpu_filter.cpp:3972-4202).
+        const bool merged = d.gx == nullptr;
+        const int magBody = merged ? (w & ~15) : (w & ~7);
         for (int i = threadIdx.x; i < TH * (TW / 8); i += T::NT) {
             const int ro = i / (TW / 8), co = i - ro * (TW / 8);
             const int y = y0 + ro, x = x0 + 8 * co;
             if (y >= h || x >= w) continue;
             const bool rowOk = y >= 1 && y <= h - 2, full = x + 8 <= w;
+            if (merged && !rowOk) continue;
             int16_t *pgx = (int16_t *)((uint8_t *)d.gx + (long)y * d.gxStride) + x;
             int16_t *pgy = (int16_t *)((uint8_t *)d.gy + (long)y * d.gyStride) + x;
             int16_t *pmg = (int16_t *)((uint8_t *)d.mag + (long)y * d.magStride) + x;
@@ -203,7 +207,8 @@ k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTil
                     const int gy = (int)(b2[7 + j] + 2 * b2[8 + j] + b2[9 + j]) - (int)(b0[7 + j] + 2 * b0[8 + j] + b0[9 + j]);
                     g[j] = ((uint32_t)gx & 0xffffu) | ((uint32_t)gy << 16);
                 }
-                if (full) {
+                if (merged) {
+                } else if (full) {
                     uint4 vx, vy;
                     vx.x = (g[0] & 0xffffu) | (g[1] << 16);
                     vx.y = (g[2] & 0xffffu) | (g[3] << 16);
@@ -230,7 +235,7 @@ k_chain_batch(const ChainRow *__restrict__ plan, uint32_t count, uint32_t numTil
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int32_t gx = (int16_t)(g[j] & 0xffffu), gy = (int32_t)g[j] >> 16;
-                const int32_t m = (body || x + j < magBody) ? mag_body(gx, gy) : mag_tail(gx, gy);
+                const int32_t m = (body || x + j < magBody) ? mag_body(gx, gy) : merged ? mag_trunc(gx, gy) : mag_tail(gx, gy);
                 mw[j >> 1] |= ((uint32_t)m & 0xffffu) << (16 * (j & 1));
                 pw[j >> 2] |= phase_px(gx, gy) << (8 * (j & 3));
             }
@@ -307,8 +312,10 @@ bool image_ok(const mvx_chain_image_t &im)
     const uint32_t w = im.width, h = im.height;
     if (w < 2 || h < 2 || (w & 1u) || (h & 1u) || w > 65536u || h > 65536u) return false;
     if ((uint64_t)h * im.srcStride >= (1ull << 31)) return false;
-    if (!plane_ok_cb(im.src, w, im.srcStride, 1) || !plane_ok_cb(im.gx, w, im.gxStride, 2) ||
-        !plane_ok_cb(im.gy, w, im.gyStride, 2) || !plane_ok_cb(im.mag, w, im.magStride, 2) ||
+    // gx and gy: both stored, or both NULL (virtual gradients, merged rules)
+    const bool merged = !im.gx && !im.gy;
+    if (!plane_ok_cb(im.src, w, im.srcStride, 1) || (!merged && (!plane_ok_cb(im.gx, w, im.gxStride, 2) ||
+        !plane_ok_cb(im.gy, w, im.gyStride, 2))) || !plane_ok_cb(im.mag, w, im.magStride, 2) ||
         !plane_ok_cb(im.phase, w, im.phaseStride, 1))
         return false;
     if (im.half && !plane_ok_cb(im.half, w / 2, im.halfStride, 1)) return false;
@@ -337,7 +344,7 @@ extern "C" int mvx_ChainBatch_prepare_batchPD(const uint8_t *src, uint32_t srcSt
                                               uint32_t tile, uint32_t workgroupsPerCU, void *workspace, size_t workspaceSize,
                                               mvx_chain_plan_t *plan)
 {
-    if (!src || !widths || !heights || nbatch == 0 || !gx || !gy || !mag || !phase || !plan) return MVX_ERROR_INVALID_PARAMETERS;
+    if (!src || !widths || !heights || nbatch == 0 || !gx != !gy || !mag || !phase || !plan) return MVX_ERROR_INVALID_PARAMETERS;
     std::vector<mvx_chain_image_t> images(nbatch);
     for (uint32_t i = 0; i < nbatch; i++) {
         if (heights[i] > maxHeight || (half && heights[i] / 2 > halfMaxHeight)) return MVX_ERROR_INVALID_PARAMETERS;
@@ -346,9 +353,9 @@ extern "C" int mvx_ChainBatch_prepare_batchPD(const uint8_t *src, uint32_t srcSt
         im.width = widths[i];
         im.height = heights[i];
         im.srcStride = srcStride;
-        im.gx = (int16_t *)((uint8_t *)gx + (size_t)i * maxHeight * gxStride);
+        im.gx = gx ? (int16_t *)((uint8_t *)gx + (size_t)i * maxHeight * gxStride) : nullptr;
         im.gxStride = gxStride;
-        im.gy = (int16_t *)((uint8_t *)gy + (size_t)i * maxHeight * gyStride);
+        im.gy = gy ? (int16_t *)((uint8_t *)gy + (size_t)i * maxHeight * gyStride) : nullptr;
         im.gyStride = gyStride;
         im.mag = (int16_t *)((uint8_t *)mag + (size_t)i * maxHeight * magStride);
         im.magStride = magStride;

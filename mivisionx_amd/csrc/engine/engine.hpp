@@ -52,9 +52,11 @@ struct ParamSpec {
 };
 
 struct Node;
+struct UserKernel;
 using ValidateFn = vx_status (*)(::_vx_node *node);
 using InitFn = vx_status (*)(::_vx_node *node);
 using ExecFn = vx_status (*)(::_vx_node *node, hipStream_t stream);
+using ValidRectFn = void (*)(::_vx_node *node);
 
 // one row of the kernel table (the role of ago_kernel_list.cpp's entries and the command
 // protocol of agoKernel_* functions: validate fills output meta data, initialize sizes the node's
@@ -66,8 +68,31 @@ struct KernelDef {
     ValidateFn validate;
     InitFn initialize;  // may be null (no workspace)
     ExecFn execute;
-    uint32_t targets;   // AGO_TARGET_AFFINITY_GPU (this engine has no CPU target)
+    uint32_t targets;   // the query_target_support answer of a built-in kernel: AGO_TARGET_AFFINITY_GPU
+    ValidRectFn validRect = nullptr;  // valid_rect_callback: output valid rectangles from the inputs' (null: unchanged)
+    UserKernel *user = nullptr;       // set for kernels added with vxAddUserKernel
 };
+
+// a kernel added with vxAddUserKernel (vx_api.cpp:2584-2810 of the reference): its callbacks and attributes
+struct UserKernel {
+    vx_kernel_f func = nullptr;
+    vx_kernel_validate_f validate = nullptr;
+    vx_kernel_initialize_f init = nullptr;
+    vx_kernel_deinitialize_f deinit = nullptr;
+    amd_kernel_query_target_support_f queryTarget = nullptr;  // null: CPU only
+    bool gpuBufferAccess = false;   // VX_KERNEL_ATTRIBUTE_AMD_GPU_BUFFER_ACCESS_ENABLE
+    vx_size localDataSize = 0;      // VX_KERNEL_LOCAL_DATA_SIZE: host bytes given to each node
+    bool finalized = false;
+    int module = -1;                // index in the context's module list, -1 when added by the application
+    std::string name;
+};
+
+// the targets a node's kernel supports (its query_target_support command); 0 with *status set on error
+uint32_t target_support(::_vx_node *node, vx_status *status);
+// output valid rectangles of every node, in execution order (agoComputeImageValidRectangleOutputs)
+void propagate_valid_rects(::_vx_graph *graph);
+// user kernels: release what initialize made (deinitialize callback, host local data)
+void deinit_user_node(::_vx_node *node);
 
 const std::vector<KernelDef> &kernel_table();
 
@@ -97,6 +122,9 @@ enum : vx_enum {
     AMD_SCALE_GAUSSIAN_HALF_U8_U8_3x3 = 0xd000b5,
     AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5 = 0xd000b6,
     AMD_SOBEL_S16S16_U8_3x3_GXY = 0xd000c3,
+    AMD_SOBEL_PHASE_U8_U8_3x3 = 0xd000b8,
+    AMD_SOBEL_MAGNITUDE_S16_U8_3x3 = 0xd000c1,
+    AMD_SOBEL_MAGNITUDE_PHASE_S16U8_U8_3x3 = 0xd000c2,
     AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM = 0xd000da,
     AMD_CANNY_SOBEL_U16_U8_3x3_L2NORM = 0xd000db,
     AMD_CANNY_SOBEL_U16_U8_5x5_L1NORM = 0xd000dc,
@@ -110,8 +138,9 @@ enum : vx_enum {
     AMD_SCALE_IMAGE_U8_U8_AREA = 0xd000f4,
 };
 
-// what executes a node: its own execute command, or one of the graph's supernodes
-enum FusedKind : int { FUSED_NONE = 0, FUSED_VISION = 1, FUSED_CHAIN = 2 };
+// what executes a node: its own execute command, one of the graph's supernodes, or (FUSED_MERGE) the
+// merged kernel of the Sobel node whose virtual gradients it reads (the reference's node-merge rules)
+enum FusedKind : int { FUSED_NONE = 0, FUSED_VISION = 1, FUSED_CHAIN = 2, FUSED_MERGE = 3 };
 
 }  // namespace mvxe
 
@@ -128,8 +157,17 @@ struct _vx_reference {
     virtual ~_vx_reference() {}
 };
 
+struct ModuleInfo {        // a module loaded by vxLoadKernels
+    std::string name, path;
+    void *handle;
+};
+
 struct _vx_context : _vx_reference {
     std::recursive_mutex lock;
+    std::vector<ModuleInfo> modules;
+    int importingModule = -1;         // module whose vxPublishKernels is running
+    std::vector<struct _vx_kernel *> removedKernels;   // vxRemoveKernel'd, freed with the context
+    vx_enum nextUserKernelId = 0, nextUserLibraryId = 1;
     std::vector<_vx_reference *> refs;  // every live object
     AgoTargetAffinityInfo affinity{};
     vx_border_t immediateBorder{VX_BORDER_UNDEFINED, {}};
@@ -165,6 +203,8 @@ struct _vx_image : _vx_reference {
     bool uniform = false;
     vx_pixel_value_t uniformValue{};
     struct _vx_pyramid *parent = nullptr;
+    vx_rectangle_t validRect{0, 0, 0, 0};
+    bool validRectSet = false;           // false: the whole image is valid
     std::vector<ImageMap> maps;
     vx_map_id nextMap = 1;
     _vx_image(_vx_context *c) : _vx_reference(VX_TYPE_IMAGE, c) {}
@@ -239,7 +279,22 @@ struct _vx_remap : _vx_reference {
 
 struct _vx_kernel : _vx_reference {
     const mvxe::KernelDef *def;
+    mvxe::KernelDef own{};           // a user kernel's own table row (def == &own)
+    mvxe::UserKernel user;
     _vx_kernel(_vx_context *c, const mvxe::KernelDef *d) : _vx_reference(VX_TYPE_KERNEL, c), def(d) {}
+};
+
+// the meta data a user kernel's validate callback gives each output (vxSetMetaFormatAttribute)
+struct _vx_meta_format : _vx_reference {
+    vx_enum objType = 0;
+    uint32_t width = 0, height = 0;
+    vx_df_image format = VX_DF_IMAGE_VIRT;
+    vx_enum itemType = 0, scalarType = 0;
+    vx_size capacity = 0, levels = 0;
+    vx_float32 scale = 0.f;
+    vx_kernel_image_valid_rectangle_f validRect = nullptr;
+    _vx_meta_format(_vx_context *c) : _vx_reference(VX_TYPE_META_FORMAT, c) {}
+    ~_vx_meta_format() { magic = 0; }
 };
 
 struct _vx_parameter : _vx_reference {
@@ -261,6 +316,16 @@ struct _vx_node : _vx_reference {
     size_t localSize = 0;
     uint32_t target = AGO_TARGET_AFFINITY_GPU;
     int fused = mvxe::FUSED_NONE;  // executed by one of the graph's supernodes (FusedKind)
+    // a Sobel node whose virtual gx / gy only these nodes read: it runs the merged kernel
+    // (SobelMagnitudePhase / SobelMagnitude / SobelPhase) and writes their outputs itself
+    struct _vx_node *mergedMag = nullptr, *mergedPhase = nullptr;
+    uint32_t variant = 0;     // kernel choice made by the validate command (per kernel; 0 = default)
+    // user kernels: host local data (VX_NODE_LOCAL_DATA_SIZE / _PTR), initialize state, valid-rect callbacks
+    void *localPtr = nullptr;
+    vx_size userLocalSize = 0;
+    std::vector<uint8_t> localHost;
+    bool userInitialized = false;
+    std::vector<vx_kernel_image_valid_rectangle_f> validRectCb;
     _vx_node(_vx_context *c) : _vx_reference(VX_TYPE_NODE, c) {}
 };
 
@@ -277,6 +342,7 @@ struct _vx_graph : _vx_reference {
     hipStream_t stream = nullptr;
     AgoTargetAffinityInfo affinity{};
     uint32_t optimizerFlags = 0;
+    bool allowMerge = true;           // node-merge rules on (no AGO_GRAPH_OPTIMIZER_FLAG_NO_NODE_MERGE)
     vx_perf_t perf{};
     mvxe::FusedVision *fused = nullptr;
     int fusedAt = -1;                 // position in `order` where the supernode runs
@@ -286,8 +352,8 @@ struct _vx_graph : _vx_reference {
     std::vector<_vx_scalar *> writtenScalars;
     // vxScheduleGraph worker
     std::thread worker;
-    std::mutex runLock;
-    bool scheduled = false;
+    std::mutex runLock;               // held while the graph verifies or executes, and while it is invalidated
+    std::atomic<bool> scheduled{false};
     vx_status lastStatus = VX_SUCCESS;
     _vx_graph(_vx_context *c) : _vx_reference(VX_TYPE_GRAPH, c) {}
 };

@@ -328,13 +328,16 @@ struct Importer {
         return r;
     }
 
+    // the context's kernels: the built-in table and user kernels (vxAddUserKernel, modules loaded by
+    // "import"), by full name or by the part after the last '.'
     const KernelDef *find_kernel(const std::string &name)
     {
-        for (const KernelDef &k : kernel_table())
-            if (name == k.name) return &k;
-        for (const KernelDef &k : kernel_table()) {
-            const char *p = strrchr(k.name, '.');
-            if (p && name == p + 1) return &k;
+        std::lock_guard<std::recursive_mutex> lk(graph->context->lock);
+        for (_vx_kernel *k : graph->context->kernels)
+            if (name == k->def->name) return k->def;
+        for (_vx_kernel *k : graph->context->kernels) {
+            const char *p = strrchr(k->def->name, '.');
+            if (p && name == p + 1) return k->def;
         }
         return nullptr;
     }

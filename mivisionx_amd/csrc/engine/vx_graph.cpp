@@ -32,11 +32,13 @@ struct FusedVision {
     DevBuf ws;
     size_t wsSize = 0;
     DevBuf count;  // corner count when the Harris node has no num_corners scalar
+    mvx_vision_lane_t *lane = nullptr;  // this graph's own side stream for the pyramid / pick stages
 };
 
 struct ChainNodes {
     _vx_node *gauss, *sobel, *mag, *phase, *scale;  // scale may be null
     _vx_image *in, *blur, *gx, *gy, *magOut, *phaseOut, *half;
+    bool merged;  // virtual gx / gy read only by mag and phase: the merged SobelMagnitudePhase rules
 };
 
 struct FusedChains {
@@ -77,6 +79,8 @@ namespace mvxe {
 _vx_node *create_node(_vx_graph *graph, const KernelDef *def, std::initializer_list<_vx_reference *> params)
 {
     if (!is_ref(graph, VX_TYPE_GRAPH) || !def) return nullptr;
+    if (graph->scheduled) return nullptr;   // a vxScheduleGraph worker may be executing it
+    std::lock_guard<std::mutex> lk(graph->runLock);
     _vx_node *n = new _vx_node(graph->context);
     n->kernel = def;
     n->graph = graph;
@@ -99,9 +103,11 @@ _vx_node *create_node(_vx_graph *graph, const KernelDef *def, std::initializer_l
 void invalidate_graph(_vx_graph *g)
 {
     for (_vx_node *n : g->nodes) {
+        deinit_user_node(n);
         n->local.release();
         n->localSize = 0;
         n->fused = FUSED_NONE;
+        n->mergedMag = n->mergedPhase = nullptr;
     }
     if (g->chains) {
         if (g->chains->graphExec) mvx_graph_destroy(g->chains->graphExec);
@@ -111,6 +117,7 @@ void invalidate_graph(_vx_graph *g)
     }
     g->chainsAt = -1;
     if (g->fused) {
+        if (g->fused->lane) mvx_VisionGraph_lane_destroy(g->fused->lane);
         g->fused->ws.release();
         g->fused->count.release();
         delete g->fused;
@@ -172,8 +179,13 @@ static FusedVision *find_fusion(_vx_graph *g)
         std::vector<_vx_node *> c1 = consumers(g, yuv);
         if (c1.size() != 1 || c1[0]->kernel->id != VX_KERNEL_CHANNEL_EXTRACT) continue;
         _vx_node *ex = c1[0];
+        // the extracted channel must be the luma plane, at the RGB image's size
+        _vx_scalar *chan = is_ref(ex->params[1], VX_TYPE_SCALAR) ? (_vx_scalar *)ex->params[1] : nullptr;
+        if (!chan || (chan->value.e != VX_CHANNEL_Y && chan->value.e != VX_CHANNEL_0)) continue;
         _vx_image *luma = image_of(ex->params[2]);
-        if (!luma || !luma->isVirtual) continue;
+        _vx_image *rgbIn = image_of(cc->params[0]);
+        if (!luma || !luma->isVirtual || !rgbIn || rgbIn->format != VX_DF_IMAGE_RGB) continue;
+        if (luma->width != rgbIn->width || luma->height != rgbIn->height || luma->format != VX_DF_IMAGE_U8) continue;
         FusedVision f;
         f.cc = cc;
         f.ex = ex;
@@ -236,6 +248,7 @@ static vx_status fused_prepare(_vx_graph *g)
         if (s) return s;
         fr.numCorners = (uint64_t *)f->count.ptr();
     }
+    if (!f->lane && mvx_VisionGraph_lane_create(&f->lane) != 0) f->lane = nullptr;   // without one, every stage runs in order
     f->wsSize = mvx_VisionGraph_workspace_size(&f->params, 1);
     vx_status s = f->ws.alloc(f->wsSize);
     if (s) return s;
@@ -305,6 +318,27 @@ static bool written_in_graph(_vx_graph *g, _vx_reference *r)
     return false;
 }
 
+// The node-merge rules for Sobel (ago_drama_remove.cpp:155-182): Sobel_S16S16_U8_3x3_GXY whose gx and gy
+// are both virtual and read by no node other than one Magnitude and/or one Phase node, each reading
+// (gx, gy) in that order, becomes SobelMagnitudePhase, SobelMagnitude or SobelPhase.  Returns false when
+// the rules do not apply; otherwise *mag / *phase are the absorbed nodes (one may be null).
+static bool sobel_merge_targets(_vx_graph *g, _vx_node *sobel, _vx_node **mag, _vx_node **phase)
+{
+    _vx_image *in = nullptr, *gx = nullptr, *gy = nullptr;
+    *mag = *phase = nullptr;
+    if (!sobel_io(sobel, &in, &gx, &gy) || !gx || !gy || gx == gy || !gx->isVirtual || !gy->isVirtual) return false;
+    std::set<_vx_node *> readers;
+    for (_vx_node *n : consumers(g, gx)) readers.insert(n);
+    for (_vx_node *n : consumers(g, gy)) readers.insert(n);
+    for (_vx_node *n : readers) {
+        _vx_image *a = nullptr, *b = nullptr, *o = nullptr;
+        if (gradpair_io(n, false, &a, &b, &o) && a == gx && b == gy && !*mag) *mag = n;
+        else if (gradpair_io(n, true, &a, &b, &o) && a == gx && b == gy && !*phase) *phase = n;
+        else return false;   // the gradients are used outside the rule
+    }
+    return *mag || *phase;
+}
+
 // every Gaussian3x3 (input: a U8 graph input of even size; output: virtual, read only by one
 // Sobel3x3 with both outputs) whose gradients feed one Magnitude and one Phase node, plus a bilinear
 // ScaleImage of the same input to exactly half size when the graph has one
@@ -320,7 +354,7 @@ static FusedChains *find_chains(_vx_graph *g)
         std::vector<_vx_node *> c = consumers(g, blur);
         _vx_image *sin = nullptr, *gx = nullptr, *gy = nullptr;
         if (c.size() != 1 || c[0]->fused || !sobel_io(c[0], &sin, &gx, &gy) || sin != blur || !gx || !gy) continue;
-        ChainNodes ch{gs, c[0], nullptr, nullptr, nullptr, in, blur, gx, gy, nullptr, nullptr, nullptr};
+        ChainNodes ch{gs, c[0], nullptr, nullptr, nullptr, in, blur, gx, gy, nullptr, nullptr, nullptr, false};
         for (_vx_node *n : consumers(g, gx)) {
             _vx_image *a = nullptr, *b = nullptr, *o = nullptr;
             if (n->fused) continue;
@@ -328,6 +362,13 @@ static FusedChains *find_chains(_vx_graph *g)
             else if (gradpair_io(n, true, &a, &b, &o) && a == gx && b == gy && !ch.phase) { ch.phase = n; ch.phaseOut = o; }
         }
         if (!ch.mag || !ch.phase) continue;
+        {
+            _vx_node *mm = nullptr, *mp = nullptr;
+            if (g->allowMerge && sobel_merge_targets(g, ch.sobel, &mm, &mp)) {
+                if (mm != ch.mag || mp != ch.phase) continue;
+                ch.merged = true;
+            }
+        }
         for (_vx_node *n : consumers(g, in)) {
             _vx_image *si = nullptr, *so = nullptr;
             if (n->fused || used.count(n) || !scale_io(n, &si, &so) || si != in || !so) continue;
@@ -351,8 +392,10 @@ static vx_status chains_prepare(_vx_graph *g)
     for (const ChainNodes &c : f->chains) {
         mvx_chain_image_t im{};
         im.src = c.in->plane_ptr(0); im.width = c.in->width; im.height = c.in->height; im.srcStride = c.in->planes[0].stride;
-        im.gx = (int16_t *)c.gx->plane_ptr(0); im.gxStride = c.gx->planes[0].stride;
-        im.gy = (int16_t *)c.gy->plane_ptr(0); im.gyStride = c.gy->planes[0].stride;
+        if (!c.merged) {
+            im.gx = (int16_t *)c.gx->plane_ptr(0); im.gxStride = c.gx->planes[0].stride;
+            im.gy = (int16_t *)c.gy->plane_ptr(0); im.gyStride = c.gy->planes[0].stride;
+        }
         im.mag = (int16_t *)c.magOut->plane_ptr(0); im.magStride = c.magOut->planes[0].stride;
         im.phase = c.phaseOut->plane_ptr(0); im.phaseStride = c.phaseOut->planes[0].stride;
         if (c.half) { im.half = c.half->plane_ptr(0); im.halfStride = c.half->planes[0].stride; }
@@ -465,18 +508,34 @@ vx_status verify_graph(_vx_graph *g)
         n->status = s;
         if (s) return s;
     }
-    // 4. target affinity: node > graph > context > AGO_DEFAULT_TARGET (read into the context) > GPU.
-    //    Every kernel here is a GPU kernel; a CPU request falls back with a warning, as the reference
-    //    does for kernels without the requested target (ago_drama_alloc.cpp:565-727).
+    // 4. target affinity: node > graph > context > AGO_DEFAULT_TARGET (read into the context) > GPU,
+    //    against what each kernel's query_target_support answers (ago_drama_alloc.cpp:565-727).  Every
+    //    built-in kernel is a GPU kernel, so a CPU request falls back with a warning; a user kernel runs
+    //    where it says it can.
     for (_vx_node *n : g->order) {
+        vx_status ts = VX_SUCCESS;
+        const uint32_t supported = target_support(n, &ts);
+        if (ts) return ts;
+        if (!(supported & (AGO_TARGET_AFFINITY_CPU | AGO_TARGET_AFFINITY_GPU))) {
+            log(n, VX_FAILURE, "ERROR: kernel %s not supported yet\n", n->kernel->name);
+            return VX_FAILURE;
+        }
         uint32_t want = n->affinity.device_type ? n->affinity.device_type
                       : g->affinity.device_type ? g->affinity.device_type
                       : ctx->affinity.device_type ? ctx->affinity.device_type : AGO_TARGET_AFFINITY_GPU;
-        if (!(want & n->kernel->targets))
-            log(n, VX_SUCCESS, "WARNING: %s: no CPU implementation in this engine; node runs on the GPU\n", n->kernel->name);
-        n->target = AGO_TARGET_AFFINITY_GPU;
+        if (want & supported) {
+            n->target = want & AGO_TARGET_AFFINITY_GPU ? AGO_TARGET_AFFINITY_GPU : AGO_TARGET_AFFINITY_CPU;
+        } else {
+            n->target = supported & AGO_TARGET_AFFINITY_GPU ? AGO_TARGET_AFFINITY_GPU : AGO_TARGET_AFFINITY_CPU;
+            log(n, VX_SUCCESS, "WARNING: kernel %s not supported on %s -- falling back to %s\n", n->kernel->name,
+                want == AGO_TARGET_AFFINITY_CPU ? "CPU" : "GPU", n->target == AGO_TARGET_AFFINITY_GPU ? "GPU" : "CPU");
+        }
     }
-    // 5. fusion
+    propagate_valid_rects(g);
+    // 5. fusion.  The node-merge rules change which reference kernel runs (and so its bits); they follow
+    //    the reference's NO_NODE_MERGE flag only.  The supernodes compute exactly what their nodes would
+    //    and are turned off by either flag or MVX_NO_FUSION.
+    g->allowMerge = !(g->optimizerFlags & AGO_GRAPH_OPTIMIZER_FLAG_NO_NODE_MERGE);
     const bool noFusion = (g->optimizerFlags & (AGO_GRAPH_OPTIMIZER_FLAG_NO_NODE_MERGE | AGO_GRAPH_OPTIMIZER_FLAG_NO_SUPERNODE_MERGE)) ||
                           getenv("MVX_NO_FUSION");
     if (!noFusion && (g->fused = find_fusion(g))) {
@@ -494,6 +553,27 @@ vx_status verify_graph(_vx_graph *g)
         for (size_t i = 0; i < g->order.size() && g->chainsAt < 0; i++)
             if (g->order[i]->fused == FUSED_CHAIN) g->chainsAt = (int)i;
     }
+    // virtual images that no kernel touches: the gradients of merged Sobel nodes and of merged chains
+    std::set<_vx_reference *> unallocated;
+    if (g->chains)
+        for (const ChainNodes &c : g->chains->chains)
+            if (c.merged) { unallocated.insert(c.gx); unallocated.insert(c.gy); }
+    if (g->allowMerge) {
+        for (_vx_node *n : g->order) {
+            _vx_node *mm = nullptr, *mp = nullptr;
+            if (n->fused || !sobel_merge_targets(g, n, &mm, &mp)) continue;
+            if ((mm && mm->fused) || (mp && mp->fused)) continue;
+            n->mergedMag = mm;
+            n->mergedPhase = mp;
+            for (_vx_node *m : {mm, mp}) if (m) m->fused = FUSED_MERGE;
+            _vx_image *in = nullptr, *gx = nullptr, *gy = nullptr;
+            sobel_io(n, &in, &gx, &gy);
+            unallocated.insert(gx);
+            unallocated.insert(gy);
+            log(g, VX_SUCCESS, "INFO: vxVerifyGraph: merged %s%s%s into %s\n", short_name(n->kernel), mm ? " + magnitude" : "",
+                mp ? " + phase" : "", mm && mp ? "SobelMagnitudePhase_S16U8_U8_3x3" : mm ? "SobelMagnitude_S16_U8_3x3" : "SobelPhase_U8_U8_3x3");
+        }
+    }
     // 6. device memory: every object a node touches, each node's workspace, one stream
     vx_status s = ctx->ensure_hip();
     if (s) return s;
@@ -509,6 +589,7 @@ vx_status verify_graph(_vx_graph *g)
                 _vx_image *im = (_vx_image *)r;
                 // the fused supernode keeps the virtual YUV image and luma in LDS: nothing to allocate
                 if (g->fused && im->isVirtual && (r == g->fused->cc->params[1] || r == g->fused->ex->params[2])) continue;
+                if (im->isVirtual && unallocated.count(r)) continue;
                 // the chain batch keeps each blurred image in LDS
                 if (g->chains && im->isVirtual && n->fused == FUSED_CHAIN) {
                     bool blur = false;
@@ -528,6 +609,7 @@ vx_status verify_graph(_vx_graph *g)
             }
         }
         if (n->fused) continue;
+        if (n->mergedMag || n->mergedPhase) continue;   // the merged kernels need no workspace
         if (n->kernel->initialize) {
             if ((s = n->kernel->initialize(n))) return s;
             if (n->localSize && (s = n->local.alloc(n->localSize))) return s;
@@ -542,7 +624,8 @@ vx_status verify_graph(_vx_graph *g)
     if (getenv("MVX_VERBOSE")) {
         for (_vx_node *n : g->order)
             log(g, VX_SUCCESS, "  node %s%s\n", short_name(n->kernel),
-                n->fused == FUSED_VISION ? " (fused vision supernode)" : n->fused == FUSED_CHAIN ? " (chain batch supernode)" : "");
+                n->fused == FUSED_VISION ? " (fused vision supernode)" : n->fused == FUSED_CHAIN ? " (chain batch supernode)"
+                : n->fused == FUSED_MERGE ? " (merged into its Sobel node)" : (n->mergedMag || n->mergedPhase) ? " (merged kernel)" : "");
     }
     return VX_SUCCESS;
 }
@@ -564,11 +647,12 @@ vx_status execute_graph(_vx_graph *g)
             if (r) { log(g, r, "ERROR: vxProcessGraph: chain batch supernode returned %d\n", r); s = r; }
             continue;
         }
+        if (n->fused == FUSED_MERGE) continue;
         if (n->fused) {
             if ((int)i != g->fusedAt) continue;
             FusedVision *f = g->fused;
             fused_params(f);  // thresholds and scalars are read at every run
-            const int r = mvx_VisionGraph_run(g->stream, &f->params, 1, f->ws.ptr(), f->wsSize);
+            const int r = mvx_VisionGraph_run_lane(g->stream, f->lane, &f->params, 1, f->ws.ptr(), f->wsSize);
             if (r) { log(g, r, "ERROR: vxProcessGraph: fused vision supernode returned %d\n", r); s = r; }
             continue;
         }
@@ -672,21 +756,29 @@ VX_API_ENTRY vx_status VX_API_CALL vxQueryGraph(vx_graph graph, vx_enum attribut
     return VX_SUCCESS;
 }
 
+// Changing what a verified graph runs releases its device state (invalidate_graph).  A graph that a
+// vxScheduleGraph worker may be executing is refused; otherwise the change waits for any
+// vxProcessGraph / vxVerifyGraph in progress on another thread (runLock).
 VX_API_ENTRY vx_status VX_API_CALL vxSetGraphAttribute(vx_graph graph, vx_enum attribute, const void *ptr, vx_size size)
 {
     if (!is_ref(graph, VX_TYPE_GRAPH)) return VX_ERROR_INVALID_REFERENCE;
     if (!ptr) return VX_ERROR_INVALID_PARAMETERS;
+    if (graph->scheduled) return VX_ERROR_GRAPH_SCHEDULED;
     switch (attribute) {
-    case VX_GRAPH_ATTRIBUTE_AMD_AFFINITY:
+    case VX_GRAPH_ATTRIBUTE_AMD_AFFINITY: {
         CHECK_SIZE(AgoTargetAffinityInfo);
+        std::lock_guard<std::mutex> lk(graph->runLock);
         graph->affinity = *(const AgoTargetAffinityInfo *)ptr;
         invalidate_graph(graph);
         break;
-    case VX_GRAPH_ATTRIBUTE_AMD_OPTIMIZER_FLAGS:
+    }
+    case VX_GRAPH_ATTRIBUTE_AMD_OPTIMIZER_FLAGS: {
         CHECK_SIZE(vx_uint32);
+        std::lock_guard<std::mutex> lk(graph->runLock);
         graph->optimizerFlags = *(const vx_uint32 *)ptr;
         invalidate_graph(graph);
         break;
+    }
     case VX_GRAPH_ATTRIBUTE_AMD_IMPORT_FROM_TEXT:
         CHECK_SIZE(AgoGraphImportInfo);
         return import_graph_text(graph, (const AgoGraphImportInfo *)ptr);
@@ -700,6 +792,7 @@ VX_API_ENTRY vx_status VX_API_CALL vxSetGraphAttribute(vx_graph graph, vx_enum a
 VX_API_ENTRY vx_node VX_API_CALL vxCreateGenericNode(vx_graph graph, vx_kernel kernel)
 {
     if (!is_ref(graph, VX_TYPE_GRAPH) || !is_ref(kernel, VX_TYPE_KERNEL)) return nullptr;
+    if (kernel->def->user && !kernel->def->user->finalized) return nullptr;   // vxFinalizeKernel first
     return create_node(graph, kernel->def, {});
 }
 
@@ -709,10 +802,15 @@ VX_API_ENTRY vx_status VX_API_CALL vxSetParameterByIndex(vx_node node, vx_uint32
     if (index >= node->params.size()) return VX_ERROR_INVALID_PARAMETERS;
     if (value && !is_ref(value)) return VX_ERROR_INVALID_REFERENCE;
     if (value && value->type != node->kernel->params[index].type) return VX_ERROR_INVALID_TYPE;
+    _vx_graph *g = node->graph;
+    if (g->scheduled) return VX_ERROR_GRAPH_SCHEDULED;
+    std::lock_guard<std::mutex> lk(g->runLock);
+    // the reference refuses parameter changes on a verified graph (vx_api.cpp:3809-3810)
+    if (g->verified) return VX_ERROR_NOT_SUPPORTED;
     if (value) value->intCount++;
     if (node->params[index]) release_ref(node->params[index], false);
     node->params[index] = value;
-    invalidate_graph(node->graph);
+    invalidate_graph(g);
     return VX_SUCCESS;
 }
 
@@ -724,10 +822,12 @@ VX_API_ENTRY vx_status VX_API_CALL vxQueryNode(vx_node node, vx_enum attribute, 
     case VX_NODE_STATUS: PUT(vx_status, node->status); break;
     case VX_NODE_PERFORMANCE: CHECK_SIZE(vx_perf_t); *(vx_perf_t *)ptr = node->perf; break;
     case VX_NODE_BORDER: CHECK_SIZE(vx_border_t); *(vx_border_t *)ptr = node->border; break;
-    case VX_NODE_LOCAL_DATA_SIZE: PUT(vx_size, node->localSize); break;
+    case VX_NODE_LOCAL_DATA_SIZE: PUT(vx_size, node->kernel->user ? (node->localHost.empty() ? node->userLocalSize : node->localHost.size())
+                                                               : node->localSize); break;
     case VX_NODE_PARAMETERS: PUT(vx_uint32, node->params.size()); break;
     case VX_NODE_ATTRIBUTE_AMD_AFFINITY: CHECK_SIZE(AgoTargetAffinityInfo); *(AgoTargetAffinityInfo *)ptr = node->affinity; break;
     case VX_NODE_ATTRIBUTE_AMD_HIP_STREAM: PUT(hipStream_t, node->graph->stream); break;
+    case VX_NODE_LOCAL_DATA_PTR: PUT(void *, node->localPtr); break;
     default: return VX_ERROR_NOT_SUPPORTED;
     }
     return VX_SUCCESS;
@@ -737,7 +837,20 @@ VX_API_ENTRY vx_status VX_API_CALL vxSetNodeAttribute(vx_node node, vx_enum attr
 {
     if (!is_ref(node, VX_TYPE_NODE)) return VX_ERROR_INVALID_REFERENCE;
     if (!ptr) return VX_ERROR_INVALID_PARAMETERS;
+    // a user kernel's initialize callback hands its local data over while the graph verifies
+    if (attribute == VX_NODE_LOCAL_DATA_PTR) {
+        CHECK_SIZE(void *);
+        node->localPtr = *(void *const *)ptr;
+        return VX_SUCCESS;
+    }
+    if (node->graph->scheduled) return VX_ERROR_GRAPH_SCHEDULED;
+    std::lock_guard<std::mutex> lk(node->graph->runLock);
     switch (attribute) {
+    case VX_NODE_LOCAL_DATA_SIZE:
+        CHECK_SIZE(vx_size);
+        if (!node->kernel->user) return VX_ERROR_NOT_SUPPORTED;
+        node->userLocalSize = *(const vx_size *)ptr;
+        break;
     case VX_NODE_BORDER: CHECK_SIZE(vx_border_t); node->border = *(const vx_border_t *)ptr; break;
     case VX_NODE_ATTRIBUTE_AMD_AFFINITY: CHECK_SIZE(AgoTargetAffinityInfo); node->affinity = *(const AgoTargetAffinityInfo *)ptr; break;
     default: return VX_ERROR_NOT_SUPPORTED;
@@ -759,8 +872,10 @@ VX_API_ENTRY void VX_API_CALL vxRemoveNode(vx_node *node)
     if (!node || !is_ref(*node, VX_TYPE_NODE)) return;
     _vx_node *n = *node;
     _vx_graph *g = n->graph;
+    if (g->scheduled) return;
     auto it = std::find(g->nodes.begin(), g->nodes.end(), n);
     if (it != g->nodes.end()) {
+        std::lock_guard<std::mutex> lk(g->runLock);
         invalidate_graph(g);
         g->nodes.erase(it);
         release_ref(n, false);

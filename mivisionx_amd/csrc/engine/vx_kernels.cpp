@@ -9,6 +9,7 @@
 // (ago_drama_divide.cpp, cited per node below), but inside one execute command: every kernel is a
 // gfx950 launch behind mvx_kernels.h on the graph's stream, including the Canny trace and the
 // Harris sort-and-pick that the reference runs on the CPU.
+#include <algorithm>
 #include <cmath>
 
 #include "engine.hpp"
@@ -104,9 +105,30 @@ vx_status v_sobel(_vx_node *n)
     return VX_SUCCESS;
 }
 
+// the output image of an absorbed Magnitude / Phase node (Khronos order: out last; AMD order: out first)
+_vx_image *gradpair_out(_vx_node *m)
+{
+    return img(m, (m->kernel->id == AMD_MAGNITUDE_S16_S16S16 || m->kernel->id == AMD_PHASE_U8_S16S16) ? 0 : 2);
+}
+
+// a Sobel node whose virtual gradients only a Magnitude and/or a Phase node read runs the merged kernel
+// the reference's node-merge rules choose (ago_drama_remove.cpp:155-182) and writes their outputs
+vx_status x_sobel_merged(_vx_node *n, hipStream_t st, _vx_image *in)
+{
+    _vx_image *m = n->mergedMag ? gradpair_out(n->mergedMag) : nullptr, *p = n->mergedPhase ? gradpair_out(n->mergedPhase) : nullptr;
+    const uint32_t w = in->width, h = in->height;
+    vx_status s;
+    if (m && p) s = rc(n, mvx_SobelMagnitudePhase_S16U8_U8_3x3(st, w, h, (int16_t *)P(m), S(m), P(p), S(p), P(in), S(in)));
+    else if (m) s = rc(n, mvx_SobelMagnitude_S16_U8_3x3(st, w, h, (int16_t *)P(m), S(m), P(in), S(in)));
+    else s = rc(n, mvx_SobelPhase_U8_U8_3x3(st, w, h, P(p), S(p), P(in), S(in)));
+    for (_vx_node *a : {n->mergedMag, n->mergedPhase}) if (a) a->status = s;
+    return s;
+}
+
 vx_status x_sobel(_vx_node *n, hipStream_t st)
 {
     _vx_image *in = img(n, 0), *gx = img(n, 1), *gy = img(n, 2);
+    if (n->mergedMag || n->mergedPhase) return x_sobel_merged(n, st, in);
     return rc(n, mvx_Sobel_S16S16_U8_3x3_GXY(st, in->width, in->height, gx ? (int16_t *)P(gx) : nullptr, gx ? S(gx) : 0,
                                              gy ? (int16_t *)P(gy) : nullptr, gy ? S(gy) : 0, P(in), S(in)));
 }
@@ -122,7 +144,34 @@ vx_status v_amd_sobel(_vx_node *n)
 vx_status x_amd_sobel(_vx_node *n, hipStream_t st)
 {
     _vx_image *gx = img(n, 0), *gy = img(n, 1), *in = img(n, 2);
+    if (n->mergedMag || n->mergedPhase) return x_sobel_merged(n, st, in);
     return rc(n, mvx_Sobel_S16S16_U8_3x3_GXY(st, in->width, in->height, (int16_t *)P(gx), S(gx), (int16_t *)P(gy), S(gy), P(in), S(in)));
+}
+
+// AMD merged forms named directly (ago_kernel_list.cpp:416-418; wrappers ago_kernel_api.cpp:15958-16110):
+// SobelMagnitude_S16_U8_3x3 (mag, in), SobelPhase_U8_U8_3x3 (phase, in),
+// SobelMagnitudePhase_S16U8_U8_3x3 (mag, phase, in)
+template <int OUTS> vx_status v_amd_sobel_merged(_vx_node *n)
+{
+    vx_status s = need_input(n, OUTS, VX_DF_IMAGE_U8);
+    if (s) return s;
+    _vx_image *in = img(n, OUTS);
+    if (n->kernel->id == AMD_SOBEL_PHASE_U8_U8_3x3) return set_output(n, 0, in->width, in->height, VX_DF_IMAGE_U8);
+    s = set_output(n, 0, in->width, in->height, VX_DF_IMAGE_S16);
+    if (!s && OUTS == 2) s = set_output(n, 1, in->width, in->height, VX_DF_IMAGE_U8);
+    return s;
+}
+
+template <vx_enum ID> vx_status x_amd_sobel_merged(_vx_node *n, hipStream_t st)
+{
+    if (ID == AMD_SOBEL_MAGNITUDE_PHASE_S16U8_U8_3x3) {
+        _vx_image *m = img(n, 0), *p = img(n, 1), *in = img(n, 2);
+        return rc(n, mvx_SobelMagnitudePhase_S16U8_U8_3x3(st, in->width, in->height, (int16_t *)P(m), S(m), P(p), S(p), P(in), S(in)));
+    }
+    _vx_image *o = img(n, 0), *in = img(n, 1);
+    if (ID == AMD_SOBEL_MAGNITUDE_S16_U8_3x3)
+        return rc(n, mvx_SobelMagnitude_S16_U8_3x3(st, in->width, in->height, (int16_t *)P(o), S(o), P(in), S(in)));
+    return rc(n, mvx_SobelPhase_U8_U8_3x3(st, in->width, in->height, P(o), S(o), P(in), S(in)));
 }
 
 // ---------------------------------------------------------------------- Magnitude / Phase (A7, A8)
@@ -158,6 +207,24 @@ template <int IO> vx_status x_phase(_vx_node *n, hipStream_t st)
 // (ago_drama_divide.cpp:244-256 and the unused-output removal, ago_drama_remove.cpp:989).  RGB -> NV12 /
 // YUV4 chroma planes are not produced.
 // IYUV / NV12 -> RGB: COLOR_CONVERT_RGB_IYUV / _RGB_NV12 (ago_drama_divide.cpp:80-113).
+// true when every node of the graph that reads `yuv` is a channel_extract of its Y channel: the RGB
+// conversion then needs only its luma kernel (the reference divides RGB -> IYUV into Y_RGB, IU_RGB and
+// IV_RGB, ago_drama_divide.cpp:244-256, and drops the kernels whose outputs nobody reads,
+// ago_drama_remove.cpp:989)
+bool luma_only_readers(_vx_node *n, _vx_image *yuv)
+{
+    for (_vx_node *m : n->graph->nodes)
+        for (size_t i = 0; i < m->params.size(); i++) {
+            if (m->params[i] != yuv || m->kernel->params[i].dir == VX_OUTPUT) continue;
+            if (m->kernel->id != VX_KERNEL_CHANNEL_EXTRACT) return false;
+            _vx_scalar *ch = scl(m, 1);
+            if (!ch || (ch->value.e != VX_CHANNEL_Y && ch->value.e != VX_CHANNEL_0)) return false;
+        }
+    return true;
+}
+
+enum : uint32_t { COLOR_ALL_PLANES = 0, COLOR_LUMA_ONLY = 1 };
+
 vx_status v_color(_vx_node *n)
 {
     _vx_image *src = img(n, 0);
@@ -177,10 +244,11 @@ vx_status v_color(_vx_node *n)
     if (want != VX_DF_IMAGE_IYUV && want != VX_DF_IMAGE_NV12 && want != VX_DF_IMAGE_YUV4)
         return fail(n, VX_ERROR_NOT_SUPPORTED, "RGB converts to the luma plane of IYUV / NV12 / YUV4 only");
     if ((s = set_output(n, 1, img(n, 0)->width, img(n, 0)->height, want))) return s;
-    if (!o->isVirtual && want != VX_DF_IMAGE_IYUV)
+    n->variant = (o->isVirtual && luma_only_readers(n, o)) ? COLOR_LUMA_ONLY : COLOR_ALL_PLANES;
+    if (n->variant != COLOR_LUMA_ONLY && want != VX_DF_IMAGE_IYUV)
         return fail(n, VX_ERROR_NOT_SUPPORTED, "RGB -> NV12 / YUV4 chroma planes are not produced: the output must be IYUV, or a "
-                                               "virtual image read through channel_extract(Y)");
-    if (!o->isVirtual && ((o->width & 1u) || (o->height & 1u)))
+                                               "virtual image read only through channel_extract(Y)");
+    if (n->variant != COLOR_LUMA_ONLY && ((o->width & 1u) || (o->height & 1u)))
         return fail(n, VX_ERROR_INVALID_DIMENSION, "YUV 4:2:0 sizes must be even");
     return VX_SUCCESS;
 }
@@ -192,7 +260,7 @@ vx_status x_color(_vx_node *n, hipStream_t st)
         return rc(n, mvx_ColorConvert_RGB_IYUV(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1), P(in, 2), S(in, 2)));
     if (in->format == VX_DF_IMAGE_NV12)
         return rc(n, mvx_ColorConvert_RGB_NV12(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1)));
-    if (!o->isVirtual && o->format == VX_DF_IMAGE_IYUV)
+    if (n->variant != COLOR_LUMA_ONLY && o->format == VX_DF_IMAGE_IYUV)
         return rc(n, mvx_ColorConvert_IYUV_RGB(st, o->width, o->height, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in),
                                                S(in)));
     return rc(n, mvx_ColorConvert_Y_RGB(st, o->width, o->height, P(o), S(o), P(in), S(in)));
@@ -917,6 +985,136 @@ vx_status x_lk(_vx_node *n, hipStream_t st)
                                             (uint32_t)scl(n, 9)->value.sz, n->local.ptr(), n->localSize));
 }
 
+// ---------------------------------------------------------------------- valid rectangles --------
+// The reference's valid_rect_callback commands (ago_kernel_api.cpp, one per AMD kernel, e.g. Gaussian
+// :15317-15326, Magnitude :10357-10364, ScaleImage :19897-19908, ScaleGaussianHalf 5x5 :15444-15453),
+// composed per node the way the dividers split it.  Kernels without one (warps, remap, corner and
+// tracking nodes, the Canny trace) leave their outputs' rectangles as they are.
+vx_rectangle_t rect_of(_vx_image *im)
+{
+    vx_rectangle_t r;
+    vxGetValidRegionImage(im, &r);
+    return r;
+}
+
+void set_rect(_vx_image *im, vx_rectangle_t r)
+{
+    if (!im) return;
+    im->validRect = r;
+    im->validRectSet = true;
+}
+
+// start + k clamped to the output size, end - k clamped at 0 (the 3x3 / 5x5 / 7x7 stencil rules)
+vx_rectangle_t shrink(vx_rectangle_t r, int k, _vx_image *out)
+{
+    return vx_rectangle_t{std::min(r.start_x + k, out->width), std::min(r.start_y + k, out->height),
+                          (vx_uint32)std::max((int)r.end_x - k, 0), (vx_uint32)std::max((int)r.end_y - k, 0)};
+}
+
+// ScaleGaussianHalf 5x5 (:15444-15453); with `inner` the 3x3 form, one more pixel in (:15510-15518)
+vx_rectangle_t half_rect(vx_rectangle_t r, _vx_image *out, bool inner)
+{
+    const int k = inner ? 1 : 0;
+    return vx_rectangle_t{std::min(((r.start_x + 1) >> 1) + k, out->width), std::min(((r.start_y + 1) >> 1) + k, out->height),
+                          (vx_uint32)std::max((int)((r.end_x + 1) >> 1) - k, 0), (vx_uint32)std::max((int)((r.end_y + 1) >> 1) - k, 0)};
+}
+
+// The ScaleImage kernels' callback (:19897-19908) reads its parameters in the Khronos order (input
+// first) although the AMD kernels take the output first, so it rescales the OUTPUT's rectangle into the
+// INPUT image and leaves the output's as it was.  Reproduced as such.
+void scale_rect_quirk(_vx_image *in, _vx_image *out)
+{
+    const vx_rectangle_t r = rect_of(out);
+    const float wo = (float)in->width, ho = (float)in->height, wi = (float)out->width, hi = (float)out->height;
+    set_rect(in, vx_rectangle_t{(vx_uint32)(((r.start_x + 0.5f) * wo / wi) - 0.5f), (vx_uint32)(((r.start_y + 0.5f) * ho / hi) - 0.5f),
+                                (vx_uint32)(((r.end_x + 0.5f) * wo / wi) - 0.5f), (vx_uint32)(((r.end_y + 0.5f) * ho / hi) - 0.5f)});
+}
+
+template <int IN, int OUT> void vr_copy(_vx_node *n) { set_rect(img(n, OUT), rect_of(img(n, IN))); }
+template <int IN, int OUT, int K> void vr_shrink(_vx_node *n)
+{
+    if (img(n, OUT)) set_rect(img(n, OUT), shrink(rect_of(img(n, IN)), K, img(n, OUT)));
+}
+template <int IN, int OUT1, int OUT2, int K> void vr_shrink2(_vx_node *n)
+{
+    vr_shrink<IN, OUT1, K>(n);
+    vr_shrink<IN, OUT2, K>(n);
+}
+template <int A, int B, int OUT> void vr_intersect(_vx_node *n)
+{
+    const vx_rectangle_t a = rect_of(img(n, A)), b = rect_of(img(n, B));
+    set_rect(img(n, OUT), vx_rectangle_t{std::max(a.start_x, b.start_x), std::max(a.start_y, b.start_y), std::min(a.end_x, b.end_x),
+                                         std::min(a.end_y, b.end_y)});
+}
+
+void vr_extract(_vx_node *n)
+{
+    _vx_image *in = img(n, 0), *o = img(n, 2);
+    const vx_rectangle_t r = rect_of(in);
+    if (o->width == in->width && o->height == in->height) set_rect(o, r);
+    else set_rect(o, vx_rectangle_t{(r.start_x + 1) >> 1, (r.start_y + 1) >> 1, (r.end_x + 1) >> 1, (r.end_y + 1) >> 1});
+}
+
+template <int IN, int OUT> void vr_scale(_vx_node *n)
+{
+    _vx_image *in = img(n, IN), *o = img(n, OUT);
+    if (in->width == o->width && in->height == o->height) set_rect(o, rect_of(in));   // the copy kernel
+    else scale_rect_quirk(in, o);
+}
+
+void vr_pyramid(_vx_node *n)
+{
+    _vx_pyramid *p = (_vx_pyramid *)n->params[1];
+    set_rect(p->images[0], rect_of(img(n, 0)));
+    for (size_t l = 1; l < p->images.size(); l++) set_rect(p->images[l], half_rect(rect_of(p->images[l - 1]), p->images[l], false));
+}
+
+void vr_halfscale(_vx_node *n)
+{
+    _vx_image *in = img(n, 0), *o = img(n, 1);
+    const int k = scl(n, 2) ? scl(n, 2)->value.i32 : 5;
+    if (k == 1) scale_rect_quirk(in, o);
+    else set_rect(o, half_rect(rect_of(in), o, k == 3));
+}
+
+// CannySobel shrinks by gradient/2, the non-max suppression by one more; the trace has no rule
+void vr_canny(_vx_node *n)
+{
+    const int g = scl(n, 2) ? scl(n, 2)->value.i32 : 3;
+    set_rect(img(n, 4), shrink(rect_of(img(n, 0)), g / 2 + 1, img(n, 4)));
+}
+
+template <int G> void vr_amd_canny_sobel(_vx_node *n) { vr_shrink<1, 0, G / 2>(n); }
+
+ValidRectFn valid_rect_rule(vx_enum id)
+{
+    switch (id) {
+    case VX_KERNEL_COLOR_CONVERT: return vr_copy<0, 1>;
+    case VX_KERNEL_CHANNEL_EXTRACT: return vr_extract;
+    case VX_KERNEL_GAUSSIAN_3x3: case VX_KERNEL_BOX_3x3: case VX_KERNEL_MEDIAN_3x3: return vr_shrink<0, 1, 1>;
+    case VX_KERNEL_SOBEL_3x3: return vr_shrink2<0, 1, 2, 1>;
+    case VX_KERNEL_MAGNITUDE: case VX_KERNEL_PHASE: return vr_intersect<0, 1, 2>;
+    case VX_KERNEL_SCALE_IMAGE: return vr_scale<0, 1>;
+    case VX_KERNEL_GAUSSIAN_PYRAMID: return vr_pyramid;
+    case VX_KERNEL_HALFSCALE_GAUSSIAN: return vr_halfscale;
+    case VX_KERNEL_CANNY_EDGE_DETECTOR: return vr_canny;
+    case AMD_MAGNITUDE_S16_S16S16: case AMD_PHASE_U8_S16S16: return vr_intersect<1, 2, 0>;
+    case AMD_CHANNEL_COPY_U8_U8: case AMD_COLOR_CONVERT_Y_RGB: return vr_copy<1, 0>;
+    case AMD_BOX_U8_U8_3x3: case AMD_MEDIAN_U8_U8_3x3: case AMD_GAUSSIAN_U8_U8_3x3: return vr_shrink<1, 0, 1>;
+    case AMD_SOBEL_S16S16_U8_3x3_GXY: return vr_shrink2<2, 0, 1, 1>;
+    case AMD_SOBEL_MAGNITUDE_S16_U8_3x3: case AMD_SOBEL_PHASE_U8_U8_3x3: return vr_shrink<1, 0, 1>;
+    case AMD_SOBEL_MAGNITUDE_PHASE_S16U8_U8_3x3: return vr_shrink2<2, 0, 1, 1>;
+    case AMD_SCALE_GAUSSIAN_HALF_U8_U8_5x5: return [](_vx_node *n) { set_rect(img(n, 0), half_rect(rect_of(img(n, 1)), img(n, 0), false)); };
+    case AMD_SCALE_GAUSSIAN_HALF_U8_U8_3x3: return [](_vx_node *n) { set_rect(img(n, 0), half_rect(rect_of(img(n, 1)), img(n, 0), true)); };
+    case AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM: case AMD_CANNY_SOBEL_U16_U8_3x3_L2NORM: return vr_amd_canny_sobel<3>;
+    case AMD_CANNY_SOBEL_U16_U8_5x5_L1NORM: case AMD_CANNY_SOBEL_U16_U8_5x5_L2NORM: return vr_amd_canny_sobel<5>;
+    case AMD_CANNY_SOBEL_U16_U8_7x7_L1NORM: case AMD_CANNY_SOBEL_U16_U8_7x7_L2NORM: return vr_amd_canny_sobel<7>;
+    case AMD_SCALE_IMAGE_U8_U8_NEAREST: case AMD_SCALE_IMAGE_U8_U8_BILINEAR: case AMD_SCALE_IMAGE_U8_U8_BILINEAR_REPLICATE:
+    case AMD_SCALE_IMAGE_U8_U8_BILINEAR_CONSTANT: case AMD_SCALE_IMAGE_U8_U8_AREA: return vr_scale<1, 0>;
+    default: return nullptr;
+    }
+}
+
 }  // namespace
 
 namespace mvxe {
@@ -930,9 +1128,15 @@ vx_status prepare_node(_vx_node *n, hipStream_t st)
 
 float harris_norm_factor(int gradient, int block) { return harris_norm(gradient, block); }
 
+static std::vector<KernelDef> with_valid_rects(std::vector<KernelDef> t)
+{
+    for (KernelDef &k : t) k.validRect = valid_rect_rule(k.id);
+    return t;
+}
+
 const std::vector<KernelDef> &kernel_table()
 {
-    static const std::vector<KernelDef> table = {
+    static const std::vector<KernelDef> table = with_valid_rects({
         // Khronos vision nodes (ago_kernel_list.cpp:183-260)
         {VX_KERNEL_COLOR_CONVERT, "org.khronos.openvx.color_convert", {{IN, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}},
          v_color, nullptr, x_color, AGO_TARGET_AFFINITY_GPU},
@@ -1054,6 +1258,15 @@ const std::vector<KernelDef> &kernel_table()
         {AMD_SOBEL_S16S16_U8_3x3_GXY, "com.amd.openvx.Sobel_S16S16_U8_3x3_GXY",
          {{OUT, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_sobel, nullptr, x_amd_sobel,
          AGO_TARGET_AFFINITY_GPU},
+        {AMD_SOBEL_MAGNITUDE_S16_U8_3x3, "com.amd.openvx.SobelMagnitude_S16_U8_3x3",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_sobel_merged<1>, nullptr,
+         x_amd_sobel_merged<AMD_SOBEL_MAGNITUDE_S16_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SOBEL_PHASE_U8_U8_3x3, "com.amd.openvx.SobelPhase_U8_U8_3x3",
+         {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_sobel_merged<1>, nullptr,
+         x_amd_sobel_merged<AMD_SOBEL_PHASE_U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
+        {AMD_SOBEL_MAGNITUDE_PHASE_S16U8_U8_3x3, "com.amd.openvx.SobelMagnitudePhase_S16U8_U8_3x3",
+         {{OUT, VX_TYPE_IMAGE, false}, {OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_sobel_merged<2>, nullptr,
+         x_amd_sobel_merged<AMD_SOBEL_MAGNITUDE_PHASE_S16U8_U8_3x3>, AGO_TARGET_AFFINITY_GPU},
         {AMD_CANNY_SOBEL_U16_U8_3x3_L1NORM, "com.amd.openvx.CannySobel_U16_U8_3x3_L1NORM",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_canny_sobel, nullptr, x_amd_canny_sobel<3, false>,
          AGO_TARGET_AFFINITY_GPU},
@@ -1083,7 +1296,7 @@ const std::vector<KernelDef> &kernel_table()
          x_amd_scale<3>, AGO_TARGET_AFFINITY_GPU},
         {AMD_SCALE_IMAGE_U8_U8_AREA, "com.amd.openvx.ScaleImage_U8_U8_Area",
          {{OUT, VX_TYPE_IMAGE, false}, {IN, VX_TYPE_IMAGE, false}}, v_amd_scale_k<4>, nullptr, x_amd_scale<4>, AGO_TARGET_AFFINITY_GPU},
-    };
+    });
     return table;
 }
 

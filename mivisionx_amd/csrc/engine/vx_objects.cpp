@@ -6,6 +6,8 @@
 // layout rules so that kernels see the same bytes: row stride ALIGN16(width * bytes per pixel)
 // (ago_util.cpp:1107), zero-initialised buffers with zero guard bytes on both sides
 // (ago_util.cpp:187-200), array count word in front of the items (ago_internal.h:171).
+#include <dlfcn.h>
+
 #include "engine.hpp"
 
 #include <cmath>
@@ -110,6 +112,7 @@ static void destroy(_vx_reference *r)
         break;
     case VX_TYPE_NODE: {
         _vx_node *n = (_vx_node *)r;
+        mvxe::deinit_user_node(n);
         for (_vx_reference *p : n->params) release_ref(p, false);
         n->local.release();
         break;
@@ -193,6 +196,9 @@ VX_API_ENTRY vx_status VX_API_CALL vxReleaseContext(vx_context *context)
         }
     }
     for (_vx_kernel *k : ctx->kernels) { k->magic = 0; delete k; }
+    for (_vx_kernel *k : ctx->removedKernels) { k->magic = 0; delete k; }
+    for (const ModuleInfo &m : ctx->modules)
+        if (m.handle) dlclose(m.handle);
     ctx->magic = 0;
     delete ctx;
     *context = nullptr;
@@ -220,7 +226,12 @@ VX_API_ENTRY vx_status VX_API_CALL vxQueryContext(vx_context context, vx_enum at
     case VX_CONTEXT_VENDOR_ID: PUT(vx_uint16, VX_ID_AMD); break;
     case VX_CONTEXT_VERSION: PUT(vx_uint16, VX_VERSION); break;
     case VX_CONTEXT_UNIQUE_KERNELS: PUT(vx_uint32, context->kernels.size()); break;
-    case VX_CONTEXT_MODULES: PUT(vx_uint32, 0); break;
+    case VX_CONTEXT_MODULES: {
+        vx_uint32 n = 0;
+        for (const ModuleInfo &m : context->modules) n += m.path.empty() ? 0 : 1;
+        PUT(vx_uint32, n);
+        break;
+    }
     case VX_CONTEXT_REFERENCES: PUT(vx_uint32, context->refs.size()); break;
     case VX_CONTEXT_OPTICAL_FLOW_MAX_WINDOW_DIMENSION: PUT(vx_size, 9); break;
     case VX_CONTEXT_IMPLEMENTATION:
@@ -307,14 +318,6 @@ template <typename T> static vx_status release_typed(T **p, vx_enum type)
     return VX_SUCCESS;
 }
 
-VX_API_ENTRY vx_status VX_API_CALL vxLoadKernels(vx_context context, const vx_char *module)
-{
-    if (!is_ref(context, VX_TYPE_CONTEXT)) return VX_ERROR_INVALID_REFERENCE;
-    // the vision-node kernels are built in; other modules are outside this engine
-    log(context, VX_ERROR_INVALID_MODULE, "ERROR: vxLoadKernels(%s): no such module in this engine\n", module ? module : "(null)");
-    return VX_ERROR_INVALID_MODULE;
-}
-
 // ------------------------------------------------------------------------------ kernels ----------
 
 VX_API_ENTRY vx_kernel VX_API_CALL vxGetKernelByName(vx_context context, const vx_char *name)
@@ -345,7 +348,7 @@ VX_API_ENTRY vx_status VX_API_CALL vxQueryKernel(vx_kernel kernel, vx_enum attri
         strcpy((char *)ptr, kernel->def->name);
         break;
     case VX_KERNEL_ENUM: PUT(vx_enum, kernel->def->id); break;
-    case VX_KERNEL_LOCAL_DATA_SIZE: PUT(vx_size, 0); break;
+    case VX_KERNEL_LOCAL_DATA_SIZE: PUT(vx_size, kernel->def->user ? kernel->def->user->localDataSize : 0); break;
     default: return VX_ERROR_NOT_SUPPORTED;
     }
     return VX_SUCCESS;
@@ -724,11 +727,13 @@ VX_API_ENTRY vx_status VX_API_CALL vxUnmapImagePatch(vx_image image, vx_map_id m
     return VX_ERROR_INVALID_PARAMETERS;
 }
 
+// the image's valid rectangle: the whole image, until graph verification propagates the rectangles of
+// the kernels that write it (vx_api.cpp:1946; agoComputeImageValidRectangleOutputs)
 VX_API_ENTRY vx_status VX_API_CALL vxGetValidRegionImage(vx_image image, vx_rectangle_t *rect)
 {
     if (!is_ref(image, VX_TYPE_IMAGE)) return VX_ERROR_INVALID_REFERENCE;
     if (!rect) return VX_ERROR_INVALID_PARAMETERS;
-    *rect = vx_rectangle_t{0, 0, image->width, image->height};
+    *rect = image->validRectSet ? image->validRect : vx_rectangle_t{0, 0, image->width, image->height};
     return VX_SUCCESS;
 }
 
@@ -776,9 +781,18 @@ VX_API_ENTRY vx_status VX_API_CALL vxCopyScalar(vx_scalar scalar, void *user_ptr
 {
     if (!is_ref(scalar, VX_TYPE_SCALAR)) return VX_ERROR_INVALID_REFERENCE;
     if (!user_ptr || user_mem_type != VX_MEMORY_TYPE_HOST) return VX_ERROR_INVALID_PARAMETERS;
-    if (usage == VX_READ_ONLY) memcpy(user_ptr, &scalar->value, scalar->item_size());
-    else if (usage == VX_WRITE_ONLY) memcpy(&scalar->value, user_ptr, scalar->item_size());
-    else return VX_ERROR_INVALID_PARAMETERS;
+    // a scalar that kernels write has a device copy, which is current while a graph runs (a CPU user
+    // node reads and writes it between GPU nodes)
+    const size_t n = scalar->item_size();
+    if (usage == VX_READ_ONLY) {
+        if (scalar->dev.ptr() && hipMemcpy(&scalar->value, scalar->dev.ptr(), n, hipMemcpyDeviceToHost) != hipSuccess) return VX_FAILURE;
+        memcpy(user_ptr, &scalar->value, n);
+    } else if (usage == VX_WRITE_ONLY) {
+        memcpy(&scalar->value, user_ptr, n);
+        if (scalar->dev.ptr() && hipMemcpy(scalar->dev.ptr(), &scalar->value, n, hipMemcpyHostToDevice) != hipSuccess) return VX_FAILURE;
+    } else {
+        return VX_ERROR_INVALID_PARAMETERS;
+    }
     return VX_SUCCESS;
 }
 

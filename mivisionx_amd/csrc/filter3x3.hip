@@ -8,7 +8,7 @@
 //
 // Rows 1..h-2 are produced, matching the reference wrappers (ago_kernel_api.cpp:15281-15282 and
 // the Sobel/Box/Median equivalents).
-#include "mvx_common.hpp"
+#include "arith.hpp"
 
 namespace mvx {
 
@@ -143,6 +143,93 @@ k_sobel3x3(uint32_t w, uint32_t h, int16_t *__restrict__ gx, uint32_t gxStride, 
     }
 }
 
+// Merged Sobel -> Magnitude (+ Phase): the kernels the reference's graph optimizer substitutes when
+// Sobel's gx and gy are virtual and read only by Magnitude / Phase (ago_drama_remove.cpp:155-182):
+// HafCpu_SobelMagnitudePhase_S16U8_U8_3x3 (ago_haf_cpu_filter.cpp:3972-4202) and
+// HafCpu_SobelMagnitude_S16_U8_3x3 (:2468-2682), rows 1..h-2.  Their 16-pixel SSE body covers columns
+// [0, w & ~15) and rounds the magnitude half to even (the Magnitude node's body rule); the scalar
+// postfix (the last w mod 16 columns) truncates.  A 16-pixel chunk starts at a multiple of 16, so it lies
+// wholly in the body or wholly in the postfix.  The phase is the Phase node's per-pixel rule.
+template <bool PHASE>
+__global__ void __launch_bounds__(F3_BX * F3_BY)
+k_sobel_magphase(uint32_t w, uint32_t h, int16_t *__restrict__ mag, uint32_t magStride, uint8_t *__restrict__ phase,
+                 uint32_t phaseStride, const uint8_t *__restrict__ src, uint32_t srcStride)
+{
+    const uint32_t cx = blockIdx.x * F3_BX + threadIdx.x;
+    const uint32_t x0 = cx * 16;
+    if (x0 >= w) return;
+    const int y0 = 1 + (int)(blockIdx.y * F3_BY + threadIdx.y) * F3_ROWS;
+    const int y1 = min(y0 + F3_ROWS, (int)h - 1);
+    if (y0 >= y1) return;
+    const long n = (long)h * srcStride;
+    const int nv = min(16, (int)(w - x0));
+    const bool tail = x0 >= (w & ~15u);
+
+    Row18 ra, rb, rc;
+    load_row18(src, n, (long)(y0 - 1) * srcStride + x0, ra);
+    load_row18(src, n, (long)y0 * srcStride + x0, rb);
+    for (int y = y0; y < y1; y++) {
+        load_row18(src, n, (long)(y + 1) * srcStride + x0, rc);
+        int32_t om[16];
+        uint32_t op[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int dxa = (int)ra.p[i + 2] - (int)ra.p[i];
+            const int dxb = (int)rb.p[i + 2] - (int)rb.p[i];
+            const int dxc = (int)rc.p[i + 2] - (int)rc.p[i];
+            const int ha = (int)(ra.p[i] + 2u * ra.p[i + 1] + ra.p[i + 2]);
+            const int hc = (int)(rc.p[i] + 2u * rc.p[i + 1] + rc.p[i + 2]);
+            const int gx = dxa + 2 * dxb + dxc, gy = hc - ha;
+            om[i] = tail ? mag_trunc(gx, gy) : mag_body(gx, gy);
+            if (PHASE) op[i] = phase_px(gx, gy);
+        }
+        st16s16_partial((int16_t *)((uint8_t *)mag + (long)y * magStride) + x0, om, nv);
+        if (PHASE) st16_partial(phase + (long)y * phaseStride + x0, op, nv);
+        ra = rb;
+        rb = rc;
+    }
+}
+
+// Sobel gradient at (x, y) from linear-address 3x3 reads (the zero guard outside [0, n))
+MVX_DEV void grad_at(const uint8_t *src, long n, long stride, int x, int y, int &gx, int &gy)
+{
+    const long bm = (long)(y - 1) * stride + x, b0 = (long)y * stride + x, bp = (long)(y + 1) * stride + x;
+    const int a0 = (int)ld_u8(src, n, bm - 1), a1 = (int)ld_u8(src, n, bm), a2 = (int)ld_u8(src, n, bm + 1);
+    const int c0 = (int)ld_u8(src, n, bp - 1), c1 = (int)ld_u8(src, n, bp), c2 = (int)ld_u8(src, n, bp + 1);
+    gx = (a2 - a0) + 2 * ((int)ld_u8(src, n, b0 + 1) - (int)ld_u8(src, n, b0 - 1)) + (c2 - c0);
+    gy = (c0 + 2 * c1 + c2) - (a0 + 2 * a1 + a2);
+}
+
+// HafCpu_SobelPhase_U8_U8_3x3 (ago_haf_cpu_filter.cpp:3946-3970): Sobel GXY into the node's scratch and
+// Phase from it, with scratch rows ALIGN16(w) BYTES apart although a row holds w int16 values.  Rows
+// are written top to bottom, so each overwrites the second half of the previous one, and Phase reads
+// them back at the same spacing: output pixel (x, y) takes the gradient of (x - ALIGN16(w)/2, y + 1)
+// when x >= ALIGN16(w)/2 and y + 1 <= h - 2, and its own gradient otherwise.  No scratch is used here:
+// each pixel evaluates the gradient the last writer of its scratch cell computed.  One thread per 16
+// pixels of a row.
+__global__ void __launch_bounds__(256)
+k_sobel_phase_merged(uint32_t w, uint32_t h, uint8_t *__restrict__ phase, uint32_t phaseStride, const uint8_t *__restrict__ src,
+                     uint32_t srcStride)
+{
+    const uint32_t x0 = (blockIdx.x * 16 + threadIdx.x) * 16;
+    const int y = 1 + (int)(blockIdx.y * 16 + threadIdx.y);
+    if (x0 >= w || y > (int)h - 2) return;
+    const long n = (long)h * srcStride;
+    const int half = (int)(((w + 15u) & ~15u) >> 1);
+    const int nv = min(16, (int)(w - x0));
+    uint32_t o[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int x = (int)x0 + i;
+        int sx = x, sy = y;
+        if (x >= half && y + 1 <= (int)h - 2) { sx = x - half; sy = y + 1; }
+        int gx = 0, gy = 0;
+        if (i < nv) grad_at(src, n, srcStride, sx, sy, gx, gy);
+        o[i] = phase_px(gx, gy);
+    }
+    st16_partial(phase + (long)y * phaseStride + x0, o, nv);
+}
+
 // stride must hold the row (ALIGN16 of the byte width) and keep 16-byte row alignment
 inline bool plane_ok(const void *p, uint32_t w, uint32_t stride, uint32_t bpp)
 {
@@ -198,6 +285,47 @@ extern "C" int mvx_Sobel_S16S16_U8_3x3_GXY(mvx_stream_t stream, uint32_t w, uint
     uint32_t strips = (h - 2 + F3_ROWS - 1) / F3_ROWS;
     dim3 grid((chunks + F3_BX - 1) / F3_BX, (strips + F3_BY - 1) / F3_BY);
     hipLaunchKernelGGL(k_sobel3x3, grid, dim3(F3_BX, F3_BY), 0, stream_of(stream), w, h, gx, gxStride, gy, gyStride, src, srcStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_SobelMagnitudePhase_S16U8_U8_3x3(mvx_stream_t stream, uint32_t w, uint32_t h, int16_t *mag, uint32_t magStride,
+                                                    uint8_t *phase, uint32_t phaseStride, const uint8_t *src, uint32_t srcStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok(mag, w, magStride, 2) || (phase && !plane_ok(phase, w, phaseStride, 1)) || !plane_ok(src, w, srcStride, 1))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    if (h < 3) return MVX_SUCCESS;
+    uint32_t chunks = (w + 15) / 16;
+    uint32_t strips = (h - 2 + F3_ROWS - 1) / F3_ROWS;
+    dim3 grid((chunks + F3_BX - 1) / F3_BX, (strips + F3_BY - 1) / F3_BY);
+    if (phase)
+        hipLaunchKernelGGL(k_sobel_magphase<true>, grid, dim3(F3_BX, F3_BY), 0, stream_of(stream), w, h, mag, magStride, phase,
+                           phaseStride, src, srcStride);
+    else
+        hipLaunchKernelGGL(k_sobel_magphase<false>, grid, dim3(F3_BX, F3_BY), 0, stream_of(stream), w, h, mag, magStride,
+                           (uint8_t *)nullptr, 0u, src, srcStride);
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_SobelMagnitude_S16_U8_3x3(mvx_stream_t stream, uint32_t w, uint32_t h, int16_t *mag, uint32_t magStride,
+                                             const uint8_t *src, uint32_t srcStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!mag) return MVX_ERROR_INVALID_PARAMETERS;
+    return mvx_SobelMagnitudePhase_S16U8_U8_3x3(stream, w, h, mag, magStride, nullptr, 0, src, srcStride);
+}
+
+extern "C" int mvx_SobelPhase_U8_U8_3x3(mvx_stream_t stream, uint32_t w, uint32_t h, uint8_t *phase, uint32_t phaseStride,
+                                        const uint8_t *src, uint32_t srcStride)
+{
+    if (w == 0 || h == 0) return MVX_SUCCESS;
+    if (!plane_ok(phase, w, phaseStride, 1) || !plane_ok(src, w, srcStride, 1)) return MVX_ERROR_INVALID_PARAMETERS;
+    if (h < 3) return MVX_SUCCESS;
+    const uint32_t chunks = (w + 15) / 16;
+    dim3 grid((chunks + 15) / 16, (h - 2 + 15) / 16);
+    hipLaunchKernelGGL(k_sobel_phase_merged, grid, dim3(16, 16), 0, stream_of(stream), w, h, phase, phaseStride, src, srcStride);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }

@@ -795,60 +795,107 @@ extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_p
     return MVX_SUCCESS;
 }
 
-namespace mvx {
-
-// The stream the pyramid levels 2.. and the Harris pick run on beside the hysteresis stages, with the
-// fork and join events: one set per device, default priority, created on first use (outside any
-// capture, normally) and kept until the process ends.  The mutex covers the enqueue sequence, so
-// calls from several host threads reuse the events safely.
-struct SideLane {
+// A lane: the stream the pyramid levels 2.. and the Harris pick run on beside the hysteresis stages,
+// with its fork and join events, all at the default priority.  Each engine graph owns one
+// (mvx_VisionGraph_lane_create at verification, destroyed with the graph), so concurrent graphs never
+// share a side stream; mvx_VisionGraph_run uses one shared lane per device, created on first use and
+// kept until the process ends, whose mutex orders the callers that share it.
+struct mvx_vision_lane {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    bool ok = false, tried = false;
+    std::mutex mu;
 };
-static std::mutex g_sideMu;
-static SideLane g_side[64];
 
-static SideLane *side_lane()
+namespace mvx {
+
+static int lane_init(mvx_vision_lane *L)
+{
+    if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) return MVX_FAILURE;
+    if (hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) != hipSuccess) return MVX_FAILURE;
+    if (hipEventCreateWithFlags(&L->join, hipEventDisableTiming) != hipSuccess) return MVX_FAILURE;
+    return MVX_SUCCESS;
+}
+
+static void lane_free(mvx_vision_lane *L)
+{
+    if (L->join) (void)hipEventDestroy(L->join);
+    if (L->fork) (void)hipEventDestroy(L->fork);
+    if (L->stream) (void)hipStreamDestroy(L->stream);
+    L->join = L->fork = nullptr;
+    L->stream = nullptr;
+}
+
+static std::mutex g_sharedMu;
+static mvx_vision_lane *g_shared[64];
+static bool g_sharedTried[64];
+
+static mvx_vision_lane *shared_lane()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    SideLane &L = g_side[dev];
-    if (!L.tried) {
-        L.tried = true;
-        L.ok = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
-               hipEventCreateWithFlags(&L.fork, hipEventDisableTiming) == hipSuccess &&
-               hipEventCreateWithFlags(&L.join, hipEventDisableTiming) == hipSuccess;
+    std::lock_guard<std::mutex> lk(g_sharedMu);
+    if (!g_sharedTried[dev]) {
+        g_sharedTried[dev] = true;
+        mvx_vision_lane *L = new mvx_vision_lane();
+        if (lane_init(L) == MVX_SUCCESS) g_shared[dev] = L;
+        else { lane_free(L); delete L; }
     }
-    return L.ok ? &L : nullptr;
+    return g_shared[dev];
 }
 
 }  // namespace mvx
 
-// Stage 0 on the caller's stream; then stages 3 and 4 (which need only stage 0) on the side stream
-// while stages 1 and 2 run on the caller's stream; the caller's stream waits for the side stream at
-// the end, so the call orders like a single-stream sequence.  Fork and join are event waits, which a
-// hipGraph capture of the caller's stream records as dependencies.
-extern "C" int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *p, uint32_t nframes, void *workspace,
-                                   size_t workspaceSize)
+extern "C" int mvx_VisionGraph_lane_create(mvx_vision_lane_t **lane)
+{
+    if (!lane) return MVX_ERROR_INVALID_PARAMETERS;
+    *lane = nullptr;
+    mvx_vision_lane *L = new mvx_vision_lane();
+    const int rc = lane_init(L);
+    if (rc) { lane_free(L); delete L; return rc; }
+    *lane = L;
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_VisionGraph_lane_destroy(mvx_vision_lane_t *lane)
+{
+    if (!lane) return MVX_SUCCESS;
+    (void)hipStreamSynchronize(lane->stream);
+    lane_free(lane);
+    delete lane;
+    return MVX_SUCCESS;
+}
+
+// Stage 0 on the caller's stream; then stages 3 and 4 (which need only stage 0) on the lane's stream
+// while stages 1 and 2 run on the caller's stream; the caller's stream waits for the lane at the end,
+// so the call orders like a single-stream sequence.  Fork and join are event waits, which a hipGraph
+// capture of the caller's stream records as dependencies.  Once forked, the join is recorded and
+// waited for on every path, errors included, so a captured caller stream is never left forked.
+extern "C" int mvx_VisionGraph_run_lane(mvx_stream_t stream, mvx_vision_lane_t *lane, const mvx_vision_params_t *p,
+                                        uint32_t nframes, void *workspace, size_t workspaceSize)
 {
     int rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 0);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lock(g_sideMu);
-    SideLane *L = side_lane();
-    if (!L) {
+    if (!lane) {
         for (int st = 1; st < MVX_VISION_STAGES && !rc; st++)
             rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, st);
         return rc;
     }
+    std::lock_guard<std::mutex> lock(lane->mu);
     hipStream_t main = stream_of(stream);
-    const mvx_stream_t side = reinterpret_cast<mvx_stream_t>(L->stream);
-    if (hipEventRecord(L->fork, main) != hipSuccess || hipStreamWaitEvent(L->stream, L->fork, 0) != hipSuccess)
+    const mvx_stream_t side = reinterpret_cast<mvx_stream_t>(lane->stream);
+    if (hipEventRecord(lane->fork, main) != hipSuccess || hipStreamWaitEvent(lane->stream, lane->fork, 0) != hipSuccess)
         return MVX_FAILURE;
-    if ((rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 3))) return rc;
-    if ((rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 4))) return rc;
-    if (hipEventRecord(L->join, L->stream) != hipSuccess) return MVX_FAILURE;
-    if ((rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 1))) return rc;
-    if ((rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 2))) return rc;
-    return hipStreamWaitEvent(main, L->join, 0) == hipSuccess ? MVX_SUCCESS : MVX_FAILURE;
+    rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 3);
+    if (!rc) rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 4);
+    const bool joined = hipEventRecord(lane->join, lane->stream) == hipSuccess;
+    if (!rc) rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 1);
+    if (!rc) rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 2);
+    if (!joined || hipStreamWaitEvent(main, lane->join, 0) != hipSuccess) return rc ? rc : MVX_FAILURE;
+    return rc;
+}
+
+extern "C" int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *p, uint32_t nframes, void *workspace,
+                                   size_t workspaceSize)
+{
+    return mvx_VisionGraph_run_lane(stream, shared_lane(), p, nframes, workspace, workspaceSize);
 }

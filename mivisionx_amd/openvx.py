@@ -46,6 +46,7 @@ BPP = {"U8": 1, "S16": 2, "U16": 2, "RGB": 3, "RGBX": 4}
 NPT = {"U8": np.uint8, "S16": np.int16, "U16": np.uint16, "RGB": np.uint8, "RGBX": np.uint8}
 
 VX_TYPE_INT32, VX_TYPE_FLOAT32, VX_TYPE_ENUM, VX_TYPE_SIZE = 0x006, 0x00A, 0x00C, 0x00D
+VX_TYPE_UINT32 = 0x007
 VX_TYPE_KEYPOINT = 0x021
 VX_TYPE_COORDINATES2DF = 0x024
 VX_READ_ONLY = _enum(0, 0x11) + 1
@@ -99,6 +100,7 @@ VX_NODE_ATTRIBUTE_AMD_AFFINITY = _attr(0x00D, VX_TYPE_NODE) + 1
 VX_IMAGE_ATTRIBUTE_AMD_HIP_BUFFER = _attr(0x00D, VX_TYPE_IMAGE) + 6
 AGO_TARGET_AFFINITY_CPU, AGO_TARGET_AFFINITY_GPU = 0x10, 0x20
 AGO_GRAPH_OPTIMIZER_FLAG_NO_SUPERNODE_MERGE = 0x20
+AGO_GRAPH_OPTIMIZER_FLAG_NO_NODE_MERGE = 0x08
 
 KEYPOINT = np.dtype([("x", np.int32), ("y", np.int32), ("strength", np.float32), ("scale", np.float32),
                      ("orientation", np.float32), ("tracking_status", np.int32), ("error", np.float32)])
@@ -266,13 +268,15 @@ def read_image(img, fmt=None, plane=0):
 
 
 def scalar(ctx, vx_type, value):
-    ct = {VX_TYPE_INT32: c.c_int32, VX_TYPE_FLOAT32: c.c_float, VX_TYPE_ENUM: c.c_int32, VX_TYPE_SIZE: c.c_size_t}[vx_type]
+    ct = {VX_TYPE_INT32: c.c_int32, VX_TYPE_UINT32: c.c_uint32, VX_TYPE_FLOAT32: c.c_float, VX_TYPE_ENUM: c.c_int32,
+          VX_TYPE_SIZE: c.c_size_t}[vx_type]
     v = ct(value)
     return _ok(lib().vxCreateScalar(ctx, vx_type, c.byref(v)), "vxCreateScalar")
 
 
 def read_scalar(s, vx_type):
-    ct = {VX_TYPE_INT32: c.c_int32, VX_TYPE_FLOAT32: c.c_float, VX_TYPE_ENUM: c.c_int32, VX_TYPE_SIZE: c.c_size_t}[vx_type]
+    ct = {VX_TYPE_INT32: c.c_int32, VX_TYPE_UINT32: c.c_uint32, VX_TYPE_FLOAT32: c.c_float, VX_TYPE_ENUM: c.c_int32,
+          VX_TYPE_SIZE: c.c_size_t}[vx_type]
     v = ct()
     check(lib().vxCopyScalar(s, c.byref(v), VX_READ_ONLY, VX_MEMORY_TYPE_HOST), "vxCopyScalar")
     return v.value

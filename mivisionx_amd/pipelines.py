@@ -268,18 +268,23 @@ class ChainBatch:
     device planes of uneven sizes: one persistent launch (mvx_ChainBatch_*), or the same work as
     plain per-node launches, image after image (``run_per_image``)."""
 
-    def __init__(self, sources, with_half=True):
+    def __init__(self, sources, with_half=True, virtual_gradients=False):
+        """virtual_gradients: gx / gy are not stored and each image takes the merged SobelMagnitudePhase
+        rules, as the reference's node-merge rules do when the gradients are virtual"""
         K = kernels()
         K.mvx_ChainBatch_workspace_size.restype = ctypes.c_size_t
         self.src = list(sources)
+        self.virtual_gradients = virtual_gradients
         self.gx, self.gy, self.mag, self.phase, self.half, self.blur = [], [], [], [], [], []
         arr = (ChainImage * len(self.src))()
         for i, s in enumerate(self.src):
             w, h = s.width, s.height
-            gx, gy, mag, ph = Plane(w, h, "S16"), Plane(w, h, "S16"), Plane(w, h, "S16"), Plane(w, h, "U8")
+            gx, gy = (None, None) if virtual_gradients else (Plane(w, h, "S16"), Plane(w, h, "S16"))
+            mag, ph = Plane(w, h, "S16"), Plane(w, h, "U8")
             hf = Plane(w // 2, h // 2, "U8") if with_half else None
             self.gx.append(gx); self.gy.append(gy); self.mag.append(mag); self.phase.append(ph); self.half.append(hf)
-            arr[i] = ChainImage(s.t.data_ptr(), w, h, s.stride, gx.t.data_ptr(), gx.stride, gy.t.data_ptr(), gy.stride,
+            arr[i] = ChainImage(s.t.data_ptr(), w, h, s.stride, gx.t.data_ptr() if gx else None, gx.stride if gx else 0,
+                                gy.t.data_ptr() if gy else None, gy.stride if gy else 0,
                                 mag.t.data_ptr(), mag.stride, ph.t.data_ptr(), ph.stride,
                                 hf.t.data_ptr() if hf else None, hf.stride if hf else 0)
         self._arr = arr
@@ -336,6 +341,8 @@ class ChainBatch:
 
     def run_per_image(self):
         """the reference executor's model: one launch per node and image"""
+        if self.virtual_gradients:
+            raise NotImplementedError("per-image launches store the gradients")
         if not self.blur:
             self.blur = [Plane(s.width, s.height, "U8") for s in self.src]
         for s, b, gx, gy, mag, ph, hf in zip(self.src, self.blur, self.gx, self.gy, self.mag, self.phase, self.half):
@@ -354,8 +361,9 @@ class ChainBatch:
                     p.t.fill_(0)
 
     def results(self, i):
-        out = dict(gx=self.gx[i].download(), gy=self.gy[i].download(), mag=self.mag[i].download(),
-                   phase=self.phase[i].download())
+        out = dict(mag=self.mag[i].download(), phase=self.phase[i].download())
+        if not self.virtual_gradients:
+            out.update(gx=self.gx[i].download(), gy=self.gy[i].download())
         if self.half[i] is not None:
             out["half"] = self.half[i].download()
         return out

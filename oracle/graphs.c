@@ -197,3 +197,23 @@ int oracle_harris_node(int w, int h, const uint8_t *src, float k, float thr, flo
     free(list); free(in.p); free(hg3.p); free(vc.p);
     return count;
 }
+
+/* [Gaussian3x3 (virtual) ->] Sobel3x3 (virtual gx, gy) -> Magnitude and/or Phase: the kernel the
+ * merge rules choose (ago_drama_remove.cpp:155-182) on fresh outputs.  which: 0 Magnitude + Phase
+ * (SOBEL_MAGNITUDE_PHASE_S16U8_U8_3x3), 1 Magnitude only (SOBEL_MAGNITUDE_S16_U8_3x3), 2 Phase only
+ * (SOBEL_PHASE_U8_U8_3x3). */
+void oracle_sobel_merged_graph(int which, int blur, int w, int h, const uint8_t *src, int16_t *mag, uint8_t *phase)
+{
+    plane_t in = plane_new(w, h, 1), g = plane_new(w, h, 1), om = plane_new(w, h, 2), op = plane_new(w, h, 1);
+    plane_put(&in, w, h, 1, src);
+    const plane_t *s = &in;
+    if (blur) {
+        oracle_gaussian3x3(w, h, g.p, g.stride, in.p, in.stride);
+        s = &g;
+    }
+    if (which == 2) oracle_sobel_phase_merged(w, h, op.p, op.stride, s->p, s->stride);
+    else oracle_sobel_magnitude_phase(w, h, (int16_t *)om.p, om.stride, which == 0 ? op.p : NULL, op.stride, s->p, s->stride);
+    if (mag && which != 2) plane_get(&om, w, h, 2, mag);
+    if (phase && which != 1) plane_get(&op, w, h, 1, phase);
+    free(in.p); free(g.p); free(om.p); free(op.p);
+}

@@ -35,6 +35,12 @@ void oracle_median3x3(int w, int h, uint8_t *dst, int dst_stride, const uint8_t 
 void oracle_sobel3x3(int w, int h, int16_t *gx, int gx_stride, int16_t *gy, int gy_stride,
                      const uint8_t *src, int src_stride);
 
+/* merged Sobel -> Magnitude / Phase (virtual gx, gy); mag or phase may be NULL */
+void oracle_sobel_magnitude_phase(int w, int h, int16_t *mag, int mag_stride, uint8_t *phase, int phase_stride,
+                                  const uint8_t *src, int src_stride);
+void oracle_sobel_phase_merged(int w, int h, uint8_t *phase, int phase_stride, const uint8_t *src, int src_stride);
+int16_t oracle_merged_magnitude_px(int gx, int gy, int tail);
+
 /* ---- arithmetic (ago_haf_cpu_arithmetic.cpp) ---- */
 void oracle_magnitude(int w, int h, int16_t *mag, int mag_stride, const int16_t *gx, int gx_stride,
                       const int16_t *gy, int gy_stride);
@@ -116,6 +122,9 @@ int oracle_lk_run(void *state_mem, int levels, float pyr_scale, const uint8_t *c
 /* ---- nodes and graphs on reference-layout planes (graphs.c); packed in/out ---- */
 void oracle_filter3x3(int which, int w, int h, const uint8_t *src, uint8_t *dst);
 void oracle_sobel_node(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy);
+/* [Gaussian3x3 (virtual) ->] Sobel3x3 (virtual gx, gy) -> Magnitude and/or Phase, as the merge
+ * rules run it; which: 0 SobelMagnitudePhase, 1 SobelMagnitude, 2 SobelPhase */
+void oracle_sobel_merged_graph(int which, int blur, int w, int h, const uint8_t *src, int16_t *mag, uint8_t *phase);
 /* Gaussian3x3 (virtual) -> Sobel3x3 -> Magnitude + Phase */
 void oracle_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16_t *mag,
                    uint8_t *phase);

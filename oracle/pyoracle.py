@@ -76,6 +76,7 @@ class Oracle:
         L = self.lib = ctypes.CDLL(ORACLE_SO)
         L.oracle_magnitude_px.restype = ctypes.c_int16
         L.oracle_phase_px.restype = ctypes.c_uint8
+        L.oracle_merged_magnitude_px.restype = ctypes.c_int16
         L.oracle_harris_norm.restype = ctypes.c_float
         L.oracle_harris_norm_gb.restype = ctypes.c_float
 
@@ -111,6 +112,18 @@ class Oracle:
         mag = np.zeros((h, w), np.int16); ph = np.zeros((h, w), np.uint8)
         self.lib.oracle_chain1(w, h, _p(_c(src)), _p(gx), _p(gy), _p(mag), _p(ph))
         return gx, gy, mag, ph
+
+    def sobel_merged(self, src, which=0, blur=False):
+        """[Gaussian3x3 (virtual) ->] Sobel3x3 (virtual gx, gy) -> Magnitude / Phase as the merge rules run it.
+        which: 0 magnitude + phase, 1 magnitude only, 2 phase only.  Returns (mag, phase) with None for the
+        output the graph does not have."""
+        h, w = src.shape
+        mag = np.zeros((h, w), np.int16); ph = np.zeros((h, w), np.uint8)
+        self.lib.oracle_sobel_merged_graph(which, 1 if blur else 0, w, h, _p(_c(src)), _p(mag), _p(ph))
+        return (mag if which != 2 else None), (ph if which != 1 else None)
+
+    def merged_magnitude_px(self, gx, gy, tail):
+        return int(self.lib.oracle_merged_magnitude_px(gx, gy, 1 if tail else 0))
 
     # ---- colour, scale, pyramid ----
     def color_y(self, rgb):
@@ -483,6 +496,12 @@ class Reference:
                                          time_iters, ctypes.byref(sec)), "chain_half")
         out = dict(gx=gx, gy=gy, mag=mag, phase=ph, half=half)
         return (out, sec.value) if time_iters else out
+
+    def sobel_merged(self, src, which=0, blur=False):
+        h, w = src.shape
+        mag = np.zeros((h, w), np.int16); ph = np.zeros((h, w), np.uint8)
+        self._ok(self.lib.ref_sobel_merged(which, 1 if blur else 0, w, h, _p(_c(src)), _p(mag), _p(ph)), "sobel_merged")
+        return (mag if which != 2 else None), (ph if which != 1 else None)
 
     def color_y(self, rgb):
         h, w = rgb.shape[:2]

@@ -163,6 +163,30 @@ int ref_chain1(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16
     return 0;
 }
 
+// [Gaussian3x3 (virtual) ->] Sobel3x3 with VIRTUAL gx and gy -> Magnitude and/or Phase, so that the
+// reference's merge rules (ago_drama_remove.cpp:155-182) pick SobelMagnitudePhase / SobelMagnitude /
+// SobelPhase.  which: 0 magnitude + phase, 1 magnitude only, 2 phase only.
+int ref_sobel_merged(int which, int blur, int w, int h, const uint8_t *src, int16_t *mag, uint8_t *phase) {
+    Ctx c;
+    vx_image in = make_input(c, w, h, VX_DF_IMAGE_U8, src);
+    vx_image s = in;
+    if (blur) {
+        s = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_U8);
+        if (!vxGaussian3x3Node(c.graph, in, s)) return -1;
+    }
+    vx_image gx = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_S16);
+    vx_image gy = vxCreateVirtualImage(c.graph, w, h, VX_DF_IMAGE_S16);
+    vx_image om = which != 2 ? vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_S16) : nullptr;
+    vx_image op = which != 1 ? vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8) : nullptr;
+    if (!vxSobel3x3Node(c.graph, s, gx, gy)) return -1;
+    if (om && !vxMagnitudeNode(c.graph, gx, gy, om)) return -1;
+    if (op && !vxPhaseNode(c.graph, gx, gy, op)) return -1;
+    int st = run(c); if (st) return st;
+    if (om) CHECK(get(om, w, h, mag, 2));
+    if (op) CHECK(get(op, w, h, phase, 1));
+    return 0;
+}
+
 // The uneven-batch chain on one image: Gaussian3x3 (virtual) -> Sobel3x3 -> Magnitude + Phase and
 // ScaleImage bilinear of the input to (w/2, h/2), one graph; timing as ref_chain1.
 int ref_chain_half(int w, int h, const uint8_t *src, int16_t *gx, int16_t *gy, int16_t *mag, uint8_t *phase, uint8_t *half,

@@ -190,3 +190,104 @@ def test_runvx_md5_matches_rfc1321(tmp_path):
            ["x" * n for n in (55, 56, 57, 63, 64, 65, 119, 120, 128, 1000)]
     out = subprocess.run([str(exe)] + msgs, capture_output=True, check=True).stdout.decode().split()
     assert out == [hashlib.md5(m.encode()).hexdigest() for m in msgs]
+
+
+def _verify_log(vx, text, flags=None):
+    """vxVerifyGraph of a GDF with the log captured; the optimizer's choices (merges, supernodes) are logged
+    before any device work, so they are visible without a GPU"""
+    lib = vx.lib()
+    ctx = lib.vxCreateContext()
+    messages = []
+    cb = vx.LOG_CB(lambda c_, ref, status, t: messages.append(t.decode()))
+    try:
+        lib.vxRegisterLogCallback(ctx, cb, 0)
+        g = lib.vxCreateGraph(ctx)
+        if flags is not None:
+            f = ctypes.c_uint32(flags)
+            assert lib.vxSetGraphAttribute(g, vx.VX_GRAPH_ATTRIBUTE_AMD_OPTIMIZER_FLAGS, ctypes.byref(f), 4) == 0
+        assert vx.import_gdf(g, text) == 0
+        status = lib.vxVerifyGraph(g)
+        return status, "".join(messages)
+    finally:
+        h = ctypes.c_void_p(ctx)
+        lib.vxReleaseContext(ctypes.byref(h))
+
+
+SOBEL_VIRTUAL = """data a = image:333,64,U008
+data gx = image-virtual:333,64,S016
+data gy = image-virtual:333,64,S016
+data m = image:333,64,S016
+data p = image:333,64,U008
+node org.khronos.openvx.sobel_3x3 a gx gy
+"""
+
+
+def test_sobel_merge_rules_select_the_merged_kernels(vx):
+    """ago_drama_remove.cpp:155-182: virtual gx / gy read only by Magnitude and/or Phase merge into one kernel"""
+    both = SOBEL_VIRTUAL + "node org.khronos.openvx.magnitude gx gy m\nnode org.khronos.openvx.phase gx gy p\n"
+    _, log = _verify_log(vx, both)
+    assert "into SobelMagnitudePhase_S16U8_U8_3x3" in log, log
+    _, log = _verify_log(vx, SOBEL_VIRTUAL + "node org.khronos.openvx.magnitude gx gy m\n")
+    assert "into SobelMagnitude_S16_U8_3x3" in log, log
+    _, log = _verify_log(vx, SOBEL_VIRTUAL + "node org.khronos.openvx.phase gx gy p\n")
+    assert "into SobelPhase_U8_U8_3x3" in log, log
+    # the flag that turns node merging off
+    _, log = _verify_log(vx, both, flags=0x08)
+    assert "merged" not in log, log
+    # gy read by another node: the gradients are not solitary, nothing merges
+    extra = both + "data m2 = image:333,64,S016\nnode org.khronos.openvx.magnitude gy gx m2\n"
+    _, log = _verify_log(vx, extra)
+    assert "merged" not in log, log
+    # non-virtual gradients never merge
+    _, log = _verify_log(vx, both.replace("image-virtual:333,64,S016", "image:333,64,S016"))
+    assert "merged" not in log, log
+
+
+VISION = """data rgb = image:640,480,RGB2
+data yuv = image-virtual:640,480,IYUV
+data luma = image-virtual:0,0,VIRT
+data scaled = image:320,240,U008
+data pyr = pyramid:4,HALF,640,480,U008
+data edges = image:640,480,U008
+data hyst = threshold:RANGE,U008,U008:INIT,80,100
+data grad = scalar:INT32,3
+data strength = scalar:FLOAT32,0.0005
+data mind = scalar:FLOAT32,5.0
+data sens = scalar:FLOAT32,0.04
+data corners = array:KEYPOINT,1000
+data count = scalar:SIZE,0
+node org.khronos.openvx.color_convert rgb yuv
+node org.khronos.openvx.channel_extract yuv !CHANNEL_{ch} luma
+node org.khronos.openvx.scale_image luma scaled !BILINEAR
+node org.khronos.openvx.gaussian_pyramid luma pyr
+node org.khronos.openvx.canny_edge_detector luma hyst grad !NORM_L1 edges
+node org.khronos.openvx.harris_corners luma strength mind sens grad grad corners count
+"""
+
+
+def test_vision_supernode_takes_the_luma_channel_only(vx):
+    _, log = _verify_log(vx, VISION.format(ch="Y"))
+    assert "fused vision supernode" in log, log
+    # channel U of IYUV is a quarter-size plane: not the supernode's luma (its consumers then fail their size checks)
+    _, log = _verify_log(vx, VISION.format(ch="U").replace("pyramid:4,HALF,640,480", "pyramid:4,HALF,320,240")
+                         .replace("image:640,480,U008\ndata hyst", "image:320,240,U008\ndata hyst")
+                         .replace("image:320,240,U008\ndata pyr", "image:160,120,U008\ndata pyr"))
+    assert "fused vision supernode" not in log, log
+
+
+def test_scheduled_graph_refuses_changes(vx):
+    """while a vxScheduleGraph worker may run the graph, setters that would free its device state are refused"""
+    lib = vx.lib()
+    ctx = lib.vxCreateContext()
+    try:
+        g = lib.vxCreateGraph(ctx)
+        assert vx.import_gdf(g, "data a = image:64,64,U008\ndata b = image:64,64,U008\n"
+                                "node org.khronos.openvx.gaussian_3x3 a b\n") == 0
+        assert lib.vxScheduleGraph(g) == 0
+        f = ctypes.c_uint32(0)
+        assert lib.vxSetGraphAttribute(g, vx.VX_GRAPH_ATTRIBUTE_AMD_OPTIMIZER_FLAGS, ctypes.byref(f), 4) == -21  # VX_ERROR_GRAPH_SCHEDULED
+        lib.vxWaitGraph(g)   # the worker's status (no GPU here: a verification failure) is not the point
+        assert lib.vxSetGraphAttribute(g, vx.VX_GRAPH_ATTRIBUTE_AMD_OPTIMIZER_FLAGS, ctypes.byref(f), 4) == 0
+    finally:
+        h = ctypes.c_void_p(ctx)
+        lib.vxReleaseContext(ctypes.byref(h))

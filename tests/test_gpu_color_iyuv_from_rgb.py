@@ -45,3 +45,32 @@ def test_iyuv_from_rgb_node(gpu, orc):
     finally:
         cc = ctypes.c_void_p(ctx)
         L.vxReleaseContext(ctypes.byref(cc))
+
+
+@pytest.mark.parametrize("readers", ["YUV", "UV", "Y"])
+def test_virtual_iyuv_chroma_readers_get_chroma(gpu, orc, readers):
+    """RGB -> virtual IYUV -> channel_extract: the chroma kernels run whenever a U or V channel is read
+    (the reference divides RGB -> IYUV into Y, IU and IV kernels and keeps the ones whose outputs are read)"""
+    from mivisionx_amd import openvx as vx
+    L = vx.lib()
+    ctx = L.vxCreateContext()
+    try:
+        w, h = 334, 120
+        rgb = _rgb(w, h, 9)
+        g = L.vxCreateGraph(ctx)
+        src = vx.create_image(ctx, w, h, "RGB")
+        vx.write_image(src, rgb)
+        yuv = L.vxCreateVirtualImage(g, w, h, vx.DF["IYUV"])
+        assert L.vxColorConvertNode(g, src, yuv)
+        want = dict(zip("YUV", orc.color_iyuv_from_rgb(rgb)))
+        chan = dict(Y=vx.VX_CHANNEL_Y, U=vx.VX_CHANNEL_U, V=vx.VX_CHANNEL_V)
+        outs = {}
+        for k in readers:
+            outs[k] = vx.create_image(ctx, want[k].shape[1], want[k].shape[0], "U8")
+            assert L.vxChannelExtractNode(g, yuv, chan[k], outs[k])
+        assert L.vxProcessGraph(g) == 0
+        for k, o in outs.items():
+            assert_same(vx.read_image(o), want[k], f"virtual IYUV channel {k} (readers {readers})")
+    finally:
+        cc = ctypes.c_void_p(ctx)
+        L.vxReleaseContext(ctypes.byref(cc))
