@@ -398,6 +398,30 @@ int mvx_Remap_U24_U24_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, 
 int mvx_StitchMerge_RGB(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pDst, uint32_t dstStride, uint32_t count,
                         const uint8_t *const *pImages, const uint32_t *imageStrides, const uint8_t *const *pWeights,
                         const uint32_t *weightStrides);
+/* Loom's merge with its camera-selection maps (amd_loomsl/kernels/merge.cpp:176-385; OpenCL code
+ * generation only in the reference, so there is no CPU or HIP form to replace).  Per group of 8 output
+ * pixels camId0 (U8, one per group) holds a camera id (< 31: copy that camera), 31 (no camera: the
+ * pixels are not written) or a blend code (> 31: the cameras listed as 5-bit ids in camId1 / camId2, U16
+ * per group; 2 cameras up to 128, then one more for each step above 128), weighted by
+ * weight * 0.003922f (the factor Loom's generated source prints with "%f"), summed in order and packed to
+ * U8 with round-to-nearest-even and saturation.  Camera c's remapped RGB image and U8 weights are rows
+ * c*height .. c*height+height-1 of the stacked planes (Loom stacks RGBX images the same way).  Ids at or
+ * above `count` are treated as 31. */
+int mvx_StitchMerge_Loom_RGB(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pDst, uint32_t dstStride,
+                             uint32_t count, const uint8_t *pStacked, uint32_t stackedStride, const uint8_t *pWeights,
+                             uint32_t weightStride, const uint8_t *pCamId0, uint32_t camId0Stride, const uint16_t *pCamId1,
+                             uint32_t camId1Stride, const uint16_t *pCamId2, uint32_t camId2Stride);
+/* The same stitch in one pass from the camera images themselves: camera c is sampled through its 13.3
+ * remap table (rows c*height .. of pMaps, mvx_Remap_FixedMap_F32 of its F32 map) exactly as
+ * mvx_Remap_U24_U24_Bilinear_Constant (border 0) would remap it, only where the selection needs it, so
+ * the remapped frames are never written.  pCameras / cameraStrides: host arrays of `count` device
+ * pointers and pitches of srcWidth x srcHeight RGB images.  Same bytes as the remaps followed by
+ * mvx_StitchMerge_Loom_RGB. */
+int mvx_Stitch_Loom_RGB(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pDst, uint32_t dstStride, uint32_t count,
+                        const uint8_t *const *pCameras, const uint32_t *cameraStrides, uint32_t srcWidth, uint32_t srcHeight,
+                        const void *pMaps, uint32_t mapStride, const uint8_t *pWeights, uint32_t weightStride,
+                        const uint8_t *pCamId0, uint32_t camId0Stride, const uint16_t *pCamId1, uint32_t camId1Stride,
+                        const uint16_t *pCamId2, uint32_t camId2Stride);
 /* The fixed-point remap table of vxSetRemapPoint (vx_api.cpp:7865-7894, 3 fractional bits) from a
  * device float table of {x, y} pairs, on the GPU.  Sources must be below 8192 pixels in each
  * dimension (the reference uses fewer fractional bits beyond, ago_util.cpp:2450-2459). */

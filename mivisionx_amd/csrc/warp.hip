@@ -268,14 +268,11 @@ k_remap_nn(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStri
 // (HafCpu_Remap_U8_U8_Bilinear[_Constant], ago_haf_cpu_geometric.cpp:249-504) over the same 13.3 table:
 // every read of a valid entry stays inside the image (the table marks x >= W-1 and y >= H-1 invalid),
 // so packed and planar layouts give the same bytes.  Exactly w pixels per row are written.
+// one packed-RGB output pixel of the bilinear remap from its 13.3 table entry `e`
 template <bool CONSTANT>
-__global__ void __launch_bounds__(256)
-k_remap_u24(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
-            const uint8_t *__restrict__ src, uint32_t srcStride, const uint8_t *__restrict__ map, uint32_t mapStride, uint32_t border)
+MVX_DEV void remap_u24_px(const uint8_t *__restrict__ src, uint32_t srcStride, uint32_t sw, uint32_t sh, uint32_t e, uint32_t border,
+                          int32_t (&v)[3])
 {
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= dw || y >= dh) return;
-    const uint32_t e = *reinterpret_cast<const uint32_t *>(map + (uint64_t)y * mapStride + 4ull * x);
     uint32_t mx = e & 0xffffu, my = e >> 16;
     const bool xbad = mx == 0xffffu;
     if (mx == 0xffffu) mx = CONSTANT ? 0u : 8u;
@@ -284,14 +281,28 @@ k_remap_u24(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStr
     const uint32_t xi = min(mx >> 3, sw - 1), yi = min(my >> 3, sh - 1);
     const uint32_t x1 = min(xi + 1, sw - 1), y1 = min(yi + 1, sh - 1);
     const uint8_t *r0 = src + (uint64_t)yi * srcStride, *r1 = src + (uint64_t)y1 * srcStride;
-    uint8_t *d = dst + (uint64_t)y * dstStride + 3ull * x;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int32_t p00 = r0[3 * xi + c], p10 = r0[3 * x1 + c], p01 = r1[3 * xi + c], p11 = r1[3 * x1 + c];
-        int32_t v = (p00 * (8 - fx) * (8 - fy) + p10 * fx * (8 - fy) + p01 * (8 - fx) * fy + p11 * fx * fy + 32) >> 6;
-        if (CONSTANT && xbad) v = (int32_t)border;
-        d[c] = (uint8_t)sat8(v);
+        int32_t t = (p00 * (8 - fx) * (8 - fy) + p10 * fx * (8 - fy) + p01 * (8 - fx) * fy + p11 * fx * fy + 32) >> 6;
+        if (CONSTANT && xbad) t = (int32_t)border;
+        v[c] = (int32_t)sat8(t);
     }
+}
+
+template <bool CONSTANT>
+__global__ void __launch_bounds__(256)
+k_remap_u24(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+            const uint8_t *__restrict__ src, uint32_t srcStride, const uint8_t *__restrict__ map, uint32_t mapStride, uint32_t border)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= dw || y >= dh) return;
+    const uint32_t e = *reinterpret_cast<const uint32_t *>(map + (uint64_t)y * mapStride + 4ull * x);
+    int32_t v[3];
+    remap_u24_px<CONSTANT>(src, srcStride, sw, sh, e, border, v);
+    uint8_t *d = dst + (uint64_t)y * dstStride + 3ull * x;
+#pragma unroll
+    for (int c = 0; c < 3; c++) d[c] = (uint8_t)v[c];
 }
 
 // Stitch merge: the weighted sum of the remapped camera images (the role of Loom's merge kernel,
@@ -322,6 +333,111 @@ k_stitch_merge(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstSt
     uint8_t *d = dst + (uint64_t)y * dstStride + 3ull * x;
 #pragma unroll
     for (int c = 0; c < 3; c++) d[c] = (uint8_t)min(max(__float2int_rn(acc[c]), 0), 255);
+}
+
+// Loom's merge with its camera-selection maps (amd_loomsl/kernels/merge.cpp:176-385, generated OpenCL;
+// Loom has no CPU form).  Per group of 8 output pixels of a row, camId0 (U8) selects:
+//   < 31   one camera, copied (its pixel unweighted);
+//   == 31  no camera: the output pixels are not written;
+//   > 31   a blend of the cameras whose 5-bit ids camId1 (U16: bits 0-4, 5-9, 10-14) and camId2 (U16, the
+//          same) list: the first two of camId1 always, its third when camId0 > 128, then camId2's first,
+//          second and third when camId0 > 129, > 130, > 131; an id of 31 contributes nothing.
+// A blended camera contributes w * p with w = (float)weight * 0.003922f -- the kernel source prints its
+// factor 1/255 with "%f" (merge.cpp:226,256), so that is the constant the kernel runs with -- summed in
+// list order, each product and sum rounded (OpenCL's mad taken unfused).  The sum is packed to U8 with
+// round-to-nearest-even and saturation (amd_pack, v_cvt_pk_u8_f32).  Images and weights of camera c are
+// rows c*height .. c*height + height-1 of the stacked planes.  A camera id at or above `count` (no such
+// camera) is treated as 31 here instead of reading past the stack.
+struct LoomArgs {
+    const uint8_t *camId0;
+    const uint16_t *camId1, *camId2;
+    const uint8_t *wt;
+    uint32_t camId0Stride, camId1Stride, camId2Stride, wtStride;
+    uint32_t width, height, count;
+};
+
+constexpr float LOOM_WEIGHT_FACTOR = 0.003922f;
+
+MVX_DEV uint32_t loom_pack(float v) { return (uint32_t)min(max(__float2int_rn(v), 0), 255); }
+
+// Src(c, x, y, rgb) fetches camera c's pixel (x, y) of the equirectangular frame
+template <class Src>
+MVX_DEV bool loom_merge_px(const LoomArgs &a, uint32_t x, uint32_t y, const Src &src, uint32_t (&out)[3])
+{
+    const uint32_t g = x >> 3;
+    const uint32_t sel = a.camId0[(uint64_t)y * a.camId0Stride + g];
+    if (sel == 31u) return false;
+    int32_t p[3];
+    if (sel < 31u) {
+        if (sel >= a.count) return false;
+        src(sel, x, y, p);
+        for (int c = 0; c < 3; c++) out[c] = (uint32_t)p[c];
+        return true;
+    }
+    const uint32_t s1 = *(const uint16_t *)((const uint8_t *)a.camId1 + (uint64_t)y * a.camId1Stride + 2ull * g);
+    const uint32_t s2 = sel > 129u ? *(const uint16_t *)((const uint8_t *)a.camId2 + (uint64_t)y * a.camId2Stride + 2ull * g) : 0u;
+    const int n = sel > 131u ? 6 : sel > 130u ? 5 : sel > 129u ? 4 : sel > 128u ? 3 : 2;
+    float fa[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < n; k++) {
+        const uint32_t id = ((k < 3 ? s1 : s2) >> (5 * (k % 3))) & 31u;
+        float w = 0.0f;
+        p[0] = p[1] = p[2] = 0;
+        if (id < 31u && id < a.count) {
+            src(id, x, y, p);
+            w = (float)a.wt[((uint64_t)id * a.height + y) * a.wtStride + x] * LOOM_WEIGHT_FACTOR;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) fa[c] = fa[c] + w * (float)p[c];
+    }
+    for (int c = 0; c < 3; c++) out[c] = loom_pack(fa[c]);
+    return true;
+}
+
+struct StackedSrc {   // the remapped cameras, stacked RGB planes
+    const uint8_t *img;
+    uint32_t stride, height;
+    MVX_DEV void operator()(uint32_t c, uint32_t x, uint32_t y, int32_t (&p)[3]) const
+    {
+        const uint8_t *q = img + ((uint64_t)c * height + y) * stride + 3ull * x;
+        p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+    }
+};
+
+__global__ void __launch_bounds__(256)
+k_stitch_loom_merge(uint8_t *__restrict__ dst, uint32_t dstStride, LoomArgs a, StackedSrc s)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= a.width || y >= a.height) return;
+    uint32_t o[3];
+    if (!loom_merge_px(a, x, y, s, o)) return;
+    uint8_t *d = dst + (uint64_t)y * dstStride + 3ull * x;
+    d[0] = (uint8_t)o[0]; d[1] = (uint8_t)o[1]; d[2] = (uint8_t)o[2];
+}
+
+struct RemapSrc {     // the cameras themselves, remapped on the fly (bilinear, constant border 0)
+    const uint8_t *cam[MVX_STITCH_MAX_CAMERAS];
+    uint32_t camStride[MVX_STITCH_MAX_CAMERAS];
+    const uint8_t *map;      // stacked 13.3 tables, rows c*height ..
+    uint32_t mapStride, height, sw, sh;
+    MVX_DEV void operator()(uint32_t c, uint32_t x, uint32_t y, int32_t (&p)[3]) const
+    {
+        const uint32_t e = *reinterpret_cast<const uint32_t *>(map + ((uint64_t)c * height + y) * mapStride + 4ull * x);
+        remap_u24_px<true>(cam[c], camStride[c], sw, sh, e, 0u, p);
+    }
+};
+
+// Remap and merge in one pass: each output pixel samples only the cameras its selection lists, so the
+// warped camera frames never exist in HBM.  Same bytes as k_remap_u24<CONSTANT> per camera followed by
+// k_stitch_loom_merge.
+__global__ void __launch_bounds__(256)
+k_stitch_loom_fused(uint8_t *__restrict__ dst, uint32_t dstStride, LoomArgs a, RemapSrc s)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= a.width || y >= a.height) return;
+    uint32_t o[3];
+    if (!loom_merge_px(a, x, y, s, o)) return;
+    uint8_t *d = dst + (uint64_t)y * dstStride + 3ull * x;
+    d[0] = (uint8_t)o[0]; d[1] = (uint8_t)o[1]; d[2] = (uint8_t)o[2];
 }
 
 // vxSetRemapPoint (vx_api.cpp:7865-7894) for every entry: x = (u16)(sx*8 + 0.5f), y likewise,
@@ -626,6 +742,68 @@ extern "C" int mvx_Remap_FixedMap_F32(mvx_stream_t stream, uint32_t dstWidth, ui
     const dim3 block(64, 4), grid((dstWidth + 63) / 64, (dstHeight + 3) / 4);
     hipLaunchKernelGGL(k_remap_fixed, grid, block, 0, stream_of(stream), dstWidth, dstHeight, (uint8_t *)pFixedMap, fixedStride,
                        (const uint8_t *)pFloatMap, floatStride, (float)(srcWidth - 1), (float)(srcHeight - 1));
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+namespace mvx {
+static int loom_args(LoomArgs &a, uint32_t width, uint32_t height, uint32_t count, const uint8_t *pWeights, uint32_t weightStride,
+                     const uint8_t *pCamId0, uint32_t camId0Stride, const uint16_t *pCamId1, uint32_t camId1Stride,
+                     const uint16_t *pCamId2, uint32_t camId2Stride)
+{
+    const uint32_t groups = (width + 7) / 8;
+    if (!width || !height || count == 0 || count > MVX_STITCH_MAX_CAMERAS || !pWeights || !pCamId0 || !pCamId1 || !pCamId2)
+        return MVX_ERROR_INVALID_PARAMETERS;
+    if (weightStride < width || camId0Stride < groups || camId1Stride < 2 * groups || camId2Stride < 2 * groups ||
+        ((uintptr_t)pCamId1 & 1u) || ((uintptr_t)pCamId2 & 1u) || (camId1Stride & 1u) || (camId2Stride & 1u))
+        return MVX_ERROR_INVALID_PARAMETERS;
+    a = LoomArgs{pCamId0, pCamId1, pCamId2, pWeights, camId0Stride, camId1Stride, camId2Stride, weightStride, width, height, count};
+    return MVX_SUCCESS;
+}
+}  // namespace mvx
+
+extern "C" int mvx_StitchMerge_Loom_RGB(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pDst, uint32_t dstStride,
+                                       uint32_t count, const uint8_t *pStacked, uint32_t stackedStride, const uint8_t *pWeights,
+                                       uint32_t weightStride, const uint8_t *pCamId0, uint32_t camId0Stride, const uint16_t *pCamId1,
+                                       uint32_t camId1Stride, const uint16_t *pCamId2, uint32_t camId2Stride)
+{
+    LoomArgs a;
+    int rc = loom_args(a, width, height, count, pWeights, weightStride, pCamId0, camId0Stride, pCamId1, camId1Stride, pCamId2,
+                       camId2Stride);
+    if (rc) return rc;
+    if (!pDst || !pStacked || dstStride < 3 * width || stackedStride < 3 * width) return MVX_ERROR_INVALID_PARAMETERS;
+    const dim3 block(64, 4), grid((width + 63) / 64, (height + 3) / 4);
+    hipLaunchKernelGGL(k_stitch_loom_merge, grid, block, 0, stream_of(stream), pDst, dstStride, a, StackedSrc{pStacked, stackedStride, height});
+    MVX_LAUNCH_CHECK();
+    return MVX_SUCCESS;
+}
+
+extern "C" int mvx_Stitch_Loom_RGB(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pDst, uint32_t dstStride,
+                                  uint32_t count, const uint8_t *const *pCameras, const uint32_t *cameraStrides, uint32_t srcWidth,
+                                  uint32_t srcHeight, const void *pMaps, uint32_t mapStride, const uint8_t *pWeights,
+                                  uint32_t weightStride, const uint8_t *pCamId0, uint32_t camId0Stride, const uint16_t *pCamId1,
+                                  uint32_t camId1Stride, const uint16_t *pCamId2, uint32_t camId2Stride)
+{
+    LoomArgs a;
+    int rc = loom_args(a, width, height, count, pWeights, weightStride, pCamId0, camId0Stride, pCamId1, camId1Stride, pCamId2,
+                       camId2Stride);
+    if (rc) return rc;
+    if (!pDst || !pCameras || !cameraStrides || !pMaps || dstStride < 3 * width || mapStride < 4 * width || (mapStride & 3u) ||
+        ((uintptr_t)pMaps & 3u) || srcWidth < 2 || srcHeight < 2 || srcWidth >= 8192 || srcHeight >= 8192)
+        return MVX_ERROR_INVALID_PARAMETERS;
+    RemapSrc s{};
+    for (uint32_t c = 0; c < count; c++) {
+        if (!pCameras[c] || cameraStrides[c] < 3 * srcWidth) return MVX_ERROR_INVALID_PARAMETERS;
+        s.cam[c] = pCameras[c];
+        s.camStride[c] = cameraStrides[c];
+    }
+    s.map = (const uint8_t *)pMaps;
+    s.mapStride = mapStride;
+    s.height = height;
+    s.sw = srcWidth;
+    s.sh = srcHeight;
+    const dim3 block(64, 4), grid((width + 63) / 64, (height + 3) / 4);
+    hipLaunchKernelGGL(k_stitch_loom_fused, grid, block, 0, stream_of(stream), pDst, dstStride, a, s);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }

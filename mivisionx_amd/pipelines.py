@@ -373,3 +373,69 @@ class ChainBatch:
             self.destroy_graph()
         except Exception:
             pass
+
+
+class LoomStitch:
+    """The stitch variant of the uneven-batch configuration (synth.stitch_rig): cameras remapped into an
+    equirectangular frame and merged with Loom's camera-selection maps, either in one pass
+    (mvx_Stitch_Loom_RGB: each output pixel samples the cameras it selects) or in Loom's two passes
+    (mvx_Remap_U24_U24_Bilinear_Constant per camera into a stacked frame, then mvx_StitchMerge_Loom_RGB)."""
+
+    def __init__(self, rig):
+        import torch
+        self.W, self.H, self.sw, self.sh = rig["W"], rig["H"], rig["sw"], rig["sh"]
+        self.count = len(rig["images"])
+        W, H = self.W, self.H
+        self.cams = [Plane.from_array(img, "RGB") for img in rig["images"]]
+        self.fixed = torch.zeros((self.count * H, W * 4), dtype=torch.uint8, device="cuda")
+        for c, m in enumerate(rig["maps"]):
+            call("Remap_FixedMap_F32", W, H, ctypes.c_void_p(self.fixed[c * H].data_ptr()), W * 4, ctypes.c_void_p(m.data_ptr()), W * 8,
+                 self.sw, self.sh)
+        self.weights = rig["weights"].cuda()
+        self.id0, self.id1, self.id2 = rig["camId0"].cuda(), rig["camId1"].cuda(), rig["camId2"].cuda()
+        self.out = Plane(W, H, "RGB")
+        self.stacked = None
+        P, U = ctypes.c_void_p * self.count, ctypes.c_uint32 * self.count
+        self._cam_ptrs = P(*[p.t.data_ptr() for p in self.cams])
+        self._cam_strides = U(*[p.stride for p in self.cams])
+
+    def _sel_args(self):
+        g = self.W // 8
+        return (ctypes.c_void_p(self.weights.data_ptr()), self.W, ctypes.c_void_p(self.id0.data_ptr()), g,
+                ctypes.c_void_p(self.id1.data_ptr()), 2 * g, ctypes.c_void_p(self.id2.data_ptr()), 2 * g)
+
+    def run_fused(self):
+        call("Stitch_Loom_RGB", self.W, self.H, self.out, self.out.stride, self.count, self._cam_ptrs, self._cam_strides, self.sw,
+             self.sh, ctypes.c_void_p(self.fixed.data_ptr()), self.W * 4, *self._sel_args())
+
+    def run_two_pass(self):
+        import torch
+        W, H = self.W, self.H
+        if self.stacked is None:
+            self.stacked = torch.zeros((self.count * H, 3 * W), dtype=torch.uint8, device="cuda")
+        for c, cam in enumerate(self.cams):
+            call("Remap_U24_U24_Bilinear_Constant", W, H, ctypes.c_void_p(self.stacked[c * H].data_ptr()), 3 * W, self.sw, self.sh, cam,
+                 cam.stride, ctypes.c_void_p(self.fixed[c * H].data_ptr()), W * 4, ctypes.c_uint8(0))
+        call("StitchMerge_Loom_RGB", W, H, self.out, self.out.stride, self.count, ctypes.c_void_p(self.stacked.data_ptr()), 3 * W,
+             *self._sel_args())
+
+    def algorithmic_bytes(self):
+        """bytes the one-pass stitch must move at least: the camera images once, every written output
+        pixel (3 B), for every (pixel, sampled camera) its 4-byte table entry and, when blended, its
+        weight byte, and the selection maps (1 B per 8-pixel group, plus 4 B for blended groups)"""
+        import torch
+        sel = self.id0.to(torch.int32)
+        n = torch.where(sel == 31, 0, torch.where(sel < 31, 1, torch.where(sel > 131, 6, torch.where(sel > 128, sel - 126, 2))))
+        # blended groups list up to n ids; ids of 31 are not sampled
+        ids = torch.stack([(self.id1.to(torch.int32) >> (5 * k)) & 31 for k in range(3)] +
+                          [(self.id2.to(torch.int32) >> (5 * k)) & 31 for k in range(3)])
+        k = torch.arange(6, device=sel.device)[:, None, None]
+        sampled_blend = ((ids < 31) & (k < n[None]) & (sel[None] > 31)).sum().item()
+        single = (sel < 31).sum().item()
+        written = (sel != 31).sum().item() * 8
+        blend_groups = (sel > 31).sum().item()
+        cams = sum(p.height * p.width * 3 for p in self.cams)
+        return int(cams + 3 * written + 8 * (4 * single + 5 * sampled_blend) + sel.numel() + 4 * blend_groups)
+
+    def results(self):
+        return self.out.download()

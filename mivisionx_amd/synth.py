@@ -241,3 +241,70 @@ def scene_u8_device(w, h, seed=0, device="cuda"):
     u2 = _uniform01_t(seed, 400, w * h, device)
     img += (8.0 * torch.sqrt(-2.0 * torch.log(u1)) * torch.cos(2.0 * np.pi * u2)).reshape(h, w)
     return torch.clamp(torch.round(img), 0, 255).to(torch.uint8)
+
+
+def stitch_rig(W=7680, H=3840, sw=1920, sh=1080, cams=6, overlap=64, device="cuda"):
+    """The stitch variant of the uneven-batch configuration: `cams` pinhole cameras of sw x sh around
+    the vertical axis (camera c at yaw 360/cams * c degrees plus a seeded +-2 degree jitter, 90 degree
+    horizontal field of view, radial lens term k1 in +-0.1; stream 4100 + c of seed 0), each with an F32
+    map of the W x H equirectangular frame into its image (entries outside the image are -1), the U8
+    blend weights of a linear ramp over `overlap` pixels around the mid-angle seam between neighbours
+    (255 inside a camera's own sector, 0 where it does not see), and Loom's camera-selection maps for
+    groups of 8 pixels: camId0 (U8: a single camera id, 31 for none, or a blend code 128 + count - 2),
+    camId1 / camId2 (U16: three 5-bit ids each, 31 where unused).  Maps, weights and selection are torch
+    tensors on `device`; the camera images (scene generator seeds 4000 + c) are numpy arrays."""
+    import torch
+    dev = torch.device(device)
+    f = (sw / 2.0) / np.tan(np.deg2rad(45.0))
+    cx, cy = (sw - 1) / 2.0, (sh - 1) / 2.0
+    lon = ((torch.arange(W, dtype=torch.float64, device=dev) + 0.5) / W * 2.0 * np.pi - np.pi)
+    lat = (np.pi / 2.0 - (torch.arange(H, dtype=torch.float64, device=dev) + 0.5) / H * np.pi)
+    px_per_deg = W / 360.0
+    maps, weights, yaws = [], [], []
+    for c in range(cams):
+        p = uniform01(0, 4100 + c, 2)
+        yaw = 360.0 / cams * c + (-2.0 + 4.0 * p[0])
+        k1 = -0.1 + 0.2 * p[1]
+        yaws.append(yaw)
+        d = torch.remainder(lon[None, :] - np.deg2rad(yaw) + np.pi, 2.0 * np.pi) - np.pi   # yaw offset, (1, W)
+        cl = torch.cos(lat)[:, None]
+        xcam, ycam, zcam = cl * torch.sin(d), torch.sin(lat)[:, None].expand(H, 1), cl * torch.cos(d)
+        ok = zcam > 1e-6
+        zs = torch.where(ok, zcam, torch.ones_like(zcam))
+        u, v = f * xcam / zs, -f * ycam / zs
+        r2 = (u * u + v * v) / (f * f)
+        s = 1.0 + k1 * r2
+        mx, my = cx + u * s, cy + v * s
+        inside = ok & (mx >= 0) & (my >= 0) & (mx <= sw - 1) & (my <= sh - 1)
+        m = torch.stack([torch.where(inside, mx, torch.full_like(mx, -1.0)), torch.where(inside, my, torch.full_like(my, -1.0))], -1)
+        maps.append(m.to(torch.float32).contiguous())
+        # ramp: 1 up to (seam - overlap/2), 0 beyond (seam + overlap/2), seam at half the camera spacing
+        half = 180.0 / cams
+        dist_px = torch.abs(torch.rad2deg(d)) * px_per_deg                      # (1, W)
+        ramp = torch.clamp(0.5 + (half * px_per_deg - dist_px) / overlap, 0.0, 1.0)
+        wt = torch.round(ramp * 255.0).expand(H, W) * inside.to(torch.float64)
+        weights.append(wt.to(torch.uint8))
+    wts = torch.stack(weights)                                                   # (cams, H, W)
+    groups = W // 8
+    used = (wts.reshape(cams, H, groups, 8) > 0).any(-1)                           # (cams, H, groups)
+    full = (wts.reshape(cams, H, groups, 8) == 255).all(-1)
+    n = used.sum(0)
+    ids = torch.full((6, H, groups), 31, dtype=torch.int32, device=dev)
+    order = torch.cumsum(used.to(torch.int32), 0) - 1                            # rank of each used camera
+    for c in range(cams):
+        for k in range(6):
+            ids[k] = torch.where(used[c] & (order[c] == k), torch.full_like(ids[k], c), ids[k])
+    single = (n == 1)
+    single_full = torch.zeros_like(single)
+    single_id = torch.full((H, groups), 31, dtype=torch.int32, device=dev)
+    for c in range(cams):
+        hit = single & used[c]
+        single_full |= hit & full[c]
+        single_id = torch.where(hit, torch.full_like(single_id, c), single_id)
+    code = torch.where(n <= 2, torch.full_like(n, 128), 126 + n).to(torch.int32)
+    cam0 = torch.where(n == 0, torch.full_like(code, 31), torch.where(single_full, single_id, code))
+    cam1 = (ids[0] | (ids[1] << 5) | (ids[2] << 10)).to(torch.int32)
+    cam2 = (ids[3] | (ids[4] << 5) | (ids[5] << 10)).to(torch.int32)
+    images = [scene_rgb(sw, sh, 4000 + c) for c in range(cams)]
+    return dict(images=images, maps=maps, weights=wts.reshape(cams * H, W).contiguous(), camId0=cam0.to(torch.uint8).contiguous(),
+                camId1=cam1.to(torch.int16).contiguous(), camId2=cam2.to(torch.int16).contiguous(), yaws=yaws, W=W, H=H, sw=sw, sh=sh)

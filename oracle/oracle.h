@@ -112,6 +112,14 @@ void oracle_remap_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, 
 void oracle_remap_bilinear(int dw, int dh, uint8_t *dst, int dst_stride, int sw, int sh, const uint8_t *src,
                            int src_stride, const uint16_t *fixed, int constant, int border);
 
+/* ---- stitch: Loom's merge with camera-selection maps (warp.c); camId1/2 strides in bytes ---- */
+void oracle_stitch_loom(int w, int h, uint8_t *dst, int dst_stride, int count, const uint8_t *stacked, int stacked_stride,
+                        const uint8_t *wt, int wt_stride, const uint8_t *id0, int id0_stride, const uint16_t *id1, int id1_stride,
+                        const uint16_t *id2, int id2_stride);
+void oracle_stitch_loom_remap(int w, int h, int y0, int y1, uint8_t *dst, int dst_stride, int count, const uint8_t *const *cams,
+                              int cam_stride, int sw, int sh, const uint16_t *fixed, const uint8_t *wt, int wt_stride, const uint8_t *id0,
+                              int id0_stride, const uint16_t *id1, int id1_stride, const uint16_t *id2, int id2_stride);
+
 /* ---- pyramidal Lucas-Kanade (lk.c); the state block plays the reference node's local data ---- */
 size_t oracle_lk_state_bytes(int w0, int h0, int capacity);
 int oracle_lk_run(void *state_mem, int levels, float pyr_scale, const uint8_t *const *old_pyr, const uint8_t *const *new_pyr,

@@ -403,6 +403,41 @@ class Oracle:
             acc = (acc + (wk * img.astype(np.float32)).astype(np.float32)).astype(np.float32)
         return np.clip(np.rint(acc), 0, 255).astype(np.uint8)
 
+    def stitch_loom(self, stacked, weights, id0, id1, id2, dst=None):
+        """Loom's merge with camera-selection maps (oracle_stitch_loom).  stacked: (count*h, w, 3) U8,
+        weights: (count*h, w) U8, id0: (h, ceil(w/8)) U8, id1 / id2: (h, ceil(w/8)) U16; dst (h, w, 3) is
+        updated in place where the selection writes (a new zero image when None)"""
+        count_h, w = weights.shape
+        h = id0.shape[0]
+        count = count_h // h
+        out = np.zeros((h, w, 3), np.uint8) if dst is None else dst
+        self.lib.oracle_stitch_loom(w, h, _p(out), 3 * w, count, _p(_c(stacked)), 3 * w, _p(_c(weights)), w, _p(_c(id0)),
+                                    id0.shape[1], _p(_c(id1)), 2 * id1.shape[1], _p(_c(id2)), 2 * id2.shape[1])
+        return out
+
+    def stitch_loom_remap(self, cams, fixed, weights, id0, id1, id2, dst=None, workers=1):
+        """the same stitch with each camera remapped through its 13.3 table on the fly (constant border 0):
+        cams: list of (sh, sw, 3) U8; fixed: (count*h, w, 2) U16 stacked tables; rows split over `workers`
+        threads"""
+        from concurrent.futures import ThreadPoolExecutor
+        count = len(cams)
+        h = id0.shape[0]
+        w = weights.shape[1]
+        sh, sw = cams[0].shape[:2]
+        out = np.zeros((h, w, 3), np.uint8) if dst is None else dst
+        cams = [_c(x) for x in cams]
+        ptrs = (ctypes.c_void_p * count)(*[x.ctypes.data for x in cams])
+        fixed, weights, id0, id1, id2 = _c(fixed), _c(weights), _c(id0), _c(id1), _c(id2)
+        bands = np.linspace(0, h, workers + 1).astype(int)
+
+        def band(i):
+            self.lib.oracle_stitch_loom_remap(w, h, int(bands[i]), int(bands[i + 1]), _p(out), 3 * w, count, ptrs, 3 * sw, sw, sh,
+                                              _p(fixed), _p(weights), w, _p(id0), id0.shape[1], _p(id1), 2 * id1.shape[1],
+                                              _p(id2), 2 * id2.shape[1])
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(band, range(workers)))
+        return out
+
     # ---- pyramidal Lucas-Kanade ----
     def lk_state(self, w0, h0, capacity):
         """zeroed scratch state of one tracker node (the reference node's local data)"""

@@ -332,3 +332,122 @@ void oracle_remap_nearest(int dw, int dh, uint8_t *dst, int dst_stride, int sw, 
         }
     }
 }
+
+/* ---- stitch: Loom's merge with its camera-selection maps ----------------------------------------
+ * amd_loomsl/kernels/merge.cpp:176-385 generates an OpenCL kernel; Loom has no CPU form, so this is a
+ * restatement of that generated source, "parity unpinned" against any reference output and pinned by
+ * closed forms in the tests (a selected single camera is copied, 31 leaves the output untouched,
+ * complementary weights over equal images copy them, the blend-count thresholds of camId0).
+ * Per group of 8 pixels: camId0 < 31 copies that camera; 31 writes nothing; > 31 blends the cameras
+ * listed in camId1 (5-bit ids at bits 0, 5, 10) and camId2 (same): two always, a third above 128, camId2's
+ * ids above 129, 130, 131 (merge.cpp:248-290).  w = (float)weight * 0.003922f (1/255 printed by "%f" into
+ * the kernel source, :226); per channel fa = ((0 + w0*p0) + w1*p1) + ..., each step rounded (mad taken
+ * unfused); out = round-to-nearest-even, saturated (amd_pack).  Ids >= count count as 31. */
+static const float LOOM_FACTOR = 0.003922f;
+
+static uint8_t loom_pack(float v)
+{
+    float r = nearbyintf(v);
+    if (r < 0.0f) r = 0.0f;
+    if (r > 255.0f) r = 255.0f;
+    return (uint8_t)r;
+}
+
+/* fetch(ctx, camera, x, y, rgb) */
+typedef void (*loom_fetch_t)(const void *ctx, int cam, int x, int y, int rgb[3]);
+
+static int loom_px(int x, int y, int count, const uint8_t *wt, int wt_stride, int h, const uint8_t *id0, int id0_stride,
+                   const uint16_t *id1, int id1_stride, const uint16_t *id2, int id2_stride, loom_fetch_t fetch, const void *ctx,
+                   uint8_t out[3])
+{
+    const int g = x >> 3;
+    const unsigned sel = id0[(long)y * id0_stride + g];
+    int p[3];
+    if (sel == 31) return 0;
+    if (sel < 31) {
+        if ((int)sel >= count) return 0;
+        fetch(ctx, (int)sel, x, y, p);
+        for (int c = 0; c < 3; c++) out[c] = (uint8_t)p[c];
+        return 1;
+    }
+    const unsigned s1 = *(const uint16_t *)((const uint8_t *)id1 + (long)y * id1_stride + 2L * g);
+    const unsigned s2 = sel > 129 ? *(const uint16_t *)((const uint8_t *)id2 + (long)y * id2_stride + 2L * g) : 0;
+    const int n = sel > 131 ? 6 : sel > 130 ? 5 : sel > 129 ? 4 : sel > 128 ? 3 : 2;
+    float fa[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < n; k++) {
+        const unsigned id = ((k < 3 ? s1 : s2) >> (5 * (k % 3))) & 31u;
+        float w = 0.0f;
+        p[0] = p[1] = p[2] = 0;
+        if (id < 31 && (int)id < count) {
+            fetch(ctx, (int)id, x, y, p);
+            w = (float)wt[((long)id * h + y) * wt_stride + x] * LOOM_FACTOR;
+        }
+        for (int c = 0; c < 3; c++) {
+            const float prod = w * (float)p[c];
+            fa[c] = fa[c] + prod;
+        }
+    }
+    for (int c = 0; c < 3; c++) out[c] = loom_pack(fa[c]);
+    return 1;
+}
+
+typedef struct { const uint8_t *img; int stride, h; } stacked_ctx_t;
+
+static void fetch_stacked(const void *ctx, int cam, int x, int y, int rgb[3])
+{
+    const stacked_ctx_t *s = (const stacked_ctx_t *)ctx;
+    const uint8_t *q = s->img + ((long)cam * s->h + y) * s->stride + 3L * x;
+    rgb[0] = q[0]; rgb[1] = q[1]; rgb[2] = q[2];
+}
+
+void oracle_stitch_loom(int w, int h, uint8_t *dst, int dst_stride, int count, const uint8_t *stacked, int stacked_stride,
+                        const uint8_t *wt, int wt_stride, const uint8_t *id0, int id0_stride, const uint16_t *id1, int id1_stride,
+                        const uint16_t *id2, int id2_stride)
+{
+    stacked_ctx_t s = {stacked, stacked_stride, h};
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            uint8_t o[3];
+            if (loom_px(x, y, count, wt, wt_stride, h, id0, id0_stride, id1, id1_stride, id2, id2_stride, fetch_stacked, &s, o))
+                for (int c = 0; c < 3; c++) dst[(long)y * dst_stride + 3L * x + c] = o[c];
+        }
+}
+
+/* Camera `cam` sampled at output (x, y) through its 13.3 table (rows cam*h ..) as
+ * oracle_remap_bilinear with a constant border of 0 does for each channel of the packed RGB image
+ * (rows y0 .. y1-1 of the output; the others are left as they are). */
+typedef struct { const uint8_t *const *cams; int cam_stride, sw, sh, w, h; const uint16_t *fixed; } remap_ctx_t;
+
+static void fetch_remap(const void *ctx, int cam, int x, int y, int rgb[3])
+{
+    const remap_ctx_t *r = (const remap_ctx_t *)ctx;
+    const long k = ((long)cam * r->h + y) * r->w + x;
+    unsigned mx = r->fixed[2 * k], my = r->fixed[2 * k + 1];
+    const int xbad = mx == 0xffff;
+    if (mx == 0xffff) mx = 0;
+    if (my == 0xffff) my = 0;
+    const int fx = (int)(mx & 7), fy = (int)(my & 7);
+    const uint8_t *src = r->cams[cam];
+    const long n = (long)r->sh * r->cam_stride;
+    for (int c = 0; c < 3; c++) {
+        const long o = (long)(my >> 3) * r->cam_stride + 3L * (long)(mx >> 3) + c;
+        const int p00 = rd(src, n, o), p10 = rd(src, n, o + 3), p01 = rd(src, n, o + r->cam_stride),
+                  p11 = rd(src, n, o + r->cam_stride + 3);
+        int v = (p00 * (8 - fx) * (8 - fy) + p10 * fx * (8 - fy) + p01 * (8 - fx) * fy + p11 * fx * fy + 32) >> 6;
+        if (xbad) v = 0;
+        rgb[c] = sat8(v);
+    }
+}
+
+void oracle_stitch_loom_remap(int w, int h, int y0, int y1, uint8_t *dst, int dst_stride, int count, const uint8_t *const *cams,
+                              int cam_stride, int sw, int sh, const uint16_t *fixed, const uint8_t *wt, int wt_stride, const uint8_t *id0,
+                              int id0_stride, const uint16_t *id1, int id1_stride, const uint16_t *id2, int id2_stride)
+{
+    remap_ctx_t r = {cams, cam_stride, sw, sh, w, h, fixed};
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < w; x++) {
+            uint8_t o[3];
+            if (loom_px(x, y, count, wt, wt_stride, h, id0, id0_stride, id1, id1_stride, id2, id2_stride, fetch_remap, &r, o))
+                for (int c = 0; c < 3; c++) dst[(long)y * dst_stride + 3L * x + c] = o[c];
+        }
+}

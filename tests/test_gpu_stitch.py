@@ -42,3 +42,53 @@ def test_two_camera_stitch(gpu, orc):
     call("StitchMerge_RGB", w, h, out, out.stride, 3, P(*[x[1].t.data_ptr() for x in dev]), U(*[x[1].stride for x in dev]),
          P(*[x[3].t.data_ptr() for x in dev]), U(*[x[3].stride for x in dev]))
     assert_same(out.download(), orc.stitch_merge(warped, weights), "stitch merge")
+
+
+def test_loom_merge_random_selection(gpu, orc):
+    """mvx_StitchMerge_Loom_RGB against the oracle on random selection maps: every code range, ids of 31
+    and ids beyond the cameras, odd group counts"""
+    from mivisionx_amd.device import Plane
+    w, h, n = 200, 37, 5
+    rng = np.random.default_rng(3)
+    stacked = rng.integers(0, 256, (n * h, w, 3), dtype=np.uint8)
+    wts = rng.integers(0, 256, (n * h, w), dtype=np.uint8)
+    g = (w + 7) // 8
+    id0 = rng.choice(np.array([0, 1, 2, 3, 4, 7, 31, 40, 128, 129, 130, 131, 132, 250], np.uint8), (h, g))
+    id1 = rng.integers(0, 1 << 15, (h, g), dtype=np.uint16) & np.uint16(0x7FFF)
+    id2 = rng.integers(0, 1 << 15, (h, g), dtype=np.uint16)
+    ids = lambda a: np.where(((a[..., None] >> np.array([0, 5, 10])) & 31) >= n + 3, 31, (a[..., None] >> np.array([0, 5, 10])) & 31)
+    id1 = (ids(id1) << np.array([0, 5, 10])).sum(-1).astype(np.uint16)
+    id2 = (ids(id2) << np.array([0, 5, 10])).sum(-1).astype(np.uint16)
+    out = Plane(w, h, "RGB")
+    out.t.fill_(77)
+    S, Wt = torch.from_numpy(stacked).cuda(), torch.from_numpy(wts).cuda()
+    I0, I1, I2 = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (id0, id1.view(np.int16), id2.view(np.int16)))
+    call("StitchMerge_Loom_RGB", w, h, out, out.stride, n, ctypes.c_void_p(S.data_ptr()), 3 * w, ctypes.c_void_p(Wt.data_ptr()), w,
+         ctypes.c_void_p(I0.data_ptr()), g, ctypes.c_void_p(I1.data_ptr()), 2 * g, ctypes.c_void_p(I2.data_ptr()), 2 * g)
+    want = orc.stitch_loom(stacked, wts, id0, id1, id2, dst=np.full((h, w, 3), 77, np.uint8))
+    assert_same(out.download(), want, "Loom merge, random selection")
+
+
+def test_loom_stitch_config_size(gpu, orc):
+    """6 cameras of 1920x1080 RGB into a 7680x3840 equirectangular frame, linear weights over 64-pixel
+    overlaps, Loom's selection maps: the one-pass stitch equals Loom's two passes on the GPU and the
+    oracle (remap through the same 13.3 tables, then the merge)"""
+    from mivisionx_amd.pipelines import LoomStitch
+    print("stitch rig 6 x 1920x1080 -> 7680x3840", flush=True)
+    rig = synth.stitch_rig()
+    S = LoomStitch(rig)
+    S.out.t.fill_(0)
+    S.run_fused()
+    one = S.results()
+    S.out.t.fill_(0)
+    S.run_two_pass()
+    two = S.results()
+    assert_same(one, two, "one-pass vs two-pass stitch")
+    W, H = rig["W"], rig["H"]
+    fixed = S.fixed.cpu().numpy().view(np.uint16).reshape(S.count * H, W, 2)
+    args = (rig["weights"].cpu().numpy(), rig["camId0"].cpu().numpy(), rig["camId1"].cpu().numpy().view(np.uint16),
+            rig["camId2"].cpu().numpy().view(np.uint16))
+    want = orc.stitch_loom_remap(rig["images"], fixed, *args, workers=16)
+    assert_same(one, want, "stitch vs oracle")
+    sel = rig["camId0"].cpu().numpy()
+    assert (sel < 31).any() and (sel == 31).any() and (sel > 31).any()
