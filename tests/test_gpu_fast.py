@@ -45,6 +45,21 @@ def test_fast_c_abi(gpu, orc, w, h, thr, nonmax):
     assert np.array_equal(kp.view(np.uint8), want.view(np.uint8))
 
 
+@pytest.mark.parametrize("thr", [200.0, 250.0, 253.0, 254.0, 255.0, -3.5, 0.75])
+@pytest.mark.parametrize("nonmax", [0, 1])
+def test_fast_threshold_edges(gpu, orc, thr, nonmax):
+    """the strength closed form at its edges (R-1 against 254, t >= 255, negative and fractional
+    thresholds through (short)floorf) on a high-contrast input whose nine-runs pass up to 254"""
+    rng = np.random.default_rng(8)
+    for w, h in ((97, 53), (1920, 1080)):
+        src = rng.choice(np.array([0, 1, 3, 252, 254, 255], np.uint8), size=(h, w))
+        cap = 2_000_000 if w * h > 1_000_000 else 5000
+        kp, n, total = _run(src, thr, nonmax, cap)
+        want, wc = orc.fast(src, thr, nonmax, cap)
+        assert total == wc and n == len(want)
+        assert np.array_equal(kp.view(np.uint8), want.view(np.uint8))
+
+
 def test_fast_capacity(gpu, orc):
     src = synth.scene_u8(333, 77, 4)
     kp, n, total = _run(src, 12.0, 0, 50)

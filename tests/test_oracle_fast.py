@@ -41,3 +41,23 @@ def test_fast_closed_forms(orc):
     assert len(quads) == 4
     kp, n = orc.fast(np.full((40, 40), 77, np.uint8), 1.0, 0, 100)
     assert n == 0
+
+
+def _contrast(w, h, seed):
+    """high-contrast input: a random mix of near-black and near-white pixels, so nine-runs pass at
+    thresholds up to 254"""
+    rng = np.random.default_rng(seed)
+    return rng.choice(np.array([0, 1, 3, 252, 254, 255], np.uint8), size=(h, w))
+
+
+@pytest.mark.parametrize("thr", [200.0, 250.0, 253.0, 254.0, 255.0, 300.0, -3.5, 0.75])
+@pytest.mark.parametrize("nonmax", [0, 1])
+def test_fast_threshold_edges_match_reference(orc, ref, thr, nonmax):
+    """the strength closed form min(R-1, E, 254) at its edges: R-1 against 254, thresholds >= 255, and
+    negative or fractional thresholds truncated through (short)floorf"""
+    src = _contrast(97, 53, 8)
+    want, wc = ref.fast(src, thr, nonmax, 5000)
+    got, gc = orc.fast(src, thr, nonmax, 5000)
+    assert wc == gc and np.array_equal(want.view(np.uint8), got.view(np.uint8))
+    if thr < 250:
+        assert wc > 0
