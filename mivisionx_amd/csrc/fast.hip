@@ -2,32 +2,19 @@
 // for gfx950, writing the OpenVX keypoint array in the reference's raster order.
 //
 // The reference runs this on the CPU (its HIP FAST kernels are separate code); corners are found
-// row by row, so the array order is raster order.  Here one workgroup per image row counts its
-// corners, one workgroup turns the row counts into offsets, and one workgroup per row writes its
-// keypoints at those offsets (wave ballots rank the corners inside each 256-pixel chunk).  With
-// suppression the per-pixel strengths go to a byte map first, then the 3x3 test reads that map.
+// row by row, so the array order is raster order.  Here one fused pass per 64x16 tile stages the
+// tile's source bytes in LDS, finds the corners (and, with suppression, their strengths and the 3x3
+// maxima) and writes a byte plane of results plus per-row counts; a scan of the counts gives each row
+// its offset and one workgroup per row writes its keypoints there (wave ballots rank the corners).
 #include "mvx_common.hpp"
 
 namespace mvx {
 
 namespace {
 
-// the 16-pixel circle, clockwise from the top (ago_haf_cpu_fast_corners.cpp:25-56)
-__constant__ int8_t kCircle[16][2] = {{0, -3}, {1, -3}, {2, -2}, {3, -1}, {3, 0}, {3, 1}, {2, 2}, {1, 3},
-                                      {0, 3}, {-1, 3}, {-2, 2}, {-3, 1}, {-3, 0}, {-3, -1}, {-2, -2}, {-1, -3}};
-
 struct Circle {
     int c, b[16];
 };
-
-MVX_DEV Circle load_circle(const uint8_t *src, long n, uint32_t stride, int x, int y)
-{
-    Circle k;
-    k.c = (int)ld_u8(src, n, (long)y * stride + x);
-#pragma unroll
-    for (int i = 0; i < 16; i++) k.b[i] = (int)ld_u8(src, n, (long)(y + kCircle[i][1]) * stride + x + kCircle[i][0]);
-    return k;
-}
 
 // isCorner (:90-126): nine consecutive set bits on the circle
 MVX_DEV bool nine_run(uint32_t m)
@@ -62,75 +49,182 @@ MVX_DEV uint32_t minus_mask(const Circle &k, int t)
     return m;
 }
 
-MVX_DEV bool is_corner(const Circle &k, int t)
-{
-    return !early_exit(k, t) && (nine_run(plus_mask(k, t)) || nine_run(minus_mask(k, t)));
-}
-
 // checkForCornerAndGetStrength (:216-412): per sign with a nine-run at t, the largest threshold m in
 // [t, 254] for which the corner test still passes (the reference bisects between t and 255, repeating the
 // early exit at each probe); 0 for a sign without a run; the strength is the larger of the two, and a
 // map value of 0 means no corner.  The test at m passes iff some nine-pixel arc has every difference
 // d_i (b_i - c, or c - b_i) above m, i.e. m < R = max over arcs of min d_i, and the early exit does not
 // fire, i.e. m <= E = min(max(|c-b0|, |c-b8|), max(|c-b4|, |c-b12|)).  Both conditions only get harder as
-// m grows, so the bisection's answer is min(R - 1, E, 254), computed here directly.
+// m grows, so the bisection's answer is min(R - 1, E, 254), computed here directly.  Below a zero
+// threshold both signs can have runs with answers below zero; the reference then keeps the larger
+// negative value, which the byte map stores modulo 256 (a sign without a run counts as 0).
 MVX_DEV int strength(const Circle &k, int t)
 {
     if (early_exit(k, t)) return 0;
     const int E = min(max(abs(k.c - k.b[0]), abs(k.c - k.b[8])), max(abs(k.c - k.b[4]), abs(k.c - k.b[12])));
-    int best = 0;
+    int s[2] = {0, 0};
 #pragma unroll
     for (int sign = 0; sign < 2; sign++) {
         int d[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) d[i] = sign ? k.c - k.b[i] : k.b[i] - k.c;
+        // minima of every circular run of 2, 4, 8, then 9 differences
+        int m2[16], m4[16], m8[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) m2[i] = min(d[i], d[(i + 1) & 15]);
+#pragma unroll
+        for (int i = 0; i < 16; i++) m4[i] = min(m2[i], m2[(i + 2) & 15]);
+#pragma unroll
+        for (int i = 0; i < 16; i++) m8[i] = min(m4[i], m4[(i + 4) & 15]);
         int R = -1024;
 #pragma unroll
-        for (int s0 = 0; s0 < 16; s0++) {
-            int m = d[s0];
-#pragma unroll
-            for (int j = 1; j < 9; j++) m = min(m, d[(s0 + j) & 15]);
-            R = max(R, m);
-        }
-        if (R > t) best = max(best, min(min(R - 1, E), 254));
+        for (int i = 0; i < 16; i++) R = max(R, min(m8[i], d[(i + 8) & 15]));
+        if (R > t) s[sign] = min(min(R - 1, E), 254);
     }
-    return best;
+    return max(s[0], s[1]);
 }
+
+// ---------------------------------------------------------------- fused tile pass ------------------
+// One workgroup per 64x16 tile of the image.  The tile's source bytes (rows y0-4..y0+19, columns
+// x0-4..x0+67 at the reference's linear byte addresses, 0 outside the plane) go to LDS; then
+//   1. every candidate position runs the early exit on pixels 1, 5, 9, 13; survivors are compacted
+//      into an LDS list (wave ballots);
+//   2. the survivors run the full nine-run test; without suppression a corner sets its flag, with it
+//      the corners are compacted again and take the strength closed form, stored (mod 256) in an LDS
+//      strength map that covers the tile plus one pixel of border;
+//   3. with suppression the 3x3 test runs on that map.
+// The tile's results go to a byte plane (0 = none, else the strength or, without suppression, 1) and
+// the per-row counts to rowCount[y - 3] (one atomic per tile row); a scan turns those into offsets and
+// one workgroup per row writes its keypoints in raster order from the byte plane.
+constexpr int FX = 64, FY = 16;                  // output tile
+constexpr int SW = FX + 8, SH = FY + 8;          // staged source: 4 columns / rows of margin each side
+constexpr int MW = FX + 2, MH = FY + 2;          // strength map: the tile plus one pixel of border
+constexpr int NPOS = MW * MH;
 
 struct FastArgs {
     uint32_t w, h, stride, capacity;
     const uint8_t *src;
     int t;
     float threshold;
-    const uint8_t *map;   // suppression: strengths, w*h
 };
 
-// the corner decision at (x, y) of a row in 3..h-4: without suppression every x in [0, w); with it
-// x in [3, w-4] and the 3x3 test on the strength map
-template <bool SUPP> MVX_DEV bool corner_at(const FastArgs &a, int x, int y, float &s)
+// the 16-pixel circle, clockwise from the top (ago_haf_cpu_fast_corners.cpp:25-56), as offsets in the
+// staged tile (row pitch SW): (0,-3) (1,-3) (2,-2) (3,-1) (3,0) (3,1) (2,2) (1,3) (0,3) (-1,3) (-2,2)
+// (-3,1) (-3,0) (-3,-1) (-2,-2) (-1,-3)
+__constant__ int16_t kCircleOff[16] = {-3 * SW + 0, -3 * SW + 1, -2 * SW + 2, -1 * SW + 3, 3, SW + 3, 2 * SW + 2, 3 * SW + 1,
+                                       3 * SW + 0, 3 * SW - 1, 2 * SW - 2, SW - 3, -3, -SW - 3, -2 * SW - 2, -3 * SW - 1};
+
+// append `pos` to an LDS list when `keep`; list order is free (each entry is processed independently)
+MVX_DEV void push(bool keep, int pos, uint16_t *list, uint32_t *count)
 {
-    if (!SUPP) {
-        if (x >= (int)a.w) return false;
-        s = a.threshold;
-        return is_corner(load_circle(a.src, (long)a.h * a.stride, a.stride, x, y), a.t);
-    }
-    if (x < 3 || x >= (int)a.w - 3) return false;
-    const uint8_t *p = a.map + (long)y * a.w + x;
-    const int c = p[0];
-    const long w = a.w;
-    s = (float)c;
-    return c && c >= p[-w - 1] && c >= p[-w] && c >= p[-w + 1] && c >= p[-1] && c > p[1] && c > p[w - 1] && c > p[w] &&
-           c > p[w + 1];
+    const uint64_t m = __ballot(keep);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, 0);
+    if (keep) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)pos;
 }
 
-// strength map over the whole plane (0 outside x in [3, w-4], y in [3, h-4])
-__global__ void __launch_bounds__(256) k_fast_map(FastArgs a, uint8_t *__restrict__ map)
+template <bool SUPP>
+__global__ void __launch_bounds__(256)
+k_fast_tile(FastArgs a, uint8_t *__restrict__ marks, uint32_t *__restrict__ rowCount)
 {
-    const int x = (int)(blockIdx.x * 256 + threadIdx.x), y = (int)blockIdx.y;
-    if (x >= (int)a.w) return;
-    int s = 0;
-    if (y >= 3 && y < (int)a.h - 3 && x >= 3 && x < (int)a.w - 3) s = strength(load_circle(a.src, (long)a.h * a.stride, a.stride, x, y), a.t);
-    map[(long)y * a.w + x] = (uint8_t)s;
+    __shared__ uint8_t stage[SH * SW];
+    __shared__ uint8_t smap[MH * MW];
+    __shared__ uint16_t surv[NPOS], corn[NPOS];
+    __shared__ uint32_t nsurv, ncorn, rowCnt[FY];
+    const int tid = (int)threadIdx.x;
+    const int x0 = (int)blockIdx.x * FX, y0 = (int)blockIdx.y * FY;
+    const long n = (long)a.h * a.stride;
+    for (int i = tid; i < SH * SW; i += 256) {
+        const int r = i / SW, c = i - r * SW;
+        stage[i] = (uint8_t)ld_u8(a.src, n, (long)(y0 - 4 + r) * a.stride + (x0 - 4 + c));
+    }
+    for (int i = tid; i < MH * MW; i += 256) smap[i] = 0;
+    if (tid == 0) { nsurv = 0; ncorn = 0; }
+    if (tid < FY) rowCnt[tid] = 0;
+    __syncthreads();
+    // candidate position p: with suppression the map position (x0-1+i, y0-1+j), i < MW, j < MH, inside
+    // [3, w-4] x [3, h-4]; without it the tile pixel (x0+i, y0+j), i < FX, j < FY, x < w, y in [3, h-4]
+    const int pw = SUPP ? MW : FX, np = SUPP ? NPOS : FX * FY, d = SUPP ? 1 : 0;
+    auto centre = [&](int p) { const int j = p / pw, i = p - j * pw; return (j + 4 - d) * SW + (i + 4 - d); };
+    for (int p0 = 0; p0 < np; p0 += 256) {
+        const int p = p0 + tid;
+        bool keep = false;
+        if (p < np) {
+            const int j = p / pw, i = p - j * pw, x = x0 - d + i, y = y0 - d + j;
+            const bool ok = y >= 3 && y < (int)a.h - 3 && (SUPP ? (x >= 3 && x < (int)a.w - 3) : x < (int)a.w);
+            if (ok) {
+                const int o = centre(p), c = stage[o];
+                const int b0 = stage[o - 3 * SW], b4 = stage[o + 3], b8 = stage[o + 3 * SW], b12 = stage[o - 3];
+                keep = !((abs(c - b0) < a.t && abs(c - b8) < a.t) || (abs(c - b4) < a.t && abs(c - b12) < a.t));
+            }
+        }
+        push(keep, p, surv, &nsurv);
+    }
+    __syncthreads();
+    const int ns = (int)nsurv;
+    for (int q0 = 0; q0 < ns; q0 += 256) {
+        const int q = q0 + tid;
+        bool corner = false;
+        int p = 0;
+        if (q < ns) {
+            p = surv[q];
+            Circle k;
+            const int o = centre(p);
+            k.c = stage[o];
+#pragma unroll
+            for (int i = 0; i < 16; i++) k.b[i] = stage[o + kCircleOff[i]];
+            corner = nine_run(plus_mask(k, a.t)) || nine_run(minus_mask(k, a.t));
+            if (!SUPP && corner) smap[(p / FX + 1) * MW + (p % FX) + 1] = 1;
+        }
+        if (SUPP) push(corner, p, corn, &ncorn);
+    }
+    if (SUPP) {
+        __syncthreads();
+        const int nc = (int)ncorn;
+        for (int q = tid; q < nc; q += 256) {
+            const int p = corn[q];
+            Circle k;
+            const int o = centre(p);
+            k.c = stage[o];
+#pragma unroll
+            for (int i = 0; i < 16; i++) k.b[i] = stage[o + kCircleOff[i]];
+            smap[p] = (uint8_t)strength(k, a.t);   // p indexes the map directly (pw == MW)
+        }
+    }
+    __syncthreads();
+    // results: 4 pixels per thread, (x0 + 4*(tid & 15) .. +3, y0 + (tid >> 4))
+    const int ty = tid >> 4, tx = (tid & 15) * 4, y = y0 + ty;
+    uint32_t word = 0, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int x = x0 + tx + k;
+        const uint8_t *m = smap + (ty + 1) * MW + (tx + k + 1);
+        uint32_t v = 0;
+        if (SUPP) {
+            const int c = m[0];
+            if (x >= 3 && x < (int)a.w - 3 && y >= 3 && y < (int)a.h - 3 && c && c >= m[-MW - 1] && c >= m[-MW] && c >= m[-MW + 1] &&
+                c >= m[-1] && c > m[1] && c > m[MW - 1] && c > m[MW] && c > m[MW + 1])
+                v = (uint32_t)c;
+        } else {
+            v = m[0];
+        }
+        cnt += v ? 1u : 0u;
+        word |= v << (8 * k);
+    }
+    if (y < (int)a.h) {
+        uint8_t *row = marks + (long)y * a.w;
+        if (x0 + tx + 3 < (int)a.w && (((uintptr_t)(row + x0 + tx)) & 3u) == 0) {
+            *reinterpret_cast<uint32_t *>(row + x0 + tx) = word;
+        } else {
+            for (int k = 0; k < 4; k++)
+                if (x0 + tx + k < (int)a.w) row[x0 + tx + k] = (uint8_t)(word >> (8 * k));
+        }
+    }
+    if (cnt) atomicAdd(&rowCnt[ty], cnt);
+    __syncthreads();
+    if (tid < FY && y0 + tid >= 3 && y0 + tid < (int)a.h - 3 && rowCnt[tid]) atomicAdd(&rowCount[y0 + tid - 3], rowCnt[tid]);
 }
 
 // rank of this thread's flag among the 256 of the block, and the block total
@@ -149,17 +243,9 @@ MVX_DEV uint32_t block_rank(bool f, uint32_t &total)
     return base + below;
 }
 
-template <bool SUPP> __global__ void __launch_bounds__(256) k_fast_count(FastArgs a, uint32_t *__restrict__ rowCount)
+__global__ void __launch_bounds__(256) k_fast_zero(uint32_t *__restrict__ p, uint32_t n)
 {
-    const int y = 3 + (int)blockIdx.x;
-    uint32_t n = 0;
-    for (int x0 = 0; x0 < (int)a.w; x0 += 256) {
-        float s;
-        uint32_t total;
-        block_rank(corner_at<SUPP>(a, x0 + (int)threadIdx.x, y, s), total);
-        n += total;
-    }
-    if (threadIdx.x == 0) rowCount[blockIdx.x] = n;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0u;
 }
 
 // exclusive scan of the row counts (one workgroup); the array count (capped) and the corner count
@@ -192,24 +278,25 @@ k_fast_scan(const uint32_t *__restrict__ rowCount, uint32_t *__restrict__ rowOff
     }
 }
 
-// vx_keypoint_t {x, y, strength, scale, orientation, tracking_status, error}
+// vx_keypoint_t {x, y, strength, scale, orientation, tracking_status, error}; one workgroup per row
+// 3..h-4 reads the row's marks and writes its corners from the row's offset on
 template <bool SUPP>
 __global__ void __launch_bounds__(256)
-k_fast_write(FastArgs a, const uint32_t *__restrict__ rowOffset, uint8_t *__restrict__ dst)
+k_fast_write(FastArgs a, const uint8_t *__restrict__ marks, const uint32_t *__restrict__ rowOffset, uint8_t *__restrict__ dst)
 {
     const int y = 3 + (int)blockIdx.x;
     uint32_t base = rowOffset[blockIdx.x];
+    const uint8_t *row = marks + (long)y * a.w;
     for (int x0 = 0; x0 < (int)a.w; x0 += 256) {
         const int x = x0 + (int)threadIdx.x;
-        float s = 0.0f;
-        const bool f = corner_at<SUPP>(a, x, y, s);
+        const uint32_t v = x < (int)a.w ? row[x] : 0u;
         uint32_t total;
-        const uint32_t r = block_rank(f, total);
-        if (f && base + r < a.capacity) {
+        const uint32_t r = block_rank(v != 0, total);
+        if (v && base + r < a.capacity) {
             uint32_t *kp = (uint32_t *)(dst + 28ull * (base + r));
             kp[0] = (uint32_t)x;
             kp[1] = (uint32_t)y;
-            kp[2] = __float_as_uint(s);
+            kp[2] = __float_as_uint(SUPP ? (float)v : a.threshold);
             kp[3] = 0u;
             kp[4] = 0u;
             kp[5] = 1u;
@@ -220,7 +307,7 @@ k_fast_write(FastArgs a, const uint32_t *__restrict__ rowOffset, uint8_t *__rest
 }
 
 struct FastWs {
-    size_t rowCount, rowOffset, map, total;
+    size_t rowCount, rowOffset, marks, total;
 };
 
 FastWs fast_ws(uint32_t w, uint32_t h)
@@ -229,8 +316,8 @@ FastWs fast_ws(uint32_t w, uint32_t h)
     const size_t rows = h > 6 ? h - 6 : 1;
     L.rowCount = 0;
     L.rowOffset = (rows * 4 + 255) & ~(size_t)255;
-    L.map = L.rowOffset + ((rows * 4 + 255) & ~(size_t)255);
-    L.total = L.map + (((size_t)w * h + 255) & ~(size_t)255);
+    L.marks = L.rowOffset + ((rows * 4 + 255) & ~(size_t)255);
+    L.total = L.marks + (((size_t)w * h + 255) & ~(size_t)255);
     return L;
 }
 
@@ -246,13 +333,13 @@ int fast_corners(mvx_stream_t stream, uint32_t capacity, void *dstCorners, uint3
     if (workspaceSize < L.total) return MVX_ERROR_INVALID_PARAMETERS;
     uint8_t *ws = (uint8_t *)workspace;
     const uint32_t rows = h - 6;
-    FastArgs a{w, h, srcStride, capacity, src, (int)(short)floorf(threshold), threshold, ws + L.map};
+    FastArgs a{w, h, srcStride, capacity, src, (int)(short)floorf(threshold), threshold};
     hipStream_t st = stream_of(stream);
-    if (SUPP) hipLaunchKernelGGL(k_fast_map, dim3((w + 255) / 256, h), dim3(256), 0, st, a, ws + L.map);
-    hipLaunchKernelGGL(k_fast_count<SUPP>, dim3(rows), dim3(256), 0, st, a, (uint32_t *)(ws + L.rowCount));
-    hipLaunchKernelGGL(k_fast_scan, dim3(1), dim3(1024), 0, st, (const uint32_t *)(ws + L.rowCount), (uint32_t *)(ws + L.rowOffset), rows,
-                       capacity, dstCount, numCorners);
-    hipLaunchKernelGGL(k_fast_write<SUPP>, dim3(rows), dim3(256), 0, st, a, (const uint32_t *)(ws + L.rowOffset), (uint8_t *)dstCorners);
+    uint32_t *rowCount = (uint32_t *)(ws + L.rowCount), *rowOffset = (uint32_t *)(ws + L.rowOffset);
+    hipLaunchKernelGGL(k_fast_zero, dim3(min((rows + 255) / 256, 64u)), dim3(256), 0, st, rowCount, rows);
+    hipLaunchKernelGGL(k_fast_tile<SUPP>, dim3((w + FX - 1) / FX, (h + FY - 1) / FY), dim3(256), 0, st, a, ws + L.marks, rowCount);
+    hipLaunchKernelGGL(k_fast_scan, dim3(1), dim3(1024), 0, st, rowCount, rowOffset, rows, capacity, dstCount, numCorners);
+    hipLaunchKernelGGL(k_fast_write<SUPP>, dim3(rows), dim3(256), 0, st, a, ws + L.marks, rowOffset, (uint8_t *)dstCorners);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }

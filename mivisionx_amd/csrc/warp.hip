@@ -117,6 +117,182 @@ k_warp_perspective(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t
     *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
 }
 
+// ---------------------------------------------------------------- LDS-staged bilinear warps --------
+// The same per-pixel arithmetic as k_warp_affine / k_warp_perspective, for 64x16 output tiles whose
+// source footprint is staged in LDS first: the tile's four corner pixels are mapped, their bounding
+// box (plus a margin for the per-pixel rounding) is read row by row with dword loads at the linear
+// byte addresses the reference reads -- a row's columns below 0 or past the stride are the bytes of
+// the neighbouring rows, offsets outside the plane are the zero guard -- and each pixel then takes
+// its four samples from LDS.  The staged box is only a cache: a pixel whose samples fall outside it
+// (float rounding, a perspective tile that crosses the horizon, a box too large for LDS) or whose
+// offset the reference clamps reads the plane directly, as the unstaged kernels do.  Needs a 4-byte
+// aligned source with a stride that is a multiple of 4 (then every dword is wholly inside or wholly
+// outside the plane).
+constexpr int WTW = 64, WTH = 16;            // output tile: 16 threads x 4 pixels per row, 16 rows
+constexpr int WLDS_DW = 3072;                // staged dwords per tile (12 KB)
+
+struct WarpBox { int r0, r1, cA, nd; bool on; };
+
+// bounding box of the source coordinates xs in [xlo, xhi], ys in [ylo, yhi]: rows and columns that a
+// sample at trunc/floor of those coordinates and its +1 neighbours can touch, with two pixels of margin
+MVX_DEV WarpBox warp_box(float xlo, float xhi, float ylo, float yhi)
+{
+    WarpBox b{0, -1, 0, 0, false};
+    if (!(xlo >= -65536.0f && xhi <= 65536.0f && ylo >= -65536.0f && yhi <= 65536.0f)) return b;
+    const int c0 = (int)floorf(xlo) - 2, c1 = (int)floorf(xhi) + 3;
+    b.r0 = (int)floorf(ylo) - 2;
+    b.r1 = (int)floorf(yhi) + 3;
+    b.cA = c0 & ~3;                                  // floor to a multiple of 4 (two's complement)
+    b.nd = ((c1 - b.cA) >> 2) + 1;
+    b.on = (int64_t)(b.r1 - b.r0 + 1) * b.nd <= WLDS_DW;    // rows and dwords are each below 2^18 here
+    return b;
+}
+
+// stage rows r0..r1, columns cA..cA+4nd-1 (linear-address coordinates) into lds, row pitch 4nd bytes
+MVX_DEV void warp_stage(uint32_t *lds, const WarpBox &b, const uint8_t *src, uint32_t stride, uint64_t n)
+{
+    const int total = (b.r1 - b.r0 + 1) * b.nd;
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    for (int i = tid; i < total; i += (int)(blockDim.x * blockDim.y)) {
+        const int rr = i / b.nd, k = i - rr * b.nd;
+        const int64_t lin = (int64_t)(b.r0 + rr) * stride + b.cA + 4 * k;
+        lds[i] = (lin >= 0 && (uint64_t)lin + 4 <= n) ? *reinterpret_cast<const uint32_t *>(src + lin) : 0u;
+    }
+}
+
+// the four samples at (yi, xi), (yi, xi+1), (yi+1, xi), (yi+1, xi+1) when the box holds them
+MVX_DEV bool warp_lds4(const uint8_t *lds, const WarpBox &b, int32_t xi, int32_t yi, float &p0, float &p1, float &p2, float &p3)
+{
+    const int pitch = 4 * b.nd;
+    if (!b.on || yi < b.r0 || yi >= b.r1 || xi < b.cA || xi >= b.cA + pitch - 1) return false;
+    const int o = (yi - b.r0) * pitch + (xi - b.cA);
+    p0 = (float)lds[o];
+    p1 = (float)lds[o + 1];
+    p2 = (float)lds[o + pitch];
+    p3 = (float)lds[o + pitch + 1];
+    return true;
+}
+
+template <bool BORDER>
+__global__ void __launch_bounds__(256)
+k_warp_affine_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                  const uint8_t *__restrict__ src, uint32_t srcStride, Affine m, int32_t limit, uint32_t border)
+{
+    __shared__ uint32_t stage[WLDS_DW];
+    const uint64_t n = (uint64_t)sh * srcStride;
+    const float fw = (float)sw, fh = (float)sh;
+    // tile corners (the last thread of a row stores pixels up to x0 + 63)
+    const uint32_t tx0 = blockIdx.x * WTW, ty0 = blockIdx.y * WTH;
+    WarpBox b;
+    {
+        const float xa = (float)tx0, xb = (float)(tx0 + WTW - 1), ya = (float)ty0, yb = (float)(ty0 + WTH - 1);
+        const float sx[4] = {m.r00 * xa + (ya * m.r01 + m.c1), m.r00 * xb + (ya * m.r01 + m.c1), m.r00 * xa + (yb * m.r01 + m.c1),
+                             m.r00 * xb + (yb * m.r01 + m.c1)};
+        const float sy[4] = {m.r10 * xa + (ya * m.r11 + m.c2), m.r10 * xb + (ya * m.r11 + m.c2), m.r10 * xa + (yb * m.r11 + m.c2),
+                             m.r10 * xb + (yb * m.r11 + m.c2)};
+        b = warp_box(fminf(fminf(sx[0], sx[1]), fminf(sx[2], sx[3])), fmaxf(fmaxf(sx[0], sx[1]), fmaxf(sx[2], sx[3])),
+                     fminf(fminf(sy[0], sy[1]), fminf(sy[2], sy[3])), fmaxf(fmaxf(sy[0], sy[1]), fmaxf(sy[2], sy[3])));
+    }
+    if (b.on) warp_stage(stage, b, src, srcStride, n);
+    __syncthreads();
+    const uint32_t x0 = tx0 + threadIdx.x * 4;
+    const uint32_t y = ty0 + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const float xd = (float)y * m.r01 + m.c1, yd = (float)y * m.r11 + m.c2;
+    const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool col = x0 + k < dw;
+        const float xs = (col ? m.r00 * (float)(x0 + k) : 0.0f) + xd, ys = (col ? m.r10 * (float)(x0 + k) : 0.0f) + yd;
+        const int32_t xi = cvtt(xs), yi = cvtt(ys);
+        const float fx = xs - (float)xi, fy = ys - (float)yi;
+        const int32_t raw = (int32_t)((uint32_t)yi * srcStride + (uint32_t)xi);
+        const int32_t off = BORDER ? max(min(raw, limit), 0) : raw;
+        float p0, p1, p2, p3;
+        if (off != raw || !warp_lds4(lds, b, xi, yi, p0, p1, p2, p3)) {
+            p0 = (float)ldg8(src, n, off);
+            p1 = (float)ldg8(src, n, (int64_t)off + 1);
+            p2 = (float)ldg8(src, n, (int64_t)off + srcStride);
+            p3 = (float)ldg8(src, n, (int64_t)off + srcStride + 1);
+        }
+        const float omx = 1.0f - fx, omy = 1.0f - fy;
+        const float a = (p0 * omx) * omy, bb = (p1 * fx) * omy, c = (p2 * omx) * fy, d = (p3 * fx) * fy;
+        const float v = (a + bb) + (c + d);
+        uint32_t px = sat8(cvtn(v));
+        if (BORDER && !(xs >= 0.0f && xs < fw && ys >= 0.0f && ys < fh)) px = border;
+        word |= px << (8 * k);
+    }
+    *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
+// the perspective tile is staged only when the four corner denominators share a sign (the tile then
+// maps to a convex quadrilateral inside the corners' bounding box)
+__global__ void __launch_bounds__(256)
+k_warp_perspective_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
+                       const uint8_t *__restrict__ src, uint32_t srcStride, Persp m, uint32_t border)
+{
+    __shared__ uint32_t stage[WLDS_DW];
+    const uint64_t n = (uint64_t)sh * srcStride;
+    const int32_t limit = (int32_t)(sh * srcStride - 1u);
+    const uint32_t tx0 = blockIdx.x * WTW, ty0 = blockIdx.y * WTH;
+    WarpBox b{0, -1, 0, 0, false};
+    {
+        float sx[4], sy[4], sz[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const float xf = (float)(tx0 + ((c & 1) ? WTW - 1 : 0)), yf = (float)(ty0 + ((c & 2) ? WTH - 1 : 0));
+            sz[c] = m.g * xf + (yf * m.h + m.i);
+            const float z = 1.0f / sz[c];
+            sx[c] = (m.a * xf + (yf * m.b + m.c)) * z;
+            sy[c] = (m.d * xf + (yf * m.e + m.f)) * z;
+        }
+        const bool pos = sz[0] > 0.0f && sz[1] > 0.0f && sz[2] > 0.0f && sz[3] > 0.0f;
+        const bool neg = sz[0] < 0.0f && sz[1] < 0.0f && sz[2] < 0.0f && sz[3] < 0.0f;
+        if (pos || neg)
+            b = warp_box(fminf(fminf(sx[0], sx[1]), fminf(sx[2], sx[3])), fmaxf(fmaxf(sx[0], sx[1]), fmaxf(sx[2], sx[3])),
+                         fminf(fminf(sy[0], sy[1]), fminf(sy[2], sy[3])), fmaxf(fmaxf(sy[0], sy[1]), fmaxf(sy[2], sy[3])));
+    }
+    if (b.on) warp_stage(stage, b, src, srcStride, n);
+    __syncthreads();
+    const uint32_t x0 = tx0 + threadIdx.x * 4;
+    const uint32_t y = ty0 + threadIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    const float xd = (float)y * m.b + m.c, yd = (float)y * m.e + m.f, zd = (float)y * m.h + m.i;
+    const float bf = (float)border;
+    const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool col = x0 + k < dw;
+        const float xf = (float)(x0 + k);
+        const float z = 1.0f / ((col ? m.g * xf : 0.0f) + zd);
+        const float xs = ((col ? m.a * xf : 0.0f) + xd) * z, ys = ((col ? m.d * xf : 0.0f) + yd) * z;
+        const int32_t xi = (int32_t)((uint32_t)cvtt(xs) - (uint32_t)(xs < 0.0f));
+        const int32_t yi = (int32_t)((uint32_t)cvtt(ys) - (uint32_t)(ys < 0.0f));
+        const bool mask = xi >= 0 && xi < (int32_t)sw && yi >= 0 && yi < (int32_t)sh;
+        const bool mask1 = xi >= -1 && xi < (int32_t)sw && yi >= -1 && yi < (int32_t)sh;
+        const float fx = xs - (float)xi, fy = ys - (float)yi;
+        const int32_t raw = (int32_t)((uint32_t)yi * srcStride + (uint32_t)xi);
+        const int32_t off = max(min(raw, limit), 0);
+        float q0, q1, q2, q3;
+        if (off != raw || !warp_lds4(lds, b, xi, yi, q0, q1, q2, q3)) {
+            q0 = mask ? (float)ldg8(src, n, off) : 0.0f;
+            q1 = mask1 ? (float)ldg8(src, n, (int64_t)off + 1) : 0.0f;
+            q2 = mask ? (float)ldg8(src, n, (int64_t)off + srcStride) : 0.0f;
+            q3 = mask1 ? (float)ldg8(src, n, (int64_t)off + srcStride + 1) : 0.0f;
+        }
+        const float p0 = mask ? q0 : bf, p1 = mask1 ? q1 : bf, p2 = mask ? q2 : bf, p3 = mask1 ? q3 : bf;
+        const float omx = 1.0f - fx, omy = 1.0f - fy;
+        float v = (p0 * omx) * omy;
+        v = v + (p1 * fx) * omy;
+        v = v + (p2 * omx) * fy;
+        v = v + (p3 * fx) * fy;
+        word |= sat8(cvtt(v)) << (8 * k);
+    }
+    *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+}
+
 // HafCpu_Remap_U8_U8_Bilinear[_Constant] (ago_haf_cpu_geometric.cpp:249-504).  The map is read at
 // byte y*mapStride + 4x for every written pixel (pixels past w use the following entries, as the
 // reference's 16-byte map loads do); entries past the end of the map read as (0, 0), the zeroed
@@ -467,6 +643,9 @@ bool warp_args_ok(uint32_t dw, uint32_t dh, const uint8_t *dst, uint32_t dstStri
     return true;
 }
 
+// the staged kernels read the source as dwords: a 4-byte aligned base and stride
+bool staged_ok(const uint8_t *src, uint32_t srcStride) { return ((uintptr_t)src & 3u) == 0 && (srcStride & 3u) == 0; }
+
 dim3 warp_grid(uint32_t dw, uint32_t dh, dim3 block) { return dim3((((dw + 3) / 4) + block.x - 1) / block.x, (dh + block.y - 1) / block.y); }
 
 // the corner test of ago_haf_cpu_geometric.cpp:1142-1155 (host float arithmetic, no contraction)
@@ -491,6 +670,17 @@ int warp_affine(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst, uin
     if (!matrix || !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride)) return MVX_ERROR_INVALID_PARAMETERS;
     const Affine m{matrix[0], matrix[1], matrix[2], matrix[3], matrix[4], matrix[5]};
     const int32_t limit = constant ? (int32_t)(sh * srcStride - 1u) : (int32_t)((sh - 1u) * srcStride - 1u);
+    if (staged_ok(src, srcStride)) {
+        const dim3 block(WTW / 4, WTH), grid((dw + WTW - 1) / WTW, (dh + WTH - 1) / WTH);
+        if (affine_needs_border(dw, dh, sw, sh, m))
+            hipLaunchKernelGGL(k_warp_affine_lds<true>, grid, block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src,
+                               srcStride, m, limit, constant ? border : 0u);
+        else
+            hipLaunchKernelGGL(k_warp_affine_lds<false>, grid, block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src,
+                               srcStride, m, limit, 0u);
+        MVX_LAUNCH_CHECK();
+        return MVX_SUCCESS;
+    }
     const dim3 block(64, 4);
     if (affine_needs_border(dw, dh, sw, sh, m))
         hipLaunchKernelGGL(k_warp_affine<true>, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw,
@@ -507,6 +697,13 @@ int warp_perspective(mvx_stream_t stream, uint32_t dw, uint32_t dh, uint8_t *dst
 {
     if (!matrix || !warp_args_ok(dw, dh, dst, dstStride, sw, sh, src, srcStride)) return MVX_ERROR_INVALID_PARAMETERS;
     const Persp m{matrix[0], matrix[1], matrix[2], matrix[3], matrix[4], matrix[5], matrix[6], matrix[7], matrix[8]};
+    if (staged_ok(src, srcStride)) {
+        const dim3 block(WTW / 4, WTH), grid((dw + WTW - 1) / WTW, (dh + WTH - 1) / WTH);
+        hipLaunchKernelGGL(k_warp_perspective_lds, grid, block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src, srcStride, m,
+                           border);
+        MVX_LAUNCH_CHECK();
+        return MVX_SUCCESS;
+    }
     const dim3 block(64, 4);
     hipLaunchKernelGGL(k_warp_perspective, warp_grid(dw, dh, block), block, 0, stream_of(stream), dw, dh, dst, dstStride, sw, sh, src,
                        srcStride, m, border);

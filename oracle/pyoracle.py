@@ -51,10 +51,10 @@ def split_levels(flat, dims):
     return out
 
 
-def _plane(src):
-    """a packed U8 image in the reference layout: stride ALIGN16(w), zero padding"""
+def _plane(src, stride=None):
+    """a packed U8 image in the reference layout: stride ALIGN16(w) (or the given one), zero padding"""
     h, w = src.shape
-    st = (w + 15) & ~15
+    st = (w + 15) & ~15 if stride is None else int(stride)
     out = np.zeros((h, st), np.uint8)
     out[:, :w] = src
     return out, st
@@ -323,11 +323,12 @@ class Oracle:
         return res + (hvc,) if with_score else res
 
     # ---- warps and remap ----
-    def warp_affine(self, src, dw, dh, m, constant=False, border=0, padded=False):
+    def warp_affine(self, src, dw, dh, m, constant=False, border=0, padded=False, src_stride=None):
         """(dst, unpinned); m = the 2x3 VX matrix as stored (6 floats); padded: return the full
-        ALIGN16(dw) rows, including the padding pixels the reference's 4-pixel stores write"""
+        ALIGN16(dw) rows, including the padding pixels the reference's 4-pixel stores write;
+        src_stride: a source row pitch other than ALIGN16(width) (zero padding)"""
         sh, sw = src.shape
-        s, ss = _plane(src)
+        s, ss = _plane(src, src_stride)
         ds = (dw + 15) & ~15
         d = np.zeros((dh, ds), np.uint8)
         u = np.zeros((dh, dw), np.uint8)
@@ -335,9 +336,9 @@ class Oracle:
                                              _p(u), dw)
         return (d if padded else np.ascontiguousarray(d[:, :dw])), u.astype(bool)
 
-    def warp_perspective(self, src, dw, dh, m, border=0, padded=False):
+    def warp_perspective(self, src, dw, dh, m, border=0, padded=False, src_stride=None):
         sh, sw = src.shape
-        s, ss = _plane(src)
+        s, ss = _plane(src, src_stride)
         ds = (dw + 15) & ~15
         d = np.zeros((dh, ds), np.uint8)
         u = np.zeros((dh, dw), np.uint8)

@@ -183,10 +183,10 @@ def test_gdf_amd_merged_kernel_names(vx, ctx, orc):
     g = L.vxCreateGraph(ctx)
     src = vx.create_image(ctx, w, h, "U8")
     m, p, m1, p1 = (vx.create_image(ctx, w, h, f) for f in ("S16", "U8", "S16", "U8"))
-    text = ("data src = $1\ndata mag = $2\ndata ph = $3\ndata mag1 = $4\ndata ph1 = $5\n"
-            "node com.amd.openvx.SobelMagnitudePhase_S16U8_U8_3x3 mag ph src\n"
-            "node com.amd.openvx.SobelMagnitude_S16_U8_3x3 mag1 src\n"
-            "node com.amd.openvx.SobelPhase_U8_U8_3x3 ph1 src\n")
+    # $n names the n-th reference handed to the import (the GDF grammar has no "data x = $n" form)
+    text = ("node com.amd.openvx.SobelMagnitudePhase_S16U8_U8_3x3 $2 $3 $1\n"
+            "node com.amd.openvx.SobelMagnitude_S16_U8_3x3 $4 $1\n"
+            "node com.amd.openvx.SobelPhase_U8_U8_3x3 $5 $1\n")
     assert vx.import_gdf(g, text, refs=(src, m, p, m1, p1)) == 0
     vx.write_image(src, img)
     assert L.vxProcessGraph(g) == 0

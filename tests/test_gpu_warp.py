@@ -83,6 +83,43 @@ def test_warp_edge_matrices(gpu, orc):
     assert_same(_padded(d), want, "perspective with z = 0 column")
 
 
+def test_warps_staging_limits(gpu, orc):
+    """transforms whose tile footprints exceed the staged box (zoom-out by 3 and by 40, a 45 degree
+    rotation, a perspective whose horizon crosses the output) and a source the staged kernels cannot
+    read as dwords (a base one byte past a 16-byte boundary, stride not a multiple of 4)"""
+    w, h = 700, 300
+    src = synth.scene_u8(w, h, 43)
+    s = Plane.from_array(src)
+    c, sn = np.cos(np.pi / 4), np.sin(np.pi / 4)
+    mats = [np.array([3, 0, 0, 3, 1.5, 0.5], np.float32), np.array([40, 0, 0, 40, -3, -2], np.float32),
+            np.array([c, sn, -sn, c, 200.25, 10.5], np.float32)]
+    for m in mats:
+        for border in (None, 5):
+            d = _run_warp(0, s, w, h, m, border)
+            want, _ = orc.warp_affine(src, w, h, m, border is not None, border or 0, padded=True)
+            assert_same(_padded(d), want, f"affine {m.tolist()} {border}")
+    p = np.array([1, 0.02, 0.002, 0.1, 1, 0.0005, 5, 3, -0.4], np.float32)
+    for border in (None, 9):
+        d = _run_warp(1, s, w, h, p, border)
+        want, _ = orc.warp_perspective(src, w, h, p, border or 0, padded=True)
+        assert_same(_padded(d), want, f"perspective across the horizon {border}")
+    # unaligned source: a packed copy at byte offset 1 with stride w + 1 (odd)
+    st = w + 1
+    raw = torch.zeros(h * st + 64, dtype=torch.uint8, device="cuda")
+    raw[1:1 + h * st].view(h, st)[:, :w].copy_(torch.from_numpy(src).cuda())
+    aff, per = synth.warp_matrices(w, h, 3)
+    padded = np.zeros((h, st), np.uint8)
+    padded[:, :w] = src
+    for kind, m in ((0, aff), (1, per)):
+        d = Plane(w, h)
+        keep, mp = _fptr(m)
+        name = ("WarpAffine" if kind == 0 else "WarpPerspective") + "_U8_U8_Bilinear_Constant"
+        call(name, w, h, d, d.stride, w, h, ctypes.c_void_p(raw.data_ptr() + 1), st, mp, ctypes.c_uint8(7))
+        fn = orc.warp_affine if kind == 0 else orc.warp_perspective
+        want, _ = (fn(src, w, h, m, True, 7, padded=True, src_stride=st) if kind == 0 else fn(src, w, h, m, 7, padded=True, src_stride=st))
+        assert_same(_padded(d), want, f"unaligned source kind {kind}")
+
+
 def _remap(src_plane, mp, border, nearest=False):
     dh, dw = mp.shape[:2]
     fmap = torch.from_numpy(np.ascontiguousarray(mp)).cuda()
