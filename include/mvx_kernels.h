@@ -269,6 +269,17 @@ int mvx_NonMaxSupp_XY_ANY_3x3(mvx_stream_t stream, uint32_t capacity, void *pDst
                               uint32_t srcWidth, uint32_t srcHeight, const float *pSrc, uint32_t srcStride,
                               void *workspace, size_t workspaceSize);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): mvx_NonMaxSupp_XY_ANY_3x3
+ * without the workspace argument.  Its counts and offsets live in a library-owned device array of
+ * MVX_NONMAXSUPP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), enough for images
+ * whose mvx_NonMaxSupp_XY_ANY_3x3_workspace_size is at most that (e.g. 8192 x 4096); larger images
+ * return MVX_ERROR_NO_MEMORY and launch nothing.  Nothing is allocated or freed.  Every call on one
+ * GPU shares that GPU's array, so calls that overlap on one GPU (other streams, other threads) race. */
+#define MVX_NONMAXSUPP_NOALLOC_SCRATCH_BYTES (1u << 20)
+int mvx_NonMaxSupp_XY_ANY_3x3_NoAlloc(mvx_stream_t stream, uint32_t capacity, void *pDstList, uint32_t *pDstCount,
+                                      uint32_t srcWidth, uint32_t srcHeight, const float *pSrc, uint32_t srcStride);
+
 /* replace HipExec_HarrisSobel_HG3_U8_5x5 / _7x7 (hip_host_decls.h:872, :876); HafCpu_HarrisSobel_HG3_U8_5x5
  * / _7x7 (ago_haf_cpu_harris.cpp:333-594): rows R..h-R-1 (R = 2, 3), written from byte R*stride as packed
  * rows of w pixels, as the reference writes them. */

@@ -10,6 +10,7 @@
 // sequential result exactly, without copying the list back to the host.
 #include "mvx_common.hpp"
 #include "pick.hpp"
+#include <mutex>
 
 namespace mvx {
 
@@ -352,6 +353,47 @@ extern "C" int mvx_NonMaxSupp_XY_ANY_3x3(mvx_stream_t stream, uint32_t capacity,
                        (XYS *)dstList, capacity, dstCount);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
+}
+
+// ---------------------------------------------------------------- allocation-free NonMaxSupp ------
+// The per-block counts and offsets live in one file-scope device array per GPU instead of a
+// caller's workspace.  Host code never passes the array's name to a kernel: the name is a host-side
+// stand-in; the device address comes from hipGetSymbolAddress, once per device, on the first call
+// made with that device current.
+static __device__ __align__(256) uint8_t scratch[MVX_NONMAXSUPP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[MAX_DEVICES];
+
+int scratch_ptr(void **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return MVX_FAILURE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MVX_FAILURE;
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return MVX_SUCCESS;
+}
+}  // namespace
+
+extern "C" int mvx_NonMaxSupp_XY_ANY_3x3_NoAlloc(mvx_stream_t stream, uint32_t capacity, void *dstList, uint32_t *dstCount,
+                                                 uint32_t w, uint32_t h, const float *src, uint32_t srcStride)
+{
+    if (!dstList || !dstCount || !plane_ok_h(src, w, srcStride, 4)) return MVX_ERROR_INVALID_PARAMETERS;
+    const size_t need = mvx_NonMaxSupp_XY_ANY_3x3_workspace_size(w, h);
+    if (need > MVX_NONMAXSUPP_NOALLOC_SCRATCH_BYTES) return MVX_ERROR_NO_MEMORY;
+    void *ws = nullptr;
+    const int rc = scratch_ptr(&ws);
+    if (rc != MVX_SUCCESS) return rc;
+    // k_nms_count writes every count and k_scan_u32 every offset the write pass reads, so nothing
+    // left by an earlier call needs clearing
+    return mvx_NonMaxSupp_XY_ANY_3x3(stream, capacity, dstList, dstCount, w, h, src, srcStride, ws, need);
 }
 
 extern "C" size_t mvx_HarrisMergeSortAndPick_XY_XYS_workspace_size(uint32_t maxList)
