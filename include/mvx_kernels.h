@@ -137,6 +137,76 @@ int mvx_ColorConvert_IYUV_RGB(mvx_stream_t stream, uint32_t dstWidth, uint32_t d
                               uint8_t *pDstU, uint32_t uStride, uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrc,
                               uint32_t srcStride);
 
+/* The rest of the ColorConvert family (ago_haf_cpu_color_convert.cpp), each replacing the HipExec_ launcher
+ * of the same name (hip_host_decls.h:495-629) with the results of the HafCpu_ function of the same name.
+ * Widths (and, for 4:2:0 data, heights) must be even; planes as in the conventions above; 4:2:0 chroma
+ * planes are (w/2) x (h/2) samples (NV12 / NV21: interleaved pairs, 2 bytes per sample).
+ *   YUV -> RGB / RGBX, BT.709 (:280-1732; the 4:2:2 scalar tails, columns w & ~7 .. w-1, group G's terms
+ *   differently): */
+int mvx_ColorConvert_RGB_UYVY(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ColorConvert_RGB_YUYV(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ColorConvert_RGBX_UYVY(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                               const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ColorConvert_RGBX_YUYV(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                               const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ColorConvert_RGB_NV21(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pSrcLuma, uint32_t lumaStride, const uint8_t *pSrcChroma, uint32_t chromaStride);
+int mvx_ColorConvert_RGBX_NV12(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                               const uint8_t *pSrcLuma, uint32_t lumaStride, const uint8_t *pSrcChroma, uint32_t chromaStride);
+int mvx_ColorConvert_RGBX_NV21(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                               const uint8_t *pSrcLuma, uint32_t lumaStride, const uint8_t *pSrcChroma, uint32_t chromaStride);
+int mvx_ColorConvert_RGBX_IYUV(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                               const uint8_t *pSrcY, uint32_t yStride, const uint8_t *pSrcU, uint32_t uStride, const uint8_t *pSrcV,
+                               uint32_t vStride);
+/*   RGB / RGBX -> NV12, IYUV, YUV4 (the merged kernels the reference's node-merge rules select,
+ *   ago_drama_remove.cpp:38-100: NV12_RGB :2707, IYUV_RGBX :3868, NV12_RGBX :4078, YUV4_RGB :4282,
+ *   YUV4_RGBX :3740): */
+int mvx_ColorConvert_NV12_RGB(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstLuma, uint32_t lumaStride,
+                              uint8_t *pDstChroma, uint32_t chromaStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ColorConvert_NV12_RGBX(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstLuma, uint32_t lumaStride,
+                               uint8_t *pDstChroma, uint32_t chromaStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ColorConvert_IYUV_RGBX(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstY, uint32_t yStride,
+                               uint8_t *pDstU, uint32_t uStride, uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrc,
+                               uint32_t srcStride);
+int mvx_ColorConvert_YUV4_RGB(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstY, uint32_t yStride,
+                              uint8_t *pDstU, uint32_t uStride, uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrc,
+                              uint32_t srcStride);
+int mvx_ColorConvert_YUV4_RGBX(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstY, uint32_t yStride,
+                               uint8_t *pDstU, uint32_t uStride, uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrc,
+                               uint32_t srcStride);
+/*   RGBX -> luma alone (ColorConvert_Y_RGBX :3367, what the reference keeps when only Y is read): */
+int mvx_ColorConvert_Y_RGBX(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstY, uint32_t yStride,
+                            const uint8_t *pSrc, uint32_t srcStride);
+/*   RGB <-> RGBX (:2330, :2399): */
+int mvx_ColorConvert_RGBX_RGB(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pSrc, uint32_t srcStride);
+int mvx_ColorConvert_RGB_RGBX(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                              const uint8_t *pSrc, uint32_t srcStride);
+/*   YUV -> YUV byte moves: 4:2:2 -> 4:2:0 with the chroma rows' floor average (:55, :1733, :1958, :2144);
+ *   UV12_IUV interleaves and IUV_UV12 de-interleaves (cw x ch = the chroma plane's samples; NV21 callers
+ *   swap the U / V pointers, as the reference's divider does); UV_UV12 and ScaleUp2x2 replicate each
+ *   chroma sample over a 2x2 block of full-size planes (dstWidth x dstHeight = the full size). */
+int mvx_FormatConvert_IYUV_UYVY(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstY, uint32_t yStride,
+                                uint8_t *pDstU, uint32_t uStride, uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrc,
+                                uint32_t srcStride);
+int mvx_FormatConvert_IYUV_YUYV(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstY, uint32_t yStride,
+                                uint8_t *pDstU, uint32_t uStride, uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrc,
+                                uint32_t srcStride);
+int mvx_FormatConvert_NV12_UYVY(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstLuma, uint32_t lumaStride,
+                                uint8_t *pDstChroma, uint32_t chromaStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_FormatConvert_NV12_YUYV(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstLuma, uint32_t lumaStride,
+                                uint8_t *pDstChroma, uint32_t chromaStride, const uint8_t *pSrc, uint32_t srcStride);
+int mvx_FormatConvert_IUV_UV12(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstU, uint32_t uStride,
+                               uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrcChroma, uint32_t chromaStride);
+int mvx_FormatConvert_UV12_IUV(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstChroma, uint32_t chromaStride,
+                               const uint8_t *pSrcU, uint32_t uStride, const uint8_t *pSrcV, uint32_t vStride);
+int mvx_FormatConvert_UV_UV12(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDstU, uint32_t uStride,
+                              uint8_t *pDstV, uint32_t vStride, const uint8_t *pSrcChroma, uint32_t chromaStride);
+int mvx_ScaleUp2x2_U8_U8(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
+                         const uint8_t *pSrc, uint32_t srcStride);
+
 /* replaces HipExec_ScaleImage_U8_U8_Bilinear (hip_host_decls.h:728); HafCpu_ScaleImage_U8_U8_Bilinear
  * (ago_haf_cpu_geometric.cpp:1977-2105) with the scale matrix of ago_kernel_api.cpp:19897-19916.
  * dstStride must be a multiple of 16 (8-byte stores). */

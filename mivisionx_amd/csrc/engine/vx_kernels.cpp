@@ -201,16 +201,18 @@ template <int IO> vx_status x_phase(_vx_node *n, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------- ColorConvert (A2) --------
-// RGB -> IYUV: a non-virtual IYUV output gets all three planes (COLOR_CONVERT_IYUV_RGB,
-// ago_kernel_api.cpp:13447; HafCpu_ColorConvert_IYUV_RGB).  A virtual IYUV image read only through its Y
-// channel keeps only the luma kernel, ColorConvert_Y_RGB, as the reference's graph optimizer does
-// (ago_drama_divide.cpp:244-256 and the unused-output removal, ago_drama_remove.cpp:989).  RGB -> NV12 /
-// YUV4 chroma planes are not produced.
-// IYUV / NV12 -> RGB: COLOR_CONVERT_RGB_IYUV / _RGB_NV12 (ago_drama_divide.cpp:80-113).
+// The conversions the reference's divider accepts (ago_drama_divide.cpp:40-325) and the kernels its
+// node-merge rules leave (ago_drama_remove.cpp:38-100):
+//   YUV (UYVY, YUYV, IYUV, NV12, NV21) -> RGB / RGBX: COLOR_CONVERT_RGB[X]_<fmt>;
+//   RGB / RGBX -> IYUV, NV12, YUV4: the merged COLOR_CONVERT_<fmt>_RGB[X] kernel when every plane is used,
+//     only the luma kernel (COLOR_CONVERT_Y_RGB[X]) when the output is virtual and read only through its
+//     Y channel (the divider's chroma kernels have no readers and are removed, ago_drama_remove.cpp:989);
+//   RGB <-> RGBX; UYVY / YUYV -> IYUV / NV12 (FORMAT_CONVERT_*); IYUV -> NV12 (a luma copy plus
+//     UV12_IUV), IYUV -> YUV4 (copy plus ScaleUp2x2 of U and V), NV12 / NV21 -> IYUV (copy plus IUV_UV12
+//     with the chroma outputs swapped for NV21) and -> YUV4 (copy plus UV_UV12).
+// 4:2:2 images need an even width, 4:2:0 images an even width and height.
 // true when every node of the graph that reads `yuv` is a channel_extract of its Y channel: the RGB
-// conversion then needs only its luma kernel (the reference divides RGB -> IYUV into Y_RGB, IU_RGB and
-// IV_RGB, ago_drama_divide.cpp:244-256, and drops the kernels whose outputs nobody reads,
-// ago_drama_remove.cpp:989)
+// conversion then needs only its luma kernel
 bool luma_only_readers(_vx_node *n, _vx_image *yuv)
 {
     for (_vx_node *m : n->graph->nodes)
@@ -225,45 +227,101 @@ bool luma_only_readers(_vx_node *n, _vx_image *yuv)
 
 enum : uint32_t { COLOR_ALL_PLANES = 0, COLOR_LUMA_ONLY = 1 };
 
+bool is_rgb(vx_df_image f) { return f == VX_DF_IMAGE_RGB || f == VX_DF_IMAGE_RGBX; }
+bool is_422(vx_df_image f) { return f == VX_DF_IMAGE_UYVY || f == VX_DF_IMAGE_YUYV; }
+bool is_420(vx_df_image f) { return f == VX_DF_IMAGE_IYUV || f == VX_DF_IMAGE_NV12 || f == VX_DF_IMAGE_NV21; }
+
+bool color_pair_ok(vx_df_image in, vx_df_image out)
+{
+    if (is_rgb(in)) return (is_rgb(out) && out != in) || out == VX_DF_IMAGE_IYUV || out == VX_DF_IMAGE_NV12 || out == VX_DF_IMAGE_YUV4;
+    if (is_422(in)) return is_rgb(out) || out == VX_DF_IMAGE_IYUV || out == VX_DF_IMAGE_NV12;
+    if (in == VX_DF_IMAGE_IYUV) return is_rgb(out) || out == VX_DF_IMAGE_NV12 || out == VX_DF_IMAGE_YUV4;
+    if (in == VX_DF_IMAGE_NV12 || in == VX_DF_IMAGE_NV21) return is_rgb(out) || out == VX_DF_IMAGE_IYUV || out == VX_DF_IMAGE_YUV4;
+    return false;
+}
+
 vx_status v_color(_vx_node *n)
 {
-    _vx_image *src = img(n, 0);
-    if (src && (src->format == VX_DF_IMAGE_IYUV || src->format == VX_DF_IMAGE_NV12)) {
-        _vx_image *out = img(n, 1);
-        if (!out) return fail(n, VX_ERROR_INVALID_PARAMETERS, "output is not an image");
-        if (out->format != VX_DF_IMAGE_RGB && out->format != VX_DF_IMAGE_VIRT)
-            return fail(n, VX_ERROR_NOT_SUPPORTED, "YUV converts to RGB here");
-        if ((src->width & 1u) || (src->height & 1u)) return fail(n, VX_ERROR_INVALID_DIMENSION, "YUV 4:2:0 sizes must be even");
-        return set_output(n, 1, src->width, src->height, VX_DF_IMAGE_RGB);
-    }
-    vx_status s = need_input(n, 0, VX_DF_IMAGE_RGB);
-    if (s) return s;
-    _vx_image *o = img(n, 1);
+    _vx_image *in = img(n, 0), *o = img(n, 1);
+    if (!in) return fail(n, VX_ERROR_INVALID_PARAMETERS, "input is not an image");
     if (!o) return fail(n, VX_ERROR_INVALID_PARAMETERS, "output is not an image");
-    const vx_df_image want = o->format == VX_DF_IMAGE_VIRT ? VX_DF_IMAGE_IYUV : o->format;
-    if (want != VX_DF_IMAGE_IYUV && want != VX_DF_IMAGE_NV12 && want != VX_DF_IMAGE_YUV4)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "RGB converts to the luma plane of IYUV / NV12 / YUV4 only");
-    if ((s = set_output(n, 1, img(n, 0)->width, img(n, 0)->height, want))) return s;
-    n->variant = (o->isVirtual && luma_only_readers(n, o)) ? COLOR_LUMA_ONLY : COLOR_ALL_PLANES;
-    if (n->variant != COLOR_LUMA_ONLY && want != VX_DF_IMAGE_IYUV)
-        return fail(n, VX_ERROR_NOT_SUPPORTED, "RGB -> NV12 / YUV4 chroma planes are not produced: the output must be IYUV, or a "
-                                               "virtual image read only through channel_extract(Y)");
-    if (n->variant != COLOR_LUMA_ONLY && ((o->width & 1u) || (o->height & 1u)))
-        return fail(n, VX_ERROR_INVALID_DIMENSION, "YUV 4:2:0 sizes must be even");
+    if (in->width == 0 || in->height == 0) return fail(n, VX_ERROR_INVALID_DIMENSION, "input dimensions");
+    const vx_df_image want = o->format != VX_DF_IMAGE_VIRT ? o->format : is_rgb(in->format) ? VX_DF_IMAGE_IYUV : VX_DF_IMAGE_RGB;
+    if (!color_pair_ok(in->format, want)) return fail(n, VX_ERROR_NOT_SUPPORTED, "color conversion between these formats");
+    const bool yuv420 = is_420(in->format) || is_420(want), yuv422 = is_422(in->format);
+    vx_status s = set_output(n, 1, in->width, in->height, want);
+    if (s) return s;
+    n->variant = (is_rgb(in->format) && !is_rgb(want) && o->isVirtual && luma_only_readers(n, o)) ? COLOR_LUMA_ONLY : COLOR_ALL_PLANES;
+    if (n->variant == COLOR_LUMA_ONLY) return VX_SUCCESS;   // the luma kernels take any size
+    if ((yuv420 || yuv422) && (in->width & 1u)) return fail(n, VX_ERROR_INVALID_DIMENSION, "YUV widths must be even");
+    if (yuv420 && (in->height & 1u)) return fail(n, VX_ERROR_INVALID_DIMENSION, "YUV 4:2:0 heights must be even");
     return VX_SUCCESS;
 }
 
 vx_status x_color(_vx_node *n, hipStream_t st)
 {
     _vx_image *in = img(n, 0), *o = img(n, 1);
-    if (in->format == VX_DF_IMAGE_IYUV)
-        return rc(n, mvx_ColorConvert_RGB_IYUV(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1), P(in, 2), S(in, 2)));
-    if (in->format == VX_DF_IMAGE_NV12)
-        return rc(n, mvx_ColorConvert_RGB_NV12(st, o->width, o->height, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1)));
-    if (n->variant != COLOR_LUMA_ONLY && o->format == VX_DF_IMAGE_IYUV)
-        return rc(n, mvx_ColorConvert_IYUV_RGB(st, o->width, o->height, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in),
-                                               S(in)));
-    return rc(n, mvx_ColorConvert_Y_RGB(st, o->width, o->height, P(o), S(o), P(in), S(in)));
+    const uint32_t w = o->width, h = o->height;
+    const vx_df_image fi = in->format, fo = o->format;
+    if (n->variant == COLOR_LUMA_ONLY)
+        return rc(n, fi == VX_DF_IMAGE_RGB ? mvx_ColorConvert_Y_RGB(st, w, h, P(o), S(o), P(in), S(in))
+                                           : mvx_ColorConvert_Y_RGBX(st, w, h, P(o), S(o), P(in), S(in)));
+    // YUV -> RGB / RGBX
+    if (is_rgb(fo) && !is_rgb(fi)) {
+        const bool x = fo == VX_DF_IMAGE_RGBX;
+        switch (fi) {
+        case VX_DF_IMAGE_UYVY:
+            return rc(n, x ? mvx_ColorConvert_RGBX_UYVY(st, w, h, P(o), S(o), P(in), S(in)) : mvx_ColorConvert_RGB_UYVY(st, w, h, P(o), S(o), P(in), S(in)));
+        case VX_DF_IMAGE_YUYV:
+            return rc(n, x ? mvx_ColorConvert_RGBX_YUYV(st, w, h, P(o), S(o), P(in), S(in)) : mvx_ColorConvert_RGB_YUYV(st, w, h, P(o), S(o), P(in), S(in)));
+        case VX_DF_IMAGE_IYUV:
+            return rc(n, x ? mvx_ColorConvert_RGBX_IYUV(st, w, h, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1), P(in, 2), S(in, 2))
+                           : mvx_ColorConvert_RGB_IYUV(st, w, h, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1), P(in, 2), S(in, 2)));
+        case VX_DF_IMAGE_NV12:
+            return rc(n, x ? mvx_ColorConvert_RGBX_NV12(st, w, h, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1))
+                           : mvx_ColorConvert_RGB_NV12(st, w, h, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1)));
+        default:
+            return rc(n, x ? mvx_ColorConvert_RGBX_NV21(st, w, h, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1))
+                           : mvx_ColorConvert_RGB_NV21(st, w, h, P(o), S(o), P(in, 0), S(in, 0), P(in, 1), S(in, 1)));
+        }
+    }
+    if (is_rgb(fi) && is_rgb(fo))
+        return rc(n, fo == VX_DF_IMAGE_RGBX ? mvx_ColorConvert_RGBX_RGB(st, w, h, P(o), S(o), P(in), S(in))
+                                            : mvx_ColorConvert_RGB_RGBX(st, w, h, P(o), S(o), P(in), S(in)));
+    // RGB / RGBX -> YUV (every plane)
+    if (is_rgb(fi)) {
+        const bool x = fi == VX_DF_IMAGE_RGBX;
+        if (fo == VX_DF_IMAGE_IYUV)
+            return rc(n, x ? mvx_ColorConvert_IYUV_RGBX(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in), S(in))
+                           : mvx_ColorConvert_IYUV_RGB(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in), S(in)));
+        if (fo == VX_DF_IMAGE_NV12)
+            return rc(n, x ? mvx_ColorConvert_NV12_RGBX(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(in), S(in))
+                           : mvx_ColorConvert_NV12_RGB(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(in), S(in)));
+        return rc(n, x ? mvx_ColorConvert_YUV4_RGBX(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in), S(in))
+                       : mvx_ColorConvert_YUV4_RGB(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in), S(in)));
+    }
+    // 4:2:2 -> 4:2:0
+    if (is_422(fi)) {
+        const bool u = fi == VX_DF_IMAGE_UYVY;
+        if (fo == VX_DF_IMAGE_IYUV)
+            return rc(n, u ? mvx_FormatConvert_IYUV_UYVY(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in), S(in))
+                           : mvx_FormatConvert_IYUV_YUYV(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(o, 2), S(o, 2), P(in), S(in)));
+        return rc(n, u ? mvx_FormatConvert_NV12_UYVY(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(in), S(in))
+                       : mvx_FormatConvert_NV12_YUYV(st, w, h, P(o, 0), S(o, 0), P(o, 1), S(o, 1), P(in), S(in)));
+    }
+    // 4:2:0 -> 4:2:0 / YUV4: the luma plane copied, the chroma moved
+    int r = mvx_ChannelCopy_U8_U8(st, w, h, P(o, 0), S(o, 0), P(in, 0), S(in, 0));
+    if (r) return rc(n, r);
+    if (fi == VX_DF_IMAGE_IYUV) {
+        if (fo == VX_DF_IMAGE_NV12) return rc(n, mvx_FormatConvert_UV12_IUV(st, w / 2, h / 2, P(o, 1), S(o, 1), P(in, 1), S(in, 1), P(in, 2), S(in, 2)));
+        r = mvx_ScaleUp2x2_U8_U8(st, w, h, P(o, 1), S(o, 1), P(in, 1), S(in, 1));
+        return rc(n, r ? r : mvx_ScaleUp2x2_U8_U8(st, w, h, P(o, 2), S(o, 2), P(in, 2), S(in, 2)));
+    }
+    const bool nv21 = fi == VX_DF_IMAGE_NV21;   // NV21 holds V before U: the chroma outputs swap
+    const int ou = nv21 ? 2 : 1, ov = nv21 ? 1 : 2;
+    if (fo == VX_DF_IMAGE_IYUV)
+        return rc(n, mvx_FormatConvert_IUV_UV12(st, w / 2, h / 2, P(o, ou), S(o, ou), P(o, ov), S(o, ov), P(in, 1), S(in, 1)));
+    return rc(n, mvx_FormatConvert_UV_UV12(st, w, h, P(o, ou), S(o, ou), P(o, ov), S(o, ov), P(in, 1), S(in, 1)));
 }
 
 vx_status v_amd_color_y(_vx_node *n)

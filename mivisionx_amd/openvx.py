@@ -41,7 +41,8 @@ VX_ERROR_MULTIPLE_WRITERS = -23
 
 DF = {"U8": _df("U", "0", "0", "8"), "S16": _df("S", "0", "1", "6"), "U16": _df("U", "0", "1", "6"),
       "RGB": _df("R", "G", "B", "2"), "RGBX": _df("R", "G", "B", "A"), "IYUV": _df("I", "Y", "U", "V"),
-      "NV12": _df("N", "V", "1", "2"), "YUV4": _df("Y", "U", "V", "4"), "VIRT": _df("V", "I", "R", "T")}
+      "NV12": _df("N", "V", "1", "2"), "NV21": _df("N", "V", "2", "1"), "UYVY": _df("U", "Y", "V", "Y"),
+      "YUYV": _df("Y", "U", "Y", "V"), "YUV4": _df("Y", "U", "V", "4"), "VIRT": _df("V", "I", "R", "T")}
 BPP = {"U8": 1, "S16": 2, "U16": 2, "RGB": 3, "RGBX": 4}
 NPT = {"U8": np.uint8, "S16": np.int16, "U16": np.uint16, "RGB": np.uint8, "RGBX": np.uint8}
 
@@ -248,6 +249,35 @@ def write_image(img, arr, plane=0, sub=1):
     addr = Addressing(w, h, bpp, w * bpp, 1024, 1024, 1, 1, 0)
     check(lib().vxCopyImagePatch(img, c.byref(Rect(0, 0, w * sub, h * sub)), plane, c.byref(addr), arr.ctypes.data_as(P),
                                  VX_WRITE_ONLY, VX_MEMORY_TYPE_HOST), "vxCopyImagePatch(write)")
+
+
+def _plane_layout(fmt, w, h):
+    """(samples across, rows, bytes per sample, subsampling) of each plane of a w x h image of `fmt`"""
+    return {"RGB": [(w, h, 3, 1)], "RGBX": [(w, h, 4, 1)], "UYVY": [(w, h, 2, 1)], "YUYV": [(w, h, 2, 1)],
+            "IYUV": [(w, h, 1, 1), (w // 2, h // 2, 1, 2), (w // 2, h // 2, 1, 2)],
+            "NV12": [(w, h, 1, 1), (w // 2, h // 2, 2, 2)], "NV21": [(w, h, 1, 1), (w // 2, h // 2, 2, 2)],
+            "YUV4": [(w, h, 1, 1)] * 3, "U8": [(w, h, 1, 1)]}[fmt]
+
+
+def write_planes(img, fmt, w, h, planes):
+    """every plane of an image from tightly packed host arrays (rows of samples x bytes per sample)"""
+    for k, ((pw, ph, bpp, sub), arr) in enumerate(zip(_plane_layout(fmt, w, h), planes)):
+        arr = np.ascontiguousarray(arr, dtype=np.uint8).reshape(ph, pw * bpp)
+        addr = Addressing(w, h, bpp, pw * bpp, 1024, 1024, sub, sub, 0)
+        check(lib().vxCopyImagePatch(img, c.byref(Rect(0, 0, w, h)), k, c.byref(addr), arr.ctypes.data_as(P), VX_WRITE_ONLY,
+                                     VX_MEMORY_TYPE_HOST), "vxCopyImagePatch(write)")
+
+
+def read_planes(img, fmt, w, h):
+    """every plane of an image as tightly packed host arrays (rows of samples x bytes per sample)"""
+    out = []
+    for k, (pw, ph, bpp, sub) in enumerate(_plane_layout(fmt, w, h)):
+        arr = np.zeros((ph, pw * bpp), np.uint8)
+        addr = Addressing(w, h, bpp, pw * bpp, 1024, 1024, sub, sub, 0)
+        check(lib().vxCopyImagePatch(img, c.byref(Rect(0, 0, w, h)), k, c.byref(addr), arr.ctypes.data_as(P), VX_READ_ONLY,
+                                     VX_MEMORY_TYPE_HOST), "vxCopyImagePatch(read)")
+        out.append(arr)
+    return out
 
 
 def read_image(img, fmt=None, plane=0):

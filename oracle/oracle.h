@@ -149,7 +149,19 @@ int oracle_harris_node(int w, int h, const uint8_t *src, float k, float thr, flo
 
 void oracle_color_iyuv_from_rgb(int w, int h, const uint8_t *rgb, int rgbStride, uint8_t *y, int yStride, uint8_t *u,
                                 int uStride, uint8_t *v, int vStride);
+/* colour conversions beyond RGB <-> IYUV (color.c); formats of the YUV inputs */
+enum { ORACLE_FMT_IYUV = 0, ORACLE_FMT_NV12 = 1, ORACLE_FMT_NV21 = 2, ORACLE_FMT_UYVY = 3, ORACLE_FMT_YUYV = 4 };
+void oracle_cc_rgb_from_yuv(int fmt, int out_bpp, int w, int h, uint8_t *dst, int dst_stride, const uint8_t *p0, int s0,
+                            const uint8_t *p1, int s1, const uint8_t *p2, int s2);
+void oracle_cc_yuv420_from_rgb(int in_bpp, int nv12, int w, int h, const uint8_t *src, int src_stride, uint8_t *yp, int ys,
+                               uint8_t *up, int us, uint8_t *vp, int vs);
+void oracle_cc_yuv4_from_rgb(int in_bpp, int w, int h, const uint8_t *src, int src_stride, uint8_t *yp, int ys, uint8_t *up,
+                             int us, uint8_t *vp, int vs);
+void oracle_cc_rgb_rgbx(int to_rgbx, int w, int h, const uint8_t *src, int src_stride, uint8_t *dst, int dst_stride);
+int oracle_cc_format(uint32_t in_fmt, uint32_t out_fmt, int w, int h, const uint8_t *in, uint8_t *out);
+void oracle_cc_y_rgbx(int w, int h, const uint8_t *src, int src_stride, uint8_t *dst, int dst_stride);
 #ifdef __cplusplus
 }
+
 #endif
 #endif

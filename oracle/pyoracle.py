@@ -69,6 +69,42 @@ def _f32(m):
     return np.ascontiguousarray(np.asarray(m, np.float32).reshape(-1))
 
 
+# colour formats: the VX fourcc codes and each format's planes as (rows, bytes per row) of a w x h image
+def _fourcc(t):
+    return ord(t[0]) | ord(t[1]) << 8 | ord(t[2]) << 16 | ord(t[3]) << 24
+
+
+COLOR_FORMATS = {"RGB": _fourcc("RGB2"), "RGBX": _fourcc("RGBA"), "UYVY": _fourcc("UYVY"), "YUYV": _fourcc("YUYV"),
+                 "IYUV": _fourcc("IYUV"), "NV12": _fourcc("NV12"), "NV21": _fourcc("NV21"), "YUV4": _fourcc("YUV4")}
+# the conversions the reference's CPU engine runs (ago_drama_divide.cpp's ColorConvert divider)
+COLOR_CONVERSIONS = [("RGB", "RGBX"), ("RGB", "IYUV"), ("RGB", "NV12"), ("RGB", "YUV4"),
+                     ("RGBX", "RGB"), ("RGBX", "IYUV"), ("RGBX", "NV12"), ("RGBX", "YUV4"),
+                     ("UYVY", "RGB"), ("UYVY", "RGBX"), ("UYVY", "IYUV"), ("UYVY", "NV12"),
+                     ("YUYV", "RGB"), ("YUYV", "RGBX"), ("YUYV", "IYUV"), ("YUYV", "NV12"),
+                     ("IYUV", "RGB"), ("IYUV", "RGBX"), ("IYUV", "NV12"), ("IYUV", "YUV4"),
+                     ("NV12", "RGB"), ("NV12", "RGBX"), ("NV12", "IYUV"), ("NV12", "YUV4"),
+                     ("NV21", "RGB"), ("NV21", "RGBX"), ("NV21", "IYUV"), ("NV21", "YUV4")]
+
+
+def color_planes(fmt, w, h):
+    """[(rows, bytes per row)] of each plane of a w x h image of `fmt`"""
+    return {"RGB": [(h, 3 * w)], "RGBX": [(h, 4 * w)], "UYVY": [(h, 2 * w)], "YUYV": [(h, 2 * w)],
+            "IYUV": [(h, w), (h // 2, w // 2), (h // 2, w // 2)], "NV12": [(h, w), (h // 2, w)], "NV21": [(h, w), (h // 2, w)],
+            "YUV4": [(h, w), (h, w), (h, w)]}[fmt]
+
+
+def split_planes(fmt, w, h, packed):
+    out, off = [], 0
+    for rows, rb in color_planes(fmt, w, h):
+        out.append(np.ascontiguousarray(packed[off:off + rows * rb].reshape(rows, rb)))
+        off += rows * rb
+    return out
+
+
+def join_planes(planes):
+    return np.concatenate([np.ascontiguousarray(p).reshape(-1) for p in planes])
+
+
 class Oracle:
     def __init__(self):
         if not os.path.exists(ORACLE_SO):
@@ -209,6 +245,45 @@ class Oracle:
         v = np.zeros((h // 2, w // 2), np.uint8)
         self.lib.oracle_color_iyuv_from_rgb(w, h, _p(_c(rgb)), 3 * w, _p(y), w, _p(u), w // 2, _p(v), w // 2)
         return y, u, v
+
+    def color_convert(self, src_fmt, dst_fmt, w, h, planes):
+        """color_convert between any two formats the reference converts (COLOR_CONVERSIONS): a list of
+        the input's planes (tight rows) -> a list of the output's planes"""
+        out = [np.zeros(shape, np.uint8) for shape in color_planes(dst_fmt, w, h)]
+        ins = [np.ascontiguousarray(p, dtype=np.uint8) for p in planes]
+        st = lambda a: a.shape[1]
+        L = self.lib
+        yuv = {"IYUV": 0, "NV12": 1, "NV21": 2, "UYVY": 3, "YUYV": 4}
+        if dst_fmt in ("RGB", "RGBX") and src_fmt in yuv:
+            p = ins + [ins[0]] * (3 - len(ins))
+            L.oracle_cc_rgb_from_yuv(yuv[src_fmt], 3 if dst_fmt == "RGB" else 4, w, h, _p(out[0]), st(out[0]), _p(p[0]), st(p[0]),
+                                     _p(p[1]), st(p[1]), _p(p[2]), st(p[2]))
+        elif src_fmt in ("RGB", "RGBX") and dst_fmt in ("IYUV", "NV12"):
+            o = out + [out[1]] * (3 - len(out))
+            L.oracle_cc_yuv420_from_rgb(3 if src_fmt == "RGB" else 4, int(dst_fmt == "NV12"), w, h, _p(ins[0]), st(ins[0]), _p(o[0]),
+                                        st(o[0]), _p(o[1]), st(o[1]), _p(o[2]), st(o[2]))
+        elif src_fmt in ("RGB", "RGBX") and dst_fmt == "YUV4":
+            L.oracle_cc_yuv4_from_rgb(3 if src_fmt == "RGB" else 4, w, h, _p(ins[0]), st(ins[0]), _p(out[0]), w, _p(out[1]), w,
+                                      _p(out[2]), w)
+        elif (src_fmt, dst_fmt) in (("RGB", "RGBX"), ("RGBX", "RGB")):
+            L.oracle_cc_rgb_rgbx(int(dst_fmt == "RGBX"), w, h, _p(ins[0]), st(ins[0]), _p(out[0]), st(out[0]))
+        else:
+            flat = np.zeros(sum(r * b for r, b in color_planes(dst_fmt, w, h)), np.uint8)
+            if L.oracle_cc_format(COLOR_FORMATS[src_fmt], COLOR_FORMATS[dst_fmt], w, h, _p(join_planes(ins)), _p(flat)) != 0:
+                raise ValueError(f"color_convert {src_fmt} -> {dst_fmt} is not a conversion the reference runs")
+            return split_planes(dst_fmt, w, h, flat)
+        return out
+
+    def color_luma(self, src_fmt, w, h, planes):
+        """the Y plane of a color_convert whose YUV output only channel_extract(Y) reads: ColorConvert_Y_RGB
+        (oracle_color_y_rgb) or _Y_RGBX"""
+        y = np.zeros((h, w), np.uint8)
+        src = np.ascontiguousarray(planes[0], dtype=np.uint8)
+        if src_fmt == "RGB":
+            self.lib.oracle_color_y_rgb(w, h, _p(y), w, _p(src), src.shape[1])
+        else:
+            self.lib.oracle_cc_y_rgbx(w, h, _p(src), src.shape[1], _p(y), w)
+        return y
 
     def halfscale(self, src, ksize):
         """halfscale_gaussian (ago_drama_divide.cpp:1471-1488): kernel size 1 -> ScaleImage nearest,
@@ -577,6 +652,18 @@ class Reference:
         v = np.zeros((h // 2, w // 2), np.uint8)
         self._ok(self.lib.ref_color_iyuv_from_rgb(w, h, _p(_c(rgb)), _p(y), _p(u), _p(v)), "color_iyuv_from_rgb")
         return y, u, v
+
+    def color_luma(self, src_fmt, yuv_fmt, w, h, planes):
+        y = np.zeros((h, w), np.uint8)
+        self._ok(self.lib.ref_color_luma(COLOR_FORMATS[src_fmt], COLOR_FORMATS[yuv_fmt], w, h, _p(join_planes(planes)), _p(y)),
+                 f"color_luma {src_fmt} -> {yuv_fmt}")
+        return y
+
+    def color_convert(self, src_fmt, dst_fmt, w, h, planes):
+        out = np.zeros(sum(r * b for r, b in color_planes(dst_fmt, w, h)), np.uint8)
+        self._ok(self.lib.ref_color_convert(COLOR_FORMATS[src_fmt], COLOR_FORMATS[dst_fmt], w, h, _p(join_planes(planes)), _p(out)),
+                 f"color_convert {src_fmt} -> {dst_fmt}")
+        return split_planes(dst_fmt, w, h, out)
 
     def halfscale(self, src, ksize):
         sh, sw = src.shape

@@ -300,6 +300,66 @@ int ref_color_rgb_yuv(int w, int h, const uint8_t *y, const uint8_t *u, const ui
     return 0;
 }
 
+// The planes of an image format as tightly packed host buffers: (width, height, bytes per element,
+// addressing step) per plane; chroma planes of 4:2:0 formats are w/2 x h/2 samples.
+struct PlaneDesc { int w, h, bpp, step; };
+
+int planes_of(vx_df_image f, int w, int h, PlaneDesc *p) {
+    switch (f) {
+    case VX_DF_IMAGE_RGB: p[0] = {w, h, 3, 1}; return 1;
+    case VX_DF_IMAGE_RGBX: p[0] = {w, h, 4, 1}; return 1;
+    case VX_DF_IMAGE_UYVY: case VX_DF_IMAGE_YUYV: p[0] = {w, h, 2, 1}; return 1;
+    case VX_DF_IMAGE_IYUV: p[0] = {w, h, 1, 1}; p[1] = {w / 2, h / 2, 1, 2}; p[2] = {w / 2, h / 2, 1, 2}; return 3;
+    case VX_DF_IMAGE_NV12: case VX_DF_IMAGE_NV21: p[0] = {w, h, 1, 1}; p[1] = {w / 2, h / 2, 2, 2}; return 2;
+    case VX_DF_IMAGE_YUV4: p[0] = {w, h, 1, 1}; p[1] = {w, h, 1, 1}; p[2] = {w, h, 1, 1}; return 3;
+    default: return 0;
+    }
+}
+
+vx_status copy_planes(vx_image img, vx_df_image f, int w, int h, uint8_t *buf, vx_enum usage) {
+    PlaneDesc pd[3];
+    const int np = planes_of(f, w, h, pd);
+    if (!np) return VX_ERROR_INVALID_FORMAT;
+    vx_rectangle_t r = {0, 0, (vx_uint32)w, (vx_uint32)h};
+    for (int k = 0; k < np; k++) {
+        vx_imagepatch_addressing_t a = {};
+        a.dim_x = w; a.dim_y = h; a.stride_x = pd[k].bpp; a.stride_y = pd[k].w * pd[k].bpp;
+        a.scale_x = VX_SCALE_UNITY; a.scale_y = VX_SCALE_UNITY; a.step_x = pd[k].step; a.step_y = pd[k].step;
+        vx_status s = vxCopyImagePatch(img, &r, k, &a, buf, usage, VX_MEMORY_TYPE_HOST);
+        if (s != VX_SUCCESS) return s;
+        buf += (size_t)pd[k].w * pd[k].h * pd[k].bpp;
+    }
+    return VX_SUCCESS;
+}
+
+// color_convert from any format to any other: `in` and `out` hold every plane of the image, tightly
+// packed, one plane after the other (4:2:0 chroma planes at half size, NV12 / NV21 UV interleaved).
+int ref_color_convert(uint32_t inFormat, uint32_t outFormat, int w, int h, const uint8_t *in, uint8_t *out) {
+    Ctx c;
+    vx_image src = vxCreateImage(c.ctx, w, h, (vx_df_image)inFormat);
+    vx_image dst = vxCreateImage(c.ctx, w, h, (vx_df_image)outFormat);
+    CHECK(copy_planes(src, (vx_df_image)inFormat, w, h, (uint8_t *)in, VX_WRITE_ONLY));
+    if (!vxColorConvertNode(c.graph, src, dst)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(copy_planes(dst, (vx_df_image)outFormat, w, h, out, VX_READ_ONLY));
+    return 0;
+}
+
+// color_convert into a virtual image of `yuvFormat` read only by channel_extract(Y): the reference keeps
+// only the luma kernel (ColorConvert_Y_RGB / _Y_RGBX), since nothing reads the chroma planes.
+int ref_color_luma(uint32_t inFormat, uint32_t yuvFormat, int w, int h, const uint8_t *in, uint8_t *y) {
+    Ctx c;
+    vx_image src = vxCreateImage(c.ctx, w, h, (vx_df_image)inFormat);
+    CHECK(copy_planes(src, (vx_df_image)inFormat, w, h, (uint8_t *)in, VX_WRITE_ONLY));
+    vx_image yuv = vxCreateVirtualImage(c.graph, w, h, (vx_df_image)yuvFormat);
+    vx_image out = vxCreateImage(c.ctx, w, h, VX_DF_IMAGE_U8);
+    if (!vxColorConvertNode(c.graph, src, yuv)) return -1;
+    if (!vxChannelExtractNode(c.graph, yuv, VX_CHANNEL_Y, out)) return -1;
+    int s = run(c); if (s) return s;
+    CHECK(get(out, w, h, y, 1));
+    return 0;
+}
+
 // RGB -> IYUV (non-virtual output, all three planes): ColorConvert_IYUV_RGB.
 int ref_color_iyuv_from_rgb(int w, int h, const uint8_t *rgb, uint8_t *y, uint8_t *u, uint8_t *v) {
     Ctx c;

@@ -40,8 +40,12 @@ def test_iyuv_from_rgb_node(gpu, orc):
             out = vx.create_image(ctx, want.shape[1], want.shape[0], "U8")
             assert L.vxuChannelExtract(ctx, iyuv, ch, out) == 0
             assert_same(vx.read_image(out), want, f"vxuColorConvert RGB->IYUV channel {ch:#x}")
+        # RGB -> NV12 through vxu: the luma plane and the interleaved chroma of ColorConvert_NV12_RGB
         nv12 = vx.create_image(ctx, w, h, "NV12")
-        assert L.vxuColorConvert(ctx, src, nv12) != 0   # RGB -> NV12 chroma is not produced: an error, not garbage
+        assert L.vxuColorConvert(ctx, src, nv12) == 0
+        want = orc.color_convert("RGB", "NV12", w, h, [rgb.reshape(h, 3 * w)])
+        for k, (got, exp) in enumerate(zip(vx.read_planes(nv12, "NV12", w, h), want)):
+            assert_same(got, exp, f"vxuColorConvert RGB->NV12 plane {k}")
     finally:
         cc = ctypes.c_void_p(ctx)
         L.vxReleaseContext(ctypes.byref(cc))
