@@ -25,6 +25,17 @@ MVX_DEV int32_t mag_body(int32_t gx, int32_t gy)
     return (int32_t)min(r, 32767u);
 }
 
+// Canny L2 magnitude of the reference's SSE loops (ago_haf_cpu_canny.cpp:646-1056): _mm_sqrt_ps of the
+// float sum, _mm_cvtps_epi32 (round half to even) and the unsigned 16-bit pack.  f >= 0 is an
+// integer-valued float below 2^32; sqrtf is the correctly rounded square root under this build's
+// -fhip-fp32-correctly-rounded-divide-sqrt, and rint the nearest-even conversion: the same operations
+// (checked for every such float by an exhaustive GPU probe).
+MVX_DEV int l2_pack_simd(float f)
+{
+    const int k = __float2int_rn(sqrtf(f));
+    return k > 65535 ? 65535 : k;
+}
+
 // Magnitude postfix rule (columns w & ~7 .. w-1), ago_haf_cpu_arithmetic.cpp:5087-5091:
 // (vx_int16)round(sqrtf((float)(gx*gx) + (float)(gy*gy))), float adds and sqrt correctly rounded.
 MVX_DEV int32_t mag_tail(int32_t gx, int32_t gy)

@@ -6,7 +6,7 @@
 // 8-connected through weak pixels to a strong one.  Here that set is computed with a lock-free
 // union-find over the edge pixels (atomicMin linking, then one find per pixel), three passes with no
 // host round trip, in the 4-byte-per-pixel workspace the reference reserves for its XY stack.
-#include "mvx_common.hpp"
+#include "arith.hpp"
 
 namespace mvx {
 
@@ -118,24 +118,6 @@ MVX_DEV uint32_t isqrt_u64(uint64_t F)
     return k;
 }
 
-// _mm_sqrt_ps then _mm_cvtps_epi32 (nearest even) of an integer-valued float f in [0, 2^31], computed
-// exactly: the nearest integer to sqrt(f), unless the float-rounded root lands on the half-integer
-// k + 1/2 itself (sqrt(f) within half a float ulp of it), which then rounds to the even neighbour.
-// The bounds (k + 1/2 -/+ ulp/2)^2 have at most 50 significant bits, so double products are exact.
-MVX_DEV int l2_pack_simd(float f)
-{
-    const uint64_t F = (uint64_t)f;
-    const uint32_t k = isqrt_u64(F);
-    uint32_t K = (F >= (uint64_t)k * k + k + 1) ? k + 1 : k;
-    if (k > 0) {
-        const double m = (double)k + 0.5;
-        const double u = ldexp(1.0, ilogb(m) - 23);
-        const double lo = m - 0.5 * u, hi = m + 0.5 * u;
-        if (lo * lo < (double)F && (double)F < hi * hi) K = (k & 1u) ? k + 1 : k;
-    }
-    return K > 65535u ? 65535 : (int)K;
-}
-
 // (vx_int16)sqrt((Gx*Gx) + (Gy*Gy)): 32-bit int sum, correctly rounded double sqrt truncated -- the
 // integer square root -- with 0x80000000 for a negative (wrapped) sum, then 16 bits
 MVX_DEV int l2_scalar(int gx, int gy)
@@ -149,26 +131,6 @@ template <int G> struct Taps;
 template <> struct Taps<3> { static constexpr int sm[3] = {1, 2, 1}, dv[3] = {-1, 0, 1}; };
 template <> struct Taps<5> { static constexpr int sm[5] = {1, 4, 6, 4, 1}, dv[5] = {-1, -2, 0, 2, 1}; };
 template <> struct Taps<7> { static constexpr int sm[7] = {1, 6, 15, 20, 15, 6, 1}, dv[7] = {-1, -4, -5, 0, 5, 4, 1}; };
-
-// the vertical pass of the SSE loop at column c (< aligned): r0 = smoothing rows, r1 = derivative
-// rows (gradient 7: both >> 2)
-template <int G> MVX_DEV void canny_vertical(const uint8_t *src, long stride, int c, int y, int &t0, int &t1)
-{
-    constexpr int R = G / 2;
-    int s[G];
-#pragma unroll
-    for (int k = 0; k < G; k++) s[k] = src[(long)(y - R + k) * stride + c];
-    if (G == 3) {
-        t0 = s[0] + s[2] + 2 * s[1];
-        t1 = s[2] - s[0];
-    } else if (G == 5) {
-        t0 = 4 * (s[1] + s[3]) + 6 * s[2] + (s[0] + s[4]);
-        t1 = 2 * (s[3] - s[1]) + (s[4] - s[0]);
-    } else {
-        t0 = (6 * (s[1] + s[5]) + 20 * s[3] + (s[0] + s[6]) + 15 * (s[2] + s[4])) >> 2;
-        t1 = (4 * (s[5] - s[1]) + (s[6] - s[0]) + 5 * (s[4] - s[2])) >> 2;
-    }
-}
 
 // the scalar loop's value at (x, y), linear-address reads
 template <int G, bool L2> MVX_DEV uint16_t canny_scalar(const uint8_t *src, long n, long stride, int x, int y)
@@ -196,9 +158,40 @@ template <int G, bool L2> MVX_DEV uint16_t canny_scalar(const uint8_t *src, long
 
 // The SSE passes over [0, aligned), aligned = w & ~15 >= 16: scratch rows r0, r1 (r1 = r0 + ALIGN16(w))
 // with zeros before r0, after r1 and in their unwritten tails; a read past the end of r0 lands in r1
-// (and one before r1 in r0) when w is a multiple of 16.  Row y of the block stages the scratch words
-// its 256 columns read (x-R .. x+255+R) in LDS, then each thread finishes one column; columns
-// aligned-1 .. w-1 take the scalar loop's value.
+// (and one before r1 in r0) when w is a multiple of 16.  A workgroup owns 256 columns of CSR_ROWS
+// output rows.  Each of its staged scratch slots x0-R .. x0+255+R stands for one source column (or
+// none) for every row, so the thread that owns a slot keeps that column's G source bytes in a rolling
+// register window: one new byte per row.  Per row the slots' two sums go to LDS, then each thread
+// finishes one column; columns aligned-1 .. w-1 take the scalar loop's value.
+constexpr int CSR_ROWS = 16;
+
+// which scratch value a slot holds: kind 1 = (r0, r1) of source column sc, 2 = r0 <- r1 of sc (the slot
+// lies past the end of r0), 3 = r1 <- r0 of sc (before the start of r1), 0 = zeros
+MVX_DEV int canny_slot(int col, int aligned, int a16, int &sc)
+{
+    if (col >= 0 && col < aligned) { sc = col; return 1; }
+    if (col >= a16 && col - a16 < aligned) { sc = col - a16; return 2; }
+    if (col < 0 && a16 + col < aligned) { sc = a16 + col; return 3; }
+    sc = 0;
+    return 0;
+}
+
+// the vertical pass of the SSE loop over one column's G source bytes: t0 = the smoothing sum (scratch
+// row r0), t1 = the derivative sum (r1); gradient 7 shifts both right by 2
+template <int G> MVX_DEV void canny_vsum(const int (&s)[G], int &t0, int &t1)
+{
+    if (G == 3) {
+        t0 = s[0] + s[2] + 2 * s[1];
+        t1 = s[2] - s[0];
+    } else if (G == 5) {
+        t0 = 4 * (s[1] + s[3]) + 6 * s[2] + (s[0] + s[4]);
+        t1 = 2 * (s[3] - s[1]) + (s[4] - s[0]);
+    } else {
+        t0 = (6 * (s[1] + s[5]) + 20 * s[3] + (s[0] + s[6]) + 15 * (s[2] + s[4])) >> 2;
+        t1 = (4 * (s[5] - s[1]) + (s[6] - s[0]) + 5 * (s[4] - s[2])) >> 2;
+    }
+}
+
 template <int G, bool L2>
 __global__ void __launch_bounds__(256)
 k_canny_sobel_g(uint32_t w, uint32_t h, uint16_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
@@ -206,69 +199,88 @@ k_canny_sobel_g(uint32_t w, uint32_t h, uint16_t *__restrict__ dst, uint32_t dst
 {
     constexpr int R = G / 2;
     __shared__ int16_t l0[256 + 2 * R], l1[256 + 2 * R];
-    const int x0 = (int)blockIdx.x * 256, y = R + (int)blockIdx.y;
+    const int x0 = (int)blockIdx.x * 256, yb = R + (int)blockIdx.y * CSR_ROWS;
+    const int ye = min(yb + CSR_ROWS, (int)h - R);
     const int aligned = (int)(w & ~15u), a16 = (int)((w + 15u) & ~15u);
-    for (int c = (int)threadIdx.x; c < 256 + 2 * R; c += 256) {
-        const int col = x0 - R + c;
-        int v0 = 0, v1 = 0, t0, t1;
-        if (col >= 0 && col < aligned) {               // r0[col], r1[col]: this column's two sums
-            canny_vertical<G>(src, srcStride, col, y, t0, t1);
-            v0 = t0;
-            v1 = t1;
-        } else if (col >= a16 && col - a16 < aligned) { // past the end of r0: the start of r1
-            canny_vertical<G>(src, srcStride, col - a16, y, t0, t1);
-            v0 = t1;
-        } else if (col < 0 && a16 + col < aligned) {    // before the start of r1: the end of r0
-            canny_vertical<G>(src, srcStride, a16 + col, y, t0, t1);
-            v1 = t0;
-        }
-        l0[c] = (int16_t)v0;
-        l1[c] = (int16_t)v1;
+    const int tid = (int)threadIdx.x;
+    // slot A = tid, slot B = 256 + tid (only the first 2R threads own one)
+    int scA, scB;
+    const int kA = canny_slot(x0 - R + tid, aligned, a16, scA);
+    const int kB = tid < 2 * R ? canny_slot(x0 - R + 256 + tid, aligned, a16, scB) : (scB = 0, 0);
+    int wa[G], wb[G];
+#pragma unroll
+    for (int k = 0; k < G - 1; k++) {
+        wa[k + 1] = kA ? (int)src[(long)(yb - R + k) * srcStride + scA] : 0;
+        wb[k + 1] = kB ? (int)src[(long)(yb - R + k) * srcStride + scB] : 0;
     }
-    __syncthreads();
-    const int x = x0 + (int)threadIdx.x;
-    if (x >= (int)w) return;
-    uint16_t out;
-    if (x < aligned - 1) {
-        const int c = (int)threadIdx.x + R;
+    const int x = x0 + tid;
+    // the row each iteration shifts in is loaded one iteration ahead, so its latency overlaps a row of work
+    int na = kA ? (int)src[(long)(yb + R) * srcStride + scA] : 0;
+    int nb = kB ? (int)src[(long)(yb + R) * srcStride + scB] : 0;
+    for (int y = yb; y < ye; y++) {
+#pragma unroll
+        for (int k = 0; k < G - 1; k++) { wa[k] = wa[k + 1]; wb[k] = wb[k + 1]; }
+        wa[G - 1] = na;
+        wb[G - 1] = nb;
+        if (y + 1 < ye) {
+            na = kA ? (int)src[(long)(y + 1 + R) * srcStride + scA] : 0;
+            nb = kB ? (int)src[(long)(y + 1 + R) * srcStride + scB] : 0;
+        }
+        int t0, t1;
+        canny_vsum<G>(wa, t0, t1);
+        l0[tid] = (int16_t)(kA == 1 || kA == 2 ? (kA == 1 ? t0 : t1) : 0);
+        l1[tid] = (int16_t)(kA == 1 ? t1 : kA == 3 ? t0 : 0);
+        if (tid < 2 * R) {
+            canny_vsum<G>(wb, t0, t1);
+            l0[256 + tid] = (int16_t)(kB == 1 || kB == 2 ? (kB == 1 ? t0 : t1) : 0);
+            l1[256 + tid] = (int16_t)(kB == 1 ? t1 : kB == 3 ? t0 : 0);
+        }
+        __syncthreads();
+        if (x < (int)w) {
+            uint16_t out;
+            if (x < aligned - 1) {
+                const int c = tid + R;
 #define A0(d) ((int)l0[c + (d)])
 #define A1(d) ((int)l1[c + (d)])
-        int gx, gy;
-        if (G == 3) {
-            gx = w16(A0(1) - A0(-1));
-            gy = w16(-w16(w16(A1(-1) + A1(1)) + sll16(A1(0), 1)));
-        } else if (G == 5) {
-            gx = s16(sll16(w16(A0(1) - A0(-1)), 1) + w16(A0(2) - A0(-2)));
-            int t = w16(sll16(w16(A1(-1) + A1(1)), 2) + w16(A1(-2) + A1(2)));
-            t = L2 ? s16(t + w16(A1(0) * 6)) : w16(t + w16(A1(0) * 6));
-            gy = w16(-t);
-        } else {
-            int t = sll16(s16(A0(2) - A0(-2)), 2);
-            const int u = w16(s16(A0(1) - A0(-1)) * 5);
-            t = s16(t + s16(A0(3) - A0(-3)));
-            gx = s16(t + u);
-            int v = s16(w16(w16(A1(-2) + A1(2)) * 6) + w16(A1(0) * 20));
-            const int q = w16(w16(A1(-1) + A1(1)) * 15);
-            v = s16(v + s16(A1(-3) + A1(3)));
-            v = s16(v + q);
-            gy = s16(-v);
-        }
+                int gx, gy;
+                if (G == 3) {
+                    gx = w16(A0(1) - A0(-1));
+                    gy = w16(-w16(w16(A1(-1) + A1(1)) + sll16(A1(0), 1)));
+                } else if (G == 5) {
+                    gx = s16(sll16(w16(A0(1) - A0(-1)), 1) + w16(A0(2) - A0(-2)));
+                    int t = w16(sll16(w16(A1(-1) + A1(1)), 2) + w16(A1(-2) + A1(2)));
+                    t = L2 ? s16(t + w16(A1(0) * 6)) : w16(t + w16(A1(0) * 6));
+                    gy = w16(-t);
+                } else {
+                    int t = sll16(s16(A0(2) - A0(-2)), 2);
+                    const int u = w16(s16(A0(1) - A0(-1)) * 5);
+                    t = s16(t + s16(A0(3) - A0(-3)));
+                    gx = s16(t + u);
+                    int v = s16(w16(w16(A1(-2) + A1(2)) * 6) + w16(A1(0) * 20));
+                    const int q = w16(w16(A1(-1) + A1(1)) * 15);
+                    v = s16(v + s16(A1(-3) + A1(3)));
+                    v = s16(v + q);
+                    gy = s16(-v);
+                }
 #undef A0
 #undef A1
-        int mag;
-        if (!L2) {
-            mag = w16(abs16(gx) + abs16(gy));
-            if (G == 7 && mag > 0x3fff) mag = 0x3fff;
-        } else if (G == 3) {
-            mag = l2_pack_simd((float)(uint16_t)w16(gx * gx) + (float)(uint16_t)w16(gy * gy));
-        } else {
-            mag = l2_pack_simd((float)(gx * gx + gy * gy));
+                int mag;
+                if (!L2) {
+                    mag = w16(abs16(gx) + abs16(gy));
+                    if (G == 7 && mag > 0x3fff) mag = 0x3fff;
+                } else if (G == 3) {
+                    mag = l2_pack_simd((float)(uint16_t)w16(gx * gx) + (float)(uint16_t)w16(gy * gy));
+                } else {
+                    mag = l2_pack_simd((float)(gx * gx + gy * gy));
+                }
+                out = canny_out(mag, canny_sector(gx, gy));
+            } else {
+                out = canny_scalar<G, L2>(src, (long)h * srcStride, srcStride, x, y);
+            }
+            *(uint16_t *)((uint8_t *)dst + (long)y * dstStride + 2L * x) = out;
         }
-        out = canny_out(mag, canny_sector(gx, gy));
-    } else {
-        out = canny_scalar<G, L2>(src, (long)h * srcStride, srcStride, x, y);
+        __syncthreads();
     }
-    *(uint16_t *)((uint8_t *)dst + (long)y * dstStride + 2L * x) = out;
 }
 
 // widths below 16: only the scalar loop runs, from x = -1 (the slot before each row), rows in order
@@ -399,8 +411,8 @@ int canny_sobel_g(mvx_stream_t stream, uint32_t w, uint32_t h, uint16_t *dst, ui
     if (w < 16)
         hipLaunchKernelGGL((k_canny_sobel_g_narrow<G, L2>), dim3(1), dim3(64), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride);
     else
-        hipLaunchKernelGGL((k_canny_sobel_g<G, L2>), dim3((w + 255) / 256, h - 2 * (G / 2)), dim3(256), 0, stream_of(stream), w, h, dst,
-                           dstStride, src, srcStride);
+        hipLaunchKernelGGL((k_canny_sobel_g<G, L2>), dim3((w + 255) / 256, (h - 2 * (G / 2) + CSR_ROWS - 1) / CSR_ROWS), dim3(256), 0,
+                           stream_of(stream), w, h, dst, dstStride, src, srcStride);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }

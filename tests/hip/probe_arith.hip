@@ -7,6 +7,8 @@
 //     cvtpd_epi32(sqrt_pd(.)) and packs (ago_haf_cpu_arithmetic.cpp:5031-5078) -- for all
 //     0 <= gy <= gx <= 32768 (the result depends on |gx|, |gy| only);
 //   * div_by_const_rn(a, b, RN(1/b)) vs IEEE single division for every finite float a, for a given b;
+//   * l2_pack_simd(f) vs the Canny L2 SSE rule, cvtps_epi32(sqrt_ps(f)) packed to 16 bits, for every
+//     integer-valued float below 2^32;
 //   * the lane mapping of the DPP wave_shl:1 control.
 #include "../../mivisionx_amd/csrc/arith.hpp"
 
@@ -70,6 +72,24 @@ __global__ void k_div_const(float b, float y, uint32_t *out)
     }
 }
 
+// l2_pack_simd(f) against _mm_cvtps_epi32(_mm_sqrt_ps(f)) + unsigned 16-bit pack restated in double
+// (the float square root is the double one rounded to float: double rounding is innocuous for sqrt),
+// for f = (float)i, every integer 0 <= i < 2^32 (every integer-valued float below 2^32)
+__global__ void k_l2pack(uint32_t *out)
+{
+    const uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 256u;
+    for (uint32_t k = 0; k < 256; k++) {
+        const uint32_t i = (uint32_t)(base + k);
+        const float f = (float)i;
+        const float r = (float)sqrt((double)f);
+        const double want = fmin(rint((double)r), 65535.0);
+        if (l2_pack_simd(f) != (int)want) {
+            atomicAdd(out, 1u);
+            atomicCAS(out + 1, 0u, i);
+        }
+    }
+}
+
 // lane i of wave_shl:1 (DPP control 0x130) reads lane i + 1; lane 63 reads nothing (0)
 __global__ void k_dpp(int32_t *out)
 {
@@ -89,6 +109,20 @@ extern "C" int probe_div_const(float b, uint32_t *result)
     int rc = -1;
     if (hipMemset(d, 0, 8) == hipSuccess) {
         hipLaunchKernelGGL(k_div_const, dim3(65536), dim3(256), 0, 0, b, recip_rn(b), d);   // 2^32 / 256 threads
+        if (hipGetLastError() == hipSuccess && hipMemcpy(result, d, 8, hipMemcpyDeviceToHost) == hipSuccess) rc = 0;
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+// result[0] = mismatches, result[1] = the first mismatching integer
+extern "C" int probe_l2_pack(uint32_t *result)
+{
+    uint32_t *d = nullptr;
+    if (hipMalloc(&d, 8) != hipSuccess) return -1;
+    int rc = -1;
+    if (hipMemset(d, 0, 8) == hipSuccess) {
+        hipLaunchKernelGGL(k_l2pack, dim3(65536), dim3(256), 0, 0, d);   // 2^32 / 256 threads
         if (hipGetLastError() == hipSuccess && hipMemcpy(result, d, 8, hipMemcpyDeviceToHost) == hipSuccess) rc = 0;
     }
     (void)hipFree(d);

@@ -53,3 +53,12 @@ def test_dpp_wave_shift_left_lane_map(probe):
     out = (ctypes.c_int32 * 64)()
     assert probe.probe_dpp_wave_shl1(out) == 0
     assert list(out)[:63] == [1001 + i for i in range(63)]
+
+
+def test_canny_l2_pack_all_integer_floats(probe):
+    """the Canny L2 magnitude's square-root-and-round rule (correctly rounded sqrt, nearest-even
+    conversion, 16-bit saturation) against the SSE rule restated in double, for every integer-valued
+    float below 2^32"""
+    out = (ctypes.c_uint32 * 2)()
+    assert probe.probe_l2_pack(out) == 0
+    assert out[0] == 0, f"{out[0]} values differ; first integer {out[1]}"
