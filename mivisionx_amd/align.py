@@ -238,35 +238,109 @@ def _normal(seed, n, device):
 
 
 class Aligner:
-    """Adam on p over the L2 loss summed over the pyramid levels (mean per level)."""
+    """Adam on p over the L2 loss summed over the pyramid levels (mean per level).
+
+    ``coarse_init`` picks each pair's starting point from a grid of rotations and translations scored
+    on the coarsest level (the loss surface of a 40 px shift has local minima that plain descent from
+    the identity can settle in); ``run`` replays one captured HIP graph per Adam step (the fused
+    loss-and-gradient launches, the chain rule to dL/dp and the optimiser update), so a step costs its
+    kernels rather than their host-side launches."""
 
     def __init__(self, src, tgt, levels=LEVELS):
         self.n, self.h, self.w = src.shape
+        dev = src.device
         self.ps = pyramid(src, levels)
         self.pt = pyramid(tgt, levels)
         self.levels = levels
         self.l2 = [L2Grad(a, b) for a, b in zip(self.ps, self.pt)]
-        self.loss = [torch.zeros(self.n, dtype=torch.float64, device=src.device) for _ in range(levels)]
-        self.gH = [torch.zeros((self.n, 9), dtype=torch.float64, device=src.device) for _ in range(levels)]
+        self.loss = [torch.zeros(self.n, dtype=torch.float64, device=dev) for _ in range(levels)]
+        self.gH = [torch.zeros((self.n, 9), dtype=torch.float64, device=dev) for _ in range(levels)]
+        # constant matrices on the device once, so a step makes no host-to-device copies
+        self.N = torch.as_tensor(norm_matrix(self.w, self.h), device=dev)
+        self.Ni = torch.as_tensor(np.linalg.inv(norm_matrix(self.w, self.h)), device=dev)
+        self.S = [torch.diag(torch.tensor([0.5 ** l, 0.5 ** l, 1.0], dtype=torch.float64, device=dev)) for l in range(levels)]
+        self.Si = [torch.diag(torch.tensor([2.0 ** l, 2.0 ** l, 1.0], dtype=torch.float64, device=dev)) for l in range(levels)]
+        self.sc = [torch.tensor([0.5 ** l, 0.5 ** l, 1.0], dtype=torch.float64, device=dev) for l in range(levels)]
+
+    def _H(self, p):
+        P = torch.zeros((p.shape[0], 9), dtype=torch.float64, device=p.device)
+        P[:, :8] = p
+        return self.Ni @ (torch.eye(3, dtype=torch.float64, device=p.device) + P.view(-1, 3, 3)) @ self.N
 
     def loss_and_grad(self, p, levels=None):
         """(loss per pair, dL/dp) at p over the given pyramid levels (default all) -- one fused
         launch + one reduction per level"""
-        Hpix = H_from_p(p, self.w, self.h)
+        Hpix = self._H(p)
         g = torch.zeros((self.n, 9), dtype=torch.float64, device=p.device)
         levels = range(self.levels) if levels is None else levels
         for l in levels:
             lv = self.ps[l]
-            self.l2[l](level_H(Hpix, l), 1.0 / (lv.shape[1] * lv.shape[2]), self.loss[l], self.gH[l], accumulate=False)
-            g += level_grad_to_base(self.gH[l], l)
-        return sum(self.loss[l] for l in levels), grad_p_from_grad_H(g, self.w, self.h)
+            Hl = (self.S[l] @ Hpix @ self.Si[l]).reshape(-1, 9).contiguous()
+            self.l2[l](Hl, 1.0 / (lv.shape[1] * lv.shape[2]), self.loss[l], self.gH[l], accumulate=False)
+            g += (self.sc[l][:, None] * self.gH[l].view(-1, 3, 3) / self.sc[l][None, :]).reshape(-1, 9)
+        G = self.Ni.T @ g.view(-1, 3, 3) @ self.N.T
+        return sum(self.loss[l] for l in levels), G.reshape(-1, 9)[:, :8]
 
-    def run(self, steps=200, lr=4e-3, p0=None):
-        p = torch.zeros((self.n, 8), dtype=torch.float64, device=self.ps[0].device) if p0 is None else p0.clone()
+    def coarse_init(self, rotations_deg=(-8.0, -4.0, 0.0, 4.0, 8.0), shift_px=48.0, step_px=8.0):
+        """per pair, the rotation / translation grid point (about the image centre, full-resolution
+        pixels) with the smallest loss on the coarsest pyramid level"""
+        dev = self.ps[0].device
+        top = self.levels - 1
+        s = self.w / 2.0
+        shifts = torch.arange(-shift_px, shift_px + 0.5 * step_px, step_px, dtype=torch.float64)
+        best = torch.full((self.n,), float("inf"), dtype=torch.float64, device=dev)
+        p0 = torch.zeros((self.n, 8), dtype=torch.float64, device=dev)
+        cand = torch.zeros((self.n, 8), dtype=torch.float64, device=dev)
+        for r in rotations_deg:
+            th = math.radians(r)
+            for tx in shifts.tolist():
+                for ty in shifts.tolist():
+                    cand[:] = torch.tensor([math.cos(th) - 1, -math.sin(th), tx / s, math.sin(th), math.cos(th) - 1, ty / s, 0.0, 0.0],
+                                           dtype=torch.float64)
+                    loss, _ = self.loss_and_grad(cand, levels=[top])
+                    better = loss < best
+                    best = torch.where(better, loss, best)
+                    p0 = torch.where(better[:, None], cand, p0)
+        return p0
+
+    def run(self, steps=200, lr=4e-3, p0=None, graph=True):
+        dev = self.ps[0].device
+        p = torch.zeros((self.n, 8), dtype=torch.float64, device=dev) if p0 is None else p0.clone()
         p.requires_grad_(True)
-        opt = torch.optim.Adam([p], lr=lr)
+        opt = torch.optim.Adam([p], lr=lr, capturable=True)   # the same update arithmetic eager or replayed
+        if not graph:
+            for _ in range(steps):
+                _, g = self.loss_and_grad(p.detach())
+                p.grad = g
+                opt.step()
+            return p.detach()
+        step = self.capture_step(p, opt)
         for _ in range(steps):
+            step.replay()
+        return p.detach()
+
+    def capture_step(self, p, opt, warmup=3):
+        """a CUDA graph of one Adam step on p (the optimiser must be capturable); the warm-up steps
+        run eagerly on a side stream first, as graph capture requires, and are undone"""
+        p0 = p.detach().clone()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                _, g = self.loss_and_grad(p.detach())
+                p.grad = g
+                opt.step()
+        torch.cuda.current_stream().wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
             _, g = self.loss_and_grad(p.detach())
             p.grad = g
             opt.step()
-        return p.detach()
+        # back to the starting point and a fresh optimiser state
+        with torch.no_grad():
+            p.copy_(p0)
+            for st in opt.state.values():
+                for k, v in st.items():
+                    if torch.is_tensor(v):
+                        v.zero_()
+        return graph
