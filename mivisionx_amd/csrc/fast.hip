@@ -227,21 +227,6 @@ k_fast_tile(FastArgs a, uint8_t *__restrict__ marks, uint32_t *__restrict__ rowC
     if (tid < FY && y0 + tid >= 3 && y0 + tid < (int)a.h - 3 && rowCnt[tid]) atomicAdd(&rowCount[y0 + tid - 3], rowCnt[tid]);
 }
 
-// rank of this thread's flag among the 256 of the block, and the block total
-MVX_DEV uint32_t block_rank(bool f, uint32_t &total)
-{
-    __shared__ uint32_t waveSum[4];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint64_t m = __ballot(f);
-    const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) waveSum[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t k = 0; k < wave; k++) base += waveSum[k];
-    total = waveSum[0] + waveSum[1] + waveSum[2] + waveSum[3];
-    __syncthreads();
-    return base + below;
-}
 
 __global__ void __launch_bounds__(256) k_fast_zero(uint32_t *__restrict__ p, uint32_t n)
 {
@@ -278,8 +263,29 @@ k_fast_scan(const uint32_t *__restrict__ rowCount, uint32_t *__restrict__ rowOff
     }
 }
 
+// exclusive prefix of v over the 256 threads of the block, and the block total (v small)
+MVX_DEV uint32_t block_scan(uint32_t v, uint32_t &total)
+{
+    __shared__ uint32_t waveSum[4];
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)inc, d);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) waveSum[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < wave; k++) before += waveSum[k];
+    total = waveSum[0] + waveSum[1] + waveSum[2] + waveSum[3];
+    __syncthreads();
+    return before + inc - v;
+}
+
 // vx_keypoint_t {x, y, strength, scale, orientation, tracking_status, error}; one workgroup per row
-// 3..h-4 reads the row's marks and writes its corners from the row's offset on
+// 3..h-4 reads the row's marks (16 consecutive pixels per thread) and writes its corners from the row's
+// offset on, in raster order
 template <bool SUPP>
 __global__ void __launch_bounds__(256)
 k_fast_write(FastArgs a, const uint8_t *__restrict__ marks, const uint32_t *__restrict__ rowOffset, uint8_t *__restrict__ dst)
@@ -287,20 +293,31 @@ k_fast_write(FastArgs a, const uint8_t *__restrict__ marks, const uint32_t *__re
     const int y = 3 + (int)blockIdx.x;
     uint32_t base = rowOffset[blockIdx.x];
     const uint8_t *row = marks + (long)y * a.w;
-    for (int x0 = 0; x0 < (int)a.w; x0 += 256) {
-        const int x = x0 + (int)threadIdx.x;
-        const uint32_t v = x < (int)a.w ? row[x] : 0u;
+    for (int x0 = 0; x0 < (int)a.w; x0 += 256 * 16) {
+        const int xs = x0 + (int)threadIdx.x * 16;
+        uint32_t v[16], cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            v[k] = xs + k < (int)a.w ? (uint32_t)row[xs + k] : 0u;
+            cnt += v[k] ? 1u : 0u;
+        }
         uint32_t total;
-        const uint32_t r = block_rank(v != 0, total);
-        if (v && base + r < a.capacity) {
-            uint32_t *kp = (uint32_t *)(dst + 28ull * (base + r));
-            kp[0] = (uint32_t)x;
-            kp[1] = (uint32_t)y;
-            kp[2] = __float_as_uint(SUPP ? (float)v : a.threshold);
-            kp[3] = 0u;
-            kp[4] = 0u;
-            kp[5] = 1u;
-            kp[6] = 0u;
+        uint32_t r = base + block_scan(cnt, total);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (v[k]) {
+                if (r < a.capacity) {
+                    uint32_t *kp = (uint32_t *)(dst + 28ull * r);
+                    kp[0] = (uint32_t)(xs + k);
+                    kp[1] = (uint32_t)y;
+                    kp[2] = __float_as_uint(SUPP ? (float)v[k] : a.threshold);
+                    kp[3] = 0u;
+                    kp[4] = 0u;
+                    kp[5] = 1u;
+                    kp[6] = 0u;
+                }
+                r++;
+            }
         }
         base += total;
     }

@@ -129,15 +129,18 @@ k_warp_perspective(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t
 // aligned source with a stride that is a multiple of 4 (then every dword is wholly inside or wholly
 // outside the plane).
 constexpr int WTW = 64, WTH = 16;            // output tile: 16 threads x 4 pixels per row, 16 rows
-constexpr int WLDS_DW = 3072;                // staged dwords per tile (12 KB)
+constexpr int WLDS_DW = 1536;                // staged dwords per tile (6 KB: a 64x16 tile of a 2x zoom-out fits)
 
-struct WarpBox { int r0, r1, cA, nd; bool on; };
+struct WarpBox { int r0, r1, cA, nd; bool on, inside; };
 
 // bounding box of the source coordinates xs in [xlo, xhi], ys in [ylo, yhi]: rows and columns that a
-// sample at trunc/floor of those coordinates and its +1 neighbours can touch, with two pixels of margin
-MVX_DEV WarpBox warp_box(float xlo, float xhi, float ylo, float yhi)
+// sample at trunc/floor of those coordinates and its +1 neighbours can touch, with two pixels of margin.
+// inside: the box lies within the source image (columns [0, sw-1], rows [0, sh-1]); every pixel's
+// samples are then inside the image, inside the box, and at offsets the reference never clamps, and
+// its coordinates are small positive floats -- the conditions of the unchecked per-pixel path.
+MVX_DEV WarpBox warp_box(float xlo, float xhi, float ylo, float yhi, uint32_t sw, uint32_t sh)
 {
-    WarpBox b{0, -1, 0, 0, false};
+    WarpBox b{0, -1, 0, 0, false, false};
     if (!(xlo >= -65536.0f && xhi <= 65536.0f && ylo >= -65536.0f && yhi <= 65536.0f)) return b;
     const int c0 = (int)floorf(xlo) - 2, c1 = (int)floorf(xhi) + 3;
     b.r0 = (int)floorf(ylo) - 2;
@@ -145,18 +148,20 @@ MVX_DEV WarpBox warp_box(float xlo, float xhi, float ylo, float yhi)
     b.cA = c0 & ~3;                                  // floor to a multiple of 4 (two's complement)
     b.nd = ((c1 - b.cA) >> 2) + 1;
     b.on = (int64_t)(b.r1 - b.r0 + 1) * b.nd <= WLDS_DW;    // rows and dwords are each below 2^18 here
+    b.inside = b.on && c0 >= 0 && b.r0 >= 0 && c1 <= (int)sw - 1 && b.r1 <= (int)sh - 1;
     return b;
 }
 
 // stage rows r0..r1, columns cA..cA+4nd-1 (linear-address coordinates) into lds, row pitch 4nd bytes
 MVX_DEV void warp_stage(uint32_t *lds, const WarpBox &b, const uint8_t *src, uint32_t stride, uint64_t n)
 {
-    const int total = (b.r1 - b.r0 + 1) * b.nd;
-    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
-    for (int i = tid; i < total; i += (int)(blockDim.x * blockDim.y)) {
-        const int rr = i / b.nd, k = i - rr * b.nd;
-        const int64_t lin = (int64_t)(b.r0 + rr) * stride + b.cA + 4 * k;
-        lds[i] = (lin >= 0 && (uint64_t)lin + 4 <= n) ? *reinterpret_cast<const uint32_t *>(src + lin) : 0u;
+    const int rows = b.r1 - b.r0 + 1;
+    for (int rr = (int)threadIdx.y; rr < rows; rr += (int)blockDim.y) {
+        const int64_t base = (int64_t)(b.r0 + rr) * stride + b.cA;
+        for (int k = (int)threadIdx.x; k < b.nd; k += (int)blockDim.x) {
+            const int64_t lin = base + 4 * k;
+            lds[rr * b.nd + k] = (lin >= 0 && (uint64_t)lin + 4 <= n) ? *reinterpret_cast<const uint32_t *>(src + lin) : 0u;
+        }
     }
 }
 
@@ -191,7 +196,7 @@ k_warp_affine_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t 
         const float sy[4] = {m.r10 * xa + (ya * m.r11 + m.c2), m.r10 * xb + (ya * m.r11 + m.c2), m.r10 * xa + (yb * m.r11 + m.c2),
                              m.r10 * xb + (yb * m.r11 + m.c2)};
         b = warp_box(fminf(fminf(sx[0], sx[1]), fminf(sx[2], sx[3])), fmaxf(fmaxf(sx[0], sx[1]), fmaxf(sx[2], sx[3])),
-                     fminf(fminf(sy[0], sy[1]), fminf(sy[2], sy[3])), fmaxf(fmaxf(sy[0], sy[1]), fmaxf(sy[2], sy[3])));
+                     fminf(fminf(sy[0], sy[1]), fminf(sy[2], sy[3])), fmaxf(fmaxf(sy[0], sy[1]), fmaxf(sy[2], sy[3])), sw, sh);
     }
     if (b.on) warp_stage(stage, b, src, srcStride, n);
     __syncthreads();
@@ -201,6 +206,25 @@ k_warp_affine_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t 
     const float xd = (float)y * m.r01 + m.c1, yd = (float)y * m.r11 + m.c2;
     const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
     uint32_t word = 0;
+    // (a tile holding padding pixels -- computed from column 0's coefficients -- takes the checked path)
+    if (b.inside && tx0 + WTW <= dw) {
+        // every sample in the image and the box: no border test, clamp, range check or fallback
+        const int pitch = 4 * b.nd;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool col = x0 + k < dw;
+            const float xs = (col ? m.r00 * (float)(x0 + k) : 0.0f) + xd, ys = (col ? m.r10 * (float)(x0 + k) : 0.0f) + yd;
+            const int32_t xi = (int32_t)xs, yi = (int32_t)ys;
+            const float fx = xs - (float)xi, fy = ys - (float)yi;
+            const int o = (yi - b.r0) * pitch + (xi - b.cA);
+            const float p0 = (float)lds[o], p1 = (float)lds[o + 1], p2 = (float)lds[o + pitch], p3 = (float)lds[o + pitch + 1];
+            const float omx = 1.0f - fx, omy = 1.0f - fy;
+            const float a = (p0 * omx) * omy, bb = (p1 * fx) * omy, c = (p2 * omx) * fy, d = (p3 * fx) * fy;
+            word |= sat8(__float2int_rn((a + bb) + (c + d))) << (8 * k);
+        }
+        *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const bool col = x0 + k < dw;
@@ -251,7 +275,7 @@ k_warp_perspective_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint
         const bool neg = sz[0] < 0.0f && sz[1] < 0.0f && sz[2] < 0.0f && sz[3] < 0.0f;
         if (pos || neg)
             b = warp_box(fminf(fminf(sx[0], sx[1]), fminf(sx[2], sx[3])), fmaxf(fmaxf(sx[0], sx[1]), fmaxf(sx[2], sx[3])),
-                         fminf(fminf(sy[0], sy[1]), fminf(sy[2], sy[3])), fmaxf(fmaxf(sy[0], sy[1]), fmaxf(sy[2], sy[3])));
+                         fminf(fminf(sy[0], sy[1]), fminf(sy[2], sy[3])), fmaxf(fmaxf(sy[0], sy[1]), fmaxf(sy[2], sy[3])), sw, sh);
     }
     if (b.on) warp_stage(stage, b, src, srcStride, n);
     __syncthreads();
@@ -262,6 +286,30 @@ k_warp_perspective_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint
     const float bf = (float)border;
     const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
     uint32_t word = 0;
+    // (a tile holding padding pixels -- computed from column 0's coefficients -- takes the checked path)
+    if (b.inside && tx0 + WTW <= dw) {
+        // every sample in the image and the box (coordinates positive: floor is truncation)
+        const int pitch = 4 * b.nd;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool col = x0 + k < dw;
+            const float xf = (float)(x0 + k);
+            const float z = 1.0f / ((col ? m.g * xf : 0.0f) + zd);
+            const float xs = ((col ? m.a * xf : 0.0f) + xd) * z, ys = ((col ? m.d * xf : 0.0f) + yd) * z;
+            const int32_t xi = (int32_t)xs, yi = (int32_t)ys;
+            const float fx = xs - (float)xi, fy = ys - (float)yi;
+            const int o = (yi - b.r0) * pitch + (xi - b.cA);
+            const float p0 = (float)lds[o], p1 = (float)lds[o + 1], p2 = (float)lds[o + pitch], p3 = (float)lds[o + pitch + 1];
+            const float omx = 1.0f - fx, omy = 1.0f - fy;
+            float v = (p0 * omx) * omy;
+            v = v + (p1 * fx) * omy;
+            v = v + (p2 * omx) * fy;
+            v = v + (p3 * fx) * fy;
+            word |= sat8((int32_t)v) << (8 * k);
+        }
+        *reinterpret_cast<uint32_t *>(dst + (uint64_t)y * dstStride + x0) = word;
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const bool col = x0 + k < dw;
