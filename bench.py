@@ -73,18 +73,26 @@ def stage_algorithmic_bytes(stage):
     }[stage]
 
 
-def measure_nodes(reps=10):
+def measure_nodes(reps=20):
     """GPU time of the node workloads beside the headline graph (HIP events on the launch stream)"""
     import ctypes
     import numpy as np
     import torch
     from mivisionx_amd import synth
     from mivisionx_amd.device import Plane, call
-    from mivisionx_amd.pipelines import PyrLKTracker
-    from pyoracle import Oracle
+    from mivisionx_amd.pipelines import PyrLKTracker, pyramid_dims
 
     def timed(fn):
-        for _ in range(2):
+        """mean device time of one call over `reps` back-to-back calls bracketed by one pair of HIP
+        events on the launch stream, after enough warm-up calls (about 2 ms of work) that the GPU
+        clocks have ramped up"""
+        t0 = time.perf_counter()
+        for _ in range(100):
+            fn()
+            torch.cuda.synchronize()
+            if time.perf_counter() - t0 > 2e-3:
+                break
+        for _ in range(5):
             fn()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
@@ -100,7 +108,7 @@ def measure_nodes(reps=10):
                 "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "bound": "hbm"}
 
     out = {}
-    # config 1: the 1920x1080 chain, one launch per node
+    # the 1920x1080 Gaussian -> Sobel -> Magnitude + Phase chain, one launch per node
     w1, h1 = 1920, 1080
     s = Plane.from_array(synth.uniform_u8(w1, h1, 0))
     g, gx, gy, m, p = Plane(w1, h1), Plane(w1, h1, "S16"), Plane(w1, h1, "S16"), Plane(w1, h1, "S16"), Plane(w1, h1)
@@ -111,8 +119,10 @@ def measure_nodes(reps=10):
         call("Magnitude_S16_S16S16", w1, h1, m, m.stride, gx, gx.stride, gy, gy.stride)
         call("Phase_U8_S16S16", w1, h1, p, p.stride, gx, gx.stride, gy, gy.stride)
     ms = timed(chain)
-    out["chain1_1920x1080"] = bw("chain1", ms, w1 * h1, 8, w1 * h1 / (ms / 1e3) / 1e6, "Mpix/s")
-    # config 3: warps and remap at 4K
+    out["chain1_1920x1080_per_node"] = bw("chain1", ms, w1 * h1, 8, w1 * h1 / (ms / 1e3) / 1e6, "Mpix/s")
+    out["chain1_1920x1080_per_node"]["how"] = "four launches: Gaussian, Sobel GXY, Magnitude, Phase (blurred plane stored)"
+    out["chain1_1920x1080"] = measure_chain1_graph(s, timed, bw)
+    # warps and remap at 3840x2160
     src = synth.scene_u8(W, H, 1)
     sp, dp = Plane.from_array(src), Plane(W, H)
     aff, per = synth.warp_matrices(W, H, 1)
@@ -126,7 +136,7 @@ def measure_nodes(reps=10):
     fmap = torch.from_numpy(synth.barrel_map(W, H, 0.15)).cuda()
     fixed = torch.zeros((H, W * 4), dtype=torch.uint8, device="cuda")
     call("Remap_FixedMap_F32", W, H, ctypes.c_void_p(fixed.data_ptr()), W * 4, ctypes.c_void_p(fmap.data_ptr()), W * 8, W, H)
-    ms = timed(lambda: call("Remap_U8_U8_Bilinear", W, H, dp, dp.stride, W, H, sp, sp.stride, ctypes.c_void_p(fixed.data_ptr()), W * 4))
+    ms = timed(lambda: call("Remap_U8_U8_Bilinear", W, H, dp, dp.stride, W, H, sp, sp.stride, sp.stride * H, ctypes.c_void_p(fixed.data_ptr()), W * 4))
     out["remap_3840x2160"] = bw("remap", ms, W * H, 6, W * H / (ms / 1e3) / 1e6, "Mpix/s")
     # the further kernel families at 4K
     ms = timed(lambda: call("WarpPerspective_U8_U8_Nearest_Constant", W, H, dp, dp.stride, W, H, sp, sp.stride,
@@ -150,17 +160,123 @@ def measure_nodes(reps=10):
     ms = timed(lambda: call("Remap_U24_U24_Bilinear", W, H, rgbp, rgbp.stride, W, H, rgbs, rgbs.stride, ctypes.c_void_p(fixed.data_ptr()),
                             W * 4))
     out["remap_rgb_3840x2160"] = bw("remap_rgb", ms, W * H, 10, W * H / (ms / 1e3) / 1e6, "Mpix/s")
-    # config 3: pyramidal LK, 10,000 keypoints
-    orc = Oracle()
+    # pyramidal LK at 3840x2160, 10,000 keypoints, on the two 4-level GaussianPyramid nodes of the graph
+    # (level 0 a channel copy, each further level ScaleGaussianHalf 5x5 of the one before)
     old, new, _ = synth.lk_pair(W, H, 1)
-    lo = [Plane.from_array(a) for a in orc.pyramid(old, 4)]
-    ln = [Plane.from_array(a) for a in orc.pyramid(new, 4)]
-    tr = PyrLKTracker(lo, ln, 10000)
+    dims = pyramid_dims(W, H, 4)
+    imgs = [Plane.from_array(a) for a in (old, new)]
+    pyrs = [[Plane(w, h) for (w, h) in dims] for _ in range(2)]
+
+    def pyramids():
+        for img, lv in zip(imgs, pyrs):
+            call("ChannelCopy_U8_U8", W, H, lv[0], lv[0].stride, img, img.stride)
+            for k in range(1, len(dims)):
+                call("ScaleGaussianHalf_U8_U8_5x5", dims[k][0], dims[k][1], lv[k], lv[k].stride, dims[k - 1][0], dims[k - 1][1],
+                     lv[k - 1], lv[k - 1].stride)
+    pyramids()
+    tr = PyrLKTracker(pyrs[0], pyrs[1], 10000)
     tr.set_points(synth.lk_keypoints(W, H, 10000, 1))
-    ms = timed(lambda: tr.run(0xF002, 0.01, 20, False, 9))
-    out["optical_flow_pyr_lk_4levels_10000pts"] = {"ms": round(ms, 4), "value": round(10000 / (ms / 1e3), 1), "unit": "keypoints/s",
-                                                   "bound": "latency (one 64-lane workgroup per keypoint, sequential Newton steps)"}
+    ms_lk = timed(lambda: tr.run(0xF002, 0.01, 20, False, 9))
+    ms_pyr = timed(pyramids)
+
+    def graph():
+        pyramids()
+        tr.run(0xF002, 0.01, 20, False, 9)
+    ms = timed(graph)
+    out["optical_flow_pyr_lk_4levels_10000pts"] = {
+        "ms": round(ms, 4), "value": round(10000 / (ms / 1e3), 1), "unit": "keypoints/s",
+        "includes": "both 4-level GaussianPyramid nodes (3840x2160 copy + 3 ScaleGaussianHalf 5x5 levels each) and the LK node",
+        "lk_only_ms": round(ms_lk, 4), "pyramids_ms": round(ms_pyr, 4),
+        "bound": "latency (one 64-lane workgroup per keypoint, sequential Newton steps)"}
     return out
+
+
+def measure_chain1_graph(src, timed, bw, reps=200):
+    """The 1920x1080 Gaussian (virtual) -> Sobel (gx, gy stored) -> Magnitude + Phase graph through the
+    OpenVX engine: the chain supernode runs it as one launch.  Reported: the wall time of one
+    vxProcessGraph (it synchronises), and that launch's device time -- the same mvx_ChainBatch kernel,
+    plan and tile the engine chose, launched back to back and timed with HIP events on its stream."""
+    import ctypes as c
+    import torch
+    from mivisionx_amd import openvx as vx
+    from mivisionx_amd.device import Plane
+    from mivisionx_amd.pipelines import CHAIN_TILES, ChainBatch
+    w, h = src.width, src.height
+    L = vx.lib()
+    ctx = L.vxCreateContext()
+    messages = []
+    cb = vx.LOG_CB(lambda ctx_, ref, status, text: messages.append(text.decode()))
+    L.vxRegisterLogCallback(ctx, cb, 0)
+    addr = vx.Addressing(w, h, 1, src.stride, 1024, 1024, 1, 1, 0)
+    img = L.vxCreateImageFromHandle(ctx, vx.DF["U8"], c.byref(addr), (c.c_void_p * 1)(src.t.data_ptr()), vx.VX_MEMORY_TYPE_HIP)
+    g = L.vxCreateGraph(ctx)
+    blur = L.vxCreateVirtualImage(g, 0, 0, vx.DF["VIRT"])
+    o = [vx.create_image(ctx, w, h, f) for f in ("S16", "S16", "S16", "U8")]
+    assert L.vxGaussian3x3Node(g, img, blur) and L.vxSobel3x3Node(g, blur, o[0], o[1])
+    assert L.vxMagnitudeNode(g, o[0], o[1], o[2]) and L.vxPhaseNode(g, o[0], o[1], o[3])
+    vx.check(L.vxVerifyGraph(g), "vxVerifyGraph")
+    note = next((m.strip() for m in messages if "chain batch supernode" in m), "")
+    for _ in range(5):
+        vx.check(L.vxProcessGraph(g))
+    t = time.perf_counter()
+    for _ in range(reps):
+        vx.check(L.vxProcessGraph(g))
+    wall_ms = (time.perf_counter() - t) / reps * 1e3
+    tile = next((i for i, (tw, th) in enumerate(CHAIN_TILES) if f"tile {tw}x{th}" in note), 4)
+    cbatch = ChainBatch([src], with_half=False).prepare(tile)
+    k_ms = timed(cbatch.run)
+    rec = bw("chain1_graph", k_ms, w * h, 8, w * h / (k_ms / 1e3) / 1e6, "Mpix/s")
+    rec.update({"how": "vxProcessGraph: the chain supernode, one launch (device time of that launch, HIP events)",
+                "engine_log": note, "vxProcessGraph_wall_ms": round(wall_ms, 4),
+                "vxProcessGraph_wall_value": round(w * h / (wall_ms / 1e3) / 1e6, 1)})
+    gg, cc = c.c_void_p(g), c.c_void_p(ctx)
+    L.vxReleaseGraph(c.byref(gg))
+    L.vxReleaseContext(c.byref(cc))
+    return rec
+
+
+def measure_stitch(reps=10):
+    """The stitch variant of the uneven-batch configuration: 6 RGB cameras of 1920x1080 (scene seeds
+    4000+c) remapped through seeded F32 maps into a 7680x3840 equirectangular frame and merged with
+    Loom's camera-selection maps and 64-px linear blend weights.  One pass (each output pixel samples
+    only the cameras its group selects) against Loom's two passes (a U24 remap per camera into a
+    stacked frame, then the merge), each timed with HIP events on its launch stream."""
+    import torch
+    from mivisionx_amd import synth
+    from mivisionx_amd.pipelines import LoomStitch
+    st = LoomStitch(synth.stitch_rig())
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        t = sorted(a.elapsed_time(b) for a, b in ev)
+        return t[len(t) // 2]
+
+    fused_ms = timed(st.run_fused)
+    fused_out = st.out.t.clone()
+    two_ms = timed(st.run_two_pass)
+    same = bool(torch.equal(fused_out, st.out.t))
+    alg = st.algorithmic_bytes()
+    px = st.W * st.H
+    gbs = alg / (fused_ms / 1e3) / 1e9
+    return {
+        "workload": "6 x 1920x1080 RGB cameras -> 7680x3840 RGB equirectangular, bilinear 13.3 fixed-point tables from F32 maps, "
+                    "Loom camera-selection merge, 64-px linear overlaps",
+        "one_pass": {"ms": round(fused_ms, 4), "value": round(px / (fused_ms / 1e3) / 1e6, 1), "unit": "output Mpix/s",
+                     "kernel": "mvx_Stitch_Loom_RGB", "algorithmic_bytes": alg, "achieved_GBps": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "bound": "hbm"},
+        "two_pass": {"ms": round(two_ms, 4), "value": round(px / (two_ms / 1e3) / 1e6, 1), "unit": "output Mpix/s",
+                     "how": "6 x mvx_Remap_U24_U24_Bilinear_Constant into a stacked frame + mvx_StitchMerge_Loom_RGB (Loom's passes)"},
+        "one_pass_over_two_pass": round(two_ms / fused_ms, 2),
+        "outputs_identical": same,
+    }
 
 
 def measure_batched_uneven(reps=5):
@@ -245,7 +361,7 @@ def measure_batched_uneven(reps=5):
     cc = c.c_void_p(ctx)
     L.vxReleaseContext(c.byref(cc))
     return {
-        "workload": "256 U8 images, W uniform in {320..4096}, H in {240..2160} (multiples of 16), config-2 scene generator; "
+        "workload": "256 U8 images, W uniform in {320..4096}, H in {240..2160} (multiples of 16), seeded scene generator; "
                     "Gaussian3x3 -> Sobel3x3 -> Magnitude + Phase (blurred image virtual) + ScaleImage bilinear to half size",
         "images": len(sizes), "mpix": round(px / 1e6, 2),
         "batched": {"ms": round(ms_b, 3), "value": round(px / (ms_b / 1e3) / 1e6, 1), "unit": "Mpix/s",
@@ -263,17 +379,16 @@ def measure_batched_uneven(reps=5):
 
 
 def measure_alignment(pairs=64, steps=200):
-    """64 pairs of 1024x1024 F32 images: Adam on the 3-level pyramid L2 loss through the fused
-    warp + loss + adjoint kernel; time per step and the recovered homography error"""
+    """64 pairs of 1024x1024 F32 images: a coarse rotation/translation grid on the top pyramid level
+    picks each pair's start, then Adam on the 3-level pyramid L2 loss through the fused warp + loss +
+    adjoint kernel, one captured HIP graph per step; time per step and the recovered homography error"""
     import torch
     from mivisionx_amd import align
     src, tgt, Ht = align.make_pairs(pairs, 1024, 1024)
     al = align.Aligner(src, tgt)
-    p = torch.zeros((pairs, 8), dtype=torch.float64, device="cuda").requires_grad_(True)
-    opt = torch.optim.Adam([p], lr=4e-3)
     px_step = pairs * sum(a.shape[1] * a.shape[2] for a in al.ps)
     # fused kernel alone, HIP events on its stream (level 0)
-    Hl = align.level_H(align.H_from_p(p.detach(), 1024, 1024), 0)
+    Hl = align.level_H(align.H_from_p(torch.zeros((pairs, 8), dtype=torch.float64, device="cuda"), 1024, 1024), 0)
     lv = torch.zeros(pairs, dtype=torch.float64, device="cuda")
     gH = torch.zeros((pairs, 9), dtype=torch.float64, device="cuda")
     for _ in range(2):
@@ -285,19 +400,37 @@ def measure_alignment(pairs=64, steps=200):
     b.record()
     torch.cuda.synchronize()
     k_ms = a.elapsed_time(b) / 5
+    t = time.perf_counter()
+    p0 = al.coarse_init()
+    torch.cuda.synchronize()
+    init_ms = (time.perf_counter() - t) * 1e3
+    p = p0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([p], lr=4e-3, capturable=True)
+    graph = al.capture_step(p, opt)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(steps):
-        _, g = al.loss_and_grad(p.detach())
-        p.grad = g
-        opt.step()
+        graph.replay()
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t) / steps * 1e3
     err = align.corner_error(align.H_from_p(p.detach(), 1024, 1024), Ht, 1024, 1024)
+    # the same steps launched eagerly, for comparison
+    q = p0.clone().requires_grad_(True)
+    opt_e = torch.optim.Adam([q], lr=4e-3)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(20):
+        _, g = al.loss_and_grad(q.detach())
+        q.grad = g
+        opt_e.step()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t) / 20 * 1e3
     alg = pairs * 1024 * 1024 * 8
     return {
-        "workload": f"{pairs} pairs of 1024x1024 F32, 3-level pyramid L2, {steps} Adam steps",
+        "workload": f"{pairs} pairs of 1024x1024 F32, 3-level pyramid L2, coarse grid start, {steps} Adam steps",
         "ms_per_step": round(step_ms, 4), "value": round(px_step / (step_ms / 1e3) / 1e6, 1), "unit": "Mpix/s (pyramid pixels)",
+        "how": "one captured HIP graph per Adam step (3 fused warp+L2+adjoint launches, their reductions, chain rule, optimiser)",
+        "eager_ms_per_step": round(eager_ms, 4), "coarse_init_ms": round(init_ms, 1),
         "corner_error_px": {"mean": round(float(err.mean()), 4), "max": round(float(err.max()), 4),
                             "pairs_below_0.5px": int((err < 0.5).sum())},
         "kernel": {"name": "k_warp_f32_adjoint<L2> + k_reduce_partials, level 0", "ms": round(k_ms, 4), "algorithmic_bytes": alg,
@@ -368,7 +501,39 @@ def cpu_baseline(rgb, budget_s=10.0):
         spf, kind = (time.perf_counter() - t0) / iters, "port"
         sample = f"{iters} frames of the 3840x2160 graph through the oracle's scalar C port"
     return {"value": round(W * H / spf / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": kind, "sample": sample,
-            "ms_per_frame": round(spf * 1e3, 2)}
+            "ms_per_frame": round(spf * 1e3, 2), "host": host_cpu(), "all_cores": cpu_port_threads(rgb)}
+
+
+def host_cpu():
+    """the host's CPU model and the cores this process may use"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+
+
+def cpu_port_threads(rgb, max_threads=16):
+    """the oracle's C port of the graph on every usable host core: one frame per thread, threads
+    running concurrently (the port's C calls release the interpreter lock); frames per second of
+    the whole pool.  The reference engine itself is single-threaded, so this is the port, not it."""
+    from concurrent.futures import ThreadPoolExecutor
+    from pyoracle import Oracle
+    n = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    orc = Oracle()
+
+    def one(_):
+        orc.graph2(rgb, DW, DH, LEVELS, PARAMS["lower"], PARAMS["upper"], PARAMS["k"], PARAMS["thr"], PARAMS["dist"],
+                   PARAMS["capacity"])
+    with ThreadPoolExecutor(n) as pool:
+        list(pool.map(one, range(n)))          # first touch of every thread
+        t = time.perf_counter()
+        list(pool.map(one, range(2 * n)))
+        sec = time.perf_counter() - t
+    return {"value": round(2 * n * W * H / sec / 1e6, 3), "unit": "Mpix/s", "cores": n, "kind": "port",
+            "sample": f"{2 * n} frames of the 3840x2160 graph through the oracle's C port, {n} threads, one frame each at a time"}
 
 
 def traffic_from_profiles(stage, frames):
@@ -392,7 +557,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-image", action="store_true")
-    ap.add_argument("--no-nodes", action="store_true", help="skip the config-1 / config-3 node measurements")
+    ap.add_argument("--no-nodes", action="store_true", help="skip the per-node measurements (1080p chain, 4K warps, remap, LK, FAST, Canny)")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
     ap.add_argument("--no-align", action="store_true", help="skip the F32 alignment measurement")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -535,6 +700,8 @@ def main():
         out["nodes"] = measure_nodes()
     if not args.no_batch:
         out["batched_uneven"] = measure_batched_uneven()
+    if not args.no_batch:
+        out["stitch"] = measure_stitch()
     if not args.no_align:
         out["alignment"] = measure_alignment()
     if not args.no_cpu:

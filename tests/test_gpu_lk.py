@@ -1,7 +1,7 @@
 """GPU parity of OpticalFlowPyrLK: the HIP tracker through the C ABI (and through the vx API) versus
 the oracle, bit-exact on every keypoint field, including repeated calls on one workspace (the
 reference node's scratch state), lost points, initial estimates, EPSILON-only termination, the
-config-3 workload at 3840x2160 with 10,000 keypoints, and the reference engine's golden hash."""
+3840x2160 workload with 10,000 keypoints, and the reference engine's golden hash."""
 import ctypes
 import json
 import os
