@@ -198,7 +198,9 @@ k_canny_sobel_g(uint32_t w, uint32_t h, uint16_t *__restrict__ dst, uint32_t dst
                 uint32_t srcStride)
 {
     constexpr int R = G / 2;
-    __shared__ int16_t l0[256 + 2 * R], l1[256 + 2 * R];
+    // one dword per slot: every LDS access is dword aligned (16-bit slots let neighbouring reads merge
+    // into misaligned dword loads, which the LDS serves far more slowly)
+    __shared__ int32_t l0[256 + 2 * R], l1[256 + 2 * R];
     const int x0 = (int)blockIdx.x * 256, yb = R + (int)blockIdx.y * CSR_ROWS;
     const int ye = min(yb + CSR_ROWS, (int)h - R);
     const int aligned = (int)(w & ~15u), a16 = (int)((w + 15u) & ~15u);

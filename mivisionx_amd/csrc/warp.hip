@@ -165,16 +165,26 @@ MVX_DEV void warp_stage(uint32_t *lds, const WarpBox &b, const uint8_t *src, uin
     }
 }
 
+// the bytes at LDS offsets o and o+1 (o >= 0) in bits 0..15, from two dword-aligned reads and a byte
+// shift: byte pairs read as 16-bit loads at odd offsets are misaligned, which the LDS serves far more
+// slowly.  The stage arrays hold one dword past the box, so the second read stays inside them.
+MVX_DEV uint32_t lds_pair(const uint32_t *lds, int o)
+{
+    const uint32_t lo = lds[o >> 2], hi = lds[(o >> 2) + 1];
+    return __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)o & 3u);
+}
+
 // the four samples at (yi, xi), (yi, xi+1), (yi+1, xi), (yi+1, xi+1) when the box holds them
-MVX_DEV bool warp_lds4(const uint8_t *lds, const WarpBox &b, int32_t xi, int32_t yi, float &p0, float &p1, float &p2, float &p3)
+MVX_DEV bool warp_lds4(const uint32_t *lds, const WarpBox &b, int32_t xi, int32_t yi, float &p0, float &p1, float &p2, float &p3)
 {
     const int pitch = 4 * b.nd;
     if (!b.on || yi < b.r0 || yi >= b.r1 || xi < b.cA || xi >= b.cA + pitch - 1) return false;
     const int o = (yi - b.r0) * pitch + (xi - b.cA);
-    p0 = (float)lds[o];
-    p1 = (float)lds[o + 1];
-    p2 = (float)lds[o + pitch];
-    p3 = (float)lds[o + pitch + 1];
+    const uint32_t t = lds_pair(lds, o), u = lds_pair(lds, o + pitch);
+    p0 = (float)(t & 0xffu);
+    p1 = (float)((t >> 8) & 0xffu);
+    p2 = (float)(u & 0xffu);
+    p3 = (float)((u >> 8) & 0xffu);
     return true;
 }
 
@@ -183,7 +193,7 @@ __global__ void __launch_bounds__(256)
 k_warp_affine_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
                   const uint8_t *__restrict__ src, uint32_t srcStride, Affine m, int32_t limit, uint32_t border)
 {
-    __shared__ uint32_t stage[WLDS_DW];
+    __shared__ uint32_t stage[WLDS_DW + 1];
     const uint64_t n = (uint64_t)sh * srcStride;
     const float fw = (float)sw, fh = (float)sh;
     // tile corners (the last thread of a row stores pixels up to x0 + 63)
@@ -204,7 +214,7 @@ k_warp_affine_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t 
     const uint32_t y = ty0 + threadIdx.y;
     if (x0 >= dw || y >= dh) return;
     const float xd = (float)y * m.r01 + m.c1, yd = (float)y * m.r11 + m.c2;
-    const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
+    const uint32_t *lds = stage;
     uint32_t word = 0;
     // (a tile holding padding pixels -- computed from column 0's coefficients -- takes the checked path)
     if (b.inside && tx0 + WTW <= dw) {
@@ -217,7 +227,9 @@ k_warp_affine_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t 
             const int32_t xi = (int32_t)xs, yi = (int32_t)ys;
             const float fx = xs - (float)xi, fy = ys - (float)yi;
             const int o = (yi - b.r0) * pitch + (xi - b.cA);
-            const float p0 = (float)lds[o], p1 = (float)lds[o + 1], p2 = (float)lds[o + pitch], p3 = (float)lds[o + pitch + 1];
+            const uint32_t t = lds_pair(lds, o), u = lds_pair(lds, o + pitch);
+            const float p0 = (float)(t & 0xffu), p1 = (float)((t >> 8) & 0xffu), p2 = (float)(u & 0xffu),
+                        p3 = (float)((u >> 8) & 0xffu);
             const float omx = 1.0f - fx, omy = 1.0f - fy;
             const float a = (p0 * omx) * omy, bb = (p1 * fx) * omy, c = (p2 * omx) * fy, d = (p3 * fx) * fy;
             word |= sat8(__float2int_rn((a + bb) + (c + d))) << (8 * k);
@@ -256,7 +268,7 @@ __global__ void __launch_bounds__(256)
 k_warp_perspective_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t dstStride, uint32_t sw, uint32_t sh,
                        const uint8_t *__restrict__ src, uint32_t srcStride, Persp m, uint32_t border)
 {
-    __shared__ uint32_t stage[WLDS_DW];
+    __shared__ uint32_t stage[WLDS_DW + 1];
     const uint64_t n = (uint64_t)sh * srcStride;
     const int32_t limit = (int32_t)(sh * srcStride - 1u);
     const uint32_t tx0 = blockIdx.x * WTW, ty0 = blockIdx.y * WTH;
@@ -284,7 +296,7 @@ k_warp_perspective_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint
     if (x0 >= dw || y >= dh) return;
     const float xd = (float)y * m.b + m.c, yd = (float)y * m.e + m.f, zd = (float)y * m.h + m.i;
     const float bf = (float)border;
-    const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
+    const uint32_t *lds = stage;
     uint32_t word = 0;
     // (a tile holding padding pixels -- computed from column 0's coefficients -- takes the checked path)
     if (b.inside && tx0 + WTW <= dw) {
@@ -299,7 +311,9 @@ k_warp_perspective_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint
             const int32_t xi = (int32_t)xs, yi = (int32_t)ys;
             const float fx = xs - (float)xi, fy = ys - (float)yi;
             const int o = (yi - b.r0) * pitch + (xi - b.cA);
-            const float p0 = (float)lds[o], p1 = (float)lds[o + 1], p2 = (float)lds[o + pitch], p3 = (float)lds[o + pitch + 1];
+            const uint32_t t = lds_pair(lds, o), u = lds_pair(lds, o + pitch);
+            const float p0 = (float)(t & 0xffu), p1 = (float)((t >> 8) & 0xffu), p2 = (float)(u & 0xffu),
+                        p3 = (float)((u >> 8) & 0xffu);
             const float omx = 1.0f - fx, omy = 1.0f - fy;
             float v = (p0 * omx) * omy;
             v = v + (p1 * fx) * omy;
@@ -859,10 +873,19 @@ extern "C" int mvx_WarpPerspective_U8_U8_Bilinear_Constant(mvx_stream_t stream, 
     return warp_perspective(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, matrix, borderValue);
 }
 
+// The reference's srcImageBufferSize (bytes of the source buffer) guards its kernels' reads; here the
+// table only holds in-range points, so the size is checked once: it must cover the source plane.
+static bool src_buffer_ok(uint32_t srcWidth, uint32_t srcHeight, uint32_t srcStride, uint32_t srcImageBufferSize)
+{
+    if (!srcWidth || !srcHeight) return true;
+    return (uint64_t)(srcHeight - 1) * srcStride + srcWidth <= srcImageBufferSize;
+}
+
 extern "C" int mvx_Remap_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
                                         uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
-                                        uint32_t srcStride, const void *pMap, uint32_t mapStride)
+                                        uint32_t srcStride, uint32_t srcImageBufferSize, const void *pMap, uint32_t mapStride)
 {
+    if (!src_buffer_ok(srcWidth, srcHeight, srcStride, srcImageBufferSize)) return MVX_ERROR_INVALID_PARAMETERS;
     return remap(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, false, 0);
 }
 
@@ -907,15 +930,18 @@ extern "C" int mvx_WarpPerspective_U8_U8_Nearest_Constant(mvx_stream_t stream, u
 
 extern "C" int mvx_Remap_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
                                        uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
-                                       uint32_t srcStride, const void *pMap, uint32_t mapStride)
+                                       uint32_t srcStride, uint32_t srcImageBufferSize, const void *pMap, uint32_t mapStride)
 {
+    if (!src_buffer_ok(srcWidth, srcHeight, srcStride, srcImageBufferSize)) return MVX_ERROR_INVALID_PARAMETERS;
     return remap_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, false, 0);
 }
 
 extern "C" int mvx_Remap_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
                                                 uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
-                                                uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue)
+                                                uint32_t srcStride, uint32_t srcImageBufferSize, const void *pMap,
+                                                uint32_t mapStride, uint8_t borderValue)
 {
+    if (!src_buffer_ok(srcWidth, srcHeight, srcStride, srcImageBufferSize)) return MVX_ERROR_INVALID_PARAMETERS;
     return remap_nn(stream, dstWidth, dstHeight, pDst, dstStride, srcWidth, srcHeight, pSrc, srcStride, pMap, mapStride, true,
                     borderValue);
 }
