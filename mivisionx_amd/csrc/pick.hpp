@@ -21,6 +21,7 @@ struct Keypoint { int32_t x, y; float strength, scale, orientation; int32_t trac
 
 constexpr int PICK_THREADS = 1024;
 constexpr uint32_t PICK_LDS_KEYS = 8192;          // keys sorted in LDS up to this many candidates
+constexpr uint32_t PICK_NB = 32;                  // closer higher-priority candidates listed per candidate in LDS
 constexpr uint8_t ST_UNDECIDED = 0, ST_IN = 1, ST_OUT = 2;
 constexpr uint32_t HASH_EMPTY = 0xffffffffu;
 
@@ -105,14 +106,38 @@ inline PickWork pick_work(void *workspace, uint32_t maxList)
     return W;
 }
 
+// A candidate's state before the rounds: the last cells % 4 grid cells keep (0,0) from the zeroed
+// scratch, so a candidate in one of them, or within min_distance of the origin with one of them in its
+// +-2 cell window, is never kept.
+MVX_DEV uint8_t pick_initial_state(uint32_t px, uint32_t py, uint32_t cell, int gw, long cells, long firstPhantom, int md2)
+{
+    const int cx = (int)(px / cell), cy = (int)(py / cell);
+    const long own = (long)cy * gw + cx;
+    if (own >= firstPhantom) return ST_OUT;
+    if ((int)(px * px + py * py) < md2) {
+        for (long c = firstPhantom; c < cells; c++) {
+            const int tx = (int)(c % gw), ty = (int)(c / gw);
+            if (abs(tx - cx) <= 2 && abs(ty - cy) <= 2) return ST_OUT;
+        }
+    }
+    return ST_UNDECIDED;
+}
+
 // The whole pick for one list, run by one workgroup of PICK_THREADS threads.  When resetCount is set,
 // *listCount is cleared at the end (the list producer appends with atomics).
+// Up to PICK_THREADS candidates (one per thread), the rounds run in LDS: each candidate first lists the
+// higher-priority candidates closer than min_distance by a scan of the sorted keys (every lane of a
+// wave reads the same key: an LDS broadcast), and a round then reads only those lists.  Longer lists
+// (or more candidates) take the hash-table path, which probes the pixels around each candidate.
 MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetCount, uint32_t maxList, float minDistance,
                                uint32_t width, uint32_t height, Keypoint *out, uint32_t capacity,
                                uint32_t *outNumItems, uint64_t *outCount, const PickWork &Wfull)
 {
     __shared__ int64_t lkeys[PICK_LDS_KEYS];
-    __shared__ uint32_t sflag, ssum[PICK_THREADS];
+    __shared__ uint32_t sflag, soverflow, ssum[PICK_THREADS];
+    __shared__ uint16_t lnb[PICK_NB * PICK_THREADS];     // [k][candidate]: a wave's appends hit distinct banks
+    __shared__ uint8_t lstate[PICK_THREADS];
+    if (threadIdx.x == 0) soverflow = 0;
     const uint32_t n = min(*listCount, maxList);
     // use only a table of about 4n slots (>= 1024) so that the probes stay cache resident; the slots
     // beyond it stay empty, and the ones used here are emptied again at the end
@@ -131,12 +156,60 @@ MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetC
     if (np > 1) bitonic_desc(keys, np);
 
     const bool useGrid = minDistance > 2.0f;
+    uint32_t cell = 1, md2 = 0;
+    int gw = 1;
+    long cells = 0, firstPhantom = 0;
     if (useGrid) {
-        const uint32_t cell = (uint32_t)floor((double)minDistance / 1.41421356237309504880);
-        const int gw = (int)((width + cell - 1) / cell), gh = (int)((height + cell - 1) / cell);
-        const long cells = (long)gw * gh;
-        const long firstPhantom = cells & ~3L;
-        const int md2 = (int)ceilf(minDistance * minDistance);
+        cell = (uint32_t)floor((double)minDistance / 1.41421356237309504880);
+        gw = (int)((width + cell - 1) / cell);
+        const int gh = (int)((height + cell - 1) / cell);
+        cells = (long)gw * gh;
+        firstPhantom = cells & ~3L;
+        md2 = (uint32_t)(int)ceilf(minDistance * minDistance);
+    }
+    bool lds = useGrid && inLds && n <= (uint32_t)PICK_THREADS;
+    uint32_t nbCount = 0;
+    if (lds) {
+        const uint32_t r = threadIdx.x;
+        if (r < n) {
+            const int64_t key = keys[r];
+            const int px = (int)(key & 0xffff), py = (int)((key >> 16) & 0xffff);
+            lstate[r] = pick_initial_state((uint32_t)px, (uint32_t)py, cell, gw, cells, firstPhantom, (int)md2);
+            for (uint32_t j = 0; j < r; j++) {
+                const int64_t kj = keys[j];
+                const int dx = (int)(kj & 0xffff) - px, dy = (int)((kj >> 16) & 0xffff) - py;
+                if ((uint32_t)(dx * dx + dy * dy) < md2) {
+                    if (nbCount < PICK_NB) lnb[nbCount * PICK_THREADS + r] = (uint16_t)j;
+                    nbCount++;
+                }
+            }
+            if (nbCount > PICK_NB) soverflow = 1;
+        }
+        __syncthreads();
+        lds = soverflow == 0;
+    }
+    if (lds) {
+        const uint32_t r = threadIdx.x;
+        while (true) {
+            if (threadIdx.x == 0) sflag = 0;
+            __syncthreads();
+            if (r < n && lstate[r] == ST_UNDECIDED) {
+                bool pending = false, rejected = false;
+                for (uint32_t k = 0; k < nbCount; k++) {
+                    const uint8_t sj = lstate[lnb[k * PICK_THREADS + r]];
+                    if (sj == ST_IN) { rejected = true; break; }
+                    if (sj == ST_UNDECIDED) pending = true;
+                }
+                if (rejected) lstate[r] = ST_OUT;
+                else if (!pending) lstate[r] = ST_IN;
+                sflag = 1;
+            }
+            __syncthreads();
+            const uint32_t again = sflag;
+            __syncthreads();
+            if (!again) break;
+        }
+    } else if (useGrid) {
         for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) {
             const int64_t key = keys[r];
             const uint32_t px = (uint32_t)(key & 0xffff), py = (uint32_t)((key >> 16) & 0xffff);
@@ -145,17 +218,7 @@ MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetC
             while (atomicCAS(&W.hkey[s], HASH_EMPTY, hk) != HASH_EMPTY) s = (s + 1) & W.hmask;
             W.hval[s] = r;
             W.slotOf[r] = s;
-            const int cx = (int)(px / cell), cy = (int)(py / cell);
-            const long own = (long)cy * gw + cx;
-            uint8_t st = ST_UNDECIDED;
-            if (own >= firstPhantom) st = ST_OUT;
-            else if ((int)(px * px + py * py) < md2) {
-                for (long c = firstPhantom; c < cells; c++) {
-                    const int tx = (int)(c % gw), ty = (int)(c / gw);
-                    if (abs(tx - cx) <= 2 && abs(ty - cy) <= 2) st = ST_OUT;
-                }
-            }
-            W.state[r] = st;
+            W.state[r] = pick_initial_state(px, py, cell, gw, cells, firstPhantom, (int)md2);
         }
         __syncthreads();
         const int rad = (int)ceilf(minDistance);
@@ -172,7 +235,7 @@ MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetC
                     if (yy < 0 || yy >= (int)height) continue;
                     for (int dx = -rad; dx <= rad; dx++) {
                         const int xx = px + dx;
-                        if ((dx | dy) == 0 || xx < 0 || xx >= (int)width || dx * dx + dy * dy >= md2) continue;
+                        if ((dx | dy) == 0 || xx < 0 || xx >= (int)width || dx * dx + dy * dy >= (int)md2) continue;
                         const uint32_t j = hash_find(W, (uint32_t)yy * width + (uint32_t)xx);
                         if (j == HASH_EMPTY || j >= r) continue;
                         const uint8_t sj = W.state[j];
@@ -193,11 +256,12 @@ MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetC
         for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) W.state[r] = ST_IN;
         __syncthreads();
     }
+    const uint8_t *state = lds ? lstate : W.state;
 
     const uint32_t per = (n + PICK_THREADS - 1) / PICK_THREADS;
     const uint32_t r0 = threadIdx.x * per, r1 = min(r0 + per, n);
     uint32_t mine = 0;
-    for (uint32_t r = r0; r < r1; r++) mine += (W.state[r] == ST_IN);
+    for (uint32_t r = r0; r < r1; r++) mine += (state[r] == ST_IN);
     ssum[threadIdx.x] = mine;
     __syncthreads();
     for (uint32_t off = 1; off < PICK_THREADS; off <<= 1) {
@@ -209,7 +273,7 @@ MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetC
     uint32_t pos = ssum[threadIdx.x] - mine;
     const uint32_t total = ssum[PICK_THREADS - 1];
     for (uint32_t r = r0; r < r1; r++) {
-        if (W.state[r] != ST_IN) continue;
+        if (state[r] != ST_IN) continue;
         if (pos < capacity) {
             const int64_t key = keys[r];
             Keypoint kp;
@@ -224,7 +288,7 @@ MVX_DEV void harris_pick_block(const XYS *list, uint32_t *listCount, bool resetC
         }
         pos++;
     }
-    if (useGrid)
+    if (useGrid && !lds)
         for (uint32_t r = threadIdx.x; r < n; r += PICK_THREADS) W.hkey[W.slotOf[r]] = HASH_EMPTY;
     __syncthreads();
     if (threadIdx.x == 0) {
