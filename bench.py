@@ -5,9 +5,11 @@ Workload (one "step" = one pass of the graph over a batch of B resident 4K RGB f
   Y -> ScaleImage 1920x1080 bilinear; Y -> GaussianPyramid 5 levels HALF;
   Y -> CannyEdgeDetector (3, L1, 80/100); Y -> HarrisCorners (k 0.04, T 0.0005, d 5, 3, 3).
 Inputs are synthetic scenes (sum of sinusoids + rectangles + noise, mivisionx_amd.synth), already in
-HBM when the timed region starts.  The graph runs through mvx_VisionGraph_run_stage (fused tile pass,
-hysteresis union, hysteresis resolve, pyramid levels 2-4, Harris pick; all frames per launch), pyramid
-and pick on a second stream beside the two hysteresis stages.
+HBM when the timed region starts.  The graph runs through mvx_VisionGraph_run_stage_range (fused tile
+pass, hysteresis union, hysteresis resolve, pyramid levels 2-4, Harris pick) in the schedule
+mvx_VisionGraph_run uses: the tile pass of all frames and then the hysteresis stages on the main
+stream, the pyramid levels and pick on a second stream beside the hysteresis (--chunk N: the frames in
+chunks of N, a chunk's later stages on side streams overlapping the next chunk's tile pass).
 value = whole-job graph throughput in Mpix/s of 4K input.
 
 Also reported:
@@ -545,7 +547,7 @@ def traffic_from_profiles(stage, frames):
     if stage != "fused_tile_pass" or not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_frame:\s*(\d+)", open(path).read())
-    return int(m.group(1)) * frames if m else None
+    return int(int(m.group(1)) * frames) if m else None
 
 
 def main():
@@ -554,6 +556,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=16, help="4K frames per step (resident, distinct buffers)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="frames per tile-pass launch (0: the library's default, all frames in one launch)")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-image", action="store_true")
@@ -595,13 +599,14 @@ def main():
     def stream():
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
-    def run_stage(st):
-        rc = K.mvx_VisionGraph_run_stage(stream(), ctypes.byref(G.params), len(G.frames), ctypes.c_void_p(G.ws.data_ptr()),
-                                         ctypes.c_size_t(G.ws_size), st)
+    def run_stage(st, f0=0, n=None):
+        n = len(G.frames) if n is None else n
+        rc = K.mvx_VisionGraph_run_stage_range(stream(), ctypes.byref(G.params), f0, n, ctypes.c_void_p(G.ws.data_ptr()),
+                                               ctypes.c_size_t(G.ws_size), st)
         if rc:
             raise RuntimeError(f"stage {st} returned {rc}")
 
-    # warm-up with every stage bracketed by events: find the dominant stage
+    # warm-up with every stage bracketed by events (all frames per launch): find the dominant stage
     ev = {st: [] for st in range(len(STAGES))}
     for _ in range(max(1, args.warmup)):
         for st in range(len(STAGES)):
@@ -612,48 +617,56 @@ def main():
     stage_ms = {st: sum(a.elapsed_time(b) for a, b in v) / len(v) for st, v in ev.items()}
     dominant = max(stage_ms, key=stage_ms.get)
 
-    # timed region: K steps of all frames; the dominant stage bracketed by events on its stream.
-    # Stage 0 first; stages 3 and 4 need only stage 0 and run on a second stream (default priority)
-    # beside stages 1 and 2, the fork/join mvx_VisionGraph_run does inside the library.
+    # timed region: K steps of all frames in mvx_VisionGraph_run's schedule (default-priority streams);
+    # every launch of the dominant stage bracketed by events on its stream
+    nf = len(G.frames)
+    chunk = min(args.chunk, nf) if args.chunk > 0 else nf
     dom_ev = []
-    main_s, side_s = torch.cuda.current_stream(), torch.cuda.Stream()
+    main_s, hyst_s, side_s = torch.cuda.current_stream(), torch.cuda.Stream(), torch.cuda.Stream()
 
-    def timed_stage(st):
+    def timed_stage(st, f0, n):
         if st == dominant:
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(); run_stage(st); b.record()
-            dom_ev.append((a, b))
+            a.record(); run_stage(st, f0, n); b.record()
+            dom_ev.append((a, b, n))
         else:
-            run_stage(st)
+            run_stage(st, f0, n)
 
-    def forked_step():
-        timed_stage(0)
-        side_s.wait_stream(main_s)
-        with torch.cuda.stream(side_s):
-            timed_stage(3)
-            timed_stage(4)
-        timed_stage(1)
-        timed_stage(2)
+    def pipelined_step():
+        for f0 in range(0, nf, chunk):
+            n = min(chunk, nf - f0)
+            timed_stage(0, f0, n)
+            hyst_s.wait_stream(main_s)
+            side_s.wait_stream(main_s)
+            with torch.cuda.stream(side_s):
+                timed_stage(3, f0, n)
+                timed_stage(4, f0, n)
+            # the last chunk's hysteresis stays on the main stream, as in the library
+            with torch.cuda.stream(hyst_s if f0 + n < nf else main_s):
+                timed_stage(1, f0, n)
+                timed_stage(2, f0, n)
+        main_s.wait_stream(hyst_s)
         main_s.wait_stream(side_s)
 
-    # untimed: the forked step itself, so the second stream's first use is not in the timed region
+    # untimed: the pipelined step itself, so the side streams' first use is not in the timed region
     for _ in range(max(1, args.warmup)):
-        forked_step()
+        pipelined_step()
     dom_ev.clear()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
     for _ in range(args.steps):
-        forked_step()
+        pipelined_step()
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = e0.elapsed_time(e1)
-    dom_ms = sum(a.elapsed_time(b) for a, b in dom_ev) / len(dom_ev)
-    frames = args.steps * len(G.frames)
+    dom_ms = sum(a.elapsed_time(b) for a, b, _ in dom_ev) / len(dom_ev)
+    dom_frames = sum(n for _, _, n in dom_ev) / len(dom_ev)
+    frames = args.steps * nf
     value = frames * W * H / (gpu_ms / 1e3) / 1e6
-    dom_bytes = stage_algorithmic_bytes(dominant) * len(G.frames)
+    dom_bytes = stage_algorithmic_bytes(dominant) * dom_frames
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     graph_bytes = graph_algorithmic_bytes()
     graph_gbs = graph_bytes * frames / (gpu_ms / 1e3) / 1e9
@@ -666,11 +679,12 @@ def main():
         "dtype": "u8", "data": "synthetic",
         "config": {"workload": "3840x2160 RGB vision graph: ColorConvert->Y; ScaleImage 1920x1080 bilinear; "
                                "GaussianPyramid 5 HALF; Canny 3 L1 80/100; Harris 0.04/0.0005/5/3/3",
-                   "frames_per_step": len(G.frames), "width": W, "height": H, "parallelism": "single-gpu"},
+                   "frames_per_step": nf, "frames_per_tile_pass_launch": chunk, "width": W, "height": H,
+                   "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "kernel": STAGES[dominant], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(dom_bytes), "mean_launch_ms": round(dom_ms, 5),
-                     "traffic": traffic_from_profiles(STAGES[dominant], len(G.frames))},
+                     "frames_per_launch": dom_frames, "traffic": traffic_from_profiles(STAGES[dominant], dom_frames)},
         "graph_hbm": {"algorithmic_bytes_per_frame": graph_bytes, "achieved": round(graph_gbs, 1), "unit": "GB/s",
                       "frac": round(graph_gbs / HBM_PEAK_GBS, 4)},
         "stage_ms_per_step": {STAGES[k]: round(v, 4) for k, v in stage_ms.items()},

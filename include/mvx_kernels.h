@@ -414,10 +414,12 @@ int mvx_WarpPerspective_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t ds
 /* replaces HipExec_Remap_U8_U8_Bilinear (hip_host_decls.h:813); HafCpu_Remap_U8_U8_Bilinear
  * (ago_haf_cpu_geometric.cpp:249-370).  pMap: device table of {u16 x, u16 y} in 13.3 fixed point,
  * (0xffff, 0xffff) for points outside the source (mvx_Remap_FixedMap_F32 builds it), mapStride
- * bytes per row (the reference uses dstWidth*4). */
+ * bytes per row (the reference uses dstWidth*4).  srcImageBufferSize: bytes of the source buffer, as
+ * the reference passes it; MVX_ERROR_INVALID_PARAMETERS when it does not cover
+ * (srcHeight - 1) * srcStride + srcWidth bytes. */
 int mvx_Remap_U8_U8_Bilinear(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
                              uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride,
-                             const void *pMap, uint32_t mapStride);
+                             uint32_t srcImageBufferSize, const void *pMap, uint32_t mapStride);
 /* replaces HipExec_Remap_U8_U8_Bilinear_Constant (hip_host_decls.h:819); HafCpu_Remap_U8_U8_Bilinear_Constant
  * (ago_haf_cpu_geometric.cpp:372-504) */
 int mvx_Remap_U8_U8_Bilinear_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
@@ -455,12 +457,13 @@ int mvx_WarpPerspective_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dst
  * dstWidth % 4 columns */
 int mvx_Remap_U8_U8_Nearest(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst, uint32_t dstStride,
                             uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc, uint32_t srcStride,
-                            const void *pMap, uint32_t mapStride);
+                            uint32_t srcImageBufferSize, const void *pMap, uint32_t mapStride);
 /* replaces HipExec_Remap_U8_U8_Nearest_Constant (hip_host_decls.h:807); HafCpu_Remap_U8_U8_Nearest_Constant
  * (ago_haf_cpu_geometric.cpp:156-247) */
 int mvx_Remap_U8_U8_Nearest_Constant(mvx_stream_t stream, uint32_t dstWidth, uint32_t dstHeight, uint8_t *pDst,
                                      uint32_t dstStride, uint32_t srcWidth, uint32_t srcHeight, const uint8_t *pSrc,
-                                     uint32_t srcStride, const void *pMap, uint32_t mapStride, uint8_t borderValue);
+                                     uint32_t srcStride, uint32_t srcImageBufferSize, const void *pMap, uint32_t mapStride,
+                                     uint8_t borderValue);
 /* RGB (packed U24) bilinear remap for the stitch path.  The reference registers REMAP_U24_U24_BILINEAR
  * with an empty execute (ago_kernel_api.cpp:18535-18539); here each channel equals
  * mvx_Remap_U8_U8_Bilinear[_Constant] of that channel over the same 13.3 table (valid entries only
@@ -569,27 +572,37 @@ size_t mvx_VisionGraph_workspace_size(const mvx_vision_params_t *params, uint32_
 int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_params_t *params, const mvx_vision_frame_t *frames,
                             uint32_t nframes, void *workspace, size_t workspaceSize);
 /* every batch: enqueues the graph for the prepared frames; asynchronous, capturable in a hipGraph.
- * Stages 3 and 4 run on a library-owned stream of the current device (default priority) beside
- * stages 1 and 2; the caller's stream waits for them before the call's work counts as done. */
+ * The fused tile pass and then the hysteresis stages run on the caller's stream, the pyramid levels and
+ * the Harris pick beside the hysteresis on a library-owned stream of the current device (default
+ * priority); the caller's stream waits for it before the call's work counts as done. */
 int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes, void *workspace,
                         size_t workspaceSize);
-/* A lane: a default-priority side stream and its fork / join events, on which mvx_VisionGraph_run_lane
- * runs the pyramid levels and the Harris pick beside the hysteresis stages.  mvx_VisionGraph_run uses one
+/* A lane: the two default-priority side streams and their fork / join events on which
+ * mvx_VisionGraph_run_lane runs the stages after each chunk's tile pass.  mvx_VisionGraph_run uses one
  * lane per device shared by all its callers; a caller that runs several graphs concurrently gives each
  * its own lane.  Create and destroy outside stream capture. */
 typedef struct mvx_vision_lane mvx_vision_lane_t;
 int mvx_VisionGraph_lane_create(mvx_vision_lane_t **lane);
 int mvx_VisionGraph_lane_destroy(mvx_vision_lane_t *lane);
-/* mvx_VisionGraph_run with the side stages on `lane` (NULL: every stage on `stream`) */
+/* mvx_VisionGraph_run with the side stages on `lane` (NULL: every stage on `stream`, all frames at once) */
 int mvx_VisionGraph_run_lane(mvx_stream_t stream, mvx_vision_lane_t *lane, const mvx_vision_params_t *params,
                              uint32_t nframes, void *workspace, size_t workspaceSize);
+/* the same in chunks of framesPerChunk frames (0: the default, all frames in one chunk): each chunk's
+ * tile pass on the caller's stream, then its later stages on the lane's streams while the caller's
+ * stream runs the next chunk's tile pass */
+int mvx_VisionGraph_run_pipelined(mvx_stream_t stream, mvx_vision_lane_t *lane, const mvx_vision_params_t *params,
+                                  uint32_t nframes, void *workspace, size_t workspaceSize, uint32_t framesPerChunk);
 /* the same, one stage at a time (0 fused tile pass, 1 hysteresis border union, 2 hysteresis resolve,
  * 3 pyramid levels 2.., 4 Harris sort and pick).  Stage 0 comes first and 2 after 1; stages 3 and 4
- * need only stage 0 and may run on another stream concurrently with 1 and 2.  The next batch's
- * stage 0 must follow every stage of the previous batch. */
+ * need only stage 0 and may run on another stream concurrently with 1 and 2.  Frames are independent:
+ * the stages of one frame may overlap another frame's.  The next batch's stage 0 of a frame must follow
+ * every stage of that frame in the previous batch. */
 #define MVX_VISION_STAGES 5
 int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t nframes,
                               void *workspace, size_t workspaceSize, int stage);
+/* one stage for the prepared frames firstFrame .. firstFrame + nframes - 1 */
+int mvx_VisionGraph_run_stage_range(mvx_stream_t stream, const mvx_vision_params_t *params, uint32_t firstFrame,
+                                    uint32_t nframes, void *workspace, size_t workspaceSize, int stage);
 
 
 /* ------------------------------------------------------------------ batched chain ---------- */

@@ -521,6 +521,23 @@ k_front(const FrameDesc *__restrict__ frames, GraphParams P)
     else front_tile<false>(candLds[wave], F, P, tx0, ty0, tile);
 }
 
+// The unions across one tile edge: lane i holds a = my id at border position i and b = the neighbour's
+// id at position i (-1: no edge pixel, or past the edge's length); a joins b(i-1), b(i), b(i+1).  A pair
+// that a lower lane also forms is left to it (same a in lane i-1 or i+1 forming the pair with its own b,
+// or b(i +- 1) = b(i)), so a run of equal ids across the edge costs one union instead of three per pixel;
+// every distinct pair is still joined, so the components -- and their minimum ids, the roots -- are
+// the same.  Called by the whole wave.
+MVX_DEV void border_unions(gptr_t<uint32_t> par, int32_t a, int32_t b, int lane)
+{
+    const int32_t aU = __shfl_up(a, 1), bU = __shfl_up(b, 1);
+    const int32_t aD = __shfl_down(a, 1), bD = __shfl_down(b, 1);
+    const bool hasU = lane >= 1, hasD = lane + 1 < 64;
+    if (a < 0) return;
+    if (b >= 0 && b != a && !(hasU && aU == a && bU == b)) gunion(par, (uint32_t)a, (uint32_t)b);
+    if (hasU && bU >= 0 && bU != a && bU != b && aU != a) gunion(par, (uint32_t)a, (uint32_t)bU);
+    if (hasD && bD >= 0 && bD != a && bD != b && aD != a) gunion(par, (uint32_t)a, (uint32_t)bD);
+}
+
 // union the border components of each tile with those of its right, lower and diagonal neighbours;
 // one wave per tile (lane = row of the right column, then column of the bottom row), four tiles per
 // workgroup
@@ -530,7 +547,7 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
     static_assert(TW <= 64 && TH <= 64, "one lane per border pixel");
     const FrameDesc &F = frames[blockIdx.z];
     const int lane = (int)threadIdx.x & 63;
-    const int tile = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     if (tile >= P.tilesX * P.tilesY) return;
     const int tX = tile % P.tilesX, tYi = tile / P.tilesX;
     const int tw = min(TW, P.w - tX * TW), th = min(TH, P.h - tYi * TH);
@@ -538,32 +555,14 @@ k_border(const FrameDesc *__restrict__ frames, GraphParams P)
     gptr_t<const int32_t> bt = tables + (long)tile * 2 * (TW + TH);
     gptr_t<uint32_t> par = gmem(F.parent);
     // right neighbour: my right column (rows 0..th-1) against its left column, |dr| <= 1
-    if (tX + 1 < P.tilesX && lane < th) {
+    if (tX + 1 < P.tilesX) {
         gptr_t<const int32_t> nb = tables + (long)(tile + 1) * 2 * (TW + TH);
-        const int32_t a = bt[2 * TW + TH + lane];
-        if (a >= 0) {
-            for (int d = -1; d <= 1; d++) {
-                const int rr = lane + d;
-                if (rr < 0 || rr >= th) continue;
-                const int32_t b = nb[2 * TW + rr];
-                if (b >= 0 && b != a) gunion(par, (uint32_t)a, (uint32_t)b);
-            }
-        }
+        border_unions(par, lane < th ? bt[2 * TW + TH + lane] : -1, lane < th ? nb[2 * TW + lane] : -1, lane);
     }
     // lower neighbour: my bottom row against its top row, |dc| <= 1 (the tile below has the same width)
     if (tYi + 1 < P.tilesY) {
         gptr_t<const int32_t> nb = tables + (long)(tile + P.tilesX) * 2 * (TW + TH);
-        if (lane < tw) {
-            const int32_t a = bt[TW + lane];
-            if (a >= 0) {
-                for (int d = -1; d <= 1; d++) {
-                    const int cc = lane + d;
-                    if (cc < 0 || cc >= tw) continue;
-                    const int32_t b = nb[cc];
-                    if (b >= 0 && b != a) gunion(par, (uint32_t)a, (uint32_t)b);
-                }
-            }
-        }
+        border_unions(par, lane < tw ? bt[TW + lane] : -1, lane < tw ? nb[lane] : -1, lane);
         if (lane == 0) {
             // diagonals: bottom-right corner <-> lower-right tile's top-left,
             // bottom-left corner <-> lower-left tile's top-right (a full-width tile)
@@ -759,16 +758,11 @@ extern "C" int mvx_VisionGraph_prepare(mvx_stream_t stream, const mvx_vision_par
     return MVX_SUCCESS;
 }
 
-extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_params_t *p, uint32_t nframes,
-                                         void *workspace, size_t workspaceSize, int stage)
+namespace mvx {
+
+// one stage for the frames D[0 .. nframes-1]
+static void launch_stage(hipStream_t s, const GraphParams &P, const FrameDesc *D, uint32_t nframes, int stage)
 {
-    if (!p || !params_ok(*p) || nframes == 0 || !workspace || stage < 0 || stage >= MVX_VISION_STAGES)
-        return MVX_ERROR_INVALID_PARAMETERS;
-    const GraphLayout L = graph_layout(*p, nframes);
-    if (workspaceSize < L.total) return MVX_ERROR_INVALID_PARAMETERS;
-    const GraphParams P = graph_params(*p, L);
-    const FrameDesc *D = (const FrameDesc *)workspace;
-    hipStream_t s = stream_of(stream);
     switch (stage) {
     case 0:
         hipLaunchKernelGGL(k_front, dim3((P.tilesX + FTW - 1) / FTW, P.tilesY, nframes), dim3(64 * FTW), 0, s, D, P);
@@ -791,18 +785,38 @@ extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_p
         hipLaunchKernelGGL(k_pick_batched, dim3(nframes), dim3(PICK_THREADS), 0, s, D, P);
         break;
     }
+}
+
+}  // namespace mvx
+
+extern "C" int mvx_VisionGraph_run_stage_range(mvx_stream_t stream, const mvx_vision_params_t *p, uint32_t firstFrame,
+                                               uint32_t nframes, void *workspace, size_t workspaceSize, int stage)
+{
+    if (!p || !params_ok(*p) || nframes == 0 || !workspace || stage < 0 || stage >= MVX_VISION_STAGES ||
+        firstFrame > UINT32_MAX - nframes)
+        return MVX_ERROR_INVALID_PARAMETERS;
+    const GraphLayout L = graph_layout(*p, firstFrame + nframes);
+    if (workspaceSize < L.total) return MVX_ERROR_INVALID_PARAMETERS;
+    launch_stage(stream_of(stream), graph_params(*p, L), (const FrameDesc *)workspace + firstFrame, nframes, stage);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }
 
-// A lane: the stream the pyramid levels 2.. and the Harris pick run on beside the hysteresis stages,
-// with its fork and join events, all at the default priority.  Each engine graph owns one
-// (mvx_VisionGraph_lane_create at verification, destroyed with the graph), so concurrent graphs never
-// share a side stream; mvx_VisionGraph_run uses one shared lane per device, created on first use and
-// kept until the process ends, whose mutex orders the callers that share it.
+extern "C" int mvx_VisionGraph_run_stage(mvx_stream_t stream, const mvx_vision_params_t *p, uint32_t nframes,
+                                         void *workspace, size_t workspaceSize, int stage)
+{
+    return mvx_VisionGraph_run_stage_range(stream, p, 0, nframes, workspace, workspaceSize, stage);
+}
+
+// A lane: the streams the per-frame stages after the fused tile pass run on -- one for the two
+// hysteresis stages, one for the pyramid levels 2.. and the Harris pick -- with the fork and join
+// events, all at the default priority.  Each engine graph owns one (mvx_VisionGraph_lane_create at
+// verification, destroyed with the graph), so concurrent graphs never share a side stream;
+// mvx_VisionGraph_run uses one shared lane per device, created on first use and kept until the
+// process ends, whose mutex orders the callers that share it.
 struct mvx_vision_lane {
-    hipStream_t stream = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t stream = nullptr, hyst = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, joinHyst = nullptr;
     std::mutex mu;
 };
 
@@ -811,18 +825,28 @@ namespace mvx {
 static int lane_init(mvx_vision_lane *L)
 {
     if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) return MVX_FAILURE;
+    if (hipStreamCreateWithFlags(&L->hyst, hipStreamNonBlocking) != hipSuccess) return MVX_FAILURE;
     if (hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) != hipSuccess) return MVX_FAILURE;
     if (hipEventCreateWithFlags(&L->join, hipEventDisableTiming) != hipSuccess) return MVX_FAILURE;
+    if (hipEventCreateWithFlags(&L->joinHyst, hipEventDisableTiming) != hipSuccess) return MVX_FAILURE;
     return MVX_SUCCESS;
 }
 
+// frames per chunk of the pipelined run: by default all frames in one chunk.  Smaller chunks overlap a
+// chunk's later stages with the next chunk's tile pass, but measured slower on MI355X at 4K (16 frames:
+// 0.834 ms per batch in one chunk, 0.893 in two, 1.065 in four): the tile pass is bound by VALU issue,
+// and the latency-bound union-find and pick waves resident beside it slow it by more than they hide.
+static uint32_t mvx_vision_default_chunk(uint32_t nframes) { return nframes; }
+
 static void lane_free(mvx_vision_lane *L)
 {
+    if (L->joinHyst) (void)hipEventDestroy(L->joinHyst);
     if (L->join) (void)hipEventDestroy(L->join);
     if (L->fork) (void)hipEventDestroy(L->fork);
+    if (L->hyst) (void)hipStreamDestroy(L->hyst);
     if (L->stream) (void)hipStreamDestroy(L->stream);
-    L->join = L->fork = nullptr;
-    L->stream = nullptr;
+    L->joinHyst = L->join = L->fork = nullptr;
+    L->hyst = L->stream = nullptr;
 }
 
 static std::mutex g_sharedMu;
@@ -860,38 +884,70 @@ extern "C" int mvx_VisionGraph_lane_destroy(mvx_vision_lane_t *lane)
 {
     if (!lane) return MVX_SUCCESS;
     (void)hipStreamSynchronize(lane->stream);
+    (void)hipStreamSynchronize(lane->hyst);
     lane_free(lane);
     delete lane;
     return MVX_SUCCESS;
 }
 
-// Stage 0 on the caller's stream; then stages 3 and 4 (which need only stage 0) on the lane's stream
-// while stages 1 and 2 run on the caller's stream; the caller's stream waits for the lane at the end,
-// so the call orders like a single-stream sequence.  Fork and join are event waits, which a hipGraph
-// capture of the caller's stream records as dependencies.  Once forked, the join is recorded and
-// waited for on every path, errors included, so a captured caller stream is never left forked.
+// The frames go through in chunks of framesPerChunk: the fused tile pass of each chunk on the caller's
+// stream, then -- once it has finished, while the caller's stream goes on with the next chunk's tile
+// pass -- that chunk's hysteresis stages on one lane stream (the last chunk's on the caller's stream)
+// and its pyramid levels and Harris pick on the other.  At the end the caller's stream waits for both
+// lane streams, so the call orders like a single-stream sequence.  Fork and join are event waits, which a hipGraph capture of the caller's
+// stream records as dependencies.  Once forked, the joins are recorded and waited for on every path,
+// errors included, so a captured caller stream is never left forked.
+extern "C" int mvx_VisionGraph_run_pipelined(mvx_stream_t stream, mvx_vision_lane_t *lane, const mvx_vision_params_t *p,
+                                             uint32_t nframes, void *workspace, size_t workspaceSize,
+                                             uint32_t framesPerChunk)
+{
+    if (!p || !params_ok(*p) || nframes == 0 || !workspace) return MVX_ERROR_INVALID_PARAMETERS;
+    const GraphLayout L = graph_layout(*p, nframes);
+    if (workspaceSize < L.total) return MVX_ERROR_INVALID_PARAMETERS;
+    const GraphParams P = graph_params(*p, L);
+    const FrameDesc *D = (const FrameDesc *)workspace;
+    hipStream_t main = stream_of(stream);
+    if (!lane) {
+        for (int st = 0; st < MVX_VISION_STAGES; st++) launch_stage(main, P, D, nframes, st);
+        MVX_LAUNCH_CHECK();
+        return MVX_SUCCESS;
+    }
+    const uint32_t chunk = framesPerChunk ? std::min(framesPerChunk, nframes) : mvx_vision_default_chunk(nframes);
+    std::lock_guard<std::mutex> lock(lane->mu);
+    int rc = MVX_SUCCESS;
+    bool forked = false;
+    for (uint32_t f0 = 0; f0 < nframes && !rc; f0 += chunk) {
+        const uint32_t n = std::min(chunk, nframes - f0);
+        launch_stage(main, P, D + f0, n, 0);
+        if (hipEventRecord(lane->fork, main) != hipSuccess || hipStreamWaitEvent(lane->hyst, lane->fork, 0) != hipSuccess ||
+            hipStreamWaitEvent(lane->stream, lane->fork, 0) != hipSuccess) {
+            rc = MVX_FAILURE;
+            break;
+        }
+        forked = true;
+        launch_stage(lane->stream, P, D + f0, n, 3);
+        launch_stage(lane->stream, P, D + f0, n, 4);
+        // the last chunk's hysteresis follows its tile pass on the caller's stream: no stream hop on
+        // the critical path
+        hipStream_t hs = f0 + n < nframes ? lane->hyst : main;
+        launch_stage(hs, P, D + f0, n, 1);
+        launch_stage(hs, P, D + f0, n, 2);
+        if (hipGetLastError() != hipSuccess) rc = MVX_ERROR_LAUNCH;
+    }
+    if (forked) {
+        const bool j1 = hipEventRecord(lane->joinHyst, lane->hyst) == hipSuccess &&
+                        hipStreamWaitEvent(main, lane->joinHyst, 0) == hipSuccess;
+        const bool j2 = hipEventRecord(lane->join, lane->stream) == hipSuccess &&
+                        hipStreamWaitEvent(main, lane->join, 0) == hipSuccess;
+        if (!rc && !(j1 && j2)) rc = MVX_FAILURE;
+    }
+    return rc;
+}
+
 extern "C" int mvx_VisionGraph_run_lane(mvx_stream_t stream, mvx_vision_lane_t *lane, const mvx_vision_params_t *p,
                                         uint32_t nframes, void *workspace, size_t workspaceSize)
 {
-    int rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 0);
-    if (rc) return rc;
-    if (!lane) {
-        for (int st = 1; st < MVX_VISION_STAGES && !rc; st++)
-            rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, st);
-        return rc;
-    }
-    std::lock_guard<std::mutex> lock(lane->mu);
-    hipStream_t main = stream_of(stream);
-    const mvx_stream_t side = reinterpret_cast<mvx_stream_t>(lane->stream);
-    if (hipEventRecord(lane->fork, main) != hipSuccess || hipStreamWaitEvent(lane->stream, lane->fork, 0) != hipSuccess)
-        return MVX_FAILURE;
-    rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 3);
-    if (!rc) rc = mvx_VisionGraph_run_stage(side, p, nframes, workspace, workspaceSize, 4);
-    const bool joined = hipEventRecord(lane->join, lane->stream) == hipSuccess;
-    if (!rc) rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 1);
-    if (!rc) rc = mvx_VisionGraph_run_stage(stream, p, nframes, workspace, workspaceSize, 2);
-    if (!joined || hipStreamWaitEvent(main, lane->join, 0) != hipSuccess) return rc ? rc : MVX_FAILURE;
-    return rc;
+    return mvx_VisionGraph_run_pipelined(stream, lane, p, nframes, workspace, workspaceSize, 0);
 }
 
 extern "C" int mvx_VisionGraph_run(mvx_stream_t stream, const mvx_vision_params_t *p, uint32_t nframes, void *workspace,

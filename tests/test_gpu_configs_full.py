@@ -75,8 +75,8 @@ def test_256_uneven_images_virtual_gradients(batch256, orc):
 
 
 def test_fused_4k_sixteen_frames_per_launch(gpu, orc):
-    """the bench's step shape: 16 resident 4K frames in one fused launch; frames 0, 7 and 15 against the
-    oracle (the frames are built as the bench builds them)"""
+    """the bench's step shape: 16 resident 4K frames through mvx_VisionGraph_run (one tile-pass launch for
+    all 16); frames 0, 7 and 15 against the oracle (the frames are built as the bench builds them)"""
     from mivisionx_amd.pipelines import FusedVisionGraph
     W, H = 3840, 2160
     params = dict(lower=80, upper=100, k=0.04, thr=0.0005, dist=5.0, capacity=65536)
@@ -88,7 +88,7 @@ def test_fused_4k_sixteen_frames_per_launch(gpu, orc):
         fr.rgb.upload(img)
         if f in (0, 7, 15):
             inputs[f] = img
-    print("fused 4K, 16 frames per launch", flush=True)
+    print("fused 4K, 16 frames per step", flush=True)
     G.run()
     torch.cuda.synchronize()
 

@@ -133,3 +133,44 @@ def test_fused_graph_matches_oracle(gpu, orc, w, h, frames):
             assert_same(got["edges"], want["edges"], "edges " + tag)
             assert got["count"] == want["count"], "corner count " + tag
             assert_same(got["corners"], want["corners"], "corners " + tag)
+
+
+@pytest.mark.parametrize("chunk,lane", [(1, True), (3, True), (0, True), (4, False)])
+def test_pipelined_chunks_match_oracle(gpu, orc, chunk, lane):
+    """mvx_VisionGraph_run_pipelined over 7 frames: chunks of 1 and 3 frames (a ragged last chunk), the
+    default, and no lane (every stage in order on one stream); every frame against the oracle, twice"""
+    import ctypes
+    import torch
+    from mivisionx_amd import kernels
+    from mivisionx_amd.pipelines import FusedVisionGraph
+    w, h, frames, levels = 100, 64, 7, 5
+    params = dict(lower=20, upper=60, k=0.04, thr=1e-6, dist=5.0, capacity=65536)
+    G = FusedVisionGraph(w, h, w // 2, h // 2, levels, frames=frames, **params)
+    inputs = [synth.scene_rgb(w, h, 60 + f) for f in range(frames)]
+    for f, rgb in zip(G.frames, inputs):
+        f.rgb.upload(rgb)
+    K = kernels()
+    ln = ctypes.c_void_p()
+    if lane:
+        assert K.mvx_VisionGraph_lane_create(ctypes.byref(ln)) == 0
+    try:
+        for rep in range(2):
+            print(f"pipelined chunk {chunk} lane {lane} rep {rep}", flush=True)
+            rc = K.mvx_VisionGraph_run_pipelined(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), ln,
+                                                 ctypes.byref(G.params), frames, ctypes.c_void_p(G.ws.data_ptr()),
+                                                 ctypes.c_size_t(G.ws_size), chunk)
+            assert rc == 0
+            for i, (f, rgb) in enumerate(zip(G.frames, inputs)):
+                got = f.results()
+                want = orc.graph2(rgb, w // 2, h // 2, levels, params["lower"], params["upper"], params["k"],
+                                  params["thr"], params["dist"], params["capacity"])
+                tag = f"chunk {chunk} frame {i} rep {rep}"
+                assert_same(got["edges"], want["edges"], "edges " + tag)
+                for l, (a, b) in enumerate(zip(got["pyramid"], want["pyramid"])):
+                    assert_same(a, b, f"pyramid {l} " + tag)
+                assert got["count"] == want["count"], "corner count " + tag
+                assert_same(got["corners"], want["corners"], "corners " + tag)
+    finally:
+        torch.cuda.synchronize()
+        if lane:
+            K.mvx_VisionGraph_lane_destroy(ln)
