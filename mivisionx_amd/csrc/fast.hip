@@ -98,6 +98,7 @@ MVX_DEV int strength(const Circle &k, int t)
 // one workgroup per row writes its keypoints in raster order from the byte plane.
 constexpr int FX = 64, FY = 16;                  // output tile
 constexpr int SW = FX + 8, SH = FY + 8;          // staged source: 4 columns / rows of margin each side
+static_assert(SW % 4 == 0, "staged rows are whole dwords");
 constexpr int MW = FX + 2, MH = FY + 2;          // strength map: the tile plus one pixel of border
 constexpr int NPOS = MW * MH;
 
@@ -129,16 +130,26 @@ template <bool SUPP>
 __global__ void __launch_bounds__(256)
 k_fast_tile(FastArgs a, uint8_t *__restrict__ marks, uint32_t *__restrict__ rowCount)
 {
-    __shared__ uint8_t stage[SH * SW];
+    __shared__ uint32_t stage32[SH * SW / 4];
     __shared__ uint8_t smap[MH * MW];
+    uint8_t *stage = reinterpret_cast<uint8_t *>(stage32);
     __shared__ uint16_t surv[NPOS], corn[NPOS];
     __shared__ uint32_t nsurv, ncorn, rowCnt[FY];
     const int tid = (int)threadIdx.x;
     const int x0 = (int)blockIdx.x * FX, y0 = (int)blockIdx.y * FY;
     const long n = (long)a.h * a.stride;
-    for (int i = tid; i < SH * SW; i += 256) {
-        const int r = i / SW, c = i - r * SW;
-        stage[i] = (uint8_t)ld_u8(a.src, n, (long)(y0 - 4 + r) * a.stride + (x0 - 4 + c));
+    if ((((uintptr_t)a.src | a.stride) & 3u) == 0) {
+        // dword loads: x0 - 4 is a multiple of 4, so every dword lies wholly inside or outside the plane
+        for (int i = tid; i < SH * SW / 4; i += 256) {
+            const int r = i / (SW / 4), c = i - r * (SW / 4);
+            const long lin = (long)(y0 - 4 + r) * a.stride + (x0 - 4 + 4 * c);
+            stage32[i] = (lin >= 0 && lin + 4 <= n) ? *reinterpret_cast<const uint32_t *>(a.src + lin) : 0u;
+        }
+    } else {
+        for (int i = tid; i < SH * SW; i += 256) {
+            const int r = i / SW, c = i - r * SW;
+            stage[i] = (uint8_t)ld_u8(a.src, n, (long)(y0 - 4 + r) * a.stride + (x0 - 4 + c));
+        }
     }
     for (int i = tid; i < MH * MW; i += 256) smap[i] = 0;
     if (tid == 0) { nsurv = 0; ncorn = 0; }

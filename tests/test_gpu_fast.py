@@ -96,3 +96,31 @@ def test_fast_node(gpu, orc, nonmax):
     finally:
         cc = ctypes.c_void_p(ctx)
         L.vxReleaseContext(ctypes.byref(cc))
+
+
+@pytest.mark.parametrize("nonmax", [0, 1])
+def test_fast_unaligned_source(gpu, orc, nonmax):
+    """a source at an odd address with an odd stride: the tile staging's byte path (aligned planes take
+    dword loads)"""
+    from mivisionx_amd import kernels
+    w, h, stride = 333, 77, 341
+    src = synth.scene_u8(w, h, 22)
+    buf = torch.zeros(h * stride + 8, dtype=torch.uint8)
+    view = buf[1:1 + h * stride].view(h, stride)
+    view[:, :w] = torch.from_numpy(src)
+    dbuf = buf.cuda()
+    corners = torch.zeros(5000 * 28, dtype=torch.uint8, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    num = torch.zeros(1, dtype=torch.int64, device="cuda")
+    K = kernels()
+    K.mvx_FastCorners_workspace_size.restype = ctypes.c_size_t
+    wsz = K.mvx_FastCorners_workspace_size(w, h)
+    ws = torch.zeros(int(wsz), dtype=torch.uint8, device="cuda")
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    call("FastCorners_XY_U8_Supression" if nonmax else "FastCorners_XY_U8_NoSupression", 5000, vp(corners), vp(cnt),
+         vp(num), w, h, ctypes.c_void_p(dbuf.data_ptr() + 1), stride, 20.0, vp(ws), ctypes.c_size_t(int(wsz)))
+    n = int(cnt.item())
+    kp = corners.cpu().numpy()[:n * 28].view(KEYPOINT)
+    want, wc = orc.fast(src, 20.0, nonmax, 5000)
+    assert int(num.item()) == wc and n == len(want)
+    assert np.array_equal(kp.view(np.uint8), want.view(np.uint8))
