@@ -947,15 +947,16 @@ vx_status run_remap(_vx_node *n, hipStream_t st, _vx_image *o, _vx_image *in, _v
 {
     vx_status s = r->sync(st);
     if (s) return fail(n, s, "remap table upload");
+    const uint32_t srcBytes = S(in) * in->height;
     if (nearest)
         return rc(n, constant ? mvx_Remap_U8_U8_Nearest_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in),
-                                                                 S(in), r->devFixed.ptr(), r->dstWidth * 4, border)
+                                                                 S(in), srcBytes, r->devFixed.ptr(), r->dstWidth * 4, border)
                               : mvx_Remap_U8_U8_Nearest(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
-                                                        r->devFixed.ptr(), r->dstWidth * 4));
+                                                        srcBytes, r->devFixed.ptr(), r->dstWidth * 4));
     return rc(n, constant ? mvx_Remap_U8_U8_Bilinear_Constant(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
                                                               r->devFixed.ptr(), r->dstWidth * 4, border)
                           : mvx_Remap_U8_U8_Bilinear(st, o->width, o->height, P(o), S(o), in->width, in->height, P(in), S(in),
-                                                     r->devFixed.ptr(), r->dstWidth * 4));
+                                                     srcBytes, r->devFixed.ptr(), r->dstWidth * 4));
 }
 
 vx_status x_remap(_vx_node *n, hipStream_t st)

@@ -46,3 +46,16 @@ def test_parameter_validation_needs_no_gpu(bad):
     else:
         rc = lib.mvx_Gaussian_U8_U8_3x3(None, 64, 8, q, 64, p, 64)
     assert rc == -10
+
+
+@pytest.mark.parametrize("name", ["mvx_Remap_U8_U8_Bilinear", "mvx_Remap_U8_U8_Nearest", "mvx_Remap_U8_U8_Nearest_Constant"])
+def test_remap_source_buffer_size_is_checked(name):
+    """the reference's srcImageBufferSize argument: a buffer smaller than the source plane is rejected on
+    the host before any launch"""
+    import mivisionx_amd
+    fn = getattr(mivisionx_amd.kernels(), name)
+    p = ctypes.c_void_p(16 * 1024)      # never dereferenced: validation fails first
+    args = [None, 64, 8, p, 64, 64, 8, p, 64, ctypes.c_uint32(64 * 7 + 63), p, 256]
+    if name.endswith("Constant"):
+        args.append(ctypes.c_uint8(0))
+    assert fn(*args) == -10

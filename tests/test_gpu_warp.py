@@ -128,10 +128,14 @@ def _remap(src_plane, mp, border, nearest=False):
          src_plane.width, src_plane.height)
     d = Plane(dw, dh)
     name = "Remap_U8_U8_Nearest" if nearest else "Remap_U8_U8_Bilinear"
+    size = src_plane.stride * src_plane.height
     if border is None:
-        call(name, dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride,
+        call(name, dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride, size,
              ctypes.c_void_p(fixed.data_ptr()), dw * 4)
-    else:
+    elif nearest:
+        call(name + "_Constant", dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride, size,
+             ctypes.c_void_p(fixed.data_ptr()), dw * 4, ctypes.c_uint8(border))
+    else:  # the reference's Bilinear_Constant form takes no buffer size
         call(name + "_Constant", dw, dh, d, d.stride, src_plane.width, src_plane.height, src_plane, src_plane.stride,
              ctypes.c_void_p(fixed.data_ptr()), dw * 4, ctypes.c_uint8(border))
     return d, fixed.cpu().numpy().view(np.uint16).reshape(dh, dw, 2)
