@@ -147,6 +147,18 @@ def measure_nodes(reps=20):
     u16 = Plane(W, H, "U16")
     ms = timed(lambda: call("CannySobel_U16_U8_5x5_L2NORM", W, H, u16, u16.stride, sp, sp.stride))
     out["canny_sobel_5x5_l2_3840x2160"] = bw("canny_sobel_5x5_l2", ms, W * H, 3, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    # the whole Canny node (gradient 5, L2, hysteresis 80/100): gradient plane, suppression + double
+    # threshold, edge trace; 1 B in + 1 B out per pixel
+    edges, cstack = Plane(W, H), torch.zeros(4 * W * H, dtype=torch.uint8, device="cuda")
+
+    def canny_node():
+        call("CannySobel_U16_U8_5x5_L2NORM", W, H, u16, u16.stride, sp, sp.stride)
+        call("CannySuppThreshold_U8XY_U16_3x3", W, H, edges, edges.stride, u16, u16.stride, ctypes.c_void_p(cstack.data_ptr()),
+             W * H, 80, 100)
+        call("CannyEdgeTrace_U8_U8XY", W, H, edges, edges.stride, ctypes.c_void_p(cstack.data_ptr()), W * H)
+    ms = timed(canny_node)
+    out["canny_5x5_l2_node_3840x2160"] = bw("canny_node", ms, W * H, 2, W * H / (ms / 1e3) / 1e6, "Mpix/s")
+    out["canny_5x5_l2_node_3840x2160"]["how"] = "CannySobel 5x5 L2 + CannySuppThreshold + CannyEdgeTrace (tiled), one node"
     K = __import__("mivisionx_amd").kernels()
     K.mvx_FastCorners_workspace_size.restype = ctypes.c_size_t
     fws = int(K.mvx_FastCorners_workspace_size(W, H))

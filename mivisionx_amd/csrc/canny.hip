@@ -7,6 +7,7 @@
 // union-find over the edge pixels (atomicMin linking, then one find per pixel), three passes with no
 // host round trip, in the 4-byte-per-pixel workspace the reference reserves for its XY stack.
 #include "arith.hpp"
+#include "rowmask.hpp"
 
 namespace mvx {
 
@@ -299,25 +300,52 @@ __global__ void k_canny_sobel_g_narrow(uint32_t w, uint32_t h, uint16_t *dst, ui
 }
 
 // CannySuppThreshold_U8XY_U16_3x3 (ago_haf_cpu_canny.cpp:549-596): x in [1, w-2], y in [1, h-2].
+// A workgroup owns 256 columns x 16 output rows: the magnitude rows around them are staged in LDS
+// with 16-byte loads (one dword per value, so no LDS load is ever misaligned), then each thread walks
+// its column down the rows with a 3x3 window in registers.
+constexpr int SUP_W = 256, SUP_H = 16, SUP_P = SUP_W + 16;   // LDS row: columns x0-8 .. x0+263
+
 __global__ void __launch_bounds__(256)
 k_canny_supp(uint32_t w, uint32_t h, uint8_t *__restrict__ dst, uint32_t dstStride, const uint16_t *__restrict__ mag,
              uint32_t magStride, uint32_t lower, uint32_t upper)
 {
-    const int x = 1 + (int)(blockIdx.x * 256 + threadIdx.x);
-    const int y = 1 + (int)blockIdx.y;
-    if (x >= (int)w - 1 || y >= (int)h - 1) return;
-    const long ms = magStride / 2;
-    const uint16_t *p = mag + (long)y * ms + x;
-    const uint32_t v = p[0];
-    const int m = (int)(v >> 2), a = (int)(v & 3u);
-    // neighbour pairs by sector (ago_haf_cpu_canny.cpp:26-31)
-    const int dx0 = (a == 0) ? -1 : (a == 1) ? 1 : (a == 2) ? 0 : -1;
-    const int dy0 = (a == 0) ? 0 : (a == 1) ? -1 : (a == 2) ? -1 : -1;
-    const int n0 = (int)(p[dy0 * ms + dx0] >> 2);
-    const int n1 = (int)(p[-dy0 * ms - dx0] >> 2);
-    const int edge = (m > n0 && m > n1) ? m : 0;
-    const uint8_t out = edge > (int)upper ? 255 : (edge <= (int)lower ? 0 : 127);
-    dst[(long)y * dstStride + x] = out;
+    __shared__ uint32_t t[(SUP_H + 2) * SUP_P];
+    const int tid = (int)threadIdx.x;
+    const int x0 = (int)blockIdx.x * SUP_W, y0 = 1 + (int)blockIdx.y * SUP_H;   // output rows y0 .. y0+15
+    constexpr int CH = SUP_P / 8;                                               // 16-byte chunks per row
+    for (int i = tid; i < (SUP_H + 2) * CH; i += 256) {
+        const int r = i / CH, c = i - r * CH;
+        const int y = y0 - 1 + r, col = x0 - 8 + 8 * c;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        // a chunk starting inside the row lies inside its ALIGN16 bytes, which the stride covers
+        if (y < (int)h && col >= 0 && col < (int)w)
+            v = *reinterpret_cast<const uint4 *>((const uint8_t *)mag + (long)y * magStride + 2L * col);
+        uint32_t *d = t + r * SUP_P + 8 * c;
+        d[0] = v.x & 0xffffu; d[1] = v.x >> 16; d[2] = v.y & 0xffffu; d[3] = v.y >> 16;
+        d[4] = v.z & 0xffffu; d[5] = v.z >> 16; d[6] = v.w & 0xffffu; d[7] = v.w >> 16;
+    }
+    __syncthreads();
+    const int x = x0 + tid, cx = tid + 8;
+    if (x < 1 || x > (int)w - 2) return;
+    const int rows = min(SUP_H, (int)h - 1 - y0);
+    // window: a = row y-1, b = row y, c = row y+1; index 0, 1, 2 = columns x-1, x, x+1
+    uint32_t a0 = t[cx - 1], a1 = t[cx], a2 = t[cx + 1];
+    uint32_t b0 = t[SUP_P + cx - 1], b1 = t[SUP_P + cx], b2 = t[SUP_P + cx + 1];
+    uint8_t *out = dst + (long)y0 * dstStride + x;
+    for (int r = 1; r <= rows; r++) {
+        const uint32_t *n = t + (r + 1) * SUP_P + cx;
+        const uint32_t c0 = n[-1], c1 = n[0], c2 = n[1];
+        const int m = (int)(b1 >> 2), s = (int)(b1 & 3u);
+        // neighbour pairs by sector (ago_haf_cpu_canny.cpp:26-31): 0 (-1,0)/(1,0), 1 (1,-1)/(-1,1),
+        // 2 (0,-1)/(0,1), 3 (-1,-1)/(1,1)
+        const uint32_t n0 = s == 0 ? b0 : s == 1 ? a2 : s == 2 ? a1 : a0;
+        const uint32_t n1 = s == 0 ? b2 : s == 1 ? c0 : s == 2 ? c1 : c2;
+        const int edge = (m > (int)(n0 >> 2) && m > (int)(n1 >> 2)) ? m : 0;
+        *out = edge > (int)upper ? 255 : (edge <= (int)lower ? 0 : 127);
+        out += dstStride;
+        a0 = b0; a1 = b1; a2 = b2;
+        b0 = c0; b1 = c1; b2 = c2;
+    }
 }
 
 // seed the union-find: every interior edge pixel (127 or 255) is its own root.  Labels always point
@@ -398,6 +426,206 @@ k_canny_final(uint32_t w, uint32_t h, uint8_t *__restrict__ img, uint32_t stride
     *p = (lab[r] & 0x80000000u) ? 255 : 0;
 }
 
+// ---------------------------------------------------------------- tiled edge trace ----------------
+// HafCpu_CannyEdgeTrace_U8_U8XY (ago_haf_cpu_canny.cpp:598-645): every 127 pixel 8-connected to a 255
+// pixel through 127 pixels becomes 255, every other 127 becomes 0.  Per 64x32 tile (one wave) the
+// edge (127 or 255) and strong (255) pixels of the interior become 64-bit row masks (lane = row); two
+// dilations run side by side in the wave, one from the strong pixels and one from the tile-border edge
+// pixels.  Pixels the strong dilation reaches are final 255; edge pixels reached by neither are final
+// 0; the rest (components that touch the tile border and hold no strong pixel) stay 127 and get a
+// union-find id, the k-th such component of tile t the id 1 + t*CT_NB + k (at most one per border
+// pixel).  Border tables record per border pixel 0 (the strong node), an id, or -1.  A second pass
+// unions the tables across tile edges, a third re-finds the undecided components (the 127 pixels left
+// in the tile, discovered in the same order) and writes 255 where the root is the strong node, else 0.
+constexpr int CT_W = 64, CT_H = 32, CT_NB = 2 * (CT_W + CT_H);
+
+struct CtArgs {
+    uint32_t w, h, stride, tilesX, tilesY, tiles;
+    uint8_t *img;
+    uint32_t *parent;   // 1 + tiles * CT_NB: slot 0 the strong node (its own root), then CT_NB per tile
+    int32_t *border;    // tiles * CT_NB: top row, bottom row (CT_W each), left, right column (CT_H each)
+    uint32_t *ucount;   // tiles: undecided components of the tile
+};
+
+struct CtTile { int tile, tx0, ty0, tw, th; };
+
+MVX_DEV bool ct_tile(const CtArgs &a, CtTile &t)
+{
+    t.tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    if (t.tile >= (int)a.tiles) return false;
+    t.tx0 = (t.tile % (int)a.tilesX) * CT_W;
+    t.ty0 = (t.tile / (int)a.tilesX) * CT_H;
+    t.tw = min(CT_W, (int)a.w - t.tx0);
+    t.th = min(CT_H, (int)a.h - t.ty0);
+    return true;
+}
+
+// the tile's interior pixels, lane = column, one register per row (0 outside the interior)
+MVX_DEV void ct_load(const CtArgs &a, const CtTile &t, int lane, uint32_t (&v)[CT_H])
+{
+    const int x = t.tx0 + lane;
+    const bool colOk = lane < t.tw && x >= 1 && x <= (int)a.w - 2;
+    gptr_t<const uint8_t> img = gmem((const uint8_t *)a.img);
+#pragma unroll
+    for (int r = 0; r < CT_H; r++) {
+        const int y = t.ty0 + r;
+        v[r] = (colOk && r < t.th && y >= 1 && y <= (int)a.h - 2) ? (uint32_t)img[(long)y * a.stride + x] : 0u;
+    }
+}
+
+// lane r < CT_H: the mask of row r of the pixels equal to `value` (or, with value2, to either)
+MVX_DEV uint64_t ct_rows(const uint32_t (&v)[CT_H], int lane, uint32_t value, uint32_t value2)
+{
+    uint64_t m = 0;
+#pragma unroll
+    for (int r = 0; r < CT_H; r++) {
+        const uint64_t b = __ballot(v[r] == value || v[r] == value2);
+        if (lane == r) m = b;
+    }
+    return m;
+}
+
+__global__ void __launch_bounds__(256) k_ct_tile(CtArgs a)
+{
+    const int lane = (int)threadIdx.x & 63;
+    CtTile t;
+    if (!ct_tile(a, t)) return;
+    uint32_t v[CT_H];
+    ct_load(a, t, lane, v);
+    const uint64_t em = ct_rows(v, lane, 127u, 255u), sm = ct_rows(v, lane, 255u, 255u);
+    const int rr = lane & 31;
+    const uint64_t e = __shfl(em, rr);
+    uint64_t seeds;
+    if (lane < 32) seeds = __shfl(sm, rr);
+    else seeds = (rr == 0 || rr == t.th - 1) ? e : (e & (1ull | (1ull << (t.tw - 1))));
+    const uint64_t grown = grow_rows(seeds, e, lane);
+    const uint64_t other = __shfl(grown, lane ^ 32);
+    const uint64_t rm = lane < 32 ? grown : other;                  // lane r: row r reached from a strong pixel
+    const uint64_t bmAll = lane < 32 ? (other & ~grown) : 0ull;     // lane r < 32: row r undecided
+    // settle the 127 pixels: reached -> 255, in no border-touching component -> 0, undecided stay
+    gptr_t<uint8_t> img = gmem(a.img);
+#pragma unroll
+    for (int r = 0; r < CT_H; r++) {
+        if (r < t.th) {
+            const uint64_t R = readlane64(rm, r), B = readlane64(bmAll, r);
+            const bool reached = (R >> lane) & 1ull, undecided = (B >> lane) & 1ull;
+            if (v[r] == 127u && (reached || !undecided))
+                img[(long)(t.ty0 + r) * a.stride + t.tx0 + lane] = reached ? 255 : 0;
+        }
+    }
+    int32_t topv = -1, botv = -1, leftv = -1, rightv = -1;
+    {
+        const uint64_t R0 = readlane64(rm, 0), Rb = readlane64(rm, t.th - 1);
+        if (lane < t.tw && ((R0 >> lane) & 1ull)) topv = 0;
+        if (lane < t.tw && ((Rb >> lane) & 1ull)) botv = 0;
+        if (lane < t.th && (rm & 1ull)) leftv = 0;
+        if (lane < t.th && ((rm >> (t.tw - 1)) & 1ull)) rightv = 0;
+    }
+    gptr_t<uint32_t> par = gmem(a.parent);
+    if (t.tile == 0 && lane == 0) par[0] = 0;
+    uint32_t k = 0;
+    uint64_t bm = bmAll;
+    while (true) {
+        const uint64_t rows = __ballot(bm != 0ull);
+        if (!rows) break;
+        const int r0 = __builtin_ctzll(rows);
+        const int c0 = __builtin_ctzll(readlane64(bm, r0));
+        const uint32_t gid = 1u + (uint32_t)t.tile * CT_NB + k;
+        const uint64_t c = grow_rows(lane == r0 ? (1ull << c0) : 0ull, bm, lane);
+        if (lane == 0) par[gid] = gid;
+        const uint64_t C0 = readlane64(c, 0), Cb = readlane64(c, t.th - 1);
+        if (lane < t.tw && ((C0 >> lane) & 1ull)) topv = (int32_t)gid;
+        if (lane < t.tw && ((Cb >> lane) & 1ull)) botv = (int32_t)gid;
+        if (lane < t.th && (c & 1ull)) leftv = (int32_t)gid;
+        if (lane < t.th && ((c >> (t.tw - 1)) & 1ull)) rightv = (int32_t)gid;
+        bm &= ~c;
+        k++;
+    }
+    gptr_t<int32_t> bt = gmem(a.border) + (long)t.tile * CT_NB;
+    bt[lane] = topv;
+    bt[CT_W + lane] = botv;
+    if (lane < CT_H) { bt[2 * CT_W + lane] = leftv; bt[2 * CT_W + CT_H + lane] = rightv; }
+    if (lane == 0) gmem(a.ucount)[t.tile] = k;
+}
+
+// the unions across the tile's right and bottom edges and its two lower corners
+__global__ void __launch_bounds__(256) k_ct_border(CtArgs a)
+{
+    const int lane = (int)threadIdx.x & 63;
+    CtTile t;
+    if (!ct_tile(a, t)) return;
+    const int tX = t.tile % (int)a.tilesX, tY = t.tile / (int)a.tilesX;
+    gptr_t<const int32_t> tables = gmem((const int32_t *)a.border);
+    gptr_t<const int32_t> bt = tables + (long)t.tile * CT_NB;
+    gptr_t<uint32_t> par = gmem(a.parent);
+    if (tX + 1 < (int)a.tilesX) {           // a right neighbour: this tile is full width
+        gptr_t<const int32_t> nb = tables + (long)(t.tile + 1) * CT_NB;
+        border_unions(par, lane < t.th ? bt[2 * CT_W + CT_H + lane] : -1, lane < t.th ? nb[2 * CT_W + lane] : -1, lane);
+    }
+    if (tY + 1 < (int)a.tilesY) {           // a lower neighbour: this tile is full height, same width
+        gptr_t<const int32_t> nb = tables + (long)(t.tile + a.tilesX) * CT_NB;
+        border_unions(par, lane < t.tw ? bt[CT_W + lane] : -1, lane < t.tw ? nb[lane] : -1, lane);
+        if (lane == 0) {
+            if (tX + 1 < (int)a.tilesX) {
+                const int32_t p = bt[CT_W + t.tw - 1], q = tables[(long)(t.tile + a.tilesX + 1) * CT_NB];
+                if (p >= 0 && q >= 0 && p != q) gunion(par, (uint32_t)p, (uint32_t)q);
+            }
+            if (tX > 0) {
+                const int32_t p = bt[CT_W], q = tables[(long)(t.tile + a.tilesX - 1) * CT_NB + CT_W - 1];
+                if (p >= 0 && q >= 0 && p != q) gunion(par, (uint32_t)p, (uint32_t)q);
+            }
+        }
+    }
+}
+
+// the undecided components of each tile, found again in the tile pass's order, take their root's state
+__global__ void __launch_bounds__(256) k_ct_resolve(CtArgs a)
+{
+    const int lane = (int)threadIdx.x & 63;
+    CtTile t;
+    if (!ct_tile(a, t)) return;
+    if (gmem((const uint32_t *)a.ucount)[t.tile] == 0) return;
+    uint32_t v[CT_H];
+    ct_load(a, t, lane, v);
+    uint64_t bm = ct_rows(v, lane, 127u, 127u);
+    if (lane >= 32) bm = 0;
+    gptr_t<uint8_t> img = gmem(a.img);
+    uint32_t k = 0;
+    while (true) {
+        const uint64_t rows = __ballot(bm != 0ull);
+        if (!rows) break;
+        const int r0 = __builtin_ctzll(rows);
+        const int c0 = __builtin_ctzll(readlane64(bm, r0));
+        const uint64_t c = grow_rows(lane == r0 ? (1ull << c0) : 0ull, bm, lane);
+        const uint32_t g = gfind(gmem((const uint32_t *)a.parent), 1u + (uint32_t)t.tile * CT_NB + k);
+        const uint8_t val = g == 0 ? 255 : 0;
+        uint64_t crow = __ballot(c != 0ull);
+        while (crow) {
+            const int r = __builtin_ctzll(crow);
+            crow &= crow - 1;
+            if ((readlane64(c, r) >> lane) & 1ull) img[(long)(t.ty0 + r) * a.stride + t.tx0 + lane] = val;
+        }
+        bm &= ~c;
+        k++;
+    }
+}
+
+struct CtLayout { size_t parent, border, ucount, total; uint32_t tilesX, tilesY, tiles; };
+
+inline CtLayout ct_layout(uint32_t w, uint32_t h)
+{
+    CtLayout L;
+    L.tilesX = (w + CT_W - 1) / CT_W;
+    L.tilesY = (h + CT_H - 1) / CT_H;
+    L.tiles = L.tilesX * L.tilesY;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    L.parent = 0;
+    L.border = al(((size_t)L.tiles * CT_NB + 1) * 4);
+    L.ucount = L.border + al((size_t)L.tiles * CT_NB * 4);
+    L.total = L.ucount + al((size_t)L.tiles * 4);
+    return L;
+}
+
 }  // namespace mvx
 
 using namespace mvx;
@@ -458,7 +686,7 @@ extern "C" int mvx_CannySuppThreshold_U8XY_U16_3x3(mvx_stream_t stream, uint32_t
         (uint64_t)capacityOfXY < (uint64_t)w * h)
         return MVX_ERROR_INVALID_PARAMETERS;
     if (w < 3 || h < 3) return MVX_SUCCESS;
-    dim3 grid((w - 2 + 255) / 256, h - 2);
+    dim3 grid((w + SUP_W - 1) / SUP_W, (h - 2 + SUP_H - 1) / SUP_H);
     hipLaunchKernelGGL(k_canny_supp, grid, dim3(256), 0, stream_of(stream), w, h, dst, dstStride, mag, magStride,
                        lower, upper);
     MVX_LAUNCH_CHECK();
@@ -472,6 +700,20 @@ extern "C" int mvx_CannyEdgeTrace_U8_U8XY(mvx_stream_t stream, uint32_t w, uint3
     if (!plane_ok_c(img, w, stride, 1) || !xyStack || (uint64_t)capacityOfXY < (uint64_t)w * h)
         return MVX_ERROR_INVALID_PARAMETERS;
     if (w < 3 || h < 3) return MVX_SUCCESS;
+    const CtLayout L = ct_layout(w, h);
+    if (L.total <= (size_t)capacityOfXY * 4 && ((uintptr_t)xyStack & 255u) == 0) {
+        // the tiled trace, its tables in the stack's memory (about 0.75 bytes per pixel)
+        uint8_t *ws = (uint8_t *)xyStack;
+        CtArgs a{w, h, stride, L.tilesX, L.tilesY, L.tiles, img, (uint32_t *)(ws + L.parent), (int32_t *)(ws + L.border),
+                 (uint32_t *)(ws + L.ucount)};
+        const dim3 grid((L.tiles + 3) / 4);
+        hipLaunchKernelGGL(k_ct_tile, grid, dim3(256), 0, stream_of(stream), a);
+        hipLaunchKernelGGL(k_ct_border, grid, dim3(256), 0, stream_of(stream), a);
+        hipLaunchKernelGGL(k_ct_resolve, grid, dim3(256), 0, stream_of(stream), a);
+        MVX_LAUNCH_CHECK();
+        return MVX_SUCCESS;
+    }
+    // small images: a union-find over every interior edge pixel, the labels in the stack (4 bytes per pixel)
     dim3 grid((w - 2 + 255) / 256, h - 2);
     uint32_t *lab = (uint32_t *)xyStack;
     hipLaunchKernelGGL(k_canny_init, grid, dim3(256), 0, stream_of(stream), w, h, img, stride, lab);

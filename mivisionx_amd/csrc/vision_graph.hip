@@ -32,6 +32,7 @@
 // size, every pyramid step one the reference accepts.
 #include "mvx_common.hpp"
 #include "pick.hpp"
+#include "rowmask.hpp"
 #include "pyramid.hpp"
 #include <algorithm>
 #include <cstring>
@@ -47,6 +48,7 @@ constexpr int YH = TH + 2 * HY;                  // luma window rows (columns: T
 constexpr int GH = TH + 4, GW = TW + 4;          // gradients, halo 2
 constexpr int MH = TH + 2, MW = TW + 2;          // Canny magnitude / Harris score, halo 1
 constexpr int TS = TH * TP;                      // tile-local pixel index space, p = ly * TP + lx
+constexpr int TNB = 2 * (TW + TH);               // union-find slots per tile: one per border pixel, at most
 constexpr int FT = 256;                          // threads of the tile kernels (4 waves)
 constexpr int NW = FT / 64;
 
@@ -58,8 +60,8 @@ struct FrameDesc {
     void *corners; uint32_t *numItems; uint64_t *count;
     // per-frame workspace
     int32_t *border;      // tiles x 2(TW+TH) component ids of tile-border pixels (-1: none)
-    uint32_t *parent;     // 1 + tiles * TS union-find slots (sparsely used)
-    uint32_t *ulist;      // tiles x TS: a tile's undecided pixels, p | p of the component's id << 16
+    uint32_t *parent;     // 1 + tiles * TNB union-find slots: 0 the strong node, then TNB per tile
+    uint32_t *ulist;      // tiles x TS: a tile's undecided pixels, p | the component's number k << 16
     uint32_t *ucount;     // tiles: undecided pixels listed
     void *cand;           // Harris XYS candidates
     uint32_t *candCount;  // candidates appended (reset by k_pick)
@@ -145,34 +147,6 @@ MVX_DEV void lunion(int *L, int a, int b)
     }
 }
 
-MVX_DEV uint32_t gfind(gptr_t<const uint32_t> par, uint32_t p)
-{
-    uint32_t l = par[p];
-    while (l != p) { p = l; l = par[p]; }
-    return p;
-}
-
-MVX_DEV void gunion(gptr_t<uint32_t> par, uint32_t a, uint32_t b)
-{
-    while (true) {
-        a = gfind(par, a);
-        b = gfind(par, b);
-        if (a == b) return;
-        if (a < b) { uint32_t t = a; a = b; b = t; }
-        const uint32_t old = __hip_atomic_fetch_min(par + a, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == a) return;
-        a = old;
-    }
-}
-
-// the 64-bit value of lane l (l wave-uniform)
-MVX_DEV uint64_t readlane64(uint64_t v, int l)
-{
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 MVX_DEV float lane_next_f(float v) { return __int_as_float(lane_next(__float_as_int(v))); }
 
 // Raw buffer access to one plane: a scalar resource holding the plane's base, the lane's column byte
@@ -183,25 +157,6 @@ MVX_DEV float lane_next_f(float v) { return __int_as_float(lane_next(__float_as_
 MVX_DEV __amdgpu_buffer_rsrc_t plane_rsrc(const void *base)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xffffffffu, 0x00020000);
-}
-
-// Row masks, lane = row (rows 0..31 of one problem in lanes 0..31, of a second in lanes 32..63):
-// grow m inside e by 8-connectivity until nothing changes.
-MVX_DEV uint64_t grow_rows(uint64_t m, uint64_t e, int lane)
-{
-    const int r = lane & 31;
-    while (true) {
-        uint64_t up = __shfl(m, lane > 0 ? lane - 1 : 0), dn = __shfl(m, lane < 63 ? lane + 1 : 63);
-        if (r == 0) up = 0ull;
-        if (r == 31) dn = 0ull;
-        uint64_t v = m | up | dn;
-        v |= (v << 1) | (v >> 1);
-        const uint64_t n = (v & e) | m;
-        const bool changed = n != m;
-        m = n;
-        if (!__any(changed)) break;
-    }
-    return m;
 }
 
 constexpr int FTW = 4;                                     // tiles (one wave each) per workgroup of k_front
@@ -458,7 +413,8 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
     }
 
     //  (c) the undecided components, one at a time: the first set bit in row-major order is the
-    //      component's smallest index p = row*TP + column, its id 1 + tile*TS + p; flood it inside the
+    //      component's smallest index p = row*TP + column; the k-th component found gets the id
+    //      1 + tile*TNB + k (each touches a border pixel no other one does); flood it inside the
     //      undecided mask.  Border tables: top row, bottom row, left column, right column of the tile
     //      (actual extent): 0 for reached pixels (the strong node), the component id for undecided
     //      ones, -1 otherwise.
@@ -471,14 +427,14 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
         if (lane < th && ((rm >> (tw - 1)) & 1ull)) rightv = 0;
     }
     gptr_t<uint32_t> ulist = gmem(F.ulist) + (size_t)tile * TS;
-    uint32_t nU = 0;
+    uint32_t nU = 0, nComp = 0;
     uint64_t bm = bmAll;
     while (true) {
         const uint64_t rows = __ballot(bm != 0ull);
         if (!rows) break;
         const int r0 = __builtin_ctzll(rows);
         const int c0 = __builtin_ctzll(readlane64(bm, r0));
-        const uint32_t gid = 1u + (uint32_t)tile * TS + (uint32_t)(r0 * TP + c0);
+        const uint32_t gid = 1u + (uint32_t)tile * TNB + nComp;
         const uint64_t c = grow_rows(lane == r0 ? (1ull << c0) : 0ull, bm, lane);
         if (lane == 0) gmem(F.parent)[gid] = gid;
         const uint64_t C0 = readlane64(c, 0), Cb = readlane64(c, th - 1);
@@ -486,7 +442,7 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
         if (lane < tw && ((Cb >> lane) & 1ull)) botv = (int32_t)gid;
         if (lane < th && (c & 1ull)) leftv = (int32_t)gid;
         if (lane < th && ((c >> (tw - 1)) & 1ull)) rightv = (int32_t)gid;
-        // list the component's pixels for k_resolve: p | p of the component's smallest pixel << 16
+        // list the component's pixels for k_resolve: p | k << 16
         uint64_t crow = __ballot(c != 0ull);
         while (crow) {
             const int r = __builtin_ctzll(crow);
@@ -494,11 +450,12 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
             const uint64_t m = readlane64(c, r);
             if ((m >> lane) & 1ull) {
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                ulist[nU + below] = (uint32_t)(r * TP + lane) | ((uint32_t)(r0 * TP + c0) << 16);
+                ulist[nU + below] = (uint32_t)(r * TP + lane) | (nComp << 16);
             }
             nU += (uint32_t)__popcll(m);
         }
         bm &= ~c;
+        nComp++;
     }
     gptr_t<int32_t> bt = gmem(F.border) + (long)tile * 2 * (TW + TH);
     if (lane < TW) { bt[lane] = topv; bt[TW + lane] = botv; }
@@ -519,23 +476,6 @@ k_front(const FrameDesc *__restrict__ frames, GraphParams P)
     const bool inner = tx0 >= 4 && tx0 + TW + 4 <= P.w && ty0 >= 4 && ty0 + TH + 4 <= P.h;
     if (inner) front_tile<true>(candLds[wave], F, P, tx0, ty0, tile);
     else front_tile<false>(candLds[wave], F, P, tx0, ty0, tile);
-}
-
-// The unions across one tile edge: lane i holds a = my id at border position i and b = the neighbour's
-// id at position i (-1: no edge pixel, or past the edge's length); a joins b(i-1), b(i), b(i+1).  A pair
-// that a lower lane also forms is left to it (same a in lane i-1 or i+1 forming the pair with its own b,
-// or b(i +- 1) = b(i)), so a run of equal ids across the edge costs one union instead of three per pixel;
-// every distinct pair is still joined, so the components -- and their minimum ids, the roots -- are
-// the same.  Called by the whole wave.
-MVX_DEV void border_unions(gptr_t<uint32_t> par, int32_t a, int32_t b, int lane)
-{
-    const int32_t aU = __shfl_up(a, 1), bU = __shfl_up(b, 1);
-    const int32_t aD = __shfl_down(a, 1), bD = __shfl_down(b, 1);
-    const bool hasU = lane >= 1, hasD = lane + 1 < 64;
-    if (a < 0) return;
-    if (b >= 0 && b != a && !(hasU && aU == a && bU == b)) gunion(par, (uint32_t)a, (uint32_t)b);
-    if (hasU && bU >= 0 && bU != a && bU != b && aU != a) gunion(par, (uint32_t)a, (uint32_t)bU);
-    if (hasD && bD >= 0 && bD != a && bD != b && aD != a) gunion(par, (uint32_t)a, (uint32_t)bD);
 }
 
 // union the border components of each tile with those of its right, lower and diagonal neighbours;
@@ -595,8 +535,8 @@ k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
     gptr_t<const uint32_t> list = gmem((const uint32_t *)F.ulist) + (size_t)tile * TS;
     for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
         const uint32_t e = list[i];
-        const uint32_t p = e & 0xffffu, root = e >> 16;
-        const uint32_t g = gfind(gmem(F.parent), 1u + (uint32_t)tile * TS + root);
+        const uint32_t p = e & 0xffffu, comp = e >> 16;
+        const uint32_t g = gfind(gmem(F.parent), 1u + (uint32_t)tile * TNB + comp);
         gmem(F.edges)[(long)(ty0 + (int)(p / TP)) * F.edgesStride + tx0 + (int)(p % TP)] = (g == 0) ? 255 : 0;
     }
 }
@@ -642,7 +582,7 @@ inline GraphLayout graph_layout(const mvx_vision_params_t &p, uint32_t maxFrames
     L.descs = align256(sizeof(FrameDesc) * (size_t)maxFrames);
     size_t o = 0;
     L.border = o;    o = align256(o + (size_t)L.tiles * 2 * (TW + TH) * 4);
-    L.parent = o;    o = align256(o + ((size_t)L.tiles * TS + 1) * 4);
+    L.parent = o;    o = align256(o + ((size_t)L.tiles * TNB + 1) * 4);
     L.ulist = o;     o = align256(o + (size_t)L.tiles * TS * 4);
     L.ucount = o;    o = align256(o + (size_t)L.tiles * 4);
     L.cand = o;      o = align256(o + (size_t)L.candCap * sizeof(XYS));
