@@ -342,8 +342,10 @@ MVX_DEV void front_tile(uint64_t *cand, const FrameDesc &F, const GraphParams &P
             const uint64_t eb = __ballot(edgePx), sb = __ballot(strongPx);
             if (lane == r) { emLo = (uint32_t)eb; emHi = (uint32_t)(eb >> 32); smLo = (uint32_t)sb; smHi = (uint32_t)(sb >> 32); }
             // Harris 3x3 NMS: >= towards the upper-left half of the window, > towards the lower-right
-            const int c = lane_next(Vb);
-            const bool isMax = valid && c >= max(HO, Vb) && c > max(Hc, lane_next(c));
+            // the lane shifts run with every lane active: a DPP read of a lane that a short-circuited
+            // && has switched off returns 0, not that lane's value
+            const int c = lane_next(Vb), cRight = lane_next(c);
+            const bool isMax = valid & (c >= max(HO, Vb)) & (c > max(Hc, cRight));
             const uint64_t cb = __ballot(isMax);
             if (isMax) {
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cb, 0u));

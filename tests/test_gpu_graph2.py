@@ -174,3 +174,23 @@ def test_pipelined_chunks_match_oracle(gpu, orc, chunk, lane):
         torch.cuda.synchronize()
         if lane:
             K.mvx_VisionGraph_lane_destroy(ln)
+
+
+def test_fused_graph_dense_corners(gpu, orc):
+    """grey binary noise with a tiny Harris threshold: about 120 NMS maxima per 58x32 tile, where every
+    pixel's right neighbour matters to the NMS; also dense Canny edges"""
+    from mivisionx_amd.pipelines import FusedVisionGraph
+    w, h = 464, 200
+    rng = np.random.default_rng(1)
+    grey = (rng.integers(0, 2, (h, w)) * 230 + 10).astype(np.uint8)
+    rgb = np.stack([grey, grey, grey], axis=-1)
+    params = dict(lower=10, upper=40, k=0.04, thr=1e-9, dist=3.0, capacity=200000)
+    G = FusedVisionGraph(w, h, w // 2, h // 2, 3, frames=1, **params)
+    G.frames[0].rgb.upload(rgb)
+    G.run()
+    got = G.frames[0].results()
+    want = orc.graph2(rgb, w // 2, h // 2, 3, params["lower"], params["upper"], params["k"], params["thr"],
+                      params["dist"], params["capacity"])
+    assert_same(got["edges"], want["edges"], "edges")
+    assert got["count"] == want["count"], "corner count"
+    assert_same(got["corners"], want["corners"], "corners")
