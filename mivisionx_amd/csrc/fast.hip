@@ -86,12 +86,13 @@ MVX_DEV int strength(const Circle &k, int t)
 
 // ---------------------------------------------------------------- fused tile pass ------------------
 // One workgroup per 64x16 tile of the image.  The tile's source bytes (rows y0-4..y0+19, columns
-// x0-4..x0+67 at the reference's linear byte addresses, 0 outside the plane) go to LDS; then
-//   1. every candidate position runs the early exit on pixels 1, 5, 9, 13; survivors are compacted
-//      into an LDS list (wave ballots);
-//   2. the survivors run the full nine-run test; without suppression a corner sets its flag, with it
-//      the corners are compacted again and take the strength closed form, stored (mod 256) in an LDS
-//      strength map that covers the tile plus one pixel of border;
+// x0-4..x0+67 at the reference's linear byte addresses, 0 outside the plane) go to LDS; then each of
+// the four waves, on its own rows of candidate positions (lane = column, no barrier until step 3):
+//   1. runs the early exit on pixels 1, 5, 9, 13 of every candidate; survivors are compacted into the
+//      wave's LDS list (ballots);
+//   2. runs the full nine-run test on its survivors; without suppression a corner sets its flag, with
+//      it the corners are compacted again and take the strength closed form, stored (mod 256) in an
+//      LDS strength map that covers the tile plus one pixel of border;
 //   3. with suppression the 3x3 test runs on that map.
 // The tile's results go to a byte plane (0 = none, else the strength or, without suppression, 1) and
 // the per-row counts to rowCount[y - 3] (one atomic per tile row); a scan turns those into offsets and
@@ -100,7 +101,7 @@ constexpr int FX = 64, FY = 16;                  // output tile
 constexpr int SW = FX + 8, SH = FY + 8;          // staged source: 4 columns / rows of margin each side
 static_assert(SW % 4 == 0, "staged rows are whole dwords");
 constexpr int MW = FX + 2, MH = FY + 2;          // strength map: the tile plus one pixel of border
-constexpr int NPOS = MW * MH;
+constexpr int WL = 5 * 64 + 2 * MH;              // a wave's list: its at most 5 rows, plus wave 0's 2 extra columns
 
 struct FastArgs {
     uint32_t w, h, stride, capacity;
@@ -115,15 +116,20 @@ struct FastArgs {
 __constant__ int16_t kCircleOff[16] = {-3 * SW + 0, -3 * SW + 1, -2 * SW + 2, -1 * SW + 3, 3, SW + 3, 2 * SW + 2, 3 * SW + 1,
                                        3 * SW + 0, 3 * SW - 1, 2 * SW - 2, SW - 3, -3, -SW - 3, -2 * SW - 2, -3 * SW - 1};
 
-// append `pos` to an LDS list when `keep`; list order is free (each entry is processed independently)
-MVX_DEV void push(bool keep, int pos, uint16_t *list, uint32_t *count)
+// append `pos` to the wave's LDS list when `keep` (n: the list's length, wave-uniform)
+MVX_DEV void wave_push(bool keep, uint16_t pos, uint16_t *list, uint32_t &n)
 {
     const uint64_t m = __ballot(keep);
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    if (lane == 0 && m) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, 0);
-    if (keep) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)pos;
+    if (keep) list[n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = pos;
+    n += (uint32_t)__popcll(m);
+}
+
+// order a wave's LDS writes before its lanes read one another's entries
+MVX_DEV void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <bool SUPP>
@@ -131,11 +137,12 @@ __global__ void __launch_bounds__(256)
 k_fast_tile(FastArgs a, uint8_t *__restrict__ marks, uint32_t *__restrict__ rowCount)
 {
     __shared__ uint32_t stage32[SH * SW / 4];
-    __shared__ uint8_t smap[MH * MW];
+    __shared__ uint32_t smap32[(MH * MW + 3) / 4];
+    __shared__ uint16_t surv[4][WL], corn[4][WL];
+    __shared__ uint32_t rowCnt[FY];
     uint8_t *stage = reinterpret_cast<uint8_t *>(stage32);
-    __shared__ uint16_t surv[NPOS], corn[NPOS];
-    __shared__ uint32_t nsurv, ncorn, rowCnt[FY];
-    const int tid = (int)threadIdx.x;
+    uint8_t *smap = reinterpret_cast<uint8_t *>(smap32);
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int x0 = (int)blockIdx.x * FX, y0 = (int)blockIdx.y * FY;
     const long n = (long)a.h * a.stride;
     if ((((uintptr_t)a.src | a.stride) & 3u) == 0) {
@@ -151,57 +158,59 @@ k_fast_tile(FastArgs a, uint8_t *__restrict__ marks, uint32_t *__restrict__ rowC
             stage[i] = (uint8_t)ld_u8(a.src, n, (long)(y0 - 4 + r) * a.stride + (x0 - 4 + c));
         }
     }
-    for (int i = tid; i < MH * MW; i += 256) smap[i] = 0;
-    if (tid == 0) { nsurv = 0; ncorn = 0; }
+    for (int i = tid; i < (MH * MW + 3) / 4; i += 256) smap32[i] = 0u;
     if (tid < FY) rowCnt[tid] = 0;
     __syncthreads();
-    // candidate position p: with suppression the map position (x0-1+i, y0-1+j), i < MW, j < MH, inside
+    // candidate (i, j): with suppression the map position (x0-1+i, y0-1+j), i < MW, j < MH, inside
     // [3, w-4] x [3, h-4]; without it the tile pixel (x0+i, y0+j), i < FX, j < FY, x < w, y in [3, h-4]
-    const int pw = SUPP ? MW : FX, np = SUPP ? NPOS : FX * FY, d = SUPP ? 1 : 0;
-    auto centre = [&](int p) { const int j = p / pw, i = p - j * pw; return (j + 4 - d) * SW + (i + 4 - d); };
-    for (int p0 = 0; p0 < np; p0 += 256) {
-        const int p = p0 + tid;
-        bool keep = false;
-        if (p < np) {
-            const int j = p / pw, i = p - j * pw, x = x0 - d + i, y = y0 - d + j;
-            const bool ok = y >= 3 && y < (int)a.h - 3 && (SUPP ? (x >= 3 && x < (int)a.w - 3) : x < (int)a.w);
-            if (ok) {
-                const int o = centre(p), c = stage[o];
-                const int b0 = stage[o - 3 * SW], b4 = stage[o + 3], b8 = stage[o + 3 * SW], b12 = stage[o - 3];
-                keep = !((abs(c - b0) < a.t && abs(c - b8) < a.t) || (abs(c - b4) < a.t && abs(c - b12) < a.t));
-            }
-        }
-        push(keep, p, surv, &nsurv);
+    constexpr int PH = SUPP ? MH : FY, D = SUPP ? 1 : 0;
+    auto centre = [&](int i, int j) { return (j + 4 - D) * SW + (i + 4 - D); };
+    auto early = [&](int i, int j) {
+        const int x = x0 - D + i, y = y0 - D + j;
+        if (!(y >= 3 && y < (int)a.h - 3 && (SUPP ? (x >= 3 && x < (int)a.w - 3) : x < (int)a.w))) return false;
+        const int o = centre(i, j), c = stage[o];
+        const int b0 = stage[o - 3 * SW], b4 = stage[o + 3], b8 = stage[o + 3 * SW], b12 = stage[o - 3];
+        return !((abs(c - b0) < a.t && abs(c - b8) < a.t) || (abs(c - b4) < a.t && abs(c - b12) < a.t));
+    };
+    uint16_t *sl = surv[wave];
+    uint32_t ns = 0;
+    for (int j = wave; j < PH; j += 4) wave_push(early(lane, j), (uint16_t)((j << 8) | lane), sl, ns);
+    if (SUPP && wave == 0) {
+        // map columns 64 and 65 of every row
+        const int j = lane >> 1, i = FX + (lane & 1);
+        wave_push(lane < 2 * MH && early(i, j), (uint16_t)((j << 8) | i), sl, ns);
     }
-    __syncthreads();
-    const int ns = (int)nsurv;
-    for (int q0 = 0; q0 < ns; q0 += 256) {
-        const int q = q0 + tid;
+    wave_sync();
+    uint16_t *cl = corn[wave];
+    uint32_t nc = 0;
+    for (uint32_t q0 = 0; q0 < ns; q0 += 64) {
+        const uint32_t q = q0 + (uint32_t)lane;
         bool corner = false;
-        int p = 0;
+        uint16_t pos = 0;
         if (q < ns) {
-            p = surv[q];
+            pos = sl[q];
+            const int j = pos >> 8, i = pos & 0xff;
             Circle k;
-            const int o = centre(p);
+            const int o = centre(i, j);
             k.c = stage[o];
 #pragma unroll
-            for (int i = 0; i < 16; i++) k.b[i] = stage[o + kCircleOff[i]];
+            for (int e = 0; e < 16; e++) k.b[e] = stage[o + kCircleOff[e]];
             corner = nine_run(plus_mask(k, a.t)) || nine_run(minus_mask(k, a.t));
-            if (!SUPP && corner) smap[(p / FX + 1) * MW + (p % FX) + 1] = 1;
+            if (!SUPP && corner) smap[(j + 1) * MW + i + 1] = 1;
         }
-        if (SUPP) push(corner, p, corn, &ncorn);
+        if (SUPP) wave_push(corner, pos, cl, nc);
     }
     if (SUPP) {
-        __syncthreads();
-        const int nc = (int)ncorn;
-        for (int q = tid; q < nc; q += 256) {
-            const int p = corn[q];
+        wave_sync();
+        for (uint32_t q = (uint32_t)lane; q < nc; q += 64) {
+            const uint16_t pos = cl[q];
+            const int j = pos >> 8, i = pos & 0xff;
             Circle k;
-            const int o = centre(p);
+            const int o = centre(i, j);
             k.c = stage[o];
 #pragma unroll
-            for (int i = 0; i < 16; i++) k.b[i] = stage[o + kCircleOff[i]];
-            smap[p] = (uint8_t)strength(k, a.t);   // p indexes the map directly (pw == MW)
+            for (int e = 0; e < 16; e++) k.b[e] = stage[o + kCircleOff[e]];
+            smap[j * MW + i] = (uint8_t)strength(k, a.t);
         }
     }
     __syncthreads();
@@ -237,7 +246,6 @@ k_fast_tile(FastArgs a, uint8_t *__restrict__ marks, uint32_t *__restrict__ rowC
     __syncthreads();
     if (tid < FY && y0 + tid >= 3 && y0 + tid < (int)a.h - 3 && rowCnt[tid]) atomicAdd(&rowCount[y0 + tid - 3], rowCnt[tid]);
 }
-
 
 __global__ void __launch_bounds__(256) k_fast_zero(uint32_t *__restrict__ p, uint32_t n)
 {

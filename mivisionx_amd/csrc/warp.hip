@@ -222,8 +222,8 @@ k_warp_affine_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint32_t 
         const int pitch = 4 * b.nd;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const bool col = x0 + k < dw;
-            const float xs = (col ? m.r00 * (float)(x0 + k) : 0.0f) + xd, ys = (col ? m.r10 * (float)(x0 + k) : 0.0f) + yd;
+            // every column of the tile is below dw here
+            const float xs = m.r00 * (float)(x0 + k) + xd, ys = m.r10 * (float)(x0 + k) + yd;
             const int32_t xi = (int32_t)xs, yi = (int32_t)ys;
             const float fx = xs - (float)xi, fy = ys - (float)yi;
             const int o = (yi - b.r0) * pitch + (xi - b.cA);
@@ -304,10 +304,10 @@ k_warp_perspective_lds(uint32_t dw, uint32_t dh, uint8_t *__restrict__ dst, uint
         const int pitch = 4 * b.nd;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const bool col = x0 + k < dw;
+            // every column of the tile is below dw here
             const float xf = (float)(x0 + k);
-            const float z = 1.0f / ((col ? m.g * xf : 0.0f) + zd);
-            const float xs = ((col ? m.a * xf : 0.0f) + xd) * z, ys = ((col ? m.d * xf : 0.0f) + yd) * z;
+            const float z = 1.0f / (m.g * xf + zd);
+            const float xs = (m.a * xf + xd) * z, ys = (m.d * xf + yd) * z;
             const int32_t xi = (int32_t)xs, yi = (int32_t)ys;
             const float fx = xs - (float)xi, fy = ys - (float)yi;
             const int o = (yi - b.r0) * pitch + (xi - b.cA);
