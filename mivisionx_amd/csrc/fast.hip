@@ -19,12 +19,11 @@ struct Circle {
 // isCorner (:90-126): nine consecutive set bits on the circle
 MVX_DEV bool nine_run(uint32_t m)
 {
-    if (!m) return false;
-    m |= m << 16;
-    uint32_t r = m;
-#pragma unroll
-    for (int k = 1; k < 9; k++) r &= m >> k;   // bit i set iff bits i..i+8 are all set
-    return (r & 0xffffu) != 0;
+    m |= m << 16;                               // the circle twice: runs that wrap become straight
+    const uint32_t a = m & (m >> 1);            // bit i: bits i..i+1 all set
+    const uint32_t b = a & (a >> 2);            //        bits i..i+3
+    const uint32_t c = b & (b >> 4);            //        bits i..i+7
+    return (c & (m >> 8) & 0xffffu) != 0;       //        bits i..i+8, for a start i on the circle
 }
 
 // pixels 1 and 9, or 5 and 13, both within t of the candidate

@@ -160,6 +160,7 @@ MVX_DEV __amdgpu_buffer_rsrc_t plane_rsrc(const void *base)
 }
 
 constexpr int FTW = 4;                                     // tiles (one wave each) per workgroup of k_front
+constexpr int RES_TILES = 4;                               // tiles per wave of k_resolve
 constexpr int CAND_MAX = ((TW + 1) / 2) * ((TH + 1) / 2);  // Harris NMS maxima of a tile are never 8-adjacent
 
 // One wave streams one tile: luma window rows k = 0..YH-1 (image row ty0-3+k, lane = column tx0-3+lane)
@@ -529,17 +530,25 @@ k_resolve(const FrameDesc *__restrict__ frames, GraphParams P)
 {
     const FrameDesc &F = frames[blockIdx.z];
     const int lane = (int)threadIdx.x & 63;
-    const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    if (tile >= P.tilesX * P.tilesY) return;
-    const uint32_t n = gmem(F.ucount)[tile];
-    if (n == 0) return;
-    const int tx0 = (tile % P.tilesX) * TW, ty0 = (tile / P.tilesX) * TH;
-    gptr_t<const uint32_t> list = gmem((const uint32_t *)F.ulist) + (size_t)tile * TS;
-    for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
-        const uint32_t e = list[i];
-        const uint32_t p = e & 0xffffu, comp = e >> 16;
-        const uint32_t g = gfind(gmem(F.parent), 1u + (uint32_t)tile * TNB + comp);
-        gmem(F.edges)[(long)(ty0 + (int)(p / TP)) * F.edgesStride + tx0 + (int)(p % TP)] = (g == 0) ? 255 : 0;
+    const int w0 = ((int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6)) * RES_TILES;
+    const int tiles = P.tilesX * P.tilesY;
+    if (w0 >= tiles) return;
+    // most tiles list nothing: the wave reads its tiles' counts at once and visits only the busy ones
+    const uint32_t cnt = (lane < RES_TILES && w0 + lane < tiles) ? gmem(F.ucount)[w0 + lane] : 0u;
+    uint64_t busy = __ballot(cnt != 0u);
+    while (busy) {
+        const int j = __builtin_ctzll(busy);
+        busy &= busy - 1;
+        const int tile = w0 + j;
+        const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)cnt, j);
+        const int tx0 = (tile % P.tilesX) * TW, ty0 = (tile / P.tilesX) * TH;
+        gptr_t<const uint32_t> list = gmem((const uint32_t *)F.ulist) + (size_t)tile * TS;
+        for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
+            const uint32_t e = list[i];
+            const uint32_t p = e & 0xffffu, comp = e >> 16;
+            const uint32_t g = gfind(gmem(F.parent), 1u + (uint32_t)tile * TNB + comp);
+            gmem(F.edges)[(long)(ty0 + (int)(p / TP)) * F.edgesStride + tx0 + (int)(p % TP)] = (g == 0) ? 255 : 0;
+        }
     }
 }
 
@@ -713,7 +722,8 @@ static void launch_stage(hipStream_t s, const GraphParams &P, const FrameDesc *D
         hipLaunchKernelGGL(k_border, dim3((P.tilesX * P.tilesY + 3) / 4, 1, nframes), dim3(256), 0, s, D, P);
         break;
     case 2:
-        hipLaunchKernelGGL(k_resolve, dim3((P.tilesX * P.tilesY + 3) / 4, 1, nframes), dim3(256), 0, s, D, P);
+        hipLaunchKernelGGL(k_resolve, dim3((P.tilesX * P.tilesY + 4 * RES_TILES - 1) / (4 * RES_TILES), 1, nframes), dim3(256), 0,
+                           s, D, P);
         break;
     case 3:
         for (int l = 2; l < P.levels; l++) {
