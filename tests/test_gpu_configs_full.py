@@ -13,7 +13,7 @@ import torch
 from mivisionx_amd import synth
 from mivisionx_amd.device import Plane
 from mivisionx_amd.pipelines import ChainBatch
-from util import assert_same
+from util import assert_same, sha
 
 pytestmark = pytest.mark.gpu
 WORKERS = 16
@@ -107,3 +107,11 @@ def test_fused_4k_sixteen_frames_per_launch(gpu, orc):
 
     with ThreadPoolExecutor(3) as ex:
         assert sorted(ex.map(check, sorted(inputs))) == [0, 7, 15]
+
+    # the border unions run concurrently inside the tile pass: every later batch gives every frame the
+    # same edges and corners as the first one
+    first = [(sha(fr.edges.download()), int(fr.count.item())) for fr in G.frames]
+    for rep in range(4):
+        G.run()
+        torch.cuda.synchronize()
+        assert [(sha(fr.edges.download()), int(fr.count.item())) for fr in G.frames] == first, f"batch {rep + 2}"
