@@ -159,12 +159,17 @@ template <int G, bool L2> MVX_DEV uint16_t canny_scalar(const uint8_t *src, long
 
 // The SSE passes over [0, aligned), aligned = w & ~15 >= 16: scratch rows r0, r1 (r1 = r0 + ALIGN16(w))
 // with zeros before r0, after r1 and in their unwritten tails; a read past the end of r0 lands in r1
-// (and one before r1 in r0) when w is a multiple of 16.  A workgroup owns 256 columns of CSR_ROWS
-// output rows.  Each of its staged scratch slots x0-R .. x0+255+R stands for one source column (or
-// none) for every row, so the thread that owns a slot keeps that column's G source bytes in a rolling
-// register window: one new byte per row.  Per row the slots' two sums go to LDS, then each thread
-// finishes one column; columns aligned-1 .. w-1 take the scalar loop's value.
-constexpr int CSR_ROWS = 16;
+// (and one before r1 in r0) when w is a multiple of 16.  Each scratch slot x0-R .. stands for one source
+// column (or none) for every row.  One wave owns a strip of 64 - 2R output columns and CSW_ROWS rows:
+// lane l holds the slot of column x0 - R + l and keeps that column's G source bytes in a rolling
+// registers (all of the strip's rows are loaded first); its two vertical sums reach the neighbouring lanes by DPP
+// lane shifts, so no LDS and no barrier is involved.  Lanes R .. 63-R finish one column each; columns
+// aligned-1 .. w-1 take the scalar loop's value.
+// Short waves: a wave's rows are one chain of loads and dependent arithmetic, and many short waves keep
+// more of the image in flight than a few long ones (3840x2160, gradient 5, L2: 2 rows per wave 28 us,
+// 4 rows 29 us, 8 rows 35 us, 16 rows 50 us; 8 strips per workgroup 28 us, 4 strips 36 us).
+constexpr int CSW_ROWS = 2;    // output rows per wave
+constexpr int CSW_WAVES = 8;   // strips (waves) per workgroup
 
 // which scratch value a slot holds: kind 1 = (r0, r1) of source column sc, 2 = r0 <- r1 of sc (the slot
 // lies past the end of r0), 3 = r1 <- r0 of sc (before the start of r1), 0 = zeros
@@ -178,7 +183,8 @@ MVX_DEV int canny_slot(int col, int aligned, int a16, int &sc)
 }
 
 // the vertical pass of the SSE loop over one column's G source bytes: t0 = the smoothing sum (scratch
-// row r0), t1 = the derivative sum (r1); gradient 7 shifts both right by 2
+// row r0), t1 = the derivative sum (r1); gradient 7 shifts both right by 2.  |t0| <= 4080 and
+// |t1| <= 765 for every G, so the reference's 16-bit scratch values are these.
 template <int G> MVX_DEV void canny_vsum(const int (&s)[G], int &t0, int &t1)
 {
     if (G == 3) {
@@ -193,97 +199,122 @@ template <int G> MVX_DEV void canny_vsum(const int (&s)[G], int &t0, int &t1)
     }
 }
 
-template <int G, bool L2>
-__global__ void __launch_bounds__(256)
-k_canny_sobel_g(uint32_t w, uint32_t h, uint16_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
-                uint32_t srcStride)
+// the value of lane + 1 / lane - 1 (DPP wave_shl:1 / wave_shr:1; lanes past the ends read 0)
+MVX_DEV int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true); }
+MVX_DEV int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true); }
+
+// the SSE horizontal pass and packing of one column from the scratch slots at offsets -R..R
+// (A0(d) = r0 slot, A1(d) = r1 slot)
+template <int G, bool L2> MVX_DEV uint16_t canny_simd_px(const int (&a0)[G], const int (&a1)[G])
 {
     constexpr int R = G / 2;
-    // one dword per slot: every LDS access is dword aligned (16-bit slots let neighbouring reads merge
-    // into misaligned dword loads, which the LDS serves far more slowly)
-    __shared__ int32_t l0[256 + 2 * R], l1[256 + 2 * R];
-    const int x0 = (int)blockIdx.x * 256, yb = R + (int)blockIdx.y * CSR_ROWS;
-    const int ye = min(yb + CSR_ROWS, (int)h - R);
-    const int aligned = (int)(w & ~15u), a16 = (int)((w + 15u) & ~15u);
-    const int tid = (int)threadIdx.x;
-    // slot A = tid, slot B = 256 + tid (only the first 2R threads own one)
-    int scA, scB;
-    const int kA = canny_slot(x0 - R + tid, aligned, a16, scA);
-    const int kB = tid < 2 * R ? canny_slot(x0 - R + 256 + tid, aligned, a16, scB) : (scB = 0, 0);
-    int wa[G], wb[G];
-#pragma unroll
-    for (int k = 0; k < G - 1; k++) {
-        wa[k + 1] = kA ? (int)src[(long)(yb - R + k) * srcStride + scA] : 0;
-        wb[k + 1] = kB ? (int)src[(long)(yb - R + k) * srcStride + scB] : 0;
+#define A0(d) (a0[R + (d)])
+#define A1(d) (a1[R + (d)])
+    int gx, gy;
+    if (G == 3) {
+        gx = w16(A0(1) - A0(-1));
+        gy = w16(-w16(w16(A1(-1) + A1(1)) + sll16(A1(0), 1)));
+    } else if (G == 5) {
+        gx = s16(sll16(w16(A0(1) - A0(-1)), 1) + w16(A0(2) - A0(-2)));
+        int t = w16(sll16(w16(A1(-1) + A1(1)), 2) + w16(A1(-2) + A1(2)));
+        t = L2 ? s16(t + w16(A1(0) * 6)) : w16(t + w16(A1(0) * 6));
+        gy = w16(-t);
+    } else {
+        int t = sll16(s16(A0(2) - A0(-2)), 2);
+        const int u = w16(s16(A0(1) - A0(-1)) * 5);
+        t = s16(t + s16(A0(3) - A0(-3)));
+        gx = s16(t + u);
+        int v = s16(w16(w16(A1(-2) + A1(2)) * 6) + w16(A1(0) * 20));
+        const int q = w16(w16(A1(-1) + A1(1)) * 15);
+        v = s16(v + s16(A1(-3) + A1(3)));
+        v = s16(v + q);
+        gy = s16(-v);
     }
-    const int x = x0 + tid;
-    // the row each iteration shifts in is loaded one iteration ahead, so its latency overlaps a row of work
-    int na = kA ? (int)src[(long)(yb + R) * srcStride + scA] : 0;
-    int nb = kB ? (int)src[(long)(yb + R) * srcStride + scB] : 0;
-    for (int y = yb; y < ye; y++) {
-#pragma unroll
-        for (int k = 0; k < G - 1; k++) { wa[k] = wa[k + 1]; wb[k] = wb[k + 1]; }
-        wa[G - 1] = na;
-        wb[G - 1] = nb;
-        if (y + 1 < ye) {
-            na = kA ? (int)src[(long)(y + 1 + R) * srcStride + scA] : 0;
-            nb = kB ? (int)src[(long)(y + 1 + R) * srcStride + scB] : 0;
-        }
-        int t0, t1;
-        canny_vsum<G>(wa, t0, t1);
-        l0[tid] = (int16_t)(kA == 1 || kA == 2 ? (kA == 1 ? t0 : t1) : 0);
-        l1[tid] = (int16_t)(kA == 1 ? t1 : kA == 3 ? t0 : 0);
-        if (tid < 2 * R) {
-            canny_vsum<G>(wb, t0, t1);
-            l0[256 + tid] = (int16_t)(kB == 1 || kB == 2 ? (kB == 1 ? t0 : t1) : 0);
-            l1[256 + tid] = (int16_t)(kB == 1 ? t1 : kB == 3 ? t0 : 0);
-        }
-        __syncthreads();
-        if (x < (int)w) {
-            uint16_t out;
-            if (x < aligned - 1) {
-                const int c = tid + R;
-#define A0(d) ((int)l0[c + (d)])
-#define A1(d) ((int)l1[c + (d)])
-                int gx, gy;
-                if (G == 3) {
-                    gx = w16(A0(1) - A0(-1));
-                    gy = w16(-w16(w16(A1(-1) + A1(1)) + sll16(A1(0), 1)));
-                } else if (G == 5) {
-                    gx = s16(sll16(w16(A0(1) - A0(-1)), 1) + w16(A0(2) - A0(-2)));
-                    int t = w16(sll16(w16(A1(-1) + A1(1)), 2) + w16(A1(-2) + A1(2)));
-                    t = L2 ? s16(t + w16(A1(0) * 6)) : w16(t + w16(A1(0) * 6));
-                    gy = w16(-t);
-                } else {
-                    int t = sll16(s16(A0(2) - A0(-2)), 2);
-                    const int u = w16(s16(A0(1) - A0(-1)) * 5);
-                    t = s16(t + s16(A0(3) - A0(-3)));
-                    gx = s16(t + u);
-                    int v = s16(w16(w16(A1(-2) + A1(2)) * 6) + w16(A1(0) * 20));
-                    const int q = w16(w16(A1(-1) + A1(1)) * 15);
-                    v = s16(v + s16(A1(-3) + A1(3)));
-                    v = s16(v + q);
-                    gy = s16(-v);
-                }
 #undef A0
 #undef A1
-                int mag;
-                if (!L2) {
-                    mag = w16(abs16(gx) + abs16(gy));
-                    if (G == 7 && mag > 0x3fff) mag = 0x3fff;
-                } else if (G == 3) {
-                    mag = l2_pack_simd((float)(uint16_t)w16(gx * gx) + (float)(uint16_t)w16(gy * gy));
-                } else {
-                    mag = l2_pack_simd((float)(gx * gx + gy * gy));
-                }
-                out = canny_out(mag, canny_sector(gx, gy));
-            } else {
-                out = canny_scalar<G, L2>(src, (long)h * srcStride, srcStride, x, y);
-            }
-            *(uint16_t *)((uint8_t *)dst + (long)y * dstStride + 2L * x) = out;
-        }
-        __syncthreads();
+    int mag;
+    if (!L2) {
+        mag = w16(abs16(gx) + abs16(gy));
+        if (G == 7 && mag > 0x3fff) mag = 0x3fff;
+    } else if (G == 3) {
+        mag = l2_pack_simd((float)(uint16_t)w16(gx * gx) + (float)(uint16_t)w16(gy * gy));
+    } else {
+        mag = l2_pack_simd((float)(gx * gx + gy * gy));
     }
+    return canny_out(mag, canny_sector(gx, gy));
+}
+
+// one strip: EDGE strips hold slots outside [0, aligned) or columns from aligned - 1 on
+template <int G, bool L2, bool EDGE>
+MVX_DEV void canny_strip(uint32_t w, uint32_t h, uint16_t *dst, uint32_t dstStride, const uint8_t *src, uint32_t srcStride,
+                         int xs, int yb, int ye)
+{
+    constexpr int R = G / 2;
+    const int lane = (int)threadIdx.x & 63;
+    const int aligned = (int)(w & ~15u), a16 = (int)((w + 15u) & ~15u);
+    const int col = xs - R + lane;
+    int sc = col;
+    const int kind = EDGE ? canny_slot(col, aligned, a16, sc) : 1;
+    const __amdgpu_buffer_rsrc_t sR = plane_rsrc(src), dR = plane_rsrc(dst);
+    // a zero slot still loads an in-plane byte (column 0) and drops it, so the loads stay unconditional
+    auto load = [&](int r) -> int {
+        const int v = (int)__builtin_amdgcn_raw_buffer_load_b8(sR, sc, (int)((uint32_t)r * srcStride), 0);
+        return (EDGE && kind == 0) ? 0 : v;
+    };
+    // every source row of the strip is loaded before the first is used: the loads are all in flight
+    // together, and row i waits only for its own G rows
+    constexpr int NR = CSW_ROWS + G - 1;
+    int rw[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) rw[j] = (yb - R + j < ye + R) ? load(yb - R + j) : 0;
+    const int x = col;   // output column of lanes R .. 63 - R
+    const bool outLane = lane >= R && lane < 64 - R && x < (int)w;
+#pragma unroll
+    for (int i = 0; i < CSW_ROWS; i++) {
+        const int y = yb + i;
+        if (y >= ye) break;
+        int win[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) win[j] = rw[i + j];
+        int t0, t1;
+        canny_vsum<G>(win, t0, t1);
+        int a0[G], a1[G];
+        a0[R] = t0;
+        a1[R] = t1;
+        if (EDGE) {
+            a0[R] = kind == 1 ? t0 : kind == 2 ? t1 : 0;
+            a1[R] = kind == 1 ? t1 : kind == 3 ? t0 : 0;
+        }
+        // the lane shifts run with every lane active
+#pragma unroll
+        for (int d = 1; d <= R; d++) {
+            a0[R + d] = dpp_next(a0[R + d - 1]);
+            a0[R - d] = dpp_prev(a0[R - d + 1]);
+            a1[R + d] = dpp_next(a1[R + d - 1]);
+            a1[R - d] = dpp_prev(a1[R - d + 1]);
+        }
+        uint16_t out = canny_simd_px<G, L2>(a0, a1);
+        if (outLane) {
+            if (EDGE && x >= aligned - 1) out = canny_scalar<G, L2>(src, (long)h * srcStride, srcStride, x, y);
+            __builtin_amdgcn_raw_buffer_store_b16(out, dR, 2 * x, (int)((uint32_t)y * dstStride), 0);
+        }
+    }
+}
+
+template <int G, bool L2>
+__global__ void __launch_bounds__(64 * CSW_WAVES)
+k_canny_sobel_g(uint32_t w, uint32_t h, uint16_t *__restrict__ dst, uint32_t dstStride, const uint8_t *__restrict__ src,
+                uint32_t srcStride, uint32_t strips)
+{
+    constexpr int R = G / 2, CW = 64 - 2 * R;
+    const int strip = (int)blockIdx.x * CSW_WAVES + ((int)threadIdx.x >> 6);
+    if (strip >= (int)strips) return;   // the whole wave
+    const int yb = R + (int)blockIdx.y * CSW_ROWS, ye = min(yb + CSW_ROWS, (int)h - R);
+    const int xs = strip * CW, aligned = (int)(w & ~15u);
+    if (xs - R < 0 || xs - R + 63 >= aligned)
+        canny_strip<G, L2, true>(w, h, dst, dstStride, src, srcStride, xs, yb, ye);
+    else
+        canny_strip<G, L2, false>(w, h, dst, dstStride, src, srcStride, xs, yb, ye);
 }
 
 // widths below 16: only the scalar loop runs, from x = -1 (the slot before each row), rows in order
@@ -638,11 +669,15 @@ int canny_sobel_g(mvx_stream_t stream, uint32_t w, uint32_t h, uint16_t *dst, ui
     if (w == 0 || h == 0) return MVX_SUCCESS;
     if (!plane_ok_c(dst, w, dstStride, 2) || !plane_ok_c(src, w, srcStride, 1)) return MVX_ERROR_INVALID_PARAMETERS;
     if ((int)h < G) return MVX_SUCCESS;
-    if (w < 16)
+    // buffer offsets (row * stride + column) stay below 2^32
+    if ((uint64_t)h * dstStride > 0xffffffffull || (uint64_t)h * srcStride > 0xffffffffull) return MVX_ERROR_INVALID_PARAMETERS;
+    if (w < 16) {
         hipLaunchKernelGGL((k_canny_sobel_g_narrow<G, L2>), dim3(1), dim3(64), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride);
-    else
-        hipLaunchKernelGGL((k_canny_sobel_g<G, L2>), dim3((w + 255) / 256, (h - 2 * (G / 2) + CSR_ROWS - 1) / CSR_ROWS), dim3(256), 0,
-                           stream_of(stream), w, h, dst, dstStride, src, srcStride);
+    } else {
+        const uint32_t strips = (w + (64 - 2 * (G / 2)) - 1) / (64 - 2 * (G / 2));
+        hipLaunchKernelGGL((k_canny_sobel_g<G, L2>), dim3((strips + CSW_WAVES - 1) / CSW_WAVES, (h - 2 * (G / 2) + CSW_ROWS - 1) / CSW_ROWS),
+                           dim3(64 * CSW_WAVES), 0, stream_of(stream), w, h, dst, dstStride, src, srcStride, strips);
+    }
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }

@@ -30,6 +30,16 @@ namespace mvx {
 template <class T> using gptr_t = __attribute__((address_space(1))) T *;
 template <class T> MVX_DEV gptr_t<T> gmem(T *p) { return (gptr_t<T>)p; }
 
+// Raw buffer access to one plane: a scalar resource holding the plane's base, the lane's column byte
+// offset in a VGPR and the wave-uniform row offset in an SGPR, so a row's load or store needs no
+// per-lane 64-bit address arithmetic.  Word 3 is the gfx9 raw-buffer format.  The record limit is the
+// largest one, 2^32 - 1 bytes: planes of at most 32767 rows of at most 3 * 32767 + 15 bytes keep
+// every row + column offset below it, so no access is ever clipped.
+MVX_DEV __amdgpu_buffer_rsrc_t plane_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xffffffffu, 0x00020000);
+}
+
 MVX_DEV uint32_t ld_u8(const uint8_t *p, long n, long i) { return (i < 0 || i >= n) ? 0u : (uint32_t)p[i]; }
 
 // 16 bytes at a 16-byte aligned address inside the allocation

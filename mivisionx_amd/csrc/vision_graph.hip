@@ -149,16 +149,6 @@ MVX_DEV void lunion(int *L, int a, int b)
 
 MVX_DEV float lane_next_f(float v) { return __int_as_float(lane_next(__float_as_int(v))); }
 
-// Raw buffer access to one plane: a scalar resource holding the plane's base, the lane's column byte
-// offset in a VGPR and the wave-uniform row offset in an SGPR, so a row's load or store needs no
-// per-lane 64-bit address arithmetic.  Word 3 is the gfx9 raw-buffer format.  The record limit is the
-// largest one, 2^32 - 1 bytes: planes of at most 32767 rows of at most 3 * 32767 + 15 bytes keep
-// every row + column offset below it, so no access is ever clipped.
-MVX_DEV __amdgpu_buffer_rsrc_t plane_rsrc(const void *base)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xffffffffu, 0x00020000);
-}
-
 constexpr int FTW = 4;                                     // tiles (one wave each) per workgroup of k_front
 constexpr int RES_TILES = 4;                               // tiles per wave of k_resolve
 constexpr int CAND_MAX = ((TW + 1) / 2) * ((TH + 1) / 2);  // Harris NMS maxima of a tile are never 8-adjacent

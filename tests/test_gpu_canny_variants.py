@@ -33,6 +33,18 @@ def test_canny_sobel_planes(gpu, orc, w, h, g, l2):
     assert_same(got, orc.canny_sobel_ex(src, g, l2, padded=True), f"canny sobel {g} l2={l2} {w}x{h}")
 
 
+@pytest.mark.parametrize("w,h", [(3840, 24), (16, 20), (977, 33), (1024, 17), (124, 9)])
+@pytest.mark.parametrize("g,l2", VARIANTS)
+def test_canny_sobel_planes_noise(gpu, orc, w, h, g, l2):
+    """uniform and 0/255 noise (the largest gradients), widths at and between the kernel's column strips"""
+    for k, src in enumerate((synth.uniform_u8(w, h, w + h), (synth.uniform_u8(w, h, 3 * w) >= 128).astype(np.uint8) * 255)):
+        s = Plane.from_array(src)
+        d = Plane(w, h, "U16")
+        call(f"CannySobel_U16_U8_{g}x{g}_{'L2' if l2 else 'L1'}NORM", w, h, d, d.stride, s, s.stride)
+        got = d.t.cpu().numpy().view(np.uint16)
+        assert_same(got, orc.canny_sobel_ex(src, g, l2, padded=True), f"canny sobel noise {k} {g} l2={l2} {w}x{h}")
+
+
 @pytest.mark.parametrize("w,h", [(100, 60), (333, 77), (640, 480)])
 @pytest.mark.parametrize("g,l2", VARIANTS)
 def test_canny_node_variants(gpu, orc, w, h, g, l2):
