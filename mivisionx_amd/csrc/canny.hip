@@ -459,7 +459,7 @@ k_canny_final(uint32_t w, uint32_t h, uint8_t *__restrict__ img, uint32_t stride
 
 // ---------------------------------------------------------------- tiled edge trace ----------------
 // HafCpu_CannyEdgeTrace_U8_U8XY (ago_haf_cpu_canny.cpp:598-645): every 127 pixel 8-connected to a 255
-// pixel through 127 pixels becomes 255, every other 127 becomes 0.  Per 64x32 tile (one wave) the
+// pixel through 127 pixels becomes 255, every other 127 becomes 0.  Per 64x16 tile (one wave) the
 // edge (127 or 255) and strong (255) pixels of the interior become 64-bit row masks (lane = row); two
 // dilations run side by side in the wave, one from the strong pixels and one from the tile-border edge
 // pixels.  Pixels the strong dilation reaches are final 255; edge pixels reached by neither are final
@@ -468,7 +468,7 @@ k_canny_final(uint32_t w, uint32_t h, uint8_t *__restrict__ img, uint32_t stride
 // pixel).  Border tables record per border pixel 0 (the strong node), an id, or -1.  A second pass
 // unions the tables across tile edges, a third re-finds the undecided components (the 127 pixels left
 // in the tile, discovered in the same order) and writes 255 where the root is the strong node, else 0.
-constexpr int CT_W = 64, CT_H = 32, CT_NB = 2 * (CT_W + CT_H);
+constexpr int CT_W = 64, CT_H = 16, CT_NB = 2 * (CT_W + CT_H);
 
 struct CtArgs {
     uint32_t w, h, stride, tilesX, tilesY, tiles;
@@ -737,7 +737,7 @@ extern "C" int mvx_CannyEdgeTrace_U8_U8XY(mvx_stream_t stream, uint32_t w, uint3
     if (w < 3 || h < 3) return MVX_SUCCESS;
     const CtLayout L = ct_layout(w, h);
     if (L.total <= (size_t)capacityOfXY * 4 && ((uintptr_t)xyStack & 255u) == 0) {
-        // the tiled trace, its tables in the stack's memory (about 0.75 bytes per pixel)
+        // the tiled trace, its tables in the stack's memory (about 1.25 bytes per pixel)
         uint8_t *ws = (uint8_t *)xyStack;
         CtArgs a{w, h, stride, L.tilesX, L.tilesY, L.tiles, img, (uint32_t *)(ws + L.parent), (int32_t *)(ws + L.border),
                  (uint32_t *)(ws + L.ucount)};

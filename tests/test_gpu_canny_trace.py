@@ -1,6 +1,6 @@
 """GPU parity of the Canny edge trace (mvx_CannyEdgeTrace_U8_U8XY) on its own: hysteresis maps of 0 /
 127 / 255 in the interior -- random ones of several densities, long one-pixel paths that cross many
-64x32 tiles, sizes on and off the tile grid, small images that take the per-pixel union-find -- against
+64x16 tiles, sizes on and off the tile grid, small images that take the per-pixel union-find -- against
 the oracle's breadth-first flood, bit-exact."""
 import ctypes
 
@@ -31,8 +31,8 @@ def _random_map(w, h, p127, p255, seed):
     return m
 
 
-@pytest.mark.parametrize("w,h", [(64, 32), (65, 33), (128, 96), (200, 150), (333, 77), (1920, 1080), (3840, 2160),
-                                 (9, 7), (20, 5)])
+@pytest.mark.parametrize("w,h", [(64, 32), (65, 33), (64, 16), (130, 17), (128, 96), (200, 150), (333, 77), (1920, 1080),
+                                 (3840, 2160), (9, 7), (20, 5)])
 @pytest.mark.parametrize("p127,p255", [(0.45, 0.002), (0.6, 0.0005), (0.3, 0.05)])
 def test_trace_random_maps(gpu, orc, w, h, p127, p255):
     m = _random_map(w, h, p127, p255, w * 7 + h)
