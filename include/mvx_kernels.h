@@ -297,8 +297,8 @@ int mvx_CannySuppThreshold_U8XY_U16_3x3(mvx_stream_t stream, uint32_t dstWidth, 
 /* Hysteresis of HafCpu_CannyEdgeTrace_U8_U8XY (ago_haf_cpu_canny.cpp:598-644; CPU-only in the
  * reference, ago_kernel_list.cpp:486): 127 pixels 8-connected through 127s to a 255 become 255, all
  * other 127s become 0, over [1,w-2]x[1,h-2].  In place on pImg.  xyStack: >= w*h * 4 bytes of device
- * workspace (its contents are not read); a 256-byte aligned stack of a large enough image holds the
- * tiled trace's tables, else the per-pixel union-find's labels. */
+ * workspace at a 4-byte aligned address (its contents are not read; otherwise -10); the stack of a
+ * large enough image holds the tiled trace's tables, else the per-pixel union-find's labels. */
 int mvx_CannyEdgeTrace_U8_U8XY(mvx_stream_t stream, uint32_t width, uint32_t height, uint8_t *pImg,
                                uint32_t stride, void *xyStack, uint32_t capacityOfXY);
 

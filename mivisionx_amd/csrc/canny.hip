@@ -732,11 +732,12 @@ extern "C" int mvx_CannyEdgeTrace_U8_U8XY(mvx_stream_t stream, uint32_t w, uint3
                                           void *xyStack, uint32_t capacityOfXY)
 {
     if (w == 0 || h == 0) return MVX_SUCCESS;
-    if (!plane_ok_c(img, w, stride, 1) || !xyStack || (uint64_t)capacityOfXY < (uint64_t)w * h)
+    // the stack holds 4-byte items (two 16-bit coordinates); both traces keep 32-bit words in it
+    if (!plane_ok_c(img, w, stride, 1) || !xyStack || ((uintptr_t)xyStack & 3u) || (uint64_t)capacityOfXY < (uint64_t)w * h)
         return MVX_ERROR_INVALID_PARAMETERS;
     if (w < 3 || h < 3) return MVX_SUCCESS;
     const CtLayout L = ct_layout(w, h);
-    if (L.total <= (size_t)capacityOfXY * 4 && ((uintptr_t)xyStack & 255u) == 0) {
+    if (L.total <= (size_t)capacityOfXY * 4) {
         // the tiled trace, its tables in the stack's memory (about 1.25 bytes per pixel)
         uint8_t *ws = (uint8_t *)xyStack;
         CtArgs a{w, h, stride, L.tilesX, L.tilesY, L.tiles, img, (uint32_t *)(ws + L.parent), (int32_t *)(ws + L.border),

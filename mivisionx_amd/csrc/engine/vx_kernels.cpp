@@ -609,7 +609,7 @@ size_t canny_mag_bytes(uint32_t w, uint32_t h) { return align256((size_t)align16
 vx_status i_canny(_vx_node *n)
 {
     _vx_image *in = img(n, 0);
-    n->localSize = canny_mag_bytes(in->width, in->height) + (size_t)4 * in->width * in->height;
+    n->localSize = canny_mag_bytes(in->width, in->height) + (size_t)4 * in->width * in->height + 256;
     return VX_SUCCESS;
 }
 
@@ -619,7 +619,8 @@ vx_status x_canny(_vx_node *n, hipStream_t st)
     _vx_threshold *t = (_vx_threshold *)n->params[1];
     const uint32_t w = in->width, h = in->height, magStride = align16(2 * w);
     uint16_t *mag = (uint16_t *)n->local.ptr();
-    void *stack = n->local.ptr() + canny_mag_bytes(w, h);
+    // the trace's stack at a 256-byte boundary (its tables then start on whole cache lines)
+    void *stack = (void *)(((uintptr_t)(n->local.ptr() + canny_mag_bytes(w, h)) + 255u) & ~(uintptr_t)255u);
     const int g = scl(n, 2)->value.i32;
     const bool l2 = scl(n, 3)->value.e == VX_NORM_L2;
     // gradient 7 runs CANNY_SUPP_THRESHOLD_U8XY_U16_7x7, which divides the threshold object's values by

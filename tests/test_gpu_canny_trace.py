@@ -55,3 +55,21 @@ def test_trace_long_paths(gpu, orc, w, h):
     m[ys[-1], 2] = 255
     assert_same(_trace(m), orc.canny_trace(m), f"serpentine {w}x{h}")
     assert_same(_trace(m2), orc.canny_trace(m2), f"serpentine without a strong pixel {w}x{h}")
+
+
+def test_trace_stack_alignment(gpu, orc):
+    """a stack at any 4-byte aligned address takes the tiled trace (same result); one that is not
+    4-byte aligned is refused before anything runs"""
+    w, h = 640, 360
+    m = _random_map(w, h, 0.45, 0.002, 5)
+    want = orc.canny_trace(m)
+    base = torch.zeros(w * h * 4 + 512, dtype=torch.uint8, device="cuda")
+    for off in (0, 4, 64, 260):
+        p = Plane.from_array(m)
+        call("CannyEdgeTrace_U8_U8XY", w, h, p, p.stride, ctypes.c_void_p(base.data_ptr() + off), w * h)
+        assert_same(p.download(), want, f"trace with the stack at offset {off}")
+    from mivisionx_amd import kernels
+    p = Plane.from_array(m)
+    rc = kernels().mvx_CannyEdgeTrace_U8_U8XY(None, w, h, p.ptr, p.stride,
+                                              ctypes.c_void_p(base.data_ptr() + 2), w * h)
+    assert rc == -10
