@@ -29,6 +29,7 @@ namespace {
 constexpr int LK_THREADS = 64;
 constexpr int LK_MAX_WIN = 15;
 constexpr int LK_MAX_PIX = LK_MAX_WIN * LK_MAX_WIN;
+constexpr int LK_PPL = (LK_MAX_PIX + 63) / 64;   // window pixels per lane
 constexpr int W_BITS = 14;
 constexpr uint32_t TERM_EPSILON = 0xF001, TERM_BOTH = 0xF002;   // VX_TERM_CRITERIA_*
 constexpr uint32_t LK_EPS_ITER_CAP = 10000;                      // EPSILON-only has no bound in the reference
@@ -103,7 +104,6 @@ k_lk_track(LkLevel L, LkParams P, const int16_t *__restrict__ D, float2 *__restr
     __shared__ int32_t sI[LK_MAX_PIX], sIx[LK_MAX_PIX], sIy[LK_MAX_PIX];
     __shared__ float sP0[LK_MAX_PIX], sP1[LK_MAX_PIX], sP2[LK_MAX_PIX];
     __shared__ float sAcc[24];   // 0..2 results, 4..15 SSE-lane partials, 16..18 scalar partials
-    __shared__ double sDbl[2];
     const uint32_t tid = threadIdx.x;
     const uint32_t pt = blockIdx.x;
     const uint32_t n = min(*oldCount, capacity);
@@ -245,6 +245,14 @@ k_lk_track(LkLevel L, LkParams P, const int16_t *__restrict__ D, float2 *__restr
         return;
     }
     Dt = 1.f / Dt;
+    // this lane's window pixels tid + 64q: row and column (-1: none)
+    int pixY[LK_PPL], pixX[LK_PPL];
+#pragma unroll
+    for (int q = 0; q < LK_PPL; q++) {
+        const int p = (int)tid + q * LK_THREADS;
+        pixY[q] = p < npix ? p / winsz : -1;
+        pixX[q] = p < npix ? p % winsz : 0;
+    }
     float prevDx = 0.f, prevDy = 0.f, ddx = 0.f, ddy = 0.f;
     const int nB = winsz >= 8 ? (winsz - 8) / 8 + 1 : 0;   // SSE column groups of 8 in the iterations
     for (uint32_t j = 0; (j < P.iterations || P.termination == TERM_EPSILON) && j < LK_EPS_ITER_CAP;) {
@@ -259,38 +267,77 @@ k_lk_track(LkLevel L, LkParams P, const int16_t *__restrict__ D, float2 *__restr
         const int iw01 = (int)((double)((a * (1.f - b)) * (float)(1 << W_BITS)) + 0.5);
         const int iw10 = (int)((double)(((1.f - a) * b) * (float)(1 << W_BITS)) + 0.5);
         const int iw11 = (1 << W_BITS) - iw00 - iw01 - iw10;
-        for (int p = (int)tid; p < npix; p += LK_THREADS) {
-            const int y = p / winsz, x = p % winsz;
-            const uint8_t *J = L.newImg + (uint64_t)(y + iny) * JStride + (inx + x);
-            int32_t t = descale(J[0] * iw00 + J[1] * iw01 + J[JStride] * iw10 + J[JStride + 1] * iw11, W_BITS - 5);
+        // The scalar columns' terms go straight into per-lane double sums.  Each term is a float holding an
+        // integer below 2^31 in magnitude (|Ix| <= 32767 times |diff| <= 40928), and there are at most
+        // 15 * 7 of them, so every partial sum is an integer below 2^38: the double sums are exact in any
+        // order and equal the reference's row-major loop.
+        // the lane's pixels' four source bytes are all loaded before any is used
+        double sx = 0.0, sy = 0.0;
+        uint32_t j00[LK_PPL], j01[LK_PPL], j10[LK_PPL], j11[LK_PPL];
+#pragma unroll
+        for (int q = 0; q < LK_PPL; q++) {
+            j00[q] = j01[q] = j10[q] = j11[q] = 0u;
+            if (pixY[q] >= 0) {
+                const uint8_t *J = L.newImg + (uint64_t)(pixY[q] + iny) * JStride + (inx + pixX[q]);
+                j00[q] = J[0];
+                j01[q] = J[1];
+                j10[q] = J[JStride];
+                j11[q] = J[JStride + 1];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < LK_PPL; q++) {
+            if (pixY[q] < 0) continue;
+            const int p = (int)tid + q * LK_THREADS, x = pixX[q];
+            int32_t t = descale((int32_t)j00[q] * iw00 + (int32_t)j01[q] * iw01 + (int32_t)j10[q] * iw10 + (int32_t)j11[q] * iw11,
+                                W_BITS - 5);
             int32_t diff;
             if (x < 8 * nB) diff = sat16(sat16(t) - sI[p]);   // packs_epi32 then subs_epi16
             else diff = t - sI[p];
-            sP0[p] = (float)(sIx[p] * diff);
-            sP1[p] = (float)(sIy[p] * diff);
+            const float f0 = (float)(sIx[p] * diff), f1 = (float)(sIy[p] * diff);
+            if (x < 8 * nB) {
+                sP0[p] = f0;
+                sP1[p] = f1;
+            } else {
+                sx += (double)f0;
+                sy += (double)f1;
+            }
+        }
+        static_assert(LK_THREADS == 64, "the scalar-column sums are one wave's");
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sx += __shfl_xor(sx, o);
+            sy += __shfl_xor(sy, o);
         }
         __syncthreads();
         if (tid < 8) {
-            // qb0 lanes k: x-pair (8g + k/2) and (8g + 4 + k/2); qb1 lanes: +2; even k: Ix, odd k: Iy
+            // qb0 lanes k: x-pair (k/2) and (4 + k/2); qb1 lanes: +2; even k: Ix, odd k: Iy.  Windows of at
+            // most 15 columns have at most one group of 8 (nB <= 1); the lane's terms are read first and
+            // then added in the reference's order.
+            static_assert(LK_MAX_WIN < 16, "one SSE group of 8 columns per row");
             const int arr = (int)tid / 4, k = (int)tid % 4;
             const float *Pq = (k & 1) ? sP1 : sP0;
             const int c0 = 2 * arr + k / 2;
             float acc = 0.0f;
-            for (int y = 0; y < winsz; y++)
-                for (int g = 0; g < nB; g++) {
-                    acc = acc + Pq[y * winsz + 8 * g + c0];
-                    acc = acc + Pq[y * winsz + 8 * g + 4 + c0];
+            if (nB) {
+                float v0[LK_MAX_WIN], v1[LK_MAX_WIN];
+#pragma unroll
+                for (int y = 0; y < LK_MAX_WIN; y++) {
+                    v0[y] = y < winsz ? Pq[y * winsz + c0] : 0.0f;
+                    v1[y] = y < winsz ? Pq[y * winsz + 4 + c0] : 0.0f;
                 }
+#pragma unroll
+                for (int y = 0; y < LK_MAX_WIN; y++) {
+                    if (y < winsz) {
+                        acc = acc + v0[y];
+                        acc = acc + v1[y];
+                    }
+                }
+            }
             sAcc[4 + tid] = acc;
-        } else if (tid < 10) {
-            const float *Pq = (tid == 8) ? sP0 : sP1;
-            double acc = 0.0;
-            for (int y = 0; y < winsz; y++)
-                for (int x = 8 * nB; x < winsz; x++) acc += (double)Pq[y * winsz + x];
-            sDbl[tid - 8] = acc;
         }
         __syncthreads();
-        double ib1 = sDbl[0], ib2 = sDbl[1];
+        double ib1 = sx, ib2 = sy;
         if (winsz >= 8) {
             const float bb0 = sAcc[4] + sAcc[8], bb1 = sAcc[5] + sAcc[9], bb2 = sAcc[6] + sAcc[10], bb3 = sAcc[7] + sAcc[11];
             ib1 += (double)(bb0 + bb2);
@@ -301,7 +348,7 @@ k_lk_track(LkLevel L, LkParams P, const int16_t *__restrict__ D, float2 *__restr
         const float dy = (A12 * b1 - A11 * b2) * Dt;
         nextx += dx;
         nexty += dy;
-        __syncthreads();   // sP0/sP1/sAcc/sDbl are rewritten by the next iteration
+        __syncthreads();   // sP0/sP1/sAcc are rewritten by the next iteration
         if ((dx * dx + dy * dy) <= P.epsilon && (P.termination == TERM_EPSILON || P.termination == TERM_BOTH)) break;
         if (j > 0 && (double)fabsf(dx + prevDx) < 0.01 && (double)fabsf(dy + prevDy) < 0.01) {
             ddx = dx * 0.5f;
