@@ -251,36 +251,6 @@ __global__ void __launch_bounds__(256) k_fast_zero(uint32_t *__restrict__ p, uin
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0u;
 }
 
-// exclusive scan of the row counts (one workgroup); the array count (capped) and the corner count
-// (uncapped) go to the caller's device words
-__global__ void __launch_bounds__(1024)
-k_fast_scan(const uint32_t *__restrict__ rowCount, uint32_t *__restrict__ rowOffset, uint32_t rows, uint32_t capacity,
-            uint32_t *__restrict__ dstCount, uint64_t *__restrict__ numCorners)
-{
-    __shared__ uint32_t part[1024];
-    const uint32_t per = (rows + 1023) / 1024, r0 = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t r = r0; r < r0 + per && r < rows; r++) sum += rowCount[r];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {
-        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - sum;
-    for (uint32_t r = r0; r < r0 + per && r < rows; r++) {
-        rowOffset[r] = run;
-        run += rowCount[r];
-    }
-    if (threadIdx.x == 1023) {
-        const uint32_t total = part[1023];
-        *dstCount = min(total, capacity);
-        if (numCorners) *numCorners = total;
-    }
-}
-
 // exclusive prefix of v over the 256 threads of the block, and the block total (v small)
 MVX_DEV uint32_t block_scan(uint32_t v, uint32_t &total)
 {
@@ -302,14 +272,20 @@ MVX_DEV uint32_t block_scan(uint32_t v, uint32_t &total)
 }
 
 // vx_keypoint_t {x, y, strength, scale, orientation, tracking_status, error}; one workgroup per row
-// 3..h-4 reads the row's marks (16 consecutive pixels per thread) and writes its corners from the row's
-// offset on, in raster order
+// 3..h-4 sums the corner counts of the rows above it (its first index), reads the row's marks (16
+// consecutive pixels per thread) and writes its corners from there on, in raster order.  The last row's
+// workgroup, whose sum covers every row, writes the array count (capped) and the corner count (uncapped)
+// to the caller's device words.
 template <bool SUPP>
 __global__ void __launch_bounds__(256)
-k_fast_write(FastArgs a, const uint8_t *__restrict__ marks, const uint32_t *__restrict__ rowOffset, uint8_t *__restrict__ dst)
+k_fast_write(FastArgs a, const uint8_t *__restrict__ marks, const uint32_t *__restrict__ rowCount, uint32_t rows,
+             uint32_t *__restrict__ dstCount, uint64_t *__restrict__ numCorners, uint8_t *__restrict__ dst)
 {
     const int y = 3 + (int)blockIdx.x;
-    uint32_t base = rowOffset[blockIdx.x];
+    uint32_t above = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += 256) above += rowCount[i];
+    uint32_t base;
+    block_scan(above, base);
     const uint8_t *row = marks + (long)y * a.w;
     for (int x0 = 0; x0 < (int)a.w; x0 += 256 * 16) {
         const int xs = x0 + (int)threadIdx.x * 16;
@@ -339,10 +315,14 @@ k_fast_write(FastArgs a, const uint8_t *__restrict__ marks, const uint32_t *__re
         }
         base += total;
     }
+    if (blockIdx.x == rows - 1 && threadIdx.x == 0) {
+        *dstCount = min(base, a.capacity);
+        if (numCorners) *numCorners = base;
+    }
 }
 
 struct FastWs {
-    size_t rowCount, rowOffset, marks, total;
+    size_t rowCount, marks, total;
 };
 
 FastWs fast_ws(uint32_t w, uint32_t h)
@@ -350,8 +330,7 @@ FastWs fast_ws(uint32_t w, uint32_t h)
     FastWs L;
     const size_t rows = h > 6 ? h - 6 : 1;
     L.rowCount = 0;
-    L.rowOffset = (rows * 4 + 255) & ~(size_t)255;
-    L.marks = L.rowOffset + ((rows * 4 + 255) & ~(size_t)255);
+    L.marks = (rows * 4 + 255) & ~(size_t)255;
     L.total = L.marks + (((size_t)w * h + 255) & ~(size_t)255);
     return L;
 }
@@ -370,11 +349,11 @@ int fast_corners(mvx_stream_t stream, uint32_t capacity, void *dstCorners, uint3
     const uint32_t rows = h - 6;
     FastArgs a{w, h, srcStride, capacity, src, (int)(short)floorf(threshold), threshold};
     hipStream_t st = stream_of(stream);
-    uint32_t *rowCount = (uint32_t *)(ws + L.rowCount), *rowOffset = (uint32_t *)(ws + L.rowOffset);
+    uint32_t *rowCount = (uint32_t *)(ws + L.rowCount);
     hipLaunchKernelGGL(k_fast_zero, dim3(min((rows + 255) / 256, 64u)), dim3(256), 0, st, rowCount, rows);
     hipLaunchKernelGGL(k_fast_tile<SUPP>, dim3((w + FX - 1) / FX, (h + FY - 1) / FY), dim3(256), 0, st, a, ws + L.marks, rowCount);
-    hipLaunchKernelGGL(k_fast_scan, dim3(1), dim3(1024), 0, st, rowCount, rowOffset, rows, capacity, dstCount, numCorners);
-    hipLaunchKernelGGL(k_fast_write<SUPP>, dim3(rows), dim3(256), 0, st, a, ws + L.marks, rowOffset, (uint8_t *)dstCorners);
+    hipLaunchKernelGGL(k_fast_write<SUPP>, dim3(rows), dim3(256), 0, st, a, ws + L.marks, rowCount, rows, dstCount, numCorners,
+                       (uint8_t *)dstCorners);
     MVX_LAUNCH_CHECK();
     return MVX_SUCCESS;
 }
